@@ -1,0 +1,193 @@
+"""ORACLE TEST INFRASTRUCTURE — ctypes access to the two CPU checkers.
+
+* ``Oracle("orc")`` loads ``oracle/libtbem_oracle.so``: the from-scratch FP64
+  restatement of 3bem's FMM (``oracle/fmm_oracle.cpp``).
+* ``Oracle("ref")`` loads ``oracle/_ref/libtbem_ref.so``: the unmodified
+  reference sources (cpp/fmm.cpp, cpp/octree.cpp, ...) behind
+  ``oracle/ref_driver.cpp``; present only where /root/reference was there to
+  compile it.
+
+Only tests/, tests/golden/, the smoke check and bench.py's cpu_baseline leg use
+this module. The product (3bem_amd/) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBS = {
+    "orc": os.path.join(HERE, "libtbem_oracle.so"),
+    "ref": os.path.join(HERE, "_ref", "libtbem_ref.so"),
+}
+
+# Kernel ids (same numbering as include/tbem_fmm.h TBEM_K_*)
+LAPLACE_SINGLE, LAPLACE_DOUBLE, LAPLACE_HYPERSINGULAR = 0, 1, 2
+ELASTIC_U, ELASTIC_T, ELASTIC_A, ELASTIC_H = 3, 4, 5, 6
+IDENTITY_SCALAR, IDENTITY_TENSOR = 7, 8
+SCALAR_KERNELS = (0, 1, 2, 7)
+
+COUNT_NAMES = ("n_e", "n_src_cells", "n_obs_cells", "n_up_levels", "n_down_levels",
+               "p2m", "m2m", "p2p", "m2p", "p2l", "m2l", "l2l", "l2p")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_L = ctypes.POINTER(ctypes.c_long)
+_SZ = ctypes.c_size_t
+
+
+def components(kernel):
+    return 1 if kernel in SCALAR_KERNELS else 3
+
+
+def available(prefix):
+    return os.path.exists(LIBS[prefix])
+
+
+def _p(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    def __init__(self, prefix="orc"):
+        self.prefix = prefix
+        path = LIBS[prefix]
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run make -C oracle)")
+        self.lib = ctypes.CDLL(path)
+        f = self._f
+        f("fmm_create").restype = ctypes.c_void_p
+        f("fmm_create").argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    _SZ, _D, _D, _SZ, _D, _D, _D,
+                                    ctypes.c_double, _SZ, _SZ, ctypes.c_double, ctypes.c_int]
+        f("fmm_apply").argtypes = [ctypes.c_void_p, _D, _D]
+        f("fmm_counts").argtypes = [ctypes.c_void_p, _L]
+        f("fmm_c2e").argtypes = [ctypes.c_void_p, ctypes.c_int, _SZ, _D]
+        f("fmm_pairs").argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
+        f("fmm_destroy").argtypes = [ctypes.c_void_p]
+        for name in ("direct_eval",):
+            f(name).argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                _SZ, _D, _D, _SZ, _D, _D, _D, _D, _D]
+        f("kernel_eval").argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     _D, _D, _D, _D, _D]
+        f("octree").argtypes = [_D, _SZ, _SZ, _L, _L, _D, _L]
+        f("surface").restype = ctypes.c_long
+        f("surface").argtypes = [_SZ, _D]
+        f("pinv").argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]
+        if prefix == "orc":
+            for name in ("direct_eval_par", "direct_adjoint"):
+                f(name).argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    _SZ, _D, _D, _SZ, _D, _D, _D, _D, _D]
+
+    def _f(self, name):
+        return getattr(self.lib, f"{self.prefix}_{name}")
+
+    # ------------------------------------------------------------ FMM operator
+    def fmm(self, kernel, obs_locs, obs_normals, src_locs, src_normals, src_weights,
+            mac, order, leaf, d, small_cells=False, mu=1.0, nu=0.25):
+        return OracleFMM(self, kernel, mu, nu, obs_locs, obs_normals, src_locs,
+                         src_normals, src_weights, mac, order, leaf, d, small_cells)
+
+    def _direct(self, name, kernel, obs_locs, obs_normals, src_locs, src_normals,
+                src_weights, x, mu, nu, n_out):
+        ol, on, sl, sn, sw, xv = (_c(a) for a in (obs_locs, obs_normals, src_locs,
+                                                  src_normals, src_weights, x))
+        out = np.zeros(n_out)
+        self._f(name)(kernel, mu, nu, ol.shape[0], _p(ol), _p(on), sl.shape[0], _p(sl),
+                      _p(sn), _p(sw), _p(xv), _p(out))
+        return out
+
+    def direct(self, kernel, obs_locs, obs_normals, src_locs, src_normals, src_weights,
+               x, mu=1.0, nu=0.25, parallel=False):
+        """FP64 direct sum (nbody_eval, cpp/nbody_operator.h:117-141)."""
+        name = "direct_eval_par" if (parallel and self.prefix == "orc") else "direct_eval"
+        return self._direct(name, kernel, obs_locs, obs_normals, src_locs, src_normals,
+                            src_weights, x, mu, nu, components(kernel) * len(obs_locs))
+
+    def direct_adjoint(self, kernel, obs_locs, obs_normals, src_locs, src_normals,
+                       src_weights, g, mu=1.0, nu=0.25):
+        return self._direct("direct_adjoint", kernel, obs_locs, obs_normals, src_locs,
+                            src_normals, src_weights, g, mu, nu,
+                            components(kernel) * len(src_locs))
+
+    def kernel(self, kernel, obs, src, nobs, nsrc, mu=1.0, nu=0.25):
+        R = components(kernel)
+        out = np.zeros(R * R)
+        args = [_c(a) for a in (obs, src, nobs, nsrc)]
+        self._f("kernel_eval")(kernel, mu, nu, *[_p(a) for a in args], _p(out))
+        return out.reshape(R, R)
+
+    def octree(self, pts, leaf):
+        """Cells in depth-first pre-order: (cells[n,10], indices)."""
+        p = _c(pts)
+        nc, ni = ctypes.c_long(), ctypes.c_long()
+        self._f("octree")(_p(p), p.shape[0], leaf, ctypes.byref(nc), ctypes.byref(ni), None, None)
+        cells = np.zeros((nc.value, 10))
+        idx = np.zeros(ni.value, dtype=np.int64)
+        self._f("octree")(_p(p), p.shape[0], leaf, ctypes.byref(nc), ctypes.byref(ni),
+                          _p(cells), idx.ctypes.data_as(_L))
+        return cells, idx
+
+    def surface(self, order):
+        n = self._f("surface")(order, None)
+        out = np.zeros((n, 3))
+        self._f("surface")(order, _p(out))
+        return out
+
+    def pinv(self, A, threshold=0.0):
+        A = _c(A)
+        n = A.shape[0]
+        out = np.zeros((n, n))
+        self._f("pinv")(_p(A), n, threshold, _p(out))
+        return out
+
+
+class OracleFMM:
+    def __init__(self, orc, kernel, mu, nu, obs_locs, obs_normals, src_locs, src_normals,
+                 src_weights, mac, order, leaf, d, small_cells):
+        self.o = orc
+        self.kernel = kernel
+        self.R = components(kernel)
+        self.n_obs = len(obs_locs)
+        self.n_src = len(src_locs)
+        arrs = [_c(a) for a in (obs_locs, obs_normals, src_locs, src_normals, src_weights)]
+        self._keep = arrs
+        ol, on, sl, sn, sw = arrs
+        self.h = orc._f("fmm_create")(kernel, mu, nu, self.n_obs, _p(ol), _p(on), self.n_src,
+                                      _p(sl), _p(sn), _p(sw), mac, order, leaf, d,
+                                      1 if small_cells else 0)
+
+    def apply(self, x):
+        x = _c(x)
+        assert x.shape[0] == self.R * self.n_src
+        out = np.zeros(self.R * self.n_obs)
+        self.o._f("fmm_apply")(self.h, _p(x), _p(out))
+        return out
+
+    def counts(self):
+        c = np.zeros(len(COUNT_NAMES), dtype=np.int64)
+        self.o._f("fmm_counts")(self.h, c.ctypes.data_as(_L))
+        return dict(zip(COUNT_NAMES, (int(v) for v in c)))
+
+    def c2e(self, down, level):
+        n = self.R * self.counts()["n_e"]
+        out = np.zeros((n, n))
+        self.o._f("fmm_c2e")(self.h, 1 if down else 0, level, _p(out))
+        return out
+
+    def pairs(self, which):
+        key = {"p2p": (0, "p2p"), "m2l": (1, "m2l"), "m2p": (2, "m2p"), "p2l": (3, "p2l")}[which]
+        n = self.counts()[key[1]]
+        out = np.zeros((n, 14))
+        if n:
+            self.o._f("fmm_pairs")(self.h, key[0], _p(out))
+        return out
+
+    def __del__(self):
+        try:
+            self.o._f("fmm_destroy")(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,248 @@
+// ORACLE TEST INFRASTRUCTURE — not product code.
+//
+// extern "C" driver around the UNMODIFIED reference FMM sources compiled from
+// /root/reference/cpp by oracle/Makefile into oracle/_ref/libtbem_ref.so.
+// It only calls the reference's public C++ API:
+//   FMMOperator<3,R,C> ctor / apply / tasks / C2E lists   (cpp/fmm.h:131-237)
+//   nbody_eval, make_direct_nbody_operator                 (cpp/nbody_operator.h:117-152)
+//   make_octree                                            (cpp/octree.cpp:198-223)
+//   TranslationSurface<3>::make_surrounding_surface        (cpp/fmm.cpp:22-51)
+//   svd_decompose / set_threshold / svd_pseudoinverse      (cpp/blas_wrapper.cpp:67-127)
+//   Kernel<3,R,C>::operator()                              (cpp/kernel.h:19-33)
+// Only tests/, tests/golden/ and bench.py's cpu_baseline leg load it.
+
+#include "fmm.h"
+#include "nbody_operator.h"
+#include "laplace_kernels.h"
+#include "elastic_kernels.h"
+#include "identity_kernels.h"
+#include "octree.h"
+#include "geometry.h"
+#include "blas_wrapper.h"
+
+#include <memory>
+#include <vector>
+#include <cstring>
+
+using namespace tbem;
+
+namespace {
+
+// Kernel ids shared with include/tbem_fmm.h (TBEM_K_*).
+enum { K_LS = 0, K_LD = 1, K_LH = 2, K_EU = 3, K_ET = 4, K_EA = 5, K_EH = 6, K_ID1 = 7, K_ID3 = 8 };
+
+bool is_scalar(int k) { return k == K_LS || k == K_LD || k == K_LH || k == K_ID1; }
+
+std::unique_ptr<Kernel<3,1,1>> make_k1(int k) {
+    switch (k) {
+        case K_LS: return std::unique_ptr<Kernel<3,1,1>>(new LaplaceSingle<3>());
+        case K_LD: return std::unique_ptr<Kernel<3,1,1>>(new LaplaceDouble<3>());
+        case K_LH: return std::unique_ptr<Kernel<3,1,1>>(new LaplaceHypersingular<3>());
+        default:   return std::unique_ptr<Kernel<3,1,1>>(new IdentityScalar<3>());
+    }
+}
+
+std::unique_ptr<Kernel<3,3,3>> make_k3(int k, double mu, double nu) {
+    switch (k) {
+        case K_EU: return std::unique_ptr<Kernel<3,3,3>>(new ElasticDisplacement<3>(mu, nu));
+        case K_ET: return std::unique_ptr<Kernel<3,3,3>>(new ElasticTraction<3>(mu, nu));
+        case K_EA: return std::unique_ptr<Kernel<3,3,3>>(new ElasticAdjointTraction<3>(mu, nu));
+        case K_EH: return std::unique_ptr<Kernel<3,3,3>>(new ElasticHypersingular<3>(mu, nu));
+        default:   return std::unique_ptr<Kernel<3,3,3>>(new IdentityTensor<3,3,3>());
+    }
+}
+
+std::vector<Vec3<double>> to_vec3(const double* p, size_t n) {
+    std::vector<Vec3<double>> out(n);
+    for (size_t i = 0; i < n; i++) {
+        out[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    }
+    return out;
+}
+
+NBodyData<3> make_data(size_t n_obs, const double* obs_locs, const double* obs_normals,
+                       size_t n_src, const double* src_locs, const double* src_normals,
+                       const double* src_weights) {
+    NBodyData<3> d;
+    d.obs_locs = to_vec3(obs_locs, n_obs);
+    d.obs_normals = to_vec3(obs_normals, n_obs);
+    d.src_locs = to_vec3(src_locs, n_src);
+    d.src_normals = to_vec3(src_normals, n_src);
+    d.src_weights.assign(src_weights, src_weights + n_src);
+    return d;
+}
+
+struct Handle {
+    virtual ~Handle() {}
+    virtual void apply(const double* x, double* out) = 0;
+    virtual void counts(long* c) = 0;
+    virtual void c2e(int down, size_t level, double* out) = 0;
+    virtual void pairs(int which, double* out) = 0;
+};
+
+template <size_t R>
+struct HandleT: Handle {
+    std::unique_ptr<Kernel<3,R,R>> K;
+    FMMOperator<3,R,R> op;
+    HandleT(std::unique_ptr<Kernel<3,R,R>> k, const NBodyData<3>& data, const FMMConfig& cfg):
+        K(std::move(k)), op(*K, data, cfg) {}
+
+    void apply(const double* x, double* out) override {
+        std::vector<double> xv(x, x + op.n_cols());
+        auto y = op.apply(xv);
+        std::memcpy(out, y.data(), y.size() * sizeof(double));
+    }
+    void counts(long* c) override {
+        c[0] = (long)op.up_equiv_surface.pts.size();
+        c[1] = (long)(1 + op.src_oct.n_children());
+        c[2] = (long)(1 + op.obs_oct.n_children());
+        c[3] = (long)op.up_check_to_equiv.size();
+        c[4] = (long)op.down_check_to_equiv.size();
+        c[5] = (long)op.tasks.p2ms.size();
+        c[6] = (long)op.tasks.m2ms.size();
+        c[7] = (long)op.tasks.p2ps.size();
+        c[8] = (long)op.tasks.m2ps.size();
+        c[9] = (long)op.tasks.p2ls.size();
+        c[10] = (long)op.tasks.m2ls.size();
+        c[11] = (long)op.tasks.l2ls.size();
+        c[12] = (long)op.tasks.l2ps.size();
+    }
+    void c2e(int down, size_t level, double* out) override {
+        auto& v = down ? op.down_check_to_equiv[level] : op.up_check_to_equiv[level];
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    }
+    static void put_cell(const Octree<3>& c, double* o) {
+        o[0] = (double)c.level;
+        for (int d = 0; d < 3; d++) {
+            o[1 + d] = c.bounds.center[d];
+            o[4 + d] = c.bounds.half_width[d];
+        }
+    }
+    // which: 0 p2p, 1 m2l, 2 m2p, 3 p2l; 14 doubles per pair (obs cell, src cell)
+    void pairs(int which, double* out) override {
+        const std::vector<typename FMMTasks<3>::CellPairTask>* v =
+            which == 0 ? &op.tasks.p2ps : which == 1 ? &op.tasks.m2ls :
+            which == 2 ? &op.tasks.m2ps : &op.tasks.p2ls;
+        for (size_t i = 0; i < v->size(); i++) {
+            put_cell((*v)[i].obs_cell, out + 14 * i);
+            put_cell((*v)[i].src_cell, out + 14 * i + 7);
+        }
+    }
+};
+
+void dfs_cells(const Octree<3>& c, std::vector<double>& cells, std::vector<long>& idx) {
+    // per cell: level, center xyz, hw xyz, is_leaf, n_indices, offset
+    cells.push_back((double)c.level);
+    for (int d = 0; d < 3; d++) cells.push_back(c.bounds.center[d]);
+    for (int d = 0; d < 3; d++) cells.push_back(c.bounds.half_width[d]);
+    cells.push_back(c.is_leaf() ? 1.0 : 0.0);
+    cells.push_back((double)c.indices.size());
+    cells.push_back((double)idx.size());
+    for (auto i: c.indices) idx.push_back((long)i);
+    for (auto& ch: c.children) {
+        if (ch != nullptr) dfs_cells(*ch, cells, idx);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+void* ref_fmm_create(int kernel, double mu, double nu,
+                     size_t n_obs, const double* obs_locs, const double* obs_normals,
+                     size_t n_src, const double* src_locs, const double* src_normals,
+                     const double* src_weights,
+                     double mac, size_t order, size_t leaf, double d, int small_cells) {
+    auto data = make_data(n_obs, obs_locs, obs_normals, n_src, src_locs, src_normals, src_weights);
+    FMMConfig cfg(mac, order, leaf, d, small_cells != 0);
+    if (is_scalar(kernel)) {
+        return (Handle*)new HandleT<1>(make_k1(kernel), data, cfg);
+    }
+    return (Handle*)new HandleT<3>(make_k3(kernel, mu, nu), data, cfg);
+}
+
+void ref_fmm_apply(void* h, const double* x, double* out) { ((Handle*)h)->apply(x, out); }
+void ref_fmm_counts(void* h, long* c) { ((Handle*)h)->counts(c); }
+void ref_fmm_c2e(void* h, int down, size_t level, double* out) { ((Handle*)h)->c2e(down, level, out); }
+void ref_fmm_pairs(void* h, int which, double* out) { ((Handle*)h)->pairs(which, out); }
+void ref_fmm_destroy(void* h) { delete (Handle*)h; }
+
+// nbody_eval (cpp/nbody_operator.h:117-141), the reference's direct sum.
+void ref_direct_eval(int kernel, double mu, double nu,
+                     size_t n_obs, const double* obs_locs, const double* obs_normals,
+                     size_t n_src, const double* src_locs, const double* src_normals,
+                     const double* src_weights, const double* x, double* out) {
+    auto data = make_data(n_obs, obs_locs, obs_normals, n_src, src_locs, src_normals, src_weights);
+    std::vector<double> y;
+    if (is_scalar(kernel)) {
+        y = nbody_eval(*make_k1(kernel), data, x);
+    } else {
+        y = nbody_eval(*make_k3(kernel, mu, nu), data, x);
+    }
+    std::memcpy(out, y.data(), y.size() * sizeof(double));
+}
+
+// make_direct_nbody_operator(...).apply (dense matrix + dgemv), small sizes only.
+void ref_dense_apply(int kernel, double mu, double nu,
+                     size_t n_obs, const double* obs_locs, const double* obs_normals,
+                     size_t n_src, const double* src_locs, const double* src_normals,
+                     const double* src_weights, const double* x, double* out) {
+    auto data = make_data(n_obs, obs_locs, obs_normals, n_src, src_locs, src_normals, src_weights);
+    std::vector<double> y;
+    if (is_scalar(kernel)) {
+        auto op = make_direct_nbody_operator(data, *make_k1(kernel));
+        y = op.apply(std::vector<double>(x, x + op.n_cols()));
+    } else {
+        auto op = make_direct_nbody_operator(data, *make_k3(kernel, mu, nu));
+        y = op.apply(std::vector<double>(x, x + op.n_cols()));
+    }
+    std::memcpy(out, y.data(), y.size() * sizeof(double));
+}
+
+// One kernel evaluation K(obs, src, nobs, nsrc) -> R*C row-major.
+void ref_kernel_eval(int kernel, double mu, double nu, const double* obs, const double* src,
+                     const double* nobs, const double* nsrc, double* out) {
+    Vec3<double> o{obs[0], obs[1], obs[2]}, s{src[0], src[1], src[2]};
+    Vec3<double> no{nobs[0], nobs[1], nobs[2]}, ns{nsrc[0], nsrc[1], nsrc[2]};
+    if (is_scalar(kernel)) {
+        out[0] = (*make_k1(kernel))(o, s, no, ns)[0][0];
+    } else {
+        auto v = (*make_k3(kernel, mu, nu))(o, s, no, ns);
+        for (int k = 0; k < 3; k++)
+            for (int j = 0; j < 3; j++) out[3 * k + j] = v[k][j];
+    }
+}
+
+// Octree cells in DFS pre-order (children 0..7), 10 doubles per cell, plus the
+// concatenated per-cell index lists. Call with null outputs to get sizes.
+void ref_octree(const double* pts, size_t n, size_t leaf, long* n_cells, long* n_idx,
+                double* cells_out, long* idx_out) {
+    auto oct = make_octree(to_vec3(pts, n), leaf);
+    std::vector<double> cells;
+    std::vector<long> idx;
+    dfs_cells(oct, cells, idx);
+    *n_cells = (long)(cells.size() / 10);
+    *n_idx = (long)idx.size();
+    if (cells_out) std::memcpy(cells_out, cells.data(), cells.size() * sizeof(double));
+    if (idx_out) std::memcpy(idx_out, idx.data(), idx.size() * sizeof(long));
+}
+
+long ref_surface(size_t order, double* out) {
+    auto s = TranslationSurface<3>::make_surrounding_surface(order);
+    if (out) {
+        for (size_t i = 0; i < s.pts.size(); i++)
+            for (int d = 0; d < 3; d++) out[3 * i + d] = s.pts[i][d];
+    }
+    return (long)s.pts.size();
+}
+
+// Thresholded SVD pseudo-inverse of a row-major n x n matrix.
+void ref_pinv(const double* A, int n, double threshold, double* out) {
+    std::vector<double> m(A, A + (size_t)n * n);
+    auto svd = svd_decompose(m);
+    if (threshold > 0) set_threshold(svd, threshold);
+    auto p = svd_pseudoinverse(svd);
+    std::memcpy(out, p.data(), p.size() * sizeof(double));
+}
+
+} // extern "C"
