@@ -1,0 +1,825 @@
+// FMM plan + apply orchestration and the C ABI (include/tbem_fmm.h).
+//
+// Reference: FMMOperator<3,R,C> (cpp/fmm.h:131-237, cpp/fmm.cpp:71-109, 444-558).
+// The plan holds both octrees (breadth-first cells, points permuted so each cell is
+// a contiguous range), the per-level C2E matrices, and every task list turned into
+// a CSR list keyed by the cell that accumulates. apply() then runs
+//   gather -> P2M -> M2M (deepest level first) -> M2L, P2L -> L2L (root first)
+//   -> P2P + M2P + L2P fused per observation leaf -> scatter
+// on the handle's stream. Every stage owns its outputs (no atomics), so the FP64
+// result is deterministic. The reference order (P2P before M2L etc.,
+// cpp/fmm.cpp:459-545) only changes how the same terms are summed.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "c2e.h"
+#include "stages.h"
+
+namespace tbem {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+// ----------------------------------------------------------------------- the plan
+struct FmmPlan {
+    int kid = 0, R = 1, precision = TBEM_FP64;
+    KConst kc{};
+    size_t n_obs = 0, n_src = 0;
+    double mac = 0, d = 0;
+    size_t order = 0, leaf = 0;
+    bool small_cells = false;
+    Surfaces surf;
+    int n_e = 0, blk = 0;
+    HostTree st, ot;
+    std::vector<double> h_obs_locs, h_obs_normals;  // for set_obs_locs / octree queries
+    DevTree dst, dot;
+    DevPoints sp, op;
+    DevBuf<double> d_unit, d_inner, d_outer;
+    DevBuf<double> uc2e, dc2e;
+    std::vector<double> h_uc2e, h_dc2e;
+    TaskLists tl;
+    DevBuf<int> iota;            // 0..max(cells)-1: per-level cell ranges as pointers into it
+    // forward lists
+    DevList p2m, m2l, p2l, leaf_p2p, leaf_m2p;
+    std::vector<DevList> m2m;    // by source level: parents at L, entries = children
+    std::vector<DevList> l2l;    // by observation level L >= 1: cells at L, entry = parent
+    DevBuf<int> obs_leaves, src_leaves;
+    // adjoint lists
+    DevList a_l2p;               // obs leaves, entry = itself
+    std::vector<DevList> a_l2l;  // by obs level: parents at L, entries = children
+    DevList a_m2l, a_m2p;        // items = source cells
+    std::vector<DevList> a_m2m;  // by src level L >= 1: cells at L, entry = parent
+    DevList a_leaf_p2p, a_leaf_p2l;  // indexed by source leaf position
+    double p2p_pairs = 0;
+    double plan_seconds = 0;
+};
+
+struct Work {
+    hipStream_t stream = nullptr;
+    DevBuf<double> dens, mult, locals, out_sorted, xbuf, ybuf;
+    DevBuf<double> gsort, lbar, cbar, mbar, ebar, grad_sorted;
+    hipEvent_t ev[8];
+    bool timed = false;
+    explicit Work(const FmmPlan& P) {
+        TBEM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& e: ev) TBEM_HIP(hipEventCreate(&e));
+        dens.alloc(P.R * P.n_src);
+        mult.alloc((size_t)P.dst.n_cells * P.blk);
+        locals.alloc((size_t)P.dot.n_cells * P.blk);
+        out_sorted.alloc(P.R * P.n_obs);
+    }
+    ~Work() {
+        for (auto& e: ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace tbem
+
+struct tbem_fmm_s {
+    std::shared_ptr<tbem::FmmPlan> plan;
+    std::unique_ptr<tbem::Work> work;
+    std::mutex mu;
+};
+
+namespace tbem {
+
+// ------------------------------------------------------------------ list builders
+// items = cells that accumulate; pairs (a, b): the item is pair[col], the entry pair[1-col].
+static HostList list_from_pairs(const std::vector<int>& items, int n_cells_item_tree,
+                                const std::vector<std::array<int, 2>>& pairs, int col,
+                                const HostTree* pts_tree) {
+    HostList l;
+    l.items = items;
+    std::vector<int> pos(n_cells_item_tree, -1);
+    for (size_t k = 0; k < items.size(); k++) pos[items[k]] = (int)k;
+    std::vector<int> cnt(items.size() + 1, 0);
+    for (auto& p: pairs) {
+        int k = pos[p[col]];
+        if (k >= 0) cnt[k + 1]++;
+    }
+    l.ptr.assign(items.size() + 1, 0);
+    for (size_t k = 0; k < items.size(); k++) l.ptr[k + 1] = l.ptr[k] + cnt[k + 1];
+    l.cell.resize(l.ptr.back());
+    std::vector<int> fill(l.ptr.begin(), l.ptr.end() - 1);
+    for (auto& p: pairs) {
+        int k = pos[p[col]];
+        if (k >= 0) l.cell[fill[k]++] = p[1 - col];
+    }
+    if (pts_tree) cells_to_ranges(l, *pts_tree);
+    return l;
+}
+
+static std::vector<int> cells_with(const std::vector<std::array<int, 2>>& pairs, int col, int n_cells) {
+    std::vector<char> m(n_cells, 0);
+    for (auto& p: pairs) m[p[col]] = 1;
+    std::vector<int> out;
+    for (int c = 0; c < n_cells; c++)
+        if (m[c]) out.push_back(c);
+    return out;
+}
+
+// children lists of the non-leaf cells at level L
+static HostList children_list(const HostTree& t, int L) {
+    HostList l;
+    l.ptr.assign(1, 0);
+    for (int c = t.level_begin[L]; c < t.level_begin[L + 1]; c++) {
+        if (t.leaf[c]) continue;
+        l.items.push_back(c);
+        for (int k = 0; k < t.n_child[c]; k++) l.cell.push_back(t.first_child[c] + k);
+        l.ptr.push_back((int)l.cell.size());
+    }
+    return l;
+}
+
+// parent lists of all cells at level L >= 1
+static HostList parent_list(const HostTree& t, int L) {
+    HostList l;
+    l.ptr.assign(1, 0);
+    for (int c = t.level_begin[L]; c < t.level_begin[L + 1]; c++) {
+        l.items.push_back(c);
+        l.cell.push_back(t.parent[c]);
+        l.ptr.push_back((int)l.cell.size());
+    }
+    return l;
+}
+
+static HostList self_list(const HostTree& t, const std::vector<int>& cells) {
+    HostList l;
+    l.items = cells;
+    l.ptr.assign(1, 0);
+    for (int c: cells) {
+        l.cell.push_back(c);
+        l.ptr.push_back((int)l.cell.size());
+    }
+    cells_to_ranges(l, t);
+    return l;
+}
+
+static void validate(const tbem_fmm_desc* d) {
+    TBEM_REQUIRE(d != nullptr, TBEM_E_INVALID, "null descriptor");
+    TBEM_REQUIRE(kernel_valid(d->kernel), TBEM_E_INVALID, "unknown kernel id");
+    TBEM_REQUIRE(d->order > 0, TBEM_E_INVALID, "FMMConfig.order must be > 0");
+    TBEM_REQUIRE(d->min_pts_per_cell > 0, TBEM_E_INVALID, "FMMConfig.min_pts_per_cell must be > 0");
+    TBEM_REQUIRE(d->precision == TBEM_FP64 || d->precision == TBEM_MIXED, TBEM_E_INVALID,
+                 "precision must be TBEM_FP64 or TBEM_MIXED");
+    TBEM_REQUIRE(d->n_obs < (1u << 30) && d->n_src < (1u << 30), TBEM_E_UNSUPPORTED,
+                 "more than 2^30 points per side");
+    TBEM_REQUIRE(d->n_obs == 0 || (d->obs_locs && d->obs_normals), TBEM_E_INVALID, "null observation arrays");
+    TBEM_REQUIRE(d->n_src == 0 || (d->src_locs && d->src_normals && d->src_weights), TBEM_E_INVALID,
+                 "null source arrays");
+}
+
+static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s) {
+    auto t0 = std::chrono::steady_clock::now();
+    validate(d);
+    auto P = std::make_shared<FmmPlan>();
+    P->kid = d->kernel;
+    P->R = kernel_dim(d->kernel);
+    P->precision = d->precision;
+    P->kc = make_kconst(d->shear_modulus, d->poisson_ratio);
+    P->n_obs = d->n_obs;
+    P->n_src = d->n_src;
+    P->mac = d->mac;
+    P->d = d->d;
+    P->order = d->order;
+    P->leaf = d->min_pts_per_cell;
+    P->small_cells = d->account_for_small_cells != 0;
+    P->surf = make_surfaces(d->order, d->d);
+    P->n_e = P->surf.n_e;
+    P->blk = P->R * P->n_e;
+    const int leaf = (int)std::min<size_t>(d->min_pts_per_cell, (size_t)1 << 30);
+
+    P->st = build_octree(d->src_locs, (int)d->n_src, leaf, s);
+    const bool same = d->n_obs == d->n_src &&
+                      (d->obs_locs == d->src_locs ||
+                       (d->n_obs > 0 && std::memcmp(d->obs_locs, d->src_locs, 3 * sizeof(double) * d->n_obs) == 0));
+    P->ot = same ? P->st : build_octree(d->obs_locs, (int)d->n_obs, leaf, s);
+    if (d->n_obs) {
+        P->h_obs_locs.assign(d->obs_locs, d->obs_locs + 3 * d->n_obs);
+        P->h_obs_normals.assign(d->obs_normals, d->obs_normals + 3 * d->n_obs);
+    }
+    P->dst.upload(P->st, s);
+    P->dot.upload(P->ot, s);
+    P->sp.upload(P->st, d->src_locs, d->src_normals, d->src_weights, s);
+    P->op.upload(P->ot, d->obs_locs, d->obs_normals, nullptr, s);
+    P->d_unit.upload(P->surf.unit, s);
+    P->d_inner.upload(P->surf.inner, s);
+    P->d_outer.upload(P->surf.outer, s);
+
+    // check-to-equivalent operators
+    const size_t bb = (size_t)P->blk * P->blk;
+    if (d->up_check_to_equiv) {
+        TBEM_REQUIRE(d->n_up_levels >= (size_t)P->st.n_levels, TBEM_E_INVALID,
+                     "injected UC2E has fewer levels than the source tree");
+        P->uc2e.upload(d->up_check_to_equiv, bb * P->st.n_levels, s);
+    } else {
+        compute_c2e(P->kid, P->kc, P->st, P->surf, true, P->uc2e, s);
+    }
+    if (d->down_check_to_equiv) {
+        TBEM_REQUIRE(d->n_down_levels >= (size_t)P->ot.n_levels, TBEM_E_INVALID,
+                     "injected DC2E has fewer levels than the observation tree");
+        P->dc2e.upload(d->down_check_to_equiv, bb * P->ot.n_levels, s);
+    } else {
+        compute_c2e(P->kid, P->kc, P->ot, P->surf, false, P->dc2e, s);
+    }
+    P->h_uc2e = P->uc2e.to_host(s);
+    P->h_dc2e = P->dc2e.to_host(s);
+
+    // task lists (cpp/fmm.cpp:560-648)
+    dual_tree_traversal(P->ot, P->st, P->mac, P->n_e, P->small_cells, P->tl);
+    const TaskLists& tl = P->tl;
+    for (auto& p: tl.p2p) P->p2p_pairs += (double)P->ot.count[p[0]] * (double)P->st.count[p[1]];
+
+    const HostTree& ST = P->st;
+    const HostTree& OT = P->ot;
+    std::vector<int> iota(std::max(ST.n_cells, OT.n_cells) + 1);
+    for (size_t i = 0; i < iota.size(); i++) iota[i] = (int)i;
+    P->iota.upload(iota, s);
+
+    P->p2m.upload(self_list(ST, ST.leaves), s);
+    P->m2m.resize(ST.n_levels);
+    for (int L = 0; L + 1 < ST.n_levels; L++) P->m2m[L].upload(children_list(ST, L), s);
+    P->m2l.upload(list_from_pairs(cells_with(tl.m2l, 0, OT.n_cells), OT.n_cells, tl.m2l, 0, nullptr), s);
+    P->p2l.upload(list_from_pairs(cells_with(tl.p2l, 0, OT.n_cells), OT.n_cells, tl.p2l, 0, &ST), s);
+    P->l2l.resize(OT.n_levels);
+    for (int L = 1; L < OT.n_levels; L++) P->l2l[L].upload(parent_list(OT, L), s);
+    P->obs_leaves.upload(OT.leaves, s);
+    P->src_leaves.upload(ST.leaves, s);
+    P->leaf_p2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.p2p, 0, &ST), s);
+    P->leaf_m2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.m2p, 0, nullptr), s);
+
+    // transposed lists for the adjoint
+    P->a_l2p.upload(self_list(OT, OT.leaves), s);
+    P->a_l2l.resize(OT.n_levels);
+    for (int L = 0; L + 1 < OT.n_levels; L++) P->a_l2l[L].upload(children_list(OT, L), s);
+    P->a_m2l.upload(list_from_pairs(cells_with(tl.m2l, 1, ST.n_cells), ST.n_cells, tl.m2l, 1, nullptr), s);
+    P->a_m2p.upload(list_from_pairs(cells_with(tl.m2p, 1, ST.n_cells), ST.n_cells, tl.m2p, 1, &OT), s);
+    P->a_m2m.resize(ST.n_levels);
+    for (int L = 1; L < ST.n_levels; L++) P->a_m2m[L].upload(parent_list(ST, L), s);
+    P->a_leaf_p2p.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, &OT), s);
+    P->a_leaf_p2l.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr), s);
+
+    TBEM_HIP(hipStreamSynchronize(s));
+    P->plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return P;
+}
+
+// --------------------------------------------------------------------- stage glue
+static SurfStage surf_base(const FmmPlan& P) {
+    SurfStage a{};
+    a.unit = P.d_unit.p;
+    a.n_e = P.n_e;
+    a.blk = P.blk;
+    return a;
+}
+
+static void set_list(SurfStage& a, const DevList& l) {
+    a.n_items = l.n_items;
+    a.items = l.items.p;
+    a.ptr = l.ptr.p;
+    a.cell = l.cell.p;
+    a.rbeg = l.rbeg.p;
+    a.rcnt = l.rcnt.p;
+}
+
+// gather, upward pass, M2L/P2L and L2L: everything the leaf stage consumes
+template <class T>
+static void farfield_T(const FmmPlan& P, Work& W, const double* x, hipStream_t s) {
+    const int R = P.R;
+    const int Ns = (int)P.n_src;
+    TBEM_HIP(hipEventRecord(W.ev[0], s));
+    launch_gather_dens(x, P.sp.perm.p, P.sp.w.p, Ns, R, W.dens.p, s);
+    TBEM_HIP(hipEventRecord(W.ev[1], s));
+
+    // P2M (cpp/fmm.cpp:162-202): points of each source leaf -> up-check -> UC2E
+    SurfStage a = surf_base(P);
+    set_list(a, P.p2m);
+    a.tt = view(P.dst); a.t_surf = P.d_outer.p;
+    a.st = view(P.dst); a.sp = view(P.sp); a.dens = W.dens.p; a.n_src_pts = Ns;
+    a.c2e = P.uc2e.p; a.out = W.mult.p;
+    launch_surf_stage<T>(P.kid, false, false, false, a, P.kc, s);
+    TBEM_HIP(hipEventRecord(W.ev[2], s));
+
+    // M2M (cpp/fmm.cpp:204-251), deepest parents first
+    for (int L = P.st.n_levels - 2; L >= 0; L--) {
+        SurfStage m = surf_base(P);
+        set_list(m, P.m2m[L]);
+        m.tt = view(P.dst); m.t_surf = P.d_outer.p;
+        m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
+        m.c2e = P.uc2e.p; m.out = W.mult.p;
+        launch_surf_stage<T>(P.kid, true, false, false, m, P.kc, s);
+    }
+    TBEM_HIP(hipEventRecord(W.ev[3], s));
+
+    // M2L (cpp/fmm.cpp:362-382) and P2L (cpp/fmm.cpp:296-330) into locals
+    launch_fill(W.locals.p, W.locals.n, 0.0, s);
+    SurfStage m = surf_base(P);
+    set_list(m, P.m2l);
+    m.tt = view(P.dot); m.t_surf = P.d_inner.p;
+    m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
+    m.c2e = P.dc2e.p; m.out = W.locals.p;
+    launch_surf_stage<T>(P.kid, true, true, false, m, P.kc, s);
+    SurfStage pl = surf_base(P);
+    set_list(pl, P.p2l);
+    pl.tt = view(P.dot); pl.t_surf = P.d_inner.p;
+    pl.st = view(P.dst); pl.sp = view(P.sp); pl.dens = W.dens.p; pl.n_src_pts = Ns;
+    pl.c2e = P.dc2e.p; pl.out = W.locals.p;
+    launch_surf_stage<T>(P.kid, false, true, false, pl, P.kc, s);
+    TBEM_HIP(hipEventRecord(W.ev[4], s));
+
+    // L2L (cpp/fmm.cpp:384-413), root first
+    for (int L = 1; L < P.ot.n_levels; L++) {
+        SurfStage l = surf_base(P);
+        set_list(l, P.l2l[L]);
+        l.tt = view(P.dot); l.t_surf = P.d_inner.p;
+        l.st = view(P.dot); l.s_surf = P.d_outer.p; l.s_coeff = W.locals.p;
+        l.c2e = P.dc2e.p; l.out = W.locals.p;
+        launch_surf_stage<T>(P.kid, true, true, false, l, P.kc, s);
+    }
+    TBEM_HIP(hipEventRecord(W.ev[5], s));
+}
+
+// P2P + M2P + L2P per observation leaf (cpp/fmm.cpp:254-294, 332-360, 415-441)
+static LeafStage leaf_stage_args(const FmmPlan& P, Work& W) {
+    const int Ns = (int)P.n_src, No = (int)P.n_obs;
+    LeafStage f{};
+    f.n_leaves = (int)P.ot.leaves.size();
+    f.leaves = P.obs_leaves.p;
+    f.tt = view(P.dot); f.tp = view(P.op); f.t_weight = nullptr; f.n_tgt = No;
+    f.rp_ptr = P.leaf_p2p.ptr.p; f.rp_beg = P.leaf_p2p.rbeg.p; f.rp_cnt = P.leaf_p2p.rcnt.p;
+    f.ps = view(P.sp); f.pdens = W.dens.p; f.n_ps = Ns;
+    f.sl_ptr = P.leaf_m2p.ptr.p; f.sl_cell = P.leaf_m2p.cell.p;
+    f.slt = view(P.dst); f.sl_surf = P.d_inner.p; f.sl_coeff = W.mult.p;
+    f.self_surf = P.d_outer.p; f.self_coeff = W.locals.p;
+    f.unit = P.d_unit.p; f.n_e = P.n_e; f.blk = P.blk;
+    f.out_sorted = W.out_sorted.p;
+    return f;
+}
+
+template <class T>
+static void forward_T(const FmmPlan& P, Work& W, const double* x, double* y, hipStream_t s) {
+    const int R = P.R;
+    const int Ns = (int)P.n_src, No = (int)P.n_obs;
+    if (No == 0) return;
+    if (Ns == 0) {
+        launch_fill(y, (size_t)R * No, 0.0, s);
+        return;
+    }
+    farfield_T<T>(P, W, x, s);
+    launch_leaf_stage<T>(P.kid, false, leaf_stage_args(P, W), P.kc, s);
+    TBEM_HIP(hipEventRecord(W.ev[6], s));
+    launch_scatter_out(W.out_sorted.p, P.op.perm.p, No, R, y, s);
+    TBEM_HIP(hipEventRecord(W.ev[7], s));
+    W.timed = true;
+}
+
+__global__ void k_scatter_aos3(const double* __restrict__ in, const int* __restrict__ perm, int n,
+                               double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t g = (size_t)perm[i];
+    for (int m = 0; m < 3; m++) out[3 * g + m] = in[3 * (size_t)i + m];
+}
+
+// grad[3*i+m] = sum_q g_q(i) d u_q(i) / d obs_i[m], FP64, plan geometry held fixed.
+static void obs_grad(const FmmPlan& P, Work& W, const double* x, const double* g, double* grad,
+                     hipStream_t s) {
+    const int R = P.R;
+    const int Ns = (int)P.n_src, No = (int)P.n_obs;
+    if (No == 0) return;
+    if (Ns == 0) {
+        launch_fill(grad, 3 * (size_t)No, 0.0, s);
+        return;
+    }
+    farfield_T<double>(P, W, x, s);
+    if (W.gsort.n < (size_t)R * No) W.gsort.alloc((size_t)R * No);
+    if (W.grad_sorted.n < 3 * (size_t)No) W.grad_sorted.alloc(3 * (size_t)No);
+    launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
+    LeafStage f = leaf_stage_args(P, W);
+    f.grad_g = W.gsort.p;
+    f.grad_out = W.grad_sorted.p;
+    launch_leaf_grad(P.kid, f, P.kc, s);
+    hipLaunchKernelGGL(k_scatter_aos3, dim3(div_up(No, 256)), dim3(256), 0, s, W.grad_sorted.p, P.op.perm.p,
+                       No, grad);
+    TBEM_LAUNCH_CHECK();
+}
+
+// Exact transpose of forward_T: every stage transposed, run in reverse order.
+template <class T>
+static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, hipStream_t s) {
+    const int R = P.R;
+    const int Ns = (int)P.n_src, No = (int)P.n_obs;
+    if (Ns == 0) return;
+    if (No == 0) {
+        launch_fill(xbar, (size_t)R * Ns, 0.0, s);
+        return;
+    }
+    const size_t nco = (size_t)P.dot.n_cells * P.blk, ncs = (size_t)P.dst.n_cells * P.blk;
+    if (W.gsort.n < (size_t)R * No) W.gsort.alloc((size_t)R * No);
+    if (W.lbar.n < nco) { W.lbar.alloc(nco); W.cbar.alloc(nco); }
+    if (W.mbar.n < ncs) { W.mbar.alloc(ncs); W.ebar.alloc(ncs); }
+    if (W.dens.n < (size_t)R * Ns) W.dens.alloc((size_t)R * Ns);
+    launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
+    launch_fill(W.lbar.p, nco, 0.0, s);
+    launch_fill(W.cbar.p, nco, 0.0, s);
+    launch_fill(W.mbar.p, ncs, 0.0, s);
+    launch_fill(W.ebar.p, ncs, 0.0, s);
+
+    // L2P^T: observation points -> down-equivalent surface of their leaf
+    SurfStage a = surf_base(P);
+    set_list(a, P.a_l2p);
+    a.tt = view(P.dot); a.t_surf = P.d_outer.p;
+    a.st = view(P.dot); a.sp = view(P.op); a.dens = W.gsort.p; a.n_src_pts = No;
+    a.c2e = nullptr; a.out = W.lbar.p;
+    launch_surf_stage<T>(P.kid, false, false, true, a, P.kc, s);
+
+    // L2L^T, deepest level first: cbar = DC2E^T lbar, then push to parents
+    for (int L = P.ot.n_levels - 1; L >= 0; L--) {
+        const int c0 = P.ot.level_begin[L], nc = P.ot.level_begin[L + 1] - c0;
+        launch_c2e_transpose_apply(P.iota.p + c0, nc, P.dot.level.p, P.dc2e.p, P.blk, W.lbar.p, W.cbar.p, s);
+        if (L == 0) break;
+        SurfStage l = surf_base(P);
+        set_list(l, P.a_l2l[L - 1]);
+        l.tt = view(P.dot); l.t_surf = P.d_outer.p;
+        l.st = view(P.dot); l.s_surf = P.d_inner.p; l.s_coeff = W.cbar.p;
+        l.c2e = nullptr; l.out = W.lbar.p;
+        launch_surf_stage<T>(P.kid, true, true, true, l, P.kc, s);
+    }
+
+    // M2L^T and M2P^T into mbar
+    SurfStage m = surf_base(P);
+    set_list(m, P.a_m2l);
+    m.tt = view(P.dst); m.t_surf = P.d_inner.p;
+    m.st = view(P.dot); m.s_surf = P.d_inner.p; m.s_coeff = W.cbar.p;
+    m.c2e = nullptr; m.out = W.mbar.p;
+    launch_surf_stage<T>(P.kid, true, true, true, m, P.kc, s);
+    SurfStage mp = surf_base(P);
+    set_list(mp, P.a_m2p);
+    mp.tt = view(P.dst); mp.t_surf = P.d_inner.p;
+    mp.st = view(P.dot); mp.sp = view(P.op); mp.dens = W.gsort.p; mp.n_src_pts = No;
+    mp.c2e = nullptr; mp.out = W.mbar.p;
+    launch_surf_stage<T>(P.kid, false, true, true, mp, P.kc, s);
+
+    // M2M^T, root first: ebar = UC2E^T mbar, then push to children
+    for (int L = 0; L < P.st.n_levels; L++) {
+        const int c0 = P.st.level_begin[L], nc = P.st.level_begin[L + 1] - c0;
+        launch_c2e_transpose_apply(P.iota.p + c0, nc, P.dst.level.p, P.uc2e.p, P.blk, W.mbar.p, W.ebar.p, s);
+        if (L + 1 >= P.st.n_levels) break;
+        SurfStage c = surf_base(P);
+        set_list(c, P.a_m2m[L + 1]);
+        c.tt = view(P.dst); c.t_surf = P.d_inner.p;
+        c.st = view(P.dst); c.s_surf = P.d_outer.p; c.s_coeff = W.ebar.p;
+        c.c2e = nullptr; c.out = W.mbar.p;
+        launch_surf_stage<T>(P.kid, true, true, true, c, P.kc, s);
+    }
+
+    // P2P^T + P2L^T + P2M^T on every source leaf, times the source weights
+    LeafStage f{};
+    f.n_leaves = (int)P.st.leaves.size();
+    f.leaves = P.src_leaves.p;
+    f.tt = view(P.dst); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = Ns;
+    f.rp_ptr = P.a_leaf_p2p.ptr.p; f.rp_beg = P.a_leaf_p2p.rbeg.p; f.rp_cnt = P.a_leaf_p2p.rcnt.p;
+    f.ps = view(P.op); f.pdens = W.gsort.p; f.n_ps = No;
+    f.sl_ptr = P.a_leaf_p2l.ptr.p; f.sl_cell = P.a_leaf_p2l.cell.p;
+    f.slt = view(P.dot); f.sl_surf = P.d_inner.p; f.sl_coeff = W.cbar.p;
+    f.self_surf = P.d_outer.p; f.self_coeff = W.ebar.p;
+    f.unit = P.d_unit.p; f.n_e = P.n_e; f.blk = P.blk;
+    f.out_sorted = W.dens.p;
+    launch_leaf_stage<T>(P.kid, true, f, P.kc, s);
+    launch_scatter_out(W.dens.p, P.sp.perm.p, Ns, R, xbar, s);
+}
+
+static void forward(const FmmPlan& P, Work& W, const double* x, double* y, hipStream_t s) {
+    if (P.precision == TBEM_MIXED) forward_T<float>(P, W, x, y, s);
+    else forward_T<double>(P, W, x, y, s);
+}
+static void adjoint(const FmmPlan& P, Work& W, const double* g, double* xb, hipStream_t s) {
+    if (P.precision == TBEM_MIXED) adjoint_T<float>(P, W, g, xb, s);
+    else adjoint_T<double>(P, W, g, xb, s);
+}
+
+// -------------------------------------------------------------------- C boundary
+template <class F>
+static int guard(F&& f) {
+    try {
+        f();
+        return TBEM_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return TBEM_E_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return TBEM_E_INVALID;
+    }
+}
+
+static void host_roundtrip(tbem_fmm_s* h, bool adj, const double* in, double* out) {
+    FmmPlan& P = *h->plan;
+    Work& W = *h->work;
+    const size_t nin = adj ? P.R * P.n_obs : P.R * P.n_src;
+    const size_t nout = adj ? P.R * P.n_src : P.R * P.n_obs;
+    if (W.xbuf.n < nin) W.xbuf.alloc(nin);
+    if (W.ybuf.n < nout) W.ybuf.alloc(nout);
+    W.xbuf.upload(in, nin, W.stream);
+    if (adj) adjoint(P, W, W.xbuf.p, W.ybuf.p, W.stream);
+    else forward(P, W, W.xbuf.p, W.ybuf.p, W.stream);
+    W.ybuf.download(out, nout, W.stream);
+    TBEM_HIP(hipStreamSynchronize(W.stream));
+}
+
+}  // namespace tbem
+
+using namespace tbem;
+
+extern "C" {
+
+const char* tbem_last_error(void) { return g_last_error.c_str(); }
+
+int tbem_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n > 0 ? 1 : 0;
+}
+
+int tbem_fmm_create(const tbem_fmm_desc* desc, tbem_fmm_t* out) {
+    return guard([&] {
+        TBEM_REQUIRE(out != nullptr, TBEM_E_INVALID, "null output handle");
+        *out = nullptr;
+        hipStream_t s;
+        TBEM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        std::shared_ptr<FmmPlan> plan;
+        try {
+            plan = build_plan(desc, s);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+        auto h = std::unique_ptr<tbem_fmm_s>(new tbem_fmm_s);
+        h->plan = plan;
+        h->work.reset(new Work(*plan));
+        *out = h.release();
+    });
+}
+
+int tbem_fmm_apply(tbem_fmm_t h, const double* x, double* y) {
+    return guard([&] {
+        TBEM_REQUIRE(h && (x || h->plan->n_src == 0) && (y || h->plan->n_obs == 0), TBEM_E_INVALID,
+                     "null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
+        host_roundtrip(h, false, x, y);
+    });
+}
+
+int tbem_fmm_apply_dev(tbem_fmm_t h, const double* d_x, double* d_y, void* stream) {
+    return guard([&] {
+        TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
+        std::lock_guard<std::mutex> lk(h->mu);
+        hipStream_t s = stream ? (hipStream_t)stream : h->work->stream;
+        forward(*h->plan, *h->work, d_x, d_y, s);
+    });
+}
+
+int tbem_fmm_apply_adjoint(tbem_fmm_t h, const double* g, double* y) {
+    return guard([&] {
+        TBEM_REQUIRE(h && (g || h->plan->n_obs == 0) && (y || h->plan->n_src == 0), TBEM_E_INVALID,
+                     "null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
+        host_roundtrip(h, true, g, y);
+    });
+}
+
+int tbem_fmm_apply_adjoint_dev(tbem_fmm_t h, const double* d_g, double* d_y, void* stream) {
+    return guard([&] {
+        TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
+        std::lock_guard<std::mutex> lk(h->mu);
+        hipStream_t s = stream ? (hipStream_t)stream : h->work->stream;
+        adjoint(*h->plan, *h->work, d_g, d_y, s);
+    });
+}
+
+int tbem_fmm_obs_grad_dev(tbem_fmm_t h, const double* d_x, const double* d_g, double* d_grad, void* stream) {
+    return guard([&] {
+        TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
+        std::lock_guard<std::mutex> lk(h->mu);
+        hipStream_t s = stream ? (hipStream_t)stream : h->work->stream;
+        obs_grad(*h->plan, *h->work, d_x, d_g, d_grad, s);
+    });
+}
+
+int tbem_fmm_obs_grad(tbem_fmm_t h, const double* x, const double* g, double* grad) {
+    return guard([&] {
+        TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
+        std::lock_guard<std::mutex> lk(h->mu);
+        FmmPlan& P = *h->plan;
+        Work& W = *h->work;
+        const size_t nx = P.R * P.n_src, ng = P.R * P.n_obs;
+        TBEM_REQUIRE((x || nx == 0) && (g || ng == 0) && (grad || P.n_obs == 0), TBEM_E_INVALID,
+                     "null argument");
+        DevBuf<double> dx, dg, dgr(3 * P.n_obs);
+        dx.upload(x, nx, W.stream);
+        dg.upload(g, ng, W.stream);
+        obs_grad(P, W, dx.p, dg.p, dgr.p, W.stream);
+        dgr.download(grad, 3 * P.n_obs, W.stream);
+        TBEM_HIP(hipStreamSynchronize(W.stream));
+    });
+}
+
+// Moves the observation points of the (shared) plan; cells, surfaces and lists stay.
+int tbem_fmm_set_obs_locs(tbem_fmm_t h, const double* obs_locs) {
+    return guard([&] {
+        TBEM_REQUIRE(h && (obs_locs || h->plan->n_obs == 0), TBEM_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
+        FmmPlan& P = *h->plan;
+        if (P.n_obs == 0) return;
+        std::copy(obs_locs, obs_locs + 3 * P.n_obs, P.h_obs_locs.begin());
+        P.op.upload(P.ot, P.h_obs_locs.data(), P.h_obs_normals.data(), nullptr, h->work->stream);
+    });
+}
+
+size_t tbem_fmm_n_rows(tbem_fmm_t h) { return h ? h->plan->R * h->plan->n_obs : 0; }
+size_t tbem_fmm_n_cols(tbem_fmm_t h) { return h ? h->plan->R * h->plan->n_src : 0; }
+
+int tbem_fmm_clone(tbem_fmm_t h, tbem_fmm_t* out) {
+    return guard([&] {
+        TBEM_REQUIRE(h && out, TBEM_E_INVALID, "null argument");
+        auto c = std::unique_ptr<tbem_fmm_s>(new tbem_fmm_s);
+        c->plan = h->plan;
+        c->work.reset(new Work(*h->plan));
+        *out = c.release();
+    });
+}
+
+void tbem_fmm_destroy(tbem_fmm_t h) {
+    if (!h) return;
+    if (h->work) (void)hipStreamSynchronize(h->work->stream);
+    delete h;
+}
+
+int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
+    return guard([&] {
+        TBEM_REQUIRE(h && o, TBEM_E_INVALID, "null argument");
+        const FmmPlan& P = *h->plan;
+        std::memset(o, 0, sizeof(*o));
+        o->n_e = P.n_e;
+        o->n_src_cells = P.st.n_cells;
+        o->n_obs_cells = P.ot.n_cells;
+        o->n_up_levels = P.st.n_levels;
+        o->n_down_levels = P.ot.n_levels;
+        o->p2m = P.tl.p2m.size();
+        o->m2m = P.tl.m2m.size();
+        o->p2p = P.tl.p2p.size();
+        o->m2p = P.tl.m2p.size();
+        o->p2l = P.tl.p2l.size();
+        o->m2l = P.tl.m2l.size();
+        o->l2l = P.tl.l2l.size();
+        o->l2p = P.tl.l2p.size();
+        o->p2p_pairs = P.p2p_pairs;
+        o->plan_seconds = P.plan_seconds;
+    });
+}
+
+int tbem_fmm_c2e(tbem_fmm_t h, int which, size_t level, double* out) {
+    return guard([&] {
+        TBEM_REQUIRE(h && out, TBEM_E_INVALID, "null argument");
+        const FmmPlan& P = *h->plan;
+        const auto& v = which ? P.h_dc2e : P.h_uc2e;
+        const size_t bb = (size_t)P.blk * P.blk;
+        TBEM_REQUIRE((level + 1) * bb <= v.size(), TBEM_E_INVALID, "level out of range");
+        std::memcpy(out, v.data() + level * bb, bb * sizeof(double));
+    });
+}
+
+int tbem_fmm_pairs(tbem_fmm_t h, int which, double* out, size_t* n) {
+    return guard([&] {
+        TBEM_REQUIRE(h && n && which >= 0 && which <= 3, TBEM_E_INVALID, "bad argument");
+        const FmmPlan& P = *h->plan;
+        const auto& v = which == 0 ? P.tl.p2p : which == 1 ? P.tl.m2l : which == 2 ? P.tl.m2p : P.tl.p2l;
+        *n = v.size();
+        if (!out) return;
+        for (size_t i = 0; i < v.size(); i++) {
+            const HostTree* ts[2] = {&P.ot, &P.st};
+            for (int k = 0; k < 2; k++) {
+                const HostTree& t = *ts[k];
+                const int c = v[i][k];
+                double* o = out + 14 * i + 7 * k;
+                o[0] = t.level[c];
+                o[1] = t.cx[c]; o[2] = t.cy[c]; o[3] = t.cz[c];
+                for (int dd = 0; dd < 3; dd++) o[4 + dd] = t.level_hw[3 * t.level[c] + dd];
+            }
+        }
+    });
+}
+
+int tbem_fmm_octree(tbem_fmm_t h, int which, double* cells, long* indices, size_t* n_cells, size_t* n_idx) {
+    return guard([&] {
+        TBEM_REQUIRE(h && n_cells && n_idx, TBEM_E_INVALID, "null argument");
+        const HostTree& t = which ? h->plan->ot : h->plan->st;
+        // depth-first pre-order over the breadth-first cells
+        std::vector<int> order;
+        order.reserve(t.n_cells);
+        std::vector<int> stack{0};
+        while (!stack.empty() && t.n_cells > 0) {
+            int c = stack.back();
+            stack.pop_back();
+            order.push_back(c);
+            for (int k = t.n_child[c] - 1; k >= 0; k--) stack.push_back(t.first_child[c] + k);
+        }
+        size_t ni = 0;
+        for (int c: order) ni += t.count[c];
+        *n_cells = order.size();
+        *n_idx = ni;
+        size_t off = 0;
+        for (size_t k = 0; k < order.size(); k++) {
+            const int c = order[k];
+            if (cells) {
+                double* o = cells + 10 * k;
+                o[0] = t.level[c];
+                o[1] = t.cx[c]; o[2] = t.cy[c]; o[3] = t.cz[c];
+                for (int dd = 0; dd < 3; dd++) o[4 + dd] = t.level_hw[3 * t.level[c] + dd];
+                o[7] = t.leaf[c] ? 1.0 : 0.0;
+                o[8] = t.count[c];
+                o[9] = (double)off;
+            }
+            if (indices) {
+                // ascending original index inside every cell (leaves are; internal cells re-sorted)
+                std::vector<int> ids(t.perm.begin() + t.begin[c], t.perm.begin() + t.begin[c] + t.count[c]);
+                std::sort(ids.begin(), ids.end());
+                for (size_t q = 0; q < ids.size(); q++) indices[off + q] = ids[q];
+            }
+            off += t.count[c];
+        }
+    });
+}
+
+int tbem_fmm_stage_times(tbem_fmm_t h, double* ms) {
+    return guard([&] {
+        TBEM_REQUIRE(h && ms, TBEM_E_INVALID, "null argument");
+        Work& W = *h->work;
+        TBEM_REQUIRE(W.timed, TBEM_E_INVALID, "no timed apply yet");
+        TBEM_HIP(hipEventSynchronize(W.ev[7]));
+        for (int k = 0; k < 7; k++) {
+            float t = 0;
+            TBEM_HIP(hipEventElapsedTime(&t, W.ev[k], W.ev[k + 1]));
+            ms[k] = t;
+        }
+    });
+}
+
+long tbem_surface(size_t order, double* out) {
+    if (order == 0) return 0;
+    auto p = sphere_points(order);
+    if (out) std::memcpy(out, p.data(), p.size() * sizeof(double));
+    return (long)(p.size() / 3);
+}
+
+int tbem_pinv(const double* A, int n, double threshold, double* out) {
+    return guard([&] {
+        TBEM_REQUIRE(A && out && n > 0, TBEM_E_INVALID, "bad argument");
+        DevBuf<double> dA, dO((size_t)n * n);
+        dA.upload(A, (size_t)n * n);
+        gpu_pinv_batched(dA.p, 1, n, threshold > 0 ? threshold : 1e-16, dO.p, 0);
+        dO.download(out, (size_t)n * n);
+        TBEM_HIP(hipDeviceSynchronize());
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ kernel parity
+namespace tbem {
+__global__ void k_kernel_eval(int kid, KConst kc, int n, const double* obs, const double* src,
+                              const double* nobs, const double* nsrc, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int R = kernel_dim(kid);
+    tensor_ref_dispatch(kid, obs + 3 * i, src + 3 * i, nobs + 3 * i, nsrc + 3 * i, kc, out + (size_t)R * R * i);
+}
+}  // namespace tbem
+
+extern "C" int tbem_kernel_eval(int kernel, double mu, double nu, size_t n, const double* obs,
+                                const double* src, const double* nobs, const double* nsrc, double* out) {
+    return guard([&] {
+        TBEM_REQUIRE(kernel_valid(kernel), TBEM_E_INVALID, "unknown kernel id");
+        TBEM_REQUIRE(obs && src && nobs && nsrc && out, TBEM_E_INVALID, "null argument");
+        if (n == 0) return;
+        const int R = kernel_dim(kernel);
+        DevBuf<double> a, b, c, d, o((size_t)R * R * n);
+        a.upload(obs, 3 * n); b.upload(src, 3 * n); c.upload(nobs, 3 * n); d.upload(nsrc, 3 * n);
+        hipLaunchKernelGGL(k_kernel_eval, dim3(div_up(n, 256)), dim3(256), 0, 0, kernel,
+                           make_kconst(mu, nu), (int)n, a.p, b.p, c.p, d.p, o.p);
+        TBEM_LAUNCH_CHECK();
+        o.download(out, (size_t)R * R * n);
+        TBEM_HIP(hipDeviceSynchronize());
+    });
+}

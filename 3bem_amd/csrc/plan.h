@@ -1,0 +1,87 @@
+// Plan data structures: octrees, surfaces, C2E operators and task lists.
+#pragma once
+
+#include <array>
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace tbem {
+
+constexpr int kMaxKeyLevels = 21;  // 3 bits per level in a 63-bit Morton key
+
+// Octree in breadth-first numbering: level L's cells are [level_begin[L], level_begin[L+1]),
+// Morton (= depth-first) order inside a level, so the children of a cell are contiguous.
+// Points are permuted so that every cell is a contiguous range [begin, begin+count) and,
+// inside a leaf, original indices ascend (the order of Octree::indices, cpp/octree.cpp:125-136).
+struct HostTree {
+    int n_pts = 0;
+    int n_cells = 0;
+    int n_levels = 0;
+    std::vector<int> level, begin, count, first_child, n_child, parent, octant;
+    std::vector<char> leaf;
+    std::vector<double> cx, cy, cz;          // Box::center of each cell's bounds
+    std::vector<double> level_hw;            // 3 per level: Box::half_width (level 0 = expanded root)
+    std::vector<double> level_r;             // hypot(half_width) per level
+    std::vector<int> level_begin;            // n_levels + 1
+    std::vector<int> perm;                   // sorted position -> original index
+    std::vector<int> leaves;                 // leaf cell ids in depth-first order
+    std::vector<int> dfs;                    // all cell ids in depth-first pre-order
+};
+
+// Builds the octree of n AoS points on the GPU (Morton keys + radix sort), then
+// assembles cells from the sorted key runs on the host.
+HostTree build_octree(const double* pts_host, int n, int min_pts_per_cell, hipStream_t s);
+
+// Translation surfaces (cpp/fmm.h:42-66, cpp/fmm.cpp:22-51).
+struct Surfaces {
+    int n_e = 0;
+    std::vector<double> unit;   // unit sphere points = normals, 3*n_e
+    std::vector<double> inner;  // 0.3 * unit            (up-equivalent, down-check)
+    std::vector<double> outer;  // (4 - sqrt2 - 2d) * unit (up-check, down-equivalent)
+};
+Surfaces make_surfaces(size_t order, double d);
+std::vector<double> sphere_points(size_t order);
+
+// Device mirror of a HostTree (SoA).
+struct DevTree {
+    int n_cells = 0, n_levels = 0;
+    DevBuf<double> cx, cy, cz, level_r;
+    DevBuf<int> level, begin, count;
+    void upload(const HostTree& t, hipStream_t s);
+};
+
+struct DevPoints {
+    DevBuf<double> x, y, z, nx, ny, nz, w;  // sorted SoA
+    DevBuf<int> perm;                       // sorted -> original
+    void upload(const HostTree& t, const double* locs, const double* normals, const double* w,
+                hipStream_t s);
+};
+
+// A CSR list "item -> entries". Surface lists: entries are cell ids of the
+// source tree (cell[ptr[k] .. ptr[k+1]]). Point lists: entries are contiguous
+// ranges of sorted source points (rbeg/rcnt[ptr[k] .. ptr[k+1]]), the cells'
+// ranges merged where they touch.
+struct HostList {
+    std::vector<int> items, ptr, cell, rbeg, rcnt;
+};
+struct DevList {
+    int n_items = 0;
+    DevBuf<int> items, ptr, cell, rbeg, rcnt;
+    void upload(const HostList& l, hipStream_t s);
+};
+
+// Turn a cell list (ptr/cell) into merged point ranges of tree t (ptr/rbeg/rcnt).
+void cells_to_ranges(HostList& l, const HostTree& t);
+
+struct TaskLists {
+    std::vector<std::array<int, 2>> p2p, m2l, m2p, p2l;  // (obs cell, src cell)
+    std::vector<int> p2m, m2m, l2l, l2p;
+};
+
+void dual_tree_traversal(const HostTree& ot, const HostTree& st, double mac, int n_e,
+                         bool small_cells, TaskLists& tl);
+
+}  // namespace tbem

@@ -1,0 +1,98 @@
+// Device views and launchers of the FMM stage kernels.
+#pragma once
+
+#include "kernels.h"
+#include "plan.h"
+
+namespace tbem {
+
+constexpr int kBlock = 128;  // threads per workgroup of every stage kernel (two wave64s)
+
+struct TreeView {
+    const double *cx, *cy, *cz, *level_r;
+    const int *level, *begin, *count;
+};
+struct PtsView {
+    const double *x, *y, *z, *nx, *ny, *nz;
+};
+
+inline TreeView view(const DevTree& t) {
+    return TreeView{t.cx.p, t.cy.p, t.cz.p, t.level_r.p, t.level.p, t.begin.p, t.count.p};
+}
+inline PtsView view(const DevPoints& p) { return PtsView{p.x.p, p.y.p, p.z.p, p.nx.p, p.ny.p, p.nz.p}; }
+
+// Check-potential stage. For each item (a target cell of tree tt) accumulate, on
+// the cell's target surface, the field of the item's sources; then
+//   out[cell] (= or +=) C2E[level(cell)] * field     (c2e != nullptr)
+//   out[cell] (= or +=) field                        (c2e == nullptr)
+// Sources are either equivalent surfaces of cells of tree st carrying s_coeff
+// (src_surf: entries cell[ptr[item] .. ptr[item+1]]) or contiguous ranges of the
+// sorted source points carrying dens (!src_surf: ranges rbeg/rcnt[ptr[item] ..]).
+// adj = true evaluates the transposed kernel with the geometric roles swapped:
+// the target surface plays the kernel's source, each source point its observer.
+struct SurfStage {
+    int n_items;
+    const int *items, *ptr, *cell, *rbeg, *rcnt;
+    TreeView tt;
+    const double* t_surf;   // scaled target surface, 3*n_e
+    TreeView st;
+    const double* s_surf;   // scaled source surface (surface sources)
+    const double* s_coeff;  // blk per source cell
+    PtsView sp;
+    const double* dens;     // point densities, component-major in sorted order
+    int n_src_pts;
+    const double* unit;     // unit sphere = surface normals
+    int n_e, blk;
+    const double* c2e;      // per level blk*blk row-major, or nullptr
+    double* out;            // blk per cell of tt
+};
+
+// Point-target stage. For each leaf of tree tt, for each of its points i:
+//   out_sorted[q*n_tgt + i] = wscale_i * sum over
+//     (a) the sorted source points in its ranges                (P2P,   or P2P^T)
+//     (b) the surfaces (sl_surf) of the cells in its surface list (M2P,   or P2L^T)
+//     (c) its own surface self_surf with self_coeff             (L2P,   or P2M^T)
+// wscale_i = t_weight[i] when t_weight != nullptr, else 1.
+struct LeafStage {
+    int n_leaves;
+    const int* leaves;
+    TreeView tt;
+    PtsView tp;
+    const double* t_weight;
+    int n_tgt;
+    const int *rp_ptr, *rp_beg, *rp_cnt;   // point ranges, indexed by leaf position
+    PtsView ps;
+    const double* pdens;
+    int n_ps;
+    const int *sl_ptr, *sl_cell;           // indexed by leaf position
+    TreeView slt;
+    const double* sl_surf;
+    const double* sl_coeff;
+    const double* self_surf;
+    const double* self_coeff;
+    const double* unit;
+    int n_e, blk;
+    double* out_sorted;
+    const double* grad_g;   // obs-gradient mode: upstream g (sorted, component-major)
+    double* grad_out;       // obs-gradient mode: 3 per target (sorted, AoS)
+};
+
+template <class T>
+void launch_surf_stage(int kid, bool src_surf, bool add, bool adj, const SurfStage& a,
+                       const KConst& kc, hipStream_t s);
+template <class T>
+void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, hipStream_t s);
+// d/d(target coords) of the forward leaf stage contracted with grad_g
+void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s);
+
+// out[cell] = C2E[level(cell)]^T in[cell] for the listed cells (adjoint of the C2E step)
+void launch_c2e_transpose_apply(const int* cells, int n_cells, const int* level, const double* c2e,
+                                int blk, const double* in, double* out, hipStream_t s);
+
+void launch_gather_dens(const double* x, const int* perm, const double* w, int n, int R,
+                        double* dens, hipStream_t s);
+void launch_scatter_out(const double* out_sorted, const int* perm, int n, int R, double* y,
+                        hipStream_t s);
+void launch_fill(double* p, size_t n, double v, hipStream_t s);
+
+}  // namespace tbem

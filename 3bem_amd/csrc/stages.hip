@@ -1,0 +1,547 @@
+// FMM stage kernels for gfx950.
+//
+// Every stage is "accumulate the field of a stream of sources at a set of targets":
+//   * surface targets (P2M, M2M, M2L, P2L, L2L and their transposes): the n_e
+//     points of a cell's check/equivalent surface, then an FP64 C2E matvec;
+//   * point targets (P2P + M2P + L2P fused, or P2P^T + P2L^T + P2M^T): the
+//     points of one leaf.
+// One 128-thread workgroup (two wave64s) per target cell/leaf. Sources are staged
+// through LDS in 128-entry SoA tiles. Point sources come as contiguous ranges of
+// the sorted SoA arrays (neighbouring leaves of a list are merged at plan time,
+// Morton order makes most of them adjacent), so a tile is one coalesced load per
+// array; the next tile is loaded into registers while the current one is
+// computed. Surface sources are generated from the cell center and the scaled
+// unit sphere.
+// Lanes split into S = 128/ntp source slices that share the tile (ntp = target
+// slots rounded up to a power of two) and are summed through LDS at the end; in
+// the leaf stage each lane carries two targets, so one LDS read feeds two pairs.
+//
+// T = double: the plain FP64 path, absolute coordinates, so each pair's
+//   displacement is the reference's src - obs.
+// T = float: coordinates are taken relative to the target cell's center (formed
+//   in FP64, then rounded), so FP32 resolves near-field displacements; per-tile
+//   FP32 partial sums are folded into a Neumaier-compensated FP32 accumulator.
+#include <type_traits>
+
+#include "stages.h"
+
+namespace tbem {
+
+template <class T, int R, bool NRM>
+struct Tile {
+    T x[kBlock], y[kBlock], z[kBlock];
+    T nx[NRM ? kBlock : 1], ny[NRM ? kBlock : 1], nz[NRM ? kBlock : 1];
+    T d[R][kBlock];
+};
+
+// one staged source held in registers (FP64, absolute) until it is written to LDS
+template <int R, bool NRM>
+struct SrcReg {
+    double x, y, z, nx, ny, nz, d[R];
+};
+
+__device__ inline int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// moved surface point r*s + c, rounded as TranslationSurface::move (cpp/fmm.h:19-31)
+__device__ inline void moved(double r, const double* s, double cx, double cy, double cz, double* p) {
+#pragma clang fp contract(off)
+    p[0] = r * s[0] + cx;
+    p[1] = r * s[1] + cy;
+    p[2] = r * s[2] + cz;
+}
+
+template <class T, int R>
+struct Acc;
+
+template <int R>
+struct Acc<double, R> {
+    double t[R];
+    __device__ void init() {
+        for (int q = 0; q < R; q++) t[q] = 0;
+    }
+    __device__ double* tile() { return t; }
+    __device__ void flush() {}
+    __device__ double value(int q) const { return t[q]; }
+};
+
+// Per-tile FP32 partials folded into Neumaier-compensated FP32 sums.
+template <int R>
+struct Acc<float, R> {
+    float t[R], s[R], c[R];
+    __device__ void init() {
+        for (int q = 0; q < R; q++) t[q] = s[q] = c[q] = 0.f;
+    }
+    __device__ float* tile() { return t; }
+    __device__ void flush() {
+        for (int q = 0; q < R; q++) {
+            const float tt = s[q] + t[q];
+            if (fabsf(s[q]) >= fabsf(t[q])) c[q] += (s[q] - tt) + t[q];
+            else c[q] += (t[q] - tt) + s[q];
+            s[q] = tt;
+            t[q] = 0.f;
+        }
+    }
+    __device__ double value(int q) const { return (double)s[q] + (double)c[q]; }
+};
+
+template <class T>
+__device__ inline void origin(double cx, double cy, double cz, double* o) {
+    if (std::is_same<T, float>::value) {
+        o[0] = cx; o[1] = cy; o[2] = cz;
+    } else {
+        o[0] = o[1] = o[2] = 0.0;
+    }
+}
+
+// ----------------------------------------------------------------- source loaders
+template <int R, bool NRM>
+__device__ inline void load_point(SrcReg<R, NRM>& r, int k, const PtsView& sp, const double* __restrict__ dens,
+                                  int n_src) {
+    r.x = sp.x[k]; r.y = sp.y[k]; r.z = sp.z[k];
+    if (NRM) { r.nx = sp.nx[k]; r.ny = sp.ny[k]; r.nz = sp.nz[k]; }
+    for (int q = 0; q < R; q++) r.d[q] = dens[(size_t)q * n_src + k];
+}
+
+template <int R, bool NRM>
+__device__ inline void load_surf(SrcReg<R, NRM>& r, int c, int j, const TreeView& st, const double* __restrict__ surf,
+                                 const double* __restrict__ unit, const double* __restrict__ coeff, int n_e, int blk) {
+    double p[3];
+    moved(st.level_r[st.level[c]], surf + 3 * j, st.cx[c], st.cy[c], st.cz[c], p);
+    r.x = p[0]; r.y = p[1]; r.z = p[2];
+    if (NRM) { r.nx = unit[3 * j]; r.ny = unit[3 * j + 1]; r.nz = unit[3 * j + 2]; }
+    for (int q = 0; q < R; q++) r.d[q] = coeff[(size_t)c * blk + q * n_e + j];
+}
+
+template <class T, int R, bool NRM>
+__device__ inline void store_tile(Tile<T, R, NRM>& tl, int slot, const SrcReg<R, NRM>& r, const double* org) {
+    tl.x[slot] = T(r.x - org[0]);
+    tl.y[slot] = T(r.y - org[1]);
+    tl.z[slot] = T(r.z - org[2]);
+    if (NRM) { tl.nx[slot] = T(r.nx); tl.ny[slot] = T(r.ny); tl.nz[slot] = T(r.nz); }
+    for (int q = 0; q < R; q++) tl.d[q][slot] = T(r.d[q]);
+}
+
+// Streams point ranges [rbeg[e], rbeg[e]+rcnt[e]) for e in [e0, e1) through the
+// tile, prefetching the next tile into registers; compute(cnt) runs per tile.
+template <class T, int R, bool NRM, class F>
+__device__ inline void stream_ranges(Tile<T, R, NRM>& tl, int e0, int e1, const int* __restrict__ rbeg,
+                                     const int* __restrict__ rcnt, const PtsView& sp, const double* __restrict__ dens,
+                                     int n_src, const double* org, F&& compute) {
+    const int tid = threadIdx.x;
+    int e = e0, off = 0;
+    if (e >= e1) return;
+    SrcReg<R, NRM> r;
+    int cnt = min(kBlock, rcnt[e]);
+    if (tid < cnt) load_point(r, rbeg[e] + tid, sp, dens, n_src);
+    while (true) {
+        __syncthreads();
+        if (tid < cnt) store_tile(tl, tid, r, org);
+        __syncthreads();
+        const int cur = cnt;
+        off += kBlock;
+        if (off >= rcnt[e]) { e++; off = 0; }
+        const bool more = e < e1;
+        if (more) {
+            cnt = min(kBlock, rcnt[e] - off);
+            if (tid < cnt) load_point(r, rbeg[e] + off + tid, sp, dens, n_src);
+        }
+        compute(cur);
+        if (!more) break;
+    }
+}
+
+// Streams the equivalent/check surfaces of cells cell[e0..e1) (n_e points each).
+template <class T, int R, bool NRM, class F>
+__device__ inline void stream_surfaces(Tile<T, R, NRM>& tl, int e0, int e1, const int* __restrict__ cell,
+                                       const TreeView& st, const double* __restrict__ surf,
+                                       const double* __restrict__ unit, const double* __restrict__ coeff, int n_e,
+                                       int blk, const double* org, F&& compute) {
+    const int tid = threadIdx.x;
+    const int total = (e1 - e0) * n_e;
+    if (total <= 0) return;
+    SrcReg<R, NRM> r;
+    int base = 0;
+    auto fetch = [&](int b) {
+        const int v = b + tid;
+        if (v < total) {
+            const int k = v / n_e;
+            load_surf(r, cell[e0 + k], v - k * n_e, st, surf, unit, coeff, n_e, blk);
+        }
+    };
+    fetch(0);
+    while (true) {
+        const int cnt = min(kBlock, total - base);
+        __syncthreads();
+        if (tid < cnt) store_tile(tl, tid, r, org);
+        __syncthreads();
+        base += kBlock;
+        const bool more = base < total;
+        if (more) fetch(base);
+        compute(cnt);
+        if (!more) break;
+    }
+}
+
+// ---------------------------------------------------------------------- pair loop
+template <int KID, bool ADJ, class T, int R, bool NRM>
+__device__ __forceinline__ void pair(const Tile<T, R, NRM>& tl, int j, T tx, T ty, T tz, const T* tn, T* acc,
+                                     const KC<T>& kc) {
+    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
+    T v[R];
+    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+    if (!ADJ) {
+        const T dx = tl.x[j] - tx, dy = tl.y[j] - ty, dz = tl.z[j] - tz;
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        kop<KID, false, T>(dx, dy, dz, r2, tn, sn, v, acc, kc);
+    } else {
+        const T dx = tx - tl.x[j], dy = ty - tl.y[j], dz = tz - tl.z[j];
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        kop<KID, true, T>(dx, dy, dz, r2, sn, tn, v, acc, kc);
+    }
+}
+
+// ------------------------------------------------------------------ surface targets
+template <int KID, class T, bool SRC_SURF, bool ADJ>
+__global__ __launch_bounds__(kBlock) void k_surf_stage(SurfStage a, KC<T> kc, double scale, int add) {
+    constexpr int R = kernel_dim(KID);
+    constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
+    __shared__ Tile<T, R, NRM> tl;
+    __shared__ double red[R][kBlock];
+    extern __shared__ double field[];  // R * n_e
+    const int item = blockIdx.x;
+    const int cell = a.items[item];
+    const int lvl = a.tt.level[cell];
+    const double rc = a.tt.level_r[lvl];
+    const double cx = a.tt.cx[cell], cy = a.tt.cy[cell], cz = a.tt.cz[cell];
+    double org[3];
+    origin<T>(cx, cy, cz, org);
+    const int e0 = a.ptr[item], e1 = a.ptr[item + 1];
+    const int tid = threadIdx.x;
+
+    for (int t0 = 0; t0 < a.n_e; t0 += kBlock) {
+        const int nt = min(kBlock, a.n_e - t0);
+        const int ntp = next_pow2(nt), S = kBlock / ntp;
+        const int tg = tid % ntp, slice = tid / ntp;
+        const bool active = tg < nt;
+        const int i = t0 + (active ? tg : 0);
+        double p[3];
+        moved(rc, a.t_surf + 3 * i, cx, cy, cz, p);
+        const T tx = T(p[0] - org[0]), ty = T(p[1] - org[1]), tz = T(p[2] - org[2]);
+        const T tn[3] = {T(a.unit[3 * i]), T(a.unit[3 * i + 1]), T(a.unit[3 * i + 2])};
+        Acc<T, R> acc;
+        acc.init();
+        auto compute = [&](int cnt) {
+            if (active)
+                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(tl, j, tx, ty, tz, tn, acc.tile(), kc);
+            acc.flush();
+        };
+        if (SRC_SURF)
+            stream_surfaces(tl, e0, e1, a.cell, a.st, a.s_surf, a.unit, a.s_coeff, a.n_e, a.blk, org, compute);
+        else
+            stream_ranges(tl, e0, e1, a.rbeg, a.rcnt, a.sp, a.dens, a.n_src_pts, org, compute);
+        __syncthreads();
+        for (int q = 0; q < R; q++) red[q][tid] = active ? acc.value(q) : 0.0;
+        __syncthreads();
+        if (slice == 0 && active) {
+            for (int q = 0; q < R; q++) {
+                double s = 0;
+                for (int k = 0; k < S; k++) s += red[q][k * ntp + tg];
+                field[q * a.n_e + i] = s * scale;
+            }
+        }
+        __syncthreads();
+    }
+    double* o = a.out + (size_t)cell * a.blk;
+    if (a.c2e) {
+        const double* M = a.c2e + (size_t)lvl * a.blk * a.blk;
+        for (int row = tid; row < a.blk; row += kBlock) {
+            double s = 0;
+            const double* m = M + (size_t)row * a.blk;
+            for (int col = 0; col < a.blk; col++) s += m[col] * field[col];
+            o[row] = add ? o[row] + s : s;
+        }
+    } else {
+        for (int row = tid; row < a.blk; row += kBlock) o[row] = add ? o[row] + field[row] : field[row];
+    }
+}
+
+// -------------------------------------------------------------------- point targets
+// Two targets per lane: slot tg holds targets t0+tg and t0+nslot+tg.
+template <int KID, class T, bool ADJ>
+__global__ __launch_bounds__(kBlock) void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
+    constexpr int R = kernel_dim(KID);
+    constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
+    __shared__ Tile<T, R, NRM> tl;
+    __shared__ double red[2][R][kBlock];
+    const int pos = blockIdx.x;
+    const int leaf = a.leaves[pos];
+    const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
+    double org[3];
+    origin<T>(a.tt.cx[leaf], a.tt.cy[leaf], a.tt.cz[leaf], org);
+    const int pe0 = a.rp_ptr ? a.rp_ptr[pos] : 0, pe1 = a.rp_ptr ? a.rp_ptr[pos + 1] : 0;
+    const int se0 = a.sl_ptr ? a.sl_ptr[pos] : 0, se1 = a.sl_ptr ? a.sl_ptr[pos + 1] : 0;
+    const int tid = threadIdx.x;
+
+    for (int t0 = 0; t0 < m; t0 += 2 * kBlock) {
+        const int nt = min(2 * kBlock, m - t0);
+        const int nslot = (nt + 1) >> 1;
+        const int ntp = next_pow2(nslot), S = kBlock / ntp;
+        const int tg = tid % ntp, slice = tid / ntp;
+        const bool va = tg < nslot, vb = nslot + tg < nt;
+        const int ia = b + t0 + (va ? tg : 0);
+        const int ib = b + t0 + (vb ? nslot + tg : 0);
+        const T ax = T(a.tp.x[ia] - org[0]), ay = T(a.tp.y[ia] - org[1]), az = T(a.tp.z[ia] - org[2]);
+        const T bx = T(a.tp.x[ib] - org[0]), by = T(a.tp.y[ib] - org[1]), bz = T(a.tp.z[ib] - org[2]);
+        const T an[3] = {NRM ? T(a.tp.nx[ia]) : T(0), NRM ? T(a.tp.ny[ia]) : T(0), NRM ? T(a.tp.nz[ia]) : T(0)};
+        const T bn[3] = {NRM ? T(a.tp.nx[ib]) : T(0), NRM ? T(a.tp.ny[ib]) : T(0), NRM ? T(a.tp.nz[ib]) : T(0)};
+        Acc<T, R> acA, acB;
+        acA.init();
+        acB.init();
+        auto compute = [&](int cnt) {
+            if (va) {
+                for (int j = slice; j < cnt; j += S) {
+                    pair<KID, ADJ>(tl, j, ax, ay, az, an, acA.tile(), kc);
+                    pair<KID, ADJ>(tl, j, bx, by, bz, bn, acB.tile(), kc);
+                }
+            }
+            acA.flush();
+            acB.flush();
+        };
+        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.ps, a.pdens, a.n_ps, org, compute);       // (a)
+        stream_surfaces(tl, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org,
+                        compute);                                                                     // (b)
+        if (a.self_coeff)
+            stream_surfaces(tl, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org,
+                            compute);                                                                 // (c)
+        __syncthreads();
+        for (int q = 0; q < R; q++) {
+            red[0][q][tid] = va ? acA.value(q) : 0.0;
+            red[1][q][tid] = vb ? acB.value(q) : 0.0;
+        }
+        __syncthreads();
+        if (slice == 0) {
+            for (int h = 0; h < 2; h++) {
+                const bool ok = h == 0 ? va : vb;
+                if (!ok) continue;
+                const int i = h == 0 ? ia : ib;
+                const double w = a.t_weight ? a.t_weight[i] : 1.0;
+                for (int q = 0; q < R; q++) {
+                    double s = 0;
+                    for (int k = 0; k < S; k++) s += red[h][q][k * ntp + tg];
+                    a.out_sorted[(size_t)q * a.n_tgt + i] = w * (s * scale);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------- observation-coordinate gradient
+// Forward leaf stage (P2P + M2P + L2P) evaluated on dual numbers seeded on the
+// target coordinates; writes grad[3*i + m] = sum_q g_q(i) d u_q(i) / d x_i[m].
+template <int KID>
+__global__ __launch_bounds__(kBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, double scale) {
+    constexpr int R = kernel_dim(KID);
+    constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
+    __shared__ Tile<double, R, NRM> tl;
+    __shared__ double red[3][kBlock];
+    const int pos = blockIdx.x;
+    const int leaf = a.leaves[pos];
+    const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
+    const double org[3] = {0, 0, 0};
+    const int pe0 = a.rp_ptr ? a.rp_ptr[pos] : 0, pe1 = a.rp_ptr ? a.rp_ptr[pos + 1] : 0;
+    const int se0 = a.sl_ptr ? a.sl_ptr[pos] : 0, se1 = a.sl_ptr ? a.sl_ptr[pos + 1] : 0;
+    const int tid = threadIdx.x;
+    for (int t0 = 0; t0 < m; t0 += kBlock) {
+        const int nt = min(kBlock, m - t0);
+        const int ntp = next_pow2(nt), S = kBlock / ntp;
+        const int tg = tid % ntp, slice = tid / ntp;
+        const bool active = tg < nt;
+        const int i = b + t0 + (active ? tg : 0);
+        const D3 tx(a.tp.x[i], 1, 0, 0), ty(a.tp.y[i], 0, 1, 0), tz(a.tp.z[i], 0, 0, 1);
+        const D3 tn[3] = {D3(NRM ? a.tp.nx[i] : 0.0), D3(NRM ? a.tp.ny[i] : 0.0), D3(NRM ? a.tp.nz[i] : 0.0)};
+        D3 acc[R];
+        auto compute = [&](int cnt) {
+            if (!active) return;
+            for (int j = slice; j < cnt; j += S) {
+                const D3 sn[3] = {D3(NRM ? tl.nx[j] : 0.0), D3(NRM ? tl.ny[j] : 0.0), D3(NRM ? tl.nz[j] : 0.0)};
+                D3 v[R];
+                for (int q = 0; q < R; q++) v[q] = D3(tl.d[q][j]);
+                const D3 dx = D3(tl.x[j]) - tx, dy = D3(tl.y[j]) - ty, dz = D3(tl.z[j]) - tz;
+                const D3 r2 = dx * dx + dy * dy + dz * dz;
+                kop<KID, false, D3>(dx, dy, dz, r2, tn, sn, v, acc, kc);
+            }
+        };
+        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.ps, a.pdens, a.n_ps, org, compute);
+        stream_surfaces(tl, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org, compute);
+        if (a.self_coeff)
+            stream_surfaces(tl, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org, compute);
+        double gr[3] = {0, 0, 0};
+        if (active) {
+            for (int q = 0; q < R; q++) {
+                const double gq = a.grad_g[(size_t)q * a.n_tgt + i];
+                gr[0] += gq * acc[q].d0;
+                gr[1] += gq * acc[q].d1;
+                gr[2] += gq * acc[q].d2;
+            }
+        }
+        __syncthreads();
+        for (int mm = 0; mm < 3; mm++) red[mm][tid] = gr[mm];
+        __syncthreads();
+        if (slice == 0 && active) {
+            for (int mm = 0; mm < 3; mm++) {
+                double s = 0;
+                for (int k = 0; k < S; k++) s += red[mm][k * ntp + tg];
+                a.grad_out[3 * (size_t)i + mm] = s * scale;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------ helpers
+__global__ void k_c2e_transpose_apply(const int* __restrict__ cells, const int* __restrict__ level,
+                                      const double* __restrict__ c2e, int blk, const double* __restrict__ in,
+                                      double* __restrict__ out) {
+    const int cell = cells[blockIdx.x];
+    const double* M = c2e + (size_t)level[cell] * blk * blk;
+    const double* x = in + (size_t)cell * blk;
+    for (int col = threadIdx.x; col < blk; col += blockDim.x) {
+        double s = 0;
+        for (int row = 0; row < blk; row++) s += M[(size_t)row * blk + col] * x[row];
+        out[(size_t)cell * blk + col] = s;
+    }
+}
+
+__global__ void k_gather_dens(const double* __restrict__ x, const int* __restrict__ perm,
+                              const double* __restrict__ w, int n, int R, double* __restrict__ dens) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = perm[i];
+    const double wi = w ? w[i] : 1.0;
+    for (int q = 0; q < R; q++) dens[(size_t)q * n + i] = x[(size_t)q * n + g] * wi;
+}
+
+__global__ void k_scatter_out(const double* __restrict__ out_sorted, const int* __restrict__ perm, int n,
+                              int R, double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = perm[i];
+    for (int q = 0; q < R; q++) y[(size_t)q * n + g] = out_sorted[(size_t)q * n + i];
+}
+
+__global__ void k_fill(double* __restrict__ p, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void launch_c2e_transpose_apply(const int* cells, int n_cells, const int* level, const double* c2e,
+                                int blk, const double* in, double* out, hipStream_t s) {
+    if (n_cells == 0) return;
+    hipLaunchKernelGGL(k_c2e_transpose_apply, dim3(n_cells), dim3(kBlock), 0, s, cells, level, c2e, blk, in, out);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_gather_dens(const double* x, const int* perm, const double* w, int n, int R, double* dens,
+                        hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_gather_dens, dim3(div_up(n, 256)), dim3(256), 0, s, x, perm, w, n, R, dens);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_scatter_out(const double* out_sorted, const int* perm, int n, int R, double* y, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_scatter_out, dim3(div_up(n, 256)), dim3(256), 0, s, out_sorted, perm, n, R, y);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_fill(double* p, size_t n, double v, hipStream_t s) {
+    if (n == 0) return;
+    int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, p, n, v);
+    TBEM_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------------ dispatch
+template <class T>
+static KC<T> kc_of(const KConst& c);
+template <>
+KC<double> kc_of<double>(const KConst& c) { return make_kc64(c); }
+template <>
+KC<float> kc_of<float>(const KConst& c) { return make_kc32(c); }
+
+template <int KID, class T>
+static void surf_launch(bool src_surf, bool add, bool adj, const SurfStage& a, const KConst& c, hipStream_t s) {
+    if (a.n_items == 0) return;
+    const KC<T> kc = kc_of<T>(c);
+    const double scale = (double)kop_scale<KID, double>(make_kc64(c));
+    const size_t shm = (size_t)kernel_dim(KID) * a.n_e * sizeof(double);
+    dim3 g(a.n_items), b(kBlock);
+    if (src_surf) {
+        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, T, true, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+        else hipLaunchKernelGGL((k_surf_stage<KID, T, true, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+    } else {
+        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, T, false, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+        else hipLaunchKernelGGL((k_surf_stage<KID, T, false, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+    }
+    TBEM_LAUNCH_CHECK();
+}
+
+template <int KID, class T>
+static void leaf_launch(bool adj, const LeafStage& a, const KConst& c, hipStream_t s) {
+    if (a.n_leaves == 0) return;
+    const KC<T> kc = kc_of<T>(c);
+    const double scale = (double)kop_scale<KID, double>(make_kc64(c));
+    if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true>), dim3(a.n_leaves), dim3(kBlock), 0, s, a, kc, scale);
+    else hipLaunchKernelGGL((k_leaf_stage<KID, T, false>), dim3(a.n_leaves), dim3(kBlock), 0, s, a, kc, scale);
+    TBEM_LAUNCH_CHECK();
+}
+
+template <int KID, class T>
+static void grad_launch(const LeafStage& a, const KConst& c, hipStream_t s) {
+    if (a.n_leaves == 0) return;
+    const double scale = (double)kop_scale<KID, double>(make_kc64(c));
+    hipLaunchKernelGGL((k_leaf_grad<KID>), dim3(a.n_leaves), dim3(kBlock), 0, s, a, make_kcd(c), scale);
+    TBEM_LAUNCH_CHECK();
+}
+
+#define TBEM_KID_SWITCH(kid, F, ...)                                                  \
+    switch (kid) {                                                                    \
+    case K_LS: F<K_LS, T>(__VA_ARGS__); break;                                        \
+    case K_LD: F<K_LD, T>(__VA_ARGS__); break;                                        \
+    case K_LH: F<K_LH, T>(__VA_ARGS__); break;                                        \
+    case K_EU: F<K_EU, T>(__VA_ARGS__); break;                                        \
+    case K_ET: F<K_ET, T>(__VA_ARGS__); break;                                        \
+    case K_EA: F<K_EA, T>(__VA_ARGS__); break;                                        \
+    case K_EH: F<K_EH, T>(__VA_ARGS__); break;                                        \
+    case K_ID1: F<K_ID1, T>(__VA_ARGS__); break;                                      \
+    case K_ID3: F<K_ID3, T>(__VA_ARGS__); break;                                      \
+    default: throw Error(TBEM_E_INVALID, "unknown kernel id");                        \
+    }
+
+template <class T>
+void launch_surf_stage(int kid, bool src_surf, bool add, bool adj, const SurfStage& a, const KConst& kc,
+                       hipStream_t s) {
+    TBEM_KID_SWITCH(kid, surf_launch, src_surf, add, adj, a, kc, s);
+}
+
+template <class T>
+void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, hipStream_t s) {
+    TBEM_KID_SWITCH(kid, leaf_launch, adj, a, kc, s);
+}
+
+void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s) {
+    using T = double;
+    TBEM_KID_SWITCH(kid, grad_launch, a, kc, s);
+}
+
+template void launch_surf_stage<double>(int, bool, bool, bool, const SurfStage&, const KConst&, hipStream_t);
+template void launch_surf_stage<float>(int, bool, bool, bool, const SurfStage&, const KConst&, hipStream_t);
+template void launch_leaf_stage<double>(int, bool, const LeafStage&, const KConst&, hipStream_t);
+template void launch_leaf_stage<float>(int, bool, const LeafStage&, const KConst&, hipStream_t);
+
+}  // namespace tbem

@@ -1,0 +1,154 @@
+"""Synthetic point sets of the benchmark configurations (seeded, deterministic).
+
+Conventions (the reference has no generator for any of them):
+  * geometry from numpy.random.default_rng(seed), densities from default_rng(seed + 1);
+  * surface points carry unit normals and quadrature weights (area / point), as
+    nbody_src_from_bem forms them (weight = rule weight x Jacobian,
+    cpp/nbody_operator.h:64-73);
+  * volume points (cube, clusters) get random unit normals and weight 1.
+"""
+import numpy as np
+
+SEED = 20261202
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def icosphere(refinements=4):
+    """Unit icosahedron subdivided `refinements` times by edge midpoints pushed to the sphere."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [_unit(np.array(p, dtype=np.float64)) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(refinements):
+        cache = {}
+        new = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                verts.append(_unit((verts[a] + verts[b]) / 2.0))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            new += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = new
+    return np.array(verts), np.array(faces, dtype=np.int64)
+
+
+def tri_three_point(verts, tris, outward_from=None):
+    """3-point rule at barycentric (2/3,1/6,1/6) and permutations, weight = area/3."""
+    p0, p1, p2 = verts[tris[:, 0]], verts[tris[:, 1]], verts[tris[:, 2]]
+    cr = np.cross(p1 - p0, p2 - p0)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    n = _unit(cr)
+    if outward_from is not None:
+        cen = (p0 + p1 + p2) / 3.0
+        flip = np.sum(n * (cen - outward_from), axis=1) < 0
+        n[flip] *= -1
+    bary = np.array([[2 / 3, 1 / 6, 1 / 6], [1 / 6, 2 / 3, 1 / 6], [1 / 6, 1 / 6, 2 / 3]])
+    pts = np.einsum("qk,tkd->tqd", bary, np.stack([p0, p1, p2], axis=1)).reshape(-1, 3)
+    normals = np.repeat(n, 3, axis=0)
+    weights = np.repeat(area / 3.0, 3)
+    return pts, normals, weights
+
+
+def cfg1_icosphere(seed=SEED):
+    """15,360 points: 5,120-triangle icosphere, 3 points per triangle, outward normals."""
+    v, f = icosphere(4)
+    pts, nrm, w = tri_three_point(v, f, outward_from=np.zeros(3))
+    x = np.random.default_rng(seed + 1).standard_normal(len(pts))
+    return dict(locs=pts, normals=nrm, weights=w, x=x)
+
+
+def cfg2_fault_and_surface(n_total=1_000_000, seed=SEED):
+    """Half the points uniform on the free surface [-1/2,1/2]^2 x {0}, half uniform on a
+    0.5 x 0.5 fault plane dipping 45 degrees about the x axis, centred at depth 0.3
+    (centre (0, 0, -0.3); along-strike x, down-dip direction (0, cos45, -sin45))."""
+    rng = np.random.default_rng(seed)
+    n1 = n_total // 2
+    n2 = n_total - n1
+    s = np.column_stack([rng.uniform(-0.5, 0.5, n1), rng.uniform(-0.5, 0.5, n1), np.zeros(n1)])
+    sn = np.tile([0.0, 0.0, 1.0], (n1, 1))
+    sw = np.full(n1, 1.0 / n1)
+    a, b = rng.uniform(-0.25, 0.25, n2), rng.uniform(-0.25, 0.25, n2)
+    c45 = np.sqrt(0.5)
+    f = np.column_stack([a, c45 * b, -0.3 - c45 * b])
+    fn = np.tile([0.0, c45, c45], (n2, 1))
+    fw = np.full(n2, 0.25 / n2)
+    locs = np.vstack([s, f])
+    x = np.random.default_rng(seed + 1).standard_normal(3 * n_total)
+    return dict(locs=locs, normals=np.vstack([sn, fn]), weights=np.concatenate([sw, fw]), x=x)
+
+
+def cfg3_cube_clusters(n_total=4_000_000, seed=SEED):
+    """Half uniform in [0,1]^3, half from 8 Gaussian clusters (sigma 0.02, centres uniform)."""
+    rng = np.random.default_rng(seed)
+    n1 = n_total // 2
+    n2 = n_total - n1
+    u = rng.uniform(0, 1, (n1, 3))
+    centres = rng.uniform(0, 1, (8, 3))
+    lab = rng.integers(0, 8, n2)
+    g = centres[lab] + 0.02 * rng.standard_normal((n2, 3))
+    locs = np.vstack([u, g])
+    normals = _unit(rng.standard_normal((n_total, 3)))
+    x = np.random.default_rng(seed + 1).standard_normal(n_total)
+    return dict(locs=locs, normals=normals, weights=np.ones(n_total), x=x)
+
+
+def cfg4_fault_and_stations(n_tri=40_000, n_obs=200_000, seed=SEED):
+    """Planar rectangular fault (1 x 0.5, dipping 45 degrees, top at depth 0.1) meshed into
+    n_tri triangles (3 points each) and n_obs surface stations uniform in [-1,1]^2 x {0}."""
+    rng = np.random.default_rng(seed)
+    nx = int(np.sqrt(n_tri / 4.0))
+    ny = max(1, n_tri // (2 * 2 * nx))
+    while 2 * nx * ny < n_tri:
+        nx += 1
+    xs = np.linspace(-0.5, 0.5, nx + 1)
+    ys = np.linspace(0.0, 0.5, ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    c45 = np.sqrt(0.5)
+    verts = np.column_stack([X.ravel(), c45 * Y.ravel(), -0.1 - c45 * Y.ravel()])
+    tris = []
+    for i in range(nx):
+        for j in range(ny):
+            a = i * (ny + 1) + j
+            b, c, d = a + ny + 1, a + 1, a + ny + 2
+            tris += [(a, b, c), (c, b, d)]
+    tris = np.array(tris[:n_tri], dtype=np.int64)
+    src, sn, sw = tri_three_point(verts, tris)
+    obs = np.column_stack([rng.uniform(-1, 1, n_obs), rng.uniform(-1, 1, n_obs), np.zeros(n_obs)])
+    on = np.tile([0.0, 0.0, 1.0], (n_obs, 1))
+    slip = np.random.default_rng(seed + 1).standard_normal(3 * len(src))
+    slip_true = np.random.default_rng(seed + 2).standard_normal(3 * len(src))
+    return dict(src=src, src_normals=sn, src_weights=sw, obs=obs, obs_normals=on, x=slip,
+                x_true=slip_true)
+
+
+def cfg5_rough_fault(n_total=2_000_000, seed=SEED):
+    """n_total points on z = 0.05 sin(6x) cos(4y) + 0.005 * N(0,1) roughness over [0,1]^2."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, n_total)
+    y = rng.uniform(0, 1, n_total)
+    z = 0.05 * np.sin(6 * x) * np.cos(4 * y) + 0.005 * rng.standard_normal(n_total)
+    dzdx = 0.3 * np.cos(6 * x) * np.cos(4 * y)
+    dzdy = -0.2 * np.sin(6 * x) * np.sin(4 * y)
+    n = _unit(np.column_stack([-dzdx, -dzdy, np.ones(n_total)]))
+    w = np.sqrt(1 + dzdx ** 2 + dzdy ** 2) / n_total
+    return dict(locs=np.column_stack([x, y, z]), normals=n, weights=w)
+
+
+def uniform_cube(n, seed=SEED, dim=1):
+    rng = np.random.default_rng(seed)
+    locs = rng.uniform(0, 1, (n, 3))
+    normals = _unit(rng.standard_normal((n, 3)))
+    weights = rng.uniform(0.5, 1.5, n)
+    x = np.random.default_rng(seed + 1).standard_normal(dim * n)
+    return dict(locs=locs, normals=normals, weights=weights, x=x)

@@ -1,0 +1,216 @@
+"""Benchmark: FMM matvec throughput on one MI355X.
+
+Default workload: plain FP64 elastic-displacement (Kelvin, mu=1, nu=0.25) FMM
+matvec on 1M points (500k uniform on the free surface [-1/2,1/2]^2 x {0}, 500k
+on a 0.5 x 0.5 fault dipping 45 degrees centred at depth 0.3), 3-component N(0,1)
+densities, leaf 64, order 6, MAC 0.3, d 0.05.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2|cfg1|cfg3|cfg5] [--n N]
+
+One JSON line on stdout. `value` is FP64 Mpts/s = N_obs / apply time with inputs
+resident in HBM (device pointers). Beside it: the mixed FP32 path, relative L2
+error of both against an FP64 direct sum on a 10k-target subsample (GPU FP64
+direct operator, itself checked against the CPU oracle in tests/), the
+backward/forward time ratio (adjoint matvec + observation-coordinate gradient vs
+forward), the roofline of the dominant kernel (the fused P2P+M2P+L2P leaf stage)
+timed with HIP events on its stream, and the compiled reference CPU FMM timed on
+this host on a bounded sample.
+"""
+import argparse
+import importlib
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FMM matvec Mpts/s at stated rel err, FP32-MFMA vs FP64; bwd/fwd time ratio"
+# algorithmic flops per source-target pair (FMA = 2, rsqrt = 1), per kernel id
+FLOPS_PER_PAIR = {0: 11, 1: 19, 2: 0, 3: 28, 4: 60, 5: 60, 6: 120, 7: 2, 8: 6}
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name, n):
+    wl = importlib.import_module("3bem_amd.workloads")
+    if name == "cfg1":
+        g = wl.cfg1_icosphere()
+        return dict(kernel=0, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
+                    cfg=(0.3, 5, 100, 0.05, False),
+                    desc="Laplace single-layer FMM matvec, icosphere 5120 triangles x 3 points = 15,360 "
+                         "points, order 5, leaf 100, MAC 0.3")
+    if name == "cfg2":
+        g = wl.cfg2_fault_and_surface(n or 1_000_000)
+        return dict(kernel=3, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
+                    cfg=(0.3, 6, 64, 0.05, False),
+                    desc=f"elastic displacement (Kelvin) FMM matvec, {len(g['locs'])} points (half free "
+                         "surface, half 45-degree fault), mu=1 nu=0.25, order 6, leaf 64, MAC 0.3")
+    if name == "cfg3":
+        g = wl.cfg3_cube_clusters(n or 4_000_000)
+        return dict(kernel=0, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
+                    cfg=(0.3, 75, 100, 0.05, False),
+                    desc=f"Laplace single-layer FMM matvec, {len(g['locs'])} points (cube + 8 clusters), "
+                         "order 75, leaf 100, MAC 0.3")
+    if name == "cfg5":
+        g = wl.cfg5_rough_fault(n or 2_000_000)
+        x = np.random.default_rng(wl.SEED + 1).standard_normal(3 * len(g["locs"]))
+        return dict(kernel=6, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=x,
+                    cfg=(0.3, 6, 64, 0.05, False),
+                    desc=f"elastic hypersingular FMM matvec, {len(g['locs'])} points on a rough fault, "
+                         "order 6, leaf 64, MAC 0.3")
+    raise SystemExit(f"unknown config {name}")
+
+
+def time_steps(fn, steps, warmup, torch, after=None):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+        if after is not None:
+            per.append(after())
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, per
+
+
+def cpu_baseline(kernel, cfg, budget_n):
+    """Compiled reference FMM (oracle/_ref) on a bounded sample of the same geometry;
+    falls back to the oracle restatement when the reference build is absent."""
+    from oracle.oracle import Oracle, available
+    wl = importlib.import_module("3bem_amd.workloads")
+    kind = "reference" if available("ref") else "port"
+    o = Oracle("ref" if kind == "reference" else "orc")
+    g = wl.cfg2_fault_and_surface(budget_n) if kernel == 3 else wl.uniform_cube(budget_n, dim=3 if kernel >= 3 else 1)
+    t0 = time.perf_counter()
+    op = o.fmm(kernel, g["locs"], g["normals"], g["locs"], g["normals"], g["weights"], *cfg)
+    t_plan = time.perf_counter() - t0
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        op.apply(g["x"][: (3 if kernel >= 3 else 1) * budget_n])
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    return {"value": round(budget_n / t / 1e6, 5), "unit": "Mpts/s", "cores": cores, "kind": kind,
+            "sample": f"same geometry family at {budget_n} points, FMMOperator::apply median of 2 runs "
+                      f"({t:.2f} s each; plan build {t_plan:.2f} s excluded), OMP threads={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--cpu-n", type=int, default=100_000)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-extra", action="store_true", help="FP64 forward only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the FMM "
+              "sharding over several GPUs is described in DESIGN.md but not implemented", file=sys.stderr)
+        raise SystemExit(2)
+
+    import torch
+    tbem = importlib.import_module("3bem_amd")
+    torch.cuda.set_device(0)
+    w = workload(args.config, args.n)
+    kid = w["kernel"]
+    R = 1 if kid in (0, 1, 2, 7) else 3
+    n = len(w["locs"])
+    K = tbem.make_kernel(kid, 1.0, 0.25)
+    data = tbem.NBodyData(w["locs"], w["normals"], w["locs"], w["normals"], w["weights"])
+    cfg = tbem.FMMConfig(*w["cfg"])
+    log(f"[bench] {w['desc']}")
+
+    t0 = time.perf_counter()
+    op64 = tbem.FMMOperator(K, data, cfg, precision="fp64")
+    plan64 = time.perf_counter() - t0
+    info = op64.info()
+    log(f"[bench] fp64 plan {plan64:.2f}s: " + ", ".join(f"{k}={info[k]}" for k in
+        ("n_e", "n_src_cells", "n_up_levels", "p2p", "m2l", "p2p_pairs")))
+    x = torch.tensor(w["x"], device="cuda")
+
+    leaf_ms = []
+    dt64, leaf_ms = time_steps(lambda: op64.apply(x), args.steps, args.warmup, torch,
+                               after=lambda: op64.stage_times()["leaf"])
+    stages = op64.stage_times()
+    value = n / dt64 / 1e6
+    log(f"[bench] fp64 apply {dt64 * 1e3:.2f} ms -> {value:.2f} Mpts/s; stages {stages}")
+
+    leaf_avg_ms = float(np.mean(leaf_ms))
+    alg_flops = FLOPS_PER_PAIR[kid] * (info["p2p_pairs"] + n * info["n_e"])
+    achieved = alg_flops / (leaf_avg_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "pipe": "fp64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
+                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "kernel": "k_leaf_stage (P2P+M2P+L2P)", "kernel_avg_ms": round(leaf_avg_ms, 4),
+                "algorithmic_flop": alg_flops}
+
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(dt64 * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": w["desc"], "n_points": n, "kernel": type(K).__name__,
+                      "mac": cfg.mac, "order": cfg.order, "n_e": info["n_e"], "min_pts_per_cell": cfg.min_pts_per_cell,
+                      "plan_build_s": round(plan64, 3), "stage_ms": {k: round(v, 4) for k, v in stages.items()}},
+           "roofline": roofline}
+
+    if not args.skip_extra:
+        # accuracy vs FP64 direct sum on a 10k-target subsample
+        rng = np.random.default_rng(123)
+        sub = np.sort(rng.choice(n, min(n, 10_000), replace=False))
+        dsub = tbem.NBodyData(w["locs"][sub], w["normals"][sub], w["locs"], w["normals"], w["weights"])
+        direct = tbem.make_direct_nbody_operator(dsub, K)
+        t0 = time.perf_counter()
+        yd = direct.apply(w["x"])
+        t_direct = time.perf_counter() - t0
+        idx = np.concatenate([q * n + sub for q in range(R)])
+        y64 = op64.apply(x).cpu().numpy()
+        e64 = float(np.linalg.norm(y64[idx] - yd) / np.linalg.norm(yd))
+
+        t0 = time.perf_counter()
+        op32 = tbem.FMMOperator(K, data, cfg, precision="mixed")
+        plan32 = time.perf_counter() - t0
+        dt32, _ = time_steps(lambda: op32.apply(x), args.steps, args.warmup, torch)
+        y32 = op32.apply(x).cpu().numpy()
+        e32 = float(np.linalg.norm(y32[idx] - yd) / np.linalg.norm(yd))
+        e32_64 = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
+
+        g = torch.randn(R * n, dtype=torch.float64, device="cuda")
+        dt_adj, _ = time_steps(lambda: op64.apply_adjoint(g), max(2, args.steps // 2), 1, torch)
+        dt_grad, _ = time_steps(lambda: op64.obs_grad(x, g), max(2, args.steps // 2), 1, torch)
+        out.update({
+            "rel_err_fp64_vs_direct": e64,
+            "mixed": {"value": round(n / dt32 / 1e6, 3), "ms_per_step": round(dt32 * 1e3, 4), "rel_err_vs_direct": e32,
+                      "rel_err_vs_fp64_fmm": e32_64, "speedup_vs_fp64": round(dt64 / dt32, 3),
+                      "plan_build_s": round(plan32, 3)},
+            "bwd_fwd_ratio": round((dt_adj + dt_grad) / dt64, 3),
+            "bwd_ms": {"adjoint": round(dt_adj * 1e3, 3), "obs_grad": round(dt_grad * 1e3, 3)},
+            "direct_subsample_s": round(t_direct, 3),
+        })
+        log(f"[bench] rel err fp64 {e64:.3e}, mixed {e32:.3e}; mixed {dt32 * 1e3:.2f} ms; "
+            f"adjoint {dt_adj * 1e3:.2f} ms, obs_grad {dt_grad * 1e3:.2f} ms")
+
+    if not args.skip_cpu:
+        cfg_t = (cfg.mac, cfg.order, cfg.min_pts_per_cell, cfg.d, cfg.account_for_small_cells)
+        cpu_n = min(args.cpu_n, n)
+        log(f"[bench] CPU baseline on {cpu_n} points ...")
+        out["cpu_baseline"] = cpu_baseline(kid, cfg_t, cpu_n)
+        log(f"[bench] CPU baseline {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

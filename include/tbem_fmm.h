@@ -1,0 +1,237 @@
+/*
+ * tbem_fmm.h — C ABI of the MI355X (gfx950) kernel-independent FMM matvec.
+ *
+ * Drop-in boundary for 3bem's FMM and direct N-body operators. Plain C types
+ * only: pointers, sizes, ints and doubles. The C++ shim in include/tbem/fmm.hpp
+ * rebuilds the reference's operator surface on top of these entry points.
+ *
+ * Layouts follow the reference:
+ *   - point arrays are AoS xyz, as NBodyData's std::array<double,3>
+ *     (cpp/nbody_operator.h:29-36);
+ *   - densities x and outputs y are component-major, x[d*N_src + j] and
+ *     y[d*N_obs + i] (cpp/fmm.cpp:189, :289).
+ *
+ * Errors: every int-returning call gives 0 on success and a nonzero TBEM_E_*
+ * code otherwise; tbem_last_error() then describes the failure (thread-local).
+ * The reference only asserts (cpp/fmm.cpp:462-526, cpp/blas_wrapper.cpp:87).
+ *
+ * Threading: each handle owns one HIP stream; calls on one handle serialize on
+ * it, separate handles are independent. The reference apply() is const and
+ * re-entrant (cpp/fmm.cpp:550-558).
+ */
+#ifndef TBEM_FMM_H
+#define TBEM_FMM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Kernel ids. Each maps to one reference kernel class:
+ *   TBEM_K_LAPLACE_SINGLE        LaplaceSingle<3>            cpp/laplace_kernels.h:18-32
+ *   TBEM_K_LAPLACE_DOUBLE        LaplaceDouble<3>            cpp/laplace_kernels.h:34-48
+ *   TBEM_K_LAPLACE_HYPERSINGULAR LaplaceHypersingular<3> (0) cpp/laplace_kernels.h:50-64
+ *   TBEM_K_ELASTIC_DISPLACEMENT  ElasticDisplacement<3>      cpp/elastic_kernels.h:191-227
+ *   TBEM_K_ELASTIC_TRACTION      ElasticTraction<3>          cpp/elastic_kernels.h:229-269
+ *   TBEM_K_ELASTIC_ADJOINT_TRACTION ElasticAdjointTraction<3> cpp/elastic_kernels.h:271-311
+ *   TBEM_K_ELASTIC_HYPERSINGULAR ElasticHypersingular<3>     cpp/elastic_kernels.h:313-365
+ *   TBEM_K_IDENTITY_SCALAR       IdentityScalar<3>           cpp/identity_kernels.h:9-39
+ *   TBEM_K_IDENTITY_TENSOR       IdentityTensor<3,3,3>       cpp/identity_kernels.h:9-39
+ */
+enum {
+    TBEM_K_LAPLACE_SINGLE = 0,
+    TBEM_K_LAPLACE_DOUBLE = 1,
+    TBEM_K_LAPLACE_HYPERSINGULAR = 2,
+    TBEM_K_ELASTIC_DISPLACEMENT = 3,
+    TBEM_K_ELASTIC_TRACTION = 4,
+    TBEM_K_ELASTIC_ADJOINT_TRACTION = 5,
+    TBEM_K_ELASTIC_HYPERSINGULAR = 6,
+    TBEM_K_IDENTITY_SCALAR = 7,
+    TBEM_K_IDENTITY_TENSOR = 8
+};
+
+/* Arithmetic of the apply path.
+ *   TBEM_FP64  : every stage in FP64 (the plain path, parity with the reference FMM).
+ *   TBEM_MIXED : P2P and the point<->surface stages in FP32 with compensated
+ *                accumulation; M2L as per-offset FP32 MFMA GEMMs; the check-to-
+ *                equivalent pseudo-inverses (UC2E/DC2E) stay FP64. */
+enum { TBEM_FP64 = 0, TBEM_MIXED = 1 };
+
+enum {
+    TBEM_OK = 0,
+    TBEM_E_INVALID = 1,   /* bad argument (null pointer, zero order, wrong size, ...) */
+    TBEM_E_HIP = 2,       /* a HIP runtime call failed */
+    TBEM_E_NOMEM = 3,     /* device or host allocation failed */
+    TBEM_E_UNSUPPORTED = 4,
+    TBEM_E_NUMERIC = 5    /* e.g. the SVD did not converge */
+};
+
+typedef struct tbem_fmm_s* tbem_fmm_t;
+typedef struct tbem_nbody_s* tbem_nbody_t;
+
+/* Everything FMMOperator<3,R,C>(K, NBodyData, FMMConfig) receives
+ * (cpp/fmm.h:146-147, cpp/fmm.h:73-96, cpp/nbody_operator.h:29-36). */
+typedef struct {
+    int kernel;              /* TBEM_K_* */
+    double shear_modulus;    /* elastic kernels only (cpp/elastic_kernels.h:194-204) */
+    double poisson_ratio;
+
+    size_t n_obs;
+    const double* obs_locs;      /* 3*n_obs, AoS */
+    const double* obs_normals;   /* 3*n_obs, AoS */
+    size_t n_src;
+    const double* src_locs;      /* 3*n_src, AoS */
+    const double* src_normals;   /* 3*n_src, AoS */
+    const double* src_weights;   /* n_src */
+
+    /* FMMConfig{mac, order, min_pts_per_cell, d, account_for_small_cells} */
+    double mac;
+    size_t order;
+    size_t min_pts_per_cell;
+    double d;
+    int account_for_small_cells;
+
+    int precision;               /* TBEM_FP64 or TBEM_MIXED */
+
+    /* Optional: per-level check-to-equivalent matrices to use instead of the
+     * ones computed on the GPU (row-major (R*n_e)^2 blocks, levels
+     * concatenated). NULL = compute them. Used for bit-level parity runs. */
+    const double* up_check_to_equiv;
+    size_t n_up_levels;
+    const double* down_check_to_equiv;
+    size_t n_down_levels;
+} tbem_fmm_desc;
+
+/* Plan statistics (tbem_fmm_info). */
+typedef struct {
+    size_t n_e;             /* points per translation surface (cpp/fmm.cpp:22-45) */
+    size_t n_src_cells, n_obs_cells;
+    size_t n_up_levels, n_down_levels;
+    size_t p2m, m2m, p2p, m2p, p2l, m2l, l2l, l2p;   /* task counts as FMMTasks (cpp/fmm.h:98-120) */
+    double p2p_pairs;        /* sum over P2P tasks of n_obs*n_src */
+    double m2l_classes;      /* distinct translation classes (MIXED) */
+    double plan_seconds;     /* wall time of tbem_fmm_create */
+} tbem_fmm_info_t;
+
+/* ---------------------------------------------------------------- FMM operator */
+
+/* Replaces FMMOperator<3,R,C>::FMMOperator (cpp/fmm.cpp:71-101): copies the
+ * inputs to the device, builds both octrees, the C2E matrices and the task lists. */
+int tbem_fmm_create(const tbem_fmm_desc* desc, tbem_fmm_t* out);
+
+/* Replaces FMMOperator::apply (cpp/fmm.cpp:550-558). Host pointers:
+ * x has n_cols() doubles, y receives n_rows() doubles. */
+int tbem_fmm_apply(tbem_fmm_t h, const double* x, double* y);
+
+/* Same on device pointers, enqueued on `stream` (a hipStream_t; NULL = the
+ * handle's own stream). No host synchronisation. */
+int tbem_fmm_apply_dev(tbem_fmm_t h, const double* d_x, double* d_y, void* stream);
+
+/* Exact transpose of the FMM operator (every stage transposed, roles of the
+ * two trees swapped): g has n_rows() doubles, y receives n_cols(). No reference
+ * counterpart: 3bem has no adjoint; it is the backward of apply. */
+int tbem_fmm_apply_adjoint(tbem_fmm_t h, const double* g, double* y);
+int tbem_fmm_apply_adjoint_dev(tbem_fmm_t h, const double* d_g, double* d_y, void* stream);
+
+/* Gradient of 0.5*|y|^2-style losses w.r.t. observation coordinates: with
+ * u = A(obs) x and upstream g = dL/du (n_rows()), writes
+ * grad[3*i + m] = sum_k g[k*N_obs+i] d u[k*N_obs+i] / d obs_i[m] (AoS, 3*n_obs).
+ * The plan geometry (trees, surfaces) is held fixed. No reference counterpart. */
+int tbem_fmm_obs_grad(tbem_fmm_t h, const double* x, const double* g, double* grad);
+int tbem_fmm_obs_grad_dev(tbem_fmm_t h, const double* d_x, const double* d_g, double* d_grad,
+                          void* stream);
+
+/* Move observation points without rebuilding the plan (same tree cells, same
+ * surfaces); for finite-difference checks of tbem_fmm_obs_grad. */
+int tbem_fmm_set_obs_locs(tbem_fmm_t h, const double* obs_locs);
+
+/* OperatorI::n_rows / n_cols (cpp/fmm.h:235-236). */
+size_t tbem_fmm_n_rows(tbem_fmm_t h);
+size_t tbem_fmm_n_cols(tbem_fmm_t h);
+
+/* OperatorI::clone (cpp/fmm.cpp:103-109). The reference rebuilds the plan;
+ * here the immutable device plan is shared by reference count. */
+int tbem_fmm_clone(tbem_fmm_t h, tbem_fmm_t* out);
+
+void tbem_fmm_destroy(tbem_fmm_t h);
+
+int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* out);
+
+/* Copy of the level-`level` UC2E (which=0) or DC2E (which=1) matrix,
+ * (R*n_e)^2 doubles, row-major — FMMOperator::up/down_check_to_equiv. */
+int tbem_fmm_c2e(tbem_fmm_t h, int which, size_t level, double* out);
+
+/* Interaction lists for parity checks: which = 0 p2p, 1 m2l, 2 m2p, 3 p2l.
+ * 14 doubles per pair: obs cell (level, center xyz, half width xyz), then the
+ * src cell likewise. Pass out=NULL to get the count in *n. */
+int tbem_fmm_pairs(tbem_fmm_t h, int which, double* out, size_t* n);
+
+/* Octree of the source (which=0) or observation (which=1) points, cells in
+ * depth-first pre-order (children 0..7): 10 doubles per cell (level, center
+ * xyz, half width xyz, is_leaf, n_indices, offset) and concatenated index
+ * lists, as Octree<3> (cpp/octree.h:14-29). NULL outputs return sizes only. */
+int tbem_fmm_octree(tbem_fmm_t h, int which, double* cells, long* indices,
+                    size_t* n_cells, size_t* n_indices);
+
+/* Per-stage device times of the last apply, milliseconds, in the order
+ * gather, p2m, m2m, m2l, l2l, p2p(+m2p+l2p), scatter (7 doubles). */
+int tbem_fmm_stage_times(tbem_fmm_t h, double* ms7);
+
+/* ---------------------------------------------------------- direct N-body operator */
+
+/* Matrix-free replacement for make_direct_nbody_operator(data, K)
+ * (cpp/nbody_operator.h:143-152) + DenseOperator::apply (cpp/dense_operator.cpp:38-46)
+ * and for nbody_eval (cpp/nbody_operator.h:117-141): y_i = sum_j w_j K(x_i, y_j) x_j
+ * evaluated on the GPU without forming the dense matrix. */
+typedef struct {
+    int kernel;
+    double shear_modulus, poisson_ratio;
+    size_t n_obs;
+    const double* obs_locs;
+    const double* obs_normals;
+    size_t n_src;
+    const double* src_locs;
+    const double* src_normals;
+    const double* src_weights;
+    int precision;           /* TBEM_FP64 or TBEM_MIXED */
+} tbem_nbody_desc;
+
+int tbem_nbody_create(const tbem_nbody_desc* desc, tbem_nbody_t* out);
+int tbem_nbody_apply(tbem_nbody_t h, const double* x, double* y);
+int tbem_nbody_apply_dev(tbem_nbody_t h, const double* d_x, double* d_y, void* stream);
+/* y_j = w_j sum_i K(x_i, y_j)^T g_i — transpose of the direct operator. */
+int tbem_nbody_apply_adjoint(tbem_nbody_t h, const double* g, double* y);
+size_t tbem_nbody_n_rows(tbem_nbody_t h);
+size_t tbem_nbody_n_cols(tbem_nbody_t h);
+void tbem_nbody_destroy(tbem_nbody_t h);
+
+/* ----------------------------------------------------------------------- misc */
+
+/* One kernel evaluation K(obs <- src) on the GPU, R*C doubles row-major
+ * (Kernel<3,R,C>::operator(), cpp/kernel.h:19-33). For kernel parity tests. */
+int tbem_kernel_eval(int kernel, double shear_modulus, double poisson_ratio, size_t n,
+                     const double* obs, const double* src, const double* nobs,
+                     const double* nsrc, double* out);
+
+/* Translation-surface points for `order` (unit sphere), n_e*3 doubles
+ * (TranslationSurface<3>::make_surrounding_surface, cpp/fmm.cpp:22-51).
+ * Returns n_e; out may be NULL. */
+long tbem_surface(size_t order, double* out);
+
+/* Thresholded FP64 pseudo-inverse of a row-major n x n matrix on the GPU
+ * (one-sided Jacobi SVD); singular values <= threshold*sigma_max are dropped.
+ * Replaces svd_decompose + set_threshold + svd_pseudoinverse
+ * (cpp/blas_wrapper.cpp:67-127). */
+int tbem_pinv(const double* A, int n, double threshold, double* out);
+
+const char* tbem_last_error(void);
+
+/* 1 if a HIP device is usable, else 0 (no error raised). */
+int tbem_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TBEM_FMM_H */

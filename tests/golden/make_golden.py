@@ -1,0 +1,117 @@
+"""Writes tests/golden/*.npz from the compiled reference (oracle/_ref/libtbem_ref.so).
+
+Run here, where /root/reference exists to build oracle/_ref:
+    make -C oracle ref && OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+
+Single-threaded: the reference's OpenMP atomics make multi-threaded output vary
+at ~1e-12. Inputs are seeded, so each file holds both the inputs
+and the reference outputs. test_data_elastic3 is the reference's own sympy
+golden data for the 3D elastic kernels (unit_tests/test_data_elastic3), copied
+as a data fixture.
+"""
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+import importlib  # noqa: E402
+
+from oracle.oracle import Oracle, components  # noqa: E402
+
+wl = importlib.import_module("3bem_amd.workloads")
+
+assert os.environ.get("OMP_NUM_THREADS") == "1", "run with OMP_NUM_THREADS=1"
+ref = Oracle("ref")
+
+KERNELS = {0: "laplace_single", 1: "laplace_double", 3: "elastic_u", 4: "elastic_t", 5: "elastic_a",
+           6: "elastic_h", 7: "identity_scalar", 8: "identity_tensor"}
+
+
+def fmm_case(kernel, obs, on, src, sn, w, x, mac, order, leaf, d, small, name, with_lists=False):
+    op = ref.fmm(kernel, obs, on, src, sn, w, mac, order, leaf, d, small)
+    y = op.apply(x)
+    c = op.counts()
+    out = dict(kernel=kernel, obs=obs, obs_normals=on, src=src, src_normals=sn, weights=w, x=x,
+               mac=mac, order=order, leaf=leaf, d=d, small=int(small), y=y,
+               counts=np.array([c[k] for k in sorted(c)]), count_names=np.array(sorted(c)))
+    out["direct"] = ref.direct(kernel, obs, on, src, sn, w, x)
+    if with_lists:
+        out["uc2e"] = np.stack([op.c2e(False, L) for L in range(c["n_up_levels"])])
+        out["dc2e"] = np.stack([op.c2e(True, L) for L in range(c["n_down_levels"])])
+        out["p2p"] = op.pairs("p2p")
+        out["m2l"] = op.pairs("m2l")
+        out["m2p"] = op.pairs("m2p")
+        out["p2l"] = op.pairs("p2l")
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    return y
+
+
+def main():
+    shutil.copy("/root/reference/unit_tests/test_data_elastic3", os.path.join(HERE, "test_data_elastic3"))
+
+    # config 1: Laplace single layer on the icosphere, order 5, leaf 100, MAC 0.3
+    g = wl.cfg1_icosphere()
+    fmm_case(0, g["locs"], g["normals"], g["locs"], g["normals"], g["weights"], g["x"],
+             0.3, 5, 100, 0.05, False, "cfg1_laplace_icosphere")
+
+    # every kernel on a small uniform cloud, obs != src, with task lists and C2E
+    rng = np.random.default_rng(7)
+    n = 600
+    src = rng.uniform(0, 1, (n, 3))
+    obs = rng.uniform(0, 1, (n, 3))
+    sn = rng.standard_normal((n, 3)); sn /= np.linalg.norm(sn, axis=1)[:, None]
+    on = rng.standard_normal((n, 3)); on /= np.linalg.norm(on, axis=1)[:, None]
+    w = rng.uniform(0.5, 1.5, n)
+    for k, name in KERNELS.items():
+        x = rng.standard_normal(components(k) * n)
+        for small in (False, True):
+            fmm_case(k, obs, on, src, sn, w, x, 0.3, 6, 20, 0.05, small,
+                     f"small_{name}_{'sc' if small else 'nosc'}", with_lists=(k in (0, 3)))
+
+    # same point set for obs and src (self-interaction hits the r^2 cutoff), elastic U
+    x = rng.standard_normal(3 * n)
+    fmm_case(3, src, sn, src, sn, w, x, 0.3, 6, 20, 0.05, False, "self_elastic_u", with_lists=True)
+
+    # config-2-like geometry (fault + free surface), 20k points, elastic U, order 6, leaf 64
+    c2 = wl.cfg2_fault_and_surface(20000)
+    fmm_case(3, c2["locs"], c2["normals"], c2["locs"], c2["normals"], c2["weights"], c2["x"],
+             0.3, 6, 64, 0.05, False, "cfg2_small_elastic_u")
+
+    # higher order Laplace (order 35 / 75), reference unit test tolerance setting
+    for order, k in ((35, 0), (75, 1)):
+        m = 300
+        s = rng.uniform(0, 1, (m, 3)); o = rng.uniform(0, 1, (m, 3))
+        nn = rng.uniform(0, 1, (m, 3))
+        fmm_case(k, o, nn, s, nn, np.ones(m), np.ones(m), 0.35, order, 1, 0.05, False,
+                 f"highorder_{KERNELS[k]}_{order}", with_lists=True)
+
+    # octrees: random cloud, clustered cloud, degenerate line, duplicates
+    trees = {}
+    line = np.array([[float(i + k), 0.0, 0.0] for i in range(10) for k in (0, 1)])
+    dup = np.array([[1.0, 2.0, 0.0], [1.0, 2.0, 0.0]])
+    clus = np.vstack([rng.normal(0.3, 1e-3, (400, 3)), rng.uniform(0, 1, (200, 3))])
+    for name, pts, leaf in (("random", rng.uniform(-1, 1, (2000, 3)), 4), ("cluster", clus, 10),
+                            ("line", line, 1), ("dup", dup, 1), ("icosphere", g["locs"], 100)):
+        cells, idx = ref.octree(pts, leaf)
+        trees[f"{name}_pts"] = pts
+        trees[f"{name}_leaf"] = leaf
+        trees[f"{name}_cells"] = cells
+        trees[f"{name}_idx"] = idx
+    np.savez_compressed(os.path.join(HERE, "octrees.npz"), **trees)
+
+    # surfaces and the thresholded pseudo-inverse
+    surf = {f"order_{o}": ref.surface(o) for o in (1, 2, 5, 6, 10, 20, 35, 50, 75, 100, 150)}
+    A = rng.standard_normal((12, 12))
+    A[:, -1] = A[:, 0] * (1 + 1e-13)
+    surf["pinv_A"] = A
+    surf["pinv_A_1e-10"] = ref.pinv(A, 1e-10)
+    np.savez_compressed(os.path.join(HERE, "surfaces.npz"), **surf)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
