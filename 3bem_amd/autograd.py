@@ -1,0 +1,50 @@
+"""Differentiable FMM matvec for torch.
+
+    u = fmm_matvec(op, x, obs_locs)      # x: (R*N_src,) float64 CUDA tensor
+                                         # obs_locs: (N_obs, 3) float64 CUDA tensor or None
+
+Backward (3bem has no autograd; this is the hand-written backward of apply()):
+  * dL/dx        = A^T g      -- the exact transpose of the FMM operator
+                                (FMMOperator.apply_adjoint: every stage transposed,
+                                source and target roles swapped);
+  * dL/dobs_locs = obs_grad   -- gradient kernels for the near field and the
+                                local expansions (FMMOperator.obs_grad).
+The plan (trees, surfaces, interaction lists) is held fixed: moving observation
+points only changes the point coordinates inside their cells, so the forward is
+a smooth function of them and central differences of the frozen-plan forward
+reproduce the gradient.
+"""
+import torch
+
+
+class _FMMMatvec(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, obs_locs, op):
+        if obs_locs is not None:
+            op.set_obs_locs(obs_locs.detach().cpu().numpy())
+        ctx.op = op
+        ctx.save_for_backward(x)
+        ctx.has_obs = obs_locs is not None
+        return op.apply(x.contiguous())
+
+    @staticmethod
+    def backward(ctx, g):
+        (x,) = ctx.saved_tensors
+        op = ctx.op
+        g = g.contiguous()
+        gx = op.apply_adjoint(g) if ctx.needs_input_grad[0] else None
+        gobs = None
+        if ctx.has_obs and ctx.needs_input_grad[1]:
+            gobs = op.obs_grad(x.contiguous(), g)
+        return gx, gobs, None
+
+
+def fmm_matvec(op, x, obs_locs=None):
+    """Differentiable y = A(obs_locs) x on the GPU (see module docstring)."""
+    return _FMMMatvec.apply(x, obs_locs, op)
+
+
+def forward_model_loss(op, slip, u_obs, obs_locs=None):
+    """0.5 * ||A(obs) slip - u_obs||^2 (the slip-inversion misfit)."""
+    u = fmm_matvec(op, slip, obs_locs)
+    return 0.5 * torch.sum((u - u_obs) ** 2)

@@ -64,9 +64,24 @@ def build(verbose=True, jobs=None):
         print(r.stdout.decode(errors="replace"), file=sys.stderr)
         raise RuntimeError("HIP link failed")
     os.replace(tmp, LIB)
+    build_shim_example()
     if verbose:
         print(f"built {LIB} in {time.time() - t0:.1f}s")
     return LIB
+
+
+SHIM_EXAMPLE = os.path.join(HERE, "tbem_shim_example")
+
+
+def build_shim_example():
+    """C++ caller of the library through include/tbem/fmm.hpp (host code only, g++)."""
+    src = os.path.join(ROOT, "tests", "cpp", "shim_example.cpp")
+    cmd = ["g++", "-O2", "-std=c++14", "-I", os.path.join(ROOT, "include"), src, "-L", HERE, "-ltbem_fmm",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", SHIM_EXAMPLE]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        print(r.stdout.decode(errors="replace"), file=sys.stderr)
+        raise RuntimeError("shim example build failed")
 
 
 if __name__ == "__main__":

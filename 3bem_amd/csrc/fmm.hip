@@ -62,16 +62,24 @@ struct Work {
     DevBuf<double> dens, mult, locals, out_sorted, xbuf, ybuf;
     DevBuf<double> gsort, lbar, cbar, mbar, ebar, grad_sorted;
     hipEvent_t ev[8];
+    hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
     explicit Work(const FmmPlan& P) {
         TBEM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto& e: ev) TBEM_HIP(hipEventCreate(&e));
+        TBEM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        TBEM_HIP(hipEventRecord(done, stream));
         dens.alloc(P.R * P.n_src);
         mult.alloc((size_t)P.dst.n_cells * P.blk);
         locals.alloc((size_t)P.dot.n_cells * P.blk);
         out_sorted.alloc(P.R * P.n_obs);
     }
+    // The work buffers belong to the handle: a call on stream s first waits for
+    // the previous call (possibly on another stream) and marks its own end.
+    void begin(hipStream_t s) { TBEM_HIP(hipStreamWaitEvent(s, done, 0)); }
+    void end(hipStream_t s) { TBEM_HIP(hipEventRecord(done, s)); }
     ~Work() {
+        if (done) (void)hipEventDestroy(done);
         for (auto& e: ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -284,6 +292,7 @@ static void set_list(SurfStage& a, const DevList& l) {
     a.cell = l.cell.p;
     a.rbeg = l.rbeg.p;
     a.rcnt = l.rcnt.p;
+    a.rpre = l.rpre.p;
 }
 
 // gather, upward pass, M2L/P2L and L2L: everything the leaf stage consumes
@@ -351,6 +360,7 @@ static LeafStage leaf_stage_args(const FmmPlan& P, Work& W) {
     f.leaves = P.obs_leaves.p;
     f.tt = view(P.dot); f.tp = view(P.op); f.t_weight = nullptr; f.n_tgt = No;
     f.rp_ptr = P.leaf_p2p.ptr.p; f.rp_beg = P.leaf_p2p.rbeg.p; f.rp_cnt = P.leaf_p2p.rcnt.p;
+    f.rp_pre = P.leaf_p2p.rpre.p;
     f.ps = view(P.sp); f.pdens = W.dens.p; f.n_ps = Ns;
     f.sl_ptr = P.leaf_m2p.ptr.p; f.sl_cell = P.leaf_m2p.cell.p;
     f.slt = view(P.dst); f.sl_surf = P.d_inner.p; f.sl_coeff = W.mult.p;
@@ -483,6 +493,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     f.leaves = P.src_leaves.p;
     f.tt = view(P.dst); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = Ns;
     f.rp_ptr = P.a_leaf_p2p.ptr.p; f.rp_beg = P.a_leaf_p2p.rbeg.p; f.rp_cnt = P.a_leaf_p2p.rcnt.p;
+    f.rp_pre = P.a_leaf_p2p.rpre.p;
     f.ps = view(P.op); f.pdens = W.gsort.p; f.n_ps = No;
     f.sl_ptr = P.a_leaf_p2l.ptr.p; f.sl_cell = P.a_leaf_p2l.cell.p;
     f.slt = view(P.dot); f.sl_surf = P.d_inner.p; f.sl_coeff = W.cbar.p;
@@ -494,12 +505,16 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
 }
 
 static void forward(const FmmPlan& P, Work& W, const double* x, double* y, hipStream_t s) {
+    W.begin(s);
     if (P.precision == TBEM_MIXED) forward_T<float>(P, W, x, y, s);
     else forward_T<double>(P, W, x, y, s);
+    W.end(s);
 }
 static void adjoint(const FmmPlan& P, Work& W, const double* g, double* xb, hipStream_t s) {
+    W.begin(s);
     if (P.precision == TBEM_MIXED) adjoint_T<float>(P, W, g, xb, s);
     else adjoint_T<double>(P, W, g, xb, s);
+    W.end(s);
 }
 
 // -------------------------------------------------------------------- C boundary
@@ -585,8 +600,7 @@ int tbem_fmm_apply_dev(tbem_fmm_t h, const double* d_x, double* d_y, void* strea
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
-        hipStream_t s = stream ? (hipStream_t)stream : h->work->stream;
-        forward(*h->plan, *h->work, d_x, d_y, s);
+        forward(*h->plan, *h->work, d_x, d_y, (hipStream_t)stream);
     });
 }
 
@@ -603,8 +617,7 @@ int tbem_fmm_apply_adjoint_dev(tbem_fmm_t h, const double* d_g, double* d_y, voi
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
-        hipStream_t s = stream ? (hipStream_t)stream : h->work->stream;
-        adjoint(*h->plan, *h->work, d_g, d_y, s);
+        adjoint(*h->plan, *h->work, d_g, d_y, (hipStream_t)stream);
     });
 }
 
@@ -612,8 +625,10 @@ int tbem_fmm_obs_grad_dev(tbem_fmm_t h, const double* d_x, const double* d_g, do
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
-        hipStream_t s = stream ? (hipStream_t)stream : h->work->stream;
+        hipStream_t s = (hipStream_t)stream;
+        h->work->begin(s);
         obs_grad(*h->plan, *h->work, d_x, d_g, d_grad, s);
+        h->work->end(s);
     });
 }
 
@@ -627,9 +642,11 @@ int tbem_fmm_obs_grad(tbem_fmm_t h, const double* x, const double* g, double* gr
         TBEM_REQUIRE((x || nx == 0) && (g || ng == 0) && (grad || P.n_obs == 0), TBEM_E_INVALID,
                      "null argument");
         DevBuf<double> dx, dg, dgr(3 * P.n_obs);
+        W.begin(W.stream);
         dx.upload(x, nx, W.stream);
         dg.upload(g, ng, W.stream);
         obs_grad(P, W, dx.p, dg.p, dgr.p, W.stream);
+        W.end(W.stream);
         dgr.download(grad, 3 * P.n_obs, W.stream);
         TBEM_HIP(hipStreamSynchronize(W.stream));
     });
@@ -642,6 +659,7 @@ int tbem_fmm_set_obs_locs(tbem_fmm_t h, const double* obs_locs) {
         std::lock_guard<std::mutex> lk(h->mu);
         FmmPlan& P = *h->plan;
         if (P.n_obs == 0) return;
+        TBEM_HIP(hipDeviceSynchronize());  // no call of any handle sharing the plan may be in flight
         std::copy(obs_locs, obs_locs + 3 * P.n_obs, P.h_obs_locs.begin());
         P.op.upload(P.ot, P.h_obs_locs.data(), P.h_obs_normals.data(), nullptr, h->work->stream);
     });

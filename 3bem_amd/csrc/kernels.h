@@ -217,6 +217,13 @@ __device__ inline D3 rsqrt_t(const D3& x) {
 }
 template <> __device__ inline D3 cutoff_r2<D3>() { return D3(1e-12); }
 
+__host__ __device__ inline KC<double> make_kc64_dev(const KC<D3>& d) {
+    KC<double> k;
+    k.ls = d.ls.v; k.ld = d.ld.v; k.u1 = d.u1.v; k.u2 = d.u2.v; k.t1 = d.t1.v; k.t2 = d.t2.v;
+    k.h1 = d.h1.v; k.h2 = d.h2.v; k.h3 = d.h3.v; k.nu = d.nu.v;
+    return k;
+}
+
 inline KC<D3> make_kcd(const KConst& c) {
     KC<double> d = make_kc64(c);
     KC<D3> k;
@@ -309,6 +316,56 @@ __device__ __forceinline__ void kop(T dx, T dy, T dz, T r2, const T* no, const T
             acc[q] += C * val;
         }
         return;
+    }
+}
+
+
+// ------------------------------------------------- analytic observation gradients
+// G[m] += sum_k g_k d(K v)_k / d obs_m for kernels with a closed form
+// (d = src - obs, so d d / d obs = -I and d (1/r) / d obs_m = d_m / r^3).
+// The constant factor kop_scale<KID>() is applied by the caller.
+__host__ __device__ constexpr bool has_analytic_grad(int kid) {
+    return kid == K_LS || kid == K_LD || kid == K_EU || kid == K_ET;
+}
+
+template <int KID>
+__device__ __forceinline__ void grad_pair(double dx, double dy, double dz, double r2, const double* ns,
+                                          const double* v, const double* g, double* G, const KC<double>& k) {
+    double ri = rsqrt_t(r2);
+    ri = (r2 < 1e-12) ? 0.0 : ri;
+    const double ri2 = ri * ri, ri3 = ri2 * ri;
+    const double d[3] = {dx, dy, dz};
+    if (KID == K_LS) {
+        // u = v / r
+        const double s = g[0] * v[0] * ri3;
+        for (int m = 0; m < 3; m++) G[m] += s * d[m];
+    } else if (KID == K_LD) {
+        // u = (n.d) v / r^3
+        const double b = ns[0] * dx + ns[1] * dy + ns[2] * dz;
+        const double s = g[0] * v[0] * ri3;
+        for (int m = 0; m < 3; m++) G[m] += s * (3.0 * b * d[m] * ri2 - ns[m]);
+    } else if (KID == K_EU) {
+        // u_k = C2 v_k / r + d_k (d.v) / r^3
+        const double a = dx * v[0] + dy * v[1] + dz * v[2];
+        const double gd = g[0] * dx + g[1] * dy + g[2] * dz;
+        const double gv = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
+        const double c1 = k.u2 * gv + 3.0 * gd * a * ri2;
+        for (int m = 0; m < 3; m++) G[m] += ri3 * (c1 * d[m] - g[m] * a - gd * v[m]);
+    } else if (KID == K_ET) {
+        // u_k = -[C2 v_k b / r^3 + 3 d_k a b / r^5 - C2 d_k c / r^3 + C2 n_k a / r^3]
+        const double a = dx * v[0] + dy * v[1] + dz * v[2];
+        const double b = dx * ns[0] + dy * ns[1] + dz * ns[2];
+        const double c = ns[0] * v[0] + ns[1] * v[1] + ns[2] * v[2];
+        const double gv = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
+        const double gn = g[0] * ns[0] + g[1] * ns[1] + g[2] * ns[2];
+        const double gd = g[0] * dx + g[1] * dy + g[2] * dz;
+        const double ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+        const double cd = 3.0 * k.t2 * gv * b * ri5 + 15.0 * gd * a * b * ri7 - 3.0 * k.t2 * c * gd * ri5 +
+                          3.0 * k.t2 * gn * a * ri5;
+        const double cn = -k.t2 * gv * ri3 - 3.0 * gd * a * ri5;
+        const double cg = -3.0 * a * b * ri5 + k.t2 * c * ri3;
+        const double cv = -3.0 * gd * b * ri5 - k.t2 * gn * ri3;
+        for (int m = 0; m < 3; m++) G[m] -= cd * d[m] + cn * ns[m] + cg * g[m] + cv * v[m];
     }
 }
 

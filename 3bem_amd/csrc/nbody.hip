@@ -61,6 +61,7 @@ static HostList all_pairs(const HostTree& tgt, int n_src) {
         if (n_src > 0) {
             l.rbeg.push_back(0);
             l.rcnt.push_back(n_src);
+            l.rpre.push_back(0);
         }
         l.ptr.push_back((int)l.rbeg.size());
     }
@@ -78,7 +79,9 @@ struct NbodyPlan {
     DevBuf<int> obs_leaves, src_leaves;
     DevBuf<double> dens, gsort, xb, yb, outs;
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;  // end of the last call's work (buffers are per handle)
     ~NbodyPlan() {
+        if (done) (void)hipEventDestroy(done);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -106,7 +109,22 @@ static int nb_guard(F&& f) {
     }
 }
 
+static void nb_forward_impl(NbodyPlan& P, const double* x, double* y, hipStream_t s);
+static void nb_adjoint_impl(NbodyPlan& P, const double* g, double* y, hipStream_t s);
+
 static void nb_forward(NbodyPlan& P, const double* x, double* y, hipStream_t s) {
+    TBEM_HIP(hipStreamWaitEvent(s, P.done, 0));
+    nb_forward_impl(P, x, y, s);
+    TBEM_HIP(hipEventRecord(P.done, s));
+}
+
+static void nb_adjoint(NbodyPlan& P, const double* g, double* y, hipStream_t s) {
+    TBEM_HIP(hipStreamWaitEvent(s, P.done, 0));
+    nb_adjoint_impl(P, g, y, s);
+    TBEM_HIP(hipEventRecord(P.done, s));
+}
+
+static void nb_forward_impl(NbodyPlan& P, const double* x, double* y, hipStream_t s) {
     if (P.n_obs == 0) return;
     if (P.n_src == 0) {
         launch_fill(y, (size_t)P.R * P.n_obs, 0.0, s);
@@ -117,14 +135,14 @@ static void nb_forward(NbodyPlan& P, const double* x, double* y, hipStream_t s) 
     f.n_leaves = (int)P.obs_t.leaves.size();
     f.leaves = P.obs_leaves.p;
     f.tt = view(P.d_obs_t); f.tp = view(P.op); f.n_tgt = P.n_obs;
-    f.rp_ptr = P.fwd.ptr.p; f.rp_beg = P.fwd.rbeg.p; f.rp_cnt = P.fwd.rcnt.p;
+    f.rp_ptr = P.fwd.ptr.p; f.rp_beg = P.fwd.rbeg.p; f.rp_cnt = P.fwd.rcnt.p; f.rp_pre = P.fwd.rpre.p;
     f.ps = view(P.sp); f.pdens = P.dens.p; f.n_ps = P.n_src;
     f.out_sorted = y;  // identity permutation: write straight into y
     if (P.precision == TBEM_MIXED) launch_leaf_stage<float>(P.kid, false, f, P.kc, s);
     else launch_leaf_stage<double>(P.kid, false, f, P.kc, s);
 }
 
-static void nb_adjoint(NbodyPlan& P, const double* g, double* y, hipStream_t s) {
+static void nb_adjoint_impl(NbodyPlan& P, const double* g, double* y, hipStream_t s) {
     if (P.n_src == 0) return;
     if (P.n_obs == 0) {
         launch_fill(y, (size_t)P.R * P.n_src, 0.0, s);
@@ -134,7 +152,7 @@ static void nb_adjoint(NbodyPlan& P, const double* g, double* y, hipStream_t s) 
     f.n_leaves = (int)P.src_t.leaves.size();
     f.leaves = P.src_leaves.p;
     f.tt = view(P.d_src_t); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = P.n_src;
-    f.rp_ptr = P.adj.ptr.p; f.rp_beg = P.adj.rbeg.p; f.rp_cnt = P.adj.rcnt.p;
+    f.rp_ptr = P.adj.ptr.p; f.rp_beg = P.adj.rbeg.p; f.rp_cnt = P.adj.rcnt.p; f.rp_pre = P.adj.rpre.p;
     f.ps = view(P.op); f.pdens = g; f.n_ps = P.n_obs;
     f.out_sorted = y;
     if (P.precision == TBEM_MIXED) launch_leaf_stage<float>(P.kid, true, f, P.kc, s);
@@ -161,6 +179,7 @@ int tbem_nbody_create(const tbem_nbody_desc* d, tbem_nbody_t* out) {
         auto h = std::unique_ptr<tbem_nbody_s>(new tbem_nbody_s);
         NbodyPlan& P = h->p;
         TBEM_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+        TBEM_HIP(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
         hipStream_t s = P.stream;
         P.kid = d->kernel;
         P.R = kernel_dim(d->kernel);
@@ -179,6 +198,7 @@ int tbem_nbody_create(const tbem_nbody_desc* d, tbem_nbody_t* out) {
         P.obs_leaves.upload(P.obs_t.leaves, s);
         P.src_leaves.upload(P.src_t.leaves, s);
         P.dens.alloc((size_t)P.R * P.n_src);
+        TBEM_HIP(hipEventRecord(P.done, s));
         TBEM_HIP(hipStreamSynchronize(s));
         *out = h.release();
     });
@@ -187,7 +207,7 @@ int tbem_nbody_create(const tbem_nbody_desc* d, tbem_nbody_t* out) {
 int tbem_nbody_apply_dev(tbem_nbody_t h, const double* d_x, double* d_y, void* stream) {
     return nb_guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
-        nb_forward(h->p, d_x, d_y, stream ? (hipStream_t)stream : h->p.stream);
+        nb_forward(h->p, d_x, d_y, (hipStream_t)stream);
     });
 }
 

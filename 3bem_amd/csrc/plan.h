@@ -64,12 +64,14 @@ struct DevPoints {
 // source tree (cell[ptr[k] .. ptr[k+1]]). Point lists: entries are contiguous
 // ranges of sorted source points (rbeg/rcnt[ptr[k] .. ptr[k+1]]), the cells'
 // ranges merged where they touch.
+// rpre[e] is the exclusive prefix of rcnt inside the item's list, so a list is
+// a virtual array of points that tiles can cut anywhere.
 struct HostList {
-    std::vector<int> items, ptr, cell, rbeg, rcnt;
+    std::vector<int> items, ptr, cell, rbeg, rcnt, rpre;
 };
 struct DevList {
     int n_items = 0;
-    DevBuf<int> items, ptr, cell, rbeg, rcnt;
+    DevBuf<int> items, ptr, cell, rbeg, rcnt, rpre;
     void upload(const HostList& l, hipStream_t s);
 };
 

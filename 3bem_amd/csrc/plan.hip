@@ -338,6 +338,7 @@ void DevList::upload(const HostList& l, hipStream_t s) {
     cell.upload(l.cell, s);
     rbeg.upload(l.rbeg, s);
     rcnt.upload(l.rcnt, s);
+    rpre.upload(l.rpre, s);
 }
 
 void cells_to_ranges(HostList& l, const HostTree& t) {
@@ -364,6 +365,14 @@ void cells_to_ranges(HostList& l, const HostTree& t) {
     }
     l.ptr = std::move(ptr);
     l.cell.clear();
+    l.rpre.assign(l.rbeg.size(), 0);
+    for (size_t k = 0; k + 1 < l.ptr.size(); k++) {
+        int acc = 0;
+        for (int e = l.ptr[k]; e < l.ptr[k + 1]; e++) {
+            l.rpre[e] = acc;
+            acc += l.rcnt[e];
+        }
+    }
 }
 
 // ------------------------------------------------------------------ dual traversal

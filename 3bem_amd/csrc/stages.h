@@ -32,7 +32,7 @@ inline PtsView view(const DevPoints& p) { return PtsView{p.x.p, p.y.p, p.z.p, p.
 // the target surface plays the kernel's source, each source point its observer.
 struct SurfStage {
     int n_items;
-    const int *items, *ptr, *cell, *rbeg, *rcnt;
+    const int *items, *ptr, *cell, *rbeg, *rcnt, *rpre;
     TreeView tt;
     const double* t_surf;   // scaled target surface, 3*n_e
     TreeView st;
@@ -60,7 +60,7 @@ struct LeafStage {
     PtsView tp;
     const double* t_weight;
     int n_tgt;
-    const int *rp_ptr, *rp_beg, *rp_cnt;   // point ranges, indexed by leaf position
+    const int *rp_ptr, *rp_beg, *rp_cnt, *rp_pre;   // point ranges, indexed by leaf position
     PtsView ps;
     const double* pdens;
     int n_ps;

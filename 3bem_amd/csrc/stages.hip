@@ -12,9 +12,10 @@
 // array; the next tile is loaded into registers while the current one is
 // computed. Surface sources are generated from the cell center and the scaled
 // unit sphere.
-// Lanes split into S = 128/ntp source slices that share the tile (ntp = target
-// slots rounded up to a power of two) and are summed through LDS at the end; in
-// the leaf stage each lane carries two targets, so one LDS read feeds two pairs.
+// Lanes split into S = floor(128/ntp) source slices that share the tile (ntp =
+// target slots; the last 128 - S*ntp lanes sit out) and are summed through LDS
+// at the end; in the leaf stage each lane carries two targets, so one LDS read
+// feeds two pairs.
 //
 // T = double: the plain FP64 path, absolute coordinates, so each pair's
 //   displacement is the reference's src - obs.
@@ -40,11 +41,11 @@ struct SrcReg {
     double x, y, z, nx, ny, nz, d[R];
 };
 
-__device__ inline int next_pow2(int v) {
-    int p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
+// Wait for every outstanding memory access of this wave. Issued after the target
+// loads so the waitcnt pass knows they are complete; otherwise it inserts
+// conservative vmcnt(0) waits inside the pair loop, which also wait on the next
+// tile's prefetch and serialise load latency with compute.
+__device__ __forceinline__ void drain_loads() { __builtin_amdgcn_s_waitcnt(0); }
 
 // moved surface point r*s + c, rounded as TranslationSurface::move (cpp/fmm.h:19-31)
 __device__ inline void moved(double r, const double* s, double cx, double cy, double cz, double* p) {
@@ -125,12 +126,16 @@ __device__ inline void store_tile(Tile<T, R, NRM>& tl, int slot, const SrcReg<R,
     for (int q = 0; q < R; q++) tl.d[q][slot] = T(r.d[q]);
 }
 
-// Streams point ranges [rbeg[e], rbeg[e]+rcnt[e]) for e in [e0, e1) through the
-// tile, prefetching the next tile into registers; compute(cnt) runs per tile.
+// Streams the points of ranges e0..e1 (sorted points [rbeg[e], rbeg[e]+rcnt[e]))
+// through the tile, one range at a time (a tile never spans two ranges; measured
+// faster than packing tiles across range boundaries), prefetching the next tile
+// into registers while compute(cnt) runs on the current one.
 template <class T, int R, bool NRM, class F>
 __device__ inline void stream_ranges(Tile<T, R, NRM>& tl, int e0, int e1, const int* __restrict__ rbeg,
-                                     const int* __restrict__ rcnt, const PtsView& sp, const double* __restrict__ dens,
-                                     int n_src, const double* org, F&& compute) {
+                                     const int* __restrict__ rcnt, const int* __restrict__ rpre,
+                                     const PtsView& sp, const double* __restrict__ dens, int n_src,
+                                     const double* org, F&& compute) {
+    (void)rpre;
     const int tid = threadIdx.x;
     int e = e0, off = 0;
     if (e >= e1) return;
@@ -143,7 +148,10 @@ __device__ inline void stream_ranges(Tile<T, R, NRM>& tl, int e0, int e1, const 
         __syncthreads();
         const int cur = cnt;
         off += kBlock;
-        if (off >= rcnt[e]) { e++; off = 0; }
+        if (off >= rcnt[e]) {
+            e++;
+            off = 0;
+        }
         const bool more = e < e1;
         if (more) {
             cnt = min(kBlock, rcnt[e] - off);
@@ -224,14 +232,15 @@ __global__ __launch_bounds__(kBlock) void k_surf_stage(SurfStage a, KC<T> kc, do
 
     for (int t0 = 0; t0 < a.n_e; t0 += kBlock) {
         const int nt = min(kBlock, a.n_e - t0);
-        const int ntp = next_pow2(nt), S = kBlock / ntp;
+        const int ntp = nt, S = kBlock / ntp;
         const int tg = tid % ntp, slice = tid / ntp;
-        const bool active = tg < nt;
-        const int i = t0 + (active ? tg : 0);
+        const bool active = slice < S;
+        const int i = t0 + tg;
         double p[3];
         moved(rc, a.t_surf + 3 * i, cx, cy, cz, p);
         const T tx = T(p[0] - org[0]), ty = T(p[1] - org[1]), tz = T(p[2] - org[2]);
         const T tn[3] = {T(a.unit[3 * i]), T(a.unit[3 * i + 1]), T(a.unit[3 * i + 2])};
+        drain_loads();
         Acc<T, R> acc;
         acc.init();
         auto compute = [&](int cnt) {
@@ -242,11 +251,11 @@ __global__ __launch_bounds__(kBlock) void k_surf_stage(SurfStage a, KC<T> kc, do
         if (SRC_SURF)
             stream_surfaces(tl, e0, e1, a.cell, a.st, a.s_surf, a.unit, a.s_coeff, a.n_e, a.blk, org, compute);
         else
-            stream_ranges(tl, e0, e1, a.rbeg, a.rcnt, a.sp, a.dens, a.n_src_pts, org, compute);
+            stream_ranges(tl, e0, e1, a.rbeg, a.rcnt, a.rpre, a.sp, a.dens, a.n_src_pts, org, compute);
         __syncthreads();
         for (int q = 0; q < R; q++) red[q][tid] = active ? acc.value(q) : 0.0;
         __syncthreads();
-        if (slice == 0 && active) {
+        if (slice == 0) {
             for (int q = 0; q < R; q++) {
                 double s = 0;
                 for (int k = 0; k < S; k++) s += red[q][k * ntp + tg];
@@ -272,7 +281,8 @@ __global__ __launch_bounds__(kBlock) void k_surf_stage(SurfStage a, KC<T> kc, do
 // -------------------------------------------------------------------- point targets
 // Two targets per lane: slot tg holds targets t0+tg and t0+nslot+tg.
 template <int KID, class T, bool ADJ>
-__global__ __launch_bounds__(kBlock) void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
+__global__ __launch_bounds__(kBlock)
+void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
     __shared__ Tile<T, R, NRM> tl;
@@ -289,15 +299,16 @@ __global__ __launch_bounds__(kBlock) void k_leaf_stage(LeafStage a, KC<T> kc, do
     for (int t0 = 0; t0 < m; t0 += 2 * kBlock) {
         const int nt = min(2 * kBlock, m - t0);
         const int nslot = (nt + 1) >> 1;
-        const int ntp = next_pow2(nslot), S = kBlock / ntp;
+        const int ntp = nslot, S = kBlock / ntp;  // lanes >= ntp*S sit out
         const int tg = tid % ntp, slice = tid / ntp;
-        const bool va = tg < nslot, vb = nslot + tg < nt;
+        const bool va = tg < nslot && slice < S, vb = va && nslot + tg < nt;
         const int ia = b + t0 + (va ? tg : 0);
         const int ib = b + t0 + (vb ? nslot + tg : 0);
         const T ax = T(a.tp.x[ia] - org[0]), ay = T(a.tp.y[ia] - org[1]), az = T(a.tp.z[ia] - org[2]);
         const T bx = T(a.tp.x[ib] - org[0]), by = T(a.tp.y[ib] - org[1]), bz = T(a.tp.z[ib] - org[2]);
         const T an[3] = {NRM ? T(a.tp.nx[ia]) : T(0), NRM ? T(a.tp.ny[ia]) : T(0), NRM ? T(a.tp.nz[ia]) : T(0)};
         const T bn[3] = {NRM ? T(a.tp.nx[ib]) : T(0), NRM ? T(a.tp.ny[ib]) : T(0), NRM ? T(a.tp.nz[ib]) : T(0)};
+        drain_loads();
         Acc<T, R> acA, acB;
         acA.init();
         acB.init();
@@ -311,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_stage(LeafStage a, KC<T> kc, do
             acA.flush();
             acB.flush();
         };
-        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.ps, a.pdens, a.n_ps, org, compute);       // (a)
+        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org, compute);       // (a)
         stream_surfaces(tl, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org,
                         compute);                                                                     // (b)
         if (a.self_coeff)
@@ -323,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_stage(LeafStage a, KC<T> kc, do
             red[1][q][tid] = vb ? acB.value(q) : 0.0;
         }
         __syncthreads();
-        if (slice == 0) {
+        if (slice == 0 && tg < nslot) {
             for (int h = 0; h < 2; h++) {
                 const bool ok = h == 0 ? va : vb;
                 if (!ok) continue;
@@ -358,41 +369,53 @@ __global__ __launch_bounds__(kBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, do
     const int tid = threadIdx.x;
     for (int t0 = 0; t0 < m; t0 += kBlock) {
         const int nt = min(kBlock, m - t0);
-        const int ntp = next_pow2(nt), S = kBlock / ntp;
+        const int ntp = nt, S = kBlock / ntp;
         const int tg = tid % ntp, slice = tid / ntp;
-        const bool active = tg < nt;
-        const int i = b + t0 + (active ? tg : 0);
+        const bool active = slice < S;
+        const int i = b + t0 + tg;
         const D3 tx(a.tp.x[i], 1, 0, 0), ty(a.tp.y[i], 0, 1, 0), tz(a.tp.z[i], 0, 0, 1);
         const D3 tn[3] = {D3(NRM ? a.tp.nx[i] : 0.0), D3(NRM ? a.tp.ny[i] : 0.0), D3(NRM ? a.tp.nz[i] : 0.0)};
+        drain_loads();
         D3 acc[R];
+        double gt[R], G[3] = {0, 0, 0};
+        for (int q = 0; q < R; q++) gt[q] = a.grad_g[(size_t)q * a.n_tgt + i];
+        const KC<double> kc64 = make_kc64_dev(kc);
         auto compute = [&](int cnt) {
             if (!active) return;
             for (int j = slice; j < cnt; j += S) {
-                const D3 sn[3] = {D3(NRM ? tl.nx[j] : 0.0), D3(NRM ? tl.ny[j] : 0.0), D3(NRM ? tl.nz[j] : 0.0)};
-                D3 v[R];
-                for (int q = 0; q < R; q++) v[q] = D3(tl.d[q][j]);
-                const D3 dx = D3(tl.x[j]) - tx, dy = D3(tl.y[j]) - ty, dz = D3(tl.z[j]) - tz;
-                const D3 r2 = dx * dx + dy * dy + dz * dz;
-                kop<KID, false, D3>(dx, dy, dz, r2, tn, sn, v, acc, kc);
+                if (has_analytic_grad(KID)) {
+                    const double sn[3] = {NRM ? tl.nx[j] : 0.0, NRM ? tl.ny[j] : 0.0, NRM ? tl.nz[j] : 0.0};
+                    double v[R];
+                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                    const double dx = tl.x[j] - tx.v, dy = tl.y[j] - ty.v, dz = tl.z[j] - tz.v;
+                    grad_pair<KID>(dx, dy, dz, dx * dx + dy * dy + dz * dz, sn, v, gt, G, kc64);
+                } else {
+                    const D3 sn[3] = {D3(NRM ? tl.nx[j] : 0.0), D3(NRM ? tl.ny[j] : 0.0), D3(NRM ? tl.nz[j] : 0.0)};
+                    D3 v[R];
+                    for (int q = 0; q < R; q++) v[q] = D3(tl.d[q][j]);
+                    const D3 dx = D3(tl.x[j]) - tx, dy = D3(tl.y[j]) - ty, dz = D3(tl.z[j]) - tz;
+                    const D3 r2 = dx * dx + dy * dy + dz * dz;
+                    kop<KID, false, D3>(dx, dy, dz, r2, tn, sn, v, acc, kc);
+                }
             }
         };
-        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.ps, a.pdens, a.n_ps, org, compute);
+        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org, compute);
         stream_surfaces(tl, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org, compute);
         if (a.self_coeff)
             stream_surfaces(tl, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org, compute);
-        double gr[3] = {0, 0, 0};
-        if (active) {
+        double gr[3] = {G[0], G[1], G[2]};
+        if (active && !has_analytic_grad(KID)) {
             for (int q = 0; q < R; q++) {
-                const double gq = a.grad_g[(size_t)q * a.n_tgt + i];
-                gr[0] += gq * acc[q].d0;
-                gr[1] += gq * acc[q].d1;
-                gr[2] += gq * acc[q].d2;
+                gr[0] += gt[q] * acc[q].d0;
+                gr[1] += gt[q] * acc[q].d1;
+                gr[2] += gt[q] * acc[q].d2;
             }
         }
+        if (!active) gr[0] = gr[1] = gr[2] = 0;
         __syncthreads();
         for (int mm = 0; mm < 3; mm++) red[mm][tid] = gr[mm];
         __syncthreads();
-        if (slice == 0 && active) {
+        if (slice == 0) {
             for (int mm = 0; mm < 3; mm++) {
                 double s = 0;
                 for (int k = 0; k < S; k++) s += red[mm][k * ntp + tg];

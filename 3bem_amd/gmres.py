@@ -1,0 +1,103 @@
+"""Restarted GMRES and FP64-residual iterative refinement on device tensors.
+
+The reference solves with scipy's GMRES in Python (acceptance_tests/dislocation.py:27-39);
+this module keeps every vector on the GPU and calls the FMM through its device
+entry point.
+
+    x, info = gmres(apply, b, tol=1e-8, restart=50, maxiter=1000)
+    x, info = refine(apply_fp64, apply_inner, b, tol=1e-8, inner_tol=1e-4)
+
+gmres: GMRES(m) (Saad & Schultz 1986) with modified Gram-Schmidt Arnoldi and
+Givens rotations on the small Hessenberg matrix (host, FP64).
+refine: classic mixed-precision iterative refinement: r = b - A64 x with the
+FP64 operator, correction from GMRES on the cheaper inner operator (e.g. the
+FP32 FMM), x += e, until ||r|| / ||b|| <= tol.
+"""
+import math
+
+import numpy as np
+import torch
+
+
+def gmres(apply, b, x0=None, tol=1e-8, restart=50, maxiter=1000, callback=None):
+    """Solve A x = b; apply(v) -> A v on float64 device tensors.
+    Returns (x, info) with info = {"iterations", "residual", "converged", "history"}."""
+    b = b.contiguous()
+    bnorm = float(torch.linalg.norm(b))
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    if bnorm == 0.0:
+        return x, {"iterations": 0, "residual": 0.0, "converged": True, "history": [0.0]}
+    hist = []
+    it = 0
+    r = b - apply(x) if x0 is not None else b.clone()
+    beta = float(torch.linalg.norm(r))
+    hist.append(beta / bnorm)
+    while it < maxiter and beta / bnorm > tol:
+        m = restart
+        V = [r / beta]
+        H = np.zeros((m + 1, m))
+        cs, sn = np.zeros(m), np.zeros(m)
+        g = np.zeros(m + 1)
+        g[0] = beta
+        k_done = 0
+        for k in range(m):
+            w = apply(V[k]).contiguous()
+            for j in range(k + 1):
+                h = float(torch.dot(w, V[j]))
+                H[j, k] = h
+                w.sub_(V[j], alpha=h)
+            hn = float(torch.linalg.norm(w))
+            H[k + 1, k] = hn
+            for j in range(k):  # apply earlier rotations
+                t = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
+                H[j + 1, k] = -sn[j] * H[j, k] + cs[j] * H[j + 1, k]
+                H[j, k] = t
+            den = math.hypot(H[k, k], H[k + 1, k])
+            cs[k], sn[k] = (1.0, 0.0) if den == 0 else (H[k, k] / den, H[k + 1, k] / den)
+            H[k, k] = cs[k] * H[k, k] + sn[k] * H[k + 1, k]
+            H[k + 1, k] = 0.0
+            g[k + 1] = -sn[k] * g[k]
+            g[k] = cs[k] * g[k]
+            it += 1
+            k_done = k + 1
+            res = abs(g[k + 1]) / bnorm
+            hist.append(res)
+            if callback is not None:
+                callback(it, res)
+            if res <= tol or it >= maxiter or hn == 0.0:
+                break
+            V.append(w / hn)
+        y = np.linalg.solve(np.triu(H[:k_done, :k_done]), g[:k_done])
+        for j in range(k_done):
+            x.add_(V[j], alpha=float(y[j]))
+        r = b - apply(x)
+        beta = float(torch.linalg.norm(r))
+        hist[-1] = beta / bnorm  # true residual at the restart
+        if beta == 0.0:
+            break
+    res = beta / bnorm
+    return x, {"iterations": it, "residual": res, "converged": res <= tol, "history": hist}
+
+
+def refine(apply_fp64, apply_inner, b, tol=1e-8, inner_tol=1e-4, restart=50, inner_maxiter=500,
+           max_outer=30):
+    """Iterative refinement with an FP64 residual around an inexact inner solver."""
+    b = b.contiguous()
+    bnorm = float(torch.linalg.norm(b))
+    x = torch.zeros_like(b)
+    outer = []
+    inner_its = 0
+    for k in range(max_outer):
+        r = b - apply_fp64(x) if k else b.clone()
+        res = float(torch.linalg.norm(r)) / bnorm if bnorm else 0.0
+        outer.append(res)
+        if res <= tol:
+            break
+        e, info = gmres(apply_inner, r, tol=inner_tol, restart=restart, maxiter=inner_maxiter)
+        inner_its += info["iterations"]
+        x.add_(e)
+    else:
+        r = b - apply_fp64(x)
+        outer.append(float(torch.linalg.norm(r)) / bnorm)
+    return x, {"outer": len(outer) - 1, "inner_iterations": inner_its, "residual": outer[-1],
+               "converged": outer[-1] <= tol, "history": outer}

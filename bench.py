@@ -117,15 +117,25 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=100_000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-extra", action="store_true", help="FP64 forward only")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the FMM "
-              "sharding over several GPUs is described in DESIGN.md but not implemented", file=sys.stderr)
+              "sharding over several GPUs is not implemented", file=sys.stderr)
         raise SystemExit(2)
 
     import torch
     tbem = importlib.import_module("3bem_amd")
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+            dst.add_(src)  # elementwise kernel: reads 512 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        log("[bench] calibration: 3 x (256 MiB copy, 256 MiB add_)")
     w = workload(args.config, args.n)
     kid = w["kernel"]
     R = 1 if kid in (0, 1, 2, 7) else 3

@@ -15,9 +15,11 @@
  * code otherwise; tbem_last_error() then describes the failure (thread-local).
  * The reference only asserts (cpp/fmm.cpp:462-526, cpp/blas_wrapper.cpp:87).
  *
- * Threading: each handle owns one HIP stream; calls on one handle serialize on
- * it, separate handles are independent. The reference apply() is const and
- * re-entrant (cpp/fmm.cpp:550-558).
+ * Threading: each handle owns one HIP stream and one set of work buffers. Host-
+ * pointer calls run on that stream and return synchronised; _dev calls run on the
+ * stream given (NULL = the default stream). Calls on one handle are ordered, even
+ * across streams; separate handles (clones) are independent. The reference apply()
+ * is const and re-entrant (cpp/fmm.cpp:550-558).
  */
 #ifndef TBEM_FMM_H
 #define TBEM_FMM_H
@@ -125,12 +127,13 @@ int tbem_fmm_create(const tbem_fmm_desc* desc, tbem_fmm_t* out);
 int tbem_fmm_apply(tbem_fmm_t h, const double* x, double* y);
 
 /* Same on device pointers, enqueued on `stream` (a hipStream_t; NULL = the
- * handle's own stream). No host synchronisation. */
+ * default stream). No host synchronisation. Successive calls on one handle are
+ * ordered even across streams (they share the handle's work buffers). */
 int tbem_fmm_apply_dev(tbem_fmm_t h, const double* d_x, double* d_y, void* stream);
 
 /* Exact transpose of the FMM operator (every stage transposed, roles of the
  * two trees swapped): g has n_rows() doubles, y receives n_cols(). No reference
- * counterpart: 3bem has no adjoint (SURVEY fact 7); it is the backward of apply. */
+ * counterpart: 3bem has no adjoint; it is the backward of apply. */
 int tbem_fmm_apply_adjoint(tbem_fmm_t h, const double* g, double* y);
 int tbem_fmm_apply_adjoint_dev(tbem_fmm_t h, const double* d_g, double* d_y, void* stream);
 

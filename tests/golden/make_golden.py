@@ -4,7 +4,7 @@ Run here, where /root/reference exists to build oracle/_ref:
     make -C oracle ref && OMP_NUM_THREADS=1 python tests/golden/make_golden.py
 
 Single-threaded: the reference's OpenMP atomics make multi-threaded output vary
-at ~1e-12 (SURVEY fact 5). Inputs are seeded, so each file holds both the inputs
+at ~1e-12. Inputs are seeded, so each file holds both the inputs
 and the reference outputs. test_data_elastic3 is the reference's own sympy
 golden data for the 3D elastic kernels (unit_tests/test_data_elastic3), copied
 as a data fixture.

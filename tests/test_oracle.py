@@ -47,7 +47,7 @@ def test_kernel_cutoff_and_identity(orc):
 
 
 def test_elastic_u_symmetric_and_adjoint_identity(orc):
-    """U(x<-y) = U(x<-y)^T; A_kj(y<-x; n_y) = T_jk(x<-y; n_y)."""
+    """U(x<-y) = U(x<-y)^T; A_kj(y<-x; n_y) = T_jk(x<-y; n_y) (cpp/elastic_kernels.h:229-311)."""
     rng = np.random.default_rng(2)
     for _ in range(40):
         x, y = rng.standard_normal(3), rng.standard_normal(3)
