@@ -139,7 +139,7 @@ __global__ __launch_bounds__(1024) void k_jacobi_pinv(int n, double threshold, c
             }
             __syncthreads();
         }
-        if (!rotated) break;
+        if (!uni(rotated)) break;
         __syncthreads();
     }
     if (tid == 0) status[m] = (sweep < 60) ? 0 : 1;

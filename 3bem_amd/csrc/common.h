@@ -87,4 +87,13 @@ struct DevBuf {
 
 inline int div_up(size_t a, size_t b) { return (int)((a + b - 1) / b); }
 
+// Marks a value that every lane of the wave holds alike as wave-uniform
+// (readfirstlane puts it in an SGPR). Any loop or branch that contains a
+// __syncthreads() and depends on a value loaded from memory (global or LDS)
+// must go through this: the compiler cannot prove such a loaded value uniform,
+// and it lowers a barrier inside what it takes for divergent control flow with
+// per-lane exec masking, which has dropped the later LDS stores of the lanes
+// that were active in a loop's last partial iteration.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 }  // namespace tbem

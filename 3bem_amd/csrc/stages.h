@@ -6,7 +6,10 @@
 
 namespace tbem {
 
-constexpr int kBlock = 128;  // threads per workgroup of every stage kernel (two wave64s)
+// threads per workgroup (multiples of the 64-lane wavefront)
+constexpr int kSurfBlock = 128;  // surface-target stages (P2M, M2M, M2L, P2L, L2L)
+constexpr int kLeafBlock = 512;  // point-target stage (P2P + M2P + L2P)
+constexpr int kGradBlock = 256;  // observation-coordinate gradient of the point-target stage
 
 struct TreeView {
     const double *cx, *cy, *cz, *level_r;

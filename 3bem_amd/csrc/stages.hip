@@ -3,19 +3,24 @@
 // Every stage is "accumulate the field of a stream of sources at a set of targets":
 //   * surface targets (P2M, M2M, M2L, P2L, L2L and their transposes): the n_e
 //     points of a cell's check/equivalent surface, then an FP64 C2E matvec;
+//     one kSurfBlock-thread workgroup per target cell;
 //   * point targets (P2P + M2P + L2P fused, or P2P^T + P2L^T + P2M^T): the
-//     points of one leaf.
-// One 128-thread workgroup (two wave64s) per target cell/leaf. Sources are staged
-// through LDS in 128-entry SoA tiles. Point sources come as contiguous ranges of
-// the sorted SoA arrays (neighbouring leaves of a list are merged at plan time,
-// Morton order makes most of them adjacent), so a tile is one coalesced load per
-// array; the next tile is loaded into registers while the current one is
-// computed. Surface sources are generated from the cell center and the scaled
-// unit sphere.
-// Lanes split into S = floor(128/ntp) source slices that share the tile (ntp =
-// target slots; the last 128 - S*ntp lanes sit out) and are summed through LDS
-// at the end; in the leaf stage each lane carries two targets, so one LDS read
-// feeds two pairs.
+//     points of one leaf; one kLeafBlock-thread workgroup per leaf.
+// Sources are staged through LDS in NB-entry SoA tiles (NB = threads per
+// workgroup). Point sources come as contiguous ranges of the sorted SoA arrays
+// (neighbouring leaves of a list are merged at plan time; Morton order makes most
+// of them adjacent). Surface sources are generated from the cell center and the
+// scaled unit sphere. The next tile is loaded into registers while the current
+// one is computed.
+// Lanes split into S = floor(NB/ntp) source slices that share the tile (ntp =
+// target slots; the last NB - S*ntp lanes sit out), summed through LDS at the
+// end; in the leaf stage each lane carries two targets, so one LDS read feeds two
+// pairs. Workgroup width is the main performance lever of the leaf stage: wider
+// workgroups give each leaf's targets more source slices, hence more waves in
+// flight per leaf and fewer tile barriers per pair. Measured on the 1M-point
+// elastic workload (order 6, leaf 64), FP64 leaf stage: 6.4 ms at 128 threads,
+// 4.7 ms at 256, 4.4 ms at 512 with tiles packed across range boundaries. The
+// surface stages have few sources per target and prefer 128.
 //
 // T = double: the plain FP64 path, absolute coordinates, so each pair's
 //   displacement is the reference's src - obs.
@@ -28,11 +33,23 @@
 
 namespace tbem {
 
-template <class T, int R, bool NRM>
+template <class T, int R, bool NRM, int NB>
 struct Tile {
-    T x[kBlock], y[kBlock], z[kBlock];
-    T nx[NRM ? kBlock : 1], ny[NRM ? kBlock : 1], nz[NRM ? kBlock : 1];
-    T d[R][kBlock];
+    T x[NB], y[NB], z[NB];
+    T nx[NRM ? NB : 1], ny[NRM ? NB : 1], nz[NRM ? NB : 1];
+    T d[R][NB];
+};
+
+// LDS of a stage kernel. The source tile is dead once the streams end, so the
+// cross-slice reduction (RED values per thread) reuses its bytes; the range cache
+// serves the packed point stream.
+template <class T, int R, bool NRM, int NB, int RED>
+struct StageSmem {
+    union {
+        Tile<T, R, NRM, NB> tile;
+        double red[RED * NB];
+    };
+    int rpre[NB + 1], rbeg[NB];
 };
 
 // one staged source held in registers (FP64, absolute) until it is written to LDS
@@ -117,8 +134,8 @@ __device__ inline void load_surf(SrcReg<R, NRM>& r, int c, int j, const TreeView
     for (int q = 0; q < R; q++) r.d[q] = coeff[(size_t)c * blk + q * n_e + j];
 }
 
-template <class T, int R, bool NRM>
-__device__ inline void store_tile(Tile<T, R, NRM>& tl, int slot, const SrcReg<R, NRM>& r, const double* org) {
+template <class T, int R, bool NRM, int NB>
+__device__ inline void store_tile(Tile<T, R, NRM, NB>& tl, int slot, const SrcReg<R, NRM>& r, const double* org) {
     tl.x[slot] = T(r.x - org[0]);
     tl.y[slot] = T(r.y - org[1]);
     tl.z[slot] = T(r.z - org[2]);
@@ -127,47 +144,118 @@ __device__ inline void store_tile(Tile<T, R, NRM>& tl, int slot, const SrcReg<R,
 }
 
 // Streams the points of ranges e0..e1 (sorted points [rbeg[e], rbeg[e]+rcnt[e]))
-// through the tile, one range at a time (a tile never spans two ranges; measured
-// faster than packing tiles across range boundaries), prefetching the next tile
-// into registers while compute(cnt) runs on the current one.
-template <class T, int R, bool NRM, class F>
-__device__ inline void stream_ranges(Tile<T, R, NRM>& tl, int e0, int e1, const int* __restrict__ rbeg,
-                                     const int* __restrict__ rcnt, const int* __restrict__ rpre,
+// through the tile one range at a time: a tile never spans two ranges, so each
+// lane's source index is rbeg[e] + off + tid. Used by the surface stages, whose
+// point lists (P2M, P2L) are short.
+template <class T, int R, bool NRM, int NB, int RED, class F>
+__device__ inline void stream_ranges(StageSmem<T, R, NRM, NB, RED>& sm, int e0, int e1,
+                                     const int* __restrict__ rbeg, const int* __restrict__ rcnt,
                                      const PtsView& sp, const double* __restrict__ dens, int n_src,
                                      const double* org, F&& compute) {
-    (void)rpre;
     const int tid = threadIdx.x;
     int e = e0, off = 0;
     if (e >= e1) return;
     SrcReg<R, NRM> r;
-    int cnt = min(kBlock, rcnt[e]);
-    if (tid < cnt) load_point(r, rbeg[e] + tid, sp, dens, n_src);
+    int len = uni(rcnt[e]), beg = uni(rbeg[e]);
+    int cnt = min(NB, len);
+    if (tid < cnt) load_point(r, beg + tid, sp, dens, n_src);
     while (true) {
         __syncthreads();
-        if (tid < cnt) store_tile(tl, tid, r, org);
+        if (tid < cnt) store_tile(sm.tile, tid, r, org);
         __syncthreads();
         const int cur = cnt;
-        off += kBlock;
-        if (off >= rcnt[e]) {
+        off += NB;
+        if (off >= len) {
             e++;
             off = 0;
+            if (e < e1) {
+                len = uni(rcnt[e]);
+                beg = uni(rbeg[e]);
+            }
         }
         const bool more = e < e1;
         if (more) {
-            cnt = min(kBlock, rcnt[e] - off);
-            if (tid < cnt) load_point(r, rbeg[e] + off + tid, sp, dens, n_src);
+            cnt = min(NB, len - off);
+            if (tid < cnt) load_point(r, beg + off + tid, sp, dens, n_src);
         }
         compute(cur);
-        if (!more) break;
+        if (!uni(more)) break;
+    }
+}
+
+// Streams the points of ranges e0..e1 through the tile in full NB-point tiles cut
+// from the list's virtual concatenation (range e starts at offset rpre[e] of it),
+// so a tile may span range boundaries and only the last one is partial. Up to NB
+// ranges are cached in LDS; each lane finds its range by a short forward scan
+// from the range holding the tile's first point. Ranges must be nonempty: then a
+// cache that starts at the range holding point `base` covers at least
+// [base, base + NB), which is what one tile needs.
+template <class T, int R, bool NRM, int NB, int RED, class F>
+__device__ inline void stream_ranges_packed(StageSmem<T, R, NRM, NB, RED>& sm, int e0, int e1,
+                                            const int* __restrict__ rbeg, const int* __restrict__ rcnt,
+                                            const int* __restrict__ rpre, const PtsView& sp,
+                                            const double* __restrict__ dens, int n_src, const double* org,
+                                            F&& compute) {
+    const int tid = threadIdx.x;
+    if (e0 >= e1) return;
+    const int total = uni(rpre[e1 - 1] + rcnt[e1 - 1]);
+    SrcReg<R, NRM> r;
+    int lbase = e0;  // list index of cache entry 0
+    auto cache = [&](int from) {
+        __syncthreads();
+        lbase = from;
+        const int e = from + tid;
+        if (e < e1) {
+            sm.rpre[tid] = rpre[e];
+            sm.rbeg[tid] = rbeg[e];
+        }
+        if (tid == 0) sm.rpre[NB] = (from + NB < e1) ? rpre[from + NB] : total;
+        __syncthreads();
+    };
+    cache(e0);
+    int ecur = 0;  // cache entry of the range holding the current tile's first point (uniform)
+    auto fetch = [&](int base) {
+        const int v = base + tid;
+        if (v < total) {
+            const int nl = min(NB, e1 - lbase);
+            int f = ecur;
+            while (f + 1 < nl && sm.rpre[f + 1] <= v) f++;
+            load_point(r, sm.rbeg[f] + (v - sm.rpre[f]), sp, dens, n_src);
+        }
+    };
+    int base = 0;
+    fetch(0);
+    while (true) {
+        const int cnt = min(NB, total - base);
+        __syncthreads();
+        if (tid < cnt) store_tile(sm.tile, tid, r, org);
+        __syncthreads();
+        base += NB;
+        const bool more = base < total;
+        if (more) {
+            const int nl = min(NB, e1 - lbase);
+            while (uni(ecur + 1 < nl && sm.rpre[ecur + 1] <= base)) ecur++;
+            const int cache_end = uni(sm.rpre[NB]);
+            if (uni(lbase + NB < e1 && cache_end < base + NB)) {  // the next tile reaches past the cache
+                // the previous tile was covered, so cache_end >= base; equality means
+                // the range holding base is the first one past the cache
+                cache(cache_end <= base ? lbase + NB : lbase + ecur);
+                ecur = 0;
+            }
+            fetch(base);
+        }
+        compute(cnt);
+        if (!uni(more)) break;
     }
 }
 
 // Streams the equivalent/check surfaces of cells cell[e0..e1) (n_e points each).
-template <class T, int R, bool NRM, class F>
-__device__ inline void stream_surfaces(Tile<T, R, NRM>& tl, int e0, int e1, const int* __restrict__ cell,
-                                       const TreeView& st, const double* __restrict__ surf,
-                                       const double* __restrict__ unit, const double* __restrict__ coeff, int n_e,
-                                       int blk, const double* org, F&& compute) {
+template <class T, int R, bool NRM, int NB, int RED, class F>
+__device__ inline void stream_surfaces(StageSmem<T, R, NRM, NB, RED>& sm, int e0, int e1,
+                                       const int* __restrict__ cell, const TreeView& st,
+                                       const double* __restrict__ surf, const double* __restrict__ unit,
+                                       const double* __restrict__ coeff, int n_e, int blk, const double* org,
+                                       F&& compute) {
     const int tid = threadIdx.x;
     const int total = (e1 - e0) * n_e;
     if (total <= 0) return;
@@ -182,21 +270,21 @@ __device__ inline void stream_surfaces(Tile<T, R, NRM>& tl, int e0, int e1, cons
     };
     fetch(0);
     while (true) {
-        const int cnt = min(kBlock, total - base);
+        const int cnt = min(NB, total - base);
         __syncthreads();
-        if (tid < cnt) store_tile(tl, tid, r, org);
+        if (tid < cnt) store_tile(sm.tile, tid, r, org);
         __syncthreads();
-        base += kBlock;
+        base += NB;
         const bool more = base < total;
         if (more) fetch(base);
         compute(cnt);
-        if (!more) break;
+        if (!uni(more)) break;
     }
 }
 
 // ---------------------------------------------------------------------- pair loop
-template <int KID, bool ADJ, class T, int R, bool NRM>
-__device__ __forceinline__ void pair(const Tile<T, R, NRM>& tl, int j, T tx, T ty, T tz, const T* tn, T* acc,
+template <int KID, bool ADJ, class T, int R, bool NRM, int NB>
+__device__ __forceinline__ void pair(const Tile<T, R, NRM, NB>& tl, int j, T tx, T ty, T tz, const T* tn, T* acc,
                                      const KC<T>& kc) {
     const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
     T v[R];
@@ -214,25 +302,25 @@ __device__ __forceinline__ void pair(const Tile<T, R, NRM>& tl, int j, T tx, T t
 
 // ------------------------------------------------------------------ surface targets
 template <int KID, class T, bool SRC_SURF, bool ADJ>
-__global__ __launch_bounds__(kBlock) void k_surf_stage(SurfStage a, KC<T> kc, double scale, int add) {
+__global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<T> kc, double scale, int add) {
+    constexpr int NB = kSurfBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
-    __shared__ Tile<T, R, NRM> tl;
-    __shared__ double red[R][kBlock];
+    __shared__ StageSmem<T, R, NRM, NB, R> sm;
     extern __shared__ double field[];  // R * n_e
     const int item = blockIdx.x;
-    const int cell = a.items[item];
-    const int lvl = a.tt.level[cell];
+    const int cell = uni(a.items[item]);
+    const int lvl = uni(a.tt.level[cell]);
     const double rc = a.tt.level_r[lvl];
     const double cx = a.tt.cx[cell], cy = a.tt.cy[cell], cz = a.tt.cz[cell];
     double org[3];
     origin<T>(cx, cy, cz, org);
-    const int e0 = a.ptr[item], e1 = a.ptr[item + 1];
+    const int e0 = uni(a.ptr[item]), e1 = uni(a.ptr[item + 1]);
     const int tid = threadIdx.x;
 
-    for (int t0 = 0; t0 < a.n_e; t0 += kBlock) {
-        const int nt = min(kBlock, a.n_e - t0);
-        const int ntp = nt, S = kBlock / ntp;
+    for (int t0 = 0; t0 < a.n_e; t0 += NB) {
+        const int nt = min(NB, a.n_e - t0);
+        const int ntp = nt, S = NB / ntp;
         const int tg = tid % ntp, slice = tid / ntp;
         const bool active = slice < S;
         const int i = t0 + tg;
@@ -245,20 +333,20 @@ __global__ __launch_bounds__(kBlock) void k_surf_stage(SurfStage a, KC<T> kc, do
         acc.init();
         auto compute = [&](int cnt) {
             if (active)
-                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(tl, j, tx, ty, tz, tn, acc.tile(), kc);
+                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(sm.tile, j, tx, ty, tz, tn, acc.tile(), kc);
             acc.flush();
         };
         if (SRC_SURF)
-            stream_surfaces(tl, e0, e1, a.cell, a.st, a.s_surf, a.unit, a.s_coeff, a.n_e, a.blk, org, compute);
+            stream_surfaces(sm, e0, e1, a.cell, a.st, a.s_surf, a.unit, a.s_coeff, a.n_e, a.blk, org, compute);
         else
-            stream_ranges(tl, e0, e1, a.rbeg, a.rcnt, a.rpre, a.sp, a.dens, a.n_src_pts, org, compute);
+            stream_ranges(sm, e0, e1, a.rbeg, a.rcnt, a.sp, a.dens, a.n_src_pts, org, compute);
         __syncthreads();
-        for (int q = 0; q < R; q++) red[q][tid] = active ? acc.value(q) : 0.0;
+        for (int q = 0; q < R; q++) sm.red[q * NB + tid] = active ? acc.value(q) : 0.0;
         __syncthreads();
         if (slice == 0) {
             for (int q = 0; q < R; q++) {
                 double s = 0;
-                for (int k = 0; k < S; k++) s += red[q][k * ntp + tg];
+                for (int k = 0; k < S; k++) s += sm.red[q * NB + k * ntp + tg];
                 field[q * a.n_e + i] = s * scale;
             }
         }
@@ -267,43 +355,42 @@ __global__ __launch_bounds__(kBlock) void k_surf_stage(SurfStage a, KC<T> kc, do
     double* o = a.out + (size_t)cell * a.blk;
     if (a.c2e) {
         const double* M = a.c2e + (size_t)lvl * a.blk * a.blk;
-        for (int row = tid; row < a.blk; row += kBlock) {
+        for (int row = tid; row < a.blk; row += NB) {
             double s = 0;
             const double* m = M + (size_t)row * a.blk;
             for (int col = 0; col < a.blk; col++) s += m[col] * field[col];
             o[row] = add ? o[row] + s : s;
         }
     } else {
-        for (int row = tid; row < a.blk; row += kBlock) o[row] = add ? o[row] + field[row] : field[row];
+        for (int row = tid; row < a.blk; row += NB) o[row] = add ? o[row] + field[row] : field[row];
     }
 }
 
 // -------------------------------------------------------------------- point targets
 // Two targets per lane: slot tg holds targets t0+tg and t0+nslot+tg.
 template <int KID, class T, bool ADJ>
-__global__ __launch_bounds__(kBlock)
-void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
+__global__ __launch_bounds__(kLeafBlock) void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
+    constexpr int NB = kLeafBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
-    __shared__ Tile<T, R, NRM> tl;
-    __shared__ double red[2][R][kBlock];
+    __shared__ StageSmem<T, R, NRM, NB, 2 * R> sm;
     const int pos = blockIdx.x;
-    const int leaf = a.leaves[pos];
-    const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
+    const int leaf = uni(a.leaves[pos]);
+    const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
     double org[3];
     origin<T>(a.tt.cx[leaf], a.tt.cy[leaf], a.tt.cz[leaf], org);
-    const int pe0 = a.rp_ptr ? a.rp_ptr[pos] : 0, pe1 = a.rp_ptr ? a.rp_ptr[pos + 1] : 0;
-    const int se0 = a.sl_ptr ? a.sl_ptr[pos] : 0, se1 = a.sl_ptr ? a.sl_ptr[pos + 1] : 0;
+    const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
+    const int se0 = a.sl_ptr ? uni(a.sl_ptr[pos]) : 0, se1 = a.sl_ptr ? uni(a.sl_ptr[pos + 1]) : 0;
     const int tid = threadIdx.x;
 
-    for (int t0 = 0; t0 < m; t0 += 2 * kBlock) {
-        const int nt = min(2 * kBlock, m - t0);
+    for (int t0 = 0; t0 < m; t0 += 2 * NB) {
+        const int nt = min(2 * NB, m - t0);
         const int nslot = (nt + 1) >> 1;
-        const int ntp = nslot, S = kBlock / ntp;  // lanes >= ntp*S sit out
+        const int ntp = nslot, S = NB / ntp;  // lanes >= ntp*S sit out
         const int tg = tid % ntp, slice = tid / ntp;
-        const bool va = tg < nslot && slice < S, vb = va && nslot + tg < nt;
-        const int ia = b + t0 + (va ? tg : 0);
-        const int ib = b + t0 + (vb ? nslot + tg : 0);
+        const bool va = slice < S, vb = va && nslot + tg < nt;
+        const int ia = b + t0 + tg;
+        const int ib = b + t0 + (vb ? nslot + tg : tg);
         const T ax = T(a.tp.x[ia] - org[0]), ay = T(a.tp.y[ia] - org[1]), az = T(a.tp.z[ia] - org[2]);
         const T bx = T(a.tp.x[ib] - org[0]), by = T(a.tp.y[ib] - org[1]), bz = T(a.tp.z[ib] - org[2]);
         const T an[3] = {NRM ? T(a.tp.nx[ia]) : T(0), NRM ? T(a.tp.ny[ia]) : T(0), NRM ? T(a.tp.nz[ia]) : T(0)};
@@ -315,34 +402,34 @@ void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
         auto compute = [&](int cnt) {
             if (va) {
                 for (int j = slice; j < cnt; j += S) {
-                    pair<KID, ADJ>(tl, j, ax, ay, az, an, acA.tile(), kc);
-                    pair<KID, ADJ>(tl, j, bx, by, bz, bn, acB.tile(), kc);
+                    pair<KID, ADJ>(sm.tile, j, ax, ay, az, an, acA.tile(), kc);
+                    pair<KID, ADJ>(sm.tile, j, bx, by, bz, bn, acB.tile(), kc);
                 }
             }
             acA.flush();
             acB.flush();
         };
-        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org, compute);       // (a)
-        stream_surfaces(tl, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org,
-                        compute);                                                                     // (b)
+        stream_ranges_packed(sm, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org,
+                             compute);                                                             // (a)
+        stream_surfaces(sm, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org,
+                        compute);                                                                  // (b)
         if (a.self_coeff)
-            stream_surfaces(tl, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org,
-                            compute);                                                                 // (c)
+            stream_surfaces(sm, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org,
+                            compute);                                                              // (c)
         __syncthreads();
         for (int q = 0; q < R; q++) {
-            red[0][q][tid] = va ? acA.value(q) : 0.0;
-            red[1][q][tid] = vb ? acB.value(q) : 0.0;
+            sm.red[q * NB + tid] = va ? acA.value(q) : 0.0;
+            sm.red[(R + q) * NB + tid] = vb ? acB.value(q) : 0.0;
         }
         __syncthreads();
-        if (slice == 0 && tg < nslot) {
+        if (slice == 0) {
             for (int h = 0; h < 2; h++) {
-                const bool ok = h == 0 ? va : vb;
-                if (!ok) continue;
-                const int i = h == 0 ? ia : ib;
+                if (h == 1 && nslot + tg >= nt) continue;
+                const int i = b + t0 + h * nslot + tg;
                 const double w = a.t_weight ? a.t_weight[i] : 1.0;
                 for (int q = 0; q < R; q++) {
                     double s = 0;
-                    for (int k = 0; k < S; k++) s += red[h][q][k * ntp + tg];
+                    for (int k = 0; k < S; k++) s += sm.red[(h * R + q) * NB + k * ntp + tg];
                     a.out_sorted[(size_t)q * a.n_tgt + i] = w * (s * scale);
                 }
             }
@@ -352,36 +439,38 @@ void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
 }
 
 // ------------------------------------------------- observation-coordinate gradient
-// Forward leaf stage (P2P + M2P + L2P) evaluated on dual numbers seeded on the
-// target coordinates; writes grad[3*i + m] = sum_q g_q(i) d u_q(i) / d x_i[m].
+// Forward leaf stage (P2P + M2P + L2P) differentiated with respect to the target
+// coordinates; writes grad[3*i + m] = sum_q g_q(i) d u_q(i) / d x_i[m]. Closed
+// forms for the kernels that have them (grad_pair), dual numbers otherwise.
 template <int KID>
-__global__ __launch_bounds__(kBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, double scale) {
+__global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, double scale) {
+    constexpr int NB = kGradBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
-    __shared__ Tile<double, R, NRM> tl;
-    __shared__ double red[3][kBlock];
+    __shared__ StageSmem<double, R, NRM, NB, 3> sm;
     const int pos = blockIdx.x;
-    const int leaf = a.leaves[pos];
-    const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
+    const int leaf = uni(a.leaves[pos]);
+    const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
     const double org[3] = {0, 0, 0};
-    const int pe0 = a.rp_ptr ? a.rp_ptr[pos] : 0, pe1 = a.rp_ptr ? a.rp_ptr[pos + 1] : 0;
-    const int se0 = a.sl_ptr ? a.sl_ptr[pos] : 0, se1 = a.sl_ptr ? a.sl_ptr[pos + 1] : 0;
+    const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
+    const int se0 = a.sl_ptr ? uni(a.sl_ptr[pos]) : 0, se1 = a.sl_ptr ? uni(a.sl_ptr[pos + 1]) : 0;
     const int tid = threadIdx.x;
-    for (int t0 = 0; t0 < m; t0 += kBlock) {
-        const int nt = min(kBlock, m - t0);
-        const int ntp = nt, S = kBlock / ntp;
+    const KC<double> kc64 = make_kc64_dev(kc);
+    for (int t0 = 0; t0 < m; t0 += NB) {
+        const int nt = min(NB, m - t0);
+        const int ntp = nt, S = NB / ntp;
         const int tg = tid % ntp, slice = tid / ntp;
         const bool active = slice < S;
         const int i = b + t0 + tg;
         const D3 tx(a.tp.x[i], 1, 0, 0), ty(a.tp.y[i], 0, 1, 0), tz(a.tp.z[i], 0, 0, 1);
         const D3 tn[3] = {D3(NRM ? a.tp.nx[i] : 0.0), D3(NRM ? a.tp.ny[i] : 0.0), D3(NRM ? a.tp.nz[i] : 0.0)};
-        drain_loads();
-        D3 acc[R];
         double gt[R], G[3] = {0, 0, 0};
         for (int q = 0; q < R; q++) gt[q] = a.grad_g[(size_t)q * a.n_tgt + i];
-        const KC<double> kc64 = make_kc64_dev(kc);
+        drain_loads();
+        D3 acc[R];
         auto compute = [&](int cnt) {
             if (!active) return;
+            const Tile<double, R, NRM, NB>& tl = sm.tile;
             for (int j = slice; j < cnt; j += S) {
                 if (has_analytic_grad(KID)) {
                     const double sn[3] = {NRM ? tl.nx[j] : 0.0, NRM ? tl.ny[j] : 0.0, NRM ? tl.nz[j] : 0.0};
@@ -399,10 +488,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, do
                 }
             }
         };
-        stream_ranges(tl, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org, compute);
-        stream_surfaces(tl, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org, compute);
+        stream_ranges_packed(sm, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org, compute);
+        stream_surfaces(sm, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org, compute);
         if (a.self_coeff)
-            stream_surfaces(tl, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org, compute);
+            stream_surfaces(sm, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org, compute);
         double gr[3] = {G[0], G[1], G[2]};
         if (active && !has_analytic_grad(KID)) {
             for (int q = 0; q < R; q++) {
@@ -413,12 +502,12 @@ __global__ __launch_bounds__(kBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, do
         }
         if (!active) gr[0] = gr[1] = gr[2] = 0;
         __syncthreads();
-        for (int mm = 0; mm < 3; mm++) red[mm][tid] = gr[mm];
+        for (int mm = 0; mm < 3; mm++) sm.red[mm * NB + tid] = gr[mm];
         __syncthreads();
         if (slice == 0) {
             for (int mm = 0; mm < 3; mm++) {
                 double s = 0;
-                for (int k = 0; k < S; k++) s += red[mm][k * ntp + tg];
+                for (int k = 0; k < S; k++) s += sm.red[mm * NB + k * ntp + tg];
                 a.grad_out[3 * (size_t)i + mm] = s * scale;
             }
         }
@@ -465,7 +554,8 @@ __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
 void launch_c2e_transpose_apply(const int* cells, int n_cells, const int* level, const double* c2e,
                                 int blk, const double* in, double* out, hipStream_t s) {
     if (n_cells == 0) return;
-    hipLaunchKernelGGL(k_c2e_transpose_apply, dim3(n_cells), dim3(kBlock), 0, s, cells, level, c2e, blk, in, out);
+    hipLaunchKernelGGL(k_c2e_transpose_apply, dim3(n_cells), dim3(kSurfBlock), 0, s, cells, level, c2e, blk, in,
+                       out);
     TBEM_LAUNCH_CHECK();
 }
 
@@ -503,7 +593,7 @@ static void surf_launch(bool src_surf, bool add, bool adj, const SurfStage& a, c
     const KC<T> kc = kc_of<T>(c);
     const double scale = (double)kop_scale<KID, double>(make_kc64(c));
     const size_t shm = (size_t)kernel_dim(KID) * a.n_e * sizeof(double);
-    dim3 g(a.n_items), b(kBlock);
+    dim3 g(a.n_items), b(kSurfBlock);
     if (src_surf) {
         if (adj) hipLaunchKernelGGL((k_surf_stage<KID, T, true, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
         else hipLaunchKernelGGL((k_surf_stage<KID, T, true, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
@@ -519,8 +609,9 @@ static void leaf_launch(bool adj, const LeafStage& a, const KConst& c, hipStream
     if (a.n_leaves == 0) return;
     const KC<T> kc = kc_of<T>(c);
     const double scale = (double)kop_scale<KID, double>(make_kc64(c));
-    if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true>), dim3(a.n_leaves), dim3(kBlock), 0, s, a, kc, scale);
-    else hipLaunchKernelGGL((k_leaf_stage<KID, T, false>), dim3(a.n_leaves), dim3(kBlock), 0, s, a, kc, scale);
+    dim3 g(a.n_leaves), b(kLeafBlock);
+    if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true>), g, b, 0, s, a, kc, scale);
+    else hipLaunchKernelGGL((k_leaf_stage<KID, T, false>), g, b, 0, s, a, kc, scale);
     TBEM_LAUNCH_CHECK();
 }
 
@@ -528,7 +619,7 @@ template <int KID, class T>
 static void grad_launch(const LeafStage& a, const KConst& c, hipStream_t s) {
     if (a.n_leaves == 0) return;
     const double scale = (double)kop_scale<KID, double>(make_kc64(c));
-    hipLaunchKernelGGL((k_leaf_grad<KID>), dim3(a.n_leaves), dim3(kBlock), 0, s, a, make_kcd(c), scale);
+    hipLaunchKernelGGL((k_leaf_grad<KID>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c), scale);
     TBEM_LAUNCH_CHECK();
 }
 

@@ -11,12 +11,12 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def make_problem(tbem, wl, K):
+def make_problem(tbem, wl, K, order=20):
     torch = pytest.importorskip("torch")
     g = wl.cfg4_fault_and_stations(n_tri=400, n_obs=1500)
     data = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["src_weights"])
-    op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, 20, 32, 0.05, False))
-    ref_u = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, 20, 32, 0.05, False)).apply(g["x_true"])
+    op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, order, 32, 0.05, False))
+    ref_u = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, order, 32, 0.05, False)).apply(g["x_true"])
     return dict(g=g, K=K, op=op, u_obs=torch.tensor(ref_u, device="cuda"), torch=torch)
 
 
@@ -27,7 +27,7 @@ def problem(tbem, wl):
 
 @pytest.fixture(scope="module")
 def problem_u(tbem, wl):
-    return make_problem(tbem, wl, tbem.ElasticDisplacement(1.0, 0.25))
+    return make_problem(tbem, wl, tbem.ElasticDisplacement(1.0, 0.25), order=75)
 
 
 def loss_and_grads(tbem, p, obs):
@@ -74,34 +74,35 @@ def test_autograd_matches_frozen_plan_finite_differences(tbem, problem):
 
 
 def test_autograd_matches_fp64_direct_sum_gradient(tbem, orc, problem_u):
-    """Gradients of the direct-sum loss (oracle) agree to the FMM's accuracy.
+    """Gradients of the direct-sum loss (oracle) agree with the FMM's to its accuracy.
 
+    Order 75 (forward error ~6e-6 against the direct sum on this geometry), so a
+    1e-3 bound separates FMM approximation error from a gradient that is wrong in
+    structure (a missing transpose, a dropped list, a sign), which is off by O(1).
     Displacement kernel: for the traction kernel with these station normals the
-    order-20 FMM itself (reference and this library alike, which agree to 1e-12)
-    is only accurate to ~20% against the direct sum, so that pairing has no
-    direct-sum gradient to compare with at a meaningful tolerance."""
+    reference's FMM is inaccurate at every order tried, leaving nothing to compare."""
     p = problem_u
     g = p["g"]
     obs = g["obs"]
+    n = obs.shape[0]
+    args = (g["obs_normals"], g["src"], g["src_normals"], g["src_weights"])
     print("CASE autograd vs direct", flush=True)
     L, gs, go = loss_and_grads(tbem, p, obs)
     u_obs = p["u_obs"].cpu().numpy()
-    u = orc.direct(3, obs, g["obs_normals"], g["src"], g["src_normals"], g["src_weights"], g["x"], parallel=True)
+    u = orc.direct(3, obs, *args, g["x"], parallel=True)
+    assert rel_err(p["op"].apply(g["x"]), u) < 1e-4
     r = u - u_obs
-    gs_ref = orc.direct_adjoint(3, obs, g["obs_normals"], g["src"], g["src_normals"], g["src_weights"], r)
-    assert rel_err(gs, gs_ref) < 1e-3
-    # observation gradient of the direct sum by central differences of single rows
-    rng = np.random.default_rng(5)
-    n = obs.shape[0]
-    for i in rng.choice(n, 8, replace=False):
-        for m in range(3):
-            h = 1e-6
-            rows = []
-            for s in (1, -1):
-                o = obs[i:i + 1].copy()
-                o[0, m] += s * h
-                rows.append(orc.direct(3, o, g["obs_normals"][i:i + 1], g["src"], g["src_normals"],
-                                       g["src_weights"], g["x"]))
-            du = (rows[0] - rows[1]) / (2 * h)
-            fd = sum(r[q * n + i] * du[q] for q in range(3))
-            assert abs(fd - go[i, m]) <= 2e-3 * np.abs(go).max() + 2e-3 * abs(fd), (i, m, fd, go[i, m])
+    gs_ref = orc.direct_adjoint(3, obs, *args, r)
+    assert rel_err(gs, gs_ref) < 1e-3, rel_err(gs, gs_ref)
+    # observation gradient of the direct sum: each row depends only on its own
+    # observation point, so one central difference per axis covers every point
+    go_ref = np.zeros_like(go)
+    h = 1e-6
+    for m in range(3):
+        op_, om = obs.copy(), obs.copy()
+        op_[:, m] += h
+        om[:, m] -= h
+        du = (orc.direct(3, op_, *args, g["x"], parallel=True) - orc.direct(3, om, *args, g["x"], parallel=True))
+        du /= 2 * h
+        go_ref[:, m] = sum(r[q * n:(q + 1) * n] * du[q * n:(q + 1) * n] for q in range(3))
+    assert rel_err(go, go_ref) < 1e-3, rel_err(go, go_ref)
