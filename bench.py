@@ -34,6 +34,11 @@ METRIC = "FMM matvec Mpts/s at stated rel err, FP32-MFMA vs FP64; bwd/fwd time r
 FLOPS_PER_PAIR = {0: 11, 1: 19, 2: 0, 3: 28, 4: 60, 5: 60, 6: 120, 7: 2, 8: 6}
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
+# HBM bytes per launch of the FP64 leaf stage on the default workload: rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE of the same command, scaled by what the counters
+# read for a plain device copy (profiles/k_leaf_stage_elastic_u_1M_order6.md).
+# Other workloads have no measured figure and report null.
+LEAF_TRAFFIC_BYTES = {("cfg2", 1_000_000): 640_564_408}
 
 
 def log(*a):
@@ -165,7 +170,8 @@ def main():
     achieved = alg_flops / (leaf_avg_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "pipe": "fp64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                "traffic": LEAF_TRAFFIC_BYTES.get((args.config, n)), "traffic_unit": "bytes per launch",
                 "kernel": "k_leaf_stage (P2P+M2P+L2P)", "kernel_avg_ms": round(leaf_avg_ms, 4),
                 "algorithmic_flop": alg_flops}
 
