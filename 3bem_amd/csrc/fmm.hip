@@ -295,9 +295,8 @@ static void set_list(SurfStage& a, const DevList& l) {
     a.rpre = l.rpre.p;
 }
 
-// gather, upward pass, M2L/P2L and L2L: everything the leaf stage consumes
-template <class T>
-static void farfield_T(const FmmPlan& P, Work& W, const double* x, hipStream_t s) {
+// gather, upward pass, M2L/P2L and L2L: everything the leaf stage consumes (FP64)
+static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s) {
     const int R = P.R;
     const int Ns = (int)P.n_src;
     TBEM_HIP(hipEventRecord(W.ev[0], s));
@@ -310,7 +309,7 @@ static void farfield_T(const FmmPlan& P, Work& W, const double* x, hipStream_t s
     a.tt = view(P.dst); a.t_surf = P.d_outer.p;
     a.st = view(P.dst); a.sp = view(P.sp); a.dens = W.dens.p; a.n_src_pts = Ns;
     a.c2e = P.uc2e.p; a.out = W.mult.p;
-    launch_surf_stage<T>(P.kid, false, false, false, a, P.kc, s);
+    launch_surf_stage(P.kid, false, false, false, a, P.kc, s);
     TBEM_HIP(hipEventRecord(W.ev[2], s));
 
     // M2M (cpp/fmm.cpp:204-251), deepest parents first
@@ -320,7 +319,7 @@ static void farfield_T(const FmmPlan& P, Work& W, const double* x, hipStream_t s
         m.tt = view(P.dst); m.t_surf = P.d_outer.p;
         m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
         m.c2e = P.uc2e.p; m.out = W.mult.p;
-        launch_surf_stage<T>(P.kid, true, false, false, m, P.kc, s);
+        launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
     }
     TBEM_HIP(hipEventRecord(W.ev[3], s));
 
@@ -331,13 +330,13 @@ static void farfield_T(const FmmPlan& P, Work& W, const double* x, hipStream_t s
     m.tt = view(P.dot); m.t_surf = P.d_inner.p;
     m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
     m.c2e = P.dc2e.p; m.out = W.locals.p;
-    launch_surf_stage<T>(P.kid, true, true, false, m, P.kc, s);
+    launch_surf_stage(P.kid, true, true, false, m, P.kc, s);
     SurfStage pl = surf_base(P);
     set_list(pl, P.p2l);
     pl.tt = view(P.dot); pl.t_surf = P.d_inner.p;
     pl.st = view(P.dst); pl.sp = view(P.sp); pl.dens = W.dens.p; pl.n_src_pts = Ns;
     pl.c2e = P.dc2e.p; pl.out = W.locals.p;
-    launch_surf_stage<T>(P.kid, false, true, false, pl, P.kc, s);
+    launch_surf_stage(P.kid, false, true, false, pl, P.kc, s);
     TBEM_HIP(hipEventRecord(W.ev[4], s));
 
     // L2L (cpp/fmm.cpp:384-413), root first
@@ -347,7 +346,7 @@ static void farfield_T(const FmmPlan& P, Work& W, const double* x, hipStream_t s
         l.tt = view(P.dot); l.t_surf = P.d_inner.p;
         l.st = view(P.dot); l.s_surf = P.d_outer.p; l.s_coeff = W.locals.p;
         l.c2e = P.dc2e.p; l.out = W.locals.p;
-        launch_surf_stage<T>(P.kid, true, true, false, l, P.kc, s);
+        launch_surf_stage(P.kid, true, true, false, l, P.kc, s);
     }
     TBEM_HIP(hipEventRecord(W.ev[5], s));
 }
@@ -379,7 +378,7 @@ static void forward_T(const FmmPlan& P, Work& W, const double* x, double* y, hip
         launch_fill(y, (size_t)R * No, 0.0, s);
         return;
     }
-    farfield_T<T>(P, W, x, s);
+    farfield(P, W, x, s);
     launch_leaf_stage<T>(P.kid, false, leaf_stage_args(P, W), P.kc, s);
     TBEM_HIP(hipEventRecord(W.ev[6], s));
     launch_scatter_out(W.out_sorted.p, P.op.perm.p, No, R, y, s);
@@ -405,7 +404,7 @@ static void obs_grad(const FmmPlan& P, Work& W, const double* x, const double* g
         launch_fill(grad, 3 * (size_t)No, 0.0, s);
         return;
     }
-    farfield_T<double>(P, W, x, s);
+    farfield(P, W, x, s);
     if (W.gsort.n < (size_t)R * No) W.gsort.alloc((size_t)R * No);
     if (W.grad_sorted.n < 3 * (size_t)No) W.grad_sorted.alloc(3 * (size_t)No);
     launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
@@ -445,7 +444,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     a.tt = view(P.dot); a.t_surf = P.d_outer.p;
     a.st = view(P.dot); a.sp = view(P.op); a.dens = W.gsort.p; a.n_src_pts = No;
     a.c2e = nullptr; a.out = W.lbar.p;
-    launch_surf_stage<T>(P.kid, false, false, true, a, P.kc, s);
+    launch_surf_stage(P.kid, false, false, true, a, P.kc, s);
 
     // L2L^T, deepest level first: cbar = DC2E^T lbar, then push to parents
     for (int L = P.ot.n_levels - 1; L >= 0; L--) {
@@ -457,7 +456,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         l.tt = view(P.dot); l.t_surf = P.d_outer.p;
         l.st = view(P.dot); l.s_surf = P.d_inner.p; l.s_coeff = W.cbar.p;
         l.c2e = nullptr; l.out = W.lbar.p;
-        launch_surf_stage<T>(P.kid, true, true, true, l, P.kc, s);
+        launch_surf_stage(P.kid, true, true, true, l, P.kc, s);
     }
 
     // M2L^T and M2P^T into mbar
@@ -466,13 +465,13 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     m.tt = view(P.dst); m.t_surf = P.d_inner.p;
     m.st = view(P.dot); m.s_surf = P.d_inner.p; m.s_coeff = W.cbar.p;
     m.c2e = nullptr; m.out = W.mbar.p;
-    launch_surf_stage<T>(P.kid, true, true, true, m, P.kc, s);
+    launch_surf_stage(P.kid, true, true, true, m, P.kc, s);
     SurfStage mp = surf_base(P);
     set_list(mp, P.a_m2p);
     mp.tt = view(P.dst); mp.t_surf = P.d_inner.p;
     mp.st = view(P.dot); mp.sp = view(P.op); mp.dens = W.gsort.p; mp.n_src_pts = No;
     mp.c2e = nullptr; mp.out = W.mbar.p;
-    launch_surf_stage<T>(P.kid, false, true, true, mp, P.kc, s);
+    launch_surf_stage(P.kid, false, true, true, mp, P.kc, s);
 
     // M2M^T, root first: ebar = UC2E^T mbar, then push to children
     for (int L = 0; L < P.st.n_levels; L++) {
@@ -484,7 +483,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         c.tt = view(P.dst); c.t_surf = P.d_inner.p;
         c.st = view(P.dst); c.s_surf = P.d_outer.p; c.s_coeff = W.ebar.p;
         c.c2e = nullptr; c.out = W.mbar.p;
-        launch_surf_stage<T>(P.kid, true, true, true, c, P.kc, s);
+        launch_surf_stage(P.kid, true, true, true, c, P.kc, s);
     }
 
     // P2P^T + P2L^T + P2M^T on every source leaf, times the source weights

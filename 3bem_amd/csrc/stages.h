@@ -48,9 +48,12 @@ struct SurfStage {
     int n_e, blk;
     const double* c2e;      // per level blk*blk row-major, or nullptr
     double* out;            // blk per cell of tt
+    double* chk_out;        // optional: the field itself (before C2E), blk per cell of tt
 };
 
-// Point-target stage. For each leaf of tree tt, for each of its points i:
+// Point-target stage (FP64, or T = float for the point sources (a) with every
+// displacement formed in FP64; the surface sources (b), (c) are always FP64).
+// For each leaf of tree tt, for each of its points i:
 //   out_sorted[q*n_tgt + i] = wscale_i * sum over
 //     (a) the sorted source points in its ranges                (P2P,   or P2P^T)
 //     (b) the surfaces (sl_surf) of the cells in its surface list (M2P,   or P2L^T)
@@ -80,7 +83,8 @@ struct LeafStage {
     double* grad_out;       // obs-gradient mode: 3 per target (sorted, AoS)
 };
 
-template <class T>
+// FP64 only: every check potential here feeds a check-to-equivalent pseudo-inverse
+// (or is the transpose of such a step), whose conditioning FP32 input cannot survive.
 void launch_surf_stage(int kid, bool src_surf, bool add, bool adj, const SurfStage& a,
                        const KConst& kc, hipStream_t s);
 template <class T>

@@ -22,31 +22,39 @@
 // 4.7 ms at 256, 4.4 ms at 512 with tiles packed across range boundaries. The
 // surface stages have few sources per target and prefer 128.
 //
-// T = double: the plain FP64 path, absolute coordinates, so each pair's
-//   displacement is the reference's src - obs.
-// T = float: coordinates are taken relative to the target cell's center (formed
-//   in FP64, then rounded), so FP32 resolves near-field displacements; per-tile
-//   FP32 partial sums are folded into a Neumaier-compensated FP32 accumulator.
+// Precision. Surface-target stages are FP64 only: each check potential they form
+// feeds a check-to-equivalent pseudo-inverse (condition number up to ~1e10 at the
+// 1e-10 SVD threshold), which turns FP32 rounding into O(1) errors at order >= 50.
+// The leaf stage runs its point sources (a) in T: FP64, or FP32 for the mixed
+// path. Tiles always hold FP64 coordinates and every displacement is formed in
+// FP64 and then rounded, so near-coincident pairs (1/r up to 1e6 near the r^2 <
+// 1e-12 cutoff) keep full relative accuracy; FP32 partial sums are folded per
+// tile into Neumaier-compensated accumulators. Surface sources (b), (c) evaluate
+// equivalent densities, whose entries grow like (1.73/0.52)^degree because the
+// equivalent sphere lies inside the cell: they stay FP64 (a few percent of the
+// pairs of a leaf).
 #include <type_traits>
 
 #include "stages.h"
 
 namespace tbem {
 
-template <class T, int R, bool NRM, int NB>
+// SoA source tile: FP64 coordinates, normals and densities in VT.
+template <class VT, int R, bool NRM, int NB>
 struct Tile {
-    T x[NB], y[NB], z[NB];
-    T nx[NRM ? NB : 1], ny[NRM ? NB : 1], nz[NRM ? NB : 1];
-    T d[R][NB];
+    double x[NB], y[NB], z[NB];
+    VT nx[NRM ? NB : 1], ny[NRM ? NB : 1], nz[NRM ? NB : 1];
+    VT d[R][NB];
 };
 
-// LDS of a stage kernel. The source tile is dead once the streams end, so the
-// cross-slice reduction (RED values per thread) reuses its bytes; the range cache
-// serves the packed point stream.
-template <class T, int R, bool NRM, int NB, int RED>
+// LDS of a stage kernel: the point tile (values in PT), the surface tile (FP64),
+// and the cross-slice reduction (RED doubles per thread) share bytes, since they
+// are live one at a time; the range cache serves the packed point stream.
+template <class PT, int R, bool NRM, int NB, int RED>
 struct StageSmem {
     union {
-        Tile<T, R, NRM, NB> tile;
+        Tile<PT, R, NRM, NB> pt;
+        Tile<double, R, NRM, NB> st;
         double red[RED * NB];
     };
     int rpre[NB + 1], rbeg[NB];
@@ -106,15 +114,6 @@ struct Acc<float, R> {
     __device__ double value(int q) const { return (double)s[q] + (double)c[q]; }
 };
 
-template <class T>
-__device__ inline void origin(double cx, double cy, double cz, double* o) {
-    if (std::is_same<T, float>::value) {
-        o[0] = cx; o[1] = cy; o[2] = cz;
-    } else {
-        o[0] = o[1] = o[2] = 0.0;
-    }
-}
-
 // ----------------------------------------------------------------- source loaders
 template <int R, bool NRM>
 __device__ inline void load_point(SrcReg<R, NRM>& r, int k, const PtsView& sp, const double* __restrict__ dens,
@@ -134,24 +133,24 @@ __device__ inline void load_surf(SrcReg<R, NRM>& r, int c, int j, const TreeView
     for (int q = 0; q < R; q++) r.d[q] = coeff[(size_t)c * blk + q * n_e + j];
 }
 
-template <class T, int R, bool NRM, int NB>
-__device__ inline void store_tile(Tile<T, R, NRM, NB>& tl, int slot, const SrcReg<R, NRM>& r, const double* org) {
-    tl.x[slot] = T(r.x - org[0]);
-    tl.y[slot] = T(r.y - org[1]);
-    tl.z[slot] = T(r.z - org[2]);
-    if (NRM) { tl.nx[slot] = T(r.nx); tl.ny[slot] = T(r.ny); tl.nz[slot] = T(r.nz); }
-    for (int q = 0; q < R; q++) tl.d[q][slot] = T(r.d[q]);
+template <class VT, int R, bool NRM, int NB>
+__device__ inline void store_tile(Tile<VT, R, NRM, NB>& tl, int slot, const SrcReg<R, NRM>& r) {
+    tl.x[slot] = r.x;
+    tl.y[slot] = r.y;
+    tl.z[slot] = r.z;
+    if (NRM) { tl.nx[slot] = VT(r.nx); tl.ny[slot] = VT(r.ny); tl.nz[slot] = VT(r.nz); }
+    for (int q = 0; q < R; q++) tl.d[q][slot] = VT(r.d[q]);
 }
 
 // Streams the points of ranges e0..e1 (sorted points [rbeg[e], rbeg[e]+rcnt[e]))
 // through the tile one range at a time: a tile never spans two ranges, so each
 // lane's source index is rbeg[e] + off + tid. Used by the surface stages, whose
 // point lists (P2M, P2L) are short.
-template <class T, int R, bool NRM, int NB, int RED, class F>
-__device__ inline void stream_ranges(StageSmem<T, R, NRM, NB, RED>& sm, int e0, int e1,
+template <class VT, int R, bool NRM, int NB, class F>
+__device__ inline void stream_ranges(Tile<VT, R, NRM, NB>& tl, int e0, int e1,
                                      const int* __restrict__ rbeg, const int* __restrict__ rcnt,
                                      const PtsView& sp, const double* __restrict__ dens, int n_src,
-                                     const double* org, F&& compute) {
+                                     F&& compute) {
     const int tid = threadIdx.x;
     int e = e0, off = 0;
     if (e >= e1) return;
@@ -161,7 +160,7 @@ __device__ inline void stream_ranges(StageSmem<T, R, NRM, NB, RED>& sm, int e0, 
     if (tid < cnt) load_point(r, beg + tid, sp, dens, n_src);
     while (true) {
         __syncthreads();
-        if (tid < cnt) store_tile(sm.tile, tid, r, org);
+        if (tid < cnt) store_tile(tl, tid, r);
         __syncthreads();
         const int cur = cnt;
         off += NB;
@@ -190,12 +189,11 @@ __device__ inline void stream_ranges(StageSmem<T, R, NRM, NB, RED>& sm, int e0, 
 // from the range holding the tile's first point. Ranges must be nonempty: then a
 // cache that starts at the range holding point `base` covers at least
 // [base, base + NB), which is what one tile needs.
-template <class T, int R, bool NRM, int NB, int RED, class F>
-__device__ inline void stream_ranges_packed(StageSmem<T, R, NRM, NB, RED>& sm, int e0, int e1,
+template <class VT, int R, bool NRM, int NB, class F>
+__device__ inline void stream_ranges_packed(Tile<VT, R, NRM, NB>& tl, int* s_pre, int* s_beg, int e0, int e1,
                                             const int* __restrict__ rbeg, const int* __restrict__ rcnt,
                                             const int* __restrict__ rpre, const PtsView& sp,
-                                            const double* __restrict__ dens, int n_src, const double* org,
-                                            F&& compute) {
+                                            const double* __restrict__ dens, int n_src, F&& compute) {
     const int tid = threadIdx.x;
     if (e0 >= e1) return;
     const int total = uni(rpre[e1 - 1] + rcnt[e1 - 1]);
@@ -206,10 +204,10 @@ __device__ inline void stream_ranges_packed(StageSmem<T, R, NRM, NB, RED>& sm, i
         lbase = from;
         const int e = from + tid;
         if (e < e1) {
-            sm.rpre[tid] = rpre[e];
-            sm.rbeg[tid] = rbeg[e];
+            s_pre[tid] = rpre[e];
+            s_beg[tid] = rbeg[e];
         }
-        if (tid == 0) sm.rpre[NB] = (from + NB < e1) ? rpre[from + NB] : total;
+        if (tid == 0) s_pre[NB] = (from + NB < e1) ? rpre[from + NB] : total;
         __syncthreads();
     };
     cache(e0);
@@ -219,8 +217,8 @@ __device__ inline void stream_ranges_packed(StageSmem<T, R, NRM, NB, RED>& sm, i
         if (v < total) {
             const int nl = min(NB, e1 - lbase);
             int f = ecur;
-            while (f + 1 < nl && sm.rpre[f + 1] <= v) f++;
-            load_point(r, sm.rbeg[f] + (v - sm.rpre[f]), sp, dens, n_src);
+            while (f + 1 < nl && s_pre[f + 1] <= v) f++;
+            load_point(r, s_beg[f] + (v - s_pre[f]), sp, dens, n_src);
         }
     };
     int base = 0;
@@ -228,14 +226,14 @@ __device__ inline void stream_ranges_packed(StageSmem<T, R, NRM, NB, RED>& sm, i
     while (true) {
         const int cnt = min(NB, total - base);
         __syncthreads();
-        if (tid < cnt) store_tile(sm.tile, tid, r, org);
+        if (tid < cnt) store_tile(tl, tid, r);
         __syncthreads();
         base += NB;
         const bool more = base < total;
         if (more) {
             const int nl = min(NB, e1 - lbase);
-            while (uni(ecur + 1 < nl && sm.rpre[ecur + 1] <= base)) ecur++;
-            const int cache_end = uni(sm.rpre[NB]);
+            while (uni(ecur + 1 < nl && s_pre[ecur + 1] <= base)) ecur++;
+            const int cache_end = uni(s_pre[NB]);
             if (uni(lbase + NB < e1 && cache_end < base + NB)) {  // the next tile reaches past the cache
                 // the previous tile was covered, so cache_end >= base; equality means
                 // the range holding base is the first one past the cache
@@ -250,12 +248,11 @@ __device__ inline void stream_ranges_packed(StageSmem<T, R, NRM, NB, RED>& sm, i
 }
 
 // Streams the equivalent/check surfaces of cells cell[e0..e1) (n_e points each).
-template <class T, int R, bool NRM, int NB, int RED, class F>
-__device__ inline void stream_surfaces(StageSmem<T, R, NRM, NB, RED>& sm, int e0, int e1,
+template <int R, bool NRM, int NB, class F>
+__device__ inline void stream_surfaces(Tile<double, R, NRM, NB>& tl, int e0, int e1,
                                        const int* __restrict__ cell, const TreeView& st,
                                        const double* __restrict__ surf, const double* __restrict__ unit,
-                                       const double* __restrict__ coeff, int n_e, int blk, const double* org,
-                                       F&& compute) {
+                                       const double* __restrict__ coeff, int n_e, int blk, F&& compute) {
     const int tid = threadIdx.x;
     const int total = (e1 - e0) * n_e;
     if (total <= 0) return;
@@ -272,7 +269,7 @@ __device__ inline void stream_surfaces(StageSmem<T, R, NRM, NB, RED>& sm, int e0
     while (true) {
         const int cnt = min(NB, total - base);
         __syncthreads();
-        if (tid < cnt) store_tile(sm.tile, tid, r, org);
+        if (tid < cnt) store_tile(tl, tid, r);
         __syncthreads();
         base += NB;
         const bool more = base < total;
@@ -283,38 +280,38 @@ __device__ inline void stream_surfaces(StageSmem<T, R, NRM, NB, RED>& sm, int e0
 }
 
 // ---------------------------------------------------------------------- pair loop
-template <int KID, bool ADJ, class T, int R, bool NRM, int NB>
-__device__ __forceinline__ void pair(const Tile<T, R, NRM, NB>& tl, int j, T tx, T ty, T tz, const T* tn, T* acc,
-                                     const KC<T>& kc) {
-    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
+// One source j of the tile on one target (FP64 coordinates tx, ty, tz): the
+// displacement is formed in FP64, then the kernel runs in T.
+template <int KID, bool ADJ, class T, class VT, int R, bool NRM, int NB>
+__device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, double tx, double ty, double tz,
+                                     const T* tn, T* acc, const KC<T>& kc) {
+    const T sn[3] = {NRM ? T(tl.nx[j]) : T(0), NRM ? T(tl.ny[j]) : T(0), NRM ? T(tl.nz[j]) : T(0)};
     T v[R];
-    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+    for (int q = 0; q < R; q++) v[q] = T(tl.d[q][j]);
     if (!ADJ) {
-        const T dx = tl.x[j] - tx, dy = tl.y[j] - ty, dz = tl.z[j] - tz;
+        const T dx = T(tl.x[j] - tx), dy = T(tl.y[j] - ty), dz = T(tl.z[j] - tz);
         const T r2 = dx * dx + dy * dy + dz * dz;
         kop<KID, false, T>(dx, dy, dz, r2, tn, sn, v, acc, kc);
     } else {
-        const T dx = tx - tl.x[j], dy = ty - tl.y[j], dz = tz - tl.z[j];
+        const T dx = T(tx - tl.x[j]), dy = T(ty - tl.y[j]), dz = T(tz - tl.z[j]);
         const T r2 = dx * dx + dy * dy + dz * dz;
         kop<KID, true, T>(dx, dy, dz, r2, sn, tn, v, acc, kc);
     }
 }
 
 // ------------------------------------------------------------------ surface targets
-template <int KID, class T, bool SRC_SURF, bool ADJ>
-__global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<T> kc, double scale, int add) {
+template <int KID, bool SRC_SURF, bool ADJ>
+__global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<double> kc, double scale, int add) {
     constexpr int NB = kSurfBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
-    __shared__ StageSmem<T, R, NRM, NB, R> sm;
+    __shared__ StageSmem<double, R, NRM, NB, R> sm;
     extern __shared__ double field[];  // R * n_e
     const int item = blockIdx.x;
     const int cell = uni(a.items[item]);
     const int lvl = uni(a.tt.level[cell]);
     const double rc = a.tt.level_r[lvl];
     const double cx = a.tt.cx[cell], cy = a.tt.cy[cell], cz = a.tt.cz[cell];
-    double org[3];
-    origin<T>(cx, cy, cz, org);
     const int e0 = uni(a.ptr[item]), e1 = uni(a.ptr[item + 1]);
     const int tid = threadIdx.x;
 
@@ -326,22 +323,20 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<T> kc
         const int i = t0 + tg;
         double p[3];
         moved(rc, a.t_surf + 3 * i, cx, cy, cz, p);
-        const T tx = T(p[0] - org[0]), ty = T(p[1] - org[1]), tz = T(p[2] - org[2]);
-        const T tn[3] = {T(a.unit[3 * i]), T(a.unit[3 * i + 1]), T(a.unit[3 * i + 2])};
+        const double tn[3] = {a.unit[3 * i], a.unit[3 * i + 1], a.unit[3 * i + 2]};
         drain_loads();
-        Acc<T, R> acc;
-        acc.init();
+        double acc[R];
+        for (int q = 0; q < R; q++) acc[q] = 0.0;
         auto compute = [&](int cnt) {
             if (active)
-                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(sm.tile, j, tx, ty, tz, tn, acc.tile(), kc);
-            acc.flush();
+                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(sm.st, j, p[0], p[1], p[2], tn, acc, kc);
         };
         if (SRC_SURF)
-            stream_surfaces(sm, e0, e1, a.cell, a.st, a.s_surf, a.unit, a.s_coeff, a.n_e, a.blk, org, compute);
+            stream_surfaces(sm.st, e0, e1, a.cell, a.st, a.s_surf, a.unit, a.s_coeff, a.n_e, a.blk, compute);
         else
-            stream_ranges(sm, e0, e1, a.rbeg, a.rcnt, a.sp, a.dens, a.n_src_pts, org, compute);
+            stream_ranges(sm.st, e0, e1, a.rbeg, a.rcnt, a.sp, a.dens, a.n_src_pts, compute);
         __syncthreads();
-        for (int q = 0; q < R; q++) sm.red[q * NB + tid] = active ? acc.value(q) : 0.0;
+        for (int q = 0; q < R; q++) sm.red[q * NB + tid] = active ? acc[q] : 0.0;
         __syncthreads();
         if (slice == 0) {
             for (int q = 0; q < R; q++) {
@@ -351,6 +346,10 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<T> kc
             }
         }
         __syncthreads();
+    }
+    if (a.chk_out) {
+        double* c = a.chk_out + (size_t)cell * a.blk;
+        for (int row = tid; row < a.blk; row += NB) c[row] = field[row];
     }
     double* o = a.out + (size_t)cell * a.blk;
     if (a.c2e) {
@@ -367,18 +366,19 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<T> kc
 }
 
 // -------------------------------------------------------------------- point targets
-// Two targets per lane: slot tg holds targets t0+tg and t0+nslot+tg.
+// Two targets per lane: slot tg holds targets t0+tg and t0+nslot+tg. Point
+// sources accumulate in T (acA, acB); surface sources in FP64 (sfA, sfB, which
+// are acA, acB themselves when T is double).
 template <int KID, class T, bool ADJ>
-__global__ __launch_bounds__(kLeafBlock) void k_leaf_stage(LeafStage a, KC<T> kc, double scale) {
+__global__ __launch_bounds__(kLeafBlock) void k_leaf_stage(LeafStage a, KC<T> kc, KC<double> kc64, double scale) {
     constexpr int NB = kLeafBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
     __shared__ StageSmem<T, R, NRM, NB, 2 * R> sm;
     const int pos = blockIdx.x;
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
-    double org[3];
-    origin<T>(a.tt.cx[leaf], a.tt.cy[leaf], a.tt.cz[leaf], org);
     const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
     const int se0 = a.sl_ptr ? uni(a.sl_ptr[pos]) : 0, se1 = a.sl_ptr ? uni(a.sl_ptr[pos + 1]) : 0;
     const int tid = threadIdx.x;
@@ -391,35 +391,56 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_stage(LeafStage a, KC<T> kc
         const bool va = slice < S, vb = va && nslot + tg < nt;
         const int ia = b + t0 + tg;
         const int ib = b + t0 + (vb ? nslot + tg : tg);
-        const T ax = T(a.tp.x[ia] - org[0]), ay = T(a.tp.y[ia] - org[1]), az = T(a.tp.z[ia] - org[2]);
-        const T bx = T(a.tp.x[ib] - org[0]), by = T(a.tp.y[ib] - org[1]), bz = T(a.tp.z[ib] - org[2]);
-        const T an[3] = {NRM ? T(a.tp.nx[ia]) : T(0), NRM ? T(a.tp.ny[ia]) : T(0), NRM ? T(a.tp.nz[ia]) : T(0)};
-        const T bn[3] = {NRM ? T(a.tp.nx[ib]) : T(0), NRM ? T(a.tp.ny[ib]) : T(0), NRM ? T(a.tp.nz[ib]) : T(0)};
+        const double ax = a.tp.x[ia], ay = a.tp.y[ia], az = a.tp.z[ia];
+        const double bx = a.tp.x[ib], by = a.tp.y[ib], bz = a.tp.z[ib];
+        const double an64[3] = {NRM ? a.tp.nx[ia] : 0.0, NRM ? a.tp.ny[ia] : 0.0, NRM ? a.tp.nz[ia] : 0.0};
+        const double bn64[3] = {NRM ? a.tp.nx[ib] : 0.0, NRM ? a.tp.ny[ib] : 0.0, NRM ? a.tp.nz[ib] : 0.0};
+        const T an[3] = {T(an64[0]), T(an64[1]), T(an64[2])};
+        const T bn[3] = {T(bn64[0]), T(bn64[1]), T(bn64[2])};
         drain_loads();
         Acc<T, R> acA, acB;
         acA.init();
         acB.init();
-        auto compute = [&](int cnt) {
+        double sfA[SPLIT ? R : 1], sfB[SPLIT ? R : 1];
+        for (int q = 0; q < (SPLIT ? R : 1); q++) sfA[q] = sfB[q] = 0.0;
+        auto points = [&](int cnt) {
             if (va) {
                 for (int j = slice; j < cnt; j += S) {
-                    pair<KID, ADJ>(sm.tile, j, ax, ay, az, an, acA.tile(), kc);
-                    pair<KID, ADJ>(sm.tile, j, bx, by, bz, bn, acB.tile(), kc);
+                    pair<KID, ADJ>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                    pair<KID, ADJ>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
                 }
             }
             acA.flush();
             acB.flush();
         };
-        stream_ranges_packed(sm, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org,
-                             compute);                                                             // (a)
-        stream_surfaces(sm, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org,
-                        compute);                                                                  // (b)
+        auto surfaces = [&](int cnt) {
+            if (va) {
+                double *sa, *sb;
+                if constexpr (SPLIT) {
+                    sa = sfA;
+                    sb = sfB;
+                } else {
+                    sa = acA.tile();
+                    sb = acB.tile();
+                }
+                for (int j = slice; j < cnt; j += S) {
+                    pair<KID, ADJ>(sm.st, j, ax, ay, az, an64, sa, kc64);
+                    pair<KID, ADJ>(sm.st, j, bx, by, bz, bn64, sb, kc64);
+                }
+            }
+        };
+        stream_ranges_packed(sm.pt, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
+                             a.n_ps, points);                                                     // (a)
+        stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk,
+                        surfaces);                                                                // (b)
         if (a.self_coeff)
-            stream_surfaces(sm, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org,
-                            compute);                                                              // (c)
+            stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk,
+                            surfaces);                                                            // (c)
         __syncthreads();
         for (int q = 0; q < R; q++) {
-            sm.red[q * NB + tid] = va ? acA.value(q) : 0.0;
-            sm.red[(R + q) * NB + tid] = vb ? acB.value(q) : 0.0;
+            const double ea = SPLIT ? sfA[q] : 0.0, eb = SPLIT ? sfB[q] : 0.0;
+            sm.red[q * NB + tid] = va ? acA.value(q) + ea : 0.0;
+            sm.red[(R + q) * NB + tid] = vb ? acB.value(q) + eb : 0.0;
         }
         __syncthreads();
         if (slice == 0) {
@@ -451,7 +472,6 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
     const int pos = blockIdx.x;
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
-    const double org[3] = {0, 0, 0};
     const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
     const int se0 = a.sl_ptr ? uni(a.sl_ptr[pos]) : 0, se1 = a.sl_ptr ? uni(a.sl_ptr[pos + 1]) : 0;
     const int tid = threadIdx.x;
@@ -470,7 +490,7 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
         D3 acc[R];
         auto compute = [&](int cnt) {
             if (!active) return;
-            const Tile<double, R, NRM, NB>& tl = sm.tile;
+            const Tile<double, R, NRM, NB>& tl = sm.st;
             for (int j = slice; j < cnt; j += S) {
                 if (has_analytic_grad(KID)) {
                     const double sn[3] = {NRM ? tl.nx[j] : 0.0, NRM ? tl.ny[j] : 0.0, NRM ? tl.nz[j] : 0.0};
@@ -488,10 +508,11 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
                 }
             }
         };
-        stream_ranges_packed(sm, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens, a.n_ps, org, compute);
-        stream_surfaces(sm, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, org, compute);
+        stream_ranges_packed(sm.st, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
+                             a.n_ps, compute);
+        stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, compute);
         if (a.self_coeff)
-            stream_surfaces(sm, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, org, compute);
+            stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, compute);
         double gr[3] = {G[0], G[1], G[2]};
         if (active && !has_analytic_grad(KID)) {
             for (int q = 0; q < R; q++) {
@@ -590,16 +611,16 @@ KC<float> kc_of<float>(const KConst& c) { return make_kc32(c); }
 template <int KID, class T>
 static void surf_launch(bool src_surf, bool add, bool adj, const SurfStage& a, const KConst& c, hipStream_t s) {
     if (a.n_items == 0) return;
-    const KC<T> kc = kc_of<T>(c);
-    const double scale = (double)kop_scale<KID, double>(make_kc64(c));
+    const KC<double> kc = make_kc64(c);
+    const double scale = (double)kop_scale<KID, double>(kc);
     const size_t shm = (size_t)kernel_dim(KID) * a.n_e * sizeof(double);
     dim3 g(a.n_items), b(kSurfBlock);
     if (src_surf) {
-        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, T, true, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
-        else hipLaunchKernelGGL((k_surf_stage<KID, T, true, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, true, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+        else hipLaunchKernelGGL((k_surf_stage<KID, true, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
     } else {
-        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, T, false, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
-        else hipLaunchKernelGGL((k_surf_stage<KID, T, false, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, false, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+        else hipLaunchKernelGGL((k_surf_stage<KID, false, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
     }
     TBEM_LAUNCH_CHECK();
 }
@@ -608,10 +629,11 @@ template <int KID, class T>
 static void leaf_launch(bool adj, const LeafStage& a, const KConst& c, hipStream_t s) {
     if (a.n_leaves == 0) return;
     const KC<T> kc = kc_of<T>(c);
-    const double scale = (double)kop_scale<KID, double>(make_kc64(c));
+    const KC<double> kc64 = make_kc64(c);
+    const double scale = (double)kop_scale<KID, double>(kc64);
     dim3 g(a.n_leaves), b(kLeafBlock);
-    if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true>), g, b, 0, s, a, kc, scale);
-    else hipLaunchKernelGGL((k_leaf_stage<KID, T, false>), g, b, 0, s, a, kc, scale);
+    if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true>), g, b, 0, s, a, kc, kc64, scale);
+    else hipLaunchKernelGGL((k_leaf_stage<KID, T, false>), g, b, 0, s, a, kc, kc64, scale);
     TBEM_LAUNCH_CHECK();
 }
 
@@ -637,9 +659,9 @@ static void grad_launch(const LeafStage& a, const KConst& c, hipStream_t s) {
     default: throw Error(TBEM_E_INVALID, "unknown kernel id");                        \
     }
 
-template <class T>
 void launch_surf_stage(int kid, bool src_surf, bool add, bool adj, const SurfStage& a, const KConst& kc,
                        hipStream_t s) {
+    using T = double;
     TBEM_KID_SWITCH(kid, surf_launch, src_surf, add, adj, a, kc, s);
 }
 
@@ -653,8 +675,6 @@ void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t
     TBEM_KID_SWITCH(kid, grad_launch, a, kc, s);
 }
 
-template void launch_surf_stage<double>(int, bool, bool, bool, const SurfStage&, const KConst&, hipStream_t);
-template void launch_surf_stage<float>(int, bool, bool, bool, const SurfStage&, const KConst&, hipStream_t);
 template void launch_leaf_stage<double>(int, bool, const LeafStage&, const KConst&, hipStream_t);
 template void launch_leaf_stage<float>(int, bool, const LeafStage&, const KConst&, hipStream_t);
 
