@@ -14,8 +14,10 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 
 #include "c2e.h"
+#include "m2l_mfma.h"
 #include "stages.h"
 
 namespace tbem {
@@ -53,6 +55,7 @@ struct FmmPlan {
     DevList a_m2l, a_m2p;        // items = source cells
     std::vector<DevList> a_m2m;  // by src level L >= 1: cells at L, entry = parent
     DevList a_leaf_p2p, a_leaf_p2l;  // indexed by source leaf position
+    M2lMfma m2l_mfma;            // mixed precision: M2L as grouped FP32 MFMA GEMMs
     double p2p_pairs = 0;
     double plan_seconds = 0;
 };
@@ -61,6 +64,8 @@ struct Work {
     hipStream_t stream = nullptr;
     DevBuf<double> dens, mult, locals, out_sorted, xbuf, ybuf;
     DevBuf<double> gsort, lbar, cbar, mbar, ebar, grad_sorted;
+    DevBuf<double> chk_up;  // up-check potential per source cell (input of the MFMA M2L)
+    DevBuf<float> m2l_y;    // per-pair MFMA M2L output
     hipEvent_t ev[8];
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
@@ -73,6 +78,12 @@ struct Work {
         mult.alloc((size_t)P.dst.n_cells * P.blk);
         locals.alloc((size_t)P.dot.n_cells * P.blk);
         out_sorted.alloc(P.R * P.n_obs);
+        if (P.m2l_mfma.enabled) {
+            chk_up.alloc((size_t)P.dst.n_cells * P.blk);
+            chk_up.zero(stream);  // cells without a P2M/M2M stage keep a zero potential
+            m2l_y.alloc((size_t)P.m2l_mfma.n_pairs * P.m2l_mfma.ystride);
+            TBEM_HIP(hipStreamSynchronize(stream));
+        }
     }
     // The work buffers belong to the handle: a call on stream s first waits for
     // the previous call (possibly on another stream) and marks its own end.
@@ -251,7 +262,10 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->p2m.upload(self_list(ST, ST.leaves), s);
     P->m2m.resize(ST.n_levels);
     for (int L = 0; L + 1 < ST.n_levels; L++) P->m2m[L].upload(children_list(ST, L), s);
-    P->m2l.upload(list_from_pairs(cells_with(tl.m2l, 0, OT.n_cells), OT.n_cells, tl.m2l, 0, nullptr), s);
+    const HostList m2l_h = list_from_pairs(cells_with(tl.m2l, 0, OT.n_cells), OT.n_cells, tl.m2l, 0, nullptr);
+    P->m2l.upload(m2l_h, s);
+    if (P->precision == TBEM_MIXED)
+        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, same, m2l_h, P->surf, P->uc2e.p, P->dc2e.p, s);
     P->p2l.upload(list_from_pairs(cells_with(tl.p2l, 0, OT.n_cells), OT.n_cells, tl.p2l, 0, &ST), s);
     P->l2l.resize(OT.n_levels);
     for (int L = 1; L < OT.n_levels; L++) P->l2l[L].upload(parent_list(OT, L), s);
@@ -295,8 +309,9 @@ static void set_list(SurfStage& a, const DevList& l) {
     a.rpre = l.rpre.p;
 }
 
-// gather, upward pass, M2L/P2L and L2L: everything the leaf stage consumes (FP64)
-static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s) {
+// gather, upward pass, M2L/P2L and L2L: everything the leaf stage consumes. FP64,
+// except the M2L when mfma is set (mixed precision, MFMA operators built).
+static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, bool mfma) {
     const int R = P.R;
     const int Ns = (int)P.n_src;
     TBEM_HIP(hipEventRecord(W.ev[0], s));
@@ -309,6 +324,7 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s) 
     a.tt = view(P.dst); a.t_surf = P.d_outer.p;
     a.st = view(P.dst); a.sp = view(P.sp); a.dens = W.dens.p; a.n_src_pts = Ns;
     a.c2e = P.uc2e.p; a.out = W.mult.p;
+    a.chk_out = mfma ? W.chk_up.p : nullptr;
     launch_surf_stage(P.kid, false, false, false, a, P.kc, s);
     TBEM_HIP(hipEventRecord(W.ev[2], s));
 
@@ -319,18 +335,26 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s) 
         m.tt = view(P.dst); m.t_surf = P.d_outer.p;
         m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
         m.c2e = P.uc2e.p; m.out = W.mult.p;
+        m.chk_out = mfma ? W.chk_up.p : nullptr;
         launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
     }
     TBEM_HIP(hipEventRecord(W.ev[3], s));
 
     // M2L (cpp/fmm.cpp:362-382) and P2L (cpp/fmm.cpp:296-330) into locals
     launch_fill(W.locals.p, W.locals.n, 0.0, s);
-    SurfStage m = surf_base(P);
-    set_list(m, P.m2l);
-    m.tt = view(P.dot); m.t_surf = P.d_inner.p;
-    m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
-    m.c2e = P.dc2e.p; m.out = W.locals.p;
-    launch_surf_stage(P.kid, true, true, false, m, P.kc, s);
+    if (mfma) {
+        // S_c * up-check potential = DC2E K_c UC2E check = the FP64 stage's DC2E K_c mult
+        launch_m2l_mfma(P.m2l_mfma, P.m2l.cell.p, W.chk_up.p, W.m2l_y.p, s);
+        launch_m2l_reduce(P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, W.m2l_y.p, P.m2l_mfma.ystride, P.blk,
+                          W.locals.p, s);
+    } else {
+        SurfStage m = surf_base(P);
+        set_list(m, P.m2l);
+        m.tt = view(P.dot); m.t_surf = P.d_inner.p;
+        m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
+        m.c2e = P.dc2e.p; m.out = W.locals.p;
+        launch_surf_stage(P.kid, true, true, false, m, P.kc, s);
+    }
     SurfStage pl = surf_base(P);
     set_list(pl, P.p2l);
     pl.tt = view(P.dot); pl.t_surf = P.d_inner.p;
@@ -378,7 +402,7 @@ static void forward_T(const FmmPlan& P, Work& W, const double* x, double* y, hip
         launch_fill(y, (size_t)R * No, 0.0, s);
         return;
     }
-    farfield(P, W, x, s);
+    farfield(P, W, x, s, std::is_same<T, float>::value && P.m2l_mfma.enabled);
     launch_leaf_stage<T>(P.kid, false, leaf_stage_args(P, W), P.kc, s);
     TBEM_HIP(hipEventRecord(W.ev[6], s));
     launch_scatter_out(W.out_sorted.p, P.op.perm.p, No, R, y, s);
@@ -404,7 +428,7 @@ static void obs_grad(const FmmPlan& P, Work& W, const double* x, const double* g
         launch_fill(grad, 3 * (size_t)No, 0.0, s);
         return;
     }
-    farfield(P, W, x, s);
+    farfield(P, W, x, s, false);
     if (W.gsort.n < (size_t)R * No) W.gsort.alloc((size_t)R * No);
     if (W.grad_sorted.n < 3 * (size_t)No) W.grad_sorted.alloc(3 * (size_t)No);
     launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
@@ -702,6 +726,7 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
         o->l2l = P.tl.l2l.size();
         o->l2p = P.tl.l2p.size();
         o->p2p_pairs = P.p2p_pairs;
+        o->m2l_classes = P.m2l_mfma.enabled ? P.m2l_mfma.n_classes : 0;
         o->plan_seconds = P.plan_seconds;
     });
 }

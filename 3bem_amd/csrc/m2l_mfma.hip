@@ -1,0 +1,324 @@
+// M2L of the mixed-precision path as grouped FP32 MFMA GEMMs (gfx950).
+//
+// Plan: pairs -> classes (level_t, level_s, integer offset); per class, in FP64 on
+// the GPU: K_c assembled from the kernel's reference expression (tensor_ref, as the
+// C2E systems), then S_c = DC2E * (K_c * UC2E) by a tiled FP64 GEMM, then rounded
+// to FP32 in MFMA A-operand order. Work units are (class, up to 64 pairs); units of
+// one class go to one XCD (workgroup i runs on XCD i % 8), so each S_c is fetched
+// from HBM once and then served by that XCD's L2.
+//
+// Apply: one workgroup (4 wave64s) per unit stages its 64 up-check potentials in
+// LDS as FP32 (k x 64; column n of row k stored at n ^ 16*(k%4), which keeps both
+// the staging writes and the B-operand reads free of bank conflicts), then each
+// wave runs row tiles of S_c:
+// v_mfma_f32_16x16x4f32 with A from global (coalesced 256 B per wave and k-step,
+// L2-resident) and B from LDS, FP32 accumulation over k = blk. Results go to a
+// per-pair buffer Y (no atomics); a second kernel sums each target's pairs in FP64
+// and adds them to its down-equivalent density.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <unordered_map>
+
+#include "m2l_mfma.h"
+
+namespace tbem {
+
+namespace {
+
+constexpr int kUnitCols = 64;   // pairs per work unit (= LDS row length of the staged columns)
+constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 KiB of LDS
+constexpr int kXcds = 8;
+constexpr int kMaxChunk = 16384;  // classes per plan-time batch (grid y/z limit)
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// K_c for a chunk of classes: rows (d1, i) = component d1 at target check point i,
+// columns (d2, j) = component d2 of source equivalent point j.
+__global__ void k_m2l_assemble(int kid, KConst kc, int n_e, int R, const double* __restrict__ tc,
+                               const double* __restrict__ sc, const double* __restrict__ tr,
+                               const double* __restrict__ sr, const double* __restrict__ inner,
+                               const double* __restrict__ unit, double* __restrict__ mats) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.y;
+    const int n = R * n_e;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_e * n_e; p += gridDim.x * blockDim.x) {
+        const int i = p / n_e, j = p % n_e;
+        double ck[3], eq[3];
+        for (int d = 0; d < 3; d++) {
+            ck[d] = tr[c] * inner[3 * i + d] + tc[3 * c + d];
+            eq[d] = sr[c] * inner[3 * j + d] + sc[3 * c + d];
+        }
+        double kv[9];
+        tensor_ref_dispatch(kid, ck, eq, unit + 3 * i, unit + 3 * j, kc, kv);
+        double* M = mats + (size_t)c * n * n;
+        for (int d1 = 0; d1 < R; d1++)
+            for (int d2 = 0; d2 < R; d2++) M[(size_t)(d1 * n_e + i) * n + d2 * n_e + j] = kv[d1 * R + d2];
+    }
+}
+
+// C[c] = A[ai[c]] * B[bi[c]], n x n row-major FP64; 64 x 64 tile per workgroup.
+__global__ __launch_bounds__(256) void k_gemm64(int n, const double* __restrict__ A0, const int* __restrict__ ai,
+                                                const double* __restrict__ B0, const int* __restrict__ bi,
+                                                double* __restrict__ C0) {
+    const int c = blockIdx.z;
+    const double* A = A0 + (size_t)ai[c] * n * n;
+    const double* B = B0 + (size_t)bi[c] * n * n;
+    double* C = C0 + (size_t)c * n * n;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    __shared__ double As[16][65], Bs[16][65];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        for (int e = threadIdx.x; e < 1024; e += 256) {
+            const int r = e / 16, k = e % 16;
+            As[k][r] = (m0 + r < n && k0 + k < n) ? A[(size_t)(m0 + r) * n + k0 + k] : 0.0;
+            const int kb = e / 64, cb = e % 64;
+            Bs[kb][cb] = (k0 + kb < n && n0 + cb < n) ? B[(size_t)(k0 + kb) * n + n0 + cb] : 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < 16; k++) {
+            double a[4], b[4];
+            for (int i = 0; i < 4; i++) a[i] = As[k][ty * 4 + i];
+            for (int j = 0; j < 4; j++) b[j] = Bs[k][tx * 4 + j];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            const int r = m0 + ty * 4 + i, cc = n0 + tx * 4 + j;
+            if (r < n && cc < n) C[(size_t)r * n + cc] = acc[i][j];
+        }
+}
+
+// FP64 row-major S_c -> FP32 A-operand order (zero padded to mt*16 x kt*4).
+__global__ void k_m2l_swizzle(int n, int mt, int kt, const double* __restrict__ S64, float* __restrict__ S32) {
+    const int c = blockIdx.y;
+    const size_t per = (size_t)mt * kt * 64;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x) {
+        const int rt = (int)(e / ((size_t)kt * 64)), ks = (int)((e / 64) % kt), l = (int)(e % 64);
+        const int row = rt * 16 + l % 16, col = ks * 4 + l / 16;
+        S32[c * per + e] = (row < n && col < n) ? (float)S64[(size_t)c * n * n + (size_t)row * n + col] : 0.f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int ystride, const float* __restrict__ S,
+                                                  const int* __restrict__ ucls, const int* __restrict__ uoff,
+                                                  const int* __restrict__ ucnt, const int* __restrict__ colpair,
+                                                  const int* __restrict__ pair_src,
+                                                  const double* __restrict__ chk, float* __restrict__ Y) {
+    extern __shared__ float Bs[];  // [kt*4][64], swizzled
+    const int u = blockIdx.x;
+    const int cls = uni(ucls[u]), off = uni(uoff[u]), cnt = uni(ucnt[u]);
+    if (cnt == 0) return;  // padding unit: uniform exit before any barrier
+    const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
+    const int kpad = kt * 4;
+    // stage: lane = column (pair), waves stride over k
+    const double* col = lane < cnt ? chk + (size_t)pair_src[colpair[off + lane]] * n : nullptr;
+    for (int kk = wave; kk < kpad; kk += 4)
+        Bs[kk * 64 + (lane ^ ((kk & 3) << 4))] = (col && kk < n) ? (float)col[kk] : 0.f;
+    __syncthreads();
+    // B operand of column tile ct at k-step ks: row ks*4 + lq, column ct*16 + lr,
+    // stored at ((ct ^ lq) * 16 + lr)
+    const int lr = lane % 16, lq = lane / 16;
+    const float* b = Bs + lq * 64 + lr;
+    const int s0 = (0 ^ lq) * 16, s1 = (1 ^ lq) * 16, s2 = (2 ^ lq) * 16, s3 = (3 ^ lq) * 16;
+    for (int rt = wave; rt < mt; rt += 4) {
+        f32x4 acc[4] = {};
+        const float* A = S + ((size_t)cls * mt + rt) * kt * 64 + lane;
+#pragma unroll 4
+        for (int ks = 0; ks < kt; ks++) {
+            const float a = A[(size_t)ks * 64];
+            const float* bk = b + ks * 4 * 64;
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s0], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s1], acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s2], acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s3], acc[3], 0, 0, 0);
+        }
+        // D[4*lq + r][lr] of column tile ct -> Y[pair][rt*16 + 4*lq + r]
+        for (int ct = 0; ct < 4; ct++) {
+            const int nn = ct * 16 + lr;
+            if (nn < cnt) *(f32x4*)(Y + (size_t)colpair[off + nn] * ystride + rt * 16 + 4 * lq) = acc[ct];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_m2l_reduce(const int* __restrict__ items, const int* __restrict__ ptr,
+                                                    const float* __restrict__ Y, int ystride, int blk,
+                                                    double* __restrict__ locals) {
+    const int k = blockIdx.x;
+    const int t = items[k], p0 = ptr[k], p1 = ptr[k + 1];
+    for (int row = threadIdx.x; row < blk; row += blockDim.x) {
+        double s = 0;
+        for (int p = p0; p < p1; p++) s += (double)Y[(size_t)p * ystride + row];
+        locals[(size_t)t * blk + row] += s;
+    }
+}
+
+struct ClassKey {
+    unsigned long long k;
+    bool operator==(const ClassKey& o) const { return k == o.k; }
+};
+struct ClassHash {
+    size_t operator()(const ClassKey& c) const { return std::hash<unsigned long long>()(c.k); }
+};
+
+}  // namespace
+
+void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
+                    bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
+                    const double* d_dc2e, hipStream_t s) {
+    auto t0 = std::chrono::steady_clock::now();
+    M = M2lMfma{};
+    const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
+    M.blk = n;
+    if (!same_tree || n > kMaxBlk || m2l.cell.empty()) return;
+    // the staged columns must fit one workgroup's LDS on this device
+    const size_t shm = (size_t)((n + 3) / 4) * 4 * kUnitCols * sizeof(float);
+    int dev = 0, max_shm = 0;
+    TBEM_HIP(hipGetDevice(&dev));
+    TBEM_HIP(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (shm > (size_t)max_shm) return;
+    if (shm > 65536)
+        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    const int n_pairs = (int)m2l.cell.size();
+
+    // classes: (level_t, level_s, q) with q = (c_s - c_t) / finer half-width, integral
+    std::unordered_map<ClassKey, int, ClassHash> cls_of;
+    std::vector<int> pair_cls(n_pairs), rep_t, rep_s;
+    for (size_t k = 0; k + 1 < m2l.ptr.size(); k++) {
+        const int t = m2l.items[k];
+        for (int p = m2l.ptr[k]; p < m2l.ptr[k + 1]; p++) {
+            const int sc = m2l.cell[p];
+            const int Lt = ot.level[t], Ls = st.level[sc];
+            const double ct[3] = {ot.cx[t], ot.cy[t], ot.cz[t]}, cs[3] = {st.cx[sc], st.cy[sc], st.cz[sc]};
+            unsigned long long key = ((unsigned long long)Lt << 59) | ((unsigned long long)Ls << 54);
+            for (int a = 0; a < 3; a++) {
+                const double hf = std::min(ot.level_hw[3 * Lt + a], st.level_hw[3 * Ls + a]);
+                const double q = (cs[a] - ct[a]) / hf;
+                const double qi = std::nearbyint(q);
+                if (!(std::fabs(q - qi) < 1e-6) || std::fabs(qi) >= (1 << 17)) return;  // not on one grid
+                key |= (unsigned long long)((long long)qi + (1 << 17)) << (36 - 18 * a);
+            }
+            auto it = cls_of.find(ClassKey{key});
+            int c;
+            if (it == cls_of.end()) {
+                c = (int)rep_t.size();
+                cls_of.emplace(ClassKey{key}, c);
+                rep_t.push_back(t);
+                rep_s.push_back(sc);
+            } else {
+                c = it->second;
+            }
+            pair_cls[p] = c;
+        }
+    }
+    const int nc = (int)rep_t.size();
+    const int mt = (n + 15) / 16, kt = (n + 3) / 4;
+    M.n_classes = nc;
+    M.mt = mt;
+    M.kt = kt;
+    M.ystride = mt * 16;
+    M.n_pairs = n_pairs;
+
+    // operators, in chunks of classes (FP64 temporaries bounded to ~0.5 GB each)
+    const size_t nn = (size_t)n * n, per = (size_t)mt * kt * 64;
+    M.S.alloc((size_t)nc * per);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nc, (size_t)kMaxChunk, (size_t)(5e8 / (nn * 8))}));
+    DevBuf<double> K((size_t)chunk * nn), T1((size_t)chunk * nn), S64((size_t)chunk * nn);
+    DevBuf<double> d_tc, d_sc, d_tr, d_sr, d_inner, d_unit;
+    DevBuf<int> d_iota, d_ls, d_lt;
+    d_inner.upload(surf.inner, s);
+    d_unit.upload(surf.unit, s);
+    std::vector<int> iota(chunk);
+    for (int i = 0; i < chunk; i++) iota[i] = i;
+    d_iota.upload(iota, s);
+    for (int c0 = 0; c0 < nc; c0 += chunk) {
+        const int m = std::min(chunk, nc - c0);
+        std::vector<double> tc(3 * m), scn(3 * m), tr(m), sr(m);
+        std::vector<int> lt(m), ls(m);
+        for (int i = 0; i < m; i++) {
+            const int t = rep_t[c0 + i], sc = rep_s[c0 + i];
+            tc[3 * i] = ot.cx[t]; tc[3 * i + 1] = ot.cy[t]; tc[3 * i + 2] = ot.cz[t];
+            scn[3 * i] = st.cx[sc]; scn[3 * i + 1] = st.cy[sc]; scn[3 * i + 2] = st.cz[sc];
+            lt[i] = ot.level[t];
+            ls[i] = st.level[sc];
+            tr[i] = ot.level_r[lt[i]];
+            sr[i] = st.level_r[ls[i]];
+        }
+        d_tc.upload(tc, s); d_sc.upload(scn, s); d_tr.upload(tr, s); d_sr.upload(sr, s);
+        d_lt.upload(lt, s); d_ls.upload(ls, s);
+        hipLaunchKernelGGL(k_m2l_assemble, dim3(std::max(1, std::min(64, div_up((size_t)n_e * n_e, 256))), m),
+                           dim3(256), 0, s, kid, kc, n_e, R, d_tc.p, d_sc.p, d_tr.p, d_sr.p, d_inner.p, d_unit.p,
+                           K.p);
+        TBEM_LAUNCH_CHECK();
+        const dim3 gg(div_up(n, 64), div_up(n, 64), m);
+        hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, K.p, d_iota.p, d_uc2e, d_ls.p, T1.p);
+        TBEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, d_dc2e, d_lt.p, T1.p, d_iota.p, S64.p);
+        TBEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_m2l_swizzle, dim3(std::max(1, std::min(256, div_up(per, 256))), m), dim3(256), 0, s,
+                           n, mt, kt, S64.p, M.S.p + (size_t)c0 * per);
+        TBEM_LAUNCH_CHECK();
+        TBEM_HIP(hipStreamSynchronize(s));  // the host vectors above are reused next chunk
+    }
+
+    // work units: classes binned to XCDs by pair count, units interleaved so that
+    // workgroup u lands on XCD u % 8 with the units of its bin
+    std::vector<std::vector<int>> pairs_of(nc);
+    for (int p = 0; p < n_pairs; p++) pairs_of[pair_cls[p]].push_back(p);
+    std::vector<int> order(nc);
+    for (int c = 0; c < nc; c++) order[c] = c;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return pairs_of[a].size() > pairs_of[b].size(); });
+    std::vector<size_t> load(kXcds, 0);
+    std::vector<std::vector<std::array<int, 3>>> bins(kXcds);  // (class, off, cnt)
+    std::vector<int> colpair;
+    colpair.reserve(n_pairs);
+    for (int c: order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        const auto& pl = pairs_of[c];
+        for (size_t j = 0; j < pl.size(); j += kUnitCols) {
+            const int cnt = (int)std::min<size_t>(kUnitCols, pl.size() - j);
+            bins[x].push_back({c, (int)colpair.size(), cnt});
+            for (int q = 0; q < cnt; q++) colpair.push_back(pl[j + q]);
+        }
+        load[x] += (pl.size() + kUnitCols - 1) / kUnitCols;
+    }
+    size_t rounds = 0;
+    for (auto& b: bins) rounds = std::max(rounds, b.size());
+    std::vector<int> ucls(rounds * kXcds, 0), uoff(rounds * kXcds, 0), ucnt(rounds * kXcds, 0);
+    for (size_t r = 0; r < rounds; r++)
+        for (int x = 0; x < kXcds; x++)
+            if (r < bins[x].size()) {
+                ucls[r * kXcds + x] = bins[x][r][0];
+                uoff[r * kXcds + x] = bins[x][r][1];
+                ucnt[r * kXcds + x] = bins[x][r][2];
+            }
+    M.n_units = (int)(rounds * kXcds);
+    M.unit_class.upload(ucls, s);
+    M.unit_off.upload(uoff, s);
+    M.unit_cnt.upload(ucnt, s);
+    M.colpair.upload(colpair, s);
+    TBEM_HIP(hipStreamSynchronize(s));
+    M.enabled = true;
+    M.plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const double* chk_up, float* Y, hipStream_t s) {
+    if (!M.enabled || M.n_units == 0) return;
+    const size_t shm = (size_t)M.kt * 4 * kUnitCols * sizeof(float);
+    hipLaunchKernelGGL(k_m2l_mfma, dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride, M.S.p,
+                       M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, pair_src, chk_up, Y);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_m2l_reduce(int n_items, const int* items, const int* ptr, const float* Y, int ystride, int blk,
+                       double* locals, hipStream_t s) {
+    if (n_items == 0) return;
+    hipLaunchKernelGGL(k_m2l_reduce, dim3(n_items), dim3(256), 0, s, items, ptr, Y, ystride, blk, locals);
+    TBEM_LAUNCH_CHECK();
+}
+
+}  // namespace tbem

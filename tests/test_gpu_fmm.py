@@ -231,6 +231,40 @@ def test_gpu_fmm_mixed_close_to_fp64(tbem, name):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("kid,order,leaf", [(3, 6, 64), (3, 20, 64), (0, 50, 100), (3, 50, 128)])
+def test_gpu_mfma_m2l_matches_fp64(tbem, wl, kid, order, leaf):
+    """Mixed precision with the M2L on FP32 MFMA (one octree for obs and src):
+    the operators are in use (translation classes built) and the matvec agrees
+    with the plain FP64 FMM to the FP32 bound."""
+    g = wl.cfg2_fault_and_surface(30000)
+    K = tbem.make_kernel(kid, 1.0, 0.25)
+    data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
+    x = g["x"] if kid >= 3 else g["x"][:30000]
+    cfg = tbem.FMMConfig(0.3, order, leaf, 0.05, False)
+    case("mfma m2l", kid, order)
+    op32 = tbem.FMMOperator(K, data, cfg, precision="mixed")
+    assert op32.info()["m2l_classes"] > 0
+    y64 = tbem.FMMOperator(K, data, cfg).apply(x)
+    err = rel_err(op32.apply(x), y64)
+    print(f"  classes {op32.info()['m2l_classes']:.0f}, mixed vs FP64 FMM {err:.3e}")
+    assert err < 2e-6
+
+
+def test_gpu_mfma_m2l_falls_back_for_distinct_trees(tbem):
+    """obs != src: the cells of the two trees are on different grids, so the
+    mixed path keeps the FP64 M2L (no classes) and stays accurate."""
+    rng = np.random.default_rng(11)
+    src, obs = rng.uniform(0, 1, (8000, 3)), rng.uniform(0, 1, (6000, 3))
+    n = np.zeros_like(src)
+    data = tbem.NBodyData(obs, np.zeros_like(obs), src, n, np.ones(8000))
+    cfg = tbem.FMMConfig(0.3, 20, 50, 0.05, False)
+    x = rng.standard_normal(8000)
+    case("mfma fallback")
+    op32 = tbem.FMMOperator(tbem.LaplaceSingle(), data, cfg, precision="mixed")
+    assert op32.info()["m2l_classes"] == 0
+    assert rel_err(op32.apply(x), tbem.FMMOperator(tbem.LaplaceSingle(), data, cfg).apply(x)) < 1e-6
+
+
 # --------------------------------------------------------------------- edge cases
 def test_gpu_empty_and_degenerate_inputs(tbem):
     K = tbem.LaplaceSingle()
