@@ -8,6 +8,12 @@ namespace tbem {
 void gpu_pinv_batched(const double* d_mats, int nmat, int n, double threshold, double* d_out,
                       hipStream_t s);
 
+// One-sided Jacobi SVD of nmat row-major n x n FP64 matrices A: on return, row i of
+// W[m] is column i of A V (= sigma_i u_i) and row i of V[m] is v_i, so that
+// A = sum_i W_i V_i^T; the order of the singular triplets is unspecified.
+void gpu_svd_batched(const double* d_mats, int nmat, int n, DevBuf<double>& W, DevBuf<double>& V,
+                     hipStream_t s);
+
 // Per-level UC2E (up = true: equivalent surface inner, check surface outer) or DC2E
 // (up = false: equivalent outer, check inner) of tree t, levels concatenated.
 void compute_c2e(int kid, const KConst& kc, const HostTree& t, const Surfaces& surf, bool up,

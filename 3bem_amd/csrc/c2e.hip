@@ -143,6 +143,7 @@ __global__ __launch_bounds__(1024) void k_jacobi_pinv(int n, double threshold, c
         __syncthreads();
     }
     if (tid == 0) status[m] = (sweep < 60) ? 0 : 1;
+    if (out == nullptr) return;  // SVD only: W (rows = columns of A V) and V are the factors
 
     // singular values = column norms of A V
     for (int i = tid; i < n; i += nthr) {
@@ -177,6 +178,19 @@ void gpu_pinv_batched(const double* d_mats, int nmat, int n, double threshold, d
     DevBuf<int> st(nmat);
     hipLaunchKernelGGL(k_jacobi_pinv, dim3(nmat), dim3(1024), n * sizeof(double), s, n, threshold,
                        d_mats, W.p, V.p, d_out, st.p);
+    TBEM_LAUNCH_CHECK();
+    auto hs = st.to_host(s);
+    for (int v: hs) TBEM_REQUIRE(v == 0, TBEM_E_NUMERIC, "Jacobi SVD did not converge in 60 sweeps");
+}
+
+void gpu_svd_batched(const double* d_mats, int nmat, int n, DevBuf<double>& W, DevBuf<double>& V,
+                     hipStream_t s) {
+    W.alloc((size_t)nmat * n * n);
+    V.alloc((size_t)nmat * n * n);
+    if (nmat == 0 || n == 0) return;
+    DevBuf<int> st(nmat);
+    hipLaunchKernelGGL(k_jacobi_pinv, dim3(nmat), dim3(1024), n * sizeof(double), s, n, 0.0, d_mats, W.p, V.p,
+                       (double*)nullptr, st.p);
     TBEM_LAUNCH_CHECK();
     auto hs = st.to_host(s);
     for (int v: hs) TBEM_REQUIRE(v == 0, TBEM_E_NUMERIC, "Jacobi SVD did not converge in 60 sweeps");

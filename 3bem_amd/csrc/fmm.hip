@@ -325,6 +325,7 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
     a.st = view(P.dst); a.sp = view(P.sp); a.dens = W.dens.p; a.n_src_pts = Ns;
     a.c2e = P.uc2e.p; a.out = W.mult.p;
     a.chk_out = mfma ? W.chk_up.p : nullptr;
+    a.chk_basis = P.m2l_mfma.Bu.p;
     launch_surf_stage(P.kid, false, false, false, a, P.kc, s);
     TBEM_HIP(hipEventRecord(W.ev[2], s));
 
@@ -336,6 +337,7 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
         m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
         m.c2e = P.uc2e.p; m.out = W.mult.p;
         m.chk_out = mfma ? W.chk_up.p : nullptr;
+        m.chk_basis = P.m2l_mfma.Bu.p;
         launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
     }
     TBEM_HIP(hipEventRecord(W.ev[3], s));
@@ -343,9 +345,9 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
     // M2L (cpp/fmm.cpp:362-382) and P2L (cpp/fmm.cpp:296-330) into locals
     launch_fill(W.locals.p, W.locals.n, 0.0, s);
     if (mfma) {
-        // S_c * up-check potential = DC2E K_c UC2E check = the FP64 stage's DC2E K_c mult
+        // Vd M_c Bu check = DC2E K_c UC2E check = the FP64 stage's DC2E K_c mult
         launch_m2l_mfma(P.m2l_mfma, P.m2l.cell.p, W.chk_up.p, W.m2l_y.p, s);
-        launch_m2l_reduce(P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, W.m2l_y.p, P.m2l_mfma.ystride, P.blk,
+        launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.dot.level.p, W.m2l_y.p,
                           W.locals.p, s);
     } else {
         SurfStage m = surf_base(P);

@@ -7,20 +7,22 @@ namespace tbem {
 
 // Plan part. Every M2L pair (target t, source s) of the same octree falls in a
 // class (level(t), level(s), integer offset of the centers in units of the finer
-// half-width); all pairs of a class share one operator
-//   S_c = DC2E[level t] * K_c * UC2E[level s]        (blk x blk, formed in FP64)
+// half-width); all pairs of a class share one operator. With the modal factors
+// UC2E = Ru Bu and DC2E = Vd Ld of the C2E matrices (Jacobi SVD), the FP64 M2L
+// contribution DC2E K_c UC2E check_s is computed as
+//   Vd[level t] * M_c * (Bu[level s] * check_s),   M_c = Ld K_c Ru  (blk x blk)
 // with K_c(check_i <- equiv_j) the kernel between the target's down-check and the
-// source's up-equivalent surface. S_c maps the source's up-check potential (the
-// field of its points on its check sphere, natural size) straight to the
-// target's down-equivalent density, so the FP32 data never passes through a
-// pseudo-inverse, whose conditioning FP32 input cannot survive. S_c is stored in
-// FP32, pre-swizzled into v_mfma_f32_16x16x4f32 A-operand order:
-//   S[((c*mt + rt)*kt + ks)*64 + lane] = S_c[rt*16 + lane%16][ks*4 + lane/16].
+// source's up-equivalent surface. Only M_c and the modal input Bu * check_s are
+// FP32: the FP32 data never pass through a pseudo-inverse, whose conditioning FP32
+// input cannot survive, and in the modal basis they carry no cancellation. M_c is
+// stored pre-swizzled into v_mfma_f32_16x16x4f32 A-operand order:
+//   S[((c*mt + rt)*kt + ks)*64 + lane] = M_c[rt*16 + lane%16][ks*4 + lane/16].
 struct M2lMfma {
     bool enabled = false;
     int n_classes = 0, blk = 0, mt = 0, kt = 0, ystride = 0;
     int n_units = 0, n_pairs = 0;
     DevBuf<float> S;
+    DevBuf<double> Bu, Vd;  // per source / observation level, blk x blk row-major
     DevBuf<int> unit_class, unit_off, unit_cnt, colpair;
     double plan_seconds = 0;
 };
@@ -32,11 +34,12 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
                     bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
                     const double* d_dc2e, hipStream_t s);
 
-// Y[p] (ystride floats per pair) = S_{class(p)} * float(chk_up[src(p)]) for every pair.
+// Y[p] (ystride floats per pair) = M_{class(p)} * float(chk_up[src(p)]) for every pair,
+// chk_up = Bu * up-check potential per source cell.
 void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const double* chk_up, float* Y, hipStream_t s);
 
-// locals[t] += sum over the pairs p of target t of Y[p] (FP64 sum), for every M2L target.
-void launch_m2l_reduce(int n_items, const int* items, const int* ptr, const float* Y, int ystride, int blk,
-                       double* locals, hipStream_t s);
+// locals[t] += Vd[level t] * (sum over the pairs p of target t of Y[p]), in FP64.
+void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const int* level,
+                       const float* Y, double* locals, hipStream_t s);
 
 }  // namespace tbem

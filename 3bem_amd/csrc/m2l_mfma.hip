@@ -1,9 +1,19 @@
 // M2L of the mixed-precision path as grouped FP32 MFMA GEMMs (gfx950).
 //
-// Plan: pairs -> classes (level_t, level_s, integer offset); per class, in FP64 on
-// the GPU: K_c assembled from the kernel's reference expression (tensor_ref, as the
-// C2E systems), then S_c = DC2E * (K_c * UC2E) by a tiled FP64 GEMM, then rounded
-// to FP32 in MFMA A-operand order. Work units are (class, up to 64 pairs); units of
+// Plan: pairs -> classes (level_t, level_s, integer offset). The C2E matrices are
+// factored by Jacobi SVD, C2E = sum_i w_i v_i^T (w_i = s_i a_i in equivalent space,
+// v_i in check space, dropped modes have s_i ~ 0), which gives per level
+//   up:   Bu = [v_i^T] (rows),        Ru = [w_i] (columns)      UC2E = Ru Bu
+//   down: Ld = [s_i v_i^T] (rows),    Vd = [w_i / s_i] (columns) DC2E = Vd Ld
+// and per class, in FP64 on the GPU: K_c assembled from the kernel's reference
+// expression (tensor_ref, as the C2E systems), then M_c = Ld * (K_c * Ru) by a tiled
+// FP64 GEMM, rounded to FP32 in MFMA A-operand order. The FP32 data are the
+// up-check potential in modal form, Bu * check (computed in FP64 by the P2M/M2M
+// epilogue), and M_c's output z; the target adds Vd * sum(z) in FP64. In this
+// basis each entry and the input it multiplies scale together mode by mode, so the
+// FP32 sums carry no cancellation: measured in a single-pair model at orders
+// 20-150, <= 5e-7 relative, against up to 3e-4 for the operator DC2E K_c UC2E
+// applied in FP32 in the physical basis. Work units are (class, up to 64 pairs); units of
 // one class go to one XCD (workgroup i runs on XCD i % 8), so each S_c is fetched
 // from HBM once and then served by that XCD's L2.
 //
@@ -20,6 +30,7 @@
 #include <cmath>
 #include <unordered_map>
 
+#include "c2e.h"
 #include "m2l_mfma.h"
 
 namespace tbem {
@@ -145,15 +156,61 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int yst
     }
 }
 
+// locals[t] += Vd[level t] * sum_p Y[p], sums in FP64
 __global__ __launch_bounds__(256) void k_m2l_reduce(const int* __restrict__ items, const int* __restrict__ ptr,
+                                                    const int* __restrict__ level, const double* __restrict__ Vd,
                                                     const float* __restrict__ Y, int ystride, int blk,
                                                     double* __restrict__ locals) {
+    extern __shared__ double z[];  // blk
     const int k = blockIdx.x;
-    const int t = items[k], p0 = ptr[k], p1 = ptr[k + 1];
+    const int t = uni(items[k]), p0 = uni(ptr[k]), p1 = uni(ptr[k + 1]);
     for (int row = threadIdx.x; row < blk; row += blockDim.x) {
         double s = 0;
         for (int p = p0; p < p1; p++) s += (double)Y[(size_t)p * ystride + row];
+        z[row] = s;
+    }
+    __syncthreads();
+    const double* V = Vd + (size_t)level[t] * blk * blk;
+    for (int row = threadIdx.x; row < blk; row += blockDim.x) {
+        double s = 0;
+        const double* v = V + (size_t)row * blk;
+        for (int col = 0; col < blk; col++) s += v[col] * z[col];
         locals[(size_t)t * blk + row] += s;
+    }
+}
+
+// Per-level modal factors of C2E = sum_i W_i V_i^T (rows of W and V from the
+// Jacobi SVD). Modes with |W_i| <= 1e-13 max_j |W_j| (the ones the pseudo-inverse
+// dropped) are zeroed. up: Bu rows V_i, Ru columns W_i; down: Ld rows s_i V_i,
+// Vd columns W_i / s_i.
+__global__ __launch_bounds__(256) void k_modal_factors(int n, int up, const double* __restrict__ W,
+                                                       const double* __restrict__ V, double* __restrict__ Lrows,
+                                                       double* __restrict__ Rcols) {
+    extern __shared__ double sv[];  // n norms + 1
+    const int L = blockIdx.x;
+    const double* w = W + (size_t)L * n * n;
+    const double* v = V + (size_t)L * n * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double s = 0;
+        for (int j = 0; j < n; j++) s += w[(size_t)i * n + j] * w[(size_t)i * n + j];
+        sv[i] = sqrt(s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0;
+        for (int i = 0; i < n; i++) mx = fmax(mx, sv[i]);
+        sv[n] = mx;
+    }
+    __syncthreads();
+    const double cut = 1e-13 * sv[n];
+    double* Lr = Lrows + (size_t)L * n * n;
+    double* Rc = Rcols + (size_t)L * n * n;
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        const bool keep = sv[i] > cut;
+        const double vij = v[(size_t)i * n + j], wij = w[(size_t)i * n + j];
+        Lr[(size_t)i * n + j] = keep ? (up ? vij : sv[i] * vij) : 0.0;     // row i
+        Rc[(size_t)j * n + i] = keep ? (up ? wij : wij / sv[i]) : 0.0;    // column i
     }
 }
 
@@ -223,8 +280,27 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.ystride = mt * 16;
     M.n_pairs = n_pairs;
 
-    // operators, in chunks of classes (FP64 temporaries bounded to ~0.5 GB each)
+    // modal factors of UC2E (source tree levels) and DC2E (observation tree levels)
     const size_t nn = (size_t)n * n, per = (size_t)mt * kt * 64;
+    DevBuf<double> Ru, Ld;
+    {
+        DevBuf<double> W, V;
+        gpu_svd_batched(d_uc2e, st.n_levels, n, W, V, s);
+        M.Bu.alloc((size_t)st.n_levels * nn);
+        Ru.alloc((size_t)st.n_levels * nn);
+        hipLaunchKernelGGL(k_modal_factors, dim3(st.n_levels), dim3(256), (n + 1) * sizeof(double), s, n, 1, W.p,
+                           V.p, M.Bu.p, Ru.p);
+        TBEM_LAUNCH_CHECK();
+        gpu_svd_batched(d_dc2e, ot.n_levels, n, W, V, s);
+        Ld.alloc((size_t)ot.n_levels * nn);
+        M.Vd.alloc((size_t)ot.n_levels * nn);
+        hipLaunchKernelGGL(k_modal_factors, dim3(ot.n_levels), dim3(256), (n + 1) * sizeof(double), s, n, 0, W.p,
+                           V.p, Ld.p, M.Vd.p);
+        TBEM_LAUNCH_CHECK();
+        TBEM_HIP(hipStreamSynchronize(s));
+    }
+
+    // operators M_c = Ld * K_c * Ru, in chunks of classes (FP64 temporaries <= ~0.5 GB each)
     M.S.alloc((size_t)nc * per);
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nc, (size_t)kMaxChunk, (size_t)(5e8 / (nn * 8))}));
     DevBuf<double> K((size_t)chunk * nn), T1((size_t)chunk * nn), S64((size_t)chunk * nn);
@@ -255,9 +331,9 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
                            K.p);
         TBEM_LAUNCH_CHECK();
         const dim3 gg(div_up(n, 64), div_up(n, 64), m);
-        hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, K.p, d_iota.p, d_uc2e, d_ls.p, T1.p);
+        hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, K.p, d_iota.p, Ru.p, d_ls.p, T1.p);
         TBEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, d_dc2e, d_lt.p, T1.p, d_iota.p, S64.p);
+        hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, Ld.p, d_lt.p, T1.p, d_iota.p, S64.p);
         TBEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_m2l_swizzle, dim3(std::max(1, std::min(256, div_up(per, 256))), m), dim3(256), 0, s,
                            n, mt, kt, S64.p, M.S.p + (size_t)c0 * per);
@@ -314,10 +390,11 @@ void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const double* chk_up
     TBEM_LAUNCH_CHECK();
 }
 
-void launch_m2l_reduce(int n_items, const int* items, const int* ptr, const float* Y, int ystride, int blk,
-                       double* locals, hipStream_t s) {
-    if (n_items == 0) return;
-    hipLaunchKernelGGL(k_m2l_reduce, dim3(n_items), dim3(256), 0, s, items, ptr, Y, ystride, blk, locals);
+void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const int* level,
+                       const float* Y, double* locals, hipStream_t s) {
+    if (!M.enabled || n_items == 0) return;
+    hipLaunchKernelGGL(k_m2l_reduce, dim3(n_items), dim3(256), M.blk * sizeof(double), s, items, ptr, level,
+                       M.Vd.p, Y, M.ystride, M.blk, locals);
     TBEM_LAUNCH_CHECK();
 }
 

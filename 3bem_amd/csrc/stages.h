@@ -48,7 +48,8 @@ struct SurfStage {
     int n_e, blk;
     const double* c2e;      // per level blk*blk row-major, or nullptr
     double* out;            // blk per cell of tt
-    double* chk_out;        // optional: the field itself (before C2E), blk per cell of tt
+    double* chk_out;        // optional: chk_basis[level] * field (or the field itself), blk per cell of tt
+    const double* chk_basis;  // optional: per level blk*blk row-major
 };
 
 // Point-target stage (FP64, or T = float for the point sources (a) with every
