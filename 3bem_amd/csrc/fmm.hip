@@ -64,7 +64,7 @@ struct Work {
     hipStream_t stream = nullptr;
     DevBuf<double> dens, mult, locals, out_sorted, xbuf, ybuf;
     DevBuf<double> gsort, lbar, cbar, mbar, ebar, grad_sorted;
-    DevBuf<double> chk_up;  // up-check potential per source cell (input of the MFMA M2L)
+    DevBuf<float> chk_up;   // modal up-check potential per source cell (input of the MFMA M2L)
     DevBuf<float> m2l_y;    // per-pair MFMA M2L output
     hipEvent_t ev[8];
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran

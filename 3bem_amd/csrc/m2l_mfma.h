@@ -34,9 +34,9 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
                     bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
                     const double* d_dc2e, hipStream_t s);
 
-// Y[p] (ystride floats per pair) = M_{class(p)} * float(chk_up[src(p)]) for every pair,
-// chk_up = Bu * up-check potential per source cell.
-void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const double* chk_up, float* Y, hipStream_t s);
+// Y[p] (ystride floats per pair) = M_{class(p)} * chk_up[src(p)] for every pair,
+// chk_up = Bu * up-check potential per source cell, rounded to FP32.
+void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s);
 
 // locals[t] += Vd[level t] * (sum over the pairs p of target t of Y[p]), in FP64.
 void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const int* level,

@@ -115,11 +115,36 @@ __global__ void k_m2l_swizzle(int n, int mt, int kt, const double* __restrict__ 
     }
 }
 
+// One row tile (16 rows of M_c) times the NCT staged column tiles. A comes from
+// global memory (L2-resident), loaded kPre k-steps ahead of its use so that the
+// loads' latency overlaps kPre * NCT MFMAs.
+constexpr int kPre = 8;
+
+template <int NCT>
+__device__ __forceinline__ void m2l_row_tile(const float* __restrict__ A, const float* __restrict__ b, int kt,
+                                             int s0, int s1, int s2, int s3, f32x4* acc) {
+    float cur[kPre], nxt[kPre];
+    for (int i = 0; i < kPre; i++) cur[i] = i < kt ? A[(size_t)i * 64] : 0.f;
+    for (int k0 = 0; k0 < kt; k0 += kPre) {
+        for (int i = 0; i < kPre; i++) nxt[i] = k0 + kPre + i < kt ? A[(size_t)(k0 + kPre + i) * 64] : 0.f;
+#pragma unroll
+        for (int i = 0; i < kPre; i++) {
+            if (k0 + i >= kt) break;
+            const float* bk = b + (k0 + i) * 4 * 64;
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[i], bk[s0], acc[0], 0, 0, 0);
+            if (NCT > 1) acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[i], bk[s1], acc[1], 0, 0, 0);
+            if (NCT > 2) acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[i], bk[s2], acc[2], 0, 0, 0);
+            if (NCT > 3) acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur[i], bk[s3], acc[3], 0, 0, 0);
+        }
+        for (int i = 0; i < kPre; i++) cur[i] = nxt[i];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int ystride, const float* __restrict__ S,
                                                   const int* __restrict__ ucls, const int* __restrict__ uoff,
                                                   const int* __restrict__ ucnt, const int* __restrict__ colpair,
                                                   const int* __restrict__ pair_src,
-                                                  const double* __restrict__ chk, float* __restrict__ Y) {
+                                                  const float* __restrict__ chk, float* __restrict__ Y) {
     extern __shared__ float Bs[];  // [kt*4][64], swizzled
     const int u = blockIdx.x;
     const int cls = uni(ucls[u]), off = uni(uoff[u]), cnt = uni(ucnt[u]);
@@ -127,29 +152,27 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int yst
     const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
     const int kpad = kt * 4;
     // stage: lane = column (pair), waves stride over k
-    const double* col = lane < cnt ? chk + (size_t)pair_src[colpair[off + lane]] * n : nullptr;
+    const float* col = lane < cnt ? chk + (size_t)pair_src[colpair[off + lane]] * n : nullptr;
     for (int kk = wave; kk < kpad; kk += 4)
-        Bs[kk * 64 + (lane ^ ((kk & 3) << 4))] = (col && kk < n) ? (float)col[kk] : 0.f;
+        Bs[kk * 64 + (lane ^ ((kk & 3) << 4))] = (col && kk < n) ? col[kk] : 0.f;
     __syncthreads();
     // B operand of column tile ct at k-step ks: row ks*4 + lq, column ct*16 + lr,
     // stored at ((ct ^ lq) * 16 + lr)
     const int lr = lane % 16, lq = lane / 16;
     const float* b = Bs + lq * 64 + lr;
     const int s0 = (0 ^ lq) * 16, s1 = (1 ^ lq) * 16, s2 = (2 ^ lq) * 16, s3 = (3 ^ lq) * 16;
+    const int nct = (cnt + 15) / 16;  // column tiles holding pairs (uniform)
     for (int rt = wave; rt < mt; rt += 4) {
         f32x4 acc[4] = {};
         const float* A = S + ((size_t)cls * mt + rt) * kt * 64 + lane;
-#pragma unroll 4
-        for (int ks = 0; ks < kt; ks++) {
-            const float a = A[(size_t)ks * 64];
-            const float* bk = b + ks * 4 * 64;
-            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s0], acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s1], acc[1], 0, 0, 0);
-            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s2], acc[2], 0, 0, 0);
-            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bk[s3], acc[3], 0, 0, 0);
+        switch (nct) {
+        case 1: m2l_row_tile<1>(A, b, kt, s0, s1, s2, s3, acc); break;
+        case 2: m2l_row_tile<2>(A, b, kt, s0, s1, s2, s3, acc); break;
+        case 3: m2l_row_tile<3>(A, b, kt, s0, s1, s2, s3, acc); break;
+        default: m2l_row_tile<4>(A, b, kt, s0, s1, s2, s3, acc); break;
         }
         // D[4*lq + r][lr] of column tile ct -> Y[pair][rt*16 + 4*lq + r]
-        for (int ct = 0; ct < 4; ct++) {
+        for (int ct = 0; ct < nct; ct++) {
             const int nn = ct * 16 + lr;
             if (nn < cnt) *(f32x4*)(Y + (size_t)colpair[off + nn] * ystride + rt * 16 + 4 * lq) = acc[ct];
         }
@@ -382,7 +405,7 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const double* chk_up, float* Y, hipStream_t s) {
+void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s) {
     if (!M.enabled || M.n_units == 0) return;
     const size_t shm = (size_t)M.kt * 4 * kUnitCols * sizeof(float);
     hipLaunchKernelGGL(k_m2l_mfma, dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride, M.S.p,

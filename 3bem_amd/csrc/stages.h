@@ -48,7 +48,7 @@ struct SurfStage {
     int n_e, blk;
     const double* c2e;      // per level blk*blk row-major, or nullptr
     double* out;            // blk per cell of tt
-    double* chk_out;        // optional: chk_basis[level] * field (or the field itself), blk per cell of tt
+    float* chk_out;         // optional: chk_basis[level] * field (or the field), FP32, blk per cell of tt
     const double* chk_basis;  // optional: per level blk*blk row-major
 };
 

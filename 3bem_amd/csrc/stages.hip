@@ -348,17 +348,17 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
         __syncthreads();
     }
     if (a.chk_out) {
-        double* c = a.chk_out + (size_t)cell * a.blk;
+        float* c = a.chk_out + (size_t)cell * a.blk;
         if (a.chk_basis) {
             const double* B = a.chk_basis + (size_t)lvl * a.blk * a.blk;
             for (int row = tid; row < a.blk; row += NB) {
                 double s = 0;
                 const double* b = B + (size_t)row * a.blk;
                 for (int col = 0; col < a.blk; col++) s += b[col] * field[col];
-                c[row] = s;
+                c[row] = (float)s;
             }
         } else {
-            for (int row = tid; row < a.blk; row += NB) c[row] = field[row];
+            for (int row = tid; row < a.blk; row += NB) c[row] = (float)field[row];
         }
     }
     double* o = a.out + (size_t)cell * a.blk;
