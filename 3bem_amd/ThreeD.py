@@ -310,6 +310,12 @@ class FMMOperator:
         _lib.check(self._lib.tbem_fmm_stage_times(self._h, _dp(out)))
         return dict(zip(("gather", "p2m", "m2m", "m2l", "l2l", "leaf", "scatter"), out.tolist()))
 
+    def m2l_kernel_ms(self):
+        """Device milliseconds of the M2L translation kernel alone in the last apply."""
+        out = np.zeros(1)
+        _lib.check(self._lib.tbem_fmm_m2l_ms(self._h, _dp(out)))
+        return float(out[0])
+
 
 class DirectOperator:
     """Matrix-free direct N-body operator on the GPU: replaces

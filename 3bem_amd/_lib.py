@@ -20,7 +20,7 @@ EXPORTED = (
     "tbem_fmm_apply_adjoint_dev", "tbem_fmm_obs_grad", "tbem_fmm_obs_grad_dev",
     "tbem_fmm_set_obs_locs", "tbem_fmm_n_rows", "tbem_fmm_n_cols", "tbem_fmm_clone",
     "tbem_fmm_destroy", "tbem_fmm_info", "tbem_fmm_c2e", "tbem_fmm_pairs", "tbem_fmm_octree",
-    "tbem_fmm_stage_times", "tbem_nbody_create", "tbem_nbody_apply", "tbem_nbody_apply_dev",
+    "tbem_fmm_stage_times", "tbem_fmm_m2l_ms", "tbem_nbody_create", "tbem_nbody_apply", "tbem_nbody_apply_dev",
     "tbem_nbody_apply_adjoint", "tbem_nbody_n_rows", "tbem_nbody_n_cols", "tbem_nbody_destroy",
     "tbem_kernel_eval", "tbem_surface", "tbem_pinv", "tbem_last_error", "tbem_device_available",
 )
@@ -98,6 +98,7 @@ def load(path=LIB_PATH):
         "tbem_fmm_pairs": (ctypes.c_int, [_VP, ctypes.c_int, _D, ctypes.POINTER(_SZ)]),
         "tbem_fmm_octree": (ctypes.c_int, [_VP, ctypes.c_int, _D, _L, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
         "tbem_fmm_stage_times": (ctypes.c_int, [_VP, _D]),
+        "tbem_fmm_m2l_ms": (ctypes.c_int, [_VP, _D]),
         "tbem_nbody_create": (ctypes.c_int, [ctypes.POINTER(NbodyDesc), ctypes.POINTER(_VP)]),
         "tbem_nbody_apply": (ctypes.c_int, [_VP, _D, _D]),
         "tbem_nbody_apply_dev": (ctypes.c_int, [_VP, _VP, _VP, _VP]),

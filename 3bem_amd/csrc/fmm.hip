@@ -66,7 +66,7 @@ struct Work {
     DevBuf<double> gsort, lbar, cbar, mbar, ebar, grad_sorted;
     DevBuf<float> chk_up;   // modal up-check potential per source cell (input of the MFMA M2L)
     DevBuf<float> m2l_y;    // per-pair MFMA M2L output
-    hipEvent_t ev[8];
+    hipEvent_t ev[10];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
     explicit Work(const FmmPlan& P) {
@@ -344,9 +344,11 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
 
     // M2L (cpp/fmm.cpp:362-382) and P2L (cpp/fmm.cpp:296-330) into locals
     launch_fill(W.locals.p, W.locals.n, 0.0, s);
+    TBEM_HIP(hipEventRecord(W.ev[8], s));
     if (mfma) {
         // Vd M_c Bu check = DC2E K_c UC2E check = the FP64 stage's DC2E K_c mult
         launch_m2l_mfma(P.m2l_mfma, P.m2l.cell.p, W.chk_up.p, W.m2l_y.p, s);
+        TBEM_HIP(hipEventRecord(W.ev[9], s));
         launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.dot.level.p, W.m2l_y.p,
                           W.locals.p, s);
     } else {
@@ -356,6 +358,7 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
         m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
         m.c2e = P.dc2e.p; m.out = W.locals.p;
         launch_surf_stage(P.kid, true, true, false, m, P.kc, s);
+        TBEM_HIP(hipEventRecord(W.ev[9], s));
     }
     SurfStage pl = surf_base(P);
     set_list(pl, P.p2l);
@@ -817,6 +820,18 @@ int tbem_fmm_stage_times(tbem_fmm_t h, double* ms) {
             TBEM_HIP(hipEventElapsedTime(&t, W.ev[k], W.ev[k + 1]));
             ms[k] = t;
         }
+    });
+}
+
+int tbem_fmm_m2l_ms(tbem_fmm_t h, double* ms) {
+    return guard([&] {
+        TBEM_REQUIRE(h && ms, TBEM_E_INVALID, "null argument");
+        Work& W = *h->work;
+        TBEM_REQUIRE(W.timed, TBEM_E_INVALID, "no timed apply yet");
+        TBEM_HIP(hipEventSynchronize(W.ev[7]));
+        float t = 0;
+        TBEM_HIP(hipEventElapsedTime(&t, W.ev[8], W.ev[9]));
+        *ms = t;
     });
 }
 

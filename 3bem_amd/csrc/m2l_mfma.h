@@ -29,7 +29,8 @@ struct M2lMfma {
 
 // m2l: the forward M2L list (items = target cells, cell[p] = source of pair p).
 // Leaves M.enabled false (the caller keeps the FP64 M2L) when the two trees
-// differ, an offset is not integral, or blk exceeds what one workgroup stages.
+// differ, an offset is not integral, blk exceeds what one workgroup stages, or
+// blk is below 48 (where the FP64 stage is faster).
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
                     bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
                     const double* d_dc2e, hipStream_t s);

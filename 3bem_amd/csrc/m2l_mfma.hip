@@ -39,6 +39,11 @@ namespace {
 
 constexpr int kUnitCols = 64;   // pairs per work unit (= LDS row length of the staged columns)
 constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 KiB of LDS
+// Below this operator size the FP64 M2L stage is the cheaper one: measured on the
+// 1M-point elastic workload at order 6 (blk 18), the MFMA path with its per-pair
+// buffer and modal epilogues took longer than the 0.4 ms FP64 stage; at blk 60
+// (order 20) it is already faster (2M points: 3.0 vs 5.2 ms).
+constexpr int kMinBlk = 48;
 constexpr int kXcds = 8;
 constexpr int kMaxChunk = 16384;  // classes per plan-time batch (grid y/z limit)
 
@@ -254,7 +259,7 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M = M2lMfma{};
     const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
     M.blk = n;
-    if (!same_tree || n > kMaxBlk || m2l.cell.empty()) return;
+    if (!same_tree || n > kMaxBlk || n < kMinBlk || m2l.cell.empty()) return;
     // the staged columns must fit one workgroup's LDS on this device
     const size_t shm = (size_t)((n + 3) / 4) * 4 * kUnitCols * sizeof(float);
     int dev = 0, max_shm = 0;

@@ -14,7 +14,10 @@ direct operator, itself checked against the CPU oracle in tests/), the
 backward/forward time ratio (adjoint matvec + observation-coordinate gradient vs
 forward), the roofline of the dominant kernel (the fused P2P+M2P+L2P leaf stage)
 timed with HIP events on its stream, and the compiled reference CPU FMM timed on
-this host on a bounded sample.
+this host on a bounded sample. With the default workload, `target_fp32_mfma_vs_fp64`
+adds the precision comparison at the accuracy target: 2M points, order 100 (FP64
+FMM ~4e-7 from the direct sum), FP64 against mixed precision (FP32 MFMA M2L, FP32
+point-to-point), with the roofline of the MFMA M2L kernel.
 """
 import argparse
 import importlib
@@ -112,6 +115,54 @@ def cpu_baseline(kernel, cfg, budget_n):
                       f"({t:.2f} s each; plan build {t_plan:.2f} s excluded), OMP threads={cores}"}
 
 
+def target_run(tbem, torch, steps):
+    """The FP32/MFMA-vs-FP64 comparison at the accuracy target: elastic displacement
+    FMM matvec on 2M points of the fault + free-surface geometry, order 100, leaf 256,
+    MAC 0.3 (FP64 FMM ~3e-7 from the direct sum), both precisions on the same plan
+    parameters, errors against an FP64 direct sum on a 5k-target subsample."""
+    wl = importlib.import_module("3bem_amd.workloads")
+    n = 2_000_000
+    g = wl.cfg2_fault_and_surface(n)
+    K = tbem.ElasticDisplacement(1.0, 0.25)
+    data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
+    cfg = tbem.FMMConfig(0.3, 100, 256, 0.05, False)
+    x = torch.tensor(g["x"], device="cuda")
+    rng = np.random.default_rng(321)
+    sub = np.sort(rng.choice(n, 5000, replace=False))
+    dsub = tbem.NBodyData(g["locs"][sub], g["normals"][sub], g["locs"], g["normals"], g["weights"])
+    yd = tbem.make_direct_nbody_operator(dsub, K).apply(g["x"])
+    idx = np.concatenate([q * n + sub for q in range(3)])
+    out = {"workload": "elastic displacement FMM matvec, 2,000,000 points (fault + free surface), order 100, "
+                       "leaf 256, MAC 0.3"}
+    ys = {}
+    for prec in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        op = tbem.FMMOperator(K, data, cfg, precision=prec)
+        plan = time.perf_counter() - t0
+        dt, m2l = time_steps(lambda: op.apply(x), steps, 1, torch, after=lambda: op.m2l_kernel_ms())
+        ys[prec] = op.apply(x).cpu().numpy()
+        info = op.info()
+        out[prec] = {"ms_per_step": round(dt * 1e3, 3), "Mpts_per_s": round(n / dt / 1e6, 3),
+                     "plan_build_s": round(plan, 2), "m2l_kernel_ms": round(float(np.mean(m2l)), 3),
+                     "stage_ms": {k: round(v, 3) for k, v in op.stage_times().items()},
+                     "rel_err_vs_direct": float(np.linalg.norm(ys[prec][idx] - yd) / np.linalg.norm(yd))}
+        if prec == "mixed":
+            blk = 3 * info["n_e"]
+            flop = 2.0 * info["m2l"] * blk * blk
+            ach = flop / (float(np.mean(m2l)) * 1e-3) / 1e12
+            out["m2l_mfma_roofline"] = {"bound": "mfma", "kernel": "k_m2l_mfma (FP32 v_mfma_f32_16x16x4f32)",
+                                        "achieved": round(ach, 2), "peak": 157.3, "unit": "TFLOP/s",
+                                        "frac": round(ach / 157.3, 4), "algorithmic_flop": flop,
+                                        "classes": info["m2l_classes"], "pairs": info["m2l"]}
+        log(f"[bench] target {prec}: {dt * 1e3:.2f} ms, rel err {out[prec]['rel_err_vs_direct']:.2e}, "
+            f"m2l kernel {out[prec]['m2l_kernel_ms']:.2f} ms")
+        del op
+        torch.cuda.empty_cache()
+    out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(ys["mixed"] - ys["fp64"]) / np.linalg.norm(ys["fp64"]))
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +173,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=100_000)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-extra", action="store_true", help="FP64 forward only")
+    ap.add_argument("--skip-target", action="store_true",
+                    help="skip the 2M-point order-100 FP32/MFMA vs FP64 comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -218,6 +271,9 @@ def main():
         })
         log(f"[bench] rel err fp64 {e64:.3e}, mixed {e32:.3e}; mixed {dt32 * 1e3:.2f} ms; "
             f"adjoint {dt_adj * 1e3:.2f} ms, obs_grad {dt_grad * 1e3:.2f} ms")
+
+    if not args.skip_extra and not args.skip_target and args.config == "cfg2":
+        out["target_fp32_mfma_vs_fp64"] = target_run(tbem, torch, max(3, args.steps // 2))
 
     if not args.skip_cpu:
         cfg_t = (cfg.mac, cfg.order, cfg.min_pts_per_cell, cfg.d, cfg.account_for_small_cells)

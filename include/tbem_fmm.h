@@ -181,6 +181,11 @@ int tbem_fmm_octree(tbem_fmm_t h, int which, double* cells, long* indices,
  * gather, p2m, m2m, m2l, l2l, p2p(+m2p+l2p), scatter (7 doubles). */
 int tbem_fmm_stage_times(tbem_fmm_t h, double* ms7);
 
+/* Device time of the M2L translation kernel alone in the last apply (ms): the
+ * grouped MFMA GEMM in mixed precision, the FP64 M2L stage otherwise. No
+ * reference counterpart: a profiling aid for roofline reports. */
+int tbem_fmm_m2l_ms(tbem_fmm_t h, double* ms);
+
 /* ---------------------------------------------------------- direct N-body operator */
 
 /* Matrix-free replacement for make_direct_nbody_operator(data, K)
