@@ -188,6 +188,17 @@ __device__ inline double rsqrt_t(double x) {
 }
 __device__ inline float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// One Newton step: relative error <= 1.5 * (2^-22)^2 ~ 9e-14, one-sided. Used for
+// point-to-point pairs only, whose densities are of natural size; check potentials
+// (amplified by the C2E) and equivalent-density evaluations keep rsqrt_t.
+__device__ inline double rsqrt_fast(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = x * y;
+    const double e = fma(-h, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+__device__ inline float rsqrt_fast(float x) { return __builtin_amdgcn_rsqf(x); }
+
 template <class T> __device__ inline T cutoff_r2();
 template <> __device__ inline double cutoff_r2<double>() { return 1e-12; }
 template <> __device__ inline float cutoff_r2<float>() { return 1e-12f; }
@@ -235,7 +246,7 @@ inline KC<D3> make_kcd(const KConst& c) {
 // acc[R] += K(obs <- src) v   (ADJ = false)
 // acc[R] += K(obs <- src)^T v (ADJ = true)
 // d = src - obs, r2 = |d|^2; no / ns are the obs and src normals.
-template <int KID, bool ADJ, class T>
+template <int KID, bool ADJ, class T, bool FAST = false>
 __device__ __forceinline__ void kop(T dx, T dy, T dz, T r2, const T* no, const T* ns, const T* v,
                                     T* acc, const KC<T>& k) {
     if (KID == K_LH) return;
@@ -248,7 +259,9 @@ __device__ __forceinline__ void kop(T dx, T dy, T dz, T r2, const T* no, const T
         }
         return;
     }
-    T ri = rsqrt_t(r2);
+    T ri;
+    if constexpr (FAST) ri = rsqrt_fast(r2);
+    else ri = rsqrt_t(r2);
     ri = (r2 < cutoff_r2<T>()) ? T(0) : ri;
     if (KID == K_LS) {
         acc[0] += ri * v[0];             // times 1/(4 pi) at the end

@@ -184,17 +184,38 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int yst
     }
 }
 
-// locals[t] += Vd[level t] * sum_p Y[p], sums in FP64
+// locals[t] += Vd[level t] * sum_p Y[p], sums in FP64. One 256-thread workgroup
+// per target. The target's pairs are contiguous in Y, so the block is read as
+// consecutive float4s: thread i takes float4 column i % q (q = ystride / 4) of
+// every npar-th pair starting at i / q, and the npar partial sums are combined
+// through LDS.
 __global__ __launch_bounds__(256) void k_m2l_reduce(const int* __restrict__ items, const int* __restrict__ ptr,
                                                     const int* __restrict__ level, const double* __restrict__ Vd,
                                                     const float* __restrict__ Y, int ystride, int blk,
                                                     double* __restrict__ locals) {
-    extern __shared__ double z[];  // blk
+    extern __shared__ double part[];  // npar * ystride partial sums, then z (ystride)
     const int k = blockIdx.x;
     const int t = uni(items[k]), p0 = uni(ptr[k]), p1 = uni(ptr[k + 1]);
+    const int q = ystride / 4, npar = 256 / q;
+    const int c = threadIdx.x % q, ph = threadIdx.x / q;
+    if (ph < npar) {
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        const f32x4* y4 = (const f32x4*)Y;
+        for (int p = p0 + ph; p < p1; p += npar) {
+            const f32x4 v = y4[(size_t)p * q + c];
+            s0 += (double)v[0];
+            s1 += (double)v[1];
+            s2 += (double)v[2];
+            s3 += (double)v[3];
+        }
+        double* o = part + (size_t)ph * ystride + 4 * c;
+        o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3;
+    }
+    __syncthreads();
+    double* z = part + (size_t)npar * ystride;
     for (int row = threadIdx.x; row < blk; row += blockDim.x) {
         double s = 0;
-        for (int p = p0; p < p1; p++) s += (double)Y[(size_t)p * ystride + row];
+        for (int h = 0; h < npar; h++) s += part[(size_t)h * ystride + row];
         z[row] = s;
     }
     __syncthreads();
@@ -421,8 +442,9 @@ void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up,
 void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const int* level,
                        const float* Y, double* locals, hipStream_t s) {
     if (!M.enabled || n_items == 0) return;
-    hipLaunchKernelGGL(k_m2l_reduce, dim3(n_items), dim3(256), M.blk * sizeof(double), s, items, ptr, level,
-                       M.Vd.p, Y, M.ystride, M.blk, locals);
+    const int npar = 256 / (M.ystride / 4);
+    hipLaunchKernelGGL(k_m2l_reduce, dim3(n_items), dim3(256), (size_t)(npar + 1) * M.ystride * sizeof(double), s,
+                       items, ptr, level, M.Vd.p, Y, M.ystride, M.blk, locals);
     TBEM_LAUNCH_CHECK();
 }
 

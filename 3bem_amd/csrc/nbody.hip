@@ -138,8 +138,8 @@ static void nb_forward_impl(NbodyPlan& P, const double* x, double* y, hipStream_
     f.rp_ptr = P.fwd.ptr.p; f.rp_beg = P.fwd.rbeg.p; f.rp_cnt = P.fwd.rcnt.p; f.rp_pre = P.fwd.rpre.p;
     f.ps = view(P.sp); f.pdens = P.dens.p; f.n_ps = P.n_src;
     f.out_sorted = y;  // identity permutation: write straight into y
-    if (P.precision == TBEM_MIXED) launch_leaf_stage<float>(P.kid, false, f, P.kc, s);
-    else launch_leaf_stage<double>(P.kid, false, f, P.kc, s);
+    if (P.precision == TBEM_MIXED) launch_leaf_stage<float>(P.kid, false, f, P.kc, s, false);
+    else launch_leaf_stage<double>(P.kid, false, f, P.kc, s, false);
 }
 
 static void nb_adjoint_impl(NbodyPlan& P, const double* g, double* y, hipStream_t s) {
@@ -155,8 +155,8 @@ static void nb_adjoint_impl(NbodyPlan& P, const double* g, double* y, hipStream_
     f.rp_ptr = P.adj.ptr.p; f.rp_beg = P.adj.rbeg.p; f.rp_cnt = P.adj.rcnt.p; f.rp_pre = P.adj.rpre.p;
     f.ps = view(P.op); f.pdens = g; f.n_ps = P.n_obs;
     f.out_sorted = y;
-    if (P.precision == TBEM_MIXED) launch_leaf_stage<float>(P.kid, true, f, P.kc, s);
-    else launch_leaf_stage<double>(P.kid, true, f, P.kc, s);
+    if (P.precision == TBEM_MIXED) launch_leaf_stage<float>(P.kid, true, f, P.kc, s, false);
+    else launch_leaf_stage<double>(P.kid, true, f, P.kc, s, false);
 }
 
 }  // namespace tbem

@@ -88,8 +88,11 @@ struct LeafStage {
 // (or is the transpose of such a step), whose conditioning FP32 input cannot survive.
 void launch_surf_stage(int kid, bool src_surf, bool add, bool adj, const SurfStage& a,
                        const KConst& kc, hipStream_t s);
+// fast: point-to-point pairs use the one-Newton-step FP64 rsqrt (relative error
+// <= 9e-14 per pair; the FMM's parity bar is 1e-12). The direct N-body operator
+// keeps the full one.
 template <class T>
-void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, hipStream_t s);
+void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, hipStream_t s, bool fast);
 // d/d(target coords) of the forward leaf stage contracted with grad_g
 void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s);
 

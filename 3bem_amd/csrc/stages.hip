@@ -282,7 +282,7 @@ __device__ inline void stream_surfaces(Tile<double, R, NRM, NB>& tl, int e0, int
 // ---------------------------------------------------------------------- pair loop
 // One source j of the tile on one target (FP64 coordinates tx, ty, tz): the
 // displacement is formed in FP64, then the kernel runs in T.
-template <int KID, bool ADJ, class T, class VT, int R, bool NRM, int NB>
+template <int KID, bool ADJ, bool FAST = false, class T, class VT, int R, bool NRM, int NB>
 __device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, double tx, double ty, double tz,
                                      const T* tn, T* acc, const KC<T>& kc) {
     const T sn[3] = {NRM ? T(tl.nx[j]) : T(0), NRM ? T(tl.ny[j]) : T(0), NRM ? T(tl.nz[j]) : T(0)};
@@ -291,11 +291,11 @@ __device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, doub
     if (!ADJ) {
         const T dx = T(tl.x[j] - tx), dy = T(tl.y[j] - ty), dz = T(tl.z[j] - tz);
         const T r2 = dx * dx + dy * dy + dz * dz;
-        kop<KID, false, T>(dx, dy, dz, r2, tn, sn, v, acc, kc);
+        kop<KID, false, T, FAST>(dx, dy, dz, r2, tn, sn, v, acc, kc);
     } else {
         const T dx = T(tx - tl.x[j]), dy = T(ty - tl.y[j]), dz = T(tz - tl.z[j]);
         const T r2 = dx * dx + dy * dy + dz * dz;
-        kop<KID, true, T>(dx, dy, dz, r2, sn, tn, v, acc, kc);
+        kop<KID, true, T, FAST>(dx, dy, dz, r2, sn, tn, v, acc, kc);
     }
 }
 
@@ -378,8 +378,10 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
 // -------------------------------------------------------------------- point targets
 // Two targets per lane: slot tg holds targets t0+tg and t0+nslot+tg. Point
 // sources accumulate in T (acA, acB); surface sources in FP64 (sfA, sfB, which
-// are acA, acB themselves when T is double).
-template <int KID, class T, bool ADJ>
+// are acA, acB themselves when T is double). Register budget: at 512 threads per
+// workgroup, <= 128 VGPRs keeps two workgroups per CU; one more register halves
+// the resident waves (the elastic-U FP32 instance at 139 VGPRs ran 1.5x slower).
+template <int KID, class T, bool ADJ, bool FAST>
 __global__ __launch_bounds__(kLeafBlock) void k_leaf_stage(LeafStage a, KC<T> kc, KC<double> kc64, double scale) {
     constexpr int NB = kLeafBlock;
     constexpr int R = kernel_dim(KID);
@@ -416,8 +418,8 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_stage(LeafStage a, KC<T> kc
         auto points = [&](int cnt) {
             if (va) {
                 for (int j = slice; j < cnt; j += S) {
-                    pair<KID, ADJ>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
-                    pair<KID, ADJ>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                    pair<KID, ADJ, FAST>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                    pair<KID, ADJ, FAST>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
                 }
             }
             acA.flush();
@@ -636,14 +638,20 @@ static void surf_launch(bool src_surf, bool add, bool adj, const SurfStage& a, c
 }
 
 template <int KID, class T>
-static void leaf_launch(bool adj, const LeafStage& a, const KConst& c, hipStream_t s) {
+static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c, hipStream_t s) {
     if (a.n_leaves == 0) return;
     const KC<T> kc = kc_of<T>(c);
     const KC<double> kc64 = make_kc64(c);
     const double scale = (double)kop_scale<KID, double>(kc64);
     dim3 g(a.n_leaves), b(kLeafBlock);
-    if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true>), g, b, 0, s, a, kc, kc64, scale);
-    else hipLaunchKernelGGL((k_leaf_stage<KID, T, false>), g, b, 0, s, a, kc, kc64, scale);
+    // the one-step rsqrt only differs from the full one in FP64
+    if (fast && std::is_same<T, double>::value) {
+        if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, true>), g, b, 0, s, a, kc, kc64, scale);
+        else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, true>), g, b, 0, s, a, kc, kc64, scale);
+    } else {
+        if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, false>), g, b, 0, s, a, kc, kc64, scale);
+        else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, false>), g, b, 0, s, a, kc, kc64, scale);
+    }
     TBEM_LAUNCH_CHECK();
 }
 
@@ -676,8 +684,8 @@ void launch_surf_stage(int kid, bool src_surf, bool add, bool adj, const SurfSta
 }
 
 template <class T>
-void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, hipStream_t s) {
-    TBEM_KID_SWITCH(kid, leaf_launch, adj, a, kc, s);
+void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, hipStream_t s, bool fast) {
+    TBEM_KID_SWITCH(kid, leaf_launch, adj, fast, a, kc, s);
 }
 
 void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s) {
@@ -685,7 +693,7 @@ void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t
     TBEM_KID_SWITCH(kid, grad_launch, a, kc, s);
 }
 
-template void launch_leaf_stage<double>(int, bool, const LeafStage&, const KConst&, hipStream_t);
-template void launch_leaf_stage<float>(int, bool, const LeafStage&, const KConst&, hipStream_t);
+template void launch_leaf_stage<double>(int, bool, const LeafStage&, const KConst&, hipStream_t, bool);
+template void launch_leaf_stage<float>(int, bool, const LeafStage&, const KConst&, hipStream_t, bool);
 
 }  // namespace tbem

@@ -142,9 +142,15 @@ def target_run(tbem, torch, steps):
         dt, m2l = time_steps(lambda: op.apply(x), steps, 1, torch, after=lambda: op.m2l_kernel_ms())
         ys[prec] = op.apply(x).cpu().numpy()
         info = op.info()
+        stage_ms = {k: round(v, 3) for k, v in op.stage_times().items()}
+        gv = torch.randn(3 * n, dtype=torch.float64, device="cuda")
+        dt_adj, _ = time_steps(lambda: op.apply_adjoint(gv), 2, 1, torch)
+        dt_grad, _ = time_steps(lambda: op.obs_grad(x, gv), 2, 1, torch)
         out[prec] = {"ms_per_step": round(dt * 1e3, 3), "Mpts_per_s": round(n / dt / 1e6, 3),
+                     "bwd_fwd_ratio": round((dt_adj + dt_grad) / dt, 3),
+                     "bwd_ms": {"adjoint": round(dt_adj * 1e3, 2), "obs_grad": round(dt_grad * 1e3, 2)},
                      "plan_build_s": round(plan, 2), "m2l_kernel_ms": round(float(np.mean(m2l)), 3),
-                     "stage_ms": {k: round(v, 3) for k, v in op.stage_times().items()},
+                     "stage_ms": stage_ms,
                      "rel_err_vs_direct": float(np.linalg.norm(ys[prec][idx] - yd) / np.linalg.norm(yd))}
         if prec == "mixed":
             blk = 3 * info["n_e"]
