@@ -66,6 +66,8 @@ struct Work {
     DevBuf<double> gsort, lbar, cbar, mbar, ebar, grad_sorted;
     DevBuf<float> chk_up;   // modal up-check potential per source cell (input of the MFMA M2L)
     DevBuf<float> m2l_y;    // per-pair MFMA M2L output
+    DevBuf<float> m2l_w;    // transpose: modal input per observation cell
+    DevBuf<double> e2;      // transpose: modal M2L^T contribution to ebar per source cell
     hipEvent_t ev[10];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
@@ -463,6 +465,8 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     if (W.lbar.n < nco) { W.lbar.alloc(nco); W.cbar.alloc(nco); }
     if (W.mbar.n < ncs) { W.mbar.alloc(ncs); W.ebar.alloc(ncs); }
     if (W.dens.n < (size_t)R * Ns) W.dens.alloc((size_t)R * Ns);
+    const bool mfma = std::is_same<T, float>::value && P.m2l_mfma.enabled;
+    if (mfma && W.e2.n < ncs) { W.e2.alloc(ncs); W.m2l_w.alloc(nco); }
     launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
     launch_fill(W.lbar.p, nco, 0.0, s);
     launch_fill(W.cbar.p, nco, 0.0, s);
@@ -490,13 +494,21 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         launch_surf_stage(P.kid, true, true, true, l, P.kc, s);
     }
 
-    // M2L^T and M2P^T into mbar
-    SurfStage m = surf_base(P);
-    set_list(m, P.a_m2l);
-    m.tt = view(P.dst); m.t_surf = P.d_inner.p;
-    m.st = view(P.dot); m.s_surf = P.d_inner.p; m.s_coeff = W.cbar.p;
-    m.c2e = nullptr; m.out = W.mbar.p;
-    launch_surf_stage(P.kid, true, true, true, m, P.kc, s);
+    // M2L^T into mbar (FP64), or, on MFMA, e2 = Bu^T M_c^T Vd^T lbar, added to ebar below
+    if (mfma) {
+        launch_m2l_modal_in(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.dot.level.p, W.lbar.p, W.m2l_w.p, s);
+        launch_m2l_mfma(P.m2l_mfma, nullptr, W.m2l_w.p, W.m2l_y.p, s, true);
+        launch_fill(W.e2.p, ncs, 0.0, s);
+        launch_m2l_reduce_adj(P.m2l_mfma, P.dst.level.p, W.m2l_y.p, W.e2.p, s);
+    } else {
+        SurfStage m = surf_base(P);
+        set_list(m, P.a_m2l);
+        m.tt = view(P.dst); m.t_surf = P.d_inner.p;
+        m.st = view(P.dot); m.s_surf = P.d_inner.p; m.s_coeff = W.cbar.p;
+        m.c2e = nullptr; m.out = W.mbar.p;
+        launch_surf_stage(P.kid, true, true, true, m, P.kc, s);
+    }
+    // M2P^T into mbar
     SurfStage mp = surf_base(P);
     set_list(mp, P.a_m2p);
     mp.tt = view(P.dst); mp.t_surf = P.d_inner.p;
@@ -507,7 +519,8 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     // M2M^T, root first: ebar = UC2E^T mbar, then push to children
     for (int L = 0; L < P.st.n_levels; L++) {
         const int c0 = P.st.level_begin[L], nc = P.st.level_begin[L + 1] - c0;
-        launch_c2e_transpose_apply(P.iota.p + c0, nc, P.dst.level.p, P.uc2e.p, P.blk, W.mbar.p, W.ebar.p, s);
+        launch_c2e_transpose_apply(P.iota.p + c0, nc, P.dst.level.p, P.uc2e.p, P.blk, W.mbar.p, W.ebar.p, s,
+                                   mfma ? W.e2.p : nullptr);
         if (L + 1 >= P.st.n_levels) break;
         SurfStage c = surf_base(P);
         set_list(c, P.a_m2m[L + 1]);

@@ -21,9 +21,12 @@ struct M2lMfma {
     bool enabled = false;
     int n_classes = 0, blk = 0, mt = 0, kt = 0, ystride = 0;
     int n_units = 0, n_pairs = 0;
-    DevBuf<float> S;
+    DevBuf<float> S, ST;    // M_c and M_c^T, A-operand order
     DevBuf<double> Bu, Vd;  // per source / observation level, blk x blk row-major
     DevBuf<int> unit_class, unit_off, unit_cnt, colpair;
+    DevBuf<int> pair_tgt;   // target cell of every pair
+    int n_adj = 0;          // transpose: source cells with pairs, and their pairs (CSR)
+    DevBuf<int> adj_items, adj_ptr, adj_pairs;
     double plan_seconds = 0;
 };
 
@@ -37,7 +40,16 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
 
 // Y[p] (ystride floats per pair) = M_{class(p)} * chk_up[src(p)] for every pair,
 // chk_up = Bu * up-check potential per source cell, rounded to FP32.
-void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s);
+// transpose: Y[p] = M_{class(p)}^T * chk_up[tgt(p)] (pair_src is then unused).
+void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s,
+                     bool transpose = false);
+
+// Transpose, input: w[t] = float(Vd[level t]^T lbar[t]) for every M2L target t.
+void launch_m2l_modal_in(const M2lMfma& M, int n_items, const int* items, const int* level, const double* lbar,
+                         float* w, hipStream_t s);
+
+// Transpose, output: e2[s] = Bu[level s]^T * (sum over the pairs p of source s of Y[p]), FP64.
+void launch_m2l_reduce_adj(const M2lMfma& M, const int* level, const float* Y, double* e2, hipStream_t s);
 
 // locals[t] += Vd[level t] * (sum over the pairs p of target t of Y[p]), in FP64.
 void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const int* level,

@@ -109,14 +109,17 @@ __global__ __launch_bounds__(256) void k_gemm64(int n, const double* __restrict_
         }
 }
 
-// FP64 row-major S_c -> FP32 A-operand order (zero padded to mt*16 x kt*4).
-__global__ void k_m2l_swizzle(int n, int mt, int kt, const double* __restrict__ S64, float* __restrict__ S32) {
+// FP64 row-major S_c (or its transpose) -> FP32 A-operand order (zero padded to
+// mt*16 x kt*4).
+__global__ void k_m2l_swizzle(int n, int mt, int kt, int trans, const double* __restrict__ S64,
+                              float* __restrict__ S32) {
     const int c = blockIdx.y;
     const size_t per = (size_t)mt * kt * 64;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x) {
         const int rt = (int)(e / ((size_t)kt * 64)), ks = (int)((e / 64) % kt), l = (int)(e % 64);
         const int row = rt * 16 + l % 16, col = ks * 4 + l / 16;
-        S32[c * per + e] = (row < n && col < n) ? (float)S64[(size_t)c * n * n + (size_t)row * n + col] : 0.f;
+        const size_t src = trans ? (size_t)col * n + row : (size_t)row * n + col;
+        S32[c * per + e] = (row < n && col < n) ? (float)S64[(size_t)c * n * n + src] : 0.f;
     }
 }
 
@@ -225,6 +228,63 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(const int* __restrict__ item
         const double* v = V + (size_t)row * blk;
         for (int col = 0; col < blk; col++) s += v[col] * z[col];
         locals[(size_t)t * blk + row] += s;
+    }
+}
+
+// Transpose input: w[t] = float(Vd[level t]^T lbar[t]) for the M2L targets.
+__global__ __launch_bounds__(256) void k_m2l_modal_in(const int* __restrict__ items, const int* __restrict__ level,
+                                                      const double* __restrict__ Vd, int blk,
+                                                      const double* __restrict__ lbar, float* __restrict__ w) {
+    extern __shared__ double lb[];  // blk
+    const int t = uni(items[blockIdx.x]);
+    for (int i = threadIdx.x; i < blk; i += blockDim.x) lb[i] = lbar[(size_t)t * blk + i];
+    __syncthreads();
+    const double* V = Vd + (size_t)level[t] * blk * blk;
+    for (int row = threadIdx.x; row < blk; row += blockDim.x) {  // (Vd^T lb)[row] = sum_col Vd[col][row] lb[col]
+        double s = 0;
+        for (int col = 0; col < blk; col++) s += V[(size_t)col * blk + row] * lb[col];
+        w[(size_t)t * blk + row] = (float)s;
+    }
+}
+
+// Transpose output: e2[s] = Bu[level s]^T sum_{p in pairs(s)} Y[p], sums in FP64.
+// Each pair's block is contiguous in Y; threads read it as float4 columns, npar
+// pairs in parallel, as k_m2l_reduce.
+__global__ __launch_bounds__(256) void k_m2l_reduce_adj(const int* __restrict__ items, const int* __restrict__ ptr,
+                                                        const int* __restrict__ pairs, const int* __restrict__ level,
+                                                        const double* __restrict__ Bu, const float* __restrict__ Y,
+                                                        int ystride, int blk, double* __restrict__ e2) {
+    extern __shared__ double part[];  // npar * ystride partial sums, then z (ystride)
+    const int k = blockIdx.x;
+    const int sc = uni(items[k]), j0 = uni(ptr[k]), j1 = uni(ptr[k + 1]);
+    const int q = ystride / 4, npar = 256 / q;
+    const int c = threadIdx.x % q, ph = threadIdx.x / q;
+    if (ph < npar) {
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        const f32x4* y4 = (const f32x4*)Y;
+        for (int j = j0 + ph; j < j1; j += npar) {
+            const f32x4 v = y4[(size_t)pairs[j] * q + c];
+            s0 += (double)v[0];
+            s1 += (double)v[1];
+            s2 += (double)v[2];
+            s3 += (double)v[3];
+        }
+        double* o = part + (size_t)ph * ystride + 4 * c;
+        o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3;
+    }
+    __syncthreads();
+    double* z = part + (size_t)npar * ystride;
+    for (int row = threadIdx.x; row < blk; row += blockDim.x) {
+        double s = 0;
+        for (int h = 0; h < npar; h++) s += part[(size_t)h * ystride + row];
+        z[row] = s;
+    }
+    __syncthreads();
+    const double* B = Bu + (size_t)level[sc] * blk * blk;
+    for (int row = threadIdx.x; row < blk; row += blockDim.x) {  // (Bu^T z)[row] = sum_col Bu[col][row] z[col]
+        double s = 0;
+        for (int col = 0; col < blk; col++) s += B[(size_t)col * blk + row] * z[col];
+        e2[(size_t)sc * blk + row] = s;
     }
 }
 
@@ -351,6 +411,7 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
 
     // operators M_c = Ld * K_c * Ru, in chunks of classes (FP64 temporaries <= ~0.5 GB each)
     M.S.alloc((size_t)nc * per);
+    M.ST.alloc((size_t)nc * per);
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nc, (size_t)kMaxChunk, (size_t)(5e8 / (nn * 8))}));
     DevBuf<double> K((size_t)chunk * nn), T1((size_t)chunk * nn), S64((size_t)chunk * nn);
     DevBuf<double> d_tc, d_sc, d_tr, d_sr, d_inner, d_unit;
@@ -384,8 +445,10 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
         TBEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, Ld.p, d_lt.p, T1.p, d_iota.p, S64.p);
         TBEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_m2l_swizzle, dim3(std::max(1, std::min(256, div_up(per, 256))), m), dim3(256), 0, s,
-                           n, mt, kt, S64.p, M.S.p + (size_t)c0 * per);
+        const dim3 gs(std::max(1, std::min(256, div_up(per, 256))), m);
+        hipLaunchKernelGGL(k_m2l_swizzle, gs, dim3(256), 0, s, n, mt, kt, 0, S64.p, M.S.p + (size_t)c0 * per);
+        TBEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_m2l_swizzle, gs, dim3(256), 0, s, n, mt, kt, 1, S64.p, M.ST.p + (size_t)c0 * per);
         TBEM_LAUNCH_CHECK();
         TBEM_HIP(hipStreamSynchronize(s));  // the host vectors above are reused next chunk
     }
@@ -426,16 +489,57 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.unit_off.upload(uoff, s);
     M.unit_cnt.upload(ucnt, s);
     M.colpair.upload(colpair, s);
+
+    // transpose support: target of every pair; per source cell, its pairs (a
+    // counting sort by source keeps each source's pairs contiguous, in cell order)
+    std::vector<int> pair_tgt(n_pairs), first(st.n_cells + 1, 0);
+    for (size_t k = 0; k + 1 < m2l.ptr.size(); k++)
+        for (int p = m2l.ptr[k]; p < m2l.ptr[k + 1]; p++) {
+            pair_tgt[p] = m2l.items[k];
+            first[m2l.cell[p] + 1]++;
+        }
+    for (int c = 0; c < st.n_cells; c++) first[c + 1] += first[c];
+    std::vector<int> adj_pairs(n_pairs), next(first.begin(), first.end() - 1);
+    for (int p = 0; p < n_pairs; p++) adj_pairs[next[m2l.cell[p]]++] = p;
+    std::vector<int> adj_items, adj_ptr(1, 0);
+    for (int c = 0; c < st.n_cells; c++)
+        if (first[c + 1] > first[c]) {
+            adj_items.push_back(c);
+            adj_ptr.push_back(first[c + 1]);
+        }
+    M.n_adj = (int)adj_items.size();
+    M.pair_tgt.upload(pair_tgt, s);
+    M.adj_items.upload(adj_items, s);
+    M.adj_ptr.upload(adj_ptr, s);
+    M.adj_pairs.upload(adj_pairs, s);
     TBEM_HIP(hipStreamSynchronize(s));
     M.enabled = true;
     M.plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s) {
+void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s,
+                     bool transpose) {
     if (!M.enabled || M.n_units == 0) return;
     const size_t shm = (size_t)M.kt * 4 * kUnitCols * sizeof(float);
-    hipLaunchKernelGGL(k_m2l_mfma, dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride, M.S.p,
-                       M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, pair_src, chk_up, Y);
+    hipLaunchKernelGGL(k_m2l_mfma, dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride,
+                       transpose ? M.ST.p : M.S.p, M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p,
+                       transpose ? M.pair_tgt.p : pair_src, chk_up, Y);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_m2l_modal_in(const M2lMfma& M, int n_items, const int* items, const int* level, const double* lbar,
+                         float* w, hipStream_t s) {
+    if (!M.enabled || n_items == 0) return;
+    hipLaunchKernelGGL(k_m2l_modal_in, dim3(n_items), dim3(256), M.blk * sizeof(double), s, items, level, M.Vd.p,
+                       M.blk, lbar, w);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_m2l_reduce_adj(const M2lMfma& M, const int* level, const float* Y, double* e2, hipStream_t s) {
+    if (!M.enabled || M.n_adj == 0) return;
+    const int npar = 256 / (M.ystride / 4);
+    hipLaunchKernelGGL(k_m2l_reduce_adj, dim3(M.n_adj), dim3(256), (size_t)(npar + 1) * M.ystride * sizeof(double),
+                       s, M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, level, M.Bu.p, Y, M.ystride, M.blk, e2);
     TBEM_LAUNCH_CHECK();
 }
 

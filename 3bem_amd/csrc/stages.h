@@ -96,9 +96,11 @@ void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, 
 // d/d(target coords) of the forward leaf stage contracted with grad_g
 void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s);
 
-// out[cell] = C2E[level(cell)]^T in[cell] for the listed cells (adjoint of the C2E step)
+// out[cell] = C2E[level(cell)]^T in[cell] (+ add[cell]) for the listed cells (adjoint
+// of the C2E step)
 void launch_c2e_transpose_apply(const int* cells, int n_cells, const int* level, const double* c2e,
-                                int blk, const double* in, double* out, hipStream_t s);
+                                int blk, const double* in, double* out, hipStream_t s,
+                                const double* add = nullptr);
 
 void launch_gather_dens(const double* x, const int* perm, const double* w, int n, int R,
                         double* dens, hipStream_t s);

@@ -551,14 +551,14 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
 // ------------------------------------------------------------------------ helpers
 __global__ void k_c2e_transpose_apply(const int* __restrict__ cells, const int* __restrict__ level,
                                       const double* __restrict__ c2e, int blk, const double* __restrict__ in,
-                                      double* __restrict__ out) {
+                                      const double* __restrict__ add, double* __restrict__ out) {
     const int cell = cells[blockIdx.x];
     const double* M = c2e + (size_t)level[cell] * blk * blk;
     const double* x = in + (size_t)cell * blk;
     for (int col = threadIdx.x; col < blk; col += blockDim.x) {
         double s = 0;
         for (int row = 0; row < blk; row++) s += M[(size_t)row * blk + col] * x[row];
-        out[(size_t)cell * blk + col] = s;
+        out[(size_t)cell * blk + col] = add ? s + add[(size_t)cell * blk + col] : s;
     }
 }
 
@@ -585,10 +585,10 @@ __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
 }
 
 void launch_c2e_transpose_apply(const int* cells, int n_cells, const int* level, const double* c2e,
-                                int blk, const double* in, double* out, hipStream_t s) {
+                                int blk, const double* in, double* out, hipStream_t s, const double* add) {
     if (n_cells == 0) return;
     hipLaunchKernelGGL(k_c2e_transpose_apply, dim3(n_cells), dim3(kSurfBlock), 0, s, cells, level, c2e, blk, in,
-                       out);
+                       add, out);
     TBEM_LAUNCH_CHECK();
 }
 
