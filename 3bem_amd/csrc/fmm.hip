@@ -426,7 +426,8 @@ __global__ void k_scatter_aos3(const double* __restrict__ in, const int* __restr
 }
 
 // grad[3*i+m] = sum_q g_q(i) d u_q(i) / d obs_i[m], plan geometry held fixed. The
-// gradient leaf stage is FP64; the far field is the operator's own.
+// far field is the operator's own; in mixed precision the gradient leaf stage runs
+// its point sources in FP32.
 static void obs_grad(const FmmPlan& P, Work& W, const double* x, const double* g, double* grad,
                      hipStream_t s) {
     const int R = P.R;
@@ -444,7 +445,7 @@ static void obs_grad(const FmmPlan& P, Work& W, const double* x, const double* g
     LeafStage f = leaf_stage_args(P, W);
     f.grad_g = W.gsort.p;
     f.grad_out = W.grad_sorted.p;
-    launch_leaf_grad(P.kid, f, P.kc, s);
+    launch_leaf_grad(P.kid, f, P.kc, s, P.precision == TBEM_MIXED);
     hipLaunchKernelGGL(k_scatter_aos3, dim3(div_up(No, 256)), dim3(256), 0, s, W.grad_sorted.p, P.op.perm.p,
                        No, grad);
     TBEM_LAUNCH_CHECK();

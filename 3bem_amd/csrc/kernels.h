@@ -341,43 +341,43 @@ __host__ __device__ constexpr bool has_analytic_grad(int kid) {
     return kid == K_LS || kid == K_LD || kid == K_EU || kid == K_ET;
 }
 
-template <int KID>
-__device__ __forceinline__ void grad_pair(double dx, double dy, double dz, double r2, const double* ns,
-                                          const double* v, const double* g, double* G, const KC<double>& k) {
-    double ri = rsqrt_t(r2);
-    ri = (r2 < 1e-12) ? 0.0 : ri;
-    const double ri2 = ri * ri, ri3 = ri2 * ri;
-    const double d[3] = {dx, dy, dz};
+template <int KID, class T>
+__device__ __forceinline__ void grad_pair(T dx, T dy, T dz, T r2, const T* ns, const T* v, const T* g, T* G,
+                                          const KC<T>& k) {
+    T ri = rsqrt_t(r2);
+    ri = (r2 < cutoff_r2<T>()) ? T(0) : ri;
+    const T ri2 = ri * ri, ri3 = ri2 * ri;
+    const T d[3] = {dx, dy, dz};
     if (KID == K_LS) {
         // u = v / r
-        const double s = g[0] * v[0] * ri3;
+        const T s = g[0] * v[0] * ri3;
         for (int m = 0; m < 3; m++) G[m] += s * d[m];
     } else if (KID == K_LD) {
         // u = (n.d) v / r^3
-        const double b = ns[0] * dx + ns[1] * dy + ns[2] * dz;
-        const double s = g[0] * v[0] * ri3;
-        for (int m = 0; m < 3; m++) G[m] += s * (3.0 * b * d[m] * ri2 - ns[m]);
+        const T b = ns[0] * dx + ns[1] * dy + ns[2] * dz;
+        const T s = g[0] * v[0] * ri3;
+        for (int m = 0; m < 3; m++) G[m] += s * (T(3) * b * d[m] * ri2 - ns[m]);
     } else if (KID == K_EU) {
         // u_k = C2 v_k / r + d_k (d.v) / r^3
-        const double a = dx * v[0] + dy * v[1] + dz * v[2];
-        const double gd = g[0] * dx + g[1] * dy + g[2] * dz;
-        const double gv = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
-        const double c1 = k.u2 * gv + 3.0 * gd * a * ri2;
+        const T a = dx * v[0] + dy * v[1] + dz * v[2];
+        const T gd = g[0] * dx + g[1] * dy + g[2] * dz;
+        const T gv = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
+        const T c1 = k.u2 * gv + T(3) * gd * a * ri2;
         for (int m = 0; m < 3; m++) G[m] += ri3 * (c1 * d[m] - g[m] * a - gd * v[m]);
     } else if (KID == K_ET) {
         // u_k = -[C2 v_k b / r^3 + 3 d_k a b / r^5 - C2 d_k c / r^3 + C2 n_k a / r^3]
-        const double a = dx * v[0] + dy * v[1] + dz * v[2];
-        const double b = dx * ns[0] + dy * ns[1] + dz * ns[2];
-        const double c = ns[0] * v[0] + ns[1] * v[1] + ns[2] * v[2];
-        const double gv = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
-        const double gn = g[0] * ns[0] + g[1] * ns[1] + g[2] * ns[2];
-        const double gd = g[0] * dx + g[1] * dy + g[2] * dz;
-        const double ri5 = ri3 * ri2, ri7 = ri5 * ri2;
-        const double cd = 3.0 * k.t2 * gv * b * ri5 + 15.0 * gd * a * b * ri7 - 3.0 * k.t2 * c * gd * ri5 +
-                          3.0 * k.t2 * gn * a * ri5;
-        const double cn = -k.t2 * gv * ri3 - 3.0 * gd * a * ri5;
-        const double cg = -3.0 * a * b * ri5 + k.t2 * c * ri3;
-        const double cv = -3.0 * gd * b * ri5 - k.t2 * gn * ri3;
+        const T a = dx * v[0] + dy * v[1] + dz * v[2];
+        const T b = dx * ns[0] + dy * ns[1] + dz * ns[2];
+        const T c = ns[0] * v[0] + ns[1] * v[1] + ns[2] * v[2];
+        const T gv = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
+        const T gn = g[0] * ns[0] + g[1] * ns[1] + g[2] * ns[2];
+        const T gd = g[0] * dx + g[1] * dy + g[2] * dz;
+        const T ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+        const T cd = T(3) * k.t2 * gv * b * ri5 + T(15) * gd * a * b * ri7 - T(3) * k.t2 * c * gd * ri5 +
+                     T(3) * k.t2 * gn * a * ri5;
+        const T cn = -k.t2 * gv * ri3 - T(3) * gd * a * ri5;
+        const T cg = T(-3) * a * b * ri5 + k.t2 * c * ri3;
+        const T cv = T(-3) * gd * b * ri5 - k.t2 * gn * ri3;
         for (int m = 0; m < 3; m++) G[m] -= cd * d[m] + cn * ns[m] + cg * g[m] + cv * v[m];
     }
 }

@@ -93,8 +93,9 @@ void launch_surf_stage(int kid, bool src_surf, bool add, bool adj, const SurfSta
 // keeps the full one.
 template <class T>
 void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, hipStream_t s, bool fast);
-// d/d(target coords) of the forward leaf stage contracted with grad_g
-void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s);
+// d/d(target coords) of the forward leaf stage contracted with grad_g; fp32_points:
+// point sources in FP32 (mixed precision; kernels with closed-form gradients)
+void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s, bool fp32_points);
 
 // out[cell] = C2E[level(cell)]^T in[cell] (+ add[cell]) for the listed cells (adjoint
 // of the C2E step)

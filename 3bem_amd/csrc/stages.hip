@@ -474,13 +474,17 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_stage(LeafStage a, KC<T> kc
 // ------------------------------------------------- observation-coordinate gradient
 // Forward leaf stage (P2P + M2P + L2P) differentiated with respect to the target
 // coordinates; writes grad[3*i + m] = sum_q g_q(i) d u_q(i) / d x_i[m]. Closed
-// forms for the kernels that have them (grad_pair), dual numbers otherwise.
-template <int KID>
-__global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, double scale) {
+// forms for the kernels that have them (grad_pair), dual numbers otherwise. With
+// T = float (mixed precision, closed-form kernels only) the point sources run in
+// FP32 from FP64 displacements with Neumaier-compensated per-tile sums; surface
+// sources stay FP64.
+template <int KID, class T>
+__global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, KC<T> kct, double scale) {
     constexpr int NB = kGradBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
-    __shared__ StageSmem<double, R, NRM, NB, 3> sm;
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
+    __shared__ StageSmem<T, R, NRM, NB, 3> sm;
     const int pos = blockIdx.x;
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
@@ -497,35 +501,62 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
         const D3 tx(a.tp.x[i], 1, 0, 0), ty(a.tp.y[i], 0, 1, 0), tz(a.tp.z[i], 0, 0, 1);
         const D3 tn[3] = {D3(NRM ? a.tp.nx[i] : 0.0), D3(NRM ? a.tp.ny[i] : 0.0), D3(NRM ? a.tp.nz[i] : 0.0)};
         double gt[R], G[3] = {0, 0, 0};
-        for (int q = 0; q < R; q++) gt[q] = a.grad_g[(size_t)q * a.n_tgt + i];
+        T gtT[R];
+        for (int q = 0; q < R; q++) {
+            gt[q] = a.grad_g[(size_t)q * a.n_tgt + i];
+            gtT[q] = T(gt[q]);
+        }
         drain_loads();
         D3 acc[R];
-        auto compute = [&](int cnt) {
-            if (!active) return;
-            const Tile<double, R, NRM, NB>& tl = sm.st;
-            for (int j = slice; j < cnt; j += S) {
-                if (has_analytic_grad(KID)) {
-                    const double sn[3] = {NRM ? tl.nx[j] : 0.0, NRM ? tl.ny[j] : 0.0, NRM ? tl.nz[j] : 0.0};
-                    double v[R];
-                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
-                    const double dx = tl.x[j] - tx.v, dy = tl.y[j] - ty.v, dz = tl.z[j] - tz.v;
-                    grad_pair<KID>(dx, dy, dz, dx * dx + dy * dy + dz * dz, sn, v, gt, G, kc64);
-                } else {
-                    const D3 sn[3] = {D3(NRM ? tl.nx[j] : 0.0), D3(NRM ? tl.ny[j] : 0.0), D3(NRM ? tl.nz[j] : 0.0)};
-                    D3 v[R];
-                    for (int q = 0; q < R; q++) v[q] = D3(tl.d[q][j]);
-                    const D3 dx = D3(tl.x[j]) - tx, dy = D3(tl.y[j]) - ty, dz = D3(tl.z[j]) - tz;
-                    const D3 r2 = dx * dx + dy * dy + dz * dz;
-                    kop<KID, false, D3>(dx, dy, dz, r2, tn, sn, v, acc, kc);
-                }
+        Acc<T, 3> GP;
+        GP.init();
+        // one FP64 source j of tile tl (analytic or dual numbers)
+        auto pair64 = [&](const auto& tl, int j) {
+            if (has_analytic_grad(KID)) {
+                const double sn[3] = {NRM ? (double)tl.nx[j] : 0.0, NRM ? (double)tl.ny[j] : 0.0,
+                                      NRM ? (double)tl.nz[j] : 0.0};
+                double v[R];
+                for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                const double dx = tl.x[j] - tx.v, dy = tl.y[j] - ty.v, dz = tl.z[j] - tz.v;
+                grad_pair<KID, double>(dx, dy, dz, dx * dx + dy * dy + dz * dz, sn, v, gt, G, kc64);
+            } else {
+                const D3 sn[3] = {D3(NRM ? (double)tl.nx[j] : 0.0), D3(NRM ? (double)tl.ny[j] : 0.0),
+                                  D3(NRM ? (double)tl.nz[j] : 0.0)};
+                D3 v[R];
+                for (int q = 0; q < R; q++) v[q] = D3((double)tl.d[q][j]);
+                const D3 dx = D3(tl.x[j]) - tx, dy = D3(tl.y[j]) - ty, dz = D3(tl.z[j]) - tz;
+                const D3 r2 = dx * dx + dy * dy + dz * dz;
+                kop<KID, false, D3>(dx, dy, dz, r2, tn, sn, v, acc, kc);
             }
         };
-        stream_ranges_packed(sm.st, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
-                             a.n_ps, compute);
-        stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, compute);
+        auto points = [&](int cnt) {
+            if (!active) return;
+            const Tile<T, R, NRM, NB>& tl = sm.pt;
+            for (int j = slice; j < cnt; j += S) {
+                if constexpr (SPLIT) {
+                    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
+                    T v[R];
+                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                    const T dx = T(tl.x[j] - tx.v), dy = T(tl.y[j] - ty.v), dz = T(tl.z[j] - tz.v);
+                    grad_pair<KID, T>(dx, dy, dz, dx * dx + dy * dy + dz * dz, sn, v, gtT, GP.tile(), kct);
+                } else {
+                    pair64(tl, j);
+                }
+            }
+            GP.flush();
+        };
+        auto surfaces = [&](int cnt) {
+            if (!active) return;
+            for (int j = slice; j < cnt; j += S) pair64(sm.st, j);
+        };
+        stream_ranges_packed(sm.pt, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
+                             a.n_ps, points);
+        stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, surfaces);
         if (a.self_coeff)
-            stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, compute);
+            stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, surfaces);
         double gr[3] = {G[0], G[1], G[2]};
+        if (SPLIT)
+            for (int mm = 0; mm < 3; mm++) gr[mm] += GP.value(mm);
         if (active && !has_analytic_grad(KID)) {
             for (int q = 0; q < R; q++) {
                 gr[0] += gt[q] * acc[q].d0;
@@ -656,10 +687,15 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
 }
 
 template <int KID, class T>
-static void grad_launch(const LeafStage& a, const KConst& c, hipStream_t s) {
+static void grad_launch(bool fp32_points, const LeafStage& a, const KConst& c, hipStream_t s) {
     if (a.n_leaves == 0) return;
     const double scale = (double)kop_scale<KID, double>(make_kc64(c));
-    hipLaunchKernelGGL((k_leaf_grad<KID>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c), scale);
+    if (fp32_points && has_analytic_grad(KID))
+        hipLaunchKernelGGL((k_leaf_grad<KID, float>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c),
+                           make_kc32(c), scale);
+    else
+        hipLaunchKernelGGL((k_leaf_grad<KID, double>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c),
+                           make_kc64(c), scale);
     TBEM_LAUNCH_CHECK();
 }
 
@@ -688,9 +724,9 @@ void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, 
     TBEM_KID_SWITCH(kid, leaf_launch, adj, fast, a, kc, s);
 }
 
-void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s) {
+void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s, bool fp32_points) {
     using T = double;
-    TBEM_KID_SWITCH(kid, grad_launch, a, kc, s);
+    TBEM_KID_SWITCH(kid, grad_launch, fp32_points, a, kc, s);
 }
 
 template void launch_leaf_stage<double>(int, bool, const LeafStage&, const KConst&, hipStream_t, bool);

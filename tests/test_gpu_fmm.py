@@ -215,6 +215,26 @@ def test_gpu_mixed_adjoint_with_mfma_transpose(tbem, wl, kid, order):
     assert abs(lhs - rhs) < 1e-6 * abs(lhs), (lhs, rhs)
 
 
+@pytest.mark.parametrize("kid", [0, 3, 4, 6])
+def test_gpu_mixed_obs_grad_matches_fp64(tbem, wl, kid):
+    """Mixed precision observation gradient (MFMA far field; FP32 point sources for
+    the kernels with closed-form gradients, FP64 dual numbers otherwise) against the
+    FP64 one, which the finite-difference tests pin."""
+    g = wl.cfg2_fault_and_surface(20000)
+    K = tbem.make_kernel(kid, 1.0, 0.25)
+    R = components(kid)
+    data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
+    cfg = tbem.FMMConfig(0.3, 50, 100, 0.05, False)
+    rng = np.random.default_rng(12)
+    x, gg = rng.standard_normal(R * 20000), rng.standard_normal(R * 20000)
+    case("mixed obs grad", kid)
+    G32 = tbem.FMMOperator(K, data, cfg, precision="mixed").obs_grad(x, gg)
+    G64 = tbem.FMMOperator(K, data, cfg).obs_grad(x, gg)
+    err = rel_err(G32, G64)
+    print(f"  mixed obs gradient vs FP64 {err:.3e}")
+    assert err < 1e-5
+
+
 # ----------------------------------------------------------------- obs gradient
 @pytest.mark.parametrize("kid", [0, 3, 4, 6])
 def test_gpu_obs_grad_matches_finite_differences(tbem, kid):
