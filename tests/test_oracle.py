@@ -104,7 +104,8 @@ def test_surface_closed_form_order5(orc):
 
 
 def test_surface_counts_and_unit_norm(orc):
-    # measured point counts from the survey (fact 2)
+    # point counts of the reference's sphere construction (cpp/fmm.cpp:22-45), as the
+    # compiled reference produces them: n_e is close to, not equal to, the order
     for order, n in ((5, 6), (6, 6), (10, 11), (20, 20), (35, 35), (50, 50), (75, 74), (100, 99), (150, 151)):
         s = orc.surface(order)
         assert s.shape[0] == n
