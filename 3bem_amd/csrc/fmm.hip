@@ -43,7 +43,6 @@ struct FmmPlan {
     DevBuf<double> uc2e, dc2e;
     std::vector<double> h_uc2e, h_dc2e;
     TaskLists tl;
-    DevBuf<int> iota;            // 0..max(cells)-1: per-level cell ranges as pointers into it
     // forward lists
     DevList p2m, m2l, p2l, leaf_p2p, leaf_m2p;
     std::vector<DevList> m2m;    // by source level: parents at L, entries = children
@@ -56,6 +55,10 @@ struct FmmPlan {
     std::vector<DevList> a_m2m;  // by src level L >= 1: cells at L, entry = parent
     DevList a_leaf_p2p, a_leaf_p2l;  // indexed by source leaf position
     M2lMfma m2l_mfma;            // mixed precision: M2L as grouped FP32 MFMA GEMMs
+    // GEMM tiles of the C2E products: the cells each stage writes
+    CellTiles t_p2m, t_m2l, t_p2l;
+    std::vector<CellTiles> t_m2m;          // by source level: the M2M parents
+    std::vector<CellTiles> t_olev, t_slev; // every cell of each observation / source level
     double p2p_pairs = 0;
     double plan_seconds = 0;
 };
@@ -68,6 +71,7 @@ struct Work {
     DevBuf<float> m2l_y;    // per-pair MFMA M2L output
     DevBuf<float> m2l_w;    // transpose: modal input per observation cell
     DevBuf<double> e2;      // transpose: modal M2L^T contribution to ebar per source cell
+    DevBuf<double> fld;     // raw check potentials of one stage, blk per cell (input of the C2E GEMMs)
     hipEvent_t ev[10];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
@@ -80,6 +84,7 @@ struct Work {
         mult.alloc((size_t)P.dst.n_cells * P.blk);
         locals.alloc((size_t)P.dot.n_cells * P.blk);
         out_sorted.alloc(P.R * P.n_obs);
+        fld.alloc((size_t)std::max(P.dst.n_cells, P.dot.n_cells) * P.blk);
         if (P.m2l_mfma.enabled) {
             chk_up.alloc((size_t)P.dst.n_cells * P.blk);
             chk_up.zero(stream);  // cells without a P2M/M2M stage keep a zero potential
@@ -257,18 +262,28 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
 
     const HostTree& ST = P->st;
     const HostTree& OT = P->ot;
-    std::vector<int> iota(std::max(ST.n_cells, OT.n_cells) + 1);
-    for (size_t i = 0; i < iota.size(); i++) iota[i] = (int)i;
-    P->iota.upload(iota, s);
 
     P->p2m.upload(self_list(ST, ST.leaves), s);
+    P->t_p2m = make_cell_tiles(ST.leaves, ST, s);
     P->m2m.resize(ST.n_levels);
-    for (int L = 0; L + 1 < ST.n_levels; L++) P->m2m[L].upload(children_list(ST, L), s);
+    P->t_m2m.resize(ST.n_levels);
+    for (int L = 0; L + 1 < ST.n_levels; L++) {
+        const HostList l = children_list(ST, L);
+        P->m2m[L].upload(l, s);
+        P->t_m2m[L] = make_cell_tiles(l.items, ST, s);
+    }
     const HostList m2l_h = list_from_pairs(cells_with(tl.m2l, 0, OT.n_cells), OT.n_cells, tl.m2l, 0, nullptr);
     P->m2l.upload(m2l_h, s);
+    P->t_m2l = make_cell_tiles(m2l_h.items, OT, s);
     if (P->precision == TBEM_MIXED)
         build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, same, m2l_h, P->surf, P->uc2e.p, P->dc2e.p, s);
-    P->p2l.upload(list_from_pairs(cells_with(tl.p2l, 0, OT.n_cells), OT.n_cells, tl.p2l, 0, &ST), s);
+    const std::vector<int> p2l_items = cells_with(tl.p2l, 0, OT.n_cells);
+    P->p2l.upload(list_from_pairs(p2l_items, OT.n_cells, tl.p2l, 0, &ST), s);
+    P->t_p2l = make_cell_tiles(p2l_items, OT, s);
+    P->t_olev.resize(OT.n_levels);
+    for (int L = 0; L < OT.n_levels; L++) P->t_olev[L] = make_level_tiles(OT, L, s);
+    P->t_slev.resize(ST.n_levels);
+    for (int L = 0; L < ST.n_levels; L++) P->t_slev[L] = make_level_tiles(ST, L, s);
     P->l2l.resize(OT.n_levels);
     for (int L = 1; L < OT.n_levels; L++) P->l2l[L].upload(parent_list(OT, L), s);
     P->obs_leaves.upload(OT.leaves, s);
@@ -320,15 +335,25 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
     launch_gather_dens(x, P.sp.perm.p, P.sp.w.p, Ns, R, W.dens.p, s);
     TBEM_HIP(hipEventRecord(W.ev[1], s));
 
+    // the up-check potentials of a stage's cells -> equivalent densities, and, for
+    // the MFMA M2L, the modal FP32 input Bu * check
+    auto up_c2e = [&](const CellTiles& T) {
+        launch_level_gemm(T, P.blk, P.uc2e.p, false, W.fld.p, nullptr, W.mult.p, nullptr, false, s);
+        if (mfma) launch_level_gemm(T, P.blk, P.m2l_mfma.Bu.p, false, W.fld.p, nullptr, nullptr, W.chk_up.p, false, s);
+    };
+    // down-check potentials -> added to the down-equivalent densities
+    auto down_c2e = [&](const CellTiles& T) {
+        launch_level_gemm(T, P.blk, P.dc2e.p, false, W.fld.p, nullptr, W.locals.p, nullptr, true, s);
+    };
+
     // P2M (cpp/fmm.cpp:162-202): points of each source leaf -> up-check -> UC2E
     SurfStage a = surf_base(P);
     set_list(a, P.p2m);
     a.tt = view(P.dst); a.t_surf = P.d_outer.p;
     a.st = view(P.dst); a.sp = view(P.sp); a.dens = W.dens.p; a.n_src_pts = Ns;
-    a.c2e = P.uc2e.p; a.out = W.mult.p;
-    a.chk_out = mfma ? W.chk_up.p : nullptr;
-    a.chk_basis = P.m2l_mfma.Bu.p;
+    a.out = W.fld.p;
     launch_surf_stage(P.kid, false, false, false, a, P.kc, s);
+    up_c2e(P.t_p2m);
     TBEM_HIP(hipEventRecord(W.ev[2], s));
 
     // M2M (cpp/fmm.cpp:204-251), deepest parents first
@@ -337,10 +362,9 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
         set_list(m, P.m2m[L]);
         m.tt = view(P.dst); m.t_surf = P.d_outer.p;
         m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
-        m.c2e = P.uc2e.p; m.out = W.mult.p;
-        m.chk_out = mfma ? W.chk_up.p : nullptr;
-        m.chk_basis = P.m2l_mfma.Bu.p;
+        m.out = W.fld.p;
         launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
+        up_c2e(P.t_m2m[L]);
     }
     TBEM_HIP(hipEventRecord(W.ev[3], s));
 
@@ -351,23 +375,25 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
         // Vd M_c Bu check = DC2E K_c UC2E check = the FP64 stage's DC2E K_c mult
         launch_m2l_mfma(P.m2l_mfma, P.m2l.cell.p, W.chk_up.p, W.m2l_y.p, s);
         TBEM_HIP(hipEventRecord(W.ev[9], s));
-        launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.dot.level.p, W.m2l_y.p,
+        launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.t_m2l, W.m2l_y.p, W.fld.p,
                           W.locals.p, s);
     } else {
         SurfStage m = surf_base(P);
         set_list(m, P.m2l);
         m.tt = view(P.dot); m.t_surf = P.d_inner.p;
         m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
-        m.c2e = P.dc2e.p; m.out = W.locals.p;
-        launch_surf_stage(P.kid, true, true, false, m, P.kc, s);
+        m.out = W.fld.p;
+        launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
         TBEM_HIP(hipEventRecord(W.ev[9], s));
+        down_c2e(P.t_m2l);
     }
     SurfStage pl = surf_base(P);
     set_list(pl, P.p2l);
     pl.tt = view(P.dot); pl.t_surf = P.d_inner.p;
     pl.st = view(P.dst); pl.sp = view(P.sp); pl.dens = W.dens.p; pl.n_src_pts = Ns;
-    pl.c2e = P.dc2e.p; pl.out = W.locals.p;
-    launch_surf_stage(P.kid, false, true, false, pl, P.kc, s);
+    pl.out = W.fld.p;
+    launch_surf_stage(P.kid, false, false, false, pl, P.kc, s);
+    down_c2e(P.t_p2l);
     TBEM_HIP(hipEventRecord(W.ev[4], s));
 
     // L2L (cpp/fmm.cpp:384-413), root first
@@ -376,8 +402,9 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
         set_list(l, P.l2l[L]);
         l.tt = view(P.dot); l.t_surf = P.d_inner.p;
         l.st = view(P.dot); l.s_surf = P.d_outer.p; l.s_coeff = W.locals.p;
-        l.c2e = P.dc2e.p; l.out = W.locals.p;
-        launch_surf_stage(P.kid, true, true, false, l, P.kc, s);
+        l.out = W.fld.p;
+        launch_surf_stage(P.kid, true, false, false, l, P.kc, s);
+        down_c2e(P.t_olev[L]);
     }
     TBEM_HIP(hipEventRecord(W.ev[5], s));
 }
@@ -479,34 +506,33 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     set_list(a, P.a_l2p);
     a.tt = view(P.dot); a.t_surf = P.d_outer.p;
     a.st = view(P.dot); a.sp = view(P.op); a.dens = W.gsort.p; a.n_src_pts = No;
-    a.c2e = nullptr; a.out = W.lbar.p;
+    a.out = W.lbar.p;
     launch_surf_stage(P.kid, false, false, true, a, P.kc, s);
 
     // L2L^T, deepest level first: cbar = DC2E^T lbar, then push to parents
     for (int L = P.ot.n_levels - 1; L >= 0; L--) {
-        const int c0 = P.ot.level_begin[L], nc = P.ot.level_begin[L + 1] - c0;
-        launch_c2e_transpose_apply(P.iota.p + c0, nc, P.dot.level.p, P.dc2e.p, P.blk, W.lbar.p, W.cbar.p, s);
+        launch_level_gemm(P.t_olev[L], P.blk, P.dc2e.p, true, W.lbar.p, nullptr, W.cbar.p, nullptr, false, s);
         if (L == 0) break;
         SurfStage l = surf_base(P);
         set_list(l, P.a_l2l[L - 1]);
         l.tt = view(P.dot); l.t_surf = P.d_outer.p;
         l.st = view(P.dot); l.s_surf = P.d_inner.p; l.s_coeff = W.cbar.p;
-        l.c2e = nullptr; l.out = W.lbar.p;
+        l.out = W.lbar.p;
         launch_surf_stage(P.kid, true, true, true, l, P.kc, s);
     }
 
     // M2L^T into mbar (FP64), or, on MFMA, e2 = Bu^T M_c^T Vd^T lbar, added to ebar below
     if (mfma) {
-        launch_m2l_modal_in(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.dot.level.p, W.lbar.p, W.m2l_w.p, s);
+        launch_m2l_modal_in(P.m2l_mfma, P.t_m2l, W.lbar.p, W.m2l_w.p, s);
         launch_m2l_mfma(P.m2l_mfma, nullptr, W.m2l_w.p, W.m2l_y.p, s, true);
         launch_fill(W.e2.p, ncs, 0.0, s);
-        launch_m2l_reduce_adj(P.m2l_mfma, P.dst.level.p, W.m2l_y.p, W.e2.p, s);
+        launch_m2l_reduce_adj(P.m2l_mfma, W.m2l_y.p, W.fld.p, W.e2.p, s);
     } else {
         SurfStage m = surf_base(P);
         set_list(m, P.a_m2l);
         m.tt = view(P.dst); m.t_surf = P.d_inner.p;
         m.st = view(P.dot); m.s_surf = P.d_inner.p; m.s_coeff = W.cbar.p;
-        m.c2e = nullptr; m.out = W.mbar.p;
+        m.out = W.mbar.p;
         launch_surf_stage(P.kid, true, true, true, m, P.kc, s);
     }
     // M2P^T into mbar
@@ -514,20 +540,19 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     set_list(mp, P.a_m2p);
     mp.tt = view(P.dst); mp.t_surf = P.d_inner.p;
     mp.st = view(P.dot); mp.sp = view(P.op); mp.dens = W.gsort.p; mp.n_src_pts = No;
-    mp.c2e = nullptr; mp.out = W.mbar.p;
+    mp.out = W.mbar.p;
     launch_surf_stage(P.kid, false, true, true, mp, P.kc, s);
 
     // M2M^T, root first: ebar = UC2E^T mbar, then push to children
     for (int L = 0; L < P.st.n_levels; L++) {
-        const int c0 = P.st.level_begin[L], nc = P.st.level_begin[L + 1] - c0;
-        launch_c2e_transpose_apply(P.iota.p + c0, nc, P.dst.level.p, P.uc2e.p, P.blk, W.mbar.p, W.ebar.p, s,
-                                   mfma ? W.e2.p : nullptr);
+        launch_level_gemm(P.t_slev[L], P.blk, P.uc2e.p, true, W.mbar.p, mfma ? W.e2.p : nullptr, W.ebar.p, nullptr,
+                          false, s);
         if (L + 1 >= P.st.n_levels) break;
         SurfStage c = surf_base(P);
         set_list(c, P.a_m2m[L + 1]);
         c.tt = view(P.dst); c.t_surf = P.d_inner.p;
         c.st = view(P.dst); c.s_surf = P.d_outer.p; c.s_coeff = W.ebar.p;
-        c.c2e = nullptr; c.out = W.mbar.p;
+        c.out = W.mbar.p;
         launch_surf_stage(P.kid, true, true, true, c, P.kc, s);
     }
 

@@ -27,6 +27,7 @@ struct M2lMfma {
     DevBuf<int> pair_tgt;   // target cell of every pair
     int n_adj = 0;          // transpose: source cells with pairs, and their pairs (CSR)
     DevBuf<int> adj_items, adj_ptr, adj_pairs;
+    CellTiles adj_tiles;    // adj_items as GEMM tiles
     double plan_seconds = 0;
 };
 
@@ -44,15 +45,17 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
 void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s,
                      bool transpose = false);
 
-// Transpose, input: w[t] = float(Vd[level t]^T lbar[t]) for every M2L target t.
-void launch_m2l_modal_in(const M2lMfma& M, int n_items, const int* items, const int* level, const double* lbar,
-                         float* w, hipStream_t s);
+// Transpose, input: w[t] = float(Vd[level t]^T lbar[t]) for every M2L target t
+// (targets: the M2L targets as GEMM tiles).
+void launch_m2l_modal_in(const M2lMfma& M, const CellTiles& targets, const double* lbar, float* w, hipStream_t s);
 
-// Transpose, output: e2[s] = Bu[level s]^T * (sum over the pairs p of source s of Y[p]), FP64.
-void launch_m2l_reduce_adj(const M2lMfma& M, const int* level, const float* Y, double* e2, hipStream_t s);
+// Transpose, output: e2[s] = Bu[level s]^T * (sum over the pairs p of source s of
+// Y[p]), FP64, for the sources with pairs; z (blk doubles per source cell) is scratch.
+void launch_m2l_reduce_adj(const M2lMfma& M, const float* Y, double* z, double* e2, hipStream_t s);
 
-// locals[t] += Vd[level t] * (sum over the pairs p of target t of Y[p]), in FP64.
-void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const int* level,
-                       const float* Y, double* locals, hipStream_t s);
+// locals[t] += Vd[level t] * (sum over the pairs p of target t of Y[p]), in FP64
+// (targets: items as GEMM tiles; z, blk doubles per target cell, is scratch).
+void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const CellTiles& targets,
+                       const float* Y, double* z, double* locals, hipStream_t s);
 
 }  // namespace tbem

@@ -9,7 +9,7 @@
 // expression (tensor_ref, as the C2E systems), then M_c = Ld * (K_c * Ru) by a tiled
 // FP64 GEMM, rounded to FP32 in MFMA A-operand order. The FP32 data are the
 // up-check potential in modal form, Bu * check (computed in FP64 by the P2M/M2M
-// epilogue), and M_c's output z; the target adds Vd * sum(z) in FP64. In this
+// stage), and M_c's output z; the target adds Vd * sum(z) in FP64. In this
 // basis each entry and the input it multiplies scale together mode by mode, so the
 // FP32 sums carry no cancellation: measured in a single-pair model at orders
 // 20-150, <= 5e-7 relative, against up to 3e-4 for the operator DC2E K_c UC2E
@@ -32,6 +32,7 @@
 
 #include "c2e.h"
 #include "m2l_mfma.h"
+#include "stages.h"
 
 namespace tbem {
 
@@ -187,16 +188,17 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int yst
     }
 }
 
-// locals[t] += Vd[level t] * sum_p Y[p], sums in FP64. One 256-thread workgroup
-// per target. The target's pairs are contiguous in Y, so the block is read as
+// z[t] = sum_p Y[p] over the pairs p of target t, in FP64 (blk entries per cell);
+// the Vd product follows as a batched per-level GEMM. One 256-thread workgroup per
+// target. The target's pairs are contiguous in Y, so the block is read as
 // consecutive float4s: thread i takes float4 column i % q (q = ystride / 4) of
 // every npar-th pair starting at i / q, and the npar partial sums are combined
-// through LDS.
-__global__ __launch_bounds__(256) void k_m2l_reduce(const int* __restrict__ items, const int* __restrict__ ptr,
-                                                    const int* __restrict__ level, const double* __restrict__ Vd,
-                                                    const float* __restrict__ Y, int ystride, int blk,
-                                                    double* __restrict__ locals) {
-    extern __shared__ double part[];  // npar * ystride partial sums, then z (ystride)
+// through LDS. With pairs given, the pairs of item k are pairs[ptr[k] .. ptr[k+1]]
+// (the transpose's source-side lists) instead of the range ptr[k] .. ptr[k+1].
+__global__ __launch_bounds__(256) void k_m2l_pair_sum(const int* __restrict__ items, const int* __restrict__ ptr,
+                                                      const int* __restrict__ pairs, const float* __restrict__ Y,
+                                                      int ystride, int blk, double* __restrict__ z) {
+    extern __shared__ double part[];  // npar * ystride partial sums
     const int k = blockIdx.x;
     const int t = uni(items[k]), p0 = uni(ptr[k]), p1 = uni(ptr[k + 1]);
     const int q = ystride / 4, npar = 256 / q;
@@ -204,8 +206,9 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(const int* __restrict__ item
     if (ph < npar) {
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         const f32x4* y4 = (const f32x4*)Y;
-        for (int p = p0 + ph; p < p1; p += npar) {
-            const f32x4 v = y4[(size_t)p * q + c];
+        for (int j = p0 + ph; j < p1; j += npar) {
+            const size_t p = pairs ? (size_t)pairs[j] : (size_t)j;
+            const f32x4 v = y4[p * q + c];
             s0 += (double)v[0];
             s1 += (double)v[1];
             s2 += (double)v[2];
@@ -215,76 +218,10 @@ __global__ __launch_bounds__(256) void k_m2l_reduce(const int* __restrict__ item
         o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3;
     }
     __syncthreads();
-    double* z = part + (size_t)npar * ystride;
     for (int row = threadIdx.x; row < blk; row += blockDim.x) {
         double s = 0;
         for (int h = 0; h < npar; h++) s += part[(size_t)h * ystride + row];
-        z[row] = s;
-    }
-    __syncthreads();
-    const double* V = Vd + (size_t)level[t] * blk * blk;
-    for (int row = threadIdx.x; row < blk; row += blockDim.x) {
-        double s = 0;
-        const double* v = V + (size_t)row * blk;
-        for (int col = 0; col < blk; col++) s += v[col] * z[col];
-        locals[(size_t)t * blk + row] += s;
-    }
-}
-
-// Transpose input: w[t] = float(Vd[level t]^T lbar[t]) for the M2L targets.
-__global__ __launch_bounds__(256) void k_m2l_modal_in(const int* __restrict__ items, const int* __restrict__ level,
-                                                      const double* __restrict__ Vd, int blk,
-                                                      const double* __restrict__ lbar, float* __restrict__ w) {
-    extern __shared__ double lb[];  // blk
-    const int t = uni(items[blockIdx.x]);
-    for (int i = threadIdx.x; i < blk; i += blockDim.x) lb[i] = lbar[(size_t)t * blk + i];
-    __syncthreads();
-    const double* V = Vd + (size_t)level[t] * blk * blk;
-    for (int row = threadIdx.x; row < blk; row += blockDim.x) {  // (Vd^T lb)[row] = sum_col Vd[col][row] lb[col]
-        double s = 0;
-        for (int col = 0; col < blk; col++) s += V[(size_t)col * blk + row] * lb[col];
-        w[(size_t)t * blk + row] = (float)s;
-    }
-}
-
-// Transpose output: e2[s] = Bu[level s]^T sum_{p in pairs(s)} Y[p], sums in FP64.
-// Each pair's block is contiguous in Y; threads read it as float4 columns, npar
-// pairs in parallel, as k_m2l_reduce.
-__global__ __launch_bounds__(256) void k_m2l_reduce_adj(const int* __restrict__ items, const int* __restrict__ ptr,
-                                                        const int* __restrict__ pairs, const int* __restrict__ level,
-                                                        const double* __restrict__ Bu, const float* __restrict__ Y,
-                                                        int ystride, int blk, double* __restrict__ e2) {
-    extern __shared__ double part[];  // npar * ystride partial sums, then z (ystride)
-    const int k = blockIdx.x;
-    const int sc = uni(items[k]), j0 = uni(ptr[k]), j1 = uni(ptr[k + 1]);
-    const int q = ystride / 4, npar = 256 / q;
-    const int c = threadIdx.x % q, ph = threadIdx.x / q;
-    if (ph < npar) {
-        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        const f32x4* y4 = (const f32x4*)Y;
-        for (int j = j0 + ph; j < j1; j += npar) {
-            const f32x4 v = y4[(size_t)pairs[j] * q + c];
-            s0 += (double)v[0];
-            s1 += (double)v[1];
-            s2 += (double)v[2];
-            s3 += (double)v[3];
-        }
-        double* o = part + (size_t)ph * ystride + 4 * c;
-        o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3;
-    }
-    __syncthreads();
-    double* z = part + (size_t)npar * ystride;
-    for (int row = threadIdx.x; row < blk; row += blockDim.x) {
-        double s = 0;
-        for (int h = 0; h < npar; h++) s += part[(size_t)h * ystride + row];
-        z[row] = s;
-    }
-    __syncthreads();
-    const double* B = Bu + (size_t)level[sc] * blk * blk;
-    for (int row = threadIdx.x; row < blk; row += blockDim.x) {  // (Bu^T z)[row] = sum_col Bu[col][row] z[col]
-        double s = 0;
-        for (int col = 0; col < blk; col++) s += B[(size_t)col * blk + row] * z[col];
-        e2[(size_t)sc * blk + row] = s;
+        z[(size_t)t * blk + row] = s;
     }
 }
 
@@ -511,6 +448,7 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.pair_tgt.upload(pair_tgt, s);
     M.adj_items.upload(adj_items, s);
     M.adj_ptr.upload(adj_ptr, s);
+    M.adj_tiles = make_cell_tiles(adj_items, st, s);
     M.adj_pairs.upload(adj_pairs, s);
     TBEM_HIP(hipStreamSynchronize(s));
     M.enabled = true;
@@ -527,29 +465,28 @@ void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up,
     TBEM_LAUNCH_CHECK();
 }
 
-void launch_m2l_modal_in(const M2lMfma& M, int n_items, const int* items, const int* level, const double* lbar,
-                         float* w, hipStream_t s) {
-    if (!M.enabled || n_items == 0) return;
-    hipLaunchKernelGGL(k_m2l_modal_in, dim3(n_items), dim3(256), M.blk * sizeof(double), s, items, level, M.Vd.p,
-                       M.blk, lbar, w);
-    TBEM_LAUNCH_CHECK();
+void launch_m2l_modal_in(const M2lMfma& M, const CellTiles& targets, const double* lbar, float* w, hipStream_t s) {
+    if (!M.enabled) return;
+    launch_level_gemm(targets, M.blk, M.Vd.p, true, lbar, nullptr, nullptr, w, false, s);
 }
 
-void launch_m2l_reduce_adj(const M2lMfma& M, const int* level, const float* Y, double* e2, hipStream_t s) {
+void launch_m2l_reduce_adj(const M2lMfma& M, const float* Y, double* z, double* e2, hipStream_t s) {
     if (!M.enabled || M.n_adj == 0) return;
     const int npar = 256 / (M.ystride / 4);
-    hipLaunchKernelGGL(k_m2l_reduce_adj, dim3(M.n_adj), dim3(256), (size_t)(npar + 1) * M.ystride * sizeof(double),
-                       s, M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, level, M.Bu.p, Y, M.ystride, M.blk, e2);
+    hipLaunchKernelGGL(k_m2l_pair_sum, dim3(M.n_adj), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
+                       M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, Y, M.ystride, M.blk, z);
     TBEM_LAUNCH_CHECK();
+    launch_level_gemm(M.adj_tiles, M.blk, M.Bu.p, true, z, nullptr, e2, nullptr, false, s);
 }
 
-void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const int* level,
-                       const float* Y, double* locals, hipStream_t s) {
+void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const CellTiles& targets,
+                       const float* Y, double* z, double* locals, hipStream_t s) {
     if (!M.enabled || n_items == 0) return;
     const int npar = 256 / (M.ystride / 4);
-    hipLaunchKernelGGL(k_m2l_reduce, dim3(n_items), dim3(256), (size_t)(npar + 1) * M.ystride * sizeof(double), s,
-                       items, ptr, level, M.Vd.p, Y, M.ystride, M.blk, locals);
+    hipLaunchKernelGGL(k_m2l_pair_sum, dim3(n_items), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
+                       items, ptr, nullptr, Y, M.ystride, M.blk, z);
     TBEM_LAUNCH_CHECK();
+    launch_level_gemm(targets, M.blk, M.Vd.p, false, z, nullptr, locals, nullptr, true, s);
 }
 
 }  // namespace tbem

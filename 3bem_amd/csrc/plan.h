@@ -75,6 +75,13 @@ struct DevList {
     void upload(const HostList& l, hipStream_t s);
 };
 
+// Cells grouped into tiles of at most 64 cells of one level (make_cell_tiles).
+struct CellTiles {
+    int n = 0;            // number of tiles
+    DevBuf<int> cells;    // the cells, grouped by level
+    DevBuf<int> tiles;    // per tile: first index into cells, cell count, level
+};
+
 // Turn a cell list (ptr/cell) into merged point ranges of tree t (ptr/rbeg/rcnt).
 void cells_to_ranges(HostList& l, const HostTree& t);
 

@@ -25,9 +25,10 @@ inline TreeView view(const DevTree& t) {
 inline PtsView view(const DevPoints& p) { return PtsView{p.x.p, p.y.p, p.z.p, p.nx.p, p.ny.p, p.nz.p}; }
 
 // Check-potential stage. For each item (a target cell of tree tt) accumulate, on
-// the cell's target surface, the field of the item's sources; then
-//   out[cell] (= or +=) C2E[level(cell)] * field     (c2e != nullptr)
-//   out[cell] (= or +=) field                        (c2e == nullptr)
+// the cell's target surface, the field of the item's sources:
+//   out[cell] (= or +=) field
+// The check-to-equivalent products that follow run batched per level
+// (launch_level_gemm).
 // Sources are either equivalent surfaces of cells of tree st carrying s_coeff
 // (src_surf: entries cell[ptr[item] .. ptr[item+1]]) or contiguous ranges of the
 // sorted source points carrying dens (!src_surf: ranges rbeg/rcnt[ptr[item] ..]).
@@ -46,11 +47,20 @@ struct SurfStage {
     int n_src_pts;
     const double* unit;     // unit sphere = surface normals
     int n_e, blk;
-    const double* c2e;      // per level blk*blk row-major, or nullptr
     double* out;            // blk per cell of tt
-    float* chk_out;         // optional: chk_basis[level] * field (or the field), FP32, blk per cell of tt
-    const double* chk_basis;  // optional: per level blk*blk row-major
 };
+
+// Tiles of at most 64 cells of one level for the batched per-level products (the
+// cells are grouped by level first).
+CellTiles make_cell_tiles(const std::vector<int>& cells, const HostTree& t, hipStream_t s);
+CellTiles make_level_tiles(const HostTree& t, int level, hipStream_t s);
+
+// For every cell c of the tiles: out[c] (= or += when add) op(M[level c]) in[c]
+// (+ addin[c] when given), op = identity or transpose, blk doubles per cell, M per
+// level blk*blk row-major. FP64; out32 receives the result rounded to FP32 instead
+// of out when given. A tiled GEMM: each level matrix is read once per 64 cells.
+void launch_level_gemm(const CellTiles& T, int blk, const double* M, bool trans, const double* in,
+                       const double* addin, double* out, float* out32, bool add, hipStream_t s);
 
 // Point-target stage (FP64, or T = float for the point sources (a) with every
 // displacement formed in FP64; the surface sources (b), (c) are always FP64).
@@ -97,11 +107,6 @@ void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, 
 // point sources in FP32 (mixed precision; kernels with closed-form gradients)
 void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s, bool fp32_points);
 
-// out[cell] = C2E[level(cell)]^T in[cell] (+ add[cell]) for the listed cells (adjoint
-// of the C2E step)
-void launch_c2e_transpose_apply(const int* cells, int n_cells, const int* level, const double* c2e,
-                                int blk, const double* in, double* out, hipStream_t s,
-                                const double* add = nullptr);
 
 void launch_gather_dens(const double* x, const int* perm, const double* w, int n, int R,
                         double* dens, hipStream_t s);

@@ -2,8 +2,9 @@
 //
 // Every stage is "accumulate the field of a stream of sources at a set of targets":
 //   * surface targets (P2M, M2M, M2L, P2L, L2L and their transposes): the n_e
-//     points of a cell's check/equivalent surface, then an FP64 C2E matvec;
-//     one kSurfBlock-thread workgroup per target cell;
+//     points of a cell's check/equivalent surface; one kSurfBlock-thread
+//     workgroup per target cell; the FP64 C2E products follow as batched per-level
+//     GEMMs (k_level_gemm);
 //   * point targets (P2P + M2P + L2P fused, or P2P^T + P2L^T + P2M^T): the
 //     points of one leaf; one kLeafBlock-thread workgroup per leaf.
 // Sources are staged through LDS in NB-entry SoA tiles (NB = threads per
@@ -33,6 +34,7 @@
 // equivalent densities, whose entries grow like (1.73/0.52)^degree because the
 // equivalent sphere lies inside the cell: they stay FP64 (a few percent of the
 // pairs of a leaf).
+#include <algorithm>
 #include <type_traits>
 
 #include "stages.h"
@@ -347,32 +349,8 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
         }
         __syncthreads();
     }
-    if (a.chk_out) {
-        float* c = a.chk_out + (size_t)cell * a.blk;
-        if (a.chk_basis) {
-            const double* B = a.chk_basis + (size_t)lvl * a.blk * a.blk;
-            for (int row = tid; row < a.blk; row += NB) {
-                double s = 0;
-                const double* b = B + (size_t)row * a.blk;
-                for (int col = 0; col < a.blk; col++) s += b[col] * field[col];
-                c[row] = (float)s;
-            }
-        } else {
-            for (int row = tid; row < a.blk; row += NB) c[row] = (float)field[row];
-        }
-    }
     double* o = a.out + (size_t)cell * a.blk;
-    if (a.c2e) {
-        const double* M = a.c2e + (size_t)lvl * a.blk * a.blk;
-        for (int row = tid; row < a.blk; row += NB) {
-            double s = 0;
-            const double* m = M + (size_t)row * a.blk;
-            for (int col = 0; col < a.blk; col++) s += m[col] * field[col];
-            o[row] = add ? o[row] + s : s;
-        }
-    } else {
-        for (int row = tid; row < a.blk; row += NB) o[row] = add ? o[row] + field[row] : field[row];
-    }
+    for (int row = tid; row < a.blk; row += NB) o[row] = add ? o[row] + field[row] : field[row];
 }
 
 // -------------------------------------------------------------------- point targets
@@ -580,16 +558,58 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
 }
 
 // ------------------------------------------------------------------------ helpers
-__global__ void k_c2e_transpose_apply(const int* __restrict__ cells, const int* __restrict__ level,
-                                      const double* __restrict__ c2e, int blk, const double* __restrict__ in,
-                                      const double* __restrict__ add, double* __restrict__ out) {
-    const int cell = cells[blockIdx.x];
-    const double* M = c2e + (size_t)level[cell] * blk * blk;
-    const double* x = in + (size_t)cell * blk;
-    for (int col = threadIdx.x; col < blk; col += blockDim.x) {
-        double s = 0;
-        for (int row = 0; row < blk; row++) s += M[(size_t)row * blk + col] * x[row];
-        out[(size_t)cell * blk + col] = add ? s + add[(size_t)cell * blk + col] : s;
+// out[c] (= or +=) op(M[level c]) in[c] (+ addin[c]) for the cells of each tile:
+// a 64 (rows) x 64 (cells) output tile per workgroup, 4 x 4 per thread, k in
+// chunks of 16 staged in LDS.
+__global__ __launch_bounds__(256) void k_level_gemm(int n, const int* __restrict__ cells,
+                                                    const int* __restrict__ tiles, const double* __restrict__ M,
+                                                    int trans, const double* __restrict__ in,
+                                                    const double* __restrict__ addin, double* __restrict__ out,
+                                                    float* __restrict__ out32, int add) {
+    const int t = blockIdx.y;
+    const int first = uni(tiles[3 * t]), cnt = uni(tiles[3 * t + 1]), lvl = uni(tiles[3 * t + 2]);
+    const double* A = M + (size_t)lvl * n * n;
+    const int m0 = blockIdx.x * 64;
+    __shared__ double As[16][65], Bs[16][65];
+    __shared__ int cl[64];
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    if (tid < 64) cl[tid] = tid < cnt ? cells[first + tid] : -1;
+    __syncthreads();
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < n; k0 += 16) {
+        for (int e = tid; e < 1024; e += 256) {
+            // A chunk: As[k][r] = op(A)[m0 + r][k0 + k]
+            int r, k;
+            if (trans) { k = e / 64; r = e % 64; }
+            else { r = e / 16; k = e % 16; }
+            const int row = m0 + r, kk = k0 + k;
+            As[k][r] = (row < n && kk < n) ? (trans ? A[(size_t)kk * n + row] : A[(size_t)row * n + kk]) : 0.0;
+            // B chunk: Bs[k][c] = in[cell c][k0 + k]
+            const int c = e / 16, kb = e % 16, cell = cl[c];
+            Bs[kb][c] = (cell >= 0 && k0 + kb < n) ? in[(size_t)cell * n + k0 + kb] : 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < 16; k++) {
+            double av[4], bv[4];
+            for (int i = 0; i < 4; i++) av[i] = As[k][ty * 4 + i];
+            for (int j = 0; j < 4; j++) bv[j] = Bs[k][tx * 4 + j];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < 4; j++) {
+        const int cell = cl[tx * 4 + j];
+        if (cell < 0) continue;
+        for (int i = 0; i < 4; i++) {
+            const int row = m0 + ty * 4 + i;
+            if (row >= n) continue;
+            const size_t o = (size_t)cell * n + row;
+            double v = acc[i][j];
+            if (addin) v += addin[o];
+            if (out32) out32[o] = (float)v;
+            else out[o] = add ? out[o] + v : v;
+        }
     }
 }
 
@@ -615,11 +635,38 @@ __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
         p[i] = v;
 }
 
-void launch_c2e_transpose_apply(const int* cells, int n_cells, const int* level, const double* c2e,
-                                int blk, const double* in, double* out, hipStream_t s, const double* add) {
-    if (n_cells == 0) return;
-    hipLaunchKernelGGL(k_c2e_transpose_apply, dim3(n_cells), dim3(kSurfBlock), 0, s, cells, level, c2e, blk, in,
-                       add, out);
+CellTiles make_cell_tiles(const std::vector<int>& cells_in, const HostTree& t, hipStream_t s) {
+    CellTiles T;
+    std::vector<int> cells(cells_in);
+    std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) { return t.level[a] < t.level[b]; });
+    std::vector<int> tiles;
+    for (size_t i = 0; i < cells.size();) {
+        const int lvl = t.level[cells[i]];
+        size_t j = i;
+        while (j < cells.size() && j - i < 64 && t.level[cells[j]] == lvl) j++;
+        tiles.push_back((int)i);
+        tiles.push_back((int)(j - i));
+        tiles.push_back(lvl);
+        i = j;
+    }
+    T.n = (int)(tiles.size() / 3);
+    T.cells.upload(cells, s);
+    T.tiles.upload(tiles, s);
+    TBEM_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    return T;
+}
+
+CellTiles make_level_tiles(const HostTree& t, int level, hipStream_t s) {
+    std::vector<int> cells;
+    for (int c = t.level_begin[level]; c < t.level_begin[level + 1]; c++) cells.push_back(c);
+    return make_cell_tiles(cells, t, s);
+}
+
+void launch_level_gemm(const CellTiles& T, int blk, const double* M, bool trans, const double* in,
+                       const double* addin, double* out, float* out32, bool add, hipStream_t s) {
+    if (T.n == 0) return;
+    hipLaunchKernelGGL(k_level_gemm, dim3(div_up(blk, 64), T.n), dim3(256), 0, s, blk, T.cells.p, T.tiles.p, M,
+                       trans ? 1 : 0, in, addin, out, out32, add ? 1 : 0);
     TBEM_LAUNCH_CHECK();
 }
 
