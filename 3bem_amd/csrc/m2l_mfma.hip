@@ -126,16 +126,26 @@ __global__ void k_m2l_swizzle(int n, int mt, int kt, int trans, const double* __
 
 // One row tile (16 rows of M_c) times the NCT staged column tiles. A comes from
 // global memory (L2-resident), loaded kPre k-steps ahead of its use so that the
-// loads' latency overlaps kPre * NCT MFMAs.
-constexpr int kPre = 8;
+// loads' latency overlaps kPre * NCT MFMAs. The last block of a row tile already
+// loads the first kPre k-steps of the wave's next row tile (An; null for none), so
+// the pipeline does not drain between row tiles: cur holds A's first kPre k-steps
+// on entry and An's on exit.
+#ifndef TBEM_M2L_KPRE
+#define TBEM_M2L_KPRE 16
+#endif
+constexpr int kPre = TBEM_M2L_KPRE;
 
 template <int NCT>
-__device__ __forceinline__ void m2l_row_tile(const float* __restrict__ A, const float* __restrict__ b, int kt,
-                                             int s0, int s1, int s2, int s3, f32x4* acc) {
-    float cur[kPre], nxt[kPre];
-    for (int i = 0; i < kPre; i++) cur[i] = i < kt ? A[(size_t)i * 64] : 0.f;
+__device__ __forceinline__ void m2l_row_tile(const float* __restrict__ A, const float* __restrict__ An,
+                                             const float* __restrict__ b, int kt, int s0, int s1, int s2, int s3,
+                                             f32x4* acc, float* cur) {
+    float nxt[kPre];
     for (int k0 = 0; k0 < kt; k0 += kPre) {
-        for (int i = 0; i < kPre; i++) nxt[i] = k0 + kPre + i < kt ? A[(size_t)(k0 + kPre + i) * 64] : 0.f;
+        if (k0 + kPre >= kt) {
+            for (int i = 0; i < kPre; i++) nxt[i] = (An && i < kt) ? An[(size_t)i * 64] : 0.f;
+        } else {
+            for (int i = 0; i < kPre; i++) nxt[i] = k0 + kPre + i < kt ? A[(size_t)(k0 + kPre + i) * 64] : 0.f;
+        }
 #pragma unroll
         for (int i = 0; i < kPre; i++) {
             if (k0 + i >= kt) break;
@@ -171,14 +181,18 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int yst
     const float* b = Bs + lq * 64 + lr;
     const int s0 = (0 ^ lq) * 16, s1 = (1 ^ lq) * 16, s2 = (2 ^ lq) * 16, s3 = (3 ^ lq) * 16;
     const int nct = (cnt + 15) / 16;  // column tiles holding pairs (uniform)
+    const float* Acls = S + (size_t)cls * mt * kt * 64 + lane;
+    float cur[kPre];
+    for (int i = 0; i < kPre; i++) cur[i] = (wave < mt && i < kt) ? Acls[((size_t)wave * kt + i) * 64] : 0.f;
     for (int rt = wave; rt < mt; rt += 4) {
         f32x4 acc[4] = {};
-        const float* A = S + ((size_t)cls * mt + rt) * kt * 64 + lane;
+        const float* A = Acls + (size_t)rt * kt * 64;
+        const float* An = rt + 4 < mt ? A + (size_t)4 * kt * 64 : nullptr;
         switch (nct) {
-        case 1: m2l_row_tile<1>(A, b, kt, s0, s1, s2, s3, acc); break;
-        case 2: m2l_row_tile<2>(A, b, kt, s0, s1, s2, s3, acc); break;
-        case 3: m2l_row_tile<3>(A, b, kt, s0, s1, s2, s3, acc); break;
-        default: m2l_row_tile<4>(A, b, kt, s0, s1, s2, s3, acc); break;
+        case 1: m2l_row_tile<1>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
+        case 2: m2l_row_tile<2>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
+        case 3: m2l_row_tile<3>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
+        default: m2l_row_tile<4>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
         }
         // D[4*lq + r][lr] of column tile ct -> Y[pair][rt*16 + 4*lq + r]
         for (int ct = 0; ct < nct; ct++) {
