@@ -302,12 +302,12 @@ __device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, doub
 }
 
 // ------------------------------------------------------------------ surface targets
-template <int KID, bool SRC_SURF, bool ADJ>
+template <int KID, bool SRC_SURF, bool ADJ, bool DUAL>
 __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<double> kc, double scale, int add) {
     constexpr int NB = kSurfBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
-    __shared__ StageSmem<double, R, NRM, NB, R> sm;
+    __shared__ StageSmem<double, R, NRM, NB, (DUAL ? 2 : 1) * R> sm;
     extern __shared__ double field[];  // R * n_e
     const int item = blockIdx.x;
     const int cell = uni(a.items[item]);
@@ -317,34 +317,59 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
     const int e0 = uni(a.ptr[item]), e1 = uni(a.ptr[item + 1]);
     const int tid = threadIdx.x;
 
-    for (int t0 = 0; t0 < a.n_e; t0 += NB) {
-        const int nt = min(NB, a.n_e - t0);
-        const int ntp = nt, S = NB / ntp;
+    // DUAL (launched for n_e > NB/2): two targets per lane (slots tg and tg + ntp;
+    // a lane without a second target evaluates the first twice and drops the copy),
+    // so one LDS read of a source feeds two pairs. Otherwise one target per lane.
+    // The sources are split over S slices, summed at the end.
+    constexpr bool dual = DUAL;
+    const int per_pass = dual ? 2 * NB : NB;
+    for (int t0 = 0; t0 < a.n_e; t0 += per_pass) {
+        const int nt = min(per_pass, a.n_e - t0);
+        const int ntp = dual ? (nt + 1) / 2 : nt, S = NB / ntp;
         const int tg = tid % ntp, slice = tid / ntp;
         const bool active = slice < S;
-        const int i = t0 + tg;
-        double p[3];
-        moved(rc, a.t_surf + 3 * i, cx, cy, cz, p);
-        const double tn[3] = {a.unit[3 * i], a.unit[3 * i + 1], a.unit[3 * i + 2]};
+        const bool two = dual && tg + ntp < nt;
+        const int i0 = t0 + tg, i1 = two ? i0 + ntp : i0;
+        double p0[3], p1[3] = {0, 0, 0};
+        moved(rc, a.t_surf + 3 * i0, cx, cy, cz, p0);
+        if (dual) moved(rc, a.t_surf + 3 * i1, cx, cy, cz, p1);
+        const double tn0[3] = {a.unit[3 * i0], a.unit[3 * i0 + 1], a.unit[3 * i0 + 2]};
+        double tn1[3] = {0, 0, 0};
+        if (dual) for (int m = 0; m < 3; m++) tn1[m] = a.unit[3 * i1 + m];
         drain_loads();
-        double acc[R];
-        for (int q = 0; q < R; q++) acc[q] = 0.0;
+        double acc0[R], acc1[R];
+        for (int q = 0; q < R; q++) acc0[q] = acc1[q] = 0.0;
         auto compute = [&](int cnt) {
-            if (active)
-                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(sm.st, j, p[0], p[1], p[2], tn, acc, kc);
+            if (!active) return;
+            if constexpr (dual) {
+                for (int j = slice; j < cnt; j += S) {
+                    pair<KID, ADJ>(sm.st, j, p0[0], p0[1], p0[2], tn0, acc0, kc);
+                    pair<KID, ADJ>(sm.st, j, p1[0], p1[1], p1[2], tn1, acc1, kc);
+                }
+            } else {
+                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(sm.st, j, p0[0], p0[1], p0[2], tn0, acc0, kc);
+            }
         };
         if (SRC_SURF)
             stream_surfaces(sm.st, e0, e1, a.cell, a.st, a.s_surf, a.unit, a.s_coeff, a.n_e, a.blk, compute);
         else
             stream_ranges(sm.st, e0, e1, a.rbeg, a.rcnt, a.sp, a.dens, a.n_src_pts, compute);
         __syncthreads();
-        for (int q = 0; q < R; q++) sm.red[q * NB + tid] = active ? acc[q] : 0.0;
+        for (int q = 0; q < R; q++) {
+            sm.red[q * NB + tid] = active ? acc0[q] : 0.0;
+            if constexpr (dual) sm.red[(R + q) * NB + tid] = active ? acc1[q] : 0.0;
+        }
         __syncthreads();
         if (slice == 0) {
             for (int q = 0; q < R; q++) {
-                double s = 0;
-                for (int k = 0; k < S; k++) s += sm.red[q * NB + k * ntp + tg];
-                field[q * a.n_e + i] = s * scale;
+                double s0 = 0;
+                for (int k = 0; k < S; k++) s0 += sm.red[q * NB + k * ntp + tg];
+                field[q * a.n_e + i0] = s0 * scale;
+                if constexpr (dual) {
+                    double s1 = 0;
+                    for (int k = 0; k < S; k++) s1 += sm.red[(R + q) * NB + k * ntp + tg];
+                    if (two) field[q * a.n_e + i1] = s1 * scale;
+                }
             }
         }
         __syncthreads();
@@ -705,12 +730,17 @@ static void surf_launch(bool src_surf, bool add, bool adj, const SurfStage& a, c
     const double scale = (double)kop_scale<KID, double>(kc);
     const size_t shm = (size_t)kernel_dim(KID) * a.n_e * sizeof(double);
     dim3 g(a.n_items), b(kSurfBlock);
-    if (src_surf) {
-        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, true, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
-        else hipLaunchKernelGGL((k_surf_stage<KID, true, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
-    } else {
-        if (adj) hipLaunchKernelGGL((k_surf_stage<KID, false, true>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
-        else hipLaunchKernelGGL((k_surf_stage<KID, false, false>), g, b, shm, s, a, kc, scale, add ? 1 : 0);
+    const int v = (src_surf ? 4 : 0) + (adj ? 2 : 0) + (a.n_e > kSurfBlock / 2 ? 1 : 0);
+    const int ad = add ? 1 : 0;
+    switch (v) {
+    case 0: hipLaunchKernelGGL((k_surf_stage<KID, false, false, false>), g, b, shm, s, a, kc, scale, ad); break;
+    case 1: hipLaunchKernelGGL((k_surf_stage<KID, false, false, true>), g, b, shm, s, a, kc, scale, ad); break;
+    case 2: hipLaunchKernelGGL((k_surf_stage<KID, false, true, false>), g, b, shm, s, a, kc, scale, ad); break;
+    case 3: hipLaunchKernelGGL((k_surf_stage<KID, false, true, true>), g, b, shm, s, a, kc, scale, ad); break;
+    case 4: hipLaunchKernelGGL((k_surf_stage<KID, true, false, false>), g, b, shm, s, a, kc, scale, ad); break;
+    case 5: hipLaunchKernelGGL((k_surf_stage<KID, true, false, true>), g, b, shm, s, a, kc, scale, ad); break;
+    case 6: hipLaunchKernelGGL((k_surf_stage<KID, true, true, false>), g, b, shm, s, a, kc, scale, ad); break;
+    default: hipLaunchKernelGGL((k_surf_stage<KID, true, true, true>), g, b, shm, s, a, kc, scale, ad); break;
     }
     TBEM_LAUNCH_CHECK();
 }

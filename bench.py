@@ -115,25 +115,22 @@ def cpu_baseline(kernel, cfg, budget_n):
                       f"({t:.2f} s each; plan build {t_plan:.2f} s excluded), OMP threads={cores}"}
 
 
-def target_run(tbem, torch, steps):
-    """The FP32/MFMA-vs-FP64 comparison at the accuracy target: elastic displacement
-    FMM matvec on 2M points of the fault + free-surface geometry, order 100, leaf 256,
-    MAC 0.3 (FP64 FMM ~3e-7 from the direct sum), both precisions on the same plan
-    parameters, errors against an FP64 direct sum on a 5k-target subsample."""
-    wl = importlib.import_module("3bem_amd.workloads")
-    n = 2_000_000
-    g = wl.cfg2_fault_and_surface(n)
-    K = tbem.ElasticDisplacement(1.0, 0.25)
+def compare_run(tbem, torch, steps, g, kid, cfg_t, n_sub, desc):
+    """FP32/MFMA against FP64 on one workload, both precisions on the same plan
+    parameters: forward time, backward (adjoint + observation gradient) time, and
+    relative L2 errors against an FP64 direct sum on an n_sub-target subsample."""
+    n = len(g["locs"])
+    R = 1 if kid in (0, 1, 2, 7) else 3
+    K = tbem.make_kernel(kid, 1.0, 0.25)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
-    cfg = tbem.FMMConfig(0.3, 100, 256, 0.05, False)
+    cfg = tbem.FMMConfig(*cfg_t)
     x = torch.tensor(g["x"], device="cuda")
     rng = np.random.default_rng(321)
-    sub = np.sort(rng.choice(n, 5000, replace=False))
+    sub = np.sort(rng.choice(n, n_sub, replace=False))
     dsub = tbem.NBodyData(g["locs"][sub], g["normals"][sub], g["locs"], g["normals"], g["weights"])
     yd = tbem.make_direct_nbody_operator(dsub, K).apply(g["x"])
-    idx = np.concatenate([q * n + sub for q in range(3)])
-    out = {"workload": "elastic displacement FMM matvec, 2,000,000 points (fault + free surface), order 100, "
-                       "leaf 256, MAC 0.3"}
+    idx = np.concatenate([q * n + sub for q in range(R)])
+    out = {"workload": desc}
     ys = {}
     for prec in ("fp64", "mixed"):
         t0 = time.perf_counter()
@@ -143,7 +140,7 @@ def target_run(tbem, torch, steps):
         ys[prec] = op.apply(x).cpu().numpy()
         info = op.info()
         stage_ms = {k: round(v, 3) for k, v in op.stage_times().items()}
-        gv = torch.randn(3 * n, dtype=torch.float64, device="cuda")
+        gv = torch.randn(R * n, dtype=torch.float64, device="cuda")
         dt_adj, _ = time_steps(lambda: op.apply_adjoint(gv), 2, 1, torch)
         dt_grad, _ = time_steps(lambda: op.obs_grad(x, gv), 2, 1, torch)
         out[prec] = {"ms_per_step": round(dt * 1e3, 3), "Mpts_per_s": round(n / dt / 1e6, 3),
@@ -152,21 +149,41 @@ def target_run(tbem, torch, steps):
                      "plan_build_s": round(plan, 2), "m2l_kernel_ms": round(float(np.mean(m2l)), 3),
                      "stage_ms": stage_ms,
                      "rel_err_vs_direct": float(np.linalg.norm(ys[prec][idx] - yd) / np.linalg.norm(yd))}
-        if prec == "mixed":
-            blk = 3 * info["n_e"]
+        if prec == "mixed" and info["m2l_classes"] > 0:
+            blk = R * info["n_e"]
             flop = 2.0 * info["m2l"] * blk * blk
             ach = flop / (float(np.mean(m2l)) * 1e-3) / 1e12
             out["m2l_mfma_roofline"] = {"bound": "mfma", "kernel": "k_m2l_mfma (FP32 v_mfma_f32_16x16x4f32)",
                                         "achieved": round(ach, 2), "peak": 157.3, "unit": "TFLOP/s",
                                         "frac": round(ach / 157.3, 4), "algorithmic_flop": flop,
                                         "classes": info["m2l_classes"], "pairs": info["m2l"]}
-        log(f"[bench] target {prec}: {dt * 1e3:.2f} ms, rel err {out[prec]['rel_err_vs_direct']:.2e}, "
+        log(f"[bench] {desc[:40]} {prec}: {dt * 1e3:.2f} ms, rel err {out[prec]['rel_err_vs_direct']:.2e}, "
             f"m2l kernel {out[prec]['m2l_kernel_ms']:.2f} ms")
         del op
         torch.cuda.empty_cache()
     out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(ys["mixed"] - ys["fp64"]) / np.linalg.norm(ys["fp64"]))
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
     return out
+
+
+def target_run(tbem, torch, steps):
+    """The FP32/MFMA-vs-FP64 comparison at the accuracy target: elastic displacement
+    FMM matvec on 2M points of the fault + free-surface geometry, order 100, leaf 256,
+    MAC 0.3 (FP64 FMM ~4e-7 from the direct sum), errors on a 5k-target subsample."""
+    wl = importlib.import_module("3bem_amd.workloads")
+    return compare_run(tbem, torch, steps, wl.cfg2_fault_and_surface(2_000_000), 3, (0.3, 100, 256, 0.05, False),
+                       5000, "elastic displacement FMM matvec, 2,000,000 points (fault + free surface), order 100, "
+                             "leaf 256, MAC 0.3")
+
+
+def laplace_run(tbem, torch, steps):
+    """The mixed-precision Laplace workload: single-layer FMM matvec on 4M points
+    (half uniform in the unit cube, half in 8 Gaussian clusters, sigma 0.02), order
+    75, leaf 100, MAC 0.3, errors on a 10k-target subsample."""
+    wl = importlib.import_module("3bem_amd.workloads")
+    return compare_run(tbem, torch, steps, wl.cfg3_cube_clusters(4_000_000), 0, (0.3, 75, 100, 0.05, False),
+                       10_000, "Laplace single-layer FMM matvec, 4,000,000 points (cube + 8 clusters), order 75, "
+                               "leaf 100, MAC 0.3")
 
 
 def main():
@@ -181,6 +198,8 @@ def main():
     ap.add_argument("--skip-extra", action="store_true", help="FP64 forward only")
     ap.add_argument("--skip-target", action="store_true",
                     help="skip the 2M-point order-100 FP32/MFMA vs FP64 comparison")
+    ap.add_argument("--skip-laplace", action="store_true",
+                    help="skip the 4M-point Laplace FP32/MFMA-vs-FP64 comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -280,6 +299,8 @@ def main():
 
     if not args.skip_extra and not args.skip_target and args.config == "cfg2":
         out["target_fp32_mfma_vs_fp64"] = target_run(tbem, torch, max(3, args.steps // 2))
+    if not args.skip_extra and not args.skip_laplace and args.config == "cfg2":
+        out["laplace_4M_fp32_mfma_vs_fp64"] = laplace_run(tbem, torch, 3)
 
     if not args.skip_cpu:
         cfg_t = (cfg.mac, cfg.order, cfg.min_pts_per_cell, cfg.d, cfg.account_for_small_cells)
