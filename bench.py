@@ -41,7 +41,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 # --pmc FETCH_SIZE and WRITE_SIZE of the same command, scaled by what the counters
 # read for a plain device copy (profiles/k_leaf_stage_elastic_u_1M_order6.md).
 # Other workloads have no measured figure and report null.
-LEAF_TRAFFIC_BYTES = {("cfg2", 1_000_000): 640_564_408}
+LEAF_TRAFFIC_BYTES = {("cfg2", 1_000_000): 639_369_793}
 
 
 def log(*a):
@@ -219,6 +219,7 @@ def main():
             dst.add_(src)  # elementwise kernel: reads 512 MiB, writes 256 MiB
         torch.cuda.synchronize()
         log("[bench] calibration: 3 x (256 MiB copy, 256 MiB add_)")
+        return
     w = workload(args.config, args.n)
     kid = w["kernel"]
     R = 1 if kid in (0, 1, 2, 7) else 3
