@@ -186,6 +186,58 @@ def laplace_run(tbem, torch, steps):
                                "leaf 100, MAC 0.3")
 
 
+def cfg4_run(tbem, torch, steps):
+    """The differentiable elastic forward model: 40k fault triangles (120k points)
+    to 200k surface stations, loss 0.5 ||u - u_obs||^2, order 75, leaf 128, MAC
+    0.3. Forward and backward (adjoint + observation gradient) times, and the
+    autograd gradients against the FP64 direct sum (dL/dslip = D^T r; dL/dobs by
+    central differences of the direct sum, step 1e-6)."""
+    wl = importlib.import_module("3bem_amd.workloads")
+    ag = importlib.import_module("3bem_amd.autograd")
+    g = wl.cfg4_fault_and_stations()
+    obs, src = g["obs"], g["src"]
+    n = len(obs)
+    K = tbem.ElasticDisplacement(1.0, 0.25)
+
+    def data(o):
+        return tbem.NBodyData(o, g["obs_normals"], src, g["src_normals"], g["src_weights"])
+
+    D = tbem.make_direct_nbody_operator(data(obs), K)
+    u_true = D.apply(g["x_true"])
+    r = D.apply(g["x"]) - u_true
+    gs_ref = D.apply_adjoint(r)
+    go_ref = np.zeros((n, 3))
+    for m in range(3):
+        op_, om = obs.copy(), obs.copy()
+        op_[:, m] += 1e-6
+        om[:, m] -= 1e-6
+        du = (tbem.make_direct_nbody_operator(data(op_), K).apply(g["x"]) -
+              tbem.make_direct_nbody_operator(data(om), K).apply(g["x"])) / 2e-6
+        go_ref[:, m] = sum(r[q * n:(q + 1) * n] * du[q * n:(q + 1) * n] for q in range(3))
+    u_obs = torch.tensor(u_true, device="cuda")
+    x = torch.tensor(g["x"], device="cuda")
+    gv = torch.tensor(r, device="cuda")
+    out = {"workload": "differentiable elastic forward model: 40,000 fault triangles (120,000 points) to 200,000 "
+                       "surface stations, loss 0.5||u - u_obs||^2, order 75, leaf 128, MAC 0.3"}
+    for prec in ("fp64", "mixed"):
+        op = tbem.FMMOperator(K, data(obs), tbem.FMMConfig(0.3, 75, 128, 0.05, False), precision=prec)
+        dt, _ = time_steps(lambda: op.apply(x), steps, 1, torch)
+        dtb, _ = time_steps(lambda: (op.apply_adjoint(gv), op.obs_grad(x, gv)), steps, 1, torch)
+        slip = torch.tensor(g["x"], device="cuda", requires_grad=True)
+        o = torch.tensor(obs, device="cuda", requires_grad=True)
+        ag.forward_model_loss(op, slip, u_obs, o).backward()
+        gs, go = slip.grad.cpu().numpy(), o.grad.cpu().numpy()
+        out[prec] = {"fwd_ms": round(dt * 1e3, 3), "bwd_ms": round(dtb * 1e3, 3),
+                     "bwd_fwd_ratio": round(dtb / dt, 3),
+                     "rel_err_dL_dslip_vs_direct": float(np.linalg.norm(gs - gs_ref) / np.linalg.norm(gs_ref)),
+                     "rel_err_dL_dobs_vs_direct": float(np.linalg.norm(go - go_ref) / np.linalg.norm(go_ref))}
+        log(f"[bench] cfg4 {prec}: fwd {dt * 1e3:.2f} ms, bwd {dtb * 1e3:.2f} ms, grad errors "
+            f"{out[prec]['rel_err_dL_dslip_vs_direct']:.2e} / {out[prec]['rel_err_dL_dobs_vs_direct']:.2e}")
+        del op
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +252,8 @@ def main():
                     help="skip the 2M-point order-100 FP32/MFMA vs FP64 comparison")
     ap.add_argument("--skip-laplace", action="store_true",
                     help="skip the 4M-point Laplace FP32/MFMA-vs-FP64 comparison")
+    ap.add_argument("--skip-cfg4", action="store_true",
+                    help="skip the differentiable forward model (120k fault points, 200k stations)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -302,6 +356,8 @@ def main():
         out["target_fp32_mfma_vs_fp64"] = target_run(tbem, torch, max(3, args.steps // 2))
     if not args.skip_extra and not args.skip_laplace and args.config == "cfg2":
         out["laplace_4M_fp32_mfma_vs_fp64"] = laplace_run(tbem, torch, 3)
+    if not args.skip_extra and not args.skip_cfg4 and args.config == "cfg2":
+        out["forward_model_cfg4"] = cfg4_run(tbem, torch, 5)
 
     if not args.skip_cpu:
         cfg_t = (cfg.mac, cfg.order, cfg.min_pts_per_cell, cfg.d, cfg.account_for_small_cells)

@@ -1,0 +1,128 @@
+"""The differentiable elastic forward model at full size: 40k fault triangles (120k
+quadrature points, slip densities N(0,1)) to 200k surface stations, loss
+0.5 ||u - u_obs||^2 with u_obs from a seeded random slip, Kelvin displacement
+kernel. Backward through the FMM (autograd: adjoint matvec for dL/dslip,
+observation-gradient kernels for dL/dobs) against
+
+  * the FP64 direct sum: dL/dslip = D^T r with the direct operator's exact
+    transpose, dL/dobs by central differences of the direct sum along each axis
+    (each row depends only on its own station);
+  * central finite differences of the frozen-plan FMM loss itself, step 1e-4 for
+    the station coordinates. The loss is exactly quadratic in slip, so the slip
+    check uses a unit step (a central difference of a quadratic is exact), which
+    keeps the loss's own rounding out of the comparison: at order 75 a 1e-4 step
+    leaves ~1e-3 relative noise from the FMM's rounding, against a gradient that
+    matches the direct sum to 3e-6;
+  * the FP64 operator's gradients, for the mixed-precision operator. The stations
+    are at least 0.1 from the fault, so no leaf pair is near enough for P2P and,
+    with distinct observation and source trees, the M2L stays FP64: here the mixed
+    operator reproduces the FP64 one exactly.
+
+Order 75, leaf 128: forward 1.3e-5 from the direct sum on this geometry."""
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def model(tbem, wl):
+    torch = pytest.importorskip("torch")
+    g = wl.cfg4_fault_and_stations()
+    obs, src = g["obs"], g["src"]
+    assert len(src) == 120_000 and len(obs) == 200_000
+    K = tbem.ElasticDisplacement(1.0, 0.25)
+
+    def data(o):
+        return tbem.NBodyData(o, g["obs_normals"], src, g["src_normals"], g["src_weights"])
+
+    D = tbem.make_direct_nbody_operator(data(obs), K)
+    u_true = D.apply(g["x_true"])
+    u_dir = D.apply(g["x"])
+    r = u_dir - u_true
+    n = len(obs)
+    go_ref = np.zeros((n, 3))
+    h = 1e-6
+    for m in range(3):
+        op_, om = obs.copy(), obs.copy()
+        op_[:, m] += h
+        om[:, m] -= h
+        du = (tbem.make_direct_nbody_operator(data(op_), K).apply(g["x"]) -
+              tbem.make_direct_nbody_operator(data(om), K).apply(g["x"])) / (2 * h)
+        go_ref[:, m] = sum(r[q * n:(q + 1) * n] * du[q * n:(q + 1) * n] for q in range(3))
+    cfg = tbem.FMMConfig(0.3, 75, 128, 0.05, False)
+    ops = {p: tbem.FMMOperator(K, data(obs), cfg, precision=p) for p in ("fp64", "mixed")}
+    return dict(torch=torch, g=g, obs=obs, u_dir=u_dir, u_obs=torch.tensor(u_true, device="cuda"),
+                gs_ref=D.apply_adjoint(r), go_ref=go_ref, ops=ops, grads={})
+
+
+def grads(p, prec):
+    if prec not in p["grads"]:
+        torch = p["torch"]
+        ag = importlib.import_module("3bem_amd.autograd")
+        slip = torch.tensor(p["g"]["x"], device="cuda", requires_grad=True)
+        o = torch.tensor(p["obs"], device="cuda", requires_grad=True)
+        ag.forward_model_loss(p["ops"][prec], slip, p["u_obs"], o).backward()
+        p["grads"][prec] = (slip.grad.cpu().numpy(), o.grad.cpu().numpy())
+    return p["grads"][prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cfg4_gradients_match_fp64_direct_sum(model, prec):
+    p = model
+    print(f"CASE cfg4 full size vs direct sum, {prec}", flush=True)
+    gs, go = grads(p, prec)
+    e_u = rel_err(p["ops"][prec].apply(p["g"]["x"]), p["u_dir"])
+    e_s, e_o = rel_err(gs, p["gs_ref"]), rel_err(go, p["go_ref"])
+    print(f"  forward {e_u:.2e}, dL/dslip {e_s:.2e}, dL/dobs {e_o:.2e}")
+    # FMM approximation error of the forward is 1.3e-5 at this order
+    assert e_u < 5e-5
+    assert e_s < 2e-5
+    assert e_o < 1e-3
+
+
+def test_cfg4_mixed_gradients_match_fp64_operator(model):
+    p = model
+    print("CASE cfg4 mixed vs fp64 gradients", flush=True)
+    gs64, go64 = grads(p, "fp64")
+    gs32, go32 = grads(p, "mixed")
+    e_s, e_o = rel_err(gs32, gs64), rel_err(go32, go64)
+    print(f"  dL/dslip {e_s:.2e}, dL/dobs {e_o:.2e}")
+    assert e_s < 1e-6 and e_o < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cfg4_gradients_match_finite_differences(model, prec):
+    p = model
+    torch = p["torch"]
+    ag = importlib.import_module("3bem_amd.autograd")
+    op = p["ops"][prec]
+    obs, slip0 = p["obs"], p["g"]["x"]
+    print(f"CASE cfg4 finite differences, {prec}", flush=True)
+    gs, go = grads(p, prec)
+
+    def loss(slip, o):
+        with torch.no_grad():
+            return ag.forward_model_loss(op, torch.tensor(slip, device="cuda"), p["u_obs"],
+                                         torch.tensor(o, device="cuda")).item()
+
+    rng = np.random.default_rng(5)
+    for j in rng.choice(slip0.size, 6, replace=False):
+        sp, sm = slip0.copy(), slip0.copy()
+        sp[j] += 1.0
+        sm[j] -= 1.0
+        fd = (loss(sp, obs) - loss(sm, obs)) / 2.0
+        assert abs(fd - gs[j]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(gs).max()), (j, fd, gs[j])
+    for i in rng.choice(obs.shape[0], 8, replace=False):
+        for m in range(3):
+            h = 1e-4
+            op_, om = obs.copy(), obs.copy()
+            op_[i, m] += h
+            om[i, m] -= h
+            fd = (loss(slip0, op_) - loss(slip0, om)) / (2 * h)
+            assert abs(fd - go[i, m]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(go).max()), (i, m, fd, go[i, m])
+    op.set_obs_locs(obs)
