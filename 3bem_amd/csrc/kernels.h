@@ -283,51 +283,58 @@ __device__ __forceinline__ void kop(T dx, T dy, T dz, T r2, const T* no, const T
         return;
     }
     const T ri3 = ri2 * ri;
-    const T d[3] = {dx, dy, dz};
     if (KID == K_ET || KID == K_EA) {
         const T* n = (KID == K_ET) ? ns : no;
         const T dn = dx * n[0] + dy * n[1] + dz * n[2];
         const T nv = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
-        const T a = T(3) * dv * ri2;
         // K_ET:  -C1 ri3 [ (C2 v_k + 3 d_k dv ri2) dn - C2 (d_k nv - n_k dv) ]        (fwd)
         //        -C1 ri3 [ (C2 v_j + 3 d_j dv ri2) dn - C2 (n_j dv - d_j nv) ]        (adj)
         // K_EA:   C1 ri3 [ (C2 v_k + 3 d_k dv ri2) dn + C2 (d_k nv - n_k dv) ]        (fwd)
         //         C1 ri3 [ (C2 v_j + 3 d_j dv ri2) dn + C2 (n_j dv - d_j nv) ]        (adj)
-        const T s = (KID == K_ET) ? -ri3 : ri3;           // times C1 at the end
-        const T sg = (KID == K_ET) ? T(-1) : T(1);
-        const T sa = ADJ ? T(-1) : T(1);
-        for (int q = 0; q < 3; q++) {
-            const T base = (k.t2 * v[q] + d[q] * a) * dn;
-            const T rot = k.t2 * (d[q] * nv - n[q] * dv);
-            acc[q] += s * (base + sg * sa * rot);
-        }
+        // collected per vector: acc += cv v + cd d + cn n (C1 applied at the end)
+        const T s = (KID == K_ET) ? -ri3 : ri3;
+        constexpr int sgsa = ((KID == K_ET) ? -1 : 1) * (ADJ ? -1 : 1);
+        const T st2 = s * k.t2;
+        const T cv = st2 * dn;
+        const T cd = s * (T(3) * dv * ri2 * dn) + T(sgsa) * (st2 * nv);
+        const T cn = T(-sgsa) * (st2 * dv);
+        acc[0] += cv * v[0] + cd * dx + cn * n[0];
+        acc[1] += cv * v[1] + cd * dy + cn * n[1];
+        acc[2] += cv * v[2] + cd * dz + cn * n[2];
         return;
     }
     if (KID == K_EH) {
-        const T dr[3] = {dx * ri, dy * ri, dz * ri};
+        // r-hat = d ri; all terms are C1 ri3 times a combination of r-hat, the
+        // observation normal m, the density v and the source normal n, collected
+        // per vector: acc += ri3 (ca ri d + cm m + cv v + cn n) (C1 at the end)
         const T drdm = (dx * no[0] + dy * no[1] + dz * no[2]) * ri;
         const T drdn = (dx * ns[0] + dy * ns[1] + dz * ns[2]) * ri;
         const T nn = ns[0] * no[0] + ns[1] * no[1] + ns[2] * no[2];
         const T drv = dv * ri;
         const T nov = no[0] * v[0] + no[1] * v[1] + no[2] * v[2];
         const T nsv = ns[0] * v[0] + ns[1] * v[1] + ns[2] * v[2];
-        const T C = ri3;  // times hyp_C1 at the end
         const T d3 = T(3) * drdn;
-        for (int q = 0; q < 3; q++) {
-            T val;
-            if (!ADJ) {
-                val = d3 * (k.t2 * no[q] * drv + k.nu * (dr[q] * nov + v[q] * drdm) -
-                            T(5) * dr[q] * drv * drdm) +
-                      k.t2 * (T(3) * nsv * dr[q] * drdm + v[q] * nn + ns[q] * nov) +
-                      k.h3 * (ns[q] * drv * drdm + nn * dr[q] * drv) + k.h2 * nsv * no[q];
-            } else {
-                val = d3 * (k.t2 * nov * dr[q] + k.nu * (no[q] * drv + v[q] * drdm) -
-                            T(5) * drv * dr[q] * drdm) +
-                      k.t2 * (T(3) * ns[q] * drv * drdm + v[q] * nn + nsv * no[q]) +
-                      k.h3 * (nsv * dr[q] * drdm + nn * drv * dr[q]) + k.h2 * ns[q] * nov;
-            }
-            acc[q] += C * val;
+        const T pd = drv * drdm;
+        T ca, cm, cv, cn;
+        if (!ADJ) {
+            // d3 (t2 m drv + nu (r nov + v drdm) - 5 r drv drdm)
+            //   + t2 (3 nsv r drdm + v nn + n nov) + h3 (n drv drdm + nn r drv) + h2 nsv m
+            ca = d3 * (k.nu * nov - T(5) * pd) + T(3) * k.t2 * nsv * drdm + k.h3 * nn * drv;
+            cm = d3 * k.t2 * drv + k.h2 * nsv;
+            cv = d3 * k.nu * drdm + k.t2 * nn;
+            cn = k.t2 * nov + k.h3 * pd;
+        } else {
+            // d3 (t2 nov r + nu (m drv + v drdm) - 5 drv r drdm)
+            //   + t2 (3 n drv drdm + v nn + nsv m) + h3 (nsv r drdm + nn drv r) + h2 n nov
+            ca = d3 * (k.t2 * nov - T(5) * pd) + k.h3 * (nsv * drdm + nn * drv);
+            cm = d3 * k.nu * drv + k.t2 * nsv;
+            cv = d3 * k.nu * drdm + k.t2 * nn;
+            cn = T(3) * k.t2 * pd + k.h2 * nov;
         }
+        const T ca4 = ri3 * ri * ca, cm4 = ri3 * cm, cv4 = ri3 * cv, cn4 = ri3 * cn;
+        acc[0] += ca4 * dx + cm4 * no[0] + cv4 * v[0] + cn4 * ns[0];
+        acc[1] += ca4 * dy + cm4 * no[1] + cv4 * v[1] + cn4 * ns[1];
+        acc[2] += ca4 * dz + cm4 * no[2] + cv4 * v[2] + cn4 * ns[2];
         return;
     }
 }

@@ -9,6 +9,19 @@ namespace tbem {
 // threads per workgroup (multiples of the 64-lane wavefront)
 constexpr int kSurfBlock = 128;  // surface-target stages (P2M, M2M, M2L, P2L, L2L)
 constexpr int kLeafBlock = 512;  // point-target stage (P2P + M2P + L2P)
+// Workgroup width of the point-target stage per kernel. The hypersingular kernel
+// needs 150-174 VGPRs: 256-thread workgroups capped at 168 VGPRs (three waves per
+// SIMD) hold 12 waves per CU instead of one 512-thread workgroup's 8, which measured
+// 1.1x faster on 2M points; the traction kernels measured no better that way.
+template <int KID>
+constexpr int leaf_block() {
+    return KID == 6 ? 256 : kLeafBlock;
+}
+// minimum resident waves per SIMD the compiler must allow (caps the VGPRs)
+template <int KID>
+constexpr int leaf_waves() {
+    return KID == 6 ? 3 : 1;
+}
 constexpr int kGradBlock = 256;  // observation-coordinate gradient of the point-target stage
 
 struct TreeView {

@@ -238,6 +238,58 @@ def cfg4_run(tbem, torch, steps):
     return out
 
 
+def cfg5_run(tbem, torch):
+    """The BEM solve: elastic hypersingular FMM on the rough fault (2M points, order
+    50, leaf 128, MAC 0.3) as the second-kind system (alpha I + H) x = b, alpha =
+    2 ||H|| from 8 power iterations, b the traction of a uniform unit remote stress.
+    Plain FP64 GMRES(50) to a 1e-8 relative residual against FP64-residual iterative
+    refinement around the mixed-precision FMM (inner GMRES to 1e-4), both timed end
+    to end with the operators already built."""
+    wl = importlib.import_module("3bem_amd.workloads")
+    gm = importlib.import_module("3bem_amd.gmres")
+    n = 2_000_000
+    g = wl.cfg5_rough_fault(n)
+    K = tbem.ElasticHypersingular(1.0, 0.25)
+    data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
+    cfg = tbem.FMMConfig(0.3, 50, 128, 0.05, False)
+    op64 = tbem.FMMOperator(K, data, cfg, precision="fp64")
+    op32 = tbem.FMMOperator(K, data, cfg, precision="mixed")
+    v = torch.randn(3 * n, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    for _ in range(8):
+        v = op32.apply(v)
+        v = v / torch.linalg.norm(v)
+    alpha = 2.0 * float(torch.linalg.norm(op64.apply(v)))
+    b = torch.tensor(g["normals"].T.reshape(-1).copy(), device="cuda")
+    out = {"workload": "elastic hypersingular BEM solve, 2,000,000 points on the rough fault, order 50, leaf 128, "
+                       "MAC 0.3, (alpha I + H) x = b, alpha = 2||H||, to a 1e-8 relative FP64 residual"}
+    for name, op in (("fp64", op64), ("mixed", op32)):
+        dt, _ = time_steps(lambda: op.apply(b), 3, 1, torch)
+        out[f"{name}_matvec_ms"] = round(dt * 1e3, 2)
+    A64 = lambda x: alpha * x + op64.apply(x)
+    A32 = lambda x: alpha * x + op32.apply(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x64, i64 = gm.gmres(A64, b, tol=1e-8, restart=50, maxiter=500)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x, info = gm.refine(A64, A32, b, tol=1e-8, inner_tol=1e-4, restart=50)
+    torch.cuda.synchronize()
+    tr = time.perf_counter() - t0
+    bn = float(torch.linalg.norm(b))
+    out["fp64_gmres"] = {"s": round(t64, 3), "iterations": i64["iterations"],
+                         "residual": float(torch.linalg.norm(b - A64(x64))) / bn}
+    out["mixed_refinement"] = {"s": round(tr, 3), "outer": info["outer"], "inner_iterations": info["inner_iterations"],
+                               "residual": float(torch.linalg.norm(b - A64(x))) / bn,
+                               "rel_diff_vs_fp64_solution": float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))}
+    out["speedup_refinement_vs_fp64_gmres"] = round(t64 / tr, 3)
+    log(f"[bench] cfg5: fp64 GMRES {t64:.2f} s ({i64['iterations']} it), refinement {tr:.2f} s "
+        f"({info['outer']} outer, {info['inner_iterations']} inner)")
+    del op64, op32
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +306,8 @@ def main():
                     help="skip the 4M-point Laplace FP32/MFMA-vs-FP64 comparison")
     ap.add_argument("--skip-cfg4", action="store_true",
                     help="skip the differentiable forward model (120k fault points, 200k stations)")
+    ap.add_argument("--skip-cfg5", action="store_true",
+                    help="skip the 2M-point hypersingular GMRES solve")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -358,6 +412,8 @@ def main():
         out["laplace_4M_fp32_mfma_vs_fp64"] = laplace_run(tbem, torch, 3)
     if not args.skip_extra and not args.skip_cfg4 and args.config == "cfg2":
         out["forward_model_cfg4"] = cfg4_run(tbem, torch, 5)
+    if not args.skip_extra and not args.skip_cfg5 and args.config == "cfg2":
+        out["bem_solve_cfg5"] = cfg5_run(tbem, torch)
 
     if not args.skip_cpu:
         cfg_t = (cfg.mac, cfg.order, cfg.min_pts_per_cell, cfg.d, cfg.account_for_small_cells)
