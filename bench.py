@@ -188,16 +188,17 @@ def laplace_run(tbem, torch, steps):
 
 def cfg4_run(tbem, torch, steps):
     """The differentiable elastic forward model: 40k fault triangles (120k points)
-    to 200k surface stations, loss 0.5 ||u - u_obs||^2, order 75, leaf 128, MAC
-    0.3. Forward and backward (adjoint + observation gradient) times, and the
-    autograd gradients against the FP64 direct sum (dL/dslip = D^T r; dL/dobs by
-    central differences of the direct sum, step 1e-6)."""
+    to 200k surface stations through the elastic traction (double-layer) kernel
+    with the fault normals, loss 0.5 ||u - u_obs||^2, order 150, leaf 256, MAC 0.3.
+    Forward and backward (adjoint + observation gradient) times, and the autograd
+    gradients against the FP64 direct sum (dL/dslip = D^T r; dL/dobs by central
+    differences of the direct sum, step 1e-6)."""
     wl = importlib.import_module("3bem_amd.workloads")
     ag = importlib.import_module("3bem_amd.autograd")
     g = wl.cfg4_fault_and_stations()
     obs, src = g["obs"], g["src"]
     n = len(obs)
-    K = tbem.ElasticDisplacement(1.0, 0.25)
+    K = tbem.ElasticTraction(1.0, 0.25)
 
     def data(o):
         return tbem.NBodyData(o, g["obs_normals"], src, g["src_normals"], g["src_weights"])
@@ -218,9 +219,10 @@ def cfg4_run(tbem, torch, steps):
     x = torch.tensor(g["x"], device="cuda")
     gv = torch.tensor(r, device="cuda")
     out = {"workload": "differentiable elastic forward model: 40,000 fault triangles (120,000 points) to 200,000 "
-                       "surface stations, loss 0.5||u - u_obs||^2, order 75, leaf 128, MAC 0.3"}
+                       "surface stations, elastic traction kernel, loss 0.5||u - u_obs||^2, order 150, leaf 256, "
+                       "MAC 0.3"}
     for prec in ("fp64", "mixed"):
-        op = tbem.FMMOperator(K, data(obs), tbem.FMMConfig(0.3, 75, 128, 0.05, False), precision=prec)
+        op = tbem.FMMOperator(K, data(obs), tbem.FMMConfig(0.3, 150, 256, 0.05, False), precision=prec)
         dt, _ = time_steps(lambda: op.apply(x), steps, 1, torch)
         dtb, _ = time_steps(lambda: (op.apply_adjoint(gv), op.obs_grad(x, gv)), steps, 1, torch)
         slip = torch.tensor(g["x"], device="cuda", requires_grad=True)
