@@ -23,8 +23,8 @@
 // wave runs row tiles of S_c:
 // v_mfma_f32_16x16x4f32 with A from global (coalesced 256 B per wave and k-step,
 // L2-resident) and B from LDS, FP32 accumulation over k = blk. Results go to a
-// per-pair buffer Y (no atomics); a second kernel sums each target's pairs in FP64
-// and adds them to its down-equivalent density.
+// per-pair buffer Y (no atomics); k_m2l_pair_sum sums each target's pairs in FP64,
+// and a batched per-level GEMM adds Vd times the sums to the down-equivalent densities.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -42,7 +42,7 @@ constexpr int kUnitCols = 64;   // pairs per work unit (= LDS row length of the 
 constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 KiB of LDS
 // Below this operator size the FP64 M2L stage is the cheaper one: measured on the
 // 1M-point elastic workload at order 6 (blk 18), the MFMA path with its per-pair
-// buffer and modal epilogues took longer than the 0.4 ms FP64 stage; at blk 60
+// buffer and modal products took longer than the 0.4 ms FP64 stage; at blk 60
 // (order 20) it is already faster (2M points: 3.0 vs 5.2 ms).
 constexpr int kMinBlk = 48;
 constexpr int kXcds = 8;

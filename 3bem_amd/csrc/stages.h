@@ -1,6 +1,8 @@
 // Device views and launchers of the FMM stage kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "plan.h"
 
@@ -9,18 +11,28 @@ namespace tbem {
 // threads per workgroup (multiples of the 64-lane wavefront)
 constexpr int kSurfBlock = 128;  // surface-target stages (P2M, M2M, M2L, P2L, L2L)
 constexpr int kLeafBlock = 512;  // point-target stage (P2P + M2P + L2P)
-// Workgroup width of the point-target stage per kernel. The hypersingular kernel
-// needs 150-174 VGPRs: 256-thread workgroups capped at 168 VGPRs (three waves per
-// SIMD) hold 12 waves per CU instead of one 512-thread workgroup's 8, which measured
-// 1.1x faster on 2M points; the traction kernels measured no better that way.
-template <int KID>
-constexpr int leaf_block() {
-    return KID == 6 ? 256 : kLeafBlock;
+// Shape of the point-target stage per kernel and precision: workgroup width and
+// the minimum resident waves per SIMD the compiler must allow (which caps the
+// VGPRs at 512 / waves). Measured on 2M points:
+//  * hypersingular (150-174 VGPRs uncapped) and the FP32 traction kernels (156-163,
+//    which spill under a 128 cap): 256-thread workgroups at three waves (<= 168
+//    VGPRs, 12 waves per CU instead of one 512-thread workgroup's 8);
+//  * FP64 traction kernels: 512 threads at four waves (125 VGPRs, from 148-152; two
+//    workgroups per CU): adjoint-traction forward 75 -> 65 ms, traction adjoint 84
+//    -> 74 ms;
+//  * the rest (<= 128 VGPRs already): 512 threads, no cap; a cap at four waves made
+//    the FP64 elastic-U leaf 16% slower (101 VGPRs, less latency hiding).
+template <int KID, class T>
+constexpr bool leaf_narrow() {
+    return KID == 6 || ((KID == 4 || KID == 5) && std::is_same<T, float>::value);
 }
-// minimum resident waves per SIMD the compiler must allow (caps the VGPRs)
-template <int KID>
+template <int KID, class T>
+constexpr int leaf_block() {
+    return leaf_narrow<KID, T>() ? 256 : kLeafBlock;
+}
+template <int KID, class T>
 constexpr int leaf_waves() {
-    return KID == 6 ? 3 : 1;
+    return leaf_narrow<KID, T>() ? 3 : ((KID == 4 || KID == 5) ? 4 : 1);
 }
 constexpr int kGradBlock = 256;  // observation-coordinate gradient of the point-target stage
 
@@ -60,7 +72,7 @@ struct SurfStage {
     int n_src_pts;
     const double* unit;     // unit sphere = surface normals
     int n_e, blk;
-    double* out;            // blk per cell of tt
+    double* out;            // blk per cell of tt: the raw check potentials (= or += with add)
 };
 
 // Tiles of at most 64 cells of one level for the batched per-level products (the

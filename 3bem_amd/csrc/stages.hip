@@ -383,11 +383,12 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
 // sources accumulate in T (acA, acB); surface sources in FP64 (sfA, sfB, which
 // are acA, acB themselves when T is double). Register budget: at 512 threads per
 // workgroup, <= 128 VGPRs keeps two workgroups per CU; one more register halves
-// the resident waves (the elastic-U FP32 instance at 139 VGPRs ran 1.5x slower).
+// the resident waves (the elastic-U FP32 instance at 139 VGPRs ran 1.5x slower),
+// so leaf_waves() makes the compiler hold that budget (leaf_block() for the rest).
 template <int KID, class T, bool ADJ, bool FAST>
-__global__ __launch_bounds__(leaf_block<KID>(), leaf_waves<KID>()) void k_leaf_stage(LeafStage a, KC<T> kc, KC<double> kc64,
+__global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) void k_leaf_stage(LeafStage a, KC<T> kc, KC<double> kc64,
                                                                   double scale) {
-    constexpr int NB = leaf_block<KID>();
+    constexpr int NB = leaf_block<KID, T>();
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
     constexpr bool SPLIT = !std::is_same<T, double>::value;
@@ -752,7 +753,7 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
     const KC<T> kc = kc_of<T>(c);
     const KC<double> kc64 = make_kc64(c);
     const double scale = (double)kop_scale<KID, double>(kc64);
-    dim3 g(a.n_leaves), b(leaf_block<KID>());
+    dim3 g(a.n_leaves), b(leaf_block<KID, T>());
     // the one-step rsqrt only differs from the full one in FP64
     if (fast && std::is_same<T, double>::value) {
         if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, true>), g, b, 0, s, a, kc, kc64, scale);
