@@ -478,25 +478,23 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
 
 // ------------------------------------------------- observation-coordinate gradient
 // Forward leaf stage (P2P + M2P + L2P) differentiated with respect to the target
-// coordinates; writes grad[3*i + m] = sum_q g_q(i) d u_q(i) / d x_i[m]. Closed
-// forms for the kernels that have them (grad_pair), dual numbers otherwise. With
-// T = float (mixed precision, closed-form kernels only) the point sources run in
-// FP32 from FP64 displacements with Neumaier-compensated per-tile sums; surface
-// sources stay FP64.
-template <int KID, class T>
-__global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, KC<T> kct, double scale) {
+// coordinates; writes grad[3*i + m] = sum_q g_q(i) d u_q(i) / d x_i[m]. This
+// kernel serves the kernels without a closed-form gradient (adjoint traction,
+// hypersingular, identities), in FP64 dual numbers, one target per lane;
+// k_leaf_grad2 below serves the others.
+template <int KID>
+__global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc, double scale) {
+    static_assert(!has_analytic_grad(KID), "closed-form kernels use k_leaf_grad2");
     constexpr int NB = kGradBlock;
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
-    constexpr bool SPLIT = !std::is_same<T, double>::value;
-    __shared__ StageSmem<T, R, NRM, NB, 3> sm;
+    __shared__ StageSmem<double, R, NRM, NB, 3> sm;
     const int pos = blockIdx.x;
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
     const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
     const int se0 = a.sl_ptr ? uni(a.sl_ptr[pos]) : 0, se1 = a.sl_ptr ? uni(a.sl_ptr[pos + 1]) : 0;
     const int tid = threadIdx.x;
-    const KC<double> kc64 = make_kc64_dev(kc);
     for (int t0 = 0; t0 < m; t0 += NB) {
         const int nt = min(NB, m - t0);
         const int ntp = nt, S = NB / ntp;
@@ -505,79 +503,160 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
         const int i = b + t0 + tg;
         const D3 tx(a.tp.x[i], 1, 0, 0), ty(a.tp.y[i], 0, 1, 0), tz(a.tp.z[i], 0, 0, 1);
         const D3 tn[3] = {D3(NRM ? a.tp.nx[i] : 0.0), D3(NRM ? a.tp.ny[i] : 0.0), D3(NRM ? a.tp.nz[i] : 0.0)};
-        double gt[R], G[3] = {0, 0, 0};
-        T gtT[R];
-        for (int q = 0; q < R; q++) {
-            gt[q] = a.grad_g[(size_t)q * a.n_tgt + i];
-            gtT[q] = T(gt[q]);
-        }
+        double gt[R];
+        for (int q = 0; q < R; q++) gt[q] = a.grad_g[(size_t)q * a.n_tgt + i];
         drain_loads();
         D3 acc[R];
-        Acc<T, 3> GP;
-        GP.init();
-        // one FP64 source j of tile tl (analytic or dual numbers)
-        auto pair64 = [&](const auto& tl, int j) {
-            if (has_analytic_grad(KID)) {
-                const double sn[3] = {NRM ? (double)tl.nx[j] : 0.0, NRM ? (double)tl.ny[j] : 0.0,
-                                      NRM ? (double)tl.nz[j] : 0.0};
-                double v[R];
-                for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
-                const double dx = tl.x[j] - tx.v, dy = tl.y[j] - ty.v, dz = tl.z[j] - tz.v;
-                grad_pair<KID, double>(dx, dy, dz, dx * dx + dy * dy + dz * dz, sn, v, gt, G, kc64);
-            } else {
-                const D3 sn[3] = {D3(NRM ? (double)tl.nx[j] : 0.0), D3(NRM ? (double)tl.ny[j] : 0.0),
-                                  D3(NRM ? (double)tl.nz[j] : 0.0)};
-                D3 v[R];
-                for (int q = 0; q < R; q++) v[q] = D3((double)tl.d[q][j]);
-                const D3 dx = D3(tl.x[j]) - tx, dy = D3(tl.y[j]) - ty, dz = D3(tl.z[j]) - tz;
-                const D3 r2 = dx * dx + dy * dy + dz * dz;
-                kop<KID, false, D3>(dx, dy, dz, r2, tn, sn, v, acc, kc);
-            }
+        // one FP64 source j of tile tl, differentiated by the dual parts of the target
+        auto pair_d = [&](const Tile<double, R, NRM, NB>& tl, int j) {
+            const D3 sn[3] = {D3(NRM ? tl.nx[j] : 0.0), D3(NRM ? tl.ny[j] : 0.0), D3(NRM ? tl.nz[j] : 0.0)};
+            D3 v[R];
+            for (int q = 0; q < R; q++) v[q] = D3(tl.d[q][j]);
+            const D3 dx = D3(tl.x[j]) - tx, dy = D3(tl.y[j]) - ty, dz = D3(tl.z[j]) - tz;
+            const D3 r2 = dx * dx + dy * dy + dz * dz;
+            kop<KID, false, D3>(dx, dy, dz, r2, tn, sn, v, acc, kc);
         };
         auto points = [&](int cnt) {
             if (!active) return;
-            const Tile<T, R, NRM, NB>& tl = sm.pt;
-            for (int j = slice; j < cnt; j += S) {
-                if constexpr (SPLIT) {
-                    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
-                    T v[R];
-                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
-                    const T dx = T(tl.x[j] - tx.v), dy = T(tl.y[j] - ty.v), dz = T(tl.z[j] - tz.v);
-                    grad_pair<KID, T>(dx, dy, dz, dx * dx + dy * dy + dz * dz, sn, v, gtT, GP.tile(), kct);
-                } else {
-                    pair64(tl, j);
-                }
-            }
-            GP.flush();
+            for (int j = slice; j < cnt; j += S) pair_d(sm.pt, j);
         };
         auto surfaces = [&](int cnt) {
             if (!active) return;
-            for (int j = slice; j < cnt; j += S) pair64(sm.st, j);
+            for (int j = slice; j < cnt; j += S) pair_d(sm.st, j);
         };
         stream_ranges_packed(sm.pt, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
                              a.n_ps, points);
         stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, surfaces);
         if (a.self_coeff)
             stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, surfaces);
-        double gr[3] = {G[0], G[1], G[2]};
-        if (SPLIT)
-            for (int mm = 0; mm < 3; mm++) gr[mm] += GP.value(mm);
-        if (active && !has_analytic_grad(KID)) {
+        double gr[3] = {0, 0, 0};
+        if (active) {
             for (int q = 0; q < R; q++) {
                 gr[0] += gt[q] * acc[q].d0;
                 gr[1] += gt[q] * acc[q].d1;
                 gr[2] += gt[q] * acc[q].d2;
             }
         }
-        if (!active) gr[0] = gr[1] = gr[2] = 0;
         __syncthreads();
         for (int mm = 0; mm < 3; mm++) sm.red[mm * NB + tid] = gr[mm];
         __syncthreads();
         if (slice == 0) {
             for (int mm = 0; mm < 3; mm++) {
-                double s = 0;
-                for (int k = 0; k < S; k++) s += sm.red[mm * NB + k * ntp + tg];
-                a.grad_out[3 * (size_t)i + mm] = s * scale;
+                double sum = 0;
+                for (int k = 0; k < S; k++) sum += sm.red[mm * NB + k * ntp + tg];
+                a.grad_out[3 * (size_t)i + mm] = sum * scale;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The same gradient for the kernels with closed forms, two targets per lane (as
+// k_leaf_stage: slots tg and nslot + tg, a lane without a second target repeats
+// the first and drops it), so one LDS read of a source feeds two pairs. Point
+// sources in T with Neumaier-compensated per-tile sums when T is float; surface
+// sources FP64.
+// Four waves per SIMD (<= 128 VGPRs) except the FP32 elastic-U instance, which
+// spills under that cap (measured 2M points, order 100: 30.1 ms uncapped against
+// 33.5 capped; the FP64 and traction instances are 1-5% faster capped).
+template <int KID, class T>
+constexpr int grad_waves() {
+    return (KID == K_EU && std::is_same<T, float>::value) ? 1 : 4;
+}
+
+template <int KID, class T>
+__global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_grad2(LeafStage a, KC<double> kc64,
+                                                                                 KC<T> kct, double scale) {
+    static_assert(has_analytic_grad(KID), "closed-form gradient kernels only");
+    constexpr int NB = kGradBlock;
+    constexpr int R = kernel_dim(KID);
+    constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
+    __shared__ StageSmem<T, R, NRM, NB, 6> sm;
+    const int pos = blockIdx.x;
+    const int leaf = uni(a.leaves[pos]);
+    const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
+    const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
+    const int se0 = a.sl_ptr ? uni(a.sl_ptr[pos]) : 0, se1 = a.sl_ptr ? uni(a.sl_ptr[pos + 1]) : 0;
+    const int tid = threadIdx.x;
+    for (int t0 = 0; t0 < m; t0 += 2 * NB) {
+        const int nt = min(2 * NB, m - t0);
+        const int nslot = (nt + 1) >> 1;
+        const int ntp = nslot, S = NB / ntp;  // lanes >= ntp*S sit out
+        const int tg = tid % ntp, slice = tid / ntp;
+        const bool va = slice < S, vb = va && nslot + tg < nt;
+        const int ia = b + t0 + tg;
+        const int ib = b + t0 + (vb ? nslot + tg : tg);
+        const double ax = a.tp.x[ia], ay = a.tp.y[ia], az = a.tp.z[ia];
+        const double bx = a.tp.x[ib], by = a.tp.y[ib], bz = a.tp.z[ib];
+        double gA[R], gB[R], GA[3] = {0, 0, 0}, GB[3] = {0, 0, 0};
+        T gAT[R], gBT[R];
+        for (int q = 0; q < R; q++) {
+            gA[q] = a.grad_g[(size_t)q * a.n_tgt + ia];
+            gB[q] = a.grad_g[(size_t)q * a.n_tgt + ib];
+            gAT[q] = T(gA[q]);
+            gBT[q] = T(gB[q]);
+        }
+        drain_loads();
+        Acc<T, 3> PA, PB;
+        PA.init();
+        PB.init();
+        auto points = [&](int cnt) {
+            if (va) {
+                const Tile<T, R, NRM, NB>& tl = sm.pt;
+                for (int j = slice; j < cnt; j += S) {
+                    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
+                    T v[R];
+                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                    const double sx = tl.x[j], sy = tl.y[j], sz = tl.z[j];
+                    const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
+                    const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
+                    if constexpr (SPLIT) {
+                        grad_pair<KID, T>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
+                        grad_pair<KID, T>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
+                    } else {
+                        grad_pair<KID, double>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                        grad_pair<KID, double>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+                    }
+                }
+            }
+            PA.flush();
+            PB.flush();
+        };
+        auto surfaces = [&](int cnt) {
+            if (!va) return;
+            const Tile<double, R, NRM, NB>& tl = sm.st;
+            for (int j = slice; j < cnt; j += S) {
+                const double sn[3] = {NRM ? tl.nx[j] : 0.0, NRM ? tl.ny[j] : 0.0, NRM ? tl.nz[j] : 0.0};
+                double v[R];
+                for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                const double dxa = tl.x[j] - ax, dya = tl.y[j] - ay, dza = tl.z[j] - az;
+                const double dxb = tl.x[j] - bx, dyb = tl.y[j] - by, dzb = tl.z[j] - bz;
+                grad_pair<KID, double>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                grad_pair<KID, double>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+            }
+        };
+        stream_ranges_packed(sm.pt, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
+                             a.n_ps, points);
+        stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, surfaces);
+        if (a.self_coeff)
+            stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, surfaces);
+        __syncthreads();
+        for (int mm = 0; mm < 3; mm++) {
+            const double pa = SPLIT ? PA.value(mm) : 0.0, pb = SPLIT ? PB.value(mm) : 0.0;
+            sm.red[mm * NB + tid] = va ? GA[mm] + pa : 0.0;
+            sm.red[(3 + mm) * NB + tid] = vb ? GB[mm] + pb : 0.0;
+        }
+        __syncthreads();
+        if (slice == 0) {
+            for (int h = 0; h < 2; h++) {
+                if (h == 1 && nslot + tg >= nt) continue;
+                const int i = b + t0 + h * nslot + tg;
+                for (int mm = 0; mm < 3; mm++) {
+                    double sum = 0;
+                    for (int k = 0; k < S; k++) sum += sm.red[(h * 3 + mm) * NB + k * ntp + tg];
+                    a.grad_out[3 * (size_t)i + mm] = sum * scale;
+                }
             }
         }
         __syncthreads();
@@ -769,12 +848,16 @@ template <int KID, class T>
 static void grad_launch(bool fp32_points, const LeafStage& a, const KConst& c, hipStream_t s) {
     if (a.n_leaves == 0) return;
     const double scale = (double)kop_scale<KID, double>(make_kc64(c));
-    if (fp32_points && has_analytic_grad(KID))
-        hipLaunchKernelGGL((k_leaf_grad<KID, float>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c),
-                           make_kc32(c), scale);
-    else
-        hipLaunchKernelGGL((k_leaf_grad<KID, double>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c),
-                           make_kc64(c), scale);
+    if constexpr (has_analytic_grad(KID)) {
+        if (fp32_points)
+            hipLaunchKernelGGL((k_leaf_grad2<KID, float>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kc64(c),
+                               make_kc32(c), scale);
+        else
+            hipLaunchKernelGGL((k_leaf_grad2<KID, double>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a,
+                               make_kc64(c), make_kc64(c), scale);
+    } else {
+        hipLaunchKernelGGL((k_leaf_grad<KID>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c), scale);
+    }
     TBEM_LAUNCH_CHECK();
 }
 
