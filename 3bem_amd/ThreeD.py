@@ -253,14 +253,26 @@ class FMMOperator:
         return self._run(self._lib.tbem_fmm_apply_adjoint, self._lib.tbem_fmm_apply_adjoint_dev, g,
                          self.n_rows(), self.n_cols())
 
-    def obs_grad(self, x, g):
-        """(N_obs, 3) array: d/d obs_i of sum_k g_k(i) u_k(i), u = apply(x); plan held fixed."""
+    def forward_token(self):
+        """Token of the far field the operator holds after its last apply (0: none)."""
+        return int(self._lib.tbem_fmm_forward_token(self._h))
+
+    def obs_grad(self, x, g, token=None):
+        """(N_obs, 3) array: d/d obs_i of sum_k g_k(i) u_k(i), u = apply(x); plan held fixed.
+        token: forward_token() of the apply(x) this gradient belongs to; while no other
+        apply has run on this operator since, its far field is reused instead of
+        recomputed (device tensors only; x must be that apply's input)."""
         if _is_torch(x):
             import torch
             out = torch.empty(3 * self.data.n_obs, dtype=torch.float64, device=x.device)
-            _lib.check(self._lib.tbem_fmm_obs_grad_dev(
-                self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(g.data_ptr()),
-                ctypes.c_void_p(out.data_ptr()), _torch_stream()))
+            if token:
+                _lib.check(self._lib.tbem_fmm_obs_grad_reuse_dev(
+                    self._h, ctypes.c_uint64(token), ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+                    ctypes.c_void_p(out.data_ptr()), _torch_stream()))
+            else:
+                _lib.check(self._lib.tbem_fmm_obs_grad_dev(
+                    self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(g.data_ptr()),
+                    ctypes.c_void_p(out.data_ptr()), _torch_stream()))
             return out.view(-1, 3)
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
         g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1)

@@ -18,6 +18,7 @@ _VP = ctypes.c_void_p
 EXPORTED = (
     "tbem_fmm_create", "tbem_fmm_apply", "tbem_fmm_apply_dev", "tbem_fmm_apply_adjoint",
     "tbem_fmm_apply_adjoint_dev", "tbem_fmm_obs_grad", "tbem_fmm_obs_grad_dev",
+    "tbem_fmm_forward_token", "tbem_fmm_obs_grad_reuse_dev",
     "tbem_fmm_set_obs_locs", "tbem_fmm_n_rows", "tbem_fmm_n_cols", "tbem_fmm_clone",
     "tbem_fmm_destroy", "tbem_fmm_info", "tbem_fmm_c2e", "tbem_fmm_pairs", "tbem_fmm_octree",
     "tbem_fmm_stage_times", "tbem_fmm_m2l_ms", "tbem_nbody_create", "tbem_nbody_apply", "tbem_nbody_apply_dev",
@@ -88,6 +89,8 @@ def load(path=LIB_PATH):
         "tbem_fmm_apply_adjoint_dev": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tbem_fmm_obs_grad": (ctypes.c_int, [_VP, _D, _D, _D]),
         "tbem_fmm_obs_grad_dev": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+        "tbem_fmm_forward_token": (ctypes.c_uint64, [_VP]),
+        "tbem_fmm_obs_grad_reuse_dev": (ctypes.c_int, [_VP, ctypes.c_uint64, _VP, _VP, _VP, _VP]),
         "tbem_fmm_set_obs_locs": (ctypes.c_int, [_VP, _D]),
         "tbem_fmm_n_rows": (_SZ, [_VP]),
         "tbem_fmm_n_cols": (_SZ, [_VP]),

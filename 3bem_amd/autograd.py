@@ -8,7 +8,10 @@ Backward (3bem has no autograd; this is the hand-written backward of apply()):
                                 (FMMOperator.apply_adjoint: every stage transposed,
                                 source and target roles swapped);
   * dL/dobs_locs = obs_grad   -- gradient kernels for the near field and the
-                                local expansions (FMMOperator.obs_grad).
+                                local expansions (FMMOperator.obs_grad). The
+                                forward's multipole and local expansions are
+                                reused when no other apply has run on the operator
+                                in between, and recomputed otherwise.
 The plan (trees, surfaces, interaction lists) is held fixed: moving observation
 points only changes the point coordinates inside their cells, so the forward is
 a smooth function of them and central differences of the frozen-plan forward
@@ -25,7 +28,9 @@ class _FMMMatvec(torch.autograd.Function):
         ctx.op = op
         ctx.save_for_backward(x)
         ctx.has_obs = obs_locs is not None
-        return op.apply(x.contiguous())
+        y = op.apply(x.contiguous())
+        ctx.token = op.forward_token()
+        return y
 
     @staticmethod
     def backward(ctx, g):
@@ -35,7 +40,7 @@ class _FMMMatvec(torch.autograd.Function):
         gx = op.apply_adjoint(g) if ctx.needs_input_grad[0] else None
         gobs = None
         if ctx.has_obs and ctx.needs_input_grad[1]:
-            gobs = op.obs_grad(x.contiguous(), g)
+            gobs = op.obs_grad(x.contiguous(), g, token=ctx.token)
         return gx, gobs, None
 
 

@@ -75,6 +75,10 @@ struct Work {
     hipEvent_t ev[10];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
+    // far-field state: dens (of x), mult and locals hold the far field of the call
+    // that set far_token (a forward or a recomputing gradient; 0: none). The adjoint
+    // leaves mult and locals alone but overwrites dens.
+    uint64_t far_token = 0, next_token = 0;
     explicit Work(const FmmPlan& P) {
         TBEM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto& e: ev) TBEM_HIP(hipEventCreate(&e));
@@ -437,6 +441,7 @@ static void forward_T(const FmmPlan& P, Work& W, const double* x, double* y, hip
         return;
     }
     farfield(P, W, x, s, std::is_same<T, float>::value && P.m2l_mfma.enabled);
+    W.far_token = ++W.next_token;
     launch_leaf_stage<T>(P.kid, false, leaf_stage_args(P, W), P.kc, s, true);
     TBEM_HIP(hipEventRecord(W.ev[6], s));
     launch_scatter_out(W.out_sorted.p, P.op.perm.p, No, R, y, s);
@@ -455,8 +460,11 @@ __global__ void k_scatter_aos3(const double* __restrict__ in, const int* __restr
 // grad[3*i+m] = sum_q g_q(i) d u_q(i) / d obs_i[m], plan geometry held fixed. The
 // far field is the operator's own; in mixed precision the gradient leaf stage runs
 // its point sources in FP32.
+// reuse_token: a far_token of this handle; while it is still current, the far
+// field (mult, locals) of that call is reused and only the densities of x are
+// gathered again (x must then hold that call's input).
 static void obs_grad(const FmmPlan& P, Work& W, const double* x, const double* g, double* grad,
-                     hipStream_t s) {
+                     hipStream_t s, uint64_t reuse_token = 0) {
     const int R = P.R;
     const int Ns = (int)P.n_src, No = (int)P.n_obs;
     if (No == 0) return;
@@ -465,7 +473,12 @@ static void obs_grad(const FmmPlan& P, Work& W, const double* x, const double* g
         return;
     }
     // the mixed far field (MFMA M2L) agrees with the FP64 one to ~2e-7
-    farfield(P, W, x, s, P.precision == TBEM_MIXED && P.m2l_mfma.enabled);
+    if (reuse_token != 0 && reuse_token == W.far_token) {
+        launch_gather_dens(x, P.sp.perm.p, P.sp.w.p, Ns, R, W.dens.p, s);
+    } else {
+        farfield(P, W, x, s, P.precision == TBEM_MIXED && P.m2l_mfma.enabled);
+        W.far_token = ++W.next_token;
+    }
     if (W.gsort.n < (size_t)R * No) W.gsort.alloc((size_t)R * No);
     if (W.grad_sorted.n < 3 * (size_t)No) W.grad_sorted.alloc(3 * (size_t)No);
     launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
@@ -697,6 +710,24 @@ int tbem_fmm_obs_grad_dev(tbem_fmm_t h, const double* d_x, const double* d_g, do
         hipStream_t s = (hipStream_t)stream;
         h->work->begin(s);
         obs_grad(*h->plan, *h->work, d_x, d_g, d_grad, s);
+        h->work->end(s);
+    });
+}
+
+uint64_t tbem_fmm_forward_token(tbem_fmm_t h) {
+    if (!h) return 0;
+    std::lock_guard<std::mutex> lk(h->mu);
+    return h->work->far_token;
+}
+
+int tbem_fmm_obs_grad_reuse_dev(tbem_fmm_t h, uint64_t token, const double* d_x, const double* d_g,
+                                double* d_grad, void* stream) {
+    return guard([&] {
+        TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
+        std::lock_guard<std::mutex> lk(h->mu);
+        hipStream_t s = (hipStream_t)stream;
+        h->work->begin(s);
+        obs_grad(*h->plan, *h->work, d_x, d_g, d_grad, s, token);
         h->work->end(s);
     });
 }

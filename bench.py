@@ -92,6 +92,32 @@ def time_steps(fn, steps, warmup, torch, after=None):
     return (time.perf_counter() - t0) / steps, per
 
 
+def backward_times(op, x, g, steps, torch):
+    """The backward of y = op.apply(x) as autograd runs it: after an untimed forward,
+    the adjoint A^T g and the observation gradient, which reuses that forward's
+    expansions (forward token). Returns (adjoint s, obs_grad s, obs_grad s with the
+    expansions recomputed, the path taken when another apply intervened), medians
+    over steps after one warmup."""
+    adj, grad, recomp = [], [], []
+    for _ in range(steps + 1):
+        op.apply(x)
+        tok = op.forward_token()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.apply_adjoint(g)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        op.obs_grad(x, g, token=tok)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        op.obs_grad(x, g)
+        torch.cuda.synchronize()
+        adj.append(t1 - t0)
+        grad.append(t2 - t1)
+        recomp.append(time.perf_counter() - t2)
+    return tuple(statistics.median(v[1:]) for v in (adj, grad, recomp))
+
+
 def cpu_baseline(kernel, cfg, budget_n):
     """Compiled reference FMM (oracle/_ref) on a bounded sample of the same geometry;
     falls back to the oracle restatement when the reference build is absent."""
@@ -141,11 +167,11 @@ def compare_run(tbem, torch, steps, g, kid, cfg_t, n_sub, desc):
         info = op.info()
         stage_ms = {k: round(v, 3) for k, v in op.stage_times().items()}
         gv = torch.randn(R * n, dtype=torch.float64, device="cuda")
-        dt_adj, _ = time_steps(lambda: op.apply_adjoint(gv), 2, 1, torch)
-        dt_grad, _ = time_steps(lambda: op.obs_grad(x, gv), 2, 1, torch)
+        dt_adj, dt_grad, dt_grad_re = backward_times(op, x, gv, 2, torch)
         out[prec] = {"ms_per_step": round(dt * 1e3, 3), "Mpts_per_s": round(n / dt / 1e6, 3),
                      "bwd_fwd_ratio": round((dt_adj + dt_grad) / dt, 3),
-                     "bwd_ms": {"adjoint": round(dt_adj * 1e3, 2), "obs_grad": round(dt_grad * 1e3, 2)},
+                     "bwd_ms": {"adjoint": round(dt_adj * 1e3, 2), "obs_grad": round(dt_grad * 1e3, 2),
+                                "obs_grad_recomputing": round(dt_grad_re * 1e3, 2)},
                      "plan_build_s": round(plan, 2), "m2l_kernel_ms": round(float(np.mean(m2l)), 3),
                      "stage_ms": stage_ms,
                      "rel_err_vs_direct": float(np.linalg.norm(ys[prec][idx] - yd) / np.linalg.norm(yd))}
@@ -224,7 +250,8 @@ def cfg4_run(tbem, torch, steps):
     for prec in ("fp64", "mixed"):
         op = tbem.FMMOperator(K, data(obs), tbem.FMMConfig(0.3, 150, 256, 0.05, False), precision=prec)
         dt, _ = time_steps(lambda: op.apply(x), steps, 1, torch)
-        dtb, _ = time_steps(lambda: (op.apply_adjoint(gv), op.obs_grad(x, gv)), steps, 1, torch)
+        dt_adj, dt_grad, _ = backward_times(op, x, gv, steps, torch)
+        dtb = dt_adj + dt_grad
         slip = torch.tensor(g["x"], device="cuda", requires_grad=True)
         o = torch.tensor(obs, device="cuda", requires_grad=True)
         ag.forward_model_loss(op, slip, u_obs, o).backward()
@@ -394,15 +421,15 @@ def main():
         e32_64 = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
 
         g = torch.randn(R * n, dtype=torch.float64, device="cuda")
-        dt_adj, _ = time_steps(lambda: op64.apply_adjoint(g), max(2, args.steps // 2), 1, torch)
-        dt_grad, _ = time_steps(lambda: op64.obs_grad(x, g), max(2, args.steps // 2), 1, torch)
+        dt_adj, dt_grad, dt_grad_re = backward_times(op64, x, g, max(2, args.steps // 2), torch)
         out.update({
             "rel_err_fp64_vs_direct": e64,
             "mixed": {"value": round(n / dt32 / 1e6, 3), "ms_per_step": round(dt32 * 1e3, 4), "rel_err_vs_direct": e32,
                       "rel_err_vs_fp64_fmm": e32_64, "speedup_vs_fp64": round(dt64 / dt32, 3),
                       "plan_build_s": round(plan32, 3)},
             "bwd_fwd_ratio": round((dt_adj + dt_grad) / dt64, 3),
-            "bwd_ms": {"adjoint": round(dt_adj * 1e3, 3), "obs_grad": round(dt_grad * 1e3, 3)},
+            "bwd_ms": {"adjoint": round(dt_adj * 1e3, 3), "obs_grad": round(dt_grad * 1e3, 3),
+                       "obs_grad_recomputing": round(dt_grad_re * 1e3, 3)},
             "direct_subsample_s": round(t_direct, 3),
         })
         log(f"[bench] rel err fp64 {e64:.3e}, mixed {e32:.3e}; mixed {dt32 * 1e3:.2f} ms; "

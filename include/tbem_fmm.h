@@ -25,6 +25,7 @@
 #define TBEM_FMM_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -144,6 +145,19 @@ int tbem_fmm_apply_adjoint_dev(tbem_fmm_t h, const double* d_g, double* d_y, voi
 int tbem_fmm_obs_grad(tbem_fmm_t h, const double* x, const double* g, double* grad);
 int tbem_fmm_obs_grad_dev(tbem_fmm_t h, const double* d_x, const double* d_g, double* d_grad,
                           void* stream);
+
+/* The backward's reuse of the forward's upward pass (no reference counterpart:
+ * 3bem has no backward). tbem_fmm_forward_token returns a token for the far field
+ * (multipole and local expansions) that the handle's work buffers hold after its
+ * last forward (0: none). tbem_fmm_obs_grad_reuse_dev is tbem_fmm_obs_grad_dev
+ * that, while the token is still the handle's current one (no forward or
+ * recomputing gradient on this handle since; adjoints do not count), reuses that
+ * far field instead of recomputing it; otherwise it recomputes, as
+ * tbem_fmm_obs_grad_dev. d_x must hold the same values as that forward's input.
+ * Costs no memory: the expansions are the handle's own work buffers. */
+uint64_t tbem_fmm_forward_token(tbem_fmm_t h);
+int tbem_fmm_obs_grad_reuse_dev(tbem_fmm_t h, uint64_t token, const double* d_x, const double* d_g,
+                                double* d_grad, void* stream);
 
 /* Move observation points without rebuilding the plan (same tree cells, same
  * surfaces); for finite-difference checks of tbem_fmm_obs_grad. */

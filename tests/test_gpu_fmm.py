@@ -235,6 +235,35 @@ def test_gpu_mixed_obs_grad_matches_fp64(tbem, wl, kid):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_gpu_obs_grad_reuses_forward_far_field(tbem, wl, prec):
+    """The gradient that reuses the forward's expansions (forward token) equals the
+    recomputing one bit for bit, also after an adjoint in between; a token made
+    stale by another forward falls back to recomputing for the x passed."""
+    torch = pytest.importorskip("torch")
+    n = 20000
+    g = wl.cfg2_fault_and_surface(n)
+    data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
+    op = tbem.FMMOperator(tbem.ElasticDisplacement(1.0, 0.25), data, tbem.FMMConfig(0.3, 50, 100, 0.05, False),
+                          precision=prec)
+    gen = torch.Generator("cuda").manual_seed(5)
+    x, x2, gg = (torch.randn(3 * n, dtype=torch.float64, device="cuda", generator=gen) for _ in range(3))
+    case("obs grad reuse", prec)
+    assert op.forward_token() == 0
+    op.apply(x)
+    tok = op.forward_token()
+    assert tok != 0
+    op.apply_adjoint(gg)
+    reused = op.obs_grad(x, gg, token=tok)
+    assert op.forward_token() == tok
+    fresh = op.obs_grad(x, gg)
+    assert op.forward_token() != tok
+    assert torch.equal(reused, fresh)
+    op.apply(x2)
+    stale = op.obs_grad(x, gg, token=tok)
+    assert torch.equal(stale, fresh)
+
+
 # ----------------------------------------------------------------- obs gradient
 @pytest.mark.parametrize("kid", [0, 3, 4, 6])
 def test_gpu_obs_grad_matches_finite_differences(tbem, kid):
