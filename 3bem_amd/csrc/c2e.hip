@@ -171,8 +171,38 @@ __global__ __launch_bounds__(1024) void k_jacobi_pinv(int n, double threshold, c
     }
 }
 
-void gpu_pinv_batched(const double* d_mats, int nmat, int n, double threshold, double* d_out,
-                      hipStream_t s) {
+// The pseudo-inverse's own factors, in gpu_svd_batched's convention, from the SVD
+// of A (rows of W = sigma_j u_j, rows of V = v_j): pinv(A) = sum_j (1/sigma_j) v_j
+// u_j^T, so row j of Wp = v_j / sigma_j (zero for the modes the threshold drops)
+// and row j of Vp = u_j = W_j / sigma_j.
+__global__ void k_pinv_factors(int n, double threshold, const double* __restrict__ W, const double* __restrict__ V,
+                               double* __restrict__ Wp, double* __restrict__ Vp) {
+    const int m = blockIdx.x;
+    const size_t off = (size_t)m * n * n;
+    extern __shared__ double sig[];  // n, then the maximum
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double s2 = 0;
+        for (int k = 0; k < n; k++) s2 += W[off + (size_t)j * n + k] * W[off + (size_t)j * n + k];
+        sig[j] = sqrt(s2);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = 0;
+        for (int j = 0; j < n; j++) mx = fmax(mx, sig[j]);
+        sig[n] = mx;
+    }
+    __syncthreads();
+    const double cutoff = threshold * sig[n];
+    for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+        const int j = (int)(e / n);
+        const double sj = sig[j];
+        Wp[off + e] = (sj > cutoff) ? V[off + e] / sj : 0.0;
+        Vp[off + e] = (sj > 0) ? W[off + e] / sj : 0.0;
+    }
+}
+
+void gpu_pinv_batched(const double* d_mats, int nmat, int n, double threshold, double* d_out, hipStream_t s,
+                      DevBuf<double>* fac_w, DevBuf<double>* fac_v) {
     if (nmat == 0 || n == 0) return;
     DevBuf<double> W((size_t)nmat * n * n), V((size_t)nmat * n * n);
     DevBuf<int> st(nmat);
@@ -181,6 +211,14 @@ void gpu_pinv_batched(const double* d_mats, int nmat, int n, double threshold, d
     TBEM_LAUNCH_CHECK();
     auto hs = st.to_host(s);
     for (int v: hs) TBEM_REQUIRE(v == 0, TBEM_E_NUMERIC, "Jacobi SVD did not converge in 60 sweeps");
+    if (fac_w && fac_v) {
+        fac_w->alloc((size_t)nmat * n * n);
+        fac_v->alloc((size_t)nmat * n * n);
+        hipLaunchKernelGGL(k_pinv_factors, dim3(nmat), dim3(256), (n + 1) * sizeof(double), s, n, threshold, W.p,
+                           V.p, fac_w->p, fac_v->p);
+        TBEM_LAUNCH_CHECK();
+        TBEM_HIP(hipStreamSynchronize(s));  // W and V are freed on return
+    }
 }
 
 void gpu_svd_batched(const double* d_mats, int nmat, int n, DevBuf<double>& W, DevBuf<double>& V,
@@ -197,7 +235,7 @@ void gpu_svd_batched(const double* d_mats, int nmat, int n, DevBuf<double>& W, D
 }
 
 void compute_c2e(int kid, const KConst& kc, const HostTree& t, const Surfaces& surf, bool up,
-                 DevBuf<double>& out, hipStream_t s) {
+                 DevBuf<double>& out, hipStream_t s, DevBuf<double>* fac_w, DevBuf<double>* fac_v) {
     const int R = kernel_dim(kid);
     const int n_e = surf.n_e, n = R * n_e, L = t.n_levels;
     std::vector<double> centers(3 * (size_t)L), radii(L);
@@ -219,7 +257,7 @@ void compute_c2e(int kid, const KConst& kc, const HostTree& t, const Surfaces& s
                        du.p, mats.p);
     TBEM_LAUNCH_CHECK();
     out.alloc((size_t)L * n * n);
-    gpu_pinv_batched(mats.p, L, n, 1e-10, out.p, s);
+    gpu_pinv_batched(mats.p, L, n, 1e-10, out.p, s, fac_w, fac_v);
     TBEM_HIP(hipStreamSynchronize(s));
 }
 

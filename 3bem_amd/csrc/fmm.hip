@@ -242,19 +242,24 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
 
     // check-to-equivalent operators
     const size_t bb = (size_t)P->blk * P->blk;
+    // the mixed path's modal factors come from the C2E pseudo-inverses' own SVDs
+    const bool keep_fac = P->precision == TBEM_MIXED;
+    DevBuf<double> up_fac[2], down_fac[2];
     if (d->up_check_to_equiv) {
         TBEM_REQUIRE(d->n_up_levels >= (size_t)P->st.n_levels, TBEM_E_INVALID,
                      "injected UC2E has fewer levels than the source tree");
         P->uc2e.upload(d->up_check_to_equiv, bb * P->st.n_levels, s);
     } else {
-        compute_c2e(P->kid, P->kc, P->st, P->surf, true, P->uc2e, s);
+        compute_c2e(P->kid, P->kc, P->st, P->surf, true, P->uc2e, s, keep_fac ? &up_fac[0] : nullptr,
+                    keep_fac ? &up_fac[1] : nullptr);
     }
     if (d->down_check_to_equiv) {
         TBEM_REQUIRE(d->n_down_levels >= (size_t)P->ot.n_levels, TBEM_E_INVALID,
                      "injected DC2E has fewer levels than the observation tree");
         P->dc2e.upload(d->down_check_to_equiv, bb * P->ot.n_levels, s);
     } else {
-        compute_c2e(P->kid, P->kc, P->ot, P->surf, false, P->dc2e, s);
+        compute_c2e(P->kid, P->kc, P->ot, P->surf, false, P->dc2e, s, keep_fac ? &down_fac[0] : nullptr,
+                    keep_fac ? &down_fac[1] : nullptr);
     }
     P->h_uc2e = P->uc2e.to_host(s);
     P->h_dc2e = P->dc2e.to_host(s);
@@ -280,7 +285,8 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->m2l.upload(m2l_h, s);
     P->t_m2l = make_cell_tiles(m2l_h.items, OT, s);
     if (P->precision == TBEM_MIXED)
-        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, same, m2l_h, P->surf, P->uc2e.p, P->dc2e.p, s);
+        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, same, m2l_h, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac,
+                       down_fac);
     const std::vector<int> p2l_items = cells_with(tl.p2l, 0, OT.n_cells);
     P->p2l.upload(list_from_pairs(p2l_items, OT.n_cells, tl.p2l, 0, &ST), s);
     P->t_p2l = make_cell_tiles(p2l_items, OT, s);

@@ -35,9 +35,13 @@ struct M2lMfma {
 // Leaves M.enabled false (the caller keeps the FP64 M2L) when the two trees
 // differ, an offset is not integral, blk exceeds what one workgroup stages, or
 // blk is below 48 (where the FP64 stage is faster).
+// up_fac, down_fac: optionally {W, V}, the SVD factors of UC2E and DC2E per level
+// (from the pseudo-inverse's own SVD, gpu_pinv_batched); without them the C2E
+// matrices are factored again.
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
                     bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
-                    const double* d_dc2e, hipStream_t s);
+                    const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac = nullptr,
+                    const DevBuf<double>* down_fac = nullptr);
 
 // Y[p] (ystride floats per pair) = M_{class(p)} * chk_up[src(p)] for every pair,
 // chk_up = Bu * up-check potential per source cell, rounded to FP32.

@@ -286,7 +286,8 @@ struct ClassHash {
 
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
                     bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
-                    const double* d_dc2e, hipStream_t s) {
+                    const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac,
+                    const DevBuf<double>* down_fac) {
     auto t0 = std::chrono::steady_clock::now();
     M = M2lMfma{};
     const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
@@ -345,17 +346,18 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     DevBuf<double> Ru, Ld;
     {
         DevBuf<double> W, V;
-        gpu_svd_batched(d_uc2e, st.n_levels, n, W, V, s);
+        const bool have_up = up_fac && up_fac[0].n && up_fac[1].n, have_down = down_fac && down_fac[0].n && down_fac[1].n;
+        if (!have_up) gpu_svd_batched(d_uc2e, st.n_levels, n, W, V, s);
         M.Bu.alloc((size_t)st.n_levels * nn);
         Ru.alloc((size_t)st.n_levels * nn);
-        hipLaunchKernelGGL(k_modal_factors, dim3(st.n_levels), dim3(256), (n + 1) * sizeof(double), s, n, 1, W.p,
-                           V.p, M.Bu.p, Ru.p);
+        hipLaunchKernelGGL(k_modal_factors, dim3(st.n_levels), dim3(256), (n + 1) * sizeof(double), s, n, 1,
+                           have_up ? up_fac[0].p : W.p, have_up ? up_fac[1].p : V.p, M.Bu.p, Ru.p);
         TBEM_LAUNCH_CHECK();
-        gpu_svd_batched(d_dc2e, ot.n_levels, n, W, V, s);
+        if (!have_down) gpu_svd_batched(d_dc2e, ot.n_levels, n, W, V, s);
         Ld.alloc((size_t)ot.n_levels * nn);
         M.Vd.alloc((size_t)ot.n_levels * nn);
-        hipLaunchKernelGGL(k_modal_factors, dim3(ot.n_levels), dim3(256), (n + 1) * sizeof(double), s, n, 0, W.p,
-                           V.p, Ld.p, M.Vd.p);
+        hipLaunchKernelGGL(k_modal_factors, dim3(ot.n_levels), dim3(256), (n + 1) * sizeof(double), s, n, 0,
+                           have_down ? down_fac[0].p : W.p, have_down ? down_fac[1].p : V.p, Ld.p, M.Vd.p);
         TBEM_LAUNCH_CHECK();
         TBEM_HIP(hipStreamSynchronize(s));
     }
