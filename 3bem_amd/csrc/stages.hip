@@ -665,47 +665,63 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
 
 // ------------------------------------------------------------------------ helpers
 // out[c] (= or +=) op(M[level c]) in[c] (+ addin[c]) for the cells of each tile:
-// a 64 (rows) x 64 (cells) output tile per workgroup, 4 x 4 per thread, k in
-// chunks of 16 staged in LDS.
+// a 64 (rows) x 64 (cells) output tile per workgroup, 4 x 4 per thread (rows ty*4+i,
+// cells tx+16j), k in chunks of kGemmK staged in LDS; the next chunk is loaded into
+// registers while the current one is multiplied, so its latency overlaps the FMAs.
+constexpr int kGemmK = 32;
+
 __global__ __launch_bounds__(256) void k_level_gemm(int n, const int* __restrict__ cells,
                                                     const int* __restrict__ tiles, const double* __restrict__ M,
                                                     int trans, const double* __restrict__ in,
                                                     const double* __restrict__ addin, double* __restrict__ out,
                                                     float* __restrict__ out32, int add) {
+    constexpr int KC = kGemmK, PER = 64 * KC / 256;  // elements of each operand per thread and chunk
     const int t = blockIdx.y;
     const int first = uni(tiles[3 * t]), cnt = uni(tiles[3 * t + 1]), lvl = uni(tiles[3 * t + 2]);
     const double* A = M + (size_t)lvl * n * n;
     const int m0 = blockIdx.x * 64;
-    __shared__ double As[16][65], Bs[16][65];
+    __shared__ double As[KC][65], Bs[KC][65];
     __shared__ int cl[64];
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     if (tid < 64) cl[tid] = tid < cnt ? cells[first + tid] : -1;
     __syncthreads();
-    double acc[4][4] = {};
-    for (int k0 = 0; k0 < n; k0 += 16) {
-        for (int e = tid; e < 1024; e += 256) {
-            // A chunk: As[k][r] = op(A)[m0 + r][k0 + k]
-            int r, k;
-            if (trans) { k = e / 64; r = e % 64; }
-            else { r = e / 16; k = e % 16; }
+    double pa[PER], pb[PER];
+    // A chunk: As[k][r] = op(A)[m0 + r][k0 + k]; B chunk: Bs[k][c] = in[cell c][k0 + k]
+    auto fetch = [&](int k0) {
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + 256 * q;
+            const int r = trans ? e % 64 : e / KC, k = trans ? e / 64 : e % KC;
             const int row = m0 + r, kk = k0 + k;
-            As[k][r] = (row < n && kk < n) ? (trans ? A[(size_t)kk * n + row] : A[(size_t)row * n + kk]) : 0.0;
-            // B chunk: Bs[k][c] = in[cell c][k0 + k]
-            const int c = e / 16, kb = e % 16, cell = cl[c];
-            Bs[kb][c] = (cell >= 0 && k0 + kb < n) ? in[(size_t)cell * n + k0 + kb] : 0.0;
+            pa[q] = (row < n && kk < n) ? (trans ? A[(size_t)kk * n + row] : A[(size_t)row * n + kk]) : 0.0;
+            const int c = e / KC, kb = e % KC, cell = cl[c];
+            pb[q] = (cell >= 0 && k0 + kb < n) ? in[(size_t)cell * n + k0 + kb] : 0.0;
         }
+    };
+    auto store = [&]() {
+        for (int q = 0; q < PER; q++) {
+            const int e = tid + 256 * q;
+            const int r = trans ? e % 64 : e / KC, k = trans ? e / 64 : e % KC;
+            As[k][r] = pa[q];
+            Bs[e % KC][e / KC] = pb[q];
+        }
+    };
+    double acc[4][4] = {};
+    fetch(0);
+    for (int k0 = 0; k0 < n; k0 += KC) {
+        store();
         __syncthreads();
-        for (int k = 0; k < 16; k++) {
+        if (k0 + KC < n) fetch(k0 + KC);
+        for (int k = 0; k < KC; k++) {
             double av[4], bv[4];
             for (int i = 0; i < 4; i++) av[i] = As[k][ty * 4 + i];
-            for (int j = 0; j < 4; j++) bv[j] = Bs[k][tx * 4 + j];
+            for (int j = 0; j < 4; j++) bv[j] = Bs[k][tx + 16 * j];
             for (int i = 0; i < 4; i++)
                 for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
         }
         __syncthreads();
     }
     for (int j = 0; j < 4; j++) {
-        const int cell = cl[tx * 4 + j];
+        const int cell = cl[tx + 16 * j];
         if (cell < 0) continue;
         for (int i = 0; i < 4; i++) {
             const int row = m0 + ty * 4 + i;
