@@ -779,6 +779,31 @@ long orc_surface(size_t order, double* out) {
     return (long)s.size();
 }
 
+// CSR from COO triplets as cpp/sparse_operator.cpp:140-172 builds it: rows
+// counted, row pointers as prefix sums, entries placed in input order within their
+// row (duplicates kept). With x, the product as cpp/sparse_operator.cpp:19-28
+// computes it: y_i = sum over the row's entries in stored order of value * x[col].
+void orc_sparse_coo(size_t n_rows, size_t n_cols, size_t n_entries, const long* rows, const long* cols,
+                    const double* vals, long* row_ptrs, long* column_indices, double* values, const double* x,
+                    double* y) {
+    std::vector<long> count(n_rows, 0), fill(n_rows, 0);
+    for (size_t i = 0; i < n_entries; i++) count[rows[i]]++;
+    row_ptrs[0] = 0;
+    for (size_t r = 0; r < n_rows; r++) row_ptrs[r + 1] = row_ptrs[r] + count[r];
+    for (size_t i = 0; i < n_entries; i++) {
+        const long slot = row_ptrs[rows[i]] + fill[rows[i]]++;
+        column_indices[slot] = cols[i];
+        values[slot] = vals[i];
+    }
+    if (!x) return;
+    (void)n_cols;
+    for (size_t r = 0; r < n_rows; r++) {
+        double s = 0.0;
+        for (long k = row_ptrs[r]; k < row_ptrs[r + 1]; k++) s += values[k] * x[column_indices[k]];
+        y[r] = s;
+    }
+}
+
 void orc_pinv(const double* A, int n, double threshold, double* out) {
     std::vector<double> m(A, A + (size_t)n * n);
     auto p = pinv(m, n, threshold > 0 ? threshold : 1e-16);

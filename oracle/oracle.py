@@ -77,6 +77,7 @@ class Oracle:
         f("surface").restype = ctypes.c_long
         f("surface").argtypes = [_SZ, _D]
         f("pinv").argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]
+        f("sparse_coo").argtypes = [_SZ, _SZ, _SZ, _L, _L, _D, _L, _L, _D, _D, _D]
         if prefix == "orc":
             for name in ("direct_eval_par", "direct_adjoint"):
                 f(name).argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -143,6 +144,28 @@ class Oracle:
         out = np.zeros((n, n))
         self._f("pinv")(_p(A), n, threshold, _p(out))
         return out
+
+
+    def sparse_coo(self, n_rows, n_cols, rows, cols, vals, x=None):
+        """CSR from COO triplets (SparseOperator::csr_from_coo, cpp/sparse_operator.cpp:140-172)
+        and, with x, the CSR product (SparseOperator::apply, cpp/sparse_operator.cpp:19-28).
+        Returns (row_ptrs, column_indices, values, y or None)."""
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        v = _c(vals)
+        n = r.shape[0]
+        assert c.shape[0] == n and v.shape[0] == n
+        assert n == 0 or (r.min() >= 0 and r.max() < n_rows and c.min() >= 0 and c.max() < n_cols)
+        rp = np.zeros(n_rows + 1, dtype=np.int64)
+        ci = np.zeros(n, dtype=np.int64)
+        vv = np.zeros(n)
+        xv = _c(x) if x is not None else None
+        if xv is not None:
+            assert xv.shape[0] == n_cols
+        y = np.zeros(n_rows) if x is not None else None
+        self._f("sparse_coo")(n_rows, n_cols, n, r.ctypes.data_as(_L), c.ctypes.data_as(_L), _p(v),
+                              rp.ctypes.data_as(_L), ci.ctypes.data_as(_L), _p(vv), _p(xv), _p(y))
+        return rp, ci, vv, y
 
 
 class OracleFMM:

@@ -19,6 +19,7 @@
 #include "octree.h"
 #include "geometry.h"
 #include "blas_wrapper.h"
+#include "sparse_operator.h"
 
 #include <memory>
 #include <vector>
@@ -237,6 +238,26 @@ long ref_surface(size_t order, double* out) {
 }
 
 // Thresholded SVD pseudo-inverse of a row-major n x n matrix.
+// SparseOperator::csr_from_coo (cpp/sparse_operator.cpp:140-172) of n_entries COO
+// triplets; writes the CSR arrays (row_ptrs n_rows + 1, column_indices and values
+// n_entries) and, with x, SparseOperator::apply (cpp/sparse_operator.cpp:19-28).
+void ref_sparse_coo(size_t n_rows, size_t n_cols, size_t n_entries, const long* rows, const long* cols,
+                    const double* vals, long* row_ptrs, long* column_indices, double* values, const double* x,
+                    double* y) {
+    std::vector<MatrixEntry> e(n_entries);
+    for (size_t i = 0; i < n_entries; i++) e[i] = MatrixEntry{{(size_t)rows[i], (size_t)cols[i]}, vals[i]};
+    auto op = SparseOperator::csr_from_coo(n_rows, n_cols, e);
+    for (size_t i = 0; i <= n_rows; i++) row_ptrs[i] = (long)op.row_ptrs[i];
+    for (size_t i = 0; i < n_entries; i++) {
+        column_indices[i] = (long)op.column_indices[i];
+        values[i] = op.values[i];
+    }
+    if (x) {
+        auto out = op.apply(std::vector<double>(x, x + n_cols));
+        std::memcpy(y, out.data(), n_rows * sizeof(double));
+    }
+}
+
 void ref_pinv(const double* A, int n, double threshold, double* out) {
     std::vector<double> m(A, A + (size_t)n * n);
     auto svd = svd_decompose(m);

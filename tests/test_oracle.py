@@ -242,3 +242,57 @@ def test_planted_mistake_is_caught(orc):
     g = golden("small_laplace_single_nosc")
     op = orc.fmm(0, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.35, 6, 20, 0.05, False)
     assert rel_err(op.apply(g["x"]), g["y"]) > 1e-8
+
+
+def _dense_from_coo(n_rows, n_cols, rows, cols, vals):
+    A = np.zeros((n_rows, n_cols))
+    np.add.at(A, (np.asarray(rows, dtype=np.int64), np.asarray(cols, dtype=np.int64)), vals)
+    return A
+
+
+def test_sparse_known_answers(orc):
+    """The reference's own sparse cases (unit_tests/test_sparse.cpp): an empty 2x2
+    matrix gives zeros; {(0,0,1),(1,0,2)} times {0.5, 0} is {0.5, 1}; the 3x3
+    {(0,0,1),(1,0,2),(1,1,4)} times {0.5, 7, 1} is {0.5, 29, 0}; row_ptrs ends at nnz."""
+    rp, ci, v, y = orc.sparse_coo(2, 2, [], [], [], np.array([1.0, 1.0]))
+    assert rp.tolist() == [0, 0, 0] and ci.size == 0 and y.tolist() == [0.0, 0.0]
+    rp, ci, v, y = orc.sparse_coo(2, 2, [0, 1], [0, 0], [1.0, 2.0], np.array([0.5, 0.0]))
+    assert y.tolist() == [0.5, 1.0] and rp[-1] == 2 and v.size == 2
+    rp, ci, v, y = orc.sparse_coo(3, 3, [0, 1, 1], [0, 0, 1], [1.0, 2.0, 4.0], np.array([0.5, 7.0, 1.0]))
+    assert y.tolist() == [0.5, 29.0, 0.0]
+    assert rp.tolist() == [0, 1, 3, 3] and ci.tolist() == [0, 0, 1] and v.tolist() == [1.0, 2.0, 4.0]
+
+
+def test_sparse_csr_layout_and_product(orc):
+    """Random COO with duplicates and empty rows: the CSR keeps each row's entries in
+    input order (a stable sort by row), row_ptrs are the prefix sums of the row counts,
+    and the product equals the dense product of the summed matrix."""
+    rng = np.random.default_rng(11)
+    n_rows, n_cols, n = 300, 170, 4000
+    rows = rng.integers(0, n_rows, n)
+    rows[rows % 7 == 3] = 0  # leave rows 3, 10, ... empty
+    cols = rng.integers(0, n_cols, n)
+    vals = rng.standard_normal(n)
+    x = rng.standard_normal(n_cols)
+    rp, ci, v, y = orc.sparse_coo(n_rows, n_cols, rows, cols, vals, x)
+    order = np.argsort(rows, kind="stable")
+    assert np.array_equal(rp, np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n_rows))]))
+    assert np.array_equal(ci, cols[order]) and np.array_equal(v, vals[order])
+    assert rel_err(y, _dense_from_coo(n_rows, n_cols, rows, cols, vals) @ x) < 1e-14
+    # the sum runs over each row's entries in stored order, starting from 0.0
+    for r in (0, 1, 17, n_rows - 1):
+        s = 0.0
+        for k in range(rp[r], rp[r + 1]):
+            s += v[k] * x[ci[k]]
+        assert y[r] == s
+
+
+def test_sparse_oracle_matches_compiled_reference(orc, ref):
+    rng = np.random.default_rng(12)
+    for n_rows, n_cols, n in ((1, 1, 0), (5, 9, 40), (257, 64, 3000)):
+        rows, cols = rng.integers(0, n_rows, n), rng.integers(0, n_cols, n)
+        vals, x = rng.standard_normal(n), rng.standard_normal(n_cols)
+        a = orc.sparse_coo(n_rows, n_cols, rows, cols, vals, x)
+        b = ref.sparse_coo(n_rows, n_cols, rows, cols, vals, x)
+        for u, w in zip(a, b):
+            assert np.array_equal(u, w)
