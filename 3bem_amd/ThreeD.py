@@ -389,6 +389,165 @@ def make_direct_nbody_operator(data, K, precision="fp64"):
     return DirectOperator(data, K, precision)
 
 
+def _apply_dev_or_host(lib, h, host_fn, dev_fn, x, n_in, n_out):
+    if _is_torch(x):
+        import torch
+        assert x.is_cuda and x.dtype == torch.float64 and x.is_contiguous() and x.numel() == n_in
+        y = torch.empty(n_out, dtype=torch.float64, device=x.device)
+        _lib.check(dev_fn(h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), _torch_stream()))
+        return y
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+    if x.shape[0] != n_in:
+        raise ValueError(f"x has {x.shape[0]} entries, operator expects {n_in}")
+    y = np.zeros(n_out)
+    _lib.check(host_fn(h, _dp(x), _dp(y)))
+    return y
+
+
+def _szp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)) if a.size else None
+
+
+class SparseOperator:
+    """SparseOperator (cpp/sparse_operator.h:21-60) held on the GPU in CSR form.
+
+    SparseOperator(n_rows, n_cols, values, column_indices, row_ptrs) takes the CSR
+    arrays (cpp/sparse_operator.cpp:7-17); SparseOperator.csr_from_coo(n_rows, n_cols,
+    rows, cols, values) builds them from triplets (cpp/sparse_operator.cpp:140-172).
+    apply() equals the reference's SparseOperator::apply bit for bit.
+    """
+
+    def __init__(self, n_rows, n_cols, values, column_indices, row_ptrs):
+        lib = _lib.require_device()
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        c = np.ascontiguousarray(column_indices, dtype=np.uint64).reshape(-1)
+        r = np.ascontiguousarray(row_ptrs, dtype=np.uint64).reshape(-1)
+        if c.shape[0] != v.shape[0] or r.shape[0] != n_rows + 1:
+            raise ValueError("need len(column_indices) == len(values) and len(row_ptrs) == n_rows + 1")
+        h = ctypes.c_void_p()
+        _lib.check(lib.tbem_sparse_create(n_rows, n_cols, v.shape[0], _dp(v), _szp(c), _szp(r), ctypes.byref(h)))
+        self._h, self._lib = h, lib
+
+    @classmethod
+    def csr_from_coo(cls, n_rows, n_cols, rows, cols, values):
+        lib = _lib.require_device()
+        r = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+        c = np.ascontiguousarray(cols, dtype=np.uint64).reshape(-1)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if not (r.shape == c.shape == v.shape):
+            raise ValueError("rows, cols and values must have the same length")
+        h = ctypes.c_void_p()
+        _lib.check(lib.tbem_sparse_from_coo(n_rows, n_cols, v.shape[0], _szp(r), _szp(c), _dp(v), ctypes.byref(h)))
+        o = cls.__new__(cls)
+        o._h, o._lib = h, lib
+        return o
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.tbem_sparse_destroy(h)
+            self._h = None
+
+    def n_rows(self):
+        return int(self._lib.tbem_sparse_n_rows(self._h))
+
+    def n_cols(self):
+        return int(self._lib.tbem_sparse_n_cols(self._h))
+
+    def nnz(self):
+        return int(self._lib.tbem_sparse_nnz(self._h))
+
+    def _csr(self):
+        nnz = self.nnz()
+        v, c = np.zeros(nnz), np.zeros(nnz, dtype=np.uint64)
+        r = np.zeros(self.n_rows() + 1, dtype=np.uint64)
+        _lib.check(self._lib.tbem_sparse_csr(self._h, _dp(v), _szp(c), _szp(r)))
+        return v, c.astype(np.int64), r.astype(np.int64)
+
+    @property
+    def values(self):
+        return self._csr()[0]
+
+    @property
+    def column_indices(self):
+        return self._csr()[1]
+
+    @property
+    def row_ptrs(self):
+        return self._csr()[2]
+
+    def to_dense(self):
+        """SparseOperator::to_dense (cpp/sparse_operator.cpp:31-40) as an (n_rows, n_cols) array."""
+        v, c, r = self._csr()
+        out = np.zeros((self.n_rows(), self.n_cols()))
+        np.add.at(out, (np.repeat(np.arange(self.n_rows()), np.diff(r)), c), v)
+        return out
+
+    def clone(self):
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.tbem_sparse_clone(self._h, ctypes.byref(h)))
+        o = SparseOperator.__new__(SparseOperator)
+        o._h, o._lib = h, self._lib
+        return o
+
+    def right_multiply(self, other):
+        """The sparse product self @ other (SparseOperator::right_multiply,
+        cpp/sparse_operator.cpp:114-138), formed on the host like the reference's."""
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.tbem_sparse_right_multiply(self._h, other._h, ctypes.byref(h)))
+        o = SparseOperator.__new__(SparseOperator)
+        o._h, o._lib = h, self._lib
+        return o
+
+    def apply(self, x):
+        """y = A x (SparseOperator::apply, cpp/sparse_operator.cpp:19-28)."""
+        return _apply_dev_or_host(self._lib, self._h, self._lib.tbem_sparse_apply, self._lib.tbem_sparse_apply_dev,
+                                  x, self.n_cols(), self.n_rows())
+
+
+class IntegralOperator:
+    """IntegralOperator (cpp/integral_operator.h:26-64): y = N x + (G F(I x) + C x)
+    with the far field F an FMMOperator or a DirectOperator on the GPU. Arguments in
+    the order of the reference's constructor: nearfield, farfield_correction,
+    galerkin, farfield, interp."""
+
+    def __init__(self, nearfield, farfield_correction, galerkin, farfield, interp):
+        lib = _lib.require_device()
+        if isinstance(farfield, FMMOperator):
+            kind = _lib.FARFIELD_FMM
+        elif isinstance(farfield, DirectOperator):
+            kind = _lib.FARFIELD_DIRECT
+        else:
+            raise TypeError("farfield must be an FMMOperator or a DirectOperator")
+        # the library borrows these handles: keep the operators alive with this one
+        self.nearfield, self.farfield_correction, self.galerkin = nearfield, farfield_correction, galerkin
+        self.farfield, self.interp = farfield, interp
+        h = ctypes.c_void_p()
+        _lib.check(lib.tbem_integral_create(nearfield._h, farfield_correction._h, galerkin._h, kind,
+                                            farfield._h, interp._h, ctypes.byref(h)))
+        self._h, self._lib = h, lib
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.tbem_integral_destroy(h)
+            self._h = None
+
+    def n_rows(self):
+        return int(self._lib.tbem_integral_n_rows(self._h))
+
+    def n_cols(self):
+        return int(self._lib.tbem_integral_n_cols(self._h))
+
+    def clone(self):
+        return IntegralOperator(self.nearfield, self.farfield_correction, self.galerkin, self.farfield.clone(),
+                                self.interp)
+
+    def apply(self, x):
+        return _apply_dev_or_host(self._lib, self._h, self._lib.tbem_integral_apply,
+                                  self._lib.tbem_integral_apply_dev, x, self.n_cols(), self.n_rows())
+
+
 def nbody_eval(K, data, x):
     """nbody_eval(K, data, x) (cpp/nbody_operator.h:117-141), on the GPU."""
     return DirectOperator(data, K).apply(x)

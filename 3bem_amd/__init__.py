@@ -7,5 +7,5 @@ from .ThreeD import (FMMConfig, NBodyData, FMMOperator, DirectOperator, Kernel, 
                      LaplaceSingle, LaplaceDouble, LaplaceHypersingular, ElasticDisplacement,
                      ElasticTraction, ElasticAdjointTraction, ElasticHypersingular, IdentityScalar,
                      IdentityTensor, make_kernel, make_direct_nbody_operator, nbody_eval,
-                     kernel_eval, surface_points, pinv)
+                     kernel_eval, surface_points, pinv, SparseOperator, IntegralOperator)
 from ._lib import LIB_PATH, TbemError, load  # noqa: F401

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libtbem_fmm.so")
 _D = ctypes.POINTER(ctypes.c_double)
 _L = ctypes.POINTER(ctypes.c_long)
 _SZ = ctypes.c_size_t
+_SZP = ctypes.POINTER(ctypes.c_size_t)
 _VP = ctypes.c_void_p
 
 # every symbol declared in include/tbem_fmm.h
@@ -23,8 +24,13 @@ EXPORTED = (
     "tbem_fmm_destroy", "tbem_fmm_info", "tbem_fmm_c2e", "tbem_fmm_pairs", "tbem_fmm_octree",
     "tbem_fmm_stage_times", "tbem_fmm_m2l_ms", "tbem_nbody_create", "tbem_nbody_apply", "tbem_nbody_apply_dev",
     "tbem_nbody_apply_adjoint", "tbem_nbody_n_rows", "tbem_nbody_n_cols", "tbem_nbody_destroy",
+    "tbem_sparse_create", "tbem_sparse_from_coo", "tbem_sparse_apply", "tbem_sparse_apply_dev",
+    "tbem_sparse_csr", "tbem_sparse_clone", "tbem_sparse_right_multiply", "tbem_sparse_n_rows", "tbem_sparse_n_cols", "tbem_sparse_nnz",
+    "tbem_sparse_destroy", "tbem_integral_create", "tbem_integral_apply", "tbem_integral_apply_dev",
+    "tbem_integral_n_rows", "tbem_integral_n_cols", "tbem_integral_destroy",
     "tbem_kernel_eval", "tbem_surface", "tbem_pinv", "tbem_last_error", "tbem_device_available",
 )
+FARFIELD_FMM, FARFIELD_DIRECT = 0, 1
 
 FP64, MIXED = 0, 1
 
@@ -109,6 +115,23 @@ def load(path=LIB_PATH):
         "tbem_nbody_n_rows": (_SZ, [_VP]),
         "tbem_nbody_n_cols": (_SZ, [_VP]),
         "tbem_nbody_destroy": (None, [_VP]),
+        "tbem_sparse_create": (ctypes.c_int, [_SZ, _SZ, _SZ, _D, _SZP, _SZP, ctypes.POINTER(_VP)]),
+        "tbem_sparse_from_coo": (ctypes.c_int, [_SZ, _SZ, _SZ, _SZP, _SZP, _D, ctypes.POINTER(_VP)]),
+        "tbem_sparse_apply": (ctypes.c_int, [_VP, _D, _D]),
+        "tbem_sparse_apply_dev": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tbem_sparse_csr": (ctypes.c_int, [_VP, _D, _SZP, _SZP]),
+        "tbem_sparse_clone": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
+        "tbem_sparse_right_multiply": (ctypes.c_int, [_VP, _VP, ctypes.POINTER(_VP)]),
+        "tbem_sparse_n_rows": (_SZ, [_VP]),
+        "tbem_sparse_n_cols": (_SZ, [_VP]),
+        "tbem_sparse_nnz": (_SZ, [_VP]),
+        "tbem_sparse_destroy": (None, [_VP]),
+        "tbem_integral_create": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, _VP, ctypes.POINTER(_VP)]),
+        "tbem_integral_apply": (ctypes.c_int, [_VP, _D, _D]),
+        "tbem_integral_apply_dev": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+        "tbem_integral_n_rows": (_SZ, [_VP]),
+        "tbem_integral_n_cols": (_SZ, [_VP]),
+        "tbem_integral_destroy": (None, [_VP]),
         "tbem_kernel_eval": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _SZ, _D, _D, _D, _D, _D]),
         "tbem_surface": (ctypes.c_long, [_SZ, _D]),
         "tbem_pinv": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_double, _D]),

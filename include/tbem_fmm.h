@@ -228,6 +228,66 @@ size_t tbem_nbody_n_rows(tbem_nbody_t h);
 size_t tbem_nbody_n_cols(tbem_nbody_t h);
 void tbem_nbody_destroy(tbem_nbody_t h);
 
+/* ------------------------------------------------------- CSR sparse operator */
+
+/* SparseOperator (cpp/sparse_operator.h:21-60) on the GPU. The product reproduces
+ * SparseOperator::apply (cpp/sparse_operator.cpp:19-28) bit for bit: y_i sums
+ * values[k] * x[column_indices[k]] over row i's entries in stored order, from 0.0,
+ * each product rounded on its own. Limits: n_rows, n_cols < 2^31. */
+typedef struct tbem_sparse_s* tbem_sparse_t;
+
+/* SparseOperator(n_rows, n_cols, values, column_indices, row_ptrs)
+ * (cpp/sparse_operator.cpp:7-17). row_ptrs has n_rows + 1 entries, starts at 0,
+ * does not decrease and ends at nnz; every column index is below n_cols
+ * (TBEM_E_INVALID otherwise). The arrays are copied. */
+int tbem_sparse_create(size_t n_rows, size_t n_cols, size_t nnz, const double* values,
+                       const size_t* column_indices, const size_t* row_ptrs, tbem_sparse_t* out);
+/* SparseOperator::csr_from_coo (cpp/sparse_operator.cpp:140-172): the triplets
+ * (rows[i], cols[i], values[i]) in CSR, each row's entries in input order,
+ * duplicates kept. */
+int tbem_sparse_from_coo(size_t n_rows, size_t n_cols, size_t n_entries, const size_t* rows,
+                         const size_t* cols, const double* values, tbem_sparse_t* out);
+int tbem_sparse_apply(tbem_sparse_t h, const double* x, double* y);
+int tbem_sparse_apply_dev(tbem_sparse_t h, const double* d_x, double* d_y, void* stream);
+/* The CSR arrays (values nnz, column_indices nnz, row_ptrs n_rows + 1), any may be
+ * NULL: the values / column_indices / row_ptrs members (python_wrapper/linalg.cpp:25-37). */
+int tbem_sparse_csr(tbem_sparse_t h, double* values, size_t* column_indices, size_t* row_ptrs);
+int tbem_sparse_clone(tbem_sparse_t h, tbem_sparse_t* out);
+/* SparseOperator::right_multiply (cpp/sparse_operator.cpp:114-138): the sparse
+ * product a b, each row's columns ascending, every touched entry kept (explicit
+ * zeros included), the sums formed in the reference's order. Computed on the host
+ * at set-up time, like the reference; the result lives on the GPU. */
+int tbem_sparse_right_multiply(tbem_sparse_t a, tbem_sparse_t b, tbem_sparse_t* out);
+size_t tbem_sparse_n_rows(tbem_sparse_t h);
+size_t tbem_sparse_n_cols(tbem_sparse_t h);
+size_t tbem_sparse_nnz(tbem_sparse_t h);   /* SparseOperator::nnz (cpp/sparse_operator.h:37) */
+void tbem_sparse_destroy(tbem_sparse_t h);
+
+/* ------------------------------------------------------- integral operator */
+
+/* IntegralOperator<3,R,C> (cpp/integral_operator.h:26-64) with the far field on the
+ * GPU: y = N x + (G F(I x) + C x), summed per row as n_i + (g_i + c_i)
+ * (cpp/integral_operator.h:55-63). The far field F is an FMM operator (the FMM
+ * that boundary_operator leaves commented out, cpp/integral_operator.h:82-87) or
+ * the direct N-body operator (the dense far field it uses). The handles are
+ * borrowed: they must outlive the integral operator, and no other host thread may
+ * apply them while it runs. */
+typedef struct tbem_integral_s* tbem_integral_t;
+enum { TBEM_FARFIELD_FMM = 0, TBEM_FARFIELD_DIRECT = 1 };
+
+/* Argument order of the IntegralOperator constructor (cpp/integral_operator.h:34-44);
+ * farfield is a tbem_fmm_t when farfield_kind is TBEM_FARFIELD_FMM, a tbem_nbody_t
+ * when it is TBEM_FARFIELD_DIRECT. Shapes: nearfield and farfield_correction n_rows x n_cols,
+ * galerkin n_rows x F.n_rows, interp F.n_cols x n_cols. */
+int tbem_integral_create(tbem_sparse_t nearfield, tbem_sparse_t farfield_correction,
+                         tbem_sparse_t galerkin, int farfield_kind, void* farfield,
+                         tbem_sparse_t interp, tbem_integral_t* out);
+int tbem_integral_apply(tbem_integral_t h, const double* x, double* y);
+int tbem_integral_apply_dev(tbem_integral_t h, const double* d_x, double* d_y, void* stream);
+size_t tbem_integral_n_rows(tbem_integral_t h);  /* galerkin.n_rows() */
+size_t tbem_integral_n_cols(tbem_integral_t h);  /* nearfield.n_cols() */
+void tbem_integral_destroy(tbem_integral_t h);
+
 /* ----------------------------------------------------------------------- misc */
 
 /* One kernel evaluation K(obs <- src) on the GPU, R*C doubles row-major

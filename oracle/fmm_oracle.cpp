@@ -24,6 +24,7 @@
 #include <cstring>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <vector>
 #include <array>
 #include <memory>
@@ -802,6 +803,33 @@ void orc_sparse_coo(size_t n_rows, size_t n_cols, size_t n_entries, const long* 
         for (long k = row_ptrs[r]; k < row_ptrs[r + 1]; k++) s += values[k] * x[column_indices[k]];
         y[r] = s;
     }
+}
+
+// Sparse product A B as cpp/sparse_operator.cpp:114-138 forms it: a dictionary
+// keyed by row * n_cols(B) + col, each value 0.0 plus the products b_jc * a_ij in
+// traversal order (A's entries in stored order, then B's row j in stored order),
+// emitted in key order through csr_from_coo. Writes at most cap entries; returns nnz.
+long orc_sparse_right_multiply(size_t a_rows, size_t a_cols, const long* a_rp, const long* a_ci, const double* a_v,
+                               size_t b_cols, const long* b_rp, const long* b_ci, const double* b_v,
+                               long cap, long* rp, long* ci, double* v) {
+    (void)a_cols;
+    std::map<size_t, double> dok;
+    for (size_t i = 0; i < a_rows; i++)
+        for (long ka = a_rp[i]; ka < a_rp[i + 1]; ka++) {
+            const long j = a_ci[ka];
+            for (long kb = b_rp[j]; kb < b_rp[j + 1]; kb++) dok[i * b_cols + b_ci[kb]] += b_v[kb] * a_v[ka];
+        }
+    const long nnz = (long)dok.size();
+    if (nnz > cap) return nnz;
+    std::vector<long> rows, cols;
+    std::vector<double> vals;
+    for (const auto& e : dok) {
+        rows.push_back((long)(e.first / b_cols));
+        cols.push_back((long)(e.first % b_cols));
+        vals.push_back(e.second);
+    }
+    orc_sparse_coo(a_rows, b_cols, (size_t)nnz, rows.data(), cols.data(), vals.data(), rp, ci, v, nullptr, nullptr);
+    return nnz;
 }
 
 void orc_pinv(const double* A, int n, double threshold, double* out) {

@@ -78,6 +78,13 @@ class Oracle:
         f("surface").argtypes = [_SZ, _D]
         f("pinv").argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]
         f("sparse_coo").argtypes = [_SZ, _SZ, _SZ, _L, _L, _D, _L, _L, _D, _D, _D]
+        f("sparse_right_multiply").restype = ctypes.c_long
+        f("sparse_right_multiply").argtypes = [_SZ, _SZ, _L, _L, _D, _SZ, _L, _L, _D, ctypes.c_long, _L, _L, _D]
+        if prefix == "ref":
+            self.lib.ref_bem_sparse.argtypes = [ctypes.c_int, _SZ, _SZ, _D, _SZ, _D, _D, _L, _L, _D,
+                                                ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]
+            self.lib.ref_nbody_from_bem.argtypes = [_SZ, _D, _SZ, _D, _SZ, _D, _D, _SZ, _D, _D,
+                                                    _D, _D, _D, _D, _D]
         if prefix == "orc":
             for name in ("direct_eval_par", "direct_adjoint"):
                 f(name).argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -166,6 +173,50 @@ class Oracle:
         self._f("sparse_coo")(n_rows, n_cols, n, r.ctypes.data_as(_L), c.ctypes.data_as(_L), _p(v),
                               rp.ctypes.data_as(_L), ci.ctypes.data_as(_L), _p(vv), _p(xv), _p(y))
         return rp, ci, vv, y
+
+
+    def sparse_right_multiply(self, a, b):
+        """A B (SparseOperator::right_multiply, cpp/sparse_operator.cpp:114-138) of two
+        CSR triples (n_rows, n_cols, row_ptrs, column_indices, values): the same form."""
+        (ar, ac, arp, aci, av), (br, bc, brp, bci, bv) = a, b
+        assert ac == br
+        arrs = [np.ascontiguousarray(x, dtype=t) for x, t in ((arp, np.int64), (aci, np.int64), (av, np.float64),
+                                                               (brp, np.int64), (bci, np.int64), (bv, np.float64))]
+        L = lambda x: x.ctypes.data_as(_L)  # noqa: E731
+        args = [ar, ac, L(arrs[0]), L(arrs[1]), _p(arrs[2]), bc, L(arrs[3]), L(arrs[4]), _p(arrs[5])]
+        nnz = self._f("sparse_right_multiply")(*args, -1, None, None, None)  # size query
+        rp, ci, v = np.zeros(ar + 1, dtype=np.int64), np.zeros(nnz, dtype=np.int64), np.zeros(nnz)
+        got = self._f("sparse_right_multiply")(*args, nnz, L(rp), L(ci), _p(v))
+        assert got == nnz
+        return ar, bc, rp, ci, v
+
+    def bem_sparse(self, which, n_components, facets, quad):
+        """The compiled reference's make_galerkin_operator ("galerkin") or
+        make_interpolation_operator ("interp") on a triangle mesh: (n_rows, n_cols,
+        row_ptrs, column_indices, values). Reference library only."""
+        f = _c(facets)
+        qx, qw = _c(quad[0]), _c(quad[1])
+        nnz = f.shape[0] * qw.shape[0] * 3 * n_components
+        n_rows_max = n_components * f.shape[0] * max(3, qw.shape[0])
+        rp = np.zeros(n_rows_max + 1, dtype=np.int64)
+        ci = np.zeros(nnz, dtype=np.int64)
+        v = np.zeros(nnz)
+        nr, nc = _SZ(), _SZ()
+        self.lib.ref_bem_sparse({"galerkin": 0, "interp": 1}[which], n_components, f.shape[0], _p(f),
+                                qw.shape[0], _p(qx), _p(qw), rp.ctypes.data_as(_L), ci.ctypes.data_as(_L),
+                                _p(v), ctypes.byref(nr), ctypes.byref(nc))
+        return nr.value, nc.value, rp[:nr.value + 1], ci, v
+
+    def nbody_from_bem(self, obs_facets, src_facets, obs_quad, src_quad):
+        """The compiled reference's nbody_data_from_bem: (obs_locs, obs_normals,
+        src_locs, src_normals, src_weights). Reference library only."""
+        of, sf = _c(obs_facets), _c(src_facets)
+        oqx, oqw, sqx, sqw = _c(obs_quad[0]), _c(obs_quad[1]), _c(src_quad[0]), _c(src_quad[1])
+        no, ns = of.shape[0] * oqw.shape[0], sf.shape[0] * sqw.shape[0]
+        out = [np.zeros((no, 3)), np.zeros((no, 3)), np.zeros((ns, 3)), np.zeros((ns, 3)), np.zeros(ns)]
+        self.lib.ref_nbody_from_bem(of.shape[0], _p(of), sf.shape[0], _p(sf), oqw.shape[0], _p(oqx), _p(oqw),
+                                    sqw.shape[0], _p(sqx), _p(sqw), *[_p(a) for a in out])
+        return tuple(out)
 
 
 class OracleFMM:

@@ -20,6 +20,10 @@
 #include "geometry.h"
 #include "blas_wrapper.h"
 #include "sparse_operator.h"
+#include "mesh.h"
+#include "quad_rule.h"
+#include "galerkin_operator.h"
+#include "interpolation_operator.h"
 
 #include <memory>
 #include <vector>
@@ -255,6 +259,83 @@ void ref_sparse_coo(size_t n_rows, size_t n_cols, size_t n_entries, const long* 
     if (x) {
         auto out = op.apply(std::vector<double>(x, x + n_cols));
         std::memcpy(y, out.data(), n_rows * sizeof(double));
+    }
+}
+
+// SparseOperator::right_multiply (cpp/sparse_operator.cpp:114-138) of A (a_rows x
+// a_cols) and B (a_cols x b_cols) in CSR; writes at most cap entries; returns nnz.
+long ref_sparse_right_multiply(size_t a_rows, size_t a_cols, const long* a_rp, const long* a_ci, const double* a_v,
+                               size_t b_cols, const long* b_rp, const long* b_ci, const double* b_v,
+                               long cap, long* rp, long* ci, double* v) {
+    auto mk = [](size_t nr, size_t nc, const long* rp_, const long* ci_, const double* v_) {
+        const size_t nnz = (size_t)rp_[nr];
+        return SparseOperator(nr, nc, std::vector<double>(v_, v_ + nnz),
+                              std::vector<size_t>(ci_, ci_ + nnz), std::vector<size_t>(rp_, rp_ + nr + 1));
+    };
+    auto out = mk(a_rows, a_cols, a_rp, a_ci, a_v).right_multiply(mk(a_cols, b_cols, b_rp, b_ci, b_v));
+    const long nnz = (long)out.values.size();
+    if (nnz > cap) return nnz;
+    for (size_t i = 0; i <= a_rows; i++) rp[i] = (long)out.row_ptrs[i];
+    for (long k = 0; k < nnz; k++) {
+        ci[k] = (long)out.column_indices[k];
+        v[k] = out.values[k];
+    }
+    return nnz;
+}
+
+static Mesh<3> ref_mesh(size_t n_facets, const double* f) {
+    std::vector<Facet<3>> facets(n_facets);
+    for (size_t i = 0; i < n_facets; i++)
+        for (int v = 0; v < 3; v++)
+            for (int d = 0; d < 3; d++) facets[i][v][d] = f[9 * i + 3 * v + d];
+    return Mesh<3>(facets);
+}
+
+static QuadRule<2> ref_quad(size_t nq, const double* qx, const double* qw) {
+    QuadRule<2> q;
+    for (size_t i = 0; i < nq; i++) q.push_back(QuadPt<2>{{{qx[2 * i], qx[2 * i + 1]}}, qw[i]});
+    return q;
+}
+
+// make_galerkin_operator (which 0, cpp/galerkin_operator.cpp:11-45) or
+// make_interpolation_operator (which 1, cpp/interpolation_operator.cpp:11-41) on a
+// triangle mesh (facets: n_facets x 3 vertices x xyz) and a quadrature rule on the
+// reference triangle; writes the CSR arrays (nnz = n_facets * nq * 3 * n_components).
+void ref_bem_sparse(int which, size_t n_components, size_t n_facets, const double* facets, size_t nq,
+                    const double* qx, const double* qw, long* row_ptrs, long* column_indices, double* values,
+                    size_t* n_rows, size_t* n_cols) {
+    auto mesh = ref_mesh(n_facets, facets);
+    auto quad = ref_quad(nq, qx, qw);
+    auto op = which == 0 ? make_galerkin_operator(n_components, mesh, quad)
+                         : make_interpolation_operator(n_components, mesh, quad);
+    *n_rows = op.n_rows();
+    *n_cols = op.n_cols();
+    for (size_t i = 0; i < op.row_ptrs.size(); i++) row_ptrs[i] = (long)op.row_ptrs[i];
+    for (size_t i = 0; i < op.values.size(); i++) {
+        column_indices[i] = (long)op.column_indices[i];
+        values[i] = op.values[i];
+    }
+}
+
+// nbody_data_from_bem (cpp/nbody_operator.h:38-85): quadrature points, facet
+// normals and, for the sources, quadrature weight times facet jacobian.
+void ref_nbody_from_bem(size_t n_obs_facets, const double* obs_facets, size_t n_src_facets,
+                        const double* src_facets, size_t nq_obs, const double* qx_obs, const double* qw_obs,
+                        size_t nq_src, const double* qx_src, const double* qw_src, double* obs_locs,
+                        double* obs_normals, double* src_locs, double* src_normals, double* src_weights) {
+    auto data = nbody_data_from_bem(ref_mesh(n_obs_facets, obs_facets), ref_mesh(n_src_facets, src_facets),
+                                    ref_quad(nq_obs, qx_obs, qw_obs), ref_quad(nq_src, qx_src, qw_src));
+    for (size_t i = 0; i < data.obs_locs.size(); i++)
+        for (int d = 0; d < 3; d++) {
+            obs_locs[3 * i + d] = data.obs_locs[i][d];
+            obs_normals[3 * i + d] = data.obs_normals[i][d];
+        }
+    for (size_t i = 0; i < data.src_locs.size(); i++) {
+        for (int d = 0; d < 3; d++) {
+            src_locs[3 * i + d] = data.src_locs[i][d];
+            src_normals[3 * i + d] = data.src_normals[i][d];
+        }
+        src_weights[i] = data.src_weights[i];
     }
 }
 

@@ -296,3 +296,42 @@ def test_sparse_oracle_matches_compiled_reference(orc, ref):
         b = ref.sparse_coo(n_rows, n_cols, rows, cols, vals, x)
         for u, w in zip(a, b):
             assert np.array_equal(u, w)
+
+
+def _random_csr(orc, rng, n_rows, n_cols, n):
+    rows, cols = rng.integers(0, n_rows, n), rng.integers(0, n_cols, n)
+    rp, ci, v, _ = orc.sparse_coo(n_rows, n_cols, rows, cols, rng.standard_normal(n))
+    return n_rows, n_cols, rp, ci, v
+
+
+def _dense(m):
+    n_rows, n_cols, rp, ci, v = m
+    out = np.zeros((n_rows, n_cols))
+    np.add.at(out, (np.repeat(np.arange(n_rows), np.diff(rp)), ci), v)
+    return out
+
+
+def test_sparse_right_multiply_oracle(orc):
+    """A B: columns ascending in each row, the dense product's values, and every
+    touched position kept, explicit zeros included (A = [1 1], B = [1; -1] gives one
+    stored 0.0)."""
+    rng = np.random.default_rng(13)
+    A, B = _random_csr(orc, rng, 60, 45, 400), _random_csr(orc, rng, 45, 33, 250)
+    n_rows, n_cols, rp, ci, v = orc.sparse_right_multiply(A, B)
+    assert (n_rows, n_cols) == (60, 33)
+    for r in range(n_rows):
+        assert np.all(np.diff(ci[rp[r]:rp[r + 1]]) > 0)
+    assert np.abs(_dense((n_rows, n_cols, rp, ci, v)) - _dense(A) @ _dense(B)).max() < 1e-13
+    z = orc.sparse_right_multiply((1, 2, [0, 2], [0, 1], [1.0, 1.0]), (2, 1, [0, 1, 2], [0, 0], [1.0, -1.0]))
+    assert z[2].tolist() == [0, 1] and z[3].tolist() == [0] and z[4].tolist() == [0.0]
+
+
+def test_sparse_right_multiply_oracle_matches_compiled_reference(orc, ref):
+    rng = np.random.default_rng(14)
+    for shape in ((1, 1, 1, 0, 0), (30, 20, 10, 150, 80), (200, 90, 70, 2000, 900)):
+        m, k, n, na, nb = shape
+        A, B = _random_csr(orc, rng, m, k, na), _random_csr(orc, rng, k, n, nb)
+        a, b = orc.sparse_right_multiply(A, B), ref.sparse_right_multiply(A, B)
+        assert a[:2] == b[:2]
+        for u, w in zip(a[2:], b[2:]):
+            assert np.array_equal(u, w)
