@@ -13,14 +13,16 @@
 // y_i = n_i + (g_i + c_i).
 //
 // Layout in HBM: values as FP64, column indices as int32 (n_cols < 2^31), row
-// pointers as int64, plus one int32 row boundary per row block.
+// pointers as int64, plus a 16-byte descriptor per row tile.
 //
-// Kernel (k_csr_apply): one workgroup of 256 threads per row block, a run of at
-// most 256 consecutive rows holding at most kChunk entries (a longer row is a block
-// of its own and is streamed in chunks of kChunk). The block loads its entries with
-// coalesced reads, forms the products values[k] * x[col[k]] and stores them in LDS;
-// then thread t adds up row r0 + t from LDS in stored order. HBM traffic per apply:
-// 12 B per entry, 16 B per row (row pointer, output), plus the gathered x.
+// Kernel (k_csr_apply): one workgroup of 512 threads per row tile, a run of at most
+// 512 consecutive rows holding at most kChunk = 4096 entries (a longer row is a tile
+// of its own and is streamed in chunks of kChunk). The tile's first entry comes with
+// the tile, so the entry loads start without waiting for a row pointer. The block
+// loads its entries with coalesced reads, forms the products values[k] * x[col[k]]
+// and stores them in LDS; then thread t adds up row r0 + t from LDS in stored order.
+// HBM traffic per apply: 12 B per entry, 16 B per row (row pointer, output), plus
+// the gathered x.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -32,13 +34,20 @@ namespace tbem {
 
 void set_last_error(const std::string& m);
 
-constexpr int kCsrBlock = 256;   // threads per workgroup, also the most rows per block
-constexpr int kChunk = 2048;     // entries staged in LDS per pass (16 KB of products)
+constexpr int kCsrBlock = 512;   // threads per workgroup, also the most rows per tile
+constexpr int kChunk = 4096;     // entries staged in LDS per pass (32 KB of products)
 constexpr int kPerThread = kChunk / kCsrBlock;
 
-// One row block per workgroup. out_a and out_b are optional addends: with both
+// A row tile: rows r0 .. r0 + nr - 1, whose entries start at k0. The tile list ends
+// with {n_rows, 0, nnz}, so tile b's entries end at tiles[b + 1].k0.
+struct CsrTile {
+    int r0, nr;
+    long k0;
+};
+
+// One row tile per workgroup. out_a and out_b are optional addends: with both
 // given, y[r] = out_a[r] + (sum_r + out_b[r]).
-__global__ __launch_bounds__(kCsrBlock) void k_csr_apply(const int* __restrict__ blk_row,
+__global__ __launch_bounds__(kCsrBlock) void k_csr_apply(const CsrTile* __restrict__ tiles,
                                                        const long* __restrict__ rp,
                                                        const int* __restrict__ ci,
                                                        const double* __restrict__ val,
@@ -47,19 +56,17 @@ __global__ __launch_bounds__(kCsrBlock) void k_csr_apply(const int* __restrict__
                                                        const double* __restrict__ out_b,
                                                        double* __restrict__ y) {
     __shared__ double prod[kChunk];
-    __shared__ long rps[kCsrBlock + 1];
     const int t = threadIdx.x;
-    const int r0 = uni(blk_row[blockIdx.x]);
-    const int nr = uni(blk_row[blockIdx.x + 1]) - r0;  // 1..kCsrBlock
-    for (int i = t; i <= nr; i += kCsrBlock) rps[i] = rp[r0 + i];
-    __syncthreads();
-    const long k0 = rps[0];
-    const long total = rps[nr] - k0;
+    const CsrTile tl = tiles[blockIdx.x];
+    const int r0 = uni(tl.r0);
+    const int nr = uni(tl.nr);  // 1..kCsrBlock
+    const long k0 = tl.k0;
+    const long total = tiles[blockIdx.x + 1].k0 - k0;
     const int n_pass = uni((int)((total + kChunk - 1) / kChunk));
     long mb = 0, me = 0;
     if (t < nr) {
-        mb = rps[t];
-        me = rps[t + 1];
+        mb = rp[r0 + t];
+        me = rp[r0 + t + 1];
     }
     double s = 0.0;
     for (int pass = 0; pass < n_pass; pass++) {
@@ -67,11 +74,12 @@ __global__ __launch_bounds__(kCsrBlock) void k_csr_apply(const int* __restrict__
         const int n = (int)std::min<long>(kChunk, k0 + total - c0);
         int col[kPerThread];
         double v[kPerThread];
+        // entries are read once: non-temporal loads keep them from displacing x in the caches
 #pragma unroll
         for (int j = 0; j < kPerThread; j++) {
             const int i = t + j * kCsrBlock;
-            col[j] = i < n ? ci[c0 + i] : 0;
-            v[j] = i < n ? val[c0 + i] : 0.0;
+            col[j] = i < n ? __builtin_nontemporal_load(ci + c0 + i) : 0;
+            v[j] = i < n ? __builtin_nontemporal_load(val + c0 + i) : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < kPerThread; j++) {
@@ -98,20 +106,21 @@ __global__ __launch_bounds__(kCsrBlock) void k_csr_apply(const int* __restrict__
     }
 }
 
-// Row blocks: runs of consecutive rows, at most kCsrBlock rows and kChunk entries
-// each; a row longer than kChunk forms a block alone.
-static std::vector<int> row_blocks(const std::vector<long>& rp) {
+// Row tiles: runs of consecutive rows, at most kCsrBlock rows and kChunk entries
+// each; a row longer than kChunk forms a tile alone. Ends with {n_rows, 0, nnz}.
+static std::vector<CsrTile> row_tiles(const std::vector<long>& rp) {
     const int n_rows = (int)rp.size() - 1;
-    std::vector<int> b{0};
+    std::vector<CsrTile> tiles;
     int r = 0;
     while (r < n_rows) {
         const int r_start = r;
         const long k_start = rp[r];
         r++;
         while (r < n_rows && r - r_start < kCsrBlock && rp[r + 1] - k_start <= kChunk) r++;
-        b.push_back(r);
+        tiles.push_back(CsrTile{r_start, r - r_start, k_start});
     }
-    return b;
+    tiles.push_back(CsrTile{n_rows, 0, rp[n_rows]});
+    return tiles;
 }
 
 }  // namespace tbem
@@ -124,7 +133,7 @@ struct tbem_sparse_s {
     tbem::DevBuf<double> d_val;
     tbem::DevBuf<int> d_col;
     tbem::DevBuf<long> d_rp;
-    tbem::DevBuf<int> d_blk;
+    tbem::DevBuf<tbem::CsrTile> d_tiles;
     int n_blocks = 0;
     tbem::DevBuf<double> xb, yb;  // staging for the host entry point
     hipStream_t stream = nullptr;
@@ -189,12 +198,12 @@ static std::unique_ptr<tbem_sparse_s> make_sparse(size_t n_rows, size_t n_cols, 
     h->nnz = nnz;
     TBEM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     TBEM_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
-    const std::vector<int> blk = row_blocks(row_ptrs);
-    h->n_blocks = (int)blk.size() - 1;
+    const std::vector<CsrTile> tiles = row_tiles(row_ptrs);
+    h->n_blocks = (int)tiles.size() - 1;
     h->d_val.upload(values, h->stream);
     h->d_col.upload(cols, h->stream);
     h->d_rp.upload(row_ptrs, h->stream);
-    h->d_blk.upload(blk, h->stream);
+    h->d_tiles.upload(tiles, h->stream);
     TBEM_HIP(hipStreamSynchronize(h->stream));
     h->values = std::move(values);
     h->row_ptrs = std::move(row_ptrs);
@@ -216,7 +225,7 @@ static void sparse_run(tbem_sparse_s& A, const double* x, const double* a, const
                        hipStream_t s) {
     if (A.n_blocks == 0) return;
     TBEM_REQUIRE(y && (x || A.nnz == 0), TBEM_E_INVALID, "null device buffer");
-    k_csr_apply<<<A.n_blocks, kCsrBlock, 0, s>>>(A.d_blk.p, A.d_rp.p, A.d_col.p, A.d_val.p, x, a, b, y);
+    k_csr_apply<<<A.n_blocks, kCsrBlock, 0, s>>>(A.d_tiles.p, A.d_rp.p, A.d_col.p, A.d_val.p, x, a, b, y);
     TBEM_LAUNCH_CHECK();
 }
 

@@ -152,3 +152,61 @@ def uniform_cube(n, seed=SEED, dim=1):
     weights = rng.uniform(0.5, 1.5, n)
     x = np.random.default_rng(seed + 1).standard_normal(dim * n)
     return dict(locs=locs, normals=normals, weights=weights, x=x)
+
+
+def rough_fault_mesh(nx=300, ny=300, seed=SEED):
+    """Triangulated rough fault: an nx x ny grid over [0,1]^2, two triangles per cell,
+    vertices on z = 0.05 sin(6x) cos(4y) + 0.005 N(0,1). Facets (2 nx ny, 3, 3), the
+    two triangles of cell (i, j) at 2 (i ny + j) and 2 (i ny + j) + 1."""
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.linspace(0, 1, nx + 1), np.linspace(0, 1, ny + 1), indexing="ij")
+    Z = 0.05 * np.sin(6 * X) * np.cos(4 * Y) + 0.005 * rng.standard_normal(X.shape)
+    V = np.stack([X, Y, Z], axis=-1)
+    a, b, c, d = V[:-1, :-1], V[1:, :-1], V[:-1, 1:], V[1:, 1:]
+    t0 = np.stack([a, b, d], axis=2)
+    t1 = np.stack([a, d, c], axis=2)
+    return np.stack([t0, t1], axis=2).reshape(-1, 3, 3)
+
+
+def near_field_csr(nx=300, ny=300, components=3, seed=SEED):
+    """Sparsity of a Galerkin near-field operator on rough_fault_mesh(nx, ny): the row
+    of dof (d, facet f, corner b), at d n_dofs + 3 f + b, holds every (d', g, b') with g
+    one of the up to 18 facets of the 3 x 3 grid cells around f's cell, columns
+    ascending (d' n_dofs + 3 g + b', the order right_multiply leaves them in).
+    Values uniform in [-0.5, 0.5). Returns (n_rows, n_cols, row_ptrs, column_indices,
+    values) with n_rows = n_cols = components * 3 * 2 nx ny."""
+    F = 2 * nx * ny
+    n_dofs = 3 * F
+    ci_, cj_ = np.divmod(np.arange(F) // 2, ny)
+    near = np.full((F, 18), -1, dtype=np.int64)
+    k = 0
+    for di in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            i2, j2 = ci_ + di, cj_ + dj
+            ok = (i2 >= 0) & (i2 < nx) & (j2 >= 0) & (j2 < ny)
+            base = 2 * (i2 * ny + j2)
+            for t in (0, 1):
+                near[:, k] = np.where(ok, base + t, -1)
+                k += 1
+    near.sort(axis=1)  # invalid (-1) first, then ascending facets
+    valid = near >= 0
+    count = valid.sum(1)
+    # columns of one facet's rows: (d', g, b') ascending
+    cols = (np.arange(components)[:, None, None, None] * n_dofs + 3 * near[None, :, :, None].clip(0)
+            + np.arange(3)[None, None, None, :])                     # (C, F, 18, 3)
+    cols = np.transpose(cols, (1, 0, 2, 3)).reshape(F, -1)              # (F, C*18*3)
+    vmask = np.broadcast_to(valid[:, None, :, None], (F, components, 18, 3)).reshape(F, -1)
+    per_row = components * 3 * count                                    # entries of each row of facet f
+    row_len = np.broadcast_to(per_row[None, :, None], (components, F, 3)).reshape(-1)
+    row_ptrs = np.zeros(row_len.size + 1, dtype=np.int64)
+    np.cumsum(row_len, out=row_ptrs[1:])
+    packed = cols[vmask]                                                # facet-major, valid only
+    fstart = np.zeros(F + 1, dtype=np.int64)
+    np.cumsum(per_row, out=fstart[1:])
+    # row (d, f, b) copies facet f's packed columns
+    rows_f = np.broadcast_to(np.arange(F)[None, :, None], (components, F, 3)).reshape(-1)
+    idx = np.repeat(fstart[rows_f] - row_ptrs[:-1], row_len) + np.arange(row_ptrs[-1])
+    column_indices = packed[idx]
+    values = np.random.default_rng(seed + 1).random(row_ptrs[-1]) - 0.5
+    n = components * n_dofs
+    return n, n, row_ptrs, column_indices, values

@@ -37,6 +37,7 @@ METRIC = "FMM matvec Mpts/s at stated rel err, FP32-MFMA vs FP64; bwd/fwd time r
 FLOPS_PER_PAIR = {0: 11, 1: 19, 2: 0, 3: 28, 4: 60, 5: 60, 6: 120, 7: 2, 8: 6}
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per launch of the FP64 leaf stage on the default workload: rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE of the same command, scaled by what the counters
 # read for a plain device copy (profiles/k_leaf_stage_elastic_u_1M_order6.md).
@@ -319,6 +320,77 @@ def cfg5_run(tbem, torch):
     return out
 
 
+def near_field_run(tbem, torch, steps, warmup, skip_cpu):
+    """The integral operator y = N x + (G F(I x) + C x) on a triangulated rough fault
+    (300 x 300 grid cells, 180,000 facets; far field on the 3-point rule's 540,000
+    points, elastic hypersingular FMM, order 50, leaf 128, MAC 0.3) with a near field N
+    and a correction C of a Galerkin near-field operator's sparsity (every dof row
+    holds the 3 x 3 x 18 dofs of the facets in the 3 x 3 grid cells around its own:
+    1,620,000 rows, 261M entries). Times the CSR apply kernel alone (HIP events on the
+    stream it runs on) and the whole integral operator; the CPU reference
+    SparseOperator::apply runs on a bounded sample of the rows."""
+    wl = importlib.import_module("3bem_amd.workloads")
+    bem = importlib.import_module("3bem_amd.bem")
+    nx = ny = 300
+    t0 = time.perf_counter()
+    n, m, rp, ci, v = wl.near_field_csr(nx, ny)
+    N = tbem.SparseOperator(n, m, v, ci, rp)
+    C = tbem.SparseOperator(n, m, -0.1 * v, ci, rp)
+    t_build = time.perf_counter() - t0
+    nnz = int(rp[-1])
+    x = torch.rand(m, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(3)) - 0.5
+    for _ in range(warmup):
+        y = N.apply(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()  # the apply runs on torch's current stream: these events bracket its launches
+    for _ in range(steps):
+        y = N.apply(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    alg_bytes = 12 * nnz + 16 * n + 8 * m
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    out = {"workload": f"CSR near-field apply, {n} x {m}, {nnz} entries (18-facet Galerkin pattern on a "
+                       f"{nx} x {ny}-cell triangulated rough fault), FP64",
+           "csr_build_s": round(t_build, 2), "csr_apply_ms": round(ms, 4),
+           "roofline": {"bound": "hbm", "kernel": "k_csr_apply", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes": alg_bytes,
+                        "bytes_rule": "12 B per entry (FP64 value, int32 column), 16 B per row (int64 row "
+                                      "pointer, FP64 output), 8 B per column (x)"}}
+    log(f"[bench] near field: {nnz} entries, k_csr_apply {ms:.3f} ms = {gbs:.0f} GB/s")
+    if not skip_cpu:
+        from oracle.oracle import Oracle, available
+        kind = "reference" if available("ref") else "port"
+        o = Oracle("ref" if kind == "reference" else "orc")
+        rows = int(np.searchsorted(rp, 40_000_000))
+        xh = x.cpu().numpy()
+        y_cpu, t_cpu = o.sparse_csr_apply(rows, m, rp[:rows + 1], ci[:rp[rows]], v[:rp[rows]], xh, reps=3)
+        y_gpu = y.cpu().numpy()[:rows]
+        out["cpu_baseline"] = {"value": round(int(rp[rows]) / t_cpu / 1e9, 4), "unit": "G entries/s", "cores": 1,
+                               "kind": kind, "sample": f"first {rows} rows ({int(rp[rows])} entries), "
+                                                       "SparseOperator::apply mean of 3"}
+        out["gpu_g_entries_per_s"] = round(nnz / (ms * 1e-3) / 1e9, 3)
+        out["bit_exact_vs_cpu_sample"] = bool(np.array_equal(y_gpu.view(np.uint64), y_cpu.view(np.uint64)))
+        log(f"[bench] near field CPU ({kind}) {out['cpu_baseline']['value']} G entries/s; "
+            f"bit-exact {out['bit_exact_vs_cpu_sample']}")
+    del rp, ci, v
+    facets = wl.rough_fault_mesh(nx, ny)
+    q3 = (np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]), np.full(3, 1 / 6))
+    K = tbem.ElasticHypersingular(1.0, 0.25)
+    cfg = tbem.FMMConfig(0.3, 50, 128, 0.05, False)
+    for prec in ("fp64", "mixed"):
+        op = bem.integral_operator(N, C, facets, facets, q3, q3, K, cfg, precision=prec)
+        dt_far, _ = time_steps(lambda: op.farfield.apply(torch.zeros(op.farfield.n_cols(), dtype=torch.float64,
+                                                                     device="cuda")), steps, warmup, torch)
+        dt, _ = time_steps(lambda: op.apply(x), steps, warmup, torch)
+        out[f"integral_apply_ms_{prec}"] = round(dt * 1e3, 3)
+        out[f"far_field_fmm_ms_{prec}"] = round(dt_far * 1e3, 3)
+        log(f"[bench] integral operator ({prec}): {dt * 1e3:.2f} ms (FMM far field alone {dt_far * 1e3:.2f} ms)")
+        del op
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,9 +409,15 @@ def main():
                     help="skip the differentiable forward model (120k fault points, 200k stations)")
     ap.add_argument("--skip-cfg5", action="store_true",
                     help="skip the 2M-point hypersingular GMRES solve")
+    ap.add_argument("--skip-near", action="store_true",
+                    help="skip the CSR near field / integral operator section")
+    ap.add_argument("--only-near", action="store_true",
+                    help="run only the headline FP64 step and the near field / integral operator section")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.only_near:
+        args.skip_target = args.skip_laplace = args.skip_cfg4 = args.skip_cfg5 = True
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the FMM "
               "sharding over several GPUs is not implemented", file=sys.stderr)
@@ -399,7 +477,7 @@ def main():
                       "plan_build_s": round(plan64, 3), "stage_ms": {k: round(v, 4) for k, v in stages.items()}},
            "roofline": roofline}
 
-    if not args.skip_extra:
+    if not args.skip_extra and not args.only_near:
         # accuracy vs FP64 direct sum on a 10k-target subsample
         rng = np.random.default_rng(123)
         sub = np.sort(rng.choice(n, min(n, 10_000), replace=False))
@@ -443,6 +521,8 @@ def main():
         out["forward_model_cfg4"] = cfg4_run(tbem, torch, 5)
     if not args.skip_extra and not args.skip_cfg5 and args.config == "cfg2":
         out["bem_solve_cfg5"] = cfg5_run(tbem, torch)
+    if not args.skip_extra and not args.skip_near and args.config == "cfg2":
+        out["integral_operator_near_field"] = near_field_run(tbem, torch, args.steps, args.warmup, args.skip_cpu)
 
     if not args.skip_cpu:
         cfg_t = (cfg.mac, cfg.order, cfg.min_pts_per_cell, cfg.d, cfg.account_for_small_cells)

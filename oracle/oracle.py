@@ -78,6 +78,8 @@ class Oracle:
         f("surface").argtypes = [_SZ, _D]
         f("pinv").argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]
         f("sparse_coo").argtypes = [_SZ, _SZ, _SZ, _L, _L, _D, _L, _L, _D, _D, _D]
+        f("sparse_csr_apply").restype = ctypes.c_double
+        f("sparse_csr_apply").argtypes = [_SZ, _SZ, _L, _L, _D, _D, _D, ctypes.c_int]
         f("sparse_right_multiply").restype = ctypes.c_long
         f("sparse_right_multiply").argtypes = [_SZ, _SZ, _L, _L, _D, _SZ, _L, _L, _D, ctypes.c_long, _L, _L, _D]
         if prefix == "ref":
@@ -174,6 +176,17 @@ class Oracle:
                               rp.ctypes.data_as(_L), ci.ctypes.data_as(_L), _p(vv), _p(xv), _p(y))
         return rp, ci, vv, y
 
+
+    def sparse_csr_apply(self, n_rows, n_cols, row_ptrs, column_indices, values, x, reps=1):
+        """SparseOperator::apply on CSR arrays, reps times: (y, mean seconds per apply)."""
+        rp = np.ascontiguousarray(row_ptrs, dtype=np.int64)
+        ci = np.ascontiguousarray(column_indices, dtype=np.int64)
+        v, xv = _c(values), _c(x)
+        assert rp.shape[0] == n_rows + 1 and xv.shape[0] == n_cols
+        y = np.zeros(n_rows)
+        t = self._f("sparse_csr_apply")(n_rows, n_cols, rp.ctypes.data_as(_L), ci.ctypes.data_as(_L), _p(v),
+                                        _p(xv), _p(y), reps)
+        return y, t
 
     def sparse_right_multiply(self, a, b):
         """A B (SparseOperator::right_multiply, cpp/sparse_operator.cpp:114-138) of two

@@ -11,6 +11,7 @@
 //   Kernel<3,R,C>::operator()                              (cpp/kernel.h:19-33)
 // Only tests/, tests/golden/ and bench.py's cpu_baseline leg load it.
 
+#include <chrono>
 #include "fmm.h"
 #include "nbody_operator.h"
 #include "laplace_kernels.h"
@@ -281,6 +282,21 @@ long ref_sparse_right_multiply(size_t a_rows, size_t a_cols, const long* a_rp, c
         v[k] = out.values[k];
     }
     return nnz;
+}
+
+// SparseOperator::apply (cpp/sparse_operator.cpp:19-28) on CSR arrays, reps times;
+// returns the mean seconds per apply (construction excluded) and leaves y.
+double ref_sparse_csr_apply(size_t n_rows, size_t n_cols, const long* rp, const long* ci, const double* v,
+                            const double* x, double* y, int reps) {
+    const size_t nnz = (size_t)rp[n_rows];
+    SparseOperator A(n_rows, n_cols, std::vector<double>(v, v + nnz), std::vector<size_t>(ci, ci + nnz),
+                     std::vector<size_t>(rp, rp + n_rows + 1));
+    std::vector<double> xv(x, x + n_cols), out;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; r++) out = A.apply(xv);
+    auto t1 = std::chrono::steady_clock::now();
+    std::memcpy(y, out.data(), n_rows * sizeof(double));
+    return std::chrono::duration<double>(t1 - t0).count() / reps;
 }
 
 static Mesh<3> ref_mesh(size_t n_facets, const double* f) {

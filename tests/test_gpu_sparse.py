@@ -3,7 +3,7 @@
 * SparseOperator.apply equals the oracle's restatement of SparseOperator::apply
   (cpp/sparse_operator.cpp:19-28) bit for bit, on the reference's own cases
   (unit_tests/test_sparse.cpp) and on ragged matrices that exercise the row blocking:
-  empty rows, row blocks cut by the 256-row and 2048-entry limits, rows longer than
+  empty rows, row tiles cut by the 512-row and 4096-entry limits, rows longer than
   one LDS pass, duplicates, signed zeros, subnormals and wide magnitudes.
 * csr_from_coo and right_multiply equal the oracle's CSR arrays bit for bit.
 * IntegralOperator: with the direct far field, y equals N x + (G F(I x) + C x) formed
@@ -59,8 +59,8 @@ def ragged_case(rng, n_rows, n_cols, lengths, special=False):
 CASES = {
     # name: (n_rows, n_cols, row lengths)
     "short_rows": (5000, 3000, lambda r, n: r.integers(0, 6, n)),
-    "row_cap_256": (1024, 50, lambda r, n: np.ones(n, dtype=int)),
-    "chunk_edges": (8, 700, lambda r, n: np.array([2048, 2049, 1, 2047, 0, 4096, 1, 10001])),
+    "row_cap_512": (2100, 50, lambda r, n: np.ones(n, dtype=int)),
+    "chunk_edges": (12, 700, lambda r, n: np.array([2048, 2049, 4096, 4097, 1, 4095, 0, 8192, 1, 10001, 511, 513])),
     "near_field_like": (3000, 9000, lambda r, n: r.integers(60, 180, n)),
     "mostly_empty": (20000, 100, lambda r, n: (r.uniform(0, 1, n) < 0.01) * r.integers(1, 40, n)),
     "special_values": (300, 64, lambda r, n: r.integers(0, 30, n)),
