@@ -71,17 +71,19 @@ def build(verbose=True, jobs=None):
 
 
 SHIM_EXAMPLE = os.path.join(HERE, "tbem_shim_example")
+SPARSE_EXAMPLE = os.path.join(HERE, "tbem_sparse_example")
 
 
 def build_shim_example():
-    """C++ caller of the library through include/tbem/fmm.hpp (host code only, g++)."""
-    src = os.path.join(ROOT, "tests", "cpp", "shim_example.cpp")
-    cmd = ["g++", "-O2", "-std=c++14", "-I", os.path.join(ROOT, "include"), src, "-L", HERE, "-ltbem_fmm",
-           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", SHIM_EXAMPLE]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
-    if r.returncode != 0:
-        print(r.stdout.decode(errors="replace"), file=sys.stderr)
-        raise RuntimeError("shim example build failed")
+    """C++ callers of the library through include/tbem/fmm.hpp (host code only, g++)."""
+    for name, exe in (("shim_example.cpp", SHIM_EXAMPLE), ("sparse_example.cpp", SPARSE_EXAMPLE)):
+        src = os.path.join(ROOT, "tests", "cpp", name)
+        cmd = ["g++", "-O2", "-std=c++14", "-I", os.path.join(ROOT, "include"), src, "-L", HERE, "-ltbem_fmm",
+               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", exe]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            print(r.stdout.decode(errors="replace"), file=sys.stderr)
+            raise RuntimeError(f"{name} build failed")
 
 
 if __name__ == "__main__":

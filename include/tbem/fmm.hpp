@@ -1,4 +1,4 @@
-// tbem/fmm.hpp — header-only C++ surface of 3bem's FMM and direct N-body operators
+// tbem/fmm.hpp — header-only C++ surface of 3bem's FMM, direct N-body, sparse and integral operators
 // on top of the C ABI in tbem_fmm.h (link -ltbem_fmm).
 //
 // Mirrors (reference file:line):
@@ -10,6 +10,10 @@
 //   poisson_ratio members                              cpp/laplace_kernels.h, cpp/elastic_kernels.h
 //   FMMOperator<3,R,C>(K, data, config).apply(x)       cpp/fmm.h:131-237
 //   make_direct_nbody_operator(data, K), nbody_eval    cpp/nbody_operator.h:117-152
+//   MatrixEntry, SparseOperator {apply, nnz, right_multiply, csr_from_coo}
+//                                                      cpp/sparse_operator.h:12-60
+//   IntegralOperator<3,R,C>(near, correction, galerkin, farfield, interp)
+//                                                      cpp/integral_operator.h:26-64
 // Errors (nonzero status) become std::runtime_error carrying tbem_last_error().
 #ifndef TBEM_FMM_HPP
 #define TBEM_FMM_HPP
@@ -205,6 +209,142 @@ std::vector<double> nbody_eval(const Kernel<3, R, C>& K, const NBodyData<3>& dat
     auto op = make_direct_nbody_operator(data, K);
     return op.apply(std::vector<double>(x, x + op.n_cols()));
 }
+
+// MatrixEntry (cpp/sparse_operator.h:12-20).
+struct MatrixEntry {
+    size_t loc[2];
+    double value;
+    size_t row() const { return loc[0]; }
+    size_t col() const { return loc[1]; }
+};
+
+struct OperatorShape {
+    size_t n_rows;
+    size_t n_cols;
+};
+
+// SparseOperator (cpp/sparse_operator.h:21-60): the CSR arrays as the reference's
+// members, the matrix on the GPU; apply() equals SparseOperator::apply bit for bit.
+struct SparseOperator : public OperatorI {
+    std::shared_ptr<tbem_sparse_s> h;
+    const OperatorShape shape;
+    const std::vector<double> values;
+    const std::vector<size_t> column_indices;
+    const std::vector<size_t> row_ptrs;
+
+    SparseOperator(size_t n_rows, size_t n_cols, const std::vector<double>& values,
+                   const std::vector<size_t>& column_indices, const std::vector<size_t>& row_ptrs)
+        : shape{n_rows, n_cols}, values(values), column_indices(column_indices), row_ptrs(row_ptrs) {
+        if (row_ptrs.size() != n_rows + 1 || column_indices.size() != values.size())
+            throw std::invalid_argument("SparseOperator: need n_rows + 1 row_ptrs and one column per value");
+        tbem_sparse_t raw = nullptr;
+        check(tbem_sparse_create(n_rows, n_cols, values.size(), values.data(), column_indices.data(),
+                                 row_ptrs.data(), &raw));
+        h.reset(raw, tbem_sparse_destroy);
+    }
+
+    size_t n_rows() const override { return shape.n_rows; }
+    size_t n_cols() const override { return shape.n_cols; }
+    size_t nnz() const { return row_ptrs.back(); }
+
+    std::vector<double> apply(const std::vector<double>& x) const override {
+        if (x.size() != n_cols()) throw std::invalid_argument("SparseOperator::apply: x has the wrong length");
+        std::vector<double> y(n_rows());
+        check(tbem_sparse_apply(h.get(), x.data(), y.data()));
+        return y;
+    }
+
+    std::unique_ptr<OperatorI> clone() const override {
+        return std::unique_ptr<OperatorI>(new SparseOperator(*this));
+    }
+
+    // SparseOperator::right_multiply (cpp/sparse_operator.cpp:114-138), bit for bit.
+    SparseOperator right_multiply(const SparseOperator& other) const {
+        tbem_sparse_t raw = nullptr;
+        check(tbem_sparse_right_multiply(h.get(), other.h.get(), &raw));
+        return from_handle(raw);
+    }
+
+    // SparseOperator::csr_from_coo (cpp/sparse_operator.cpp:140-172).
+    static SparseOperator csr_from_coo(size_t n_rows, size_t n_cols, const std::vector<MatrixEntry>& entries) {
+        std::vector<size_t> r(entries.size()), c(entries.size());
+        std::vector<double> v(entries.size());
+        for (size_t i = 0; i < entries.size(); i++) {
+            r[i] = entries[i].row();
+            c[i] = entries[i].col();
+            v[i] = entries[i].value;
+        }
+        tbem_sparse_t raw = nullptr;
+        check(tbem_sparse_from_coo(n_rows, n_cols, entries.size(), r.data(), c.data(), v.data(), &raw));
+        return from_handle(raw);
+    }
+
+private:
+    SparseOperator(std::shared_ptr<tbem_sparse_s> handle, std::vector<double> v, std::vector<size_t> c,
+                   std::vector<size_t> r, size_t n_rows, size_t n_cols)
+        : h(std::move(handle)), shape{n_rows, n_cols}, values(std::move(v)), column_indices(std::move(c)),
+          row_ptrs(std::move(r)) {}
+
+    static SparseOperator from_handle(tbem_sparse_t raw) {
+        std::shared_ptr<tbem_sparse_s> handle(raw, tbem_sparse_destroy);
+        const size_t nr = tbem_sparse_n_rows(raw), nc = tbem_sparse_n_cols(raw), nnz = tbem_sparse_nnz(raw);
+        std::vector<double> v(nnz);
+        std::vector<size_t> c(nnz), r(nr + 1);
+        check(tbem_sparse_csr(raw, v.data(), c.data(), r.data()));
+        return SparseOperator(std::move(handle), std::move(v), std::move(c), std::move(r), nr, nc);
+    }
+};
+
+// IntegralOperator<3,R,C> (cpp/integral_operator.h:26-64): y = N x + (G F(I x) + C x)
+// on the GPU. The far field must be an FMMOperator<3,R,C> or a DirectNBodyOperator.
+template <size_t dim, size_t R, size_t C>
+struct IntegralOperator : public OperatorI {
+    const SparseOperator nearfield;
+    const SparseOperator farfield_correction;
+    const SparseOperator galerkin;
+    const std::shared_ptr<OperatorI> farfield;
+    const SparseOperator interp;
+    std::shared_ptr<tbem_integral_s> h;
+
+    IntegralOperator(const SparseOperator& nearfield, const SparseOperator& farfield_correction,
+                     const SparseOperator& galerkin, const std::shared_ptr<OperatorI>& farfield,
+                     const SparseOperator& interp)
+        : nearfield(nearfield), farfield_correction(farfield_correction), galerkin(galerkin), farfield(farfield),
+          interp(interp) {
+        int kind;
+        void* far;
+        if (auto f = dynamic_cast<FMMOperator<dim, R, C>*>(farfield.get())) {
+            kind = TBEM_FARFIELD_FMM;
+            far = f->h.get();
+        } else if (auto d = dynamic_cast<DirectNBodyOperator*>(farfield.get())) {
+            kind = TBEM_FARFIELD_DIRECT;
+            far = d->h.get();
+        } else {
+            throw std::invalid_argument("IntegralOperator: the far field must be an FMMOperator or a "
+                                        "DirectNBodyOperator");
+        }
+        tbem_integral_t raw = nullptr;
+        check(tbem_integral_create(this->nearfield.h.get(), this->farfield_correction.h.get(),
+                                   this->galerkin.h.get(), kind, far, this->interp.h.get(), &raw));
+        h.reset(raw, tbem_integral_destroy);
+    }
+
+    size_t n_rows() const override { return galerkin.n_rows(); }
+    size_t n_cols() const override { return nearfield.n_cols(); }
+
+    std::vector<double> apply(const std::vector<double>& x) const override {
+        if (x.size() != n_cols()) throw std::invalid_argument("IntegralOperator::apply: x has the wrong length");
+        std::vector<double> y(n_rows());
+        check(tbem_integral_apply(h.get(), x.data(), y.data()));
+        return y;
+    }
+
+    std::unique_ptr<OperatorI> clone() const override {
+        return std::unique_ptr<OperatorI>(new IntegralOperator(nearfield, farfield_correction, galerkin,
+                                                               std::shared_ptr<OperatorI>(farfield->clone()),
+                                                               interp));
+    }
+};
 
 }  // namespace tbem
 
