@@ -28,6 +28,9 @@ EXPORTED = (
     "tbem_sparse_csr", "tbem_sparse_clone", "tbem_sparse_right_multiply", "tbem_sparse_n_rows", "tbem_sparse_n_cols", "tbem_sparse_nnz",
     "tbem_sparse_destroy", "tbem_integral_create", "tbem_integral_apply", "tbem_integral_apply_dev",
     "tbem_integral_n_rows", "tbem_integral_n_cols", "tbem_integral_destroy",
+    "tbem_constraints_create", "tbem_constraints_size", "tbem_constraints_n_terms", "tbem_constraints_get",
+    "tbem_constraints_homogenize", "tbem_distribute_vector", "tbem_distribute_vector_dev", "tbem_condense_vector",
+    "tbem_condense_vector_dev", "tbem_condense_sparse", "tbem_constraints_destroy",
     "tbem_kernel_eval", "tbem_surface", "tbem_pinv", "tbem_last_error", "tbem_device_available",
 )
 FARFIELD_FMM, FARFIELD_DIRECT = 0, 1
@@ -132,6 +135,17 @@ def load(path=LIB_PATH):
         "tbem_integral_n_rows": (_SZ, [_VP]),
         "tbem_integral_n_cols": (_SZ, [_VP]),
         "tbem_integral_destroy": (None, [_VP]),
+        "tbem_constraints_create": (ctypes.c_int, [_SZ, _SZP, _SZP, _D, _D, ctypes.POINTER(_VP)]),
+        "tbem_constraints_size": (_SZ, [_VP]),
+        "tbem_constraints_n_terms": (_SZ, [_VP]),
+        "tbem_constraints_get": (ctypes.c_int, [_VP, _SZP, _SZP, _SZP, _D, _D]),
+        "tbem_constraints_homogenize": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
+        "tbem_distribute_vector": (ctypes.c_int, [_VP, _D, _SZ, _D]),
+        "tbem_distribute_vector_dev": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP]),
+        "tbem_condense_vector": (ctypes.c_int, [_VP, _D, _SZ, _D]),
+        "tbem_condense_vector_dev": (ctypes.c_int, [_VP, _VP, _SZ, _VP, _VP]),
+        "tbem_condense_sparse": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(_VP)]),
+        "tbem_constraints_destroy": (None, [_VP]),
         "tbem_kernel_eval": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _SZ, _D, _D, _D, _D, _D]),
         "tbem_surface": (ctypes.c_long, [_SZ, _D]),
         "tbem_pinv": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_double, _D]),

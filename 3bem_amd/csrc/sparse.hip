@@ -261,6 +261,22 @@ static void integral_apply(tbem_integral_s& I, const double* x, double* y, hipSt
     TBEM_HIP(hipEventRecord(I.done, s));
 }
 
+// For the other translation units: a CSR operator from host arrays, and a read-only
+// view of one's host copy.
+tbem_sparse_s* sparse_from_host(size_t n_rows, size_t n_cols, std::vector<double> values, std::vector<long> row_ptrs,
+                                std::vector<int> cols) {
+    return make_sparse(n_rows, n_cols, std::move(values), std::move(row_ptrs), std::move(cols)).release();
+}
+
+void sparse_host_view(tbem_sparse_s* h, size_t* n_rows, size_t* n_cols, const std::vector<double>** values,
+                      const std::vector<long>** row_ptrs, const std::vector<int>** cols) {
+    *n_rows = h->n_rows;
+    *n_cols = h->n_cols;
+    *values = &h->values;
+    *row_ptrs = &h->row_ptrs;
+    *cols = &h->cols;
+}
+
 template <class H, class Run>
 static void host_roundtrip(H& h, hipStream_t st, const double* x, size_t nin, double* y, size_t nout, Run run) {
     TBEM_REQUIRE((x || nin == 0) && (y || nout == 0), TBEM_E_INVALID, "null argument");

@@ -101,3 +101,26 @@ def refine(apply_fp64, apply_inner, b, tol=1e-8, inner_tol=1e-4, restart=50, inn
         outer.append(float(torch.linalg.norm(r)) / bnorm)
     return x, {"outer": len(outer) - 1, "inner_iterations": inner_its, "residual": outer[-1],
                "converged": outer[-1] <= tol, "history": outer}
+
+
+def constrained_operator(apply, cm, n_total):
+    """v -> condense_vector(cm, apply(distribute_vector(cm, v, n_total))): the matvec of
+    the constrained system that acceptance_tests/dislocation.py:27-39 hands to GMRES,
+    here entirely on the device (cm from 3bem_amd.constraints.from_constraints)."""
+    from . import constraints as C
+
+    def mv(v):
+        return C.condense_vector(cm, apply(C.distribute_vector(cm, v.contiguous(), n_total)).contiguous())
+    return mv
+
+
+def solve_constrained(apply, cm, b, n_total, tol=1e-8, restart=50, maxiter=1000, apply_inner=None,
+                      inner_tol=1e-4):
+    """Solve the constrained system condense(A distribute(x)) = b for the reduced x
+    (b already condensed). With apply_inner (e.g. the mixed-precision FMM), FP64-residual
+    iterative refinement around it; otherwise plain GMRES. Returns (x_reduced, info)."""
+    A = constrained_operator(apply, cm, n_total)
+    if apply_inner is None:
+        return gmres(A, b, tol=tol, restart=restart, maxiter=maxiter)
+    A_in = constrained_operator(apply_inner, cm, n_total)
+    return refine(A, A_in, b, tol=tol, inner_tol=inner_tol, restart=restart)

@@ -210,3 +210,45 @@ def near_field_csr(nx=300, ny=300, components=3, seed=SEED):
     values = np.random.default_rng(seed + 1).random(row_ptrs[-1]) - 0.5
     n = components * n_dofs
     return n, n, row_ptrs, column_indices, values
+
+
+def grid_continuity_constraints(nx, ny, components=3, all_pairs=True, bc_value=0.0):
+    """Continuity constraints of rough_fault_mesh(nx, ny) with discontinuous linear dofs
+    (dof 3 f + k at corner k of facet f, n_dofs = 6 nx ny per component): for every
+    vertex, continuity_constraint(p, q) between the dofs that meet there, for every
+    ordered pair p != q when all_pairs (as an overlap map of coincident vertices lists
+    them), else for consecutive ones; each component shifted by n_dofs; plus
+    boundary_condition(dof, bc_value) on the dofs at vertex (0, 0). Returns
+    (eq_ptr, term_dof, term_weight, rhs, n_total)."""
+    F = 2 * nx * ny
+    n_dofs = 3 * F
+    vid = lambda i, j: i * (ny + 1) + j  # noqa: E731
+    at = [[] for _ in range((nx + 1) * (ny + 1))]
+    for i in range(nx):
+        for j in range(ny):
+            f = 2 * (i * ny + j)
+            for k, v in enumerate((vid(i, j), vid(i + 1, j), vid(i + 1, j + 1))):
+                at[v].append(3 * f + k)
+            for k, v in enumerate((vid(i, j), vid(i + 1, j + 1), vid(i, j + 1))):
+                at[v].append(3 * (f + 1) + k)
+    pairs = []
+    for dofs in at:
+        if all_pairs:
+            pairs += [(p, q) for p in dofs for q in dofs if p != q]
+        else:
+            pairs += list(zip(dofs[:-1], dofs[1:]))
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    bc = np.asarray(at[0], dtype=np.int64)
+    eq_dofs, eq_w, rhs = [], [], []
+    for c in range(components):
+        s = c * n_dofs
+        eq_dofs.append(pairs + s)
+        eq_w.append(np.tile([1.0, -1.0], (len(pairs), 1)))
+        rhs.append(np.zeros(len(pairs)))
+    n_pair_eqs = components * len(pairs)
+    term_dof = np.concatenate([np.concatenate(eq_dofs).reshape(-1), np.concatenate([bc + c * n_dofs for c in range(components)])])
+    term_weight = np.concatenate([np.concatenate(eq_w).reshape(-1), np.ones(components * len(bc))])
+    rhs = np.concatenate(rhs + [np.full(components * len(bc), bc_value)])
+    eq_ptr = np.concatenate([np.arange(0, 2 * n_pair_eqs + 1, 2),
+                             2 * n_pair_eqs + np.arange(1, components * len(bc) + 1)]).astype(np.int64)
+    return eq_ptr, term_dof, term_weight, rhs, components * n_dofs

@@ -288,6 +288,45 @@ size_t tbem_integral_n_rows(tbem_integral_t h);  /* galerkin.n_rows() */
 size_t tbem_integral_n_cols(tbem_integral_t h);  /* nearfield.n_cols() */
 void tbem_integral_destroy(tbem_integral_t h);
 
+/* ------------------------------------------------------- linear constraints */
+
+/* ConstraintMatrix (cpp/constraint_matrix.h:17-25): linear constraints on the dofs,
+ * each solved for its highest dof. Set up on the host; distribute and condense run
+ * on the GPU and equal the reference's vectors bit for bit. */
+typedef struct tbem_constraints_s* tbem_constraints_t;
+
+/* from_constraints (cpp/constraint_matrix.cpp:35-52) of n_eq ConstraintEQs
+ * (cpp/constraint.h:33-52): equation e has the terms eq_ptr[e] .. eq_ptr[e+1]-1
+ * (term_dof[k], term_weight[k]) and right-hand side rhs[e]. */
+int tbem_constraints_create(size_t n_eq, const size_t* eq_ptr, const size_t* term_dof,
+                            const double* term_weight, const double* rhs, tbem_constraints_t* out);
+/* ConstraintMatrix::size (cpp/constraint_matrix.h:20-22): the number of constrained dofs. */
+size_t tbem_constraints_size(tbem_constraints_t h);
+size_t tbem_constraints_n_terms(tbem_constraints_t h);
+/* The solved constraints (RearrangedConstraintEQ, cpp/constraint.h:60-70), constrained
+ * dofs ascending: x_{constrained_dof[i]} = rhs[i] + sum over term_ptr[i] .. term_ptr[i+1]-1
+ * of term_weight[k] x_{term_dof[k]}. Sizes: tbem_constraints_size and _n_terms (+1 for
+ * term_ptr). Any pointer may be NULL. */
+int tbem_constraints_get(tbem_constraints_t h, size_t* constrained_dof, size_t* term_ptr,
+                         size_t* term_dof, double* term_weight, double* rhs);
+/* homogenize_constraints (cpp/constraint_matrix.cpp:294-303): every rhs set to 0. */
+int tbem_constraints_homogenize(tbem_constraints_t h, tbem_constraints_t* out);
+/* distribute_vector (cpp/constraint_matrix.cpp:54-82): the reduced vector (n_total -
+ * size() entries) to the full one (n_total entries). */
+int tbem_distribute_vector(tbem_constraints_t h, const double* in, size_t n_total, double* out);
+int tbem_distribute_vector_dev(tbem_constraints_t h, const double* d_in, size_t n_total, double* d_out,
+                               void* stream);
+/* condense_vector (cpp/constraint_matrix.cpp:99-125): the full vector (n_all entries)
+ * to the reduced one (n_all - size() entries). */
+int tbem_condense_vector(tbem_constraints_t h, const double* all, size_t n_all, double* out);
+int tbem_condense_vector_dev(tbem_constraints_t h, const double* d_all, size_t n_all, double* d_out,
+                             void* stream);
+/* condense_matrix for a sparse operator (cpp/constraint_matrix.cpp:252-292), on the host
+ * like the reference; the result lives on the GPU. */
+int tbem_condense_sparse(tbem_constraints_t row_cm, tbem_constraints_t col_cm, tbem_sparse_t matrix,
+                         tbem_sparse_t* out);
+void tbem_constraints_destroy(tbem_constraints_t h);
+
 /* ----------------------------------------------------------------------- misc */
 
 /* One kernel evaluation K(obs <- src) on the GPU, R*C doubles row-major

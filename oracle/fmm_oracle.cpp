@@ -806,6 +806,50 @@ void orc_sparse_coo(size_t n_rows, size_t n_cols, size_t n_entries, const long* 
     }
 }
 
+// distribute_vector as cpp/constraint_matrix.cpp:54-82 computes it, for solved
+// constraints x_{cdof[i]} = trhs[i] + sum_k tw[k] x_{tdof[k]} (cdof ascending): the
+// free dofs take in[] in order, then each constrained dof, ascending, sums its terms
+// in order starting from its rhs.
+void orc_distribute(long n_c, const long* cdof, const long* tptr, const long* tdof, const double* tw,
+                    const double* trhs, const double* in, size_t n_total, double* out) {
+    std::vector<long> which(n_total, -1);
+    for (long i = 0; i < n_c; i++) which[cdof[i]] = i;
+    size_t next = 0;
+    for (size_t d = 0; d < n_total; d++)
+        if (which[d] < 0) out[d] = in[next++];
+    for (size_t d = 0; d < n_total; d++) {
+        const long i = which[d];
+        if (i < 0) continue;
+        double val = trhs[i];
+        for (long k = tptr[i]; k < tptr[i + 1]; k++) val += tw[k] * out[tdof[k]];
+        out[d] = val;
+    }
+}
+
+// condense_vector as cpp/constraint_matrix.cpp:99-125 computes it: dofs from the
+// last down; a dof's entry is all[d] plus what it has accumulated; a constrained dof
+// passes weight * entry to each of its terms' dofs, a free dof keeps 0.0 + entry.
+void orc_condense(long n_c, const long* cdof, const long* tptr, const long* tdof, const double* tw,
+                  const double* trhs, const double* all, size_t n_total, double* out) {
+    (void)trhs;
+    std::vector<long> which(n_total, -1);
+    for (long i = 0; i < n_c; i++) which[cdof[i]] = i;
+    std::vector<double> acc(n_total, 0.0);
+    for (size_t d = n_total; d-- > 0;) {
+        const double entry = all[d] + acc[d];
+        acc[d] = 0.0;
+        const long i = which[d];
+        if (i < 0) {
+            acc[d] += entry;
+            continue;
+        }
+        for (long k = tptr[i]; k < tptr[i + 1]; k++) acc[tdof[k]] += tw[k] * entry;
+    }
+    size_t next = 0;
+    for (size_t d = 0; d < n_total; d++)
+        if (which[d] < 0) out[next++] = acc[d];
+}
+
 // The CSR product of orc_sparse_coo on ready CSR arrays, reps times; returns the
 // mean seconds per apply and leaves y.
 double orc_sparse_csr_apply(size_t n_rows, size_t n_cols, const long* rp, const long* ci, const double* v,

@@ -82,7 +82,15 @@ class Oracle:
         f("sparse_csr_apply").argtypes = [_SZ, _SZ, _L, _L, _D, _D, _D, ctypes.c_int]
         f("sparse_right_multiply").restype = ctypes.c_long
         f("sparse_right_multiply").argtypes = [_SZ, _SZ, _L, _L, _D, _SZ, _L, _L, _D, ctypes.c_long, _L, _L, _D]
+        for name in ("distribute", "condense"):
+            f(name).argtypes = [ctypes.c_long, _L, _L, _L, _D, _D, _D, _SZ, _D]
         if prefix == "ref":
+            self.lib.ref_from_constraints.restype = ctypes.c_long
+            self.lib.ref_from_constraints.argtypes = [_SZ, _L, _L, _D, _D, _L, _L, _L, _D, _D, _L]
+            self.lib.ref_condense_sparse.restype = ctypes.c_long
+            self.lib.ref_condense_sparse.argtypes = [ctypes.c_long, _L, _L, _L, _D, _D,
+                                                     ctypes.c_long, _L, _L, _L, _D, _D,
+                                                     _SZ, _SZ, _L, _L, _D, ctypes.c_long, _L, _L, _D]
             self.lib.ref_bem_sparse.argtypes = [ctypes.c_int, _SZ, _SZ, _D, _SZ, _D, _D, _L, _L, _D,
                                                 ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]
             self.lib.ref_nbody_from_bem.argtypes = [_SZ, _D, _SZ, _D, _SZ, _D, _D, _SZ, _D, _D,
@@ -202,6 +210,69 @@ class Oracle:
         got = self._f("sparse_right_multiply")(*args, nnz, L(rp), L(ci), _p(v))
         assert got == nnz
         return ar, bc, rp, ci, v
+
+    # --------------------------------------------------------------- constraints
+    # Solved constraints travel as a tuple (cdof, tptr, tdof, tw, trhs): constrained
+    # dofs ascending, x_cdof[i] = trhs[i] + sum_{k in tptr[i]..tptr[i+1]} tw[k] x_tdof[k].
+
+    def from_constraints(self, constraints):
+        """The compiled reference's from_constraints; constraints = [(terms [(dof, w)], rhs)].
+        Reference library only."""
+        ptr = [0]
+        dof, w, rhs = [], [], []
+        for terms, r in constraints:
+            for d, x in terms:
+                dof.append(d)
+                w.append(x)
+            ptr.append(len(dof))
+            rhs.append(r)
+        ptr, dof = np.asarray(ptr, dtype=np.int64), np.asarray(dof, dtype=np.int64)
+        w, rhs = np.asarray(w, dtype=np.float64), np.asarray(rhs, dtype=np.float64)
+        nt = ctypes.c_long()
+        L = lambda x: x.ctypes.data_as(_L)  # noqa: E731
+        args = (len(constraints), L(ptr), L(dof), _p(w), _p(rhs))
+        n = self.lib.ref_from_constraints(*args, None, None, None, None, None, ctypes.byref(nt))
+        out = (np.zeros(n, dtype=np.int64), np.zeros(n + 1, dtype=np.int64), np.zeros(nt.value, dtype=np.int64),
+               np.zeros(nt.value), np.zeros(n))
+        self.lib.ref_from_constraints(*args, L(out[0]), L(out[1]), L(out[2]), _p(out[3]), _p(out[4]), ctypes.byref(nt))
+        return out
+
+    @staticmethod
+    def _cm_args(cm):
+        cd, tp, td, tw, tr = (np.ascontiguousarray(a, dtype=t) for a, t in
+                              zip(cm, (np.int64, np.int64, np.int64, np.float64, np.float64)))
+        keep = (cd, tp, td, tw, tr)
+        return keep, [cd.shape[0], cd.ctypes.data_as(_L), tp.ctypes.data_as(_L), td.ctypes.data_as(_L), _p(tw), _p(tr)]
+
+    def distribute(self, cm, v, n_total):
+        """distribute_vector (cpp/constraint_matrix.cpp:54-82) for solved constraints cm."""
+        keep, a = self._cm_args(cm)
+        v = _c(v)
+        assert v.shape[0] == n_total - a[0]
+        out = np.zeros(n_total)
+        self._f("distribute")(*a, _p(v), n_total, _p(out))
+        return out
+
+    def condense(self, cm, v):
+        """condense_vector (cpp/constraint_matrix.cpp:99-125) for solved constraints cm."""
+        keep, a = self._cm_args(cm)
+        v = _c(v)
+        out = np.zeros(v.shape[0] - a[0])
+        self._f("condense")(*a, _p(v), v.shape[0], _p(out))
+        return out
+
+    def condense_sparse(self, row_cm, col_cm, n_rows, n_cols, rp, ci, v):
+        """The compiled reference's condense_matrix on a CSR matrix: (row_ptrs, column_indices,
+        values). Reference library only."""
+        k1, a1 = self._cm_args(row_cm)
+        k2, a2 = self._cm_args(col_cm)
+        rp, ci, v = np.ascontiguousarray(rp, dtype=np.int64), np.ascontiguousarray(ci, dtype=np.int64), _c(v)
+        L = lambda x: x.ctypes.data_as(_L)  # noqa: E731
+        args = (*a1, *a2, n_rows, n_cols, L(rp), L(ci), _p(v))
+        nnz = self.lib.ref_condense_sparse(*args, -1, None, None, None)
+        orp, oci, ov = np.zeros(n_rows - a1[0] + 1, dtype=np.int64), np.zeros(nnz, dtype=np.int64), np.zeros(nnz)
+        assert self.lib.ref_condense_sparse(*args, nnz, L(orp), L(oci), _p(ov)) == nnz
+        return orp, oci, ov
 
     def bem_sparse(self, which, n_components, facets, quad):
         """The compiled reference's make_galerkin_operator ("galerkin") or

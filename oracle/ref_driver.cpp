@@ -21,6 +21,7 @@
 #include "geometry.h"
 #include "blas_wrapper.h"
 #include "sparse_operator.h"
+#include "constraint_matrix.h"
 #include "mesh.h"
 #include "quad_rule.h"
 #include "galerkin_operator.h"
@@ -297,6 +298,92 @@ double ref_sparse_csr_apply(size_t n_rows, size_t n_cols, const long* rp, const 
     auto t1 = std::chrono::steady_clock::now();
     std::memcpy(y, out.data(), n_rows * sizeof(double));
     return std::chrono::duration<double>(t1 - t0).count() / reps;
+}
+
+// from_constraints (cpp/constraint_matrix.cpp:35-52) of n_eq equations (terms
+// eq_ptr[e] .. eq_ptr[e+1]-1). Writes the solved constraints, constrained dofs
+// ascending (when cdof is not NULL); returns their number and sets *n_terms.
+long ref_from_constraints(size_t n_eq, const long* eq_ptr, const long* dof, const double* w, const double* rhs,
+                          long* cdof, long* tptr, long* tdof, double* tw, double* trhs, long* n_terms) {
+    std::vector<ConstraintEQ> eqs;
+    for (size_t e = 0; e < n_eq; e++) {
+        std::vector<LinearTerm> terms;
+        for (long k = eq_ptr[e]; k < eq_ptr[e + 1]; k++) terms.push_back(LinearTerm{(size_t)dof[k], w[k]});
+        eqs.push_back(ConstraintEQ{terms, rhs[e]});
+    }
+    auto cm = from_constraints(eqs);
+    std::vector<size_t> keys;
+    for (auto& kv : cm.map) keys.push_back(kv.first);
+    std::sort(keys.begin(), keys.end());
+    long k = 0;
+    if (tptr) tptr[0] = 0;
+    for (size_t i = 0; i < keys.size(); i++) {
+        const auto& c = cm.map.at(keys[i]);
+        if (cdof) {
+            cdof[i] = (long)keys[i];
+            trhs[i] = c.rhs;
+        }
+        for (const auto& t : c.terms) {
+            if (cdof) {
+                tdof[k] = (long)t.dof;
+                tw[k] = t.weight;
+            }
+            k++;
+        }
+        if (cdof) tptr[i + 1] = k;
+    }
+    *n_terms = k;
+    return (long)keys.size();
+}
+
+static ConstraintMatrix ref_cm(long n_c, const long* cdof, const long* tptr, const long* tdof, const double* tw,
+                               const double* trhs) {
+    ConstraintMatrixData m;
+    for (long i = 0; i < n_c; i++) {
+        std::vector<LinearTerm> terms;
+        for (long k = tptr[i]; k < tptr[i + 1]; k++) terms.push_back(LinearTerm{(size_t)tdof[k], tw[k]});
+        m.insert(std::make_pair((size_t)cdof[i], RearrangedConstraintEQ{(size_t)cdof[i], terms, trhs[i]}));
+    }
+    return ConstraintMatrix{m};
+}
+
+// distribute_vector / condense_vector (cpp/constraint_matrix.cpp:54-125) for solved
+// constraints given as arrays; in has n_total - n_c entries (distribute) or n_total
+// (condense).
+void ref_distribute(long n_c, const long* cdof, const long* tptr, const long* tdof, const double* tw,
+                    const double* trhs, const double* in, size_t n_total, double* out) {
+    auto cm = ref_cm(n_c, cdof, tptr, tdof, tw, trhs);
+    auto v = distribute_vector(cm, std::vector<double>(in, in + (n_total - n_c)), n_total);
+    std::memcpy(out, v.data(), n_total * sizeof(double));
+}
+
+void ref_condense(long n_c, const long* cdof, const long* tptr, const long* tdof, const double* tw,
+                  const double* trhs, const double* all, size_t n_total, double* out) {
+    auto cm = ref_cm(n_c, cdof, tptr, tdof, tw, trhs);
+    auto v = condense_vector(cm, std::vector<double>(all, all + n_total));
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+}
+
+// condense_matrix for a CSR matrix (cpp/constraint_matrix.cpp:252-292), row and
+// column constraints given as solved arrays; writes at most cap entries, returns nnz.
+long ref_condense_sparse(long rn, const long* rcd, const long* rtp, const long* rtd, const double* rtw,
+                         const double* rrhs, long cn, const long* ccd, const long* ctp, const long* ctd,
+                         const double* ctw, const double* crhs, size_t n_rows, size_t n_cols, const long* rp,
+                         const long* ci, const double* v, long cap, long* orp, long* oci, double* ov) {
+    auto rcm = ref_cm(rn, rcd, rtp, rtd, rtw, rrhs);
+    auto ccm = ref_cm(cn, ccd, ctp, ctd, ctw, crhs);
+    const size_t nnz = (size_t)rp[n_rows];
+    SparseOperator A(n_rows, n_cols, std::vector<double>(v, v + nnz), std::vector<size_t>(ci, ci + nnz),
+                     std::vector<size_t>(rp, rp + n_rows + 1));
+    auto out = condense_matrix(rcm, ccm, A);
+    const long onnz = (long)out.values.size();
+    if (onnz > cap) return onnz;
+    for (size_t i = 0; i <= out.n_rows(); i++) orp[i] = (long)out.row_ptrs[i];
+    for (long k = 0; k < onnz; k++) {
+        oci[k] = (long)out.column_indices[k];
+        ov[k] = out.values[k];
+    }
+    return onnz;
 }
 
 static Mesh<3> ref_mesh(size_t n_facets, const double* f) {
