@@ -171,9 +171,10 @@ class FMMOperator:
     precision="mixed": FP32 P2P/surface stages with compensated sums; C2E in FP64.
     up_c2e / down_c2e: optional per-level check-to-equivalent matrices, shape
     (levels, n, n), used instead of the GPU-computed ones (parity runs).
+    plan: bytes from save_plan() of an operator built on the same inputs and config.
     """
 
-    def __init__(self, K, data, config, precision="fp64", up_c2e=None, down_c2e=None):
+    def __init__(self, K, data, config, precision="fp64", up_c2e=None, down_c2e=None, plan=None):
         lib = _lib.require_device()
         self.K = K
         self.data = data
@@ -199,7 +200,11 @@ class FMMOperator:
         if self._down is not None:
             desc.down_check_to_equiv, desc.n_down_levels = _dp(self._down), self._down.shape[0]
         h = ctypes.c_void_p()
-        _lib.check(lib.tbem_fmm_create(ctypes.byref(desc), ctypes.byref(h)))
+        if plan is None:
+            _lib.check(lib.tbem_fmm_create(ctypes.byref(desc), ctypes.byref(h)))
+        else:
+            buf = bytes(plan)
+            _lib.check(lib.tbem_fmm_create_from_plan(ctypes.byref(desc), buf, len(buf), ctypes.byref(h)))
         self._h = h
         self._lib = lib
 
@@ -215,6 +220,16 @@ class FMMOperator:
         if h is not None and h.value:
             self._lib.tbem_fmm_destroy(h)
             self._h = None
+
+    def save_plan(self):
+        """The plan as bytes (octrees, task lists, C2E): FMMOperator(K, data, config,
+        plan=...) rebuilds the operator from them without the octree builds, the
+        traversal and the C2E SVDs. Refused for other inputs or configuration."""
+        n = ctypes.c_size_t()
+        _lib.check(self._lib.tbem_fmm_plan_save(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _lib.check(self._lib.tbem_fmm_plan_save(self._h, buf, n.value, ctypes.byref(n)))
+        return buf.raw[:n.value]
 
     def n_rows(self):
         return int(self._lib.tbem_fmm_n_rows(self._h))

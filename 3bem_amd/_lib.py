@@ -17,7 +17,7 @@ _VP = ctypes.c_void_p
 
 # every symbol declared in include/tbem_fmm.h
 EXPORTED = (
-    "tbem_fmm_create", "tbem_fmm_apply", "tbem_fmm_apply_dev", "tbem_fmm_apply_adjoint",
+    "tbem_fmm_create", "tbem_fmm_plan_save", "tbem_fmm_create_from_plan", "tbem_fmm_apply", "tbem_fmm_apply_dev", "tbem_fmm_apply_adjoint",
     "tbem_fmm_apply_adjoint_dev", "tbem_fmm_obs_grad", "tbem_fmm_obs_grad_dev",
     "tbem_fmm_forward_token", "tbem_fmm_obs_grad_reuse_dev",
     "tbem_fmm_set_obs_locs", "tbem_fmm_n_rows", "tbem_fmm_n_cols", "tbem_fmm_clone",
@@ -92,6 +92,8 @@ def load(path=LIB_PATH):
     lib = ctypes.CDLL(path)
     sig = {
         "tbem_fmm_create": (ctypes.c_int, [ctypes.POINTER(FmmDesc), ctypes.POINTER(_VP)]),
+        "tbem_fmm_plan_save": (ctypes.c_int, [_VP, _VP, _SZ, ctypes.POINTER(_SZ)]),
+        "tbem_fmm_create_from_plan": (ctypes.c_int, [ctypes.POINTER(FmmDesc), _VP, _SZ, ctypes.POINTER(_VP)]),
         "tbem_fmm_apply": (ctypes.c_int, [_VP, _D, _D]),
         "tbem_fmm_apply_dev": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
         "tbem_fmm_apply_adjoint": (ctypes.c_int, [_VP, _D, _D]),

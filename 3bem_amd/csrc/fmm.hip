@@ -61,6 +61,12 @@ struct FmmPlan {
     std::vector<CellTiles> t_olev, t_slev; // every cell of each observation / source level
     double p2p_pairs = 0;
     double plan_seconds = 0;
+    // serialization: the inputs' hash, the mixed path's C2E SVD factors (host copies),
+    // and whether set_obs_locs has moved the observation points since
+    uint64_t input_hash = 0;
+    bool same_tree = false;        // one octree serves both sides
+    std::vector<double> h_fac[4];  // up W, up V, down W, down V
+    bool obs_moved = false;
 };
 
 struct Work {
@@ -203,7 +209,237 @@ static void validate(const tbem_fmm_desc* d) {
                  "null source arrays");
 }
 
-static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s) {
+// ------------------------------------------------------------- plan serialization
+// A saved plan: both octrees, the task lists, the per-level C2E matrices and, for
+// the mixed path, their SVD factors, under a header that names the inputs. Layout
+// (little endian): the 8 bytes "TBEMPLN1"; u64 payload length; u64 FNV-1a hash of the
+// payload; then the payload: the header fields (kernel, mu, nu, n_obs, n_src, mac,
+// order, min_pts_per_cell, d, account_for_small_cells, same_tree, input hash) and
+// the sections, each array as a u64 count followed by its elements.
+struct SavedPlan {
+    bool same = false;
+    HostTree st, ot;
+    std::vector<double> uc2e, dc2e, fac[4];
+    TaskLists tl;
+};
+
+static uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// The hash of what the plan depends on: counts and every input array.
+static uint64_t input_hash(const tbem_fmm_desc* d) {
+    uint64_t h = fnv1a(&d->n_obs, sizeof(d->n_obs));
+    h = fnv1a(&d->n_src, sizeof(d->n_src), h);
+    if (d->n_obs) {
+        h = fnv1a(d->obs_locs, 3 * d->n_obs * sizeof(double), h);
+        h = fnv1a(d->obs_normals, 3 * d->n_obs * sizeof(double), h);
+    }
+    if (d->n_src) {
+        h = fnv1a(d->src_locs, 3 * d->n_src * sizeof(double), h);
+        h = fnv1a(d->src_normals, 3 * d->n_src * sizeof(double), h);
+        h = fnv1a(d->src_weights, d->n_src * sizeof(double), h);
+    }
+    return h;
+}
+
+struct PlanWriter {
+    std::vector<char> b;
+    template <class T> void pod(const T& v) {
+        const char* p = (const char*)&v;
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    template <class T> void vec(const std::vector<T>& v) {
+        pod((uint64_t)v.size());
+        const char* p = (const char*)v.data();
+        b.insert(b.end(), p, p + v.size() * sizeof(T));
+    }
+};
+
+struct PlanReader {
+    const char* p;
+    size_t n, off;
+    template <class T> T pod() {
+        TBEM_REQUIRE(sizeof(T) <= n - off, TBEM_E_INVALID, "saved plan: truncated");
+        T v;
+        std::memcpy(&v, p + off, sizeof(T));
+        off += sizeof(T);
+        return v;
+    }
+    template <class T> std::vector<T> vec() {
+        const uint64_t k = pod<uint64_t>();
+        TBEM_REQUIRE(k <= (n - off) / sizeof(T), TBEM_E_INVALID, "saved plan: truncated");
+        std::vector<T> v(k);
+        if (k) std::memcpy((void*)v.data(), p + off, k * sizeof(T));
+        off += k * sizeof(T);
+        return v;
+    }
+};
+
+static void put_tree(PlanWriter& w, const HostTree& t) {
+    w.pod(t.n_pts);
+    w.pod(t.n_cells);
+    w.pod(t.n_levels);
+    for (const auto* v : {&t.level, &t.begin, &t.count, &t.first_child, &t.n_child, &t.parent, &t.octant})
+        w.vec(*v);
+    w.vec(t.leaf);
+    for (const auto* v : {&t.cx, &t.cy, &t.cz, &t.level_hw, &t.level_r}) w.vec(*v);
+    for (const auto* v : {&t.level_begin, &t.perm, &t.leaves, &t.dfs}) w.vec(*v);
+}
+
+// A tree read back is checked so that every index the kernels follow stays inside
+// its arrays: a corrupt or foreign file is refused instead of being run.
+static HostTree get_tree(PlanReader& r, int n_pts) {
+    HostTree t;
+    t.n_pts = r.pod<int>();
+    t.n_cells = r.pod<int>();
+    t.n_levels = r.pod<int>();
+    for (auto* v : {&t.level, &t.begin, &t.count, &t.first_child, &t.n_child, &t.parent, &t.octant})
+        *v = r.vec<int>();
+    t.leaf = r.vec<char>();
+    for (auto* v : {&t.cx, &t.cy, &t.cz, &t.level_hw, &t.level_r}) *v = r.vec<double>();
+    for (auto* v : {&t.level_begin, &t.perm, &t.leaves, &t.dfs}) *v = r.vec<int>();
+    bool ok = t.n_pts == n_pts && t.n_cells >= 0 && t.n_levels >= 0 && t.n_levels <= kMaxKeyLevels + 1;
+    const size_t nc = ok ? (size_t)t.n_cells : 0, nl = ok ? (size_t)t.n_levels : 0;
+    for (const auto* v : {&t.level, &t.begin, &t.count, &t.first_child, &t.n_child, &t.parent, &t.octant})
+        ok = ok && v->size() == nc;
+    ok = ok && t.leaf.size() == nc && t.cx.size() == nc && t.cy.size() == nc && t.cz.size() == nc;
+    ok = ok && t.level_hw.size() == 3 * nl && t.level_r.size() == nl && t.level_begin.size() == nl + 1;
+    ok = ok && t.perm.size() == (size_t)n_pts && t.leaves.size() <= nc && t.dfs.size() == nc;
+    TBEM_REQUIRE(ok, TBEM_E_INVALID, "saved plan: octree arrays do not fit together");
+    ok = t.level_begin[0] == 0 && t.level_begin[nl] == t.n_cells;
+    for (size_t L = 0; ok && L < nl; L++) ok = t.level_begin[L] <= t.level_begin[L + 1];
+    for (size_t c = 0; ok && c < nc; c++) {
+        ok = t.level[c] >= 0 && t.level[c] < t.n_levels && t.begin[c] >= 0 && t.count[c] >= 0 &&
+             (long)t.begin[c] + t.count[c] <= n_pts && t.n_child[c] >= 0 && t.n_child[c] <= 8 &&
+             (t.n_child[c] == 0 || (t.first_child[c] >= 0 && (long)t.first_child[c] + t.n_child[c] <= t.n_cells)) &&
+             t.parent[c] >= -1 && t.parent[c] < t.n_cells && (long)c >= t.level_begin[t.level[c]] &&
+             (long)c < t.level_begin[t.level[c] + 1] && (t.leaf[c] == 0 || t.leaf[c] == 1);
+    }
+    std::vector<char> seen(n_pts, 0);
+    for (int i : t.perm) {
+        ok = ok && i >= 0 && i < n_pts && !seen[i];
+        if (ok) seen[i] = 1;
+    }
+    for (const auto* v : {&t.leaves, &t.dfs})
+        for (int c : *v) ok = ok && c >= 0 && c < t.n_cells;
+    for (int c : t.leaves) ok = ok && t.leaf[c];
+    TBEM_REQUIRE(ok, TBEM_E_INVALID, "saved plan: octree indices out of range");
+    return t;
+}
+
+static std::vector<int> flat_pairs(const std::vector<std::array<int, 2>>& v) {
+    std::vector<int> f(2 * v.size());
+    for (size_t i = 0; i < v.size(); i++) {
+        f[2 * i] = v[i][0];
+        f[2 * i + 1] = v[i][1];
+    }
+    return f;
+}
+
+static std::vector<std::array<int, 2>> read_pairs(PlanReader& r, int n_obs_cells, int n_src_cells) {
+    const std::vector<int> f = r.vec<int>();
+    TBEM_REQUIRE(f.size() % 2 == 0, TBEM_E_INVALID, "saved plan: odd pair list");
+    std::vector<std::array<int, 2>> v(f.size() / 2);
+    for (size_t i = 0; i < v.size(); i++) {
+        v[i] = {f[2 * i], f[2 * i + 1]};
+        TBEM_REQUIRE(f[2 * i] >= 0 && f[2 * i] < n_obs_cells && f[2 * i + 1] >= 0 && f[2 * i + 1] < n_src_cells,
+                     TBEM_E_INVALID, "saved plan: task list names a cell outside the trees");
+    }
+    return v;
+}
+
+static std::vector<int> read_cells(PlanReader& r, int n_cells) {
+    std::vector<int> v = r.vec<int>();
+    for (int c : v)
+        TBEM_REQUIRE(c >= 0 && c < n_cells, TBEM_E_INVALID, "saved plan: task list names a cell outside the trees");
+    return v;
+}
+
+static std::vector<char> save_plan(const FmmPlan& P) {
+    TBEM_REQUIRE(!P.obs_moved, TBEM_E_UNSUPPORTED,
+                 "the observation points have moved since the plan was built: save a freshly built plan");
+    PlanWriter w;
+    w.pod(P.kid);
+    w.pod(P.kc.mu);
+    w.pod(P.kc.nu);
+    w.pod((uint64_t)P.n_obs);
+    w.pod((uint64_t)P.n_src);
+    w.pod(P.mac);
+    w.pod((uint64_t)P.order);
+    w.pod((uint64_t)P.leaf);
+    w.pod(P.d);
+    w.pod((int)P.small_cells);
+    const int same = P.same_tree ? 1 : 0;
+    w.pod(same);
+    w.pod(P.input_hash);
+    put_tree(w, P.st);
+    if (!same) put_tree(w, P.ot);
+    w.vec(P.h_uc2e);
+    w.vec(P.h_dc2e);
+    for (const auto& f : P.h_fac) w.vec(f);
+    for (const auto* v : {&P.tl.p2p, &P.tl.m2l, &P.tl.m2p, &P.tl.p2l}) w.vec(flat_pairs(*v));
+    for (const auto* v : {&P.tl.p2m, &P.tl.m2m, &P.tl.l2l, &P.tl.l2p}) w.vec(*v);
+    std::vector<char> out(24);
+    std::memcpy(out.data(), "TBEMPLN1", 8);
+    const uint64_t len = w.b.size(), hash = fnv1a(w.b.data(), w.b.size());
+    std::memcpy(out.data() + 8, &len, 8);
+    std::memcpy(out.data() + 16, &hash, 8);
+    out.insert(out.end(), w.b.begin(), w.b.end());
+    return out;
+}
+
+static SavedPlan load_plan(const tbem_fmm_desc* d, const void* buf, size_t n) {
+    TBEM_REQUIRE(buf && n >= 24 && std::memcmp(buf, "TBEMPLN1", 8) == 0, TBEM_E_INVALID,
+                 "not a saved FMM plan (bad magic)");
+    uint64_t len, hash;
+    std::memcpy(&len, (const char*)buf + 8, 8);
+    std::memcpy(&hash, (const char*)buf + 16, 8);
+    TBEM_REQUIRE(len == n - 24, TBEM_E_INVALID, "saved plan: truncated");
+    TBEM_REQUIRE(fnv1a((const char*)buf + 24, len) == hash, TBEM_E_INVALID, "saved plan: checksum mismatch");
+    PlanReader r{(const char*)buf + 24, (size_t)len, 0};
+    const int kid = r.pod<int>();
+    const double mu = r.pod<double>(), nu = r.pod<double>();
+    const uint64_t n_obs = r.pod<uint64_t>(), n_src = r.pod<uint64_t>();
+    const double mac = r.pod<double>();
+    const uint64_t order = r.pod<uint64_t>(), leaf = r.pod<uint64_t>();
+    const double dd = r.pod<double>();
+    const int small = r.pod<int>(), same = r.pod<int>();
+    const uint64_t ih = r.pod<uint64_t>();
+    const KConst kc = make_kconst(d->shear_modulus, d->poisson_ratio);
+    TBEM_REQUIRE(kid == d->kernel && mu == kc.mu && nu == kc.nu && n_obs == d->n_obs && n_src == d->n_src &&
+                     mac == d->mac && order == d->order && leaf == d->min_pts_per_cell && dd == d->d &&
+                     small == (d->account_for_small_cells != 0),
+                 TBEM_E_INVALID, "saved plan: built for another kernel or FMMConfig");
+    TBEM_REQUIRE(ih == input_hash(d), TBEM_E_INVALID, "saved plan: built for other points, normals or weights");
+    TBEM_REQUIRE(same == 0 || n_obs == n_src, TBEM_E_INVALID, "saved plan: shared tree with different counts");
+    SavedPlan sv;
+    sv.same = same != 0;
+    sv.st = get_tree(r, (int)n_src);
+    sv.ot = sv.same ? sv.st : get_tree(r, (int)n_obs);
+    sv.uc2e = r.vec<double>();
+    sv.dc2e = r.vec<double>();
+    for (auto& f : sv.fac) f = r.vec<double>();
+    for (auto* v : {&sv.tl.p2p, &sv.tl.m2l, &sv.tl.m2p, &sv.tl.p2l}) *v = read_pairs(r, sv.ot.n_cells, sv.st.n_cells);
+    sv.tl.p2m = read_cells(r, sv.st.n_cells);
+    sv.tl.m2m = read_cells(r, sv.st.n_cells);
+    sv.tl.l2l = read_cells(r, sv.ot.n_cells);
+    sv.tl.l2p = read_cells(r, sv.ot.n_cells);
+    TBEM_REQUIRE(r.off == r.n, TBEM_E_INVALID, "saved plan: trailing bytes");
+    const size_t blk = (size_t)kernel_dim(d->kernel) * make_surfaces(d->order, d->d).n_e, bb = blk * blk;
+    TBEM_REQUIRE(sv.uc2e.size() == bb * sv.st.n_levels && sv.dc2e.size() == bb * sv.ot.n_levels, TBEM_E_INVALID,
+                 "saved plan: C2E matrices of the wrong size");
+    for (int k = 0; k < 4; k++)
+        TBEM_REQUIRE(sv.fac[k].empty() || sv.fac[k].size() == bb * (k < 2 ? sv.st.n_levels : sv.ot.n_levels),
+                     TBEM_E_INVALID, "saved plan: C2E factors of the wrong size");
+    TBEM_REQUIRE(sv.fac[0].empty() == sv.fac[1].empty() && sv.fac[2].empty() == sv.fac[3].empty(), TBEM_E_INVALID,
+                 "saved plan: incomplete C2E factors");
+    return sv;
+}
+
+static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s, const SavedPlan* sv = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     validate(d);
     auto P = std::make_shared<FmmPlan>();
@@ -223,11 +459,19 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->blk = P->R * P->n_e;
     const int leaf = (int)std::min<size_t>(d->min_pts_per_cell, (size_t)1 << 30);
 
-    P->st = build_octree(d->src_locs, (int)d->n_src, leaf, s);
+    P->input_hash = input_hash(d);
     const bool same = d->n_obs == d->n_src &&
                       (d->obs_locs == d->src_locs ||
                        (d->n_obs > 0 && std::memcmp(d->obs_locs, d->src_locs, 3 * sizeof(double) * d->n_obs) == 0));
-    P->ot = same ? P->st : build_octree(d->obs_locs, (int)d->n_obs, leaf, s);
+    P->same_tree = same;
+    if (sv) {
+        TBEM_REQUIRE(sv->same == same, TBEM_E_INVALID, "saved plan: tree sharing differs from the inputs'");
+        P->st = sv->st;
+        P->ot = sv->ot;
+    } else {
+        P->st = build_octree(d->src_locs, (int)d->n_src, leaf, s);
+        P->ot = same ? P->st : build_octree(d->obs_locs, (int)d->n_obs, leaf, s);
+    }
     if (d->n_obs) {
         P->h_obs_locs.assign(d->obs_locs, d->obs_locs + 3 * d->n_obs);
         P->h_obs_normals.assign(d->obs_normals, d->obs_normals + 3 * d->n_obs);
@@ -245,7 +489,16 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     // the mixed path's modal factors come from the C2E pseudo-inverses' own SVDs
     const bool keep_fac = P->precision == TBEM_MIXED;
     DevBuf<double> up_fac[2], down_fac[2];
-    if (d->up_check_to_equiv) {
+    if (sv) {
+        P->uc2e.upload(sv->uc2e, s);
+        P->dc2e.upload(sv->dc2e, s);
+        if (keep_fac) {
+            for (int k = 0; k < 2; k++) {
+                if (!sv->fac[k].empty()) up_fac[k].upload(sv->fac[k], s);
+                if (!sv->fac[2 + k].empty()) down_fac[k].upload(sv->fac[2 + k], s);
+            }
+        }
+    } else if (d->up_check_to_equiv) {
         TBEM_REQUIRE(d->n_up_levels >= (size_t)P->st.n_levels, TBEM_E_INVALID,
                      "injected UC2E has fewer levels than the source tree");
         P->uc2e.upload(d->up_check_to_equiv, bb * P->st.n_levels, s);
@@ -253,7 +506,8 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         compute_c2e(P->kid, P->kc, P->st, P->surf, true, P->uc2e, s, keep_fac ? &up_fac[0] : nullptr,
                     keep_fac ? &up_fac[1] : nullptr);
     }
-    if (d->down_check_to_equiv) {
+    if (sv) {
+    } else if (d->down_check_to_equiv) {
         TBEM_REQUIRE(d->n_down_levels >= (size_t)P->ot.n_levels, TBEM_E_INVALID,
                      "injected DC2E has fewer levels than the observation tree");
         P->dc2e.upload(d->down_check_to_equiv, bb * P->ot.n_levels, s);
@@ -263,9 +517,16 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     }
     P->h_uc2e = P->uc2e.to_host(s);
     P->h_dc2e = P->dc2e.to_host(s);
+    if (keep_fac) {
+        for (int k = 0; k < 2; k++) {
+            if (up_fac[k].n) P->h_fac[k] = up_fac[k].to_host(s);
+            if (down_fac[k].n) P->h_fac[2 + k] = down_fac[k].to_host(s);
+        }
+    }
 
     // task lists (cpp/fmm.cpp:560-648)
-    dual_tree_traversal(P->ot, P->st, P->mac, P->n_e, P->small_cells, P->tl);
+    if (sv) P->tl = sv->tl;
+    else dual_tree_traversal(P->ot, P->st, P->mac, P->n_e, P->small_cells, P->tl);
     const TaskLists& tl = P->tl;
     for (auto& p: tl.p2p) P->p2p_pairs += (double)P->ot.count[p[0]] * (double)P->st.count[p[1]];
 
@@ -675,6 +936,42 @@ int tbem_fmm_create(const tbem_fmm_desc* desc, tbem_fmm_t* out) {
     });
 }
 
+int tbem_fmm_create_from_plan(const tbem_fmm_desc* desc, const void* plan, size_t plan_bytes, tbem_fmm_t* out) {
+    return guard([&] {
+        TBEM_REQUIRE(out != nullptr, TBEM_E_INVALID, "null output handle");
+        *out = nullptr;
+        validate(desc);
+        const SavedPlan sv = load_plan(desc, plan, plan_bytes);
+        hipStream_t s;
+        TBEM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        std::shared_ptr<FmmPlan> p;
+        try {
+            p = build_plan(desc, s, &sv);
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        (void)hipStreamDestroy(s);
+        auto h = std::unique_ptr<tbem_fmm_s>(new tbem_fmm_s);
+        h->plan = p;
+        h->work.reset(new Work(*p));
+        *out = h.release();
+    });
+}
+
+int tbem_fmm_plan_save(tbem_fmm_t h, void* buf, size_t cap, size_t* bytes) {
+    return guard([&] {
+        TBEM_REQUIRE(h && bytes, TBEM_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
+        const std::vector<char> b = save_plan(*h->plan);
+        *bytes = b.size();
+        if (buf) {
+            TBEM_REQUIRE(cap >= b.size(), TBEM_E_INVALID, "buffer too small for the saved plan");
+            std::memcpy(buf, b.data(), b.size());
+        }
+    });
+}
+
 int tbem_fmm_apply(tbem_fmm_t h, const double* x, double* y) {
     return guard([&] {
         TBEM_REQUIRE(h && (x || h->plan->n_src == 0) && (y || h->plan->n_obs == 0), TBEM_E_INVALID,
@@ -767,6 +1064,7 @@ int tbem_fmm_set_obs_locs(tbem_fmm_t h, const double* obs_locs) {
         if (P.n_obs == 0) return;
         TBEM_HIP(hipDeviceSynchronize());  // no call of any handle sharing the plan may be in flight
         std::copy(obs_locs, obs_locs + 3 * P.n_obs, P.h_obs_locs.begin());
+        P.obs_moved = true;
         P.op.upload(P.ot, P.h_obs_locs.data(), P.h_obs_normals.data(), nullptr, h->work->stream);
     });
 }

@@ -176,6 +176,14 @@ def compare_run(tbem, torch, steps, g, kid, cfg_t, n_sub, desc):
                      "plan_build_s": round(plan, 2), "m2l_kernel_ms": round(float(np.mean(m2l)), 3),
                      "stage_ms": stage_ms,
                      "rel_err_vs_direct": float(np.linalg.norm(ys[prec][idx] - yd) / np.linalg.norm(yd))}
+        # the same operator rebuilt from its saved plan (octrees, task lists, C2E)
+        blob = op.save_plan()
+        t0 = time.perf_counter()
+        op_l = tbem.FMMOperator(K, data, cfg, precision=prec, plan=blob)
+        out[prec]["plan_from_saved_s"] = round(time.perf_counter() - t0, 3)
+        out[prec]["saved_plan_MB"] = round(len(blob) / 2 ** 20, 1)
+        out[prec]["saved_plan_same_output"] = bool(np.array_equal(op_l.apply(x).cpu().numpy(), ys[prec]))
+        del op_l, blob
         if prec == "mixed" and info["m2l_classes"] > 0:
             blk = R * info["n_e"]
             flop = 2.0 * info["m2l"] * blk * blk

@@ -123,6 +123,20 @@ typedef struct {
  * inputs to the device, builds both octrees, the C2E matrices and the task lists. */
 int tbem_fmm_create(const tbem_fmm_desc* desc, tbem_fmm_t* out);
 
+/* Plan serialization (no reference counterpart: 3bem rebuilds its plan on every
+ * construction). tbem_fmm_plan_save writes the handle's plan (both octrees, the task
+ * lists, the per-level C2E matrices and, for TBEM_MIXED, their SVD factors) into buf
+ * and sets *bytes to its size; with buf NULL it only sets *bytes. The format starts
+ * with "TBEMPLN1" and a checksum and names the kernel, FMMConfig and a hash of every
+ * input array. A plan cannot be saved after tbem_fmm_set_obs_locs (TBEM_E_UNSUPPORTED).
+ * tbem_fmm_create_from_plan builds a handle from the same descriptor plus a saved
+ * plan, skipping the octree builds, the traversal and the C2E SVDs; it refuses
+ * (TBEM_E_INVALID) a plan saved for other inputs or configuration, a corrupt one, or
+ * one whose indices do not fit its trees. Injected C2E matrices in desc are ignored. */
+int tbem_fmm_plan_save(tbem_fmm_t h, void* buf, size_t cap, size_t* bytes);
+int tbem_fmm_create_from_plan(const tbem_fmm_desc* desc, const void* plan, size_t plan_bytes,
+                              tbem_fmm_t* out);
+
 /* Replaces FMMOperator::apply (cpp/fmm.cpp:550-558). Host pointers:
  * x has n_cols() doubles, y receives n_rows() doubles. */
 int tbem_fmm_apply(tbem_fmm_t h, const double* x, double* y);

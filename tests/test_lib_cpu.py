@@ -82,3 +82,24 @@ def test_workloads_shapes(wl):
     assert c2["locs"].shape == (1000, 3) and c2["x"].shape == (3000,)
     c4 = wl.cfg4_fault_and_stations(n_tri=400, n_obs=500)
     assert c4["src"].shape == (1200, 3) and c4["obs"].shape == (500, 3)
+
+
+def test_saved_plan_refused_before_touching_the_device(tbem):
+    """A blob that is not a saved plan is refused by the host checks (no device needed)."""
+    lib = tbem.load()
+    binding = importlib.import_module("3bem_amd._lib")
+    rng = np.random.default_rng(0)
+    pts = rng.uniform(0, 1, (10, 3))
+    P = ctypes.POINTER(ctypes.c_double)
+    d = binding.FmmDesc()
+    d.kernel, d.n_obs, d.n_src = 0, 10, 10
+    d.obs_locs = d.src_locs = pts.ctypes.data_as(P)
+    d.obs_normals = d.src_normals = pts.ctypes.data_as(P)
+    w = np.ones(10)
+    d.src_weights = w.ctypes.data_as(P)
+    d.mac, d.order, d.min_pts_per_cell, d.d = 0.3, 5, 4, 0.05
+    h = ctypes.c_void_p()
+    for blob, why in ((b"garbage!" * 4, b"magic"), (b"TBEMPLN1" + bytes(16) + b"xyz", b"truncated"),
+                      (b"TBEMPLN1" + (3).to_bytes(8, "little") + bytes(8) + b"xyz", b"checksum")):
+        assert lib.tbem_fmm_create_from_plan(ctypes.byref(d), blob, len(blob), ctypes.byref(h)) == 1
+        assert why in lib.tbem_last_error()
