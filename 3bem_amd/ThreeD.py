@@ -203,7 +203,7 @@ class FMMOperator:
         if plan is None:
             _lib.check(lib.tbem_fmm_create(ctypes.byref(desc), ctypes.byref(h)))
         else:
-            buf = bytes(plan)
+            buf = plan if isinstance(plan, bytes) else bytes(plan)
             _lib.check(lib.tbem_fmm_create_from_plan(ctypes.byref(desc), buf, len(buf), ctypes.byref(h)))
         self._h = h
         self._lib = lib

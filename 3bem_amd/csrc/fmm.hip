@@ -212,7 +212,7 @@ static void validate(const tbem_fmm_desc* d) {
 // ------------------------------------------------------------- plan serialization
 // A saved plan: both octrees, the task lists, the per-level C2E matrices and, for
 // the mixed path, their SVD factors, under a header that names the inputs. Layout
-// (little endian): the 8 bytes "TBEMPLN1"; u64 payload length; u64 FNV-1a hash of the
+// (little endian): the 8 bytes "TBEMPLN1"; u64 payload length; u64 hash64 of the
 // payload; then the payload: the header fields (kernel, mu, nu, n_obs, n_src, mac,
 // order, min_pts_per_cell, d, account_for_small_cells, same_tree, input hash) and
 // the sections, each array as a u64 count followed by its elements.
@@ -223,24 +223,52 @@ struct SavedPlan {
     TaskLists tl;
 };
 
-static uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
+// A 64-bit integrity hash, four independent 8-byte lanes per 32-byte step (several
+// GB/s): each word is multiplied, rotated and folded into its lane; the lanes, the
+// tail words and the tail bytes are then combined. It guards against corrupt files
+// and mismatched inputs, not against deliberate collisions.
+static inline uint64_t mix_word(uint64_t h, uint64_t w) {
+    w *= 0xC2B2AE3D27D4EB4Full;
+    w = (w << 31) | (w >> 33);
+    const uint64_t x = h ^ (w * 0x9E3779B185EBCA87ull);
+    return ((x << 27) | (x >> 37)) * 5 + 0x52DCE729;
+}
+
+static uint64_t hash64(const void* p, size_t n, uint64_t h = 0x9E3779B97F4A7C15ull) {
     const unsigned char* b = (const unsigned char*)p;
-    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    uint64_t lane[4] = {h, h + 1, h + 2, h + 3};
+    size_t i = 0;
+    for (; i + 32 <= n; i += 32)
+        for (int k = 0; k < 4; k++) {
+            uint64_t w;
+            std::memcpy(&w, b + i + 8 * k, 8);
+            lane[k] = mix_word(lane[k], w);
+        }
+    h = mix_word(mix_word(mix_word(mix_word(h ^ n, lane[0]), lane[1]), lane[2]), lane[3]);
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        h = mix_word(h, w);
+    }
+    for (; i < n; i++) h = (h ^ b[i]) * 0x100000001B3ull;
+    h ^= h >> 33;
+    h *= 0xFF51AFD7ED558CCDull;
+    h ^= h >> 33;
     return h;
 }
 
 // The hash of what the plan depends on: counts and every input array.
 static uint64_t input_hash(const tbem_fmm_desc* d) {
-    uint64_t h = fnv1a(&d->n_obs, sizeof(d->n_obs));
-    h = fnv1a(&d->n_src, sizeof(d->n_src), h);
+    uint64_t h = hash64(&d->n_obs, sizeof(d->n_obs));
+    h = hash64(&d->n_src, sizeof(d->n_src), h);
     if (d->n_obs) {
-        h = fnv1a(d->obs_locs, 3 * d->n_obs * sizeof(double), h);
-        h = fnv1a(d->obs_normals, 3 * d->n_obs * sizeof(double), h);
+        h = hash64(d->obs_locs, 3 * d->n_obs * sizeof(double), h);
+        h = hash64(d->obs_normals, 3 * d->n_obs * sizeof(double), h);
     }
     if (d->n_src) {
-        h = fnv1a(d->src_locs, 3 * d->n_src * sizeof(double), h);
-        h = fnv1a(d->src_normals, 3 * d->n_src * sizeof(double), h);
-        h = fnv1a(d->src_weights, d->n_src * sizeof(double), h);
+        h = hash64(d->src_locs, 3 * d->n_src * sizeof(double), h);
+        h = hash64(d->src_normals, 3 * d->n_src * sizeof(double), h);
+        h = hash64(d->src_weights, d->n_src * sizeof(double), h);
     }
     return h;
 }
@@ -330,24 +358,16 @@ static HostTree get_tree(PlanReader& r, int n_pts) {
     return t;
 }
 
-static std::vector<int> flat_pairs(const std::vector<std::array<int, 2>>& v) {
-    std::vector<int> f(2 * v.size());
-    for (size_t i = 0; i < v.size(); i++) {
-        f[2 * i] = v[i][0];
-        f[2 * i + 1] = v[i][1];
-    }
-    return f;
-}
-
 static std::vector<std::array<int, 2>> read_pairs(PlanReader& r, int n_obs_cells, int n_src_cells) {
-    const std::vector<int> f = r.vec<int>();
-    TBEM_REQUIRE(f.size() % 2 == 0, TBEM_E_INVALID, "saved plan: odd pair list");
-    std::vector<std::array<int, 2>> v(f.size() / 2);
-    for (size_t i = 0; i < v.size(); i++) {
-        v[i] = {f[2 * i], f[2 * i + 1]};
-        TBEM_REQUIRE(f[2 * i] >= 0 && f[2 * i] < n_obs_cells && f[2 * i + 1] >= 0 && f[2 * i + 1] < n_src_cells,
-                     TBEM_E_INVALID, "saved plan: task list names a cell outside the trees");
-    }
+    const uint64_t k = r.pod<uint64_t>();
+    TBEM_REQUIRE(k % 2 == 0 && k / 2 <= (r.n - r.off) / sizeof(std::array<int, 2>), TBEM_E_INVALID,
+                 "saved plan: truncated");
+    std::vector<std::array<int, 2>> v(k / 2);
+    if (k) std::memcpy((void*)v.data(), r.p + r.off, k * sizeof(int));
+    r.off += k * sizeof(int);
+    bool ok = true;
+    for (const auto& q : v) ok &= q[0] >= 0 && q[0] < n_obs_cells && q[1] >= 0 && q[1] < n_src_cells;
+    TBEM_REQUIRE(ok, TBEM_E_INVALID, "saved plan: task list names a cell outside the trees");
     return v;
 }
 
@@ -380,11 +400,15 @@ static std::vector<char> save_plan(const FmmPlan& P) {
     w.vec(P.h_uc2e);
     w.vec(P.h_dc2e);
     for (const auto& f : P.h_fac) w.vec(f);
-    for (const auto* v : {&P.tl.p2p, &P.tl.m2l, &P.tl.m2p, &P.tl.p2l}) w.vec(flat_pairs(*v));
+    for (const auto* v : {&P.tl.p2p, &P.tl.m2l, &P.tl.m2p, &P.tl.p2l}) {
+        w.pod((uint64_t)(2 * v->size()));  // as flat (obs cell, src cell) ints
+        const char* q = (const char*)v->data();
+        w.b.insert(w.b.end(), q, q + v->size() * sizeof((*v)[0]));
+    }
     for (const auto* v : {&P.tl.p2m, &P.tl.m2m, &P.tl.l2l, &P.tl.l2p}) w.vec(*v);
     std::vector<char> out(24);
     std::memcpy(out.data(), "TBEMPLN1", 8);
-    const uint64_t len = w.b.size(), hash = fnv1a(w.b.data(), w.b.size());
+    const uint64_t len = w.b.size(), hash = hash64(w.b.data(), w.b.size());
     std::memcpy(out.data() + 8, &len, 8);
     std::memcpy(out.data() + 16, &hash, 8);
     out.insert(out.end(), w.b.begin(), w.b.end());
@@ -398,7 +422,7 @@ static SavedPlan load_plan(const tbem_fmm_desc* d, const void* buf, size_t n) {
     std::memcpy(&len, (const char*)buf + 8, 8);
     std::memcpy(&hash, (const char*)buf + 16, 8);
     TBEM_REQUIRE(len == n - 24, TBEM_E_INVALID, "saved plan: truncated");
-    TBEM_REQUIRE(fnv1a((const char*)buf + 24, len) == hash, TBEM_E_INVALID, "saved plan: checksum mismatch");
+    TBEM_REQUIRE(hash64((const char*)buf + 24, len) == hash, TBEM_E_INVALID, "saved plan: checksum mismatch");
     PlanReader r{(const char*)buf + 24, (size_t)len, 0};
     const int kid = r.pod<int>();
     const double mu = r.pod<double>(), nu = r.pod<double>();
