@@ -65,7 +65,6 @@ class ConstraintMatrix:
 
     def __init__(self, handle, lib):
         self._h, self._lib = handle, lib
-        self._keep = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -105,6 +104,21 @@ def from_constraints(constraints):
     w = np.asarray(weights, dtype=np.float64)
     h = ctypes.c_void_p()
     _lib.check(lib.tbem_constraints_create(len(constraints), _szp(ptr), _szp(d), _dp(w), _dp(rhs), ctypes.byref(h)))
+    return ConstraintMatrix(h, lib)
+
+
+def from_constraint_arrays(eq_ptr, term_dof, term_weight, rhs):
+    """from_constraints on equations given as arrays: equation e has the terms
+    eq_ptr[e] .. eq_ptr[e+1]-1 (term_dof, term_weight) and right-hand side rhs[e]."""
+    lib = _lib.load()
+    ptr = np.ascontiguousarray(eq_ptr, dtype=np.uint64)
+    d = np.ascontiguousarray(term_dof, dtype=np.uint64)
+    w = np.ascontiguousarray(term_weight, dtype=np.float64)
+    r = np.ascontiguousarray(rhs, dtype=np.float64)
+    if ptr.size != r.size + 1 or d.size != w.size or int(ptr[-1]) != d.size:
+        raise ValueError("eq_ptr must have len(rhs) + 1 entries ending at len(term_dof) == len(term_weight)")
+    h = ctypes.c_void_p()
+    _lib.check(lib.tbem_constraints_create(r.size, _szp(ptr), _szp(d), _dp(w), _dp(r), ctypes.byref(h)))
     return ConstraintMatrix(h, lib)
 
 

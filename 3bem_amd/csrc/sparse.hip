@@ -151,14 +151,12 @@ struct tbem_integral_s {
     void* far = nullptr;
     size_t far_rows = 0, far_cols = 0;
     tbem::DevBuf<double> t_interp, t_far, t_near, t_corr, xb, yb;
-    hipStream_t stream = nullptr, side = nullptr;
-    hipEvent_t done = nullptr, fork = nullptr, join = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
     std::mutex mu;
     ~tbem_integral_s() {
-        for (hipEvent_t e : {done, fork, join})
-            if (e) (void)hipEventDestroy(e);
-        for (hipStream_t st : {stream, side})
-            if (st) (void)hipStreamDestroy(st);
+        if (done) (void)hipEventDestroy(done);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -241,22 +239,19 @@ static void far_apply(tbem_integral_s& I, const double* x, double* y, hipStream_
     if (rc != TBEM_OK) throw Error(rc, std::string("far field: ") + tbem_last_error());
 }
 
-// y = N x + (G F(I x) + C x). The near field and the correction run on a side
-// stream while the interpolation and the far field run on s.
+// y = N x + (G F(I x) + C x), all on stream s: the near field and the correction
+// first, then the interpolation and the far field, then the Galerkin product with
+// both addends in its epilogue. (Running the sparse products on a second stream
+// beside the FMM was slower: they contend with its kernels for the compute units and
+// HBM, and the FMM's time grows by more than theirs.)
 static void integral_apply(tbem_integral_s& I, const double* x, double* y, hipStream_t s) {
     TBEM_HIP(hipStreamWaitEvent(s, I.done, 0));
     for (tbem_sparse_t A : {I.near, I.corr, I.gal, I.interp}) TBEM_HIP(hipStreamWaitEvent(s, A->done, 0));
-    TBEM_HIP(hipEventRecord(I.fork, s));
-    TBEM_HIP(hipStreamWaitEvent(I.side, I.fork, 0));
-    sparse_run(*I.near, x, nullptr, nullptr, I.t_near.p, I.side);
-    sparse_run(*I.corr, x, nullptr, nullptr, I.t_corr.p, I.side);
-    TBEM_HIP(hipEventRecord(I.join, I.side));
+    sparse_run(*I.near, x, nullptr, nullptr, I.t_near.p, s);
+    sparse_run(*I.corr, x, nullptr, nullptr, I.t_corr.p, s);
     sparse_run(*I.interp, x, nullptr, nullptr, I.t_interp.p, s);
     far_apply(I, I.t_interp.p, I.t_far.p, s);
-    TBEM_HIP(hipStreamWaitEvent(s, I.join, 0));
-    if (I.gal->n_rows) {
-        if (I.gal->n_blocks) sparse_run(*I.gal, I.t_far.p, I.t_near.p, I.t_corr.p, y, s);
-    }
+    sparse_run(*I.gal, I.t_far.p, I.t_near.p, I.t_corr.p, y, s);
     for (tbem_sparse_t A : {I.near, I.corr, I.gal, I.interp}) TBEM_HIP(hipEventRecord(A->done, s));
     TBEM_HIP(hipEventRecord(I.done, s));
 }
@@ -457,9 +452,7 @@ int tbem_integral_create(tbem_sparse_t nearfield, tbem_sparse_t farfield_correct
         TBEM_REQUIRE(interp->n_rows == h->far_cols && interp->n_cols == n_cols, TBEM_E_INVALID,
                      "interp must map the operator's columns to the far field's columns");
         TBEM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        TBEM_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&h->done, &h->fork, &h->join})
-            TBEM_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        TBEM_HIP(hipEventCreateWithFlags(&h->done, hipEventDisableTiming));
         h->t_interp.alloc(h->far_cols);
         h->t_far.alloc(h->far_rows);
         h->t_near.alloc(n_rows);
@@ -493,7 +486,6 @@ void tbem_integral_destroy(tbem_integral_t h) {
     if (!h) return;
     (void)hipEventSynchronize(h->done);
     (void)hipStreamSynchronize(h->stream);
-    (void)hipStreamSynchronize(h->side);
     delete h;
 }
 

@@ -382,6 +382,35 @@ def near_field_run(tbem, torch, steps, warmup, skip_cpu):
         log(f"[bench] near field CPU ({kind}) {out['cpu_baseline']['value']} G entries/s; "
             f"bit-exact {out['bit_exact_vs_cpu_sample']}")
     del rp, ci, v
+    # continuity constraints of the mesh's discontinuous dofs: what wraps every matvec
+    # of the constrained solve (distribute before, condense after)
+    cons = importlib.import_module("3bem_amd.constraints")
+    t0 = time.perf_counter()
+    ep, td, tw, rhs, n_total = wl.grid_continuity_constraints(nx, ny, 3, all_pairs=False)
+    cm = cons.from_constraint_arrays(ep, td, tw, rhs)
+    t_cm = time.perf_counter() - t0
+    assert n_total == m
+    red = torch.rand(m - cm.size(), dtype=torch.float64, device="cuda")
+    full = torch.rand(m, dtype=torch.float64, device="cuda")
+    cons.distribute_vector(cm, red, m)
+    cons.condense_vector(cm, full)
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    for _ in range(steps):
+        cons.distribute_vector(cm, red, m)
+    e1.record()
+    for _ in range(steps):
+        cons.condense_vector(cm, full)
+    e2.record()
+    torch.cuda.synchronize()
+    out["constraints"] = {"dofs": m, "equations": int(len(rhs)), "constrained": cm.size(),
+                          "from_constraints_s": round(t_cm, 2),
+                          "distribute_ms": round(e0.elapsed_time(e1) / steps, 4),
+                          "condense_ms": round(e1.elapsed_time(e2) / steps, 4)}
+    log(f"[bench] constraints: {cm.size()} of {m} dofs constrained; distribute "
+        f"{out['constraints']['distribute_ms']:.3f} ms, condense {out['constraints']['condense_ms']:.3f} ms")
+    del cm, red, full
     facets = wl.rough_fault_mesh(nx, ny)
     q3 = (np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]), np.full(3, 1 / 6))
     K = tbem.ElasticHypersingular(1.0, 0.25)
