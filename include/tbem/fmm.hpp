@@ -14,6 +14,9 @@
 //                                                      cpp/sparse_operator.h:12-60
 //   IntegralOperator<3,R,C>(near, correction, galerkin, farfield, interp)
 //                                                      cpp/integral_operator.h:26-64
+//   LinearTerm, ConstraintEQ, boundary_condition, continuity_constraint,
+//   shift_constraints, from_constraints, homogenize_constraints,
+//   distribute_vector, condense_vector                 cpp/constraint.h, cpp/constraint_matrix.h
 // Errors (nonzero status) become std::runtime_error carrying tbem_last_error().
 #ifndef TBEM_FMM_HPP
 #define TBEM_FMM_HPP
@@ -345,6 +348,77 @@ struct IntegralOperator : public OperatorI {
                                                                interp));
     }
 };
+
+// LinearTerm, ConstraintEQ (cpp/constraint.h:21-52) and the helpers of cpp/constraint.cpp.
+struct LinearTerm {
+    size_t dof;
+    double weight;
+    bool operator==(const LinearTerm& o) const { return dof == o.dof && weight == o.weight; }
+};
+
+struct ConstraintEQ {
+    std::vector<LinearTerm> terms;
+    double rhs;
+};
+
+inline ConstraintEQ boundary_condition(size_t dof, double value) { return {{LinearTerm{dof, 1.0}}, value}; }
+inline ConstraintEQ continuity_constraint(size_t dof1, size_t dof2) {
+    return {{LinearTerm{dof1, 1.0}, LinearTerm{dof2, -1.0}}, 0.0};
+}
+inline std::vector<ConstraintEQ> shift_constraints(const std::vector<ConstraintEQ>& cs, size_t shift_dof) {
+    std::vector<ConstraintEQ> out;
+    for (const auto& c : cs) {
+        ConstraintEQ e{{}, c.rhs};
+        for (const auto& t : c.terms) e.terms.push_back(LinearTerm{t.dof + shift_dof, t.weight});
+        out.push_back(e);
+    }
+    return out;
+}
+
+// ConstraintMatrix (cpp/constraint_matrix.h:17-25): built on the host by
+// from_constraints; distribute_vector / condense_vector run on the GPU and equal the
+// reference's bit for bit.
+struct ConstraintMatrix {
+    std::shared_ptr<tbem_constraints_s> h;
+    size_t size() const { return tbem_constraints_size(h.get()); }
+};
+
+inline ConstraintMatrix from_constraints(const std::vector<ConstraintEQ>& constraints) {
+    std::vector<size_t> ptr{0}, dof;
+    std::vector<double> w, rhs;
+    for (const auto& c : constraints) {
+        for (const auto& t : c.terms) {
+            dof.push_back(t.dof);
+            w.push_back(t.weight);
+        }
+        ptr.push_back(dof.size());
+        rhs.push_back(c.rhs);
+    }
+    tbem_constraints_t raw = nullptr;
+    check(tbem_constraints_create(constraints.size(), ptr.data(), dof.data(), w.data(), rhs.data(), &raw));
+    return ConstraintMatrix{std::shared_ptr<tbem_constraints_s>(raw, tbem_constraints_destroy)};
+}
+
+inline ConstraintMatrix homogenize_constraints(const ConstraintMatrix& cm) {
+    tbem_constraints_t raw = nullptr;
+    check(tbem_constraints_homogenize(cm.h.get(), &raw));
+    return ConstraintMatrix{std::shared_ptr<tbem_constraints_s>(raw, tbem_constraints_destroy)};
+}
+
+inline std::vector<double> distribute_vector(const ConstraintMatrix& cm, const std::vector<double>& in,
+                                             size_t n_total_dofs) {
+    if (in.size() + cm.size() != n_total_dofs)
+        throw std::invalid_argument("distribute_vector: wrong number of reduced values");
+    std::vector<double> out(n_total_dofs);
+    check(tbem_distribute_vector(cm.h.get(), in.data(), n_total_dofs, out.data()));
+    return out;
+}
+
+inline std::vector<double> condense_vector(const ConstraintMatrix& cm, const std::vector<double>& all) {
+    std::vector<double> out(all.size() - cm.size());
+    check(tbem_condense_vector(cm.h.get(), all.data(), all.size(), out.data()));
+    return out;
+}
 
 }  // namespace tbem
 

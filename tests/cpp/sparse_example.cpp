@@ -9,7 +9,9 @@
 //          N = A, C = A, G (n_rows x n_pts, G[i, i % n_pts] = 1 + 0.001 i),
 //          I (n_pts x n_cols, I[j, j % n_cols] = 0.5), F = Laplace single-layer direct
 //          operator on the points; then A's row_ptrs (n_rows + 1 int64) and
-//          column_indices (nnz int64), and the nnz of A (A^T built by csr_from_coo).
+//          column_indices (nnz int64), the nnz of A^T A (A^T built by csr_from_coo), and 7
+//          doubles: the size, condensed value and distributed vector of the reference's
+//          CondenseThenDistribute constraint case.
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -74,6 +76,13 @@ int main(int argc, char** argv) {
         fwrite(ci.data(), 8, ci.size(), out);
         const int64_t nnz_ata = (int64_t)AtA.nnz();
         fwrite(&nnz_ata, 8, 1, out);
+        // constraints: the reference's CondenseThenDistribute case
+        auto cm = from_constraints({boundary_condition(1, 4.0), continuity_constraint(0, 2),
+                                    continuity_constraint(2, 3), continuity_constraint(3, 4)});
+        auto red = condense_vector(cm, {2.0, 4.0, 4.0, 4.0, 4.0});
+        auto full = distribute_vector(cm, red, 5);
+        const double cres[7] = {(double)cm.size(), red[0], full[0], full[1], full[2], full[3], full[4]};
+        fwrite(cres, 8, 7, out);
     } catch (const std::exception& ex) {
         fprintf(stderr, "sparse_example: %s\n", ex.what());
         fclose(out);

@@ -77,7 +77,9 @@ def test_cpp_sparse_and_integral_operator(orc, tbem, tmp_path):
     raw = np.fromfile(out, dtype=np.uint8)
     y, y2, yi = (raw[i * 8 * n_rows:(i + 1) * 8 * n_rows].view(np.float64) for i in range(3))
     rest = raw[24 * n_rows:].view(np.int64)
-    rp, ci, nnz_ata = rest[:n_rows + 1], rest[n_rows + 1:n_rows + 1 + nnz], int(rest[-1])
+    rp, ci, nnz_ata = rest[:n_rows + 1], rest[n_rows + 1:n_rows + 1 + nnz], int(rest[n_rows + 1 + nnz])
+    cres = rest[n_rows + 2 + nnz:].view(np.float64)
+    assert cres.tolist() == [4.0, 14.0, 14.0, 4.0, 14.0, 14.0, 14.0]  # unit_tests CondenseThenDistribute
     rp_o, ci_o, v_o, y_o = orc.sparse_coo(n_rows, n_cols, rows, cols, vals, x)
     assert np.array_equal(rp, rp_o) and np.array_equal(ci, ci_o)
     assert np.array_equal(y.view(np.uint64), y_o.view(np.uint64))
