@@ -25,7 +25,7 @@
 // dof's level is one more than the highest level among the constraints that name it.
 // Levels run top down, then one launch finishes the free dofs. Every product is rounded
 // on its own and every sum follows the reference's order, so both vectors equal the
-// reference's bit for bit. Continuity constraints give one to three levels.
+// reference's bit for bit. Continuity chains give a handful of levels (one per link).
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -198,13 +198,25 @@ struct ConsPlan {
 
 }  // namespace
 
+// Calls on one handle are ordered, even across streams: each waits for the previous
+// call's device work (done) before using the handle's plan and scratch, and marks its
+// own end.
 struct tbem_constraints_s {
     SolvedMap map;
     std::unique_ptr<ConsPlan> plan;
     tbem::DevBuf<double> xb, yb;
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
     std::mutex mu;
+    void ready() {
+        if (!stream) TBEM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (!done) {
+            TBEM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+            TBEM_HIP(hipEventRecord(done, stream));
+        }
+    }
     ~tbem_constraints_s() {
+        if (done) (void)hipEventDestroy(done);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -331,7 +343,10 @@ void build_plan(tbem_constraints_s& h, size_t n_total) {
 }
 
 ConsPlan& plan_for(tbem_constraints_s& h, size_t n_total) {
-    if (!h.plan || h.plan->n_total != n_total) build_plan(h, n_total);
+    if (!h.plan || h.plan->n_total != n_total) {
+        TBEM_HIP(hipEventSynchronize(h.done));  // no earlier call may still read the old plan
+        build_plan(h, n_total);
+    }
     return *h.plan;
 }
 
@@ -346,17 +361,21 @@ void launch_rows(const RowSet& r, const double* x, const double* add, int finish
 
 void distribute(tbem_constraints_s& h, const double* in, size_t n_total, double* out, hipStream_t s) {
     ConsPlan& P = plan_for(h, n_total);
+    TBEM_HIP(hipStreamWaitEvent(s, h.done, 0));
     if (P.n_free) {
         k_cons_scatter<<<tbem::div_up(P.n_free, 256), 256, 0, s>>>(P.n_free, P.free_dof.p, in, out);
         TBEM_LAUNCH_CHECK();
     }
     for (const RowSet& r : P.dist) launch_rows(r, out, nullptr, 0, true, out, s);
+    TBEM_HIP(hipEventRecord(h.done, s));
 }
 
 void condense(tbem_constraints_s& h, const double* all, size_t n_all, double* out, hipStream_t s) {
     ConsPlan& P = plan_for(h, n_all);
+    TBEM_HIP(hipStreamWaitEvent(s, h.done, 0));
     for (const RowSet& r : P.cond) launch_rows(r, P.acc.p, all, 0, false, P.acc.p, s);
     launch_rows(P.cond_free, P.acc.p, all, 1, false, out, s);
+    TBEM_HIP(hipEventRecord(h.done, s));
 }
 
 // add_entry_with_constraints (constraint_matrix.cpp:127-165): an entry on a free
@@ -448,7 +467,7 @@ int tbem_distribute_vector_dev(tbem_constraints_t h, const double* d_in, size_t 
     return cs_guard([&] {
         TBEM_REQUIRE(h && (d_out || n_total == 0), TBEM_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(h->mu);
-        if (!h->stream) TBEM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->ready();
         distribute(*h, d_in, n_total, d_out, (hipStream_t)stream);
     });
 }
@@ -457,7 +476,7 @@ int tbem_condense_vector_dev(tbem_constraints_t h, const double* d_all, size_t n
     return cs_guard([&] {
         TBEM_REQUIRE(h && (d_all || n_all == 0), TBEM_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(h->mu);
-        if (!h->stream) TBEM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->ready();
         condense(*h, d_all, n_all, d_out, (hipStream_t)stream);
     });
 }
@@ -466,7 +485,7 @@ int tbem_distribute_vector(tbem_constraints_t h, const double* in, size_t n_tota
     return cs_guard([&] {
         TBEM_REQUIRE(h && (out || n_total == 0), TBEM_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(h->mu);
-        if (!h->stream) TBEM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->ready();
         ConsPlan& P = plan_for(*h, n_total);
         TBEM_REQUIRE(in || P.n_free == 0, TBEM_E_INVALID, "null argument");
         if (h->xb.n < (size_t)P.n_free) h->xb.alloc(P.n_free);
@@ -482,7 +501,7 @@ int tbem_condense_vector(tbem_constraints_t h, const double* all, size_t n_all, 
     return cs_guard([&] {
         TBEM_REQUIRE(h && (all || n_all == 0), TBEM_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(h->mu);
-        if (!h->stream) TBEM_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->ready();
         ConsPlan& P = plan_for(*h, n_all);
         TBEM_REQUIRE(out || P.n_free == 0, TBEM_E_INVALID, "null argument");
         if (h->xb.n < n_all) h->xb.alloc(n_all);
@@ -539,6 +558,7 @@ int tbem_condense_sparse(tbem_constraints_t row_cm, tbem_constraints_t col_cm, t
 
 void tbem_constraints_destroy(tbem_constraints_t h) {
     if (!h) return;
+    if (h->done) (void)hipEventSynchronize(h->done);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     delete h;
 }
