@@ -154,6 +154,8 @@ def distribute_vector(cm, v, n_total):
 def condense_vector(cm, v):
     """The full vector v to the reduced one (len(v) - cm.size() entries)."""
     n = v.numel() if _is_torch(v) else np.asarray(v).size
+    if n < cm.size():
+        raise ValueError(f"condense_vector: {n} values for {cm.size()} constraints")
     return _vec_op(cm, cm._lib.tbem_condense_vector, cm._lib.tbem_condense_vector_dev, v, n, n - cm.size())
 
 

@@ -135,3 +135,23 @@ def test_constrained_gmres_solves_the_condensed_system(tbem):
     assert max(abs(fv[td[ep[e]]] - fv[td[ep[e] + 1]]) for e in two) < 1e-12  # continuity holds
     one = [td[ep[e]] for e in range(len(rhs)) if ep[e + 1] - ep[e] == 1]
     assert np.allclose(fv[one], 0.3, rtol=0, atol=1e-14)  # boundary values hold
+
+
+def test_constraints_edge_cases(tbem):
+    """No constraints (distribute and condense are the identity), every dof
+    constrained (empty reduced vector), and refusals for lengths that do not fit."""
+    print("CASE constraints edge cases", flush=True)
+    none = C.from_constraints([])
+    v = np.arange(5.0)
+    assert C.distribute_vector(none, v, 5).tolist() == v.tolist()
+    assert C.condense_vector(none, v).tolist() == v.tolist()
+    allc = C.from_constraints([C.boundary_condition(d, 1.0 + d) for d in range(3)])
+    assert C.distribute_vector(allc, np.zeros(0), 3).tolist() == [1.0, 2.0, 3.0]
+    assert C.condense_vector(allc, [5.0, 6.0, 7.0]).size == 0
+    with pytest.raises(ValueError):
+        C.distribute_vector(allc, np.zeros(1), 3)
+    with pytest.raises(ValueError):
+        C.condense_vector(allc, [1.0, 2.0])  # fewer values than constraints
+    far = C.from_constraints([C.boundary_condition(0, 1.0), C.boundary_condition(5, 1.0)])
+    with pytest.raises(tbem.TbemError, match="beyond"):
+        C.condense_vector(far, [1.0, 2.0, 3.0])  # a constrained dof beyond the vector

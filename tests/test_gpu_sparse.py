@@ -217,3 +217,17 @@ def test_integral_operator_shape_errors(tbem, orc):
         tbem.IntegralOperator(parts["near"][3], parts["corr"][3], interp, far, G)
     with pytest.raises(TypeError):
         tbem.IntegralOperator(parts["near"][3], parts["corr"][3], G, parts["near"][3], interp)
+
+
+def test_sparse_and_integral_empty_shapes(tbem):
+    """Zero rows, zero columns and zero entries: the products return the right
+    lengths (zeros where rows exist) without launching anything out of range."""
+    print("CASE sparse empty shapes", flush=True)
+    S = tbem.SparseOperator
+    z = S.csr_from_coo(0, 5, [], [], [])
+    assert z.apply(np.ones(5)).size == 0
+    e = S.csr_from_coo(4, 0, [], [], [])
+    assert e.apply(np.zeros(0)).tolist() == [0.0] * 4
+    r = S(3, 3, [], [], [0, 0, 0, 0])
+    assert r.apply(np.ones(3)).tolist() == [0.0] * 3 and r.nnz() == 0
+    assert S.csr_from_coo(0, 0, [], [], []).right_multiply(S.csr_from_coo(0, 2, [], [], [])).n_cols() == 2
