@@ -17,7 +17,7 @@
 // one class go to one XCD (workgroup i runs on XCD i % 8), so each S_c is fetched
 // from HBM once and then served by that XCD's L2.
 //
-// Apply: one workgroup (4 wave64s) per unit stages its 64 up-check potentials in
+// Apply: one workgroup (4 or 8 wave64s) per unit stages its 64 up-check potentials in
 // LDS as FP32 (k x 64; column n of row k stored at n ^ 16*(k%4), which keeps both
 // the staging writes and the B-operand reads free of bank conflicts), then each
 // wave runs row tiles of S_c:
@@ -46,6 +46,7 @@ constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 Ki
 // (order 20) it is already faster (2M points: 3.0 vs 5.2 ms).
 constexpr int kMinBlk = 48;
 constexpr int kXcds = 8;
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
 constexpr int kMaxChunk = 16384;  // classes per plan-time batch (grid y/z limit)
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -159,7 +160,14 @@ __device__ __forceinline__ void m2l_row_tile(const float* __restrict__ A, const 
     }
 }
 
-__global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int ystride, const float* __restrict__ S,
+// NW waves per workgroup share the unit's staged columns; wave w takes row tiles
+// w, w + NW, ... With large operators the staged columns (up to 128 KiB) leave room
+// for two workgroups per CU, and eight waves per workgroup then keep more MFMA
+// work in flight (order 100 elastic, blk 297: 5.65 -> 4.11 ms); with small operators
+// many workgroups fit anyway and four waves balance the row tiles better (order 75
+// Laplace, blk 74: 15.2 ms at four waves against 16.5 at eight).
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_m2l_mfma(int n, int mt, int kt, int ystride, const float* __restrict__ S,
                                                   const int* __restrict__ ucls, const int* __restrict__ uoff,
                                                   const int* __restrict__ ucnt, const int* __restrict__ colpair,
                                                   const int* __restrict__ pair_src,
@@ -172,7 +180,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int yst
     const int kpad = kt * 4;
     // stage: lane = column (pair), waves stride over k
     const float* col = lane < cnt ? chk + (size_t)pair_src[colpair[off + lane]] * n : nullptr;
-    for (int kk = wave; kk < kpad; kk += 4)
+    for (int kk = wave; kk < kpad; kk += NW)
         Bs[kk * 64 + (lane ^ ((kk & 3) << 4))] = (col && kk < n) ? col[kk] : 0.f;
     __syncthreads();
     // B operand of column tile ct at k-step ks: row ks*4 + lq, column ct*16 + lr,
@@ -184,10 +192,10 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(int n, int mt, int kt, int yst
     const float* Acls = S + (size_t)cls * mt * kt * 64 + lane;
     float cur[kPre];
     for (int i = 0; i < kPre; i++) cur[i] = (wave < mt && i < kt) ? Acls[((size_t)wave * kt + i) * 64] : 0.f;
-    for (int rt = wave; rt < mt; rt += 4) {
+    for (int rt = wave; rt < mt; rt += NW) {
         f32x4 acc[4] = {};
         const float* A = Acls + (size_t)rt * kt * 64;
-        const float* An = rt + 4 < mt ? A + (size_t)4 * kt * 64 : nullptr;
+        const float* An = rt + NW < mt ? A + (size_t)NW * kt * 64 : nullptr;
         switch (nct) {
         case 1: m2l_row_tile<1>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
         case 2: m2l_row_tile<2>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
@@ -299,8 +307,10 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     TBEM_HIP(hipGetDevice(&dev));
     TBEM_HIP(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     if (shm > (size_t)max_shm) return;
-    if (shm > 65536)
-        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    if (shm > 65536) {
+        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    }
     const int n_pairs = (int)m2l.cell.size();
 
     // classes: (level_t, level_s, q) with q = (c_s - c_t) / finer half-width, integral
@@ -475,9 +485,16 @@ void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up,
                      bool transpose) {
     if (!M.enabled || M.n_units == 0) return;
     const size_t shm = (size_t)M.kt * 4 * kUnitCols * sizeof(float);
-    hipLaunchKernelGGL(k_m2l_mfma, dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride,
-                       transpose ? M.ST.p : M.S.p, M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p,
-                       transpose ? M.pair_tgt.p : pair_src, chk_up, Y);
+    // eight waves when the staged columns allow at most two workgroups per CU (160 KiB of LDS)
+    const bool wide = 3 * shm > kLdsPerCu;
+    const float* S = transpose ? M.ST.p : M.S.p;
+    const int* src = transpose ? M.pair_tgt.p : pair_src;
+    if (wide)
+        hipLaunchKernelGGL(k_m2l_mfma<8>, dim3(M.n_units), dim3(512), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
+                           M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, src, chk_up, Y);
+    else
+        hipLaunchKernelGGL(k_m2l_mfma<4>, dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
+                           M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, src, chk_up, Y);
     TBEM_LAUNCH_CHECK();
 }
 
