@@ -47,6 +47,11 @@ constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 Ki
 constexpr int kMinBlk = 48;
 constexpr int kXcds = 8;
 constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
+// k-steps of A loaded ahead (see m2l_row_tile): the wide kernel (eight waves, two
+// workgroups per CU) hides more latency per wave with 16; the narrow one, with many
+// workgroups per CU, does better with 8 (order 75 Laplace: 15.2 -> 14.0 ms).
+constexpr int kPreWide = 16;
+constexpr int kPreNarrow = 8;
 constexpr int kMaxChunk = 16384;  // classes per plan-time batch (grid y/z limit)
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -131,12 +136,7 @@ __global__ void k_m2l_swizzle(int n, int mt, int kt, int trans, const double* __
 // loads the first kPre k-steps of the wave's next row tile (An; null for none), so
 // the pipeline does not drain between row tiles: cur holds A's first kPre k-steps
 // on entry and An's on exit.
-#ifndef TBEM_M2L_KPRE
-#define TBEM_M2L_KPRE 16
-#endif
-constexpr int kPre = TBEM_M2L_KPRE;
-
-template <int NCT>
+template <int NCT, int kPre>
 __device__ __forceinline__ void m2l_row_tile(const float* __restrict__ A, const float* __restrict__ An,
                                              const float* __restrict__ b, int kt, int s0, int s1, int s2, int s3,
                                              f32x4* acc, float* cur) {
@@ -166,7 +166,7 @@ __device__ __forceinline__ void m2l_row_tile(const float* __restrict__ A, const 
 // work in flight (order 100 elastic, blk 297: 5.65 -> 4.11 ms); with small operators
 // many workgroups fit anyway and four waves balance the row tiles better (order 75
 // Laplace, blk 74: 15.2 ms at four waves against 16.5 at eight).
-template <int NW>
+template <int NW, int kPre>
 __global__ __launch_bounds__(NW * 64) void k_m2l_mfma(int n, int mt, int kt, int ystride, const float* __restrict__ S,
                                                   const int* __restrict__ ucls, const int* __restrict__ uoff,
                                                   const int* __restrict__ ucnt, const int* __restrict__ colpair,
@@ -197,10 +197,10 @@ __global__ __launch_bounds__(NW * 64) void k_m2l_mfma(int n, int mt, int kt, int
         const float* A = Acls + (size_t)rt * kt * 64;
         const float* An = rt + NW < mt ? A + (size_t)NW * kt * 64 : nullptr;
         switch (nct) {
-        case 1: m2l_row_tile<1>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
-        case 2: m2l_row_tile<2>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
-        case 3: m2l_row_tile<3>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
-        default: m2l_row_tile<4>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
+        case 1: m2l_row_tile<1, kPre>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
+        case 2: m2l_row_tile<2, kPre>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
+        case 3: m2l_row_tile<3, kPre>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
+        default: m2l_row_tile<4, kPre>(A, An, b, kt, s0, s1, s2, s3, acc, cur); break;
         }
         // D[4*lq + r][lr] of column tile ct -> Y[pair][rt*16 + 4*lq + r]
         for (int ct = 0; ct < nct; ct++) {
@@ -308,8 +308,10 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     TBEM_HIP(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     if (shm > (size_t)max_shm) return;
     if (shm > 65536) {
-        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4, kPreNarrow>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)shm));
+        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<8, kPreWide>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)shm));
     }
     const int n_pairs = (int)m2l.cell.size();
 
@@ -490,10 +492,10 @@ void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up,
     const float* S = transpose ? M.ST.p : M.S.p;
     const int* src = transpose ? M.pair_tgt.p : pair_src;
     if (wide)
-        hipLaunchKernelGGL(k_m2l_mfma<8>, dim3(M.n_units), dim3(512), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
+        hipLaunchKernelGGL((k_m2l_mfma<8, kPreWide>), dim3(M.n_units), dim3(512), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
                            M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, src, chk_up, Y);
     else
-        hipLaunchKernelGGL(k_m2l_mfma<4>, dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
+        hipLaunchKernelGGL((k_m2l_mfma<4, kPreNarrow>), dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
                            M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, src, chk_up, Y);
     TBEM_LAUNCH_CHECK();
 }
