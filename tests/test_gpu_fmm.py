@@ -191,11 +191,12 @@ def test_gpu_fmm_adjoint_is_exact_transpose(tbem, kid, small):
     assert abs(lhs - rhs) < 1e-13 * scale, (lhs, rhs)
 
 
-@pytest.mark.parametrize("kid,order", [(3, 50), (0, 75)])
+@pytest.mark.parametrize("kid,order", [(3, 50), (0, 75), (3, 100)])
 def test_gpu_mixed_adjoint_with_mfma_transpose(tbem, wl, kid, order):
     """Mixed precision with the M2L transpose on MFMA (M_c^T in the modal basis):
     agrees with the FP64 adjoint to the FP32 bound, and is the transpose of the
-    mixed forward to that bound."""
+    mixed forward to that bound. Elastic order 100 (blk 297) runs the eight-wave
+    MFMA kernel; the other two the four-wave one."""
     g = wl.cfg2_fault_and_surface(30000)
     K = tbem.make_kernel(kid, 1.0, 0.25)
     R = components(kid)
@@ -206,11 +207,13 @@ def test_gpu_mixed_adjoint_with_mfma_transpose(tbem, wl, kid, order):
     case("mixed adjoint mfma", kid, order)
     op32 = tbem.FMMOperator(K, data, cfg, precision="mixed")
     assert op32.info()["m2l_classes"] > 0
+    op64 = tbem.FMMOperator(K, data, cfg)
     xt32 = op32.apply_adjoint(gg)
-    xt64 = tbem.FMMOperator(K, data, cfg).apply_adjoint(gg)
+    xt64 = op64.apply_adjoint(gg)
     err = rel_err(xt32, xt64)
-    print(f"  mixed adjoint vs FP64 adjoint {err:.3e}")
-    assert err < 1e-6
+    ef = rel_err(op32.apply(x), op64.apply(x))
+    print(f"  mixed adjoint vs FP64 adjoint {err:.3e}, forward {ef:.3e}")
+    assert err < 1e-6 and ef < 1e-6
     lhs, rhs = op32.apply(x) @ gg, x @ xt32
     assert abs(lhs - rhs) < 1e-6 * abs(lhs), (lhs, rhs)
 
