@@ -60,7 +60,7 @@ def tri_three_point(verts, tris, outward_from=None):
     return pts, normals, weights
 
 
-def cfg1_icosphere(seed=SEED):
+def icosphere_surface_15k(seed=SEED):
     """15,360 points: 5,120-triangle icosphere, 3 points per triangle, outward normals."""
     v, f = icosphere(4)
     pts, nrm, w = tri_three_point(v, f, outward_from=np.zeros(3))
@@ -68,7 +68,7 @@ def cfg1_icosphere(seed=SEED):
     return dict(locs=pts, normals=nrm, weights=w, x=x)
 
 
-def cfg2_fault_and_surface(n_total=1_000_000, seed=SEED):
+def fault_and_free_surface(n_total=1_000_000, seed=SEED):
     """Half the points uniform on the free surface [-1/2,1/2]^2 x {0}, half uniform on a
     0.5 x 0.5 fault plane dipping 45 degrees about the x axis, centred at depth 0.3
     (centre (0, 0, -0.3); along-strike x, down-dip direction (0, cos45, -sin45))."""
@@ -88,7 +88,7 @@ def cfg2_fault_and_surface(n_total=1_000_000, seed=SEED):
     return dict(locs=locs, normals=np.vstack([sn, fn]), weights=np.concatenate([sw, fw]), x=x)
 
 
-def cfg3_cube_clusters(n_total=4_000_000, seed=SEED):
+def cube_and_clusters(n_total=4_000_000, seed=SEED):
     """Half uniform in [0,1]^3, half from 8 Gaussian clusters (sigma 0.02, centres uniform)."""
     rng = np.random.default_rng(seed)
     n1 = n_total // 2
@@ -103,7 +103,7 @@ def cfg3_cube_clusters(n_total=4_000_000, seed=SEED):
     return dict(locs=locs, normals=normals, weights=np.ones(n_total), x=x)
 
 
-def cfg4_fault_and_stations(n_tri=40_000, n_obs=200_000, seed=SEED):
+def fault_and_stations(n_tri=40_000, n_obs=200_000, seed=SEED):
     """Planar rectangular fault (1 x 0.5, dipping 45 degrees, top at depth 0.1) meshed into
     n_tri triangles (3 points each) and n_obs surface stations uniform in [-1,1]^2 x {0}."""
     rng = np.random.default_rng(seed)
@@ -132,7 +132,7 @@ def cfg4_fault_and_stations(n_tri=40_000, n_obs=200_000, seed=SEED):
                 x_true=slip_true)
 
 
-def cfg5_rough_fault(n_total=2_000_000, seed=SEED):
+def rough_fault_points(n_total=2_000_000, seed=SEED):
     """n_total points on z = 0.05 sin(6x) cos(4y) + 0.005 * N(0,1) roughness over [0,1]^2."""
     rng = np.random.default_rng(seed)
     x = rng.uniform(0, 1, n_total)

@@ -5,7 +5,7 @@ matvec on 1M points (500k uniform on the free surface [-1/2,1/2]^2 x {0}, 500k
 on a 0.5 x 0.5 fault dipping 45 degrees centred at depth 0.3), 3-component N(0,1)
 densities, leaf 64, order 6, MAC 0.3, d 0.05.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2|cfg1|cfg3|cfg5] [--n N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config fault_surface|icosphere|cube_clusters|rough_fault] [--n N]
 
 One JSON line on stdout. `value` is FP64 Mpts/s = N_obs / apply time with inputs
 resident in HBM (device pointers). Beside it: the mixed FP32 path, relative L2
@@ -42,7 +42,7 @@ PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 # --pmc FETCH_SIZE and WRITE_SIZE of the same command, scaled by what the counters
 # read for a plain device copy (profiles/k_leaf_stage_elastic_u_1M_order6.md).
 # Other workloads have no measured figure and report null.
-LEAF_TRAFFIC_BYTES = {("cfg2", 1_000_000): 639_369_793}
+LEAF_TRAFFIC_BYTES = {("fault_surface", 1_000_000): 639_369_793}
 
 
 def log(*a):
@@ -51,26 +51,26 @@ def log(*a):
 
 def workload(name, n):
     wl = importlib.import_module("3bem_amd.workloads")
-    if name == "cfg1":
-        g = wl.cfg1_icosphere()
+    if name == "icosphere":
+        g = wl.icosphere_surface_15k()
         return dict(kernel=0, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
                     cfg=(0.3, 5, 100, 0.05, False),
                     desc="Laplace single-layer FMM matvec, icosphere 5120 triangles x 3 points = 15,360 "
                          "points, order 5, leaf 100, MAC 0.3")
-    if name == "cfg2":
-        g = wl.cfg2_fault_and_surface(n or 1_000_000)
+    if name == "fault_surface":
+        g = wl.fault_and_free_surface(n or 1_000_000)
         return dict(kernel=3, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
                     cfg=(0.3, 6, 64, 0.05, False),
                     desc=f"elastic displacement (Kelvin) FMM matvec, {len(g['locs'])} points (half free "
                          "surface, half 45-degree fault), mu=1 nu=0.25, order 6, leaf 64, MAC 0.3")
-    if name == "cfg3":
-        g = wl.cfg3_cube_clusters(n or 4_000_000)
+    if name == "cube_clusters":
+        g = wl.cube_and_clusters(n or 4_000_000)
         return dict(kernel=0, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
                     cfg=(0.3, 75, 100, 0.05, False),
                     desc=f"Laplace single-layer FMM matvec, {len(g['locs'])} points (cube + 8 clusters), "
                          "order 75, leaf 100, MAC 0.3")
-    if name == "cfg5":
-        g = wl.cfg5_rough_fault(n or 2_000_000)
+    if name == "rough_fault":
+        g = wl.rough_fault_points(n or 2_000_000)
         x = np.random.default_rng(wl.SEED + 1).standard_normal(3 * len(g["locs"]))
         return dict(kernel=6, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=x,
                     cfg=(0.3, 6, 64, 0.05, False),
@@ -126,7 +126,7 @@ def cpu_baseline(kernel, cfg, budget_n):
     wl = importlib.import_module("3bem_amd.workloads")
     kind = "reference" if available("ref") else "port"
     o = Oracle("ref" if kind == "reference" else "orc")
-    g = wl.cfg2_fault_and_surface(budget_n) if kernel == 3 else wl.uniform_cube(budget_n, dim=3 if kernel >= 3 else 1)
+    g = wl.fault_and_free_surface(budget_n) if kernel == 3 else wl.uniform_cube(budget_n, dim=3 if kernel >= 3 else 1)
     t0 = time.perf_counter()
     op = o.fmm(kernel, g["locs"], g["normals"], g["locs"], g["normals"], g["weights"], *cfg)
     t_plan = time.perf_counter() - t0
@@ -206,7 +206,7 @@ def target_run(tbem, torch, steps):
     FMM matvec on 2M points of the fault + free-surface geometry, order 100, leaf 256,
     MAC 0.3 (FP64 FMM ~4e-7 from the direct sum), errors on a 5k-target subsample."""
     wl = importlib.import_module("3bem_amd.workloads")
-    return compare_run(tbem, torch, steps, wl.cfg2_fault_and_surface(2_000_000), 3, (0.3, 100, 256, 0.05, False),
+    return compare_run(tbem, torch, steps, wl.fault_and_free_surface(2_000_000), 3, (0.3, 100, 256, 0.05, False),
                        5000, "elastic displacement FMM matvec, 2,000,000 points (fault + free surface), order 100, "
                              "leaf 256, MAC 0.3")
 
@@ -216,12 +216,12 @@ def laplace_run(tbem, torch, steps):
     (half uniform in the unit cube, half in 8 Gaussian clusters, sigma 0.02), order
     75, leaf 100, MAC 0.3, errors on a 10k-target subsample."""
     wl = importlib.import_module("3bem_amd.workloads")
-    return compare_run(tbem, torch, steps, wl.cfg3_cube_clusters(4_000_000), 0, (0.3, 75, 100, 0.05, False),
+    return compare_run(tbem, torch, steps, wl.cube_and_clusters(4_000_000), 0, (0.3, 75, 100, 0.05, False),
                        10_000, "Laplace single-layer FMM matvec, 4,000,000 points (cube + 8 clusters), order 75, "
                                "leaf 100, MAC 0.3")
 
 
-def cfg4_run(tbem, torch, steps):
+def forward_model_run(tbem, torch, steps):
     """The differentiable elastic forward model: 40k fault triangles (120k points)
     to 200k surface stations through the elastic traction (double-layer) kernel
     with the fault normals, loss 0.5 ||u - u_obs||^2, order 150, leaf 256, MAC 0.3.
@@ -230,7 +230,7 @@ def cfg4_run(tbem, torch, steps):
     differences of the direct sum, step 1e-6)."""
     wl = importlib.import_module("3bem_amd.workloads")
     ag = importlib.import_module("3bem_amd.autograd")
-    g = wl.cfg4_fault_and_stations()
+    g = wl.fault_and_stations()
     obs, src = g["obs"], g["src"]
     n = len(obs)
     K = tbem.ElasticTraction(1.0, 0.25)
@@ -269,14 +269,14 @@ def cfg4_run(tbem, torch, steps):
                      "bwd_fwd_ratio": round(dtb / dt, 3),
                      "rel_err_dL_dslip_vs_direct": float(np.linalg.norm(gs - gs_ref) / np.linalg.norm(gs_ref)),
                      "rel_err_dL_dobs_vs_direct": float(np.linalg.norm(go - go_ref) / np.linalg.norm(go_ref))}
-        log(f"[bench] cfg4 {prec}: fwd {dt * 1e3:.2f} ms, bwd {dtb * 1e3:.2f} ms, grad errors "
+        log(f"[bench] forward model {prec}: fwd {dt * 1e3:.2f} ms, bwd {dtb * 1e3:.2f} ms, grad errors "
             f"{out[prec]['rel_err_dL_dslip_vs_direct']:.2e} / {out[prec]['rel_err_dL_dobs_vs_direct']:.2e}")
         del op
         torch.cuda.empty_cache()
     return out
 
 
-def cfg5_run(tbem, torch):
+def bem_solve_run(tbem, torch):
     """The BEM solve: elastic hypersingular FMM on the rough fault (2M points, order
     50, leaf 128, MAC 0.3) as the second-kind system (alpha I + H) x = b, alpha =
     2 ||H|| from 8 power iterations, b the traction of a uniform unit remote stress.
@@ -286,7 +286,7 @@ def cfg5_run(tbem, torch):
     wl = importlib.import_module("3bem_amd.workloads")
     gm = importlib.import_module("3bem_amd.gmres")
     n = 2_000_000
-    g = wl.cfg5_rough_fault(n)
+    g = wl.rough_fault_points(n)
     K = tbem.ElasticHypersingular(1.0, 0.25)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
     cfg = tbem.FMMConfig(0.3, 50, 128, 0.05, False)
@@ -321,7 +321,7 @@ def cfg5_run(tbem, torch):
                                "residual": float(torch.linalg.norm(b - A64(x))) / bn,
                                "rel_diff_vs_fp64_solution": float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))}
     out["speedup_refinement_vs_fp64_gmres"] = round(t64 / tr, 3)
-    log(f"[bench] cfg5: fp64 GMRES {t64:.2f} s ({i64['iterations']} it), refinement {tr:.2f} s "
+    log(f"[bench] BEM solve: fp64 GMRES {t64:.2f} s ({i64['iterations']} it), refinement {tr:.2f} s "
         f"({info['outer']} outer, {info['inner_iterations']} inner)")
     del op64, op32
     torch.cuda.empty_cache()
@@ -433,7 +433,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--config", default="fault_surface")
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--cpu-n", type=int, default=100_000)
     ap.add_argument("--skip-cpu", action="store_true")
@@ -442,9 +442,9 @@ def main():
                     help="skip the 2M-point order-100 FP32/MFMA vs FP64 comparison")
     ap.add_argument("--skip-laplace", action="store_true",
                     help="skip the 4M-point Laplace FP32/MFMA-vs-FP64 comparison")
-    ap.add_argument("--skip-cfg4", action="store_true",
+    ap.add_argument("--skip-forward-model", action="store_true",
                     help="skip the differentiable forward model (120k fault points, 200k stations)")
-    ap.add_argument("--skip-cfg5", action="store_true",
+    ap.add_argument("--skip-bem-solve", action="store_true",
                     help="skip the 2M-point hypersingular GMRES solve")
     ap.add_argument("--skip-near", action="store_true",
                     help="skip the CSR near field / integral operator section")
@@ -454,9 +454,9 @@ def main():
                     help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.only_near:
-        args.skip_target = args.skip_laplace = args.skip_cfg4 = args.skip_cfg5 = True
+        args.skip_target = args.skip_laplace = args.skip_forward_model = args.skip_bem_solve = True
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the FMM "
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); the FMM "
               "sharding over several GPUs is not implemented", file=sys.stderr)
         raise SystemExit(2)
 
@@ -550,15 +550,15 @@ def main():
         log(f"[bench] rel err fp64 {e64:.3e}, mixed {e32:.3e}; mixed {dt32 * 1e3:.2f} ms; "
             f"adjoint {dt_adj * 1e3:.2f} ms, obs_grad {dt_grad * 1e3:.2f} ms")
 
-    if not args.skip_extra and not args.skip_target and args.config == "cfg2":
+    if not args.skip_extra and not args.skip_target and args.config == "fault_surface":
         out["target_fp32_mfma_vs_fp64"] = target_run(tbem, torch, max(3, args.steps // 2))
-    if not args.skip_extra and not args.skip_laplace and args.config == "cfg2":
+    if not args.skip_extra and not args.skip_laplace and args.config == "fault_surface":
         out["laplace_4M_fp32_mfma_vs_fp64"] = laplace_run(tbem, torch, 3)
-    if not args.skip_extra and not args.skip_cfg4 and args.config == "cfg2":
-        out["forward_model_cfg4"] = cfg4_run(tbem, torch, 5)
-    if not args.skip_extra and not args.skip_cfg5 and args.config == "cfg2":
-        out["bem_solve_cfg5"] = cfg5_run(tbem, torch)
-    if not args.skip_extra and not args.skip_near and args.config == "cfg2":
+    if not args.skip_extra and not args.skip_forward_model and args.config == "fault_surface":
+        out["forward_model_fault_stations"] = forward_model_run(tbem, torch, 5)
+    if not args.skip_extra and not args.skip_bem_solve and args.config == "fault_surface":
+        out["bem_solve_rough_fault"] = bem_solve_run(tbem, torch)
+    if not args.skip_extra and not args.skip_near and args.config == "fault_surface":
         out["integral_operator_near_field"] = near_field_run(tbem, torch, args.steps, args.warmup, args.skip_cpu)
 
     if not args.skip_cpu:

@@ -53,10 +53,10 @@ def fmm_case(kernel, obs, on, src, sn, w, x, mac, order, leaf, d, small, name, w
 def main():
     shutil.copy("/root/reference/unit_tests/test_data_elastic3", os.path.join(HERE, "test_data_elastic3"))
 
-    # config 1: Laplace single layer on the icosphere, order 5, leaf 100, MAC 0.3
-    g = wl.cfg1_icosphere()
+    # Laplace single layer on the 15,360-point icosphere, order 5, leaf 100, MAC 0.3
+    g = wl.icosphere_surface_15k()
     fmm_case(0, g["locs"], g["normals"], g["locs"], g["normals"], g["weights"], g["x"],
-             0.3, 5, 100, 0.05, False, "cfg1_laplace_icosphere")
+             0.3, 5, 100, 0.05, False, "icosphere_laplace_15k")
 
     # every kernel on a small uniform cloud, obs != src, with task lists and C2E
     rng = np.random.default_rng(7)
@@ -77,9 +77,9 @@ def main():
     fmm_case(3, src, sn, src, sn, w, x, 0.3, 6, 20, 0.05, False, "self_elastic_u", with_lists=True)
 
     # config-2-like geometry (fault + free surface), 20k points, elastic U, order 6, leaf 64
-    c2 = wl.cfg2_fault_and_surface(20000)
+    c2 = wl.fault_and_free_surface(20000)
     fmm_case(3, c2["locs"], c2["normals"], c2["locs"], c2["normals"], c2["weights"], c2["x"],
-             0.3, 6, 64, 0.05, False, "cfg2_small_elastic_u")
+             0.3, 6, 64, 0.05, False, "fault_surface_small_elastic_u")
 
     # higher order Laplace (order 35 / 75), reference unit test tolerance setting
     for order, k in ((35, 0), (75, 1)):

@@ -64,7 +64,7 @@ def test_refinement_recovers_fp64_accuracy_from_an_inexact_inner_operator():
 def test_refinement_with_oracle_fmm_as_the_fp64_operator(orc, wl):
     """The FP64 operator is the oracle FMM (CPU restatement of the reference): a
     second-kind system alpha x + A x solved with refinement around a perturbed copy."""
-    g = wl.cfg2_fault_and_surface(3000)
+    g = wl.fault_and_free_surface(3000)
     op = orc.fmm(0, g["locs"], g["normals"], g["locs"], g["normals"], g["weights"], 0.3, 5, 50, 0.05)
     n = len(g["locs"])
     apply64 = lambda v: torch.tensor(op.apply(v.numpy()))

@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 def make_problem(tbem, wl, K, order=20):
     torch = pytest.importorskip("torch")
-    g = wl.cfg4_fault_and_stations(n_tri=400, n_obs=1500)
+    g = wl.fault_and_stations(n_tri=400, n_obs=1500)
     data = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["src_weights"])
     op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, order, 32, 0.05, False))
     ref_u = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, order, 32, 0.05, False)).apply(g["x_true"])

@@ -91,8 +91,8 @@ def test_gpu_direct_vs_oracle(tbem, orc, kid):
 
 
 # -------------------------------------------------------------------- FMM parity
-@pytest.mark.parametrize("name", golden_names("small_") + ["self_elastic_u", "cfg1_laplace_icosphere",
-                                                           "cfg2_small_elastic_u"])
+@pytest.mark.parametrize("name", golden_names("small_") + ["self_elastic_u", "icosphere_laplace_15k",
+                                                           "fault_surface_small_elastic_u"])
 def test_gpu_fmm_fp64_matches_reference(tbem, name):
     g = golden(name)
     case("fmm fp64", name)
@@ -197,7 +197,7 @@ def test_gpu_mixed_adjoint_with_mfma_transpose(tbem, wl, kid, order):
     agrees with the FP64 adjoint to the FP32 bound, and is the transpose of the
     mixed forward to that bound. Elastic order 100 (blk 297) runs the eight-wave
     MFMA kernel; the other two the four-wave one."""
-    g = wl.cfg2_fault_and_surface(30000)
+    g = wl.fault_and_free_surface(30000)
     K = tbem.make_kernel(kid, 1.0, 0.25)
     R = components(kid)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
@@ -223,7 +223,7 @@ def test_gpu_mixed_obs_grad_matches_fp64(tbem, wl, kid):
     """Mixed precision observation gradient (MFMA far field; FP32 point sources for
     the kernels with closed-form gradients, FP64 dual numbers otherwise) against the
     FP64 one, which the finite-difference tests pin."""
-    g = wl.cfg2_fault_and_surface(20000)
+    g = wl.fault_and_free_surface(20000)
     K = tbem.make_kernel(kid, 1.0, 0.25)
     R = components(kid)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
@@ -245,7 +245,7 @@ def test_gpu_obs_grad_reuses_forward_far_field(tbem, wl, prec):
     stale by another forward falls back to recomputing for the x passed."""
     torch = pytest.importorskip("torch")
     n = 20000
-    g = wl.cfg2_fault_and_surface(n)
+    g = wl.fault_and_free_surface(n)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
     op = tbem.FMMOperator(tbem.ElasticDisplacement(1.0, 0.25), data, tbem.FMMConfig(0.3, 50, 100, 0.05, False),
                           precision=prec)
@@ -296,7 +296,7 @@ def test_gpu_obs_grad_matches_finite_differences(tbem, kid):
 
 
 # ------------------------------------------------------------------- mixed path
-@pytest.mark.parametrize("name", ["cfg1_laplace_icosphere", "cfg2_small_elastic_u", "small_elastic_h_nosc",
+@pytest.mark.parametrize("name", ["icosphere_laplace_15k", "fault_surface_small_elastic_u", "small_elastic_h_nosc",
                                   "small_elastic_t_sc"])
 def test_gpu_fmm_mixed_close_to_fp64(tbem, name):
     g = golden(name)
@@ -312,7 +312,7 @@ def test_gpu_mfma_m2l_matches_fp64(tbem, wl, kid, order, leaf):
     """Mixed precision with the M2L on FP32 MFMA (one octree for obs and src):
     the operators are in use (translation classes built) and the matvec agrees
     with the plain FP64 FMM to the FP32 bound."""
-    g = wl.cfg2_fault_and_surface(30000)
+    g = wl.fault_and_free_surface(30000)
     K = tbem.make_kernel(kid, 1.0, 0.25)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
     x = g["x"] if kid >= 3 else g["x"][:30000]
@@ -328,7 +328,7 @@ def test_gpu_mfma_m2l_matches_fp64(tbem, wl, kid, order, leaf):
 
 def test_gpu_mfma_m2l_off_for_small_operators(tbem, wl):
     """Order 6 elastic (18 x 18 operators): the FP64 M2L stage is used."""
-    g = wl.cfg2_fault_and_surface(20000)
+    g = wl.fault_and_free_surface(20000)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
     op = tbem.FMMOperator(tbem.ElasticDisplacement(1.0, 0.25), data, tbem.FMMConfig(0.3, 6, 64, 0.05, False),
                           precision="mixed")

@@ -40,7 +40,7 @@ CASES = {"traction": (4, 150, 256, (2e-5, 2e-5, 1e-4)), "displacement": (3, 75, 
 def model(request, tbem, wl):
     torch = pytest.importorskip("torch")
     kid, order, leaf, bounds = CASES[request.param]
-    g = wl.cfg4_fault_and_stations()
+    g = wl.fault_and_stations()
     obs, src = g["obs"], g["src"]
     assert len(src) == 120_000 and len(obs) == 200_000
     K = tbem.make_kernel(kid, 1.0, 0.25)
@@ -80,9 +80,9 @@ def grads(p, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_cfg4_gradients_match_fp64_direct_sum(model, prec):
+def test_forward_model_gradients_match_fp64_direct_sum(model, prec):
     p = model
-    print(f"CASE cfg4 full size vs direct sum, {p['name']}, {prec}", flush=True)
+    print(f"CASE forward model full size vs direct sum, {p['name']}, {prec}", flush=True)
     gs, go = grads(p, prec)
     e_u = rel_err(p["ops"][prec].apply(p["g"]["x"]), p["u_dir"])
     e_s, e_o = rel_err(gs, p["gs_ref"]), rel_err(go, p["go_ref"])
@@ -93,9 +93,9 @@ def test_cfg4_gradients_match_fp64_direct_sum(model, prec):
     assert e_o < b_o
 
 
-def test_cfg4_mixed_gradients_match_fp64_operator(model):
+def test_forward_model_mixed_gradients_match_fp64_operator(model):
     p = model
-    print(f"CASE cfg4 mixed vs fp64 gradients, {p['name']}", flush=True)
+    print(f"CASE forward model mixed vs fp64 gradients, {p['name']}", flush=True)
     gs64, go64 = grads(p, "fp64")
     gs32, go32 = grads(p, "mixed")
     e_s, e_o = rel_err(gs32, gs64), rel_err(go32, go64)
@@ -104,13 +104,13 @@ def test_cfg4_mixed_gradients_match_fp64_operator(model):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_cfg4_gradients_match_finite_differences(model, prec):
+def test_forward_model_gradients_match_finite_differences(model, prec):
     p = model
     torch = p["torch"]
     ag = importlib.import_module("3bem_amd.autograd")
     op = p["ops"][prec]
     obs, slip0 = p["obs"], p["g"]["x"]
-    print(f"CASE cfg4 finite differences, {p['name']}, {prec}", flush=True)
+    print(f"CASE forward model finite differences, {p['name']}, {prec}", flush=True)
     gs, go = grads(p, prec)
 
     def loss(slip, o):

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def system(tbem, wl):
     torch = pytest.importorskip("torch")
-    g = wl.cfg5_rough_fault(20000)
+    g = wl.rough_fault_points(20000)
     n = len(g["locs"])
     K = tbem.ElasticHypersingular(1.0, 0.25)
     data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])

@@ -73,14 +73,14 @@ def test_host_config_mirrors_reference(tbem):
 
 
 def test_workloads_shapes(wl):
-    g = wl.cfg1_icosphere()
+    g = wl.icosphere_surface_15k()
     assert g["locs"].shape == (15360, 3)
     assert np.allclose(np.linalg.norm(g["normals"], axis=1), 1)
     assert np.all(np.sum(g["normals"] * g["locs"], axis=1) > 0)  # outward
     assert abs(g["weights"].sum() - 4 * np.pi) < 0.02  # polyhedral area below the sphere's
-    c2 = wl.cfg2_fault_and_surface(1000)
+    c2 = wl.fault_and_free_surface(1000)
     assert c2["locs"].shape == (1000, 3) and c2["x"].shape == (3000,)
-    c4 = wl.cfg4_fault_and_stations(n_tri=400, n_obs=500)
+    c4 = wl.fault_and_stations(n_tri=400, n_obs=500)
     assert c4["src"].shape == (1200, 3) and c4["obs"].shape == (500, 3)
 
 

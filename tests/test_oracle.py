@@ -181,8 +181,8 @@ def test_octree_invariants(orc):
 
 
 # ---------------------------------------------------------------------------- FMM
-@pytest.mark.parametrize("name", golden_names("small_") + ["self_elastic_u", "cfg1_laplace_icosphere",
-                                                           "cfg2_small_elastic_u", "highorder_laplace_single_35",
+@pytest.mark.parametrize("name", golden_names("small_") + ["self_elastic_u", "icosphere_laplace_15k",
+                                                           "fault_surface_small_elastic_u", "highorder_laplace_single_35",
                                                            "highorder_laplace_double_75"])
 def test_oracle_fmm_matches_reference_golden(orc, name):
     """The restated FMM reproduces the compiled reference's output (single thread)."""
