@@ -231,3 +231,17 @@ def test_sparse_and_integral_empty_shapes(tbem):
     r = S(3, 3, [], [], [0, 0, 0, 0])
     assert r.apply(np.ones(3)).tolist() == [0.0] * 3 and r.nnz() == 0
     assert S.csr_from_coo(0, 0, [], [], []).right_multiply(S.csr_from_coo(0, 2, [], [], [])).n_cols() == 2
+
+
+def test_integral_operator_mixed_far_field(tbem, orc):
+    """The integral operator with the mixed-precision FMM as its far field agrees with
+    the FP64 one to the mixed path's accuracy (the sparse parts are exact in both)."""
+    rng = np.random.default_rng(60)
+    bem, obs_f, src_f, q, K, n_rows, n_cols, parts = bem_problem(tbem, orc, rng, 3)
+    print("CASE integral operator, mixed far field", flush=True)
+    cfg = tbem.FMMConfig(0.3, 20, 10, 0.05, False)
+    x = rng.standard_normal(n_cols)
+    y64 = bem.integral_operator(parts["near"][3], parts["corr"][3], obs_f, src_f, q, q, K, cfg).apply(x)
+    y32 = bem.integral_operator(parts["near"][3], parts["corr"][3], obs_f, src_f, q, q, K, cfg,
+                                precision="mixed").apply(x)
+    assert rel_err(y32, y64) < 1e-5
