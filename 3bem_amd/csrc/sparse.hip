@@ -295,6 +295,8 @@ int tbem_sparse_create(size_t n_rows, size_t n_cols, size_t nnz, const double* v
         TBEM_REQUIRE(out && row_ptrs && (nnz == 0 || (values && column_indices)), TBEM_E_INVALID,
                      "null argument");
         *out = nullptr;
+        TBEM_REQUIRE(n_rows < (1u << 31) && n_cols < (1u << 31), TBEM_E_UNSUPPORTED,
+                     "n_rows and n_cols must be below 2^31");
         std::vector<int> cols;
         check_cols(column_indices, nnz, n_cols, cols);
         std::vector<long> rp(n_rows + 1);
@@ -313,7 +315,8 @@ int tbem_sparse_from_coo(size_t n_rows, size_t n_cols, size_t n_entries, const s
     return sp_guard([&] {
         TBEM_REQUIRE(out && (n_entries == 0 || (rows && cols && values)), TBEM_E_INVALID, "null argument");
         *out = nullptr;
-        TBEM_REQUIRE(n_rows < (1u << 31), TBEM_E_UNSUPPORTED, "n_rows must be below 2^31");
+        TBEM_REQUIRE(n_rows < (1u << 31) && n_cols < (1u << 31), TBEM_E_UNSUPPORTED,
+                     "n_rows and n_cols must be below 2^31");
         std::vector<long> rp(n_rows + 1, 0);
         for (size_t i = 0; i < n_entries; i++) {
             TBEM_REQUIRE(rows[i] < n_rows, TBEM_E_INVALID, "row index out of range");

@@ -102,7 +102,8 @@ def test_bem_geometry_closed_forms():
 
 
 def test_sparse_abi_rejects_bad_csr_before_touching_the_device(tbem):
-    """Bad CSR input returns TBEM_E_INVALID from the host checks (no device needed)."""
+    """Bad CSR input returns TBEM_E_INVALID, and shapes past 32-bit indexing
+    TBEM_E_UNSUPPORTED, from the host checks (no device needed)."""
     lib = tbem.load()
     SZ = ctypes.c_size_t
     h = ctypes.c_void_p()
@@ -127,3 +128,10 @@ def test_sparse_abi_rejects_bad_csr_before_touching_the_device(tbem):
                                   cols.ctypes.data_as(ctypes.POINTER(SZ)),
                                   vals.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(h))
     assert rc == 1 and b"row index" in lib.tbem_last_error()
+    # 32-bit column indices and row counts on the device: larger shapes are refused
+    # before row_ptrs is read past its first entries.
+    assert create(1 << 31, 2, [], [], [0]) == 4
+    assert create(2, 1 << 31, [], [], [0, 0, 0]) == 4
+    assert b"2^31" in lib.tbem_last_error()
+    rc = lib.tbem_sparse_from_coo(2, 1 << 31, 0, None, None, None, ctypes.byref(h))
+    assert rc == 4 and b"2^31" in lib.tbem_last_error()
