@@ -282,22 +282,83 @@ __device__ inline void stream_surfaces(Tile<double, R, NRM, NB>& tl, int e0, int
 }
 
 // ---------------------------------------------------------------------- pair loop
-// One source j of the tile on one target (FP64 coordinates tx, ty, tz): the
-// displacement is formed in FP64, then the kernel runs in T.
-template <int KID, bool ADJ, bool FAST = false, class T, class VT, int R, bool NRM, int NB>
-__device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, double tx, double ty, double tz,
-                                     const T* tn, T* acc, const KC<T>& kc) {
-    const T sn[3] = {NRM ? T(tl.nx[j]) : T(0), NRM ? T(tl.ny[j]) : T(0), NRM ? T(tl.nz[j]) : T(0)};
+// One source of a tile read into registers: FP64 position, normal and density in
+// the tile's value type.
+template <class VT, int R, bool NRM>
+struct SrcVal {
+    double x, y, z;
+    VT n[3], d[R];
+};
+
+template <class VT, int R, bool NRM, int NB>
+__device__ __forceinline__ SrcVal<VT, R, NRM> read_src(const Tile<VT, R, NRM, NB>& tl, int j) {
+    SrcVal<VT, R, NRM> s;
+    s.x = tl.x[j];
+    s.y = tl.y[j];
+    s.z = tl.z[j];
+    s.n[0] = NRM ? tl.nx[j] : VT(0);
+    s.n[1] = NRM ? tl.ny[j] : VT(0);
+    s.n[2] = NRM ? tl.nz[j] : VT(0);
+    for (int q = 0; q < R; q++) s.d[q] = tl.d[q][j];
+    return s;
+}
+
+// One source on one target (FP64 coordinates tx, ty, tz): the displacement is
+// formed in FP64, then the kernel runs in T.
+template <int KID, bool ADJ, bool FAST = false, class T, class VT, int R, bool NRM>
+__device__ __forceinline__ void pair_v(const SrcVal<VT, R, NRM>& src, double tx, double ty, double tz,
+                                       const T* tn, T* acc, const KC<T>& kc) {
+    const T sn[3] = {T(src.n[0]), T(src.n[1]), T(src.n[2])};
     T v[R];
-    for (int q = 0; q < R; q++) v[q] = T(tl.d[q][j]);
+    for (int q = 0; q < R; q++) v[q] = T(src.d[q]);
     if (!ADJ) {
-        const T dx = T(tl.x[j] - tx), dy = T(tl.y[j] - ty), dz = T(tl.z[j] - tz);
+        const T dx = T(src.x - tx), dy = T(src.y - ty), dz = T(src.z - tz);
         const T r2 = dx * dx + dy * dy + dz * dz;
         kop<KID, false, T, FAST>(dx, dy, dz, r2, tn, sn, v, acc, kc);
     } else {
-        const T dx = T(tx - tl.x[j]), dy = T(ty - tl.y[j]), dz = T(tz - tl.z[j]);
+        const T dx = T(tx - src.x), dy = T(ty - src.y), dz = T(tz - src.z);
         const T r2 = dx * dx + dy * dy + dz * dz;
         kop<KID, true, T, FAST>(dx, dy, dz, r2, sn, tn, v, acc, kc);
+    }
+}
+
+// One source j of the tile on one target.
+template <int KID, bool ADJ, bool FAST = false, class T, class VT, int R, bool NRM, int NB>
+__device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, double tx, double ty, double tz,
+                                     const T* tn, T* acc, const KC<T>& kc) {
+    pair_v<KID, ADJ, FAST>(read_src(tl, j), tx, ty, tz, tn, acc, kc);
+}
+
+// Sources slice, slice + S, ... < cnt of a tile on two targets (a, b). With
+// PIPE the sources alternate between two register sets: each set is read from
+// LDS while the other is computed (indices clamped to cnt - 1, so every read
+// stays inside the tile), and the scheduling barriers keep the compiler from
+// moving the reads down to their use.
+template <bool PIPE, int KID, bool ADJ, bool FAST, class T, class VT, int R, bool NRM, int NB>
+__device__ __forceinline__ void pair_loop2(const Tile<VT, R, NRM, NB>& tl, int slice, int S, int cnt,
+                                           double ax, double ay, double az, const T* an, T* acA,
+                                           double bx, double by, double bz, const T* bn, T* acB,
+                                           const KC<T>& kc) {
+    if constexpr (PIPE) {
+        if (slice >= cnt) return;
+        SrcVal<VT, R, NRM> c0 = read_src(tl, slice);
+        for (int j = slice; j < cnt; j += 2 * S) {
+            const SrcVal<VT, R, NRM> c1 = read_src(tl, min(j + S, cnt - 1));
+            __builtin_amdgcn_sched_barrier(0);
+            pair_v<KID, ADJ, FAST>(c0, ax, ay, az, an, acA, kc);
+            pair_v<KID, ADJ, FAST>(c0, bx, by, bz, bn, acB, kc);
+            c0 = read_src(tl, min(j + 2 * S, cnt - 1));
+            __builtin_amdgcn_sched_barrier(0);
+            if (j + S < cnt) {
+                pair_v<KID, ADJ, FAST>(c1, ax, ay, az, an, acA, kc);
+                pair_v<KID, ADJ, FAST>(c1, bx, by, bz, bn, acB, kc);
+            }
+        }
+    } else {
+        for (int j = slice; j < cnt; j += S) {
+            pair<KID, ADJ, FAST>(tl, j, ax, ay, az, an, acA, kc);
+            pair<KID, ADJ, FAST>(tl, j, bx, by, bz, bn, acB, kc);
+        }
     }
 }
 
@@ -421,12 +482,9 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
         double sfA[SPLIT ? R : 1], sfB[SPLIT ? R : 1];
         for (int q = 0; q < (SPLIT ? R : 1); q++) sfA[q] = sfB[q] = 0.0;
         auto points = [&](int cnt) {
-            if (va) {
-                for (int j = slice; j < cnt; j += S) {
-                    pair<KID, ADJ, FAST>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
-                    pair<KID, ADJ, FAST>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
-                }
-            }
+            if (va)
+                pair_loop2<leaf_pipe<KID, T>(), KID, ADJ, FAST>(sm.pt, slice, S, cnt, ax, ay, az, an, acA.tile(),
+                                                                bx, by, bz, bn, acB.tile(), kc);
             acA.flush();
             acB.flush();
         };
