@@ -15,8 +15,9 @@
 // one is computed.
 // Lanes split into S = floor(NB/ntp) source slices that share the tile (ntp =
 // target slots; the last NB - S*ntp lanes sit out), summed through LDS at the
-// end; in the leaf stage each lane carries two targets, so one LDS read feeds two
-// pairs. Workgroup width is the main performance lever of the leaf stage: wider
+// end by the whole workgroup (slice_sum: the FP64 elastic leaf stage went 4.41 ->
+// 4.08 ms when the slice-0 lanes alone added up to 42 partials each); in the leaf
+// stage each lane carries two targets, so one LDS read feeds two pairs. Workgroup width is the main performance lever of the leaf stage: wider
 // workgroups give each leaf's targets more source slices, hence more waves in
 // flight per leaf and fewer tile barriers per pair. Measured on the 1M-point
 // elastic workload (order 6, leaf 64), FP64 leaf stage: 6.4 ms at 128 threads,
@@ -60,7 +61,44 @@ struct StageSmem {
         double red[RED * NB];
     };
     int rpre[NB + 1], rbeg[NB];
+    double part[NB];  // group sums of slice_sum
 };
+
+// Sums the S slice partials red[v*NB + k*ntp + tg] (k < S) of nv value rows and
+// ntp target slots with the whole block, and hands each total to emit(v, tg, s).
+// When G = min(S, NB / (nv*ntp)) > 1 groups fit, group g first adds slices
+// g, g+G, ... into part, then one thread per (v, tg) adds the G group sums:
+// chains of about S/G + G additions instead of S. The caller has written red and
+// synchronized; every thread of the block must call this.
+template <int NB, class F>
+__device__ __forceinline__ void slice_sum(const double* red, double* part, int nv, int ntp, int S, F&& emit) {
+    const int tid = threadIdx.x;
+    const int nout = nv * ntp;
+    const int G = min(S, NB / nout);
+    if (G <= 1) {
+        for (int o = tid; o < nout; o += NB) {
+            const int v = o / ntp, tg = o - v * ntp;
+            double s = 0;
+            for (int k = 0; k < S; k++) s += red[v * NB + k * ntp + tg];
+            emit(v, tg, s);
+        }
+        return;
+    }
+    if (tid < nout * G) {
+        const int g = tid / nout, o = tid - g * nout;
+        const int v = o / ntp, tg = o - v * ntp;
+        double s = 0;
+        for (int k = g; k < S; k += G) s += red[v * NB + k * ntp + tg];
+        part[tid] = s;
+    }
+    __syncthreads();
+    if (tid < nout) {
+        double s = 0;
+        for (int g = 0; g < G; g++) s += part[g * nout + tid];
+        const int v = tid / ntp;
+        emit(v, tid - v * ntp, s);
+    }
+}
 
 // one staged source held in registers (FP64, absolute) until it is written to LDS
 template <int R, bool NRM>
@@ -421,18 +459,11 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
             if constexpr (dual) sm.red[(R + q) * NB + tid] = active ? acc1[q] : 0.0;
         }
         __syncthreads();
-        if (slice == 0) {
-            for (int q = 0; q < R; q++) {
-                double s0 = 0;
-                for (int k = 0; k < S; k++) s0 += sm.red[q * NB + k * ntp + tg];
-                field[q * a.n_e + i0] = s0 * scale;
-                if constexpr (dual) {
-                    double s1 = 0;
-                    for (int k = 0; k < S; k++) s1 += sm.red[(R + q) * NB + k * ntp + tg];
-                    if (two) field[q * a.n_e + i1] = s1 * scale;
-                }
-            }
-        }
+        slice_sum<NB>(sm.red, sm.part, (dual ? 2 : 1) * R, ntp, S, [&](int v, int g, double s) {
+            const int h = v / R, q = v - h * R;
+            if (h == 1 && ntp + g >= nt) return;
+            field[q * a.n_e + t0 + h * ntp + g] = s * scale;
+        });
         __syncthreads();
     }
     double* o = a.out + (size_t)cell * a.blk;
@@ -518,18 +549,13 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
             sm.red[(R + q) * NB + tid] = vb ? acB.value(q) + eb : 0.0;
         }
         __syncthreads();
-        if (slice == 0) {
-            for (int h = 0; h < 2; h++) {
-                if (h == 1 && nslot + tg >= nt) continue;
-                const int i = b + t0 + h * nslot + tg;
-                const double w = a.t_weight ? a.t_weight[i] : 1.0;
-                for (int q = 0; q < R; q++) {
-                    double s = 0;
-                    for (int k = 0; k < S; k++) s += sm.red[(h * R + q) * NB + k * ntp + tg];
-                    a.out_sorted[(size_t)q * a.n_tgt + i] = w * (s * scale);
-                }
-            }
-        }
+        slice_sum<NB>(sm.red, sm.part, 2 * R, ntp, S, [&](int v, int g, double s) {
+            const int h = v / R, q = v - h * R;
+            if (h == 1 && nslot + g >= nt) return;
+            const int i = b + t0 + h * nslot + g;
+            const double w = a.t_weight ? a.t_weight[i] : 1.0;
+            a.out_sorted[(size_t)q * a.n_tgt + i] = w * (s * scale);
+        });
         __syncthreads();
     }
 }
@@ -598,13 +624,9 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
         __syncthreads();
         for (int mm = 0; mm < 3; mm++) sm.red[mm * NB + tid] = gr[mm];
         __syncthreads();
-        if (slice == 0) {
-            for (int mm = 0; mm < 3; mm++) {
-                double sum = 0;
-                for (int k = 0; k < S; k++) sum += sm.red[mm * NB + k * ntp + tg];
-                a.grad_out[3 * (size_t)i + mm] = sum * scale;
-            }
-        }
+        slice_sum<NB>(sm.red, sm.part, 3, ntp, S, [&](int mm, int g, double s) {
+            a.grad_out[3 * (size_t)(b + t0 + g) + mm] = s * scale;
+        });
         __syncthreads();
     }
 }
@@ -706,17 +728,11 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
             sm.red[(3 + mm) * NB + tid] = vb ? GB[mm] + pb : 0.0;
         }
         __syncthreads();
-        if (slice == 0) {
-            for (int h = 0; h < 2; h++) {
-                if (h == 1 && nslot + tg >= nt) continue;
-                const int i = b + t0 + h * nslot + tg;
-                for (int mm = 0; mm < 3; mm++) {
-                    double sum = 0;
-                    for (int k = 0; k < S; k++) sum += sm.red[(h * 3 + mm) * NB + k * ntp + tg];
-                    a.grad_out[3 * (size_t)i + mm] = sum * scale;
-                }
-            }
-        }
+        slice_sum<NB>(sm.red, sm.part, 6, ntp, S, [&](int v, int g, double s) {
+            const int h = v / 3, mm = v - h * 3;
+            if (h == 1 && nslot + g >= nt) return;
+            a.grad_out[3 * (size_t)(b + t0 + h * nslot + g) + mm] = s * scale;
+        });
         __syncthreads();
     }
 }
