@@ -34,17 +34,6 @@ template <int KID, class T>
 constexpr int leaf_waves() {
     return leaf_narrow<KID, T>() ? 3 : ((KID == 4 || KID == 5) ? 4 : 1);
 }
-// Point sources of the leaf stage read one iteration ahead (pair_loop2) for the
-// FP32 Laplace single layer, whose few instructions per pair leave the LDS
-// latency exposed: mixed path, 2M points, order 75, leaf 100: 15.6 -> 14.3 ms.
-// The others read in place: FP64 Laplace single layer 18.8 -> 19.5 ms (same
-// workload), FP64 Laplace double layer (1M, order 6) 8.50 -> 8.66 ms, FP64
-// elastic U (1M, order 6) 4.29 -> 4.33 ms, FP32 elastic U (2M, order 100)
-// 12.8 -> 13.8 ms.
-template <int KID, class T>
-constexpr bool leaf_pipe() {
-    return KID == 0 && std::is_same<T, float>::value;
-}
 constexpr int kGradBlock = 256;  // observation-coordinate gradient of the point-target stage
 
 struct TreeView {

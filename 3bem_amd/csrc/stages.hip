@@ -367,36 +367,15 @@ __device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, doub
     pair_v<KID, ADJ, FAST>(read_src(tl, j), tx, ty, tz, tn, acc, kc);
 }
 
-// Sources slice, slice + S, ... < cnt of a tile on two targets (a, b). With
-// PIPE the sources alternate between two register sets: each set is read from
-// LDS while the other is computed (indices clamped to cnt - 1, so every read
-// stays inside the tile), and the scheduling barriers keep the compiler from
-// moving the reads down to their use.
-template <bool PIPE, int KID, bool ADJ, bool FAST, class T, class VT, int R, bool NRM, int NB>
+// Sources slice, slice + S, ... < cnt of a tile on two targets (a, b).
+template <int KID, bool ADJ, bool FAST, class T, class VT, int R, bool NRM, int NB>
 __device__ __forceinline__ void pair_loop2(const Tile<VT, R, NRM, NB>& tl, int slice, int S, int cnt,
                                            double ax, double ay, double az, const T* an, T* acA,
                                            double bx, double by, double bz, const T* bn, T* acB,
                                            const KC<T>& kc) {
-    if constexpr (PIPE) {
-        if (slice >= cnt) return;
-        SrcVal<VT, R, NRM> c0 = read_src(tl, slice);
-        for (int j = slice; j < cnt; j += 2 * S) {
-            const SrcVal<VT, R, NRM> c1 = read_src(tl, min(j + S, cnt - 1));
-            __builtin_amdgcn_sched_barrier(0);
-            pair_v<KID, ADJ, FAST>(c0, ax, ay, az, an, acA, kc);
-            pair_v<KID, ADJ, FAST>(c0, bx, by, bz, bn, acB, kc);
-            c0 = read_src(tl, min(j + 2 * S, cnt - 1));
-            __builtin_amdgcn_sched_barrier(0);
-            if (j + S < cnt) {
-                pair_v<KID, ADJ, FAST>(c1, ax, ay, az, an, acA, kc);
-                pair_v<KID, ADJ, FAST>(c1, bx, by, bz, bn, acB, kc);
-            }
-        }
-    } else {
-        for (int j = slice; j < cnt; j += S) {
-            pair<KID, ADJ, FAST>(tl, j, ax, ay, az, an, acA, kc);
-            pair<KID, ADJ, FAST>(tl, j, bx, by, bz, bn, acB, kc);
-        }
+    for (int j = slice; j < cnt; j += S) {
+        pair<KID, ADJ, FAST>(tl, j, ax, ay, az, an, acA, kc);
+        pair<KID, ADJ, FAST>(tl, j, bx, by, bz, bn, acB, kc);
     }
 }
 
@@ -514,8 +493,8 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
         for (int q = 0; q < (SPLIT ? R : 1); q++) sfA[q] = sfB[q] = 0.0;
         auto points = [&](int cnt) {
             if (va)
-                pair_loop2<leaf_pipe<KID, T>(), KID, ADJ, FAST>(sm.pt, slice, S, cnt, ax, ay, az, an, acA.tile(),
-                                                                bx, by, bz, bn, acB.tile(), kc);
+                pair_loop2<KID, ADJ, FAST>(sm.pt, slice, S, cnt, ax, ay, az, an, acA.tile(), bx, by, bz, bn,
+                                           acB.tile(), kc);
             acA.flush();
             acB.flush();
         };

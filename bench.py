@@ -42,7 +42,7 @@ PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 # --pmc FETCH_SIZE and WRITE_SIZE of the same command, scaled by what the counters
 # read for a plain device copy (profiles/k_leaf_stage_elastic_u_1M_order6.md).
 # Other workloads have no measured figure and report null.
-LEAF_TRAFFIC_BYTES = {("fault_surface", 1_000_000): 639_369_793}
+LEAF_TRAFFIC_BYTES = {("fault_surface", 1_000_000): 644_165_984}
 
 
 def log(*a):
