@@ -267,20 +267,43 @@ def test_gpu_obs_grad_reuses_forward_far_field(tbem, wl, prec):
     assert torch.equal(stale, fresh)
 
 
+# ------------------------------------------------------------- leaf-stage shapes
+@pytest.mark.parametrize("leaf", [1, 43, 86, 171, 300, 700, 3000])
+@pytest.mark.parametrize("kid", [0, 3])
+def test_gpu_leaf_workgroup_shapes_match_oracle(tbem, orc, kid, leaf):
+    """FP64 FMM against the oracle at leaf sizes that drive every shape of the
+    leaf stage's cross-slice sum: many slices summed in groups (small leaves), one
+    group (43 and 86 with three components), more target-slot sums than the
+    workgroup has threads (171 and up), and, at 3000, one leaf of all 2400 targets
+    that the workgroup covers in three passes."""
+    g = random_set(2400, 2000, 60 + kid)
+    R = components(kid)
+    x = np.random.default_rng(kid + leaf).standard_normal(R * 2000)
+    case("leaf shapes", kid, leaf)
+    d = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
+    y = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, tbem.FMMConfig(0.3, 5, leaf, 0.05, False)).apply(x)
+    ref = orc.fmm(kid, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.3, 5, leaf, 0.05,
+                  False).apply(x)
+    assert rel_err(y, ref) < FP64_TOL
+
+
 # ----------------------------------------------------------------- obs gradient
+@pytest.mark.parametrize("n_obs,leaf", [(300, 10), (1300, 300), (1300, 1500)])
 @pytest.mark.parametrize("kid", [0, 3, 4, 6])
-def test_gpu_obs_grad_matches_finite_differences(tbem, kid):
-    """Frozen-plan central differences of <g, A(obs) x> with h = 1e-6 (FP64)."""
-    g = random_set(300, 400, 40 + kid)
+def test_gpu_obs_grad_matches_finite_differences(tbem, kid, n_obs, leaf):
+    """Frozen-plan central differences of <g, A(obs) x> with h = 1e-6 (FP64). The
+    larger leaves sum more target slots than a gradient workgroup has threads, and
+    at leaf 1500 the 1300 targets form one leaf that takes several passes."""
+    g = random_set(n_obs, 400, 40 + kid)
     R = components(kid)
     rng = np.random.default_rng(kid)
-    x, gg = rng.standard_normal(R * 400), rng.standard_normal(R * 300)
-    case("obs grad", kid)
+    x, gg = rng.standard_normal(R * 400), rng.standard_normal(R * n_obs)
+    case("obs grad", kid, n_obs, leaf)
     d = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
-    op = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, tbem.FMMConfig(0.3, 10, 10, 0.05, False))
+    op = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, tbem.FMMConfig(0.3, 10, leaf, 0.05, False))
     grad = op.obs_grad(x, gg)
     h = 1e-6
-    idx = rng.choice(300, 25, replace=False)
+    idx = rng.choice(n_obs, 25, replace=False)
     for i in idx:
         for m in range(3):
             p = g["obs"].copy()
