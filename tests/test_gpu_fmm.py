@@ -275,16 +275,23 @@ def test_gpu_leaf_workgroup_shapes_match_oracle(tbem, orc, kid, leaf):
     leaf stage's cross-slice sum: many slices summed in groups (small leaves), one
     group (43 and 86 with three components), more target-slot sums than the
     workgroup has threads (171 and up), and, at 3000, one leaf of all 2400 targets
-    that the workgroup covers in three passes."""
+    that the workgroup covers in three passes. The adjoint identity <Ax, g> =
+    <x, A^T g> checks the transposed leaf stage at the same shapes."""
     g = random_set(2400, 2000, 60 + kid)
     R = components(kid)
-    x = np.random.default_rng(kid + leaf).standard_normal(R * 2000)
+    rng = np.random.default_rng(kid + leaf)
+    x, gg = rng.standard_normal(R * 2000), rng.standard_normal(R * 2400)
     case("leaf shapes", kid, leaf)
     d = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
-    y = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, tbem.FMMConfig(0.3, 5, leaf, 0.05, False)).apply(x)
+    op = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, tbem.FMMConfig(0.3, 5, leaf, 0.05, False))
+    y = op.apply(x)
     ref = orc.fmm(kid, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.3, 5, leaf, 0.05,
                   False).apply(x)
     assert rel_err(y, ref) < FP64_TOL
+    # the transposed leaf stage (P2P^T + P2L^T + P2M^T) over the same shapes
+    xt = op.apply_adjoint(gg)
+    scale = np.abs(y).max() * np.abs(gg).sum() + np.abs(x).max() * np.abs(xt).sum()
+    assert abs(y @ gg - x @ xt) < 1e-13 * scale
 
 
 # ----------------------------------------------------------------- obs gradient
