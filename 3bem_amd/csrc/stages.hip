@@ -17,12 +17,13 @@
 // target slots; the last NB - S*ntp lanes sit out), summed through LDS at the
 // end by the whole workgroup (slice_sum: the FP64 elastic leaf stage went 4.41 ->
 // 4.08 ms when the slice-0 lanes alone added up to 42 partials each); in the leaf
-// stage each lane carries two targets, so one LDS read feeds two pairs. Workgroup width is the main performance lever of the leaf stage: wider
+// stage each lane carries two targets, so one LDS read feeds two pairs.
+// Workgroup width is the main performance lever of the leaf stage: wider
 // workgroups give each leaf's targets more source slices, hence more waves in
 // flight per leaf and fewer tile barriers per pair. Measured on the 1M-point
 // elastic workload (order 6, leaf 64), FP64 leaf stage: 6.4 ms at 128 threads,
-// 4.7 ms at 256, 4.4 ms at 512 with tiles packed across range boundaries. The
-// surface stages have few sources per target and prefer 128.
+// 4.7 ms at 256, 4.4 ms at 512 (8.6 ms at 1024) with tiles packed across range
+// boundaries. The surface stages have few sources per target and prefer 128.
 //
 // Precision. Surface-target stages are FP64 only: each check potential they form
 // feeds a check-to-equivalent pseudo-inverse (condition number up to ~1e10 at the
@@ -320,62 +321,22 @@ __device__ inline void stream_surfaces(Tile<double, R, NRM, NB>& tl, int e0, int
 }
 
 // ---------------------------------------------------------------------- pair loop
-// One source of a tile read into registers: FP64 position, normal and density in
-// the tile's value type.
-template <class VT, int R, bool NRM>
-struct SrcVal {
-    double x, y, z;
-    VT n[3], d[R];
-};
-
-template <class VT, int R, bool NRM, int NB>
-__device__ __forceinline__ SrcVal<VT, R, NRM> read_src(const Tile<VT, R, NRM, NB>& tl, int j) {
-    SrcVal<VT, R, NRM> s;
-    s.x = tl.x[j];
-    s.y = tl.y[j];
-    s.z = tl.z[j];
-    s.n[0] = NRM ? tl.nx[j] : VT(0);
-    s.n[1] = NRM ? tl.ny[j] : VT(0);
-    s.n[2] = NRM ? tl.nz[j] : VT(0);
-    for (int q = 0; q < R; q++) s.d[q] = tl.d[q][j];
-    return s;
-}
-
-// One source on one target (FP64 coordinates tx, ty, tz): the displacement is
-// formed in FP64, then the kernel runs in T.
-template <int KID, bool ADJ, bool FAST = false, class T, class VT, int R, bool NRM>
-__device__ __forceinline__ void pair_v(const SrcVal<VT, R, NRM>& src, double tx, double ty, double tz,
-                                       const T* tn, T* acc, const KC<T>& kc) {
-    const T sn[3] = {T(src.n[0]), T(src.n[1]), T(src.n[2])};
-    T v[R];
-    for (int q = 0; q < R; q++) v[q] = T(src.d[q]);
-    if (!ADJ) {
-        const T dx = T(src.x - tx), dy = T(src.y - ty), dz = T(src.z - tz);
-        const T r2 = dx * dx + dy * dy + dz * dz;
-        kop<KID, false, T, FAST>(dx, dy, dz, r2, tn, sn, v, acc, kc);
-    } else {
-        const T dx = T(tx - src.x), dy = T(ty - src.y), dz = T(tz - src.z);
-        const T r2 = dx * dx + dy * dy + dz * dz;
-        kop<KID, true, T, FAST>(dx, dy, dz, r2, sn, tn, v, acc, kc);
-    }
-}
-
-// One source j of the tile on one target.
+// One source j of the tile on one target (FP64 coordinates tx, ty, tz): the
+// displacement is formed in FP64, then the kernel runs in T.
 template <int KID, bool ADJ, bool FAST = false, class T, class VT, int R, bool NRM, int NB>
 __device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, double tx, double ty, double tz,
                                      const T* tn, T* acc, const KC<T>& kc) {
-    pair_v<KID, ADJ, FAST>(read_src(tl, j), tx, ty, tz, tn, acc, kc);
-}
-
-// Sources slice, slice + S, ... < cnt of a tile on two targets (a, b).
-template <int KID, bool ADJ, bool FAST, class T, class VT, int R, bool NRM, int NB>
-__device__ __forceinline__ void pair_loop2(const Tile<VT, R, NRM, NB>& tl, int slice, int S, int cnt,
-                                           double ax, double ay, double az, const T* an, T* acA,
-                                           double bx, double by, double bz, const T* bn, T* acB,
-                                           const KC<T>& kc) {
-    for (int j = slice; j < cnt; j += S) {
-        pair<KID, ADJ, FAST>(tl, j, ax, ay, az, an, acA, kc);
-        pair<KID, ADJ, FAST>(tl, j, bx, by, bz, bn, acB, kc);
+    const T sn[3] = {NRM ? T(tl.nx[j]) : T(0), NRM ? T(tl.ny[j]) : T(0), NRM ? T(tl.nz[j]) : T(0)};
+    T v[R];
+    for (int q = 0; q < R; q++) v[q] = T(tl.d[q][j]);
+    if (!ADJ) {
+        const T dx = T(tl.x[j] - tx), dy = T(tl.y[j] - ty), dz = T(tl.z[j] - tz);
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        kop<KID, false, T, FAST>(dx, dy, dz, r2, tn, sn, v, acc, kc);
+    } else {
+        const T dx = T(tx - tl.x[j]), dy = T(ty - tl.y[j]), dz = T(tz - tl.z[j]);
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        kop<KID, true, T, FAST>(dx, dy, dz, r2, sn, tn, v, acc, kc);
     }
 }
 
@@ -492,9 +453,12 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
         double sfA[SPLIT ? R : 1], sfB[SPLIT ? R : 1];
         for (int q = 0; q < (SPLIT ? R : 1); q++) sfA[q] = sfB[q] = 0.0;
         auto points = [&](int cnt) {
-            if (va)
-                pair_loop2<KID, ADJ, FAST>(sm.pt, slice, S, cnt, ax, ay, az, an, acA.tile(), bx, by, bz, bn,
-                                           acB.tile(), kc);
+            if (va) {
+                for (int j = slice; j < cnt; j += S) {
+                    pair<KID, ADJ, FAST>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                    pair<KID, ADJ, FAST>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                }
+            }
             acA.flush();
             acB.flush();
         };
