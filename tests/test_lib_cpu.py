@@ -103,3 +103,14 @@ def test_saved_plan_refused_before_touching_the_device(tbem):
                       (b"TBEMPLN1" + (3).to_bytes(8, "little") + bytes(8) + b"xyz", b"checksum")):
         assert lib.tbem_fmm_create_from_plan(ctypes.byref(d), blob, len(blob), ctypes.byref(h)) == 1
         assert why in lib.tbem_last_error()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with status 2 and says why,
+    before it touches torch or a device."""
+    import subprocess
+    import sys
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert p.returncode == 2
+    assert "one GPU only" in p.stderr and p.stdout == ""
