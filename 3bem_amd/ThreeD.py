@@ -172,9 +172,15 @@ class FMMOperator:
     up_c2e / down_c2e: optional per-level check-to-equivalent matrices, shape
     (levels, n, n), used instead of the GPU-computed ones (parity runs).
     plan: bytes from save_plan() of an operator built on the same inputs and config.
+    tree_boxes: for distinct observation and source points, "auto" (each tree on its
+    own points' box as the reference, except mixed precision at orders the MFMA M2L
+    serves, where both share the union box), "own" or "common" to force either.
     """
 
-    def __init__(self, K, data, config, precision="fp64", up_c2e=None, down_c2e=None, plan=None):
+    TREE_BOXES = {"auto": 0, "own": 1, "common": 2}
+
+    def __init__(self, K, data, config, precision="fp64", up_c2e=None, down_c2e=None, plan=None,
+                 tree_boxes="auto"):
         lib = _lib.require_device()
         self.K = K
         self.data = data
@@ -195,6 +201,7 @@ class FMMOperator:
         desc.min_pts_per_cell, desc.d = config.min_pts_per_cell, config.d
         desc.account_for_small_cells = 1 if config.account_for_small_cells else 0
         desc.precision = PRECISIONS[precision]
+        desc.tree_boxes = self.TREE_BOXES[tree_boxes]
         if self._up is not None:
             desc.up_check_to_equiv, desc.n_up_levels = _dp(self._up), self._up.shape[0]
         if self._down is not None:
@@ -296,9 +303,25 @@ class FMMOperator:
         return out.reshape(-1, 3)
 
     def set_obs_locs(self, obs_locs):
-        """Move the observation points without rebuilding the plan (shared with clones)."""
+        """Move the observation points without rebuilding the plan (shared with clones).
+        obs_locs: (N_obs, 3) NumPy array, or a float64 CUDA tensor (moved on the device,
+        on torch's current stream). Every point must stay inside the ball (centre,
+        radius |half width|) of the leaf it was sorted into; otherwise this raises and
+        nothing moves."""
+        if _is_torch(obs_locs):
+            import torch
+            t = obs_locs.detach()
+            assert t.is_cuda and t.dtype == torch.float64 and t.numel() == 3 * self.data.n_obs
+            t = t.contiguous()
+            _lib.check(self._lib.tbem_fmm_set_obs_locs_dev(self._h, ctypes.c_void_p(t.data_ptr()),
+                                                           _torch_stream()))
+            return
         a = _aos(obs_locs, self.data.n_obs)
         _lib.check(self._lib.tbem_fmm_set_obs_locs(self._h, _dp(a)))
+
+    def obs_version(self):
+        """Number of observation-point moves of the plan so far (shared with clones)."""
+        return int(self._lib.tbem_fmm_obs_version(self._h))
 
     def info(self):
         inf = _lib.FmmInfo()

@@ -20,7 +20,7 @@ EXPORTED = (
     "tbem_fmm_create", "tbem_fmm_plan_save", "tbem_fmm_create_from_plan", "tbem_fmm_apply", "tbem_fmm_apply_dev", "tbem_fmm_apply_adjoint",
     "tbem_fmm_apply_adjoint_dev", "tbem_fmm_obs_grad", "tbem_fmm_obs_grad_dev",
     "tbem_fmm_forward_token", "tbem_fmm_obs_grad_reuse_dev",
-    "tbem_fmm_set_obs_locs", "tbem_fmm_n_rows", "tbem_fmm_n_cols", "tbem_fmm_clone",
+    "tbem_fmm_set_obs_locs", "tbem_fmm_set_obs_locs_dev", "tbem_fmm_obs_version", "tbem_fmm_n_rows", "tbem_fmm_n_cols", "tbem_fmm_clone",
     "tbem_fmm_destroy", "tbem_fmm_info", "tbem_fmm_c2e", "tbem_fmm_pairs", "tbem_fmm_octree",
     "tbem_fmm_stage_times", "tbem_fmm_m2l_ms", "tbem_nbody_create", "tbem_nbody_apply", "tbem_nbody_apply_dev",
     "tbem_nbody_apply_adjoint", "tbem_nbody_n_rows", "tbem_nbody_n_cols", "tbem_nbody_destroy",
@@ -31,7 +31,7 @@ EXPORTED = (
     "tbem_constraints_create", "tbem_constraints_size", "tbem_constraints_n_terms", "tbem_constraints_get",
     "tbem_constraints_homogenize", "tbem_distribute_vector", "tbem_distribute_vector_dev", "tbem_condense_vector",
     "tbem_condense_vector_dev", "tbem_condense_sparse", "tbem_constraints_destroy",
-    "tbem_kernel_eval", "tbem_surface", "tbem_pinv", "tbem_last_error", "tbem_device_available",
+    "tbem_kernel_eval", "tbem_fp64_sqrt_div", "tbem_surface", "tbem_pinv", "tbem_last_error", "tbem_device_available",
 )
 FARFIELD_FMM, FARFIELD_DIRECT = 0, 1
 
@@ -46,7 +46,7 @@ class FmmDesc(ctypes.Structure):
         ("mac", ctypes.c_double), ("order", _SZ), ("min_pts_per_cell", _SZ), ("d", ctypes.c_double),
         ("account_for_small_cells", ctypes.c_int), ("precision", ctypes.c_int),
         ("up_check_to_equiv", _D), ("n_up_levels", _SZ),
-        ("down_check_to_equiv", _D), ("n_down_levels", _SZ),
+        ("down_check_to_equiv", _D), ("n_down_levels", _SZ), ("tree_boxes", ctypes.c_int),
     ]
 
 
@@ -56,6 +56,7 @@ class FmmInfo(ctypes.Structure):
         ("n_down_levels", _SZ), ("p2m", _SZ), ("m2m", _SZ), ("p2p", _SZ), ("m2p", _SZ), ("p2l", _SZ),
         ("m2l", _SZ), ("l2l", _SZ), ("l2p", _SZ), ("p2p_pairs", ctypes.c_double),
         ("m2l_classes", ctypes.c_double), ("plan_seconds", ctypes.c_double),
+        ("m2l_output_bytes", ctypes.c_double), ("work_bytes", ctypes.c_double),
     ]
 
 
@@ -103,6 +104,8 @@ def load(path=LIB_PATH):
         "tbem_fmm_forward_token": (ctypes.c_uint64, [_VP]),
         "tbem_fmm_obs_grad_reuse_dev": (ctypes.c_int, [_VP, ctypes.c_uint64, _VP, _VP, _VP, _VP]),
         "tbem_fmm_set_obs_locs": (ctypes.c_int, [_VP, _D]),
+        "tbem_fmm_set_obs_locs_dev": (ctypes.c_int, [_VP, _VP, _VP]),
+        "tbem_fmm_obs_version": (ctypes.c_uint64, [_VP]),
         "tbem_fmm_n_rows": (_SZ, [_VP]),
         "tbem_fmm_n_cols": (_SZ, [_VP]),
         "tbem_fmm_clone": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
@@ -149,6 +152,7 @@ def load(path=LIB_PATH):
         "tbem_condense_sparse": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(_VP)]),
         "tbem_constraints_destroy": (None, [_VP]),
         "tbem_kernel_eval": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, _SZ, _D, _D, _D, _D, _D]),
+        "tbem_fp64_sqrt_div": (ctypes.c_int, [_SZ, _D, _D, _D, _D]),
         "tbem_surface": (ctypes.c_long, [_SZ, _D]),
         "tbem_pinv": (ctypes.c_int, [_D, ctypes.c_int, ctypes.c_double, _D]),
         "tbem_last_error": (ctypes.c_char_p, []),

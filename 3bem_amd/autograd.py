@@ -15,7 +15,12 @@ Backward (3bem has no autograd; this is the hand-written backward of apply()):
 The plan (trees, surfaces, interaction lists) is held fixed: moving observation
 points only changes the point coordinates inside their cells, so the forward is
 a smooth function of them and central differences of the frozen-plan forward
-reproduce the gradient.
+reproduce the gradient. A point that leaves its leaf cell's ball (centre,
+radius |half width|) makes the forward raise (the frozen plan no longer holds it).
+
+The operator holds one set of observation points (shared with its clones). The
+backward runs at the points its forward used: when another forward has moved
+them since, the backward moves them back first.
 """
 import torch
 
@@ -23,11 +28,14 @@ import torch
 class _FMMMatvec(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, obs_locs, op):
+        obs = None
         if obs_locs is not None:
-            op.set_obs_locs(obs_locs.detach().cpu().numpy())
+            obs = obs_locs.detach().to(dtype=torch.float64).contiguous()
+            op.set_obs_locs(obs if obs.is_cuda else obs.numpy())
         ctx.op = op
         ctx.save_for_backward(x)
-        ctx.has_obs = obs_locs is not None
+        ctx.obs = obs
+        ctx.obs_version = op.obs_version()
         y = op.apply(x.contiguous())
         ctx.token = op.forward_token()
         return y
@@ -37,9 +45,15 @@ class _FMMMatvec(torch.autograd.Function):
         (x,) = ctx.saved_tensors
         op = ctx.op
         g = g.contiguous()
+        if op.obs_version() != ctx.obs_version:
+            if ctx.obs is None:
+                raise RuntimeError("the operator's observation points moved after this forward, which did not "
+                                   "name them: pass obs_locs to fmm_matvec to differentiate through moves")
+            op.set_obs_locs(ctx.obs if ctx.obs.is_cuda else ctx.obs.numpy())
+            ctx.obs_version = op.obs_version()
         gx = op.apply_adjoint(g) if ctx.needs_input_grad[0] else None
         gobs = None
-        if ctx.has_obs and ctx.needs_input_grad[1]:
+        if ctx.obs is not None and ctx.needs_input_grad[1]:
             gobs = op.obs_grad(x.contiguous(), g, token=ctx.token)
         return gx, gobs, None
 

@@ -11,9 +11,12 @@
 // cpp/fmm.cpp:459-545) only changes how the same terms are summed.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <type_traits>
 
 #include "c2e.h"
@@ -36,7 +39,6 @@ struct FmmPlan {
     Surfaces surf;
     int n_e = 0, blk = 0;
     HostTree st, ot;
-    std::vector<double> h_obs_locs, h_obs_normals;  // for set_obs_locs / octree queries
     DevTree dst, dot;
     DevPoints sp, op;
     DevBuf<double> d_unit, d_inner, d_outer;
@@ -65,8 +67,50 @@ struct FmmPlan {
     // and whether set_obs_locs has moved the observation points since
     uint64_t input_hash = 0;
     bool same_tree = false;        // one octree serves both sides
+    // tree layout: kTreesOwnBoxes (the reference's: each tree on its points' tight
+    // box), kTreeShared (one tree for both sides), or kTreesCommonBox (mixed precision
+    // with distinct point sets: both trees on the union box, so that their cells lie
+    // on one grid and the MFMA M2L's translation classes exist)
+    int tree_layout = 0;
     std::vector<double> h_fac[4];  // up W, up V, down W, down V
     bool obs_moved = false;
+    // Moving the observation points (set_obs_locs): every apply of any handle that
+    // shares the plan holds pts_mu shared while it enqueues; set_obs_locs holds it
+    // alone, waits for the device, checks the moved points against their frozen
+    // leaf cells and overwrites the sorted coordinates in place.
+    mutable std::shared_mutex pts_mu;
+    uint64_t obs_version = 0;       // bumped by every set_obs_locs
+    DevBuf<int> d_flag;             // set_obs_locs: count of points outside their leaf cell
+    // The MFMA M2L's per-pair output, one buffer for all handles that share the
+    // plan: the M2L sections of their calls are ordered on the device by y_done
+    // and enqueued one at a time under y_mu.
+    mutable std::mutex y_mu;
+    DevBuf<float> m2l_y;
+    hipEvent_t y_done = nullptr;
+    FmmPlan() = default;
+    FmmPlan(const FmmPlan&) = delete;
+    FmmPlan& operator=(const FmmPlan&) = delete;
+    ~FmmPlan() {
+        if (y_done) (void)hipEventDestroy(y_done);
+    }
+};
+
+// Holds the plan's shared MFMA M2L output buffer for one section of a call on
+// stream s: waits (on the device) for the previous section that used it, on
+// whichever stream, and marks the end of this one.
+class YLease {
+  public:
+    YLease(const FmmPlan& P, hipStream_t s) : P_(P), s_(s), lk_(P.y_mu) {
+        TBEM_HIP(hipStreamWaitEvent(s_, P_.y_done, 0));
+    }
+    ~YLease() { (void)hipEventRecord(P_.y_done, s_); }
+    YLease(const YLease&) = delete;
+    YLease& operator=(const YLease&) = delete;
+
+  private:
+    const FmmPlan& P_;
+    hipStream_t s_;
+    std::unique_lock<std::mutex> lk_;
 };
 
 struct Work {
@@ -74,7 +118,6 @@ struct Work {
     DevBuf<double> dens, mult, locals, out_sorted, xbuf, ybuf;
     DevBuf<double> gsort, lbar, cbar, mbar, ebar, grad_sorted;
     DevBuf<float> chk_up;   // modal up-check potential per source cell (input of the MFMA M2L)
-    DevBuf<float> m2l_y;    // per-pair MFMA M2L output
     DevBuf<float> m2l_w;    // transpose: modal input per observation cell
     DevBuf<double> e2;      // transpose: modal M2L^T contribution to ebar per source cell
     DevBuf<double> fld;     // raw check potentials of one stage, blk per cell (input of the C2E GEMMs)
@@ -98,7 +141,6 @@ struct Work {
         if (P.m2l_mfma.enabled) {
             chk_up.alloc((size_t)P.dst.n_cells * P.blk);
             chk_up.zero(stream);  // cells without a P2M/M2M stage keep a zero potential
-            m2l_y.alloc((size_t)P.m2l_mfma.n_pairs * P.m2l_mfma.ystride);
             TBEM_HIP(hipStreamSynchronize(stream));
         }
     }
@@ -202,6 +244,8 @@ static void validate(const tbem_fmm_desc* d) {
     TBEM_REQUIRE(d->min_pts_per_cell > 0, TBEM_E_INVALID, "FMMConfig.min_pts_per_cell must be > 0");
     TBEM_REQUIRE(d->precision == TBEM_FP64 || d->precision == TBEM_MIXED, TBEM_E_INVALID,
                  "precision must be TBEM_FP64 or TBEM_MIXED");
+    TBEM_REQUIRE(d->tree_boxes >= TBEM_TREES_AUTO && d->tree_boxes <= TBEM_TREES_COMMON_BOX, TBEM_E_INVALID,
+                 "tree_boxes must be TBEM_TREES_AUTO, _OWN_BOXES or _COMMON_BOX");
     TBEM_REQUIRE(d->n_obs < (1u << 30) && d->n_src < (1u << 30), TBEM_E_UNSUPPORTED,
                  "more than 2^30 points per side");
     TBEM_REQUIRE(d->n_obs == 0 || (d->obs_locs && d->obs_normals), TBEM_E_INVALID, "null observation arrays");
@@ -214,10 +258,13 @@ static void validate(const tbem_fmm_desc* d) {
 // the mixed path, their SVD factors, under a header that names the inputs. Layout
 // (little endian): the 8 bytes "TBEMPLN1"; u64 payload length; u64 hash64 of the
 // payload; then the payload: the header fields (kernel, mu, nu, n_obs, n_src, mac,
-// order, min_pts_per_cell, d, account_for_small_cells, same_tree, input hash) and
+// order, min_pts_per_cell, d, account_for_small_cells, tree layout, input hash) and
 // the sections, each array as a u64 count followed by its elements.
+constexpr int kTreesOwnBoxes = 0, kTreeShared = 1, kTreesCommonBox = 2;
+
 struct SavedPlan {
     bool same = false;
+    int layout = kTreesOwnBoxes;
     HostTree st, ot;
     std::vector<double> uc2e, dc2e, fac[4];
     TaskLists tl;
@@ -355,6 +402,22 @@ static HostTree get_tree(PlanReader& r, int n_pts) {
         for (int c : *v) ok = ok && c >= 0 && c < t.n_cells;
     for (int c : t.leaves) ok = ok && t.leaf[c];
     TBEM_REQUIRE(ok, TBEM_E_INVALID, "saved plan: octree indices out of range");
+    // structure: one root; every other cell's parent lies one level up and lists it
+    // among its children; children lie one level down (the M2M/L2L stages and
+    // their transposes follow these links)
+    ok = nc == 0 || (nl >= 1 && t.level_begin[1] == 1);
+    for (size_t c = 0; ok && c < nc; c++) {
+        const int p = t.parent[c];
+        if (t.level[c] == 0) {
+            ok = p == -1;
+        } else {
+            ok = p >= 0 && t.level[p] == t.level[c] - 1 && t.first_child[p] >= 0 &&
+                 (long)c >= t.first_child[p] && (long)c < (long)t.first_child[p] + t.n_child[p];
+        }
+        for (int k = 0; ok && k < t.n_child[c]; k++) ok = t.parent[t.first_child[c] + k] == (int)c;
+        ok = ok && (t.leaf[c] ? t.n_child[c] == 0 : t.n_child[c] > 0);
+    }
+    TBEM_REQUIRE(ok, TBEM_E_INVALID, "saved plan: octree parent and child links disagree");
     return t;
 }
 
@@ -392,11 +455,11 @@ static std::vector<char> save_plan(const FmmPlan& P) {
     w.pod((uint64_t)P.leaf);
     w.pod(P.d);
     w.pod((int)P.small_cells);
-    const int same = P.same_tree ? 1 : 0;
-    w.pod(same);
+    const int layout = P.tree_layout;
+    w.pod(layout);
     w.pod(P.input_hash);
     put_tree(w, P.st);
-    if (!same) put_tree(w, P.ot);
+    if (layout != kTreeShared) put_tree(w, P.ot);
     w.vec(P.h_uc2e);
     w.vec(P.h_dc2e);
     for (const auto& f : P.h_fac) w.vec(f);
@@ -430,7 +493,7 @@ static SavedPlan load_plan(const tbem_fmm_desc* d, const void* buf, size_t n) {
     const double mac = r.pod<double>();
     const uint64_t order = r.pod<uint64_t>(), leaf = r.pod<uint64_t>();
     const double dd = r.pod<double>();
-    const int small = r.pod<int>(), same = r.pod<int>();
+    const int small = r.pod<int>(), layout = r.pod<int>();
     const uint64_t ih = r.pod<uint64_t>();
     const KConst kc = make_kconst(d->shear_modulus, d->poisson_ratio);
     TBEM_REQUIRE(kid == d->kernel && mu == kc.mu && nu == kc.nu && n_obs == d->n_obs && n_src == d->n_src &&
@@ -438,9 +501,13 @@ static SavedPlan load_plan(const tbem_fmm_desc* d, const void* buf, size_t n) {
                      small == (d->account_for_small_cells != 0),
                  TBEM_E_INVALID, "saved plan: built for another kernel or FMMConfig");
     TBEM_REQUIRE(ih == input_hash(d), TBEM_E_INVALID, "saved plan: built for other points, normals or weights");
-    TBEM_REQUIRE(same == 0 || n_obs == n_src, TBEM_E_INVALID, "saved plan: shared tree with different counts");
+    TBEM_REQUIRE(layout == kTreesOwnBoxes || layout == kTreeShared || layout == kTreesCommonBox, TBEM_E_INVALID,
+                 "saved plan: unknown tree layout");
+    TBEM_REQUIRE(layout != kTreeShared || n_obs == n_src, TBEM_E_INVALID,
+                 "saved plan: shared tree with different counts");
     SavedPlan sv;
-    sv.same = same != 0;
+    sv.same = layout == kTreeShared;
+    sv.layout = layout;
     sv.st = get_tree(r, (int)n_src);
     sv.ot = sv.same ? sv.st : get_tree(r, (int)n_obs);
     sv.uc2e = r.vec<double>();
@@ -463,8 +530,30 @@ static SavedPlan load_plan(const tbem_fmm_desc* d, const void* buf, size_t n) {
     return sv;
 }
 
+// Phase times of a plan build, printed to stderr when TBEM_PLAN_TIMING is set.
+struct PhaseClock {
+    bool on;
+    hipStream_t s;
+    std::chrono::steady_clock::time_point t;
+    std::string line;
+    explicit PhaseClock(hipStream_t st) : on(std::getenv("TBEM_PLAN_TIMING") != nullptr), s(st),
+                                          t(std::chrono::steady_clock::now()) {}
+    void mark(const char* what) {
+        if (!on) return;
+        (void)hipStreamSynchronize(s);
+        const auto now = std::chrono::steady_clock::now();
+        line += std::string(" ") + what + " " +
+                std::to_string(std::chrono::duration<double, std::milli>(now - t).count()).substr(0, 7) + " ms;";
+        t = now;
+    }
+    ~PhaseClock() {
+        if (on) std::fprintf(stderr, "[tbem plan]%s\n", line.c_str());
+    }
+};
+
 static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s, const SavedPlan* sv = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
+    PhaseClock clk(s);
     validate(d);
     auto P = std::make_shared<FmmPlan>();
     P->kid = d->kernel;
@@ -488,18 +577,30 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
                       (d->obs_locs == d->src_locs ||
                        (d->n_obs > 0 && std::memcmp(d->obs_locs, d->src_locs, 3 * sizeof(double) * d->n_obs) == 0));
     P->same_tree = same;
+    const bool common = d->tree_boxes == TBEM_TREES_COMMON_BOX ||
+                        (d->tree_boxes == TBEM_TREES_AUTO && P->precision == TBEM_MIXED && m2l_mfma_size_ok(P->blk));
+    P->tree_layout = same ? kTreeShared : (common && d->n_obs > 0 && d->n_src > 0) ? kTreesCommonBox : kTreesOwnBoxes;
     if (sv) {
         TBEM_REQUIRE(sv->same == same, TBEM_E_INVALID, "saved plan: tree sharing differs from the inputs'");
+        TBEM_REQUIRE(sv->layout == P->tree_layout, TBEM_E_INVALID,
+                     "saved plan: its trees were laid out on other boxes (precision or tree_boxes differ)");
         P->st = sv->st;
         P->ot = sv->ot;
+    } else if (P->tree_layout == kTreesCommonBox) {
+        double a[6], b[6];
+        point_bounds(d->src_locs, (int)d->n_src, a);
+        point_bounds(d->obs_locs, (int)d->n_obs, b);
+        for (int k = 0; k < 3; k++) {
+            a[k] = std::min(a[k], b[k]);
+            a[3 + k] = std::max(a[3 + k], b[3 + k]);
+        }
+        P->st = build_octree(d->src_locs, (int)d->n_src, leaf, s, a);
+        P->ot = build_octree(d->obs_locs, (int)d->n_obs, leaf, s, a);
     } else {
         P->st = build_octree(d->src_locs, (int)d->n_src, leaf, s);
         P->ot = same ? P->st : build_octree(d->obs_locs, (int)d->n_obs, leaf, s);
     }
-    if (d->n_obs) {
-        P->h_obs_locs.assign(d->obs_locs, d->obs_locs + 3 * d->n_obs);
-        P->h_obs_normals.assign(d->obs_normals, d->obs_normals + 3 * d->n_obs);
-    }
+    clk.mark("octrees");
     P->dst.upload(P->st, s);
     P->dot.upload(P->ot, s);
     P->sp.upload(P->st, d->src_locs, d->src_normals, d->src_weights, s);
@@ -508,6 +609,7 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->d_inner.upload(P->surf.inner, s);
     P->d_outer.upload(P->surf.outer, s);
 
+    clk.mark("uploads");
     // check-to-equivalent operators
     const size_t bb = (size_t)P->blk * P->blk;
     // the mixed path's modal factors come from the C2E pseudo-inverses' own SVDs
@@ -548,9 +650,11 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         }
     }
 
+    clk.mark("c2e");
     // task lists (cpp/fmm.cpp:560-648)
     if (sv) P->tl = sv->tl;
     else dual_tree_traversal(P->ot, P->st, P->mac, P->n_e, P->small_cells, P->tl);
+    clk.mark("traversal");
     const TaskLists& tl = P->tl;
     for (auto& p: tl.p2p) P->p2p_pairs += (double)P->ot.count[p[0]] * (double)P->st.count[p[1]];
 
@@ -569,9 +673,15 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     const HostList m2l_h = list_from_pairs(cells_with(tl.m2l, 0, OT.n_cells), OT.n_cells, tl.m2l, 0, nullptr);
     P->m2l.upload(m2l_h, s);
     P->t_m2l = make_cell_tiles(m2l_h.items, OT, s);
+    clk.mark("lists");
     if (P->precision == TBEM_MIXED)
-        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, same, m2l_h, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac,
+        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, P->tree_layout != kTreesOwnBoxes, m2l_h, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac,
                        down_fac);
+    clk.mark("m2l_mfma");
+    TBEM_HIP(hipEventCreateWithFlags(&P->y_done, hipEventDisableTiming));
+    TBEM_HIP(hipEventRecord(P->y_done, s));
+    if (P->m2l_mfma.enabled) P->m2l_y.alloc((size_t)P->m2l_mfma.n_pairs * P->m2l_mfma.ystride);
+    P->d_flag.alloc(1);
     const std::vector<int> p2l_items = cells_with(tl.p2l, 0, OT.n_cells);
     P->p2l.upload(list_from_pairs(p2l_items, OT.n_cells, tl.p2l, 0, &ST), s);
     P->t_p2l = make_cell_tiles(p2l_items, OT, s);
@@ -597,6 +707,7 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->a_leaf_p2p.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, &OT), s);
     P->a_leaf_p2l.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr), s);
 
+    clk.mark("rest");
     TBEM_HIP(hipStreamSynchronize(s));
     P->plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return P;
@@ -668,9 +779,10 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
     TBEM_HIP(hipEventRecord(W.ev[8], s));
     if (mfma) {
         // Vd M_c Bu check = DC2E K_c UC2E check = the FP64 stage's DC2E K_c mult
-        launch_m2l_mfma(P.m2l_mfma, P.m2l.cell.p, W.chk_up.p, W.m2l_y.p, s);
+        YLease y(P, s);
+        launch_m2l_mfma(P.m2l_mfma, P.m2l.cell.p, W.chk_up.p, P.m2l_y.p, s);
         TBEM_HIP(hipEventRecord(W.ev[9], s));
-        launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.t_m2l, W.m2l_y.p, W.fld.p,
+        launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.t_m2l, P.m2l_y.p, W.fld.p,
                           W.locals.p, s);
     } else {
         SurfStage m = surf_base(P);
@@ -828,9 +940,10 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     // M2L^T into mbar (FP64), or, on MFMA, e2 = Bu^T M_c^T Vd^T lbar, added to ebar below
     if (mfma) {
         launch_m2l_modal_in(P.m2l_mfma, P.t_m2l, W.lbar.p, W.m2l_w.p, s);
-        launch_m2l_mfma(P.m2l_mfma, nullptr, W.m2l_w.p, W.m2l_y.p, s, true);
         launch_fill(W.e2.p, ncs, 0.0, s);
-        launch_m2l_reduce_adj(P.m2l_mfma, W.m2l_y.p, W.fld.p, W.e2.p, s);
+        YLease y(P, s);
+        launch_m2l_mfma(P.m2l_mfma, nullptr, W.m2l_w.p, P.m2l_y.p, s, true);
+        launch_m2l_reduce_adj(P.m2l_mfma, P.m2l_y.p, W.fld.p, W.e2.p, s);
     } else {
         SurfStage m = surf_base(P);
         set_list(m, P.a_m2l);
@@ -875,6 +988,65 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     f.out_sorted = W.dens.p;
     launch_leaf_stage<T>(P.kid, true, f, P.kc, s, true);
     launch_scatter_out(W.dens.p, P.sp.perm.p, Ns, R, xbar, s);
+}
+
+// ------------------------------------------------- moving the observation points
+// Counts the moved observation points (AoS, caller's order) that lie outside the
+// ball of the leaf they were sorted into: centre c, radius |half width|, the
+// radius the dual-tree MAC test gives the cell (cpp/fmm.cpp:577-586), with a 1e-9
+// relative allowance for the rounding of the subdivided centres. (A box test
+// would refuse every move off a flat point set, whose cells have zero extent
+// along its normal.) NaN coordinates count as outside. One workgroup per
+// observation leaf.
+__global__ void k_obs_check(const double* __restrict__ locs, const int* __restrict__ perm,
+                            const int* __restrict__ leaves, TreeView t, int* __restrict__ n_out) {
+    const int leaf = leaves[blockIdx.x];
+    const int b = t.begin[leaf], m = t.count[leaf];
+    const double r = t.level_r[t.level[leaf]];
+    const double c[3] = {t.cx[leaf], t.cy[leaf], t.cz[leaf]};
+    int bad = 0;
+    for (int i = b + threadIdx.x; i < b + m; i += blockDim.x) {
+        const size_t g = (size_t)perm[i];
+        const double dx = locs[3 * g] - c[0], dy = locs[3 * g + 1] - c[1], dz = locs[3 * g + 2] - c[2];
+        bad += !(dx * dx + dy * dy + dz * dz <= r * r * (1.0 + 1e-9));
+    }
+    if (bad) atomicAdd(n_out, bad);
+}
+
+// sorted SoA coordinates <- AoS points in the caller's order
+__global__ void k_obs_move(const double* __restrict__ locs, const int* __restrict__ perm, int n,
+                           double* __restrict__ x, double* __restrict__ y, double* __restrict__ z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t g = (size_t)perm[i];
+    x[i] = locs[3 * g];
+    y[i] = locs[3 * g + 1];
+    z[i] = locs[3 * g + 2];
+}
+
+// Overwrites the plan's observation coordinates with d_locs (device, AoS, 3*n_obs)
+// after checking that every point stays inside its leaf's ball; on failure nothing
+// changes. The caller holds P.pts_mu alone, so no handle can enqueue meanwhile.
+static void move_obs(FmmPlan& P, const double* d_locs, hipStream_t s) {
+    const int No = (int)P.n_obs;
+    TBEM_HIP(hipDeviceSynchronize());  // no call of any handle sharing the plan is in flight
+    TBEM_HIP(hipMemsetAsync(P.d_flag.p, 0, sizeof(int), s));
+    const int n_leaves = (int)P.ot.leaves.size();
+    hipLaunchKernelGGL(k_obs_check, dim3(n_leaves), dim3(256), 0, s, d_locs, P.op.perm.p, P.obs_leaves.p,
+                       view(P.dot), P.d_flag.p);
+    TBEM_LAUNCH_CHECK();
+    int n_out = 0;
+    TBEM_HIP(hipMemcpyAsync(&n_out, P.d_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TBEM_HIP(hipStreamSynchronize(s));
+    TBEM_REQUIRE(n_out == 0, TBEM_E_INVALID,
+                 std::to_string(n_out) + " observation points moved outside their leaf cells: the frozen "
+                 "plan no longer holds them; build a new operator");
+    hipLaunchKernelGGL(k_obs_move, dim3(div_up(No, 256)), dim3(256), 0, s, d_locs, P.op.perm.p, No, P.op.x.p,
+                       P.op.y.p, P.op.z.p);
+    TBEM_LAUNCH_CHECK();
+    TBEM_HIP(hipStreamSynchronize(s));
+    P.obs_moved = true;
+    P.obs_version++;
 }
 
 static void forward(const FmmPlan& P, Work& W, const double* x, double* y, hipStream_t s) {
@@ -987,6 +1159,7 @@ int tbem_fmm_plan_save(tbem_fmm_t h, void* buf, size_t cap, size_t* bytes) {
     return guard([&] {
         TBEM_REQUIRE(h && bytes, TBEM_E_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         const std::vector<char> b = save_plan(*h->plan);
         *bytes = b.size();
         if (buf) {
@@ -1001,6 +1174,7 @@ int tbem_fmm_apply(tbem_fmm_t h, const double* x, double* y) {
         TBEM_REQUIRE(h && (x || h->plan->n_src == 0) && (y || h->plan->n_obs == 0), TBEM_E_INVALID,
                      "null argument");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         host_roundtrip(h, false, x, y);
     });
 }
@@ -1009,6 +1183,7 @@ int tbem_fmm_apply_dev(tbem_fmm_t h, const double* d_x, double* d_y, void* strea
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         forward(*h->plan, *h->work, d_x, d_y, (hipStream_t)stream);
     });
 }
@@ -1018,6 +1193,7 @@ int tbem_fmm_apply_adjoint(tbem_fmm_t h, const double* g, double* y) {
         TBEM_REQUIRE(h && (g || h->plan->n_obs == 0) && (y || h->plan->n_src == 0), TBEM_E_INVALID,
                      "null argument");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         host_roundtrip(h, true, g, y);
     });
 }
@@ -1026,6 +1202,7 @@ int tbem_fmm_apply_adjoint_dev(tbem_fmm_t h, const double* d_g, double* d_y, voi
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         adjoint(*h->plan, *h->work, d_g, d_y, (hipStream_t)stream);
     });
 }
@@ -1034,6 +1211,7 @@ int tbem_fmm_obs_grad_dev(tbem_fmm_t h, const double* d_x, const double* d_g, do
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         hipStream_t s = (hipStream_t)stream;
         h->work->begin(s);
         obs_grad(*h->plan, *h->work, d_x, d_g, d_grad, s);
@@ -1052,6 +1230,7 @@ int tbem_fmm_obs_grad_reuse_dev(tbem_fmm_t h, uint64_t token, const double* d_x,
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         hipStream_t s = (hipStream_t)stream;
         h->work->begin(s);
         obs_grad(*h->plan, *h->work, d_x, d_g, d_grad, s, token);
@@ -1063,6 +1242,7 @@ int tbem_fmm_obs_grad(tbem_fmm_t h, const double* x, const double* g, double* gr
     return guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         std::lock_guard<std::mutex> lk(h->mu);
+        std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
         FmmPlan& P = *h->plan;
         Work& W = *h->work;
         const size_t nx = P.R * P.n_src, ng = P.R * P.n_obs;
@@ -1086,11 +1266,28 @@ int tbem_fmm_set_obs_locs(tbem_fmm_t h, const double* obs_locs) {
         std::lock_guard<std::mutex> lk(h->mu);
         FmmPlan& P = *h->plan;
         if (P.n_obs == 0) return;
-        TBEM_HIP(hipDeviceSynchronize());  // no call of any handle sharing the plan may be in flight
-        std::copy(obs_locs, obs_locs + 3 * P.n_obs, P.h_obs_locs.begin());
-        P.obs_moved = true;
-        P.op.upload(P.ot, P.h_obs_locs.data(), P.h_obs_normals.data(), nullptr, h->work->stream);
+        std::unique_lock<std::shared_mutex> pl(P.pts_mu);
+        DevBuf<double> d;
+        d.upload(obs_locs, 3 * P.n_obs, h->work->stream);
+        move_obs(P, d.p, h->work->stream);
     });
+}
+
+int tbem_fmm_set_obs_locs_dev(tbem_fmm_t h, const double* d_obs_locs, void* stream) {
+    return guard([&] {
+        TBEM_REQUIRE(h && (d_obs_locs || h->plan->n_obs == 0), TBEM_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
+        FmmPlan& P = *h->plan;
+        if (P.n_obs == 0) return;
+        std::unique_lock<std::shared_mutex> pl(P.pts_mu);
+        move_obs(P, d_obs_locs, (hipStream_t)stream);
+    });
+}
+
+uint64_t tbem_fmm_obs_version(tbem_fmm_t h) {
+    if (!h) return 0;
+    std::shared_lock<std::shared_mutex> pl(h->plan->pts_mu);
+    return h->plan->obs_version;
 }
 
 size_t tbem_fmm_n_rows(tbem_fmm_t h) { return h ? h->plan->R * h->plan->n_obs : 0; }
@@ -1115,6 +1312,7 @@ void tbem_fmm_destroy(tbem_fmm_t h) {
 int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
     return guard([&] {
         TBEM_REQUIRE(h && o, TBEM_E_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(h->mu);
         const FmmPlan& P = *h->plan;
         std::memset(o, 0, sizeof(*o));
         o->n_e = P.n_e;
@@ -1133,6 +1331,14 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
         o->p2p_pairs = P.p2p_pairs;
         o->m2l_classes = P.m2l_mfma.enabled ? P.m2l_mfma.n_classes : 0;
         o->plan_seconds = P.plan_seconds;
+        o->m2l_output_bytes = (double)P.m2l_y.n * sizeof(float);
+        const Work& W = *h->work;
+        double wb = 0;
+        for (const DevBuf<double>* b : {&W.dens, &W.mult, &W.locals, &W.out_sorted, &W.xbuf, &W.ybuf, &W.gsort,
+                                        &W.lbar, &W.cbar, &W.mbar, &W.ebar, &W.grad_sorted, &W.e2, &W.fld})
+            wb += (double)b->n * sizeof(double);
+        for (const DevBuf<float>* b : {&W.chk_up, &W.m2l_w}) wb += (double)b->n * sizeof(float);
+        o->work_bytes = wb;
     });
 }
 
@@ -1265,6 +1471,33 @@ __global__ void k_kernel_eval(int kid, KConst kc, int n, const double* obs, cons
     tensor_ref_dispatch(kid, obs + 3 * i, src + 3 * i, nobs + 3 * i, nsrc + 3 * i, kc, out + (size_t)R * R * i);
 }
 }  // namespace tbem
+
+namespace tbem {
+// sqrt(a) and a / b as the device code of tensor_ref evaluates them
+__global__ void k_fp64_sqrt_div(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                double* __restrict__ sq, double* __restrict__ q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sq[i] = sqrt(a[i]);
+    q[i] = a[i] / b[i];
+}
+}  // namespace tbem
+
+extern "C" int tbem_fp64_sqrt_div(size_t n, const double* a, const double* b, double* sqrt_a, double* a_div_b) {
+    return guard([&] {
+        TBEM_REQUIRE(a && b && sqrt_a && a_div_b, TBEM_E_INVALID, "null argument");
+        TBEM_REQUIRE(n < (size_t(1) << 30), TBEM_E_UNSUPPORTED, "too many values");
+        if (n == 0) return;
+        DevBuf<double> da, db, ds(n), dq(n);
+        da.upload(a, n);
+        db.upload(b, n);
+        hipLaunchKernelGGL(k_fp64_sqrt_div, dim3(div_up(n, 256)), dim3(256), 0, 0, (int)n, da.p, db.p, ds.p, dq.p);
+        TBEM_LAUNCH_CHECK();
+        ds.download(sqrt_a, n);
+        dq.download(a_div_b, n);
+        TBEM_HIP(hipDeviceSynchronize());
+    });
+}
 
 extern "C" int tbem_kernel_eval(int kernel, double mu, double nu, size_t n, const double* obs,
                                 const double* src, const double* nobs, const double* nsrc, double* out) {

@@ -31,15 +31,19 @@ struct M2lMfma {
     double plan_seconds = 0;
 };
 
+// Whether the MFMA M2L serves operators of this size on the current device: blk
+// from 48 (below, the FP64 stage is faster) up to what one workgroup stages in LDS.
+bool m2l_mfma_size_ok(int blk);
+
 // m2l: the forward M2L list (items = target cells, cell[p] = source of pair p).
-// Leaves M.enabled false (the caller keeps the FP64 M2L) when the two trees
-// differ, an offset is not integral, blk exceeds what one workgroup stages, or
-// blk is below 48 (where the FP64 stage is faster).
+// Leaves M.enabled false (the caller keeps the FP64 M2L) when the trees' cells are
+// not on one grid (same_grid false: distinct trees on their own root boxes), an
+// offset is not integral, or m2l_mfma_size_ok(blk) is false.
 // up_fac, down_fac: optionally {W, V}, the SVD factors of UC2E and DC2E per level
 // (from the pseudo-inverse's own SVD, gpu_pinv_batched); without them the C2E
 // matrices are factored again.
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
+                    bool same_grid, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
                     const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac = nullptr,
                     const DevBuf<double>* down_fac = nullptr);
 

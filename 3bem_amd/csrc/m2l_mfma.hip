@@ -292,21 +292,26 @@ struct ClassHash {
 
 }  // namespace
 
+bool m2l_mfma_size_ok(int n) {
+    if (n > kMaxBlk || n < kMinBlk) return false;
+    // the staged columns must fit one workgroup's LDS on this device
+    const size_t shm = (size_t)((n + 3) / 4) * 4 * kUnitCols * sizeof(float);
+    int dev = 0, max_shm = 0;
+    TBEM_HIP(hipGetDevice(&dev));
+    TBEM_HIP(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    return shm <= (size_t)max_shm;
+}
+
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    bool same_tree, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
+                    bool same_grid, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
                     const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac,
                     const DevBuf<double>* down_fac) {
     auto t0 = std::chrono::steady_clock::now();
     M = M2lMfma{};
     const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
     M.blk = n;
-    if (!same_tree || n > kMaxBlk || n < kMinBlk || m2l.cell.empty()) return;
-    // the staged columns must fit one workgroup's LDS on this device
+    if (!same_grid || !m2l_mfma_size_ok(n) || m2l.cell.empty()) return;
     const size_t shm = (size_t)((n + 3) / 4) * 4 * kUnitCols * sizeof(float);
-    int dev = 0, max_shm = 0;
-    TBEM_HIP(hipGetDevice(&dev));
-    TBEM_HIP(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    if (shm > (size_t)max_shm) return;
     if (shm > 65536) {
         TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4, kPreNarrow>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)shm));

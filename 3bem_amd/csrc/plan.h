@@ -32,8 +32,14 @@ struct HostTree {
 };
 
 // Builds the octree of n AoS points on the GPU (Morton keys + radix sort), then
-// assembles cells from the sorted key runs on the host.
-HostTree build_octree(const double* pts_host, int n, int min_pts_per_cell, hipStream_t s);
+// assembles cells from the sorted key runs on the host. The root box is the points'
+// tight bounding box (the reference's), or box_lo_hi (lo xyz, hi xyz) when given,
+// which must contain every point: two trees built on one box put their cells on one
+// grid (same centres and half widths level by level).
+HostTree build_octree(const double* pts_host, int n, int min_pts_per_cell, hipStream_t s,
+                      const double* box_lo_hi = nullptr);
+// Tight bounding box of n AoS points: lo xyz, hi xyz.
+void point_bounds(const double* pts_host, int n, double* lo_hi);
 
 // Translation surfaces (cpp/fmm.h:42-66, cpp/fmm.cpp:22-51).
 struct Surfaces {

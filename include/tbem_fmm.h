@@ -104,7 +104,18 @@ typedef struct {
     size_t n_up_levels;
     const double* down_check_to_equiv;
     size_t n_down_levels;
+
+    /* Boxes of distinct observation and source trees (no reference counterpart;
+     * ignored when one point set serves both sides): TBEM_TREES_AUTO (0, the
+     * default) builds them as the reference does, each on its own points' bounding
+     * box (cpp/fmm.cpp:88-89), except in TBEM_MIXED precision with an operator the
+     * MFMA M2L serves (48 <= R*n_e <= 512), where both go on the union box so that
+     * their cells lie on one grid and the M2L pairs fall into translation classes;
+     * TBEM_TREES_OWN_BOXES and TBEM_TREES_COMMON_BOX force either layout. */
+    int tree_boxes;
 } tbem_fmm_desc;
+
+enum { TBEM_TREES_AUTO = 0, TBEM_TREES_OWN_BOXES = 1, TBEM_TREES_COMMON_BOX = 2 };
 
 /* Plan statistics (tbem_fmm_info). */
 typedef struct {
@@ -115,6 +126,10 @@ typedef struct {
     double p2p_pairs;        /* sum over P2P tasks of n_obs*n_src */
     double m2l_classes;      /* distinct translation classes (MIXED) */
     double plan_seconds;     /* wall time of tbem_fmm_create */
+    double m2l_output_bytes; /* the MFMA M2L's per-pair output (MIXED): one buffer per plan,
+                                shared by its clones, whose M2L sections take turns on it */
+    double work_bytes;       /* device buffers of this handle's own (each clone has its own;
+                                the adjoint and gradient add theirs on first use) */
 } tbem_fmm_info_t;
 
 /* ---------------------------------------------------------------- FMM operator */
@@ -174,8 +189,22 @@ int tbem_fmm_obs_grad_reuse_dev(tbem_fmm_t h, uint64_t token, const double* d_x,
                                 double* d_grad, void* stream);
 
 /* Move observation points without rebuilding the plan (same tree cells, same
- * surfaces); for finite-difference checks of tbem_fmm_obs_grad. */
+ * surfaces, same lists); for the differentiable forward model and for
+ * finite-difference checks of tbem_fmm_obs_grad. No reference counterpart (3bem
+ * rebuilds FMMOperator for new points, cpp/fmm.cpp:71-101). obs_locs is AoS,
+ * 3*n_obs, in the caller's order. Every moved point must stay inside the ball of
+ * the leaf it was sorted into (centre, radius |half width|: the cell as the
+ * dual-tree MAC test sees it, cpp/fmm.cpp:577-586): otherwise the call returns
+ * TBEM_E_INVALID and changes nothing. The move applies to every handle that shares
+ * the plan (clones): it waits for their calls in flight and blocks new ones until
+ * it returns. _dev takes a device pointer and runs on the caller's stream. */
 int tbem_fmm_set_obs_locs(tbem_fmm_t h, const double* obs_locs);
+int tbem_fmm_set_obs_locs_dev(tbem_fmm_t h, const double* d_obs_locs, void* stream);
+
+/* Counter of the plan's observation-point moves (0 for a new plan; shared by
+ * clones): a caller that keeps a result can tell whether the points it was
+ * computed at are still the plan's. No reference counterpart. */
+uint64_t tbem_fmm_obs_version(tbem_fmm_t h);
 
 /* OperatorI::n_rows / n_cols (cpp/fmm.h:235-236). */
 size_t tbem_fmm_n_rows(tbem_fmm_t h);
@@ -342,6 +371,12 @@ int tbem_condense_sparse(tbem_constraints_t row_cm, tbem_constraints_t col_cm, t
 void tbem_constraints_destroy(tbem_constraints_t h);
 
 /* ----------------------------------------------------------------------- misc */
+
+/* sqrt(a[i]) and a[i] / b[i] evaluated on the GPU exactly as the device code of
+ * the reference-order kernel tensors evaluates them. No reference counterpart: a
+ * parity aid that lets a test check that gfx950's FP64 square root and division
+ * are correctly rounded (the reference's C2E entries depend on it bit for bit). */
+int tbem_fp64_sqrt_div(size_t n, const double* a, const double* b, double* sqrt_a, double* a_div_b);
 
 /* One kernel evaluation K(obs <- src) on the GPU, R*C doubles row-major
  * (Kernel<3,R,C>::operator(), cpp/kernel.h:19-33). For kernel parity tests. */

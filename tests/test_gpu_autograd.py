@@ -106,3 +106,58 @@ def test_autograd_matches_fp64_direct_sum_gradient(tbem, orc, problem_u):
         du /= 2 * h
         go_ref[:, m] = sum(r[q * n:(q + 1) * n] * du[q * n:(q + 1) * n] for q in range(3))
     assert rel_err(go, go_ref) < 1e-3, rel_err(go, go_ref)
+
+
+def test_autograd_backward_uses_its_own_forward_stations(tbem, problem):
+    """loss = f(obs1) + f(obs2) on one operator: the second forward moves the shared
+    stations, so the first term's backward must move them back. Both gradients
+    equal the sums of the single-call gradients, bit for bit."""
+    p = problem
+    torch = p["torch"]
+    ag = importlib.import_module("3bem_amd.autograd")
+    obs1 = p["g"]["obs"].copy()
+    obs2 = obs1.copy()
+    obs2[:, 2] += 1e-6 * np.sin(np.arange(obs1.shape[0]))  # stays inside every leaf cell
+    print("CASE autograd two forwards", flush=True)
+    L1, gs1, go1 = loss_and_grads(tbem, p, obs1)
+    L2, gs2, go2 = loss_and_grads(tbem, p, obs2)
+    slip = torch.tensor(p["g"]["x"], device="cuda", requires_grad=True)
+    o1 = torch.tensor(obs1, device="cuda", requires_grad=True)
+    o2 = torch.tensor(obs2, device="cuda", requires_grad=True)
+    L = ag.forward_model_loss(p["op"], slip, p["u_obs"], o1) + ag.forward_model_loss(p["op"], slip, p["u_obs"], o2)
+    L.backward()
+    assert abs(L.item() - (L1 + L2)) <= 1e-14 * abs(L1 + L2)
+    np.testing.assert_array_equal(o1.grad.cpu().numpy(), go1)
+    np.testing.assert_array_equal(o2.grad.cpu().numpy(), go2)
+    assert rel_err(slip.grad.cpu().numpy(), gs1 + gs2) < 1e-15
+    p["op"].set_obs_locs(obs1)
+
+
+def test_set_obs_locs_device_and_leaf_box_check(tbem, problem):
+    """Moving stations from a device tensor equals moving them from the host; a
+    station pushed out of its leaf cell is refused and nothing moves."""
+    p = problem
+    torch = p["torch"]
+    op = p["op"]
+    obs = p["g"]["obs"].copy()
+    x = p["g"]["x"]
+    print("CASE set_obs_locs device + box check", flush=True)
+    moved = obs.copy()
+    moved[:, 0] += 1e-7 * np.cos(np.arange(obs.shape[0]))
+    v0 = op.obs_version()
+    op.set_obs_locs(moved)
+    y_host = op.apply(x)
+    op.set_obs_locs(obs)
+    op.set_obs_locs(torch.tensor(moved, device="cuda"))
+    assert op.obs_version() == v0 + 3
+    np.testing.assert_array_equal(op.apply(x), y_host)
+    cells, _ = op.octree("obs")
+    span = cells[0, 4:7]  # root half widths
+    far = moved.copy()
+    far[7, :] += 4.0 * span  # outside the root cell, hence outside its leaf
+    for arg in (far, torch.tensor(far, device="cuda")):
+        with pytest.raises(tbem.TbemError, match="outside their leaf cells"):
+            op.set_obs_locs(arg)
+        np.testing.assert_array_equal(op.apply(x), y_host)  # still at `moved`
+    assert op.obs_version() == v0 + 3
+    op.set_obs_locs(obs)

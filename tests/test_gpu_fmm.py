@@ -48,8 +48,42 @@ def test_gpu_kernels_vs_reference_sympy_rows(tbem):
                 assert abs(got - r["value"]) / ((abs(got) + abs(r["value"])) / 2) < 1e-8
 
 
+def test_gpu_fp64_sqrt_and_division_correctly_rounded(tbem):
+    """gfx950's FP64 square root and division, as the library's device code evaluates
+    them, equal numpy's IEEE correctly rounded results bit for bit: on random values
+    over 40 binades, near perfect squares and ulp neighbours of exact quotients,
+    subnormals, powers of two and the values where a faithful (1-ulp) result would
+    round the other way."""
+    import ctypes
+    from importlib import import_module
+    lib = import_module("3bem_amd._lib").require_device()
+    rng = np.random.default_rng(17)
+    n = 1 << 20
+    a = rng.uniform(0.5, 1.0, n) * np.exp2(rng.integers(-20, 20, n).astype(np.float64))
+    b = rng.uniform(0.5, 1.0, n) * np.exp2(rng.integers(-20, 20, n).astype(np.float64))
+    k = rng.uniform(1, 2 ** 26, 4096).round()
+    sq = k * k  # perfect squares, then their ulp neighbours
+    a[:4096], a[4096:8192], a[8192:12288] = sq, np.nextafter(sq, 0), np.nextafter(sq, np.inf)
+    q = rng.uniform(1, 2 ** 20, 4096).round()
+    b[12288:16384] = q
+    a[12288:16384] = np.nextafter(q * 3.0, np.inf)  # quotients just above an exact 3
+    a[16384:16400] = np.exp2(np.arange(-1074, -1058, dtype=np.float64))  # subnormals
+    a[16400:16500] = np.exp2(rng.integers(-1000, 1000, 100).astype(np.float64))
+    b[16400:16500] = np.exp2(rng.integers(-20, 20, 100).astype(np.float64))
+    P = lambda v: v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    gs, gq = np.zeros(n), np.zeros(n)
+    case("fp64 sqrt/div rounding", n)
+    assert lib.tbem_fp64_sqrt_div(n, P(a), P(b), P(gs), P(gq)) == 0
+    bad_s = np.flatnonzero(gs.view(np.uint64) != np.sqrt(a).view(np.uint64))
+    bad_q = np.flatnonzero(gq.view(np.uint64) != (a / b).view(np.uint64))
+    assert bad_s.size == 0, [(a[i], gs[i], np.sqrt(a[i])) for i in bad_s[:5]]
+    assert bad_q.size == 0, [(a[i], b[i], gq[i], a[i] / b[i]) for i in bad_q[:5]]
+
+
 def test_gpu_kernel_tensor_matches_oracle_bitwise(tbem, orc):
-    """The GPU's reference-order tensor (used for C2E) equals the oracle's entry by entry."""
+    """The GPU's reference-order tensor (used for C2E) equals the oracle's entry by
+    entry, bit for bit: the same expression order with contraction off, and correctly
+    rounded square roots and divisions (previous test)."""
     rng = np.random.default_rng(11)
     n = 200
     o, s = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
@@ -59,7 +93,8 @@ def test_gpu_kernel_tensor_matches_oracle_bitwise(tbem, orc):
         K = tbem.make_kernel(kid, 1.7, 0.3)
         g = tbem.kernel_eval(K, o, s, no, ns)
         c = np.stack([orc.kernel(kid, o[i], s[i], no[i], ns[i], 1.7, 0.3) for i in range(n)])
-        assert np.max(np.abs(g - c) / np.maximum(np.abs(c), 1e-300)) <= 2.3e-16
+        bad = np.argwhere(g.view(np.uint64) != c.view(np.uint64))
+        assert bad.size == 0, (kid, [(tuple(i), g[tuple(i)], c[tuple(i)]) for i in bad[:5]])
 
 
 # --------------------------------------------------------------------- direct sum
@@ -269,7 +304,7 @@ def test_gpu_obs_grad_reuses_forward_far_field(tbem, wl, prec):
 
 # ------------------------------------------------------------- leaf-stage shapes
 @pytest.mark.parametrize("leaf", [1, 43, 86, 171, 300, 700, 3000])
-@pytest.mark.parametrize("kid", [0, 3])
+@pytest.mark.parametrize("kid", [0, 3, 6])
 def test_gpu_leaf_workgroup_shapes_match_oracle(tbem, orc, kid, leaf):
     """FP64 FMM against the oracle at leaf sizes that drive every shape of the
     leaf stage's cross-slice sum: many slices summed in groups (small leaves), one
@@ -292,6 +327,32 @@ def test_gpu_leaf_workgroup_shapes_match_oracle(tbem, orc, kid, leaf):
     xt = op.apply_adjoint(gg)
     scale = np.abs(y).max() * np.abs(gg).sum() + np.abs(x).max() * np.abs(xt).sum()
     assert abs(y @ gg - x @ xt) < 1e-13 * scale
+
+
+@pytest.mark.parametrize("leaf", [1, 43, 171, 300, 700, 3000])
+@pytest.mark.parametrize("kid", [3, 4, 5, 6])
+def test_gpu_mixed_leaf_workgroup_shapes(tbem, kid, leaf):
+    """The FP32 leaf stage (Neumaier-compensated tiles) over the same shapes: the
+    256-thread instances (traction and adjoint traction in FP32, hypersingular) and
+    the 512-thread elastic-U one, from many slices per target down to one (S = 1 from
+    257 targets on 256 threads) and leaves of more than 512 targets, which take
+    several passes. Against the FP64 HIP path (itself pinned to the oracle above and
+    by the golden fixtures), plus the adjoint identity of the mixed operator."""
+    g = random_set(2400, 2000, 80 + kid)
+    R = components(kid)
+    rng = np.random.default_rng(kid + 7 * leaf)
+    x, gg = rng.standard_normal(R * 2000), rng.standard_normal(R * 2400)
+    case("mixed leaf shapes", kid, leaf)
+    d = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
+    K = tbem.make_kernel(kid, 1.0, 0.25)
+    cfg = tbem.FMMConfig(0.3, 5, leaf, 0.05, False)
+    op = tbem.FMMOperator(K, d, cfg, precision="mixed")
+    y = op.apply(x)
+    y64 = tbem.FMMOperator(K, d, cfg).apply(x)
+    assert rel_err(y, y64) < 1e-5, rel_err(y, y64)
+    xt = op.apply_adjoint(gg)
+    scale = np.abs(y).max() * np.abs(gg).sum() + np.abs(x).max() * np.abs(xt).sum()
+    assert abs(y @ gg - x @ xt) < 1e-6 * scale
 
 
 # ----------------------------------------------------------------- obs gradient

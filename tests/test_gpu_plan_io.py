@@ -42,6 +42,16 @@ def test_saved_plan_reproduces_operator(tbem, prec, shared):
         assert np.array_equal(c1, c2) and np.array_equal(p1, p2)
     assert np.array_equal(op.c2e(False, 1), op2.c2e(False, 1))
     assert op2.save_plan() == blob
+    if prec == "mixed":
+        assert i1["m2l_classes"] > 0  # distinct trees too: both on the union box
+    if not shared:
+        # a plan's trees were laid out for its precision's boxes: the other layout refuses it
+        other = "fp64" if prec == "mixed" else "mixed"
+        with pytest.raises(tbem.TbemError, match="other boxes"):
+            tbem.FMMOperator(K, data, cfg, precision=other, plan=blob)
+        op3 = tbem.FMMOperator(K, data, cfg, precision=other, tree_boxes="common" if prec == "mixed" else "own",
+                               plan=blob)
+        assert op3.info()["n_obs_cells"] == i1["n_obs_cells"] and op3.info()["p2p"] == i1["p2p"]
 
 
 def test_saved_plan_refusals(tbem):
@@ -67,6 +77,75 @@ def test_saved_plan_refusals(tbem):
         tbem.FMMOperator(K, data, cfg, plan=blob[:-8])
     with pytest.raises(tbem.TbemError, match="magic"):
         tbem.FMMOperator(K, data, cfg, plan=b"NOTAPLAN" + blob[8:])
-    op.set_obs_locs(data.obs_locs + 1e-3)
+    op.set_obs_locs(data.obs_locs)  # a move, even to the same coordinates
     with pytest.raises(tbem.TbemError, match="moved"):
         op.save_plan()
+
+
+M64 = (1 << 64) - 1
+
+
+def _rotl(v, r):
+    return ((v << r) | (v >> (64 - r))) & M64
+
+
+def _mix_word(h, w):
+    w = (w * 0xC2B2AE3D27D4EB4F) & M64
+    w = _rotl(w, 31)
+    x = h ^ ((w * 0x9E3779B185EBCA87) & M64)
+    return (_rotl(x, 27) * 5 + 0x52DCE729) & M64
+
+
+def plan_hash(b, h=0x9E3779B97F4A7C15):
+    """The saved plan's 64-bit payload checksum (four multiply-rotate lanes per 32
+    bytes, then the tail words and bytes, then a final avalanche), written out here
+    so that a test can re-seal a payload it has edited."""
+    n = len(b)
+    words = np.frombuffer(b[: n - n % 8], dtype="<u8").tolist()
+    lane = [h, h + 1, h + 2, h + 3]
+    i = 0
+    while 4 * (i + 1) <= len(words) and 32 * (i + 1) <= n:
+        for k in range(4):
+            lane[k] = _mix_word(lane[k], words[4 * i + k])
+        i += 1
+    h = _mix_word(_mix_word(_mix_word(_mix_word(h ^ n, lane[0]), lane[1]), lane[2]), lane[3])
+    for w in words[4 * i:]:
+        h = _mix_word(h, w)
+    for c in b[len(words) * 8:]:
+        h = ((h ^ c) * 0x100000001B3) & M64
+    h ^= h >> 33
+    h = (h * 0xFF51AFD7ED558CCD) & M64
+    h ^= h >> 33
+    return h
+
+
+def test_saved_plan_with_bad_parent_link_is_refused(tbem):
+    """A payload whose checksum is right but whose octree names a wrong parent for
+    one cell (parent -1 below the root, or a cell of the same level) is refused while
+    the plan is read on the host; nothing is applied."""
+    data, _ = problem(tbem, 7, None, 6_000)
+    K = tbem.LaplaceSingle()
+    cfg = tbem.FMMConfig(0.3, 8, 64, 0.05, False)
+    print("CASE plan io bad parent", flush=True)
+    blob = tbem.FMMOperator(K, data, cfg).save_plan()
+    payload = bytearray(blob[24:])
+    assert plan_hash(bytes(payload)) == int.from_bytes(blob[16:24], "little")  # the checksum as written
+    off = 84 + 12  # header fields, then n_pts, n_cells, n_levels of the source tree
+    vecs = []
+    for _ in range(7):  # level, begin, count, first_child, n_child, parent, octant
+        cnt = int.from_bytes(payload[off:off + 8], "little")
+        vecs.append((off + 8, cnt))
+        off += 8 + 4 * cnt
+    (lvl_off, n_cells), (par_off, _) = vecs[0], vecs[5]
+    level = np.frombuffer(bytes(payload[lvl_off:lvl_off + 4 * n_cells]), dtype="<i4")
+    parent = np.frombuffer(bytes(payload[par_off:par_off + 4 * n_cells]), dtype="<i4")
+    c = int(np.flatnonzero(level == 2)[0])
+    same_level = int(np.flatnonzero(level == 2)[1])
+    for bad in (-1, same_level, int(parent[c]) + 1 if level[int(parent[c]) + 1] == 1 else 0):
+        if bad == parent[c]:
+            continue
+        p = bytearray(payload)
+        p[par_off + 4 * c:par_off + 4 * c + 4] = int(bad).to_bytes(4, "little", signed=True)
+        sealed = blob[:16] + plan_hash(bytes(p)).to_bytes(8, "little") + bytes(p)
+        with pytest.raises(tbem.TbemError, match="parent and child links"):
+            tbem.FMMOperator(K, data, cfg, plan=sealed)

@@ -55,9 +55,11 @@ def divergent_barrier_loops(asm_text):
             finish()
             fn, blocks, parent, cur = m.group(1), {"entry": (None, [])}, {}, "entry"
             continue
-        m = re.match(r"^\.(LBB\w+):(.*)$", line)
+        # labelled blocks (.LBB..) and unlabelled fall-through blocks ("; %bb.N:"),
+        # which carry their own loop annotations
+        m = re.match(r"^(?:\.(LBB\w+)|; %(bb\.\d+)):(.*)$", line)
         if m and fn:
-            cur, info = m.group(1), m.group(2)
+            cur, info = m.group(1) or m.group(2), m.group(3)
             h = None
             mh = re.search(r"Header=(BB\w+) Depth=", info)
             if mh:
@@ -84,6 +86,15 @@ _Z3badv:
 \ts_andn2_b64 exec, exec, vcc
 \ts_cbranch_execz .LBB0_2
 .LBB0_2:
+\ts_endpgm
+_Z5afterv:
+.LBB2_1:                                ; =>This Inner Loop Header: Depth=1
+\tv_add_u32_e32 v1, 1, v1
+\ts_andn2_b64 exec, exec, vcc
+\ts_cbranch_execnz .LBB2_1
+; %bb.2:
+\ts_or_b64 exec, exec, s[0:1]
+\ts_barrier
 \ts_endpgm
 _Z4goodv:
 .LBB1_1:                                ; =>This Loop Header: Depth=1
