@@ -4,6 +4,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <typeinfo>
 
 #include "elastic_kernels.h"
 #include "identity_kernels.h"
@@ -16,33 +17,38 @@ static void check(int rc) {
     if (rc != TBEM_OK) throw std::runtime_error(std::string("libtbem_fmm: ") + tbem_last_error());
 }
 
+// Exact types only: a user subclass of one of these kernels may override call(), so
+// it stays with the reference's own operator.
 template <size_t R, size_t C>
 int mi355x_kernel_id(const Kernel<3, R, C>& K, double* mu, double* nu) {
-    if (dynamic_cast<const LaplaceSingle<3>*>(&K)) return TBEM_K_LAPLACE_SINGLE;
-    if (dynamic_cast<const LaplaceDouble<3>*>(&K)) return TBEM_K_LAPLACE_DOUBLE;
-    if (dynamic_cast<const LaplaceHypersingular<3>*>(&K)) return TBEM_K_LAPLACE_HYPERSINGULAR;
+    const std::type_info& t = typeid(K);
+    if (t == typeid(LaplaceSingle<3>)) return TBEM_K_LAPLACE_SINGLE;
+    if (t == typeid(LaplaceDouble<3>)) return TBEM_K_LAPLACE_DOUBLE;
+    if (t == typeid(LaplaceHypersingular<3>)) return TBEM_K_LAPLACE_HYPERSINGULAR;
+    if (t == typeid(IdentityTensor<3, R, C>)) return R == 1 ? TBEM_K_IDENTITY_SCALAR : TBEM_K_IDENTITY_TENSOR;
+    const int id = t == typeid(ElasticDisplacement<3>)      ? TBEM_K_ELASTIC_DISPLACEMENT
+                   : t == typeid(ElasticTraction<3>)        ? TBEM_K_ELASTIC_TRACTION
+                   : t == typeid(ElasticAdjointTraction<3>) ? TBEM_K_ELASTIC_ADJOINT_TRACTION
+                   : t == typeid(ElasticHypersingular<3>)   ? TBEM_K_ELASTIC_HYPERSINGULAR
+                                                            : -1;
+    if (id < 0) return -1;
+    // the four elastic kernels keep mu and nu in the same public members
+    // (cpp/elastic_kernels.h:194-204, 233-243, 275-285, 317-329)
+    // (dynamic_cast: the scalar instantiation has these types only as unrelated classes)
     if (auto e = dynamic_cast<const ElasticDisplacement<3>*>(&K)) {
         *mu = e->shear_modulus;
         *nu = e->poisson_ratio;
-        return TBEM_K_ELASTIC_DISPLACEMENT;
-    }
-    if (auto e = dynamic_cast<const ElasticTraction<3>*>(&K)) {
+    } else if (auto e = dynamic_cast<const ElasticTraction<3>*>(&K)) {
         *mu = e->shear_modulus;
         *nu = e->poisson_ratio;
-        return TBEM_K_ELASTIC_TRACTION;
-    }
-    if (auto e = dynamic_cast<const ElasticAdjointTraction<3>*>(&K)) {
+    } else if (auto e = dynamic_cast<const ElasticAdjointTraction<3>*>(&K)) {
         *mu = e->shear_modulus;
         *nu = e->poisson_ratio;
-        return TBEM_K_ELASTIC_ADJOINT_TRACTION;
-    }
-    if (auto e = dynamic_cast<const ElasticHypersingular<3>*>(&K)) {
+    } else if (auto e = dynamic_cast<const ElasticHypersingular<3>*>(&K)) {
         *mu = e->shear_modulus;
         *nu = e->poisson_ratio;
-        return TBEM_K_ELASTIC_HYPERSINGULAR;
     }
-    if (dynamic_cast<const IdentityTensor<3, R, C>*>(&K)) return R == 1 ? TBEM_K_IDENTITY_SCALAR : TBEM_K_IDENTITY_TENSOR;
-    return -1;
+    return id;
 }
 
 // constructor body of cpp/fmm.cpp:71-101: the NBodyData vectors of std::array<double,3>
