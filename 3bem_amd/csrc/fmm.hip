@@ -50,17 +50,25 @@ struct FmmPlan {
     std::vector<DevList> m2m;    // by source level: parents at L, entries = children
     std::vector<DevList> l2l;    // by observation level L >= 1: cells at L, entry = parent
     DevBuf<int> obs_leaves, src_leaves;
-    // adjoint lists
-    DevList a_l2p;               // obs leaves, entry = itself
-    std::vector<DevList> a_l2l;  // by obs level: parents at L, entries = children
-    DevList a_m2l, a_m2p;        // items = source cells
-    std::vector<DevList> a_m2m;  // by src level L >= 1: cells at L, entry = parent
-    DevList a_leaf_p2p, a_leaf_p2l;  // indexed by source leaf position
+    // adjoint lists: built on the first adjoint call (adjoint_lists), since a
+    // forward-only caller never needs them and they cost a second pass over every
+    // task list (0.9 s of a 4M-point plan)
+    struct AdjLists {
+        DevList l2p;                 // obs leaves, entry = itself
+        std::vector<DevList> l2l;    // by obs level: parents at L, entries = children
+        DevList m2l, m2p;            // items = source cells
+        std::vector<DevList> m2m;    // by src level L >= 1: cells at L, entry = parent
+        DevList leaf_p2p, leaf_p2l;  // indexed by source leaf position
+        std::vector<CellTiles> t_slev;  // every cell of each source level
+    };
+    mutable std::mutex adj_mu;
+    mutable std::unique_ptr<AdjLists> adj;
+    const AdjLists& adjoint_lists(hipStream_t s) const;
     M2lMfma m2l_mfma;            // mixed precision: M2L as grouped FP32 MFMA GEMMs
     // GEMM tiles of the C2E products: the cells each stage writes
     CellTiles t_p2m, t_m2l, t_p2l;
     std::vector<CellTiles> t_m2m;          // by source level: the M2M parents
-    std::vector<CellTiles> t_olev, t_slev; // every cell of each observation / source level
+    std::vector<CellTiles> t_olev;         // every cell of each observation level
     double p2p_pairs = 0;
     double plan_seconds = 0;
     // serialization: the inputs' hash, the mixed path's C2E SVD factors (host copies),
@@ -167,26 +175,44 @@ namespace tbem {
 
 // ------------------------------------------------------------------ list builders
 // items = cells that accumulate; pairs (a, b): the item is pair[col], the entry pair[1-col].
+// Entries of an item keep the pairs' order. Host threads take contiguous chunks of
+// the pairs: per-chunk counts, then each chunk fills its slots after the earlier
+// chunks' (the same result as one pass in order).
 static HostList list_from_pairs(const std::vector<int>& items, int n_cells_item_tree,
                                 const std::vector<std::array<int, 2>>& pairs, int col,
                                 const HostTree* pts_tree) {
     HostList l;
     l.items = items;
+    const size_t ni = items.size(), np = pairs.size();
     std::vector<int> pos(n_cells_item_tree, -1);
-    for (size_t k = 0; k < items.size(); k++) pos[items[k]] = (int)k;
-    std::vector<int> cnt(items.size() + 1, 0);
-    for (auto& p: pairs) {
-        int k = pos[p[col]];
-        if (k >= 0) cnt[k + 1]++;
+    for (size_t k = 0; k < ni; k++) pos[items[k]] = (int)k;
+    const size_t nch = np < (size_t(1) << 20) ? 1 : (size_t)plan_threads();
+    const size_t per = (np + nch - 1) / std::max<size_t>(nch, 1);
+    std::vector<std::vector<int>> cnt(nch, std::vector<int>(ni, 0));
+    parallel_for(nch, [&](size_t c) {
+        for (size_t i = c * per; i < std::min(np, (c + 1) * per); i++) {
+            const int k = pos[pairs[i][col]];
+            if (k >= 0) cnt[c][k]++;
+        }
+    });
+    l.ptr.assign(ni + 1, 0);
+    for (size_t k = 0; k < ni; k++) {
+        int run = l.ptr[k];
+        for (size_t c = 0; c < nch; c++) {
+            const int v = cnt[c][k];
+            cnt[c][k] = run;  // chunk c's first slot of item k
+            run += v;
+        }
+        l.ptr[k + 1] = run;
     }
-    l.ptr.assign(items.size() + 1, 0);
-    for (size_t k = 0; k < items.size(); k++) l.ptr[k + 1] = l.ptr[k] + cnt[k + 1];
     l.cell.resize(l.ptr.back());
-    std::vector<int> fill(l.ptr.begin(), l.ptr.end() - 1);
-    for (auto& p: pairs) {
-        int k = pos[p[col]];
-        if (k >= 0) l.cell[fill[k]++] = p[1 - col];
-    }
+    parallel_for(nch, [&](size_t c) {
+        std::vector<int>& fill = cnt[c];
+        for (size_t i = c * per; i < std::min(np, (c + 1) * per); i++) {
+            const int k = pos[pairs[i][col]];
+            if (k >= 0) l.cell[fill[k]++] = pairs[i][1 - col];
+        }
+    });
     if (pts_tree) cells_to_ranges(l, *pts_tree);
     return l;
 }
@@ -675,8 +701,8 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->t_m2l = make_cell_tiles(m2l_h.items, OT, s);
     clk.mark("lists");
     if (P->precision == TBEM_MIXED)
-        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, P->tree_layout != kTreesOwnBoxes, m2l_h, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac,
-                       down_fac);
+        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, P->dot, P->dst, P->tree_layout != kTreesOwnBoxes, m2l_h,
+                       P->m2l, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac, down_fac);
     clk.mark("m2l_mfma");
     TBEM_HIP(hipEventCreateWithFlags(&P->y_done, hipEventDisableTiming));
     TBEM_HIP(hipEventRecord(P->y_done, s));
@@ -687,8 +713,6 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->t_p2l = make_cell_tiles(p2l_items, OT, s);
     P->t_olev.resize(OT.n_levels);
     for (int L = 0; L < OT.n_levels; L++) P->t_olev[L] = make_level_tiles(OT, L, s);
-    P->t_slev.resize(ST.n_levels);
-    for (int L = 0; L < ST.n_levels; L++) P->t_slev[L] = make_level_tiles(ST, L, s);
     P->l2l.resize(OT.n_levels);
     for (int L = 1; L < OT.n_levels; L++) P->l2l[L].upload(parent_list(OT, L), s);
     P->obs_leaves.upload(OT.leaves, s);
@@ -696,21 +720,35 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->leaf_p2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.p2p, 0, &ST), s);
     P->leaf_m2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.m2p, 0, nullptr), s);
 
-    // transposed lists for the adjoint
-    P->a_l2p.upload(self_list(OT, OT.leaves), s);
-    P->a_l2l.resize(OT.n_levels);
-    for (int L = 0; L + 1 < OT.n_levels; L++) P->a_l2l[L].upload(children_list(OT, L), s);
-    P->a_m2l.upload(list_from_pairs(cells_with(tl.m2l, 1, ST.n_cells), ST.n_cells, tl.m2l, 1, nullptr), s);
-    P->a_m2p.upload(list_from_pairs(cells_with(tl.m2p, 1, ST.n_cells), ST.n_cells, tl.m2p, 1, &OT), s);
-    P->a_m2m.resize(ST.n_levels);
-    for (int L = 1; L < ST.n_levels; L++) P->a_m2m[L].upload(parent_list(ST, L), s);
-    P->a_leaf_p2p.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, &OT), s);
-    P->a_leaf_p2l.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr), s);
-
     clk.mark("rest");
     TBEM_HIP(hipStreamSynchronize(s));
     P->plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return P;
+}
+
+// The transposed lists of the adjoint, built once per plan on the first call that
+// needs them (on that call's stream, which is then synchronized: another handle of
+// the plan may use them on another stream right after).
+const FmmPlan::AdjLists& FmmPlan::adjoint_lists(hipStream_t s) const {
+    std::lock_guard<std::mutex> lk(adj_mu);
+    if (adj) return *adj;
+    std::unique_ptr<AdjLists> A(new AdjLists);
+    const HostTree& ST = st;
+    const HostTree& OT = ot;
+    A->l2p.upload(self_list(OT, OT.leaves), s);
+    A->l2l.resize(OT.n_levels);
+    for (int L = 0; L + 1 < OT.n_levels; L++) A->l2l[L].upload(children_list(OT, L), s);
+    A->m2l.upload(list_from_pairs(cells_with(tl.m2l, 1, ST.n_cells), ST.n_cells, tl.m2l, 1, nullptr), s);
+    A->m2p.upload(list_from_pairs(cells_with(tl.m2p, 1, ST.n_cells), ST.n_cells, tl.m2p, 1, &OT), s);
+    A->m2m.resize(ST.n_levels);
+    for (int L = 1; L < ST.n_levels; L++) A->m2m[L].upload(parent_list(ST, L), s);
+    A->leaf_p2p.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, &OT), s);
+    A->leaf_p2l.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr), s);
+    A->t_slev.resize(ST.n_levels);
+    for (int L = 0; L < ST.n_levels; L++) A->t_slev[L] = make_level_tiles(ST, L, s);
+    TBEM_HIP(hipStreamSynchronize(s));
+    adj = std::move(A);
+    return *adj;
 }
 
 // --------------------------------------------------------------------- stage glue
@@ -910,6 +948,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     if (W.mbar.n < ncs) { W.mbar.alloc(ncs); W.ebar.alloc(ncs); }
     if (W.dens.n < (size_t)R * Ns) W.dens.alloc((size_t)R * Ns);
     const bool mfma = std::is_same<T, float>::value && P.m2l_mfma.enabled;
+    const FmmPlan::AdjLists& A = P.adjoint_lists(s);
     if (mfma && W.e2.n < ncs) { W.e2.alloc(ncs); W.m2l_w.alloc(nco); }
     launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
     launch_fill(W.lbar.p, nco, 0.0, s);
@@ -919,7 +958,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
 
     // L2P^T: observation points -> down-equivalent surface of their leaf
     SurfStage a = surf_base(P);
-    set_list(a, P.a_l2p);
+    set_list(a, A.l2p);
     a.tt = view(P.dot); a.t_surf = P.d_outer.p;
     a.st = view(P.dot); a.sp = view(P.op); a.dens = W.gsort.p; a.n_src_pts = No;
     a.out = W.lbar.p;
@@ -930,7 +969,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         launch_level_gemm(P.t_olev[L], P.blk, P.dc2e.p, true, W.lbar.p, nullptr, W.cbar.p, nullptr, false, s);
         if (L == 0) break;
         SurfStage l = surf_base(P);
-        set_list(l, P.a_l2l[L - 1]);
+        set_list(l, A.l2l[L - 1]);
         l.tt = view(P.dot); l.t_surf = P.d_outer.p;
         l.st = view(P.dot); l.s_surf = P.d_inner.p; l.s_coeff = W.cbar.p;
         l.out = W.lbar.p;
@@ -946,7 +985,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         launch_m2l_reduce_adj(P.m2l_mfma, P.m2l_y.p, W.fld.p, W.e2.p, s);
     } else {
         SurfStage m = surf_base(P);
-        set_list(m, P.a_m2l);
+        set_list(m, A.m2l);
         m.tt = view(P.dst); m.t_surf = P.d_inner.p;
         m.st = view(P.dot); m.s_surf = P.d_inner.p; m.s_coeff = W.cbar.p;
         m.out = W.mbar.p;
@@ -954,7 +993,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     }
     // M2P^T into mbar
     SurfStage mp = surf_base(P);
-    set_list(mp, P.a_m2p);
+    set_list(mp, A.m2p);
     mp.tt = view(P.dst); mp.t_surf = P.d_inner.p;
     mp.st = view(P.dot); mp.sp = view(P.op); mp.dens = W.gsort.p; mp.n_src_pts = No;
     mp.out = W.mbar.p;
@@ -962,11 +1001,11 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
 
     // M2M^T, root first: ebar = UC2E^T mbar, then push to children
     for (int L = 0; L < P.st.n_levels; L++) {
-        launch_level_gemm(P.t_slev[L], P.blk, P.uc2e.p, true, W.mbar.p, mfma ? W.e2.p : nullptr, W.ebar.p, nullptr,
+        launch_level_gemm(A.t_slev[L], P.blk, P.uc2e.p, true, W.mbar.p, mfma ? W.e2.p : nullptr, W.ebar.p, nullptr,
                           false, s);
         if (L + 1 >= P.st.n_levels) break;
         SurfStage c = surf_base(P);
-        set_list(c, P.a_m2m[L + 1]);
+        set_list(c, A.m2m[L + 1]);
         c.tt = view(P.dst); c.t_surf = P.d_inner.p;
         c.st = view(P.dst); c.s_surf = P.d_outer.p; c.s_coeff = W.ebar.p;
         c.out = W.mbar.p;
@@ -978,10 +1017,10 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     f.n_leaves = (int)P.st.leaves.size();
     f.leaves = P.src_leaves.p;
     f.tt = view(P.dst); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = Ns;
-    f.rp_ptr = P.a_leaf_p2p.ptr.p; f.rp_beg = P.a_leaf_p2p.rbeg.p; f.rp_cnt = P.a_leaf_p2p.rcnt.p;
-    f.rp_pre = P.a_leaf_p2p.rpre.p;
+    f.rp_ptr = A.leaf_p2p.ptr.p; f.rp_beg = A.leaf_p2p.rbeg.p; f.rp_cnt = A.leaf_p2p.rcnt.p;
+    f.rp_pre = A.leaf_p2p.rpre.p;
     f.ps = view(P.op); f.pdens = W.gsort.p; f.n_ps = No;
-    f.sl_ptr = P.a_leaf_p2l.ptr.p; f.sl_cell = P.a_leaf_p2l.cell.p;
+    f.sl_ptr = A.leaf_p2l.ptr.p; f.sl_cell = A.leaf_p2l.cell.p;
     f.slt = view(P.dot); f.sl_surf = P.d_inner.p; f.sl_coeff = W.cbar.p;
     f.self_surf = P.d_outer.p; f.self_coeff = W.ebar.p;
     f.unit = P.d_unit.p; f.n_e = P.n_e; f.blk = P.blk;

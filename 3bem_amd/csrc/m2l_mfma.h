@@ -42,10 +42,11 @@ bool m2l_mfma_size_ok(int blk);
 // up_fac, down_fac: optionally {W, V}, the SVD factors of UC2E and DC2E per level
 // (from the pseudo-inverse's own SVD, gpu_pinv_batched); without them the C2E
 // matrices are factored again.
+// dot, dst: the trees on the device; dm2l: the device copy of m2l.
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    bool same_grid, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
-                    const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac = nullptr,
-                    const DevBuf<double>* down_fac = nullptr);
+                    const DevTree& dot, const DevTree& dst, bool same_grid, const HostList& m2l,
+                    const DevList& dm2l, const Surfaces& surf, const double* d_uc2e, const double* d_dc2e,
+                    hipStream_t s, const DevBuf<double>* up_fac = nullptr, const DevBuf<double>* down_fac = nullptr);
 
 // Y[p] (ystride floats per pair) = M_{class(p)} * chk_up[src(p)] for every pair,
 // chk_up = Bu * up-check potential per source cell, rounded to FP32.

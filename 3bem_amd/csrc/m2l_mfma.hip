@@ -28,7 +28,6 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <unordered_map>
 
 #include "c2e.h"
 #include "m2l_mfma.h"
@@ -282,13 +281,79 @@ __global__ __launch_bounds__(256) void k_modal_factors(int n, int up, const doub
     }
 }
 
-struct ClassKey {
-    unsigned long long k;
-    bool operator==(const ClassKey& o) const { return k == o.k; }
-};
-struct ClassHash {
-    size_t operator()(const ClassKey& c) const { return std::hash<unsigned long long>()(c.k); }
-};
+// target of every pair: pair p of item k (p in [ptr[k], ptr[k+1])) has target items[k]
+__global__ void k_pair_targets(const int* __restrict__ items, const int* __restrict__ ptr, int* __restrict__ tgt) {
+    const int k = blockIdx.x, t = items[k];
+    for (int p = ptr[k] + threadIdx.x; p < ptr[k + 1]; p += blockDim.x) tgt[p] = t;
+}
+
+// Translation class of pair p: (level t, level s, integer offset of the centres in
+// units of the finer half width per axis, each biased by 2^17 into 18 bits). bad is
+// set when an offset is not within 1e-6 of an integer (cells not on one grid) or
+// out of range. idx[p] = p (the values the sort carries).
+__global__ void k_class_keys(int n, const int* __restrict__ tgt, const int* __restrict__ src, TreeView ot,
+                             const double* __restrict__ ohw, TreeView st, const double* __restrict__ shw,
+                             unsigned long long* __restrict__ key, int* __restrict__ idx, int* __restrict__ bad) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int t = tgt[p], c = src[p];
+    const int Lt = ot.level[t], Ls = st.level[c];
+    const double ct[3] = {ot.cx[t], ot.cy[t], ot.cz[t]}, cs[3] = {st.cx[c], st.cy[c], st.cz[c]};
+    unsigned long long k = ((unsigned long long)Lt << 59) | ((unsigned long long)Ls << 54);
+    bool ok = true;
+    for (int a = 0; a < 3; a++) {
+        const double hf = fmin(ohw[3 * Lt + a], shw[3 * Ls + a]);
+        const double q = (cs[a] - ct[a]) / hf;
+        const double qi = rint(q);
+        ok = ok && fabs(q - qi) < 1e-6 && fabs(qi) < (double)(1 << 17);
+        k |= (unsigned long long)((long long)qi + (1 << 17)) << (36 - 18 * a);
+    }
+    key[p] = k;
+    idx[p] = p;
+    if (!ok) atomicOr(bad, 1);
+}
+
+// flag[i] = 1 where sorted key i starts a run of equal keys
+template <class K>
+__global__ void k_run_starts(int n, const K* __restrict__ key, int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+}
+
+// start[pos[i]] = i for every run start i; start[runs] = n
+__global__ void k_run_compact(int n, const int* __restrict__ flag, const int* __restrict__ pos, int runs,
+                              int* __restrict__ start) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) start[pos[i]] = i;
+    if (i == 0) start[runs] = n;
+}
+
+// the representative pair of each class: its first pair (in pair order)
+__global__ void k_class_reps(int nc, const int* __restrict__ cstart, const int* __restrict__ sorted_pairs,
+                             const int* __restrict__ tgt, const int* __restrict__ src, int* __restrict__ rep_t,
+                             int* __restrict__ rep_s) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int p = sorted_pairs[cstart[c]];
+    rep_t[c] = tgt[p];
+    rep_s[c] = src[p];
+}
+
+// key of a pair for the transpose's grouping: its source cell
+__global__ void k_source_keys(int n, const int* __restrict__ src, unsigned int* __restrict__ key,
+                              int* __restrict__ idx) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    key[p] = (unsigned int)src[p];
+    idx[p] = p;
+}
+
+__global__ void k_gather_int(int n, const int* __restrict__ idx, const unsigned int* __restrict__ v,
+                             int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int)v[idx[i]];
+}
 
 }  // namespace
 
@@ -303,9 +368,9 @@ bool m2l_mfma_size_ok(int n) {
 }
 
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    bool same_grid, const HostList& m2l, const Surfaces& surf, const double* d_uc2e,
-                    const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac,
-                    const DevBuf<double>* down_fac) {
+                    const DevTree& dot, const DevTree& dst, bool same_grid, const HostList& m2l,
+                    const DevList& dm2l, const Surfaces& surf, const double* d_uc2e, const double* d_dc2e,
+                    hipStream_t s, const DevBuf<double>* up_fac, const DevBuf<double>* down_fac) {
     auto t0 = std::chrono::steady_clock::now();
     M = M2lMfma{};
     const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
@@ -320,37 +385,40 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     }
     const int n_pairs = (int)m2l.cell.size();
 
-    // classes: (level_t, level_s, q) with q = (c_s - c_t) / finer half-width, integral
-    std::unordered_map<ClassKey, int, ClassHash> cls_of;
-    std::vector<int> pair_cls(n_pairs), rep_t, rep_s;
-    for (size_t k = 0; k + 1 < m2l.ptr.size(); k++) {
-        const int t = m2l.items[k];
-        for (int p = m2l.ptr[k]; p < m2l.ptr[k + 1]; p++) {
-            const int sc = m2l.cell[p];
-            const int Lt = ot.level[t], Ls = st.level[sc];
-            const double ct[3] = {ot.cx[t], ot.cy[t], ot.cz[t]}, cs[3] = {st.cx[sc], st.cy[sc], st.cz[sc]};
-            unsigned long long key = ((unsigned long long)Lt << 59) | ((unsigned long long)Ls << 54);
-            for (int a = 0; a < 3; a++) {
-                const double hf = std::min(ot.level_hw[3 * Lt + a], st.level_hw[3 * Ls + a]);
-                const double q = (cs[a] - ct[a]) / hf;
-                const double qi = std::nearbyint(q);
-                if (!(std::fabs(q - qi) < 1e-6) || std::fabs(qi) >= (1 << 17)) return;  // not on one grid
-                key |= (unsigned long long)((long long)qi + (1 << 17)) << (36 - 18 * a);
-            }
-            auto it = cls_of.find(ClassKey{key});
-            int c;
-            if (it == cls_of.end()) {
-                c = (int)rep_t.size();
-                cls_of.emplace(ClassKey{key}, c);
-                rep_t.push_back(t);
-                rep_s.push_back(sc);
-            } else {
-                c = it->second;
-            }
-            pair_cls[p] = c;
-        }
-    }
-    const int nc = (int)rep_t.size();
+    // classes: (level_t, level_s, q) with q = (c_s - c_t) / finer half-width, integral;
+    // on the GPU: a key per pair, a stable sort of the pairs by key, the key runs as
+    // classes (numbered in key order; each represented by its first pair)
+    const int gp = div_up(std::max(n_pairs, 1), 256);
+    DevBuf<int> tgt(n_pairs), colpair(n_pairs), itmp(n_pairs), flag(n_pairs), pos(n_pairs), bad(1);
+    DevBuf<unsigned long long> key(n_pairs), ktmp(n_pairs);
+    DevBuf<double> ohw, shw;
+    ohw.upload(ot.level_hw, s);
+    shw.upload(st.level_hw, s);
+    TBEM_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_pair_targets, dim3(dm2l.n_items), dim3(64), 0, s, dm2l.items.p, dm2l.ptr.p, tgt.p);
+    TBEM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_class_keys, dim3(gp), dim3(256), 0, s, n_pairs, tgt.p, dm2l.cell.p, view(dot), ohw.p,
+                       view(dst), shw.p, key.p, colpair.p, bad.p);
+    TBEM_LAUNCH_CHECK();
+    int h_bad = 0;
+    TBEM_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TBEM_HIP(hipStreamSynchronize(s));
+    if (h_bad) return;  // not on one grid
+    radix_sort_pairs_u64(key.p, colpair.p, ktmp.p, itmp.p, n_pairs, 64, s);
+    hipLaunchKernelGGL(k_run_starts<unsigned long long>, dim3(gp), dim3(256), 0, s, n_pairs, key.p, flag.p);
+    TBEM_LAUNCH_CHECK();
+    const int nc = device_exclusive_scan(flag.p, n_pairs, pos.p, s);
+    DevBuf<int> cstart(nc + 1), drep_t(nc), drep_s(nc);
+    hipLaunchKernelGGL(k_run_compact, dim3(gp), dim3(256), 0, s, n_pairs, flag.p, pos.p, nc, cstart.p);
+    TBEM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_class_reps, dim3(div_up(nc, 256)), dim3(256), 0, s, nc, cstart.p, colpair.p, tgt.p,
+                       dm2l.cell.p, drep_t.p, drep_s.p);
+    TBEM_LAUNCH_CHECK();
+    std::vector<int> rep_t(nc), rep_s(nc), hstart(nc + 1);
+    drep_t.download(rep_t.data(), nc, s);
+    drep_s.download(rep_s.data(), nc, s);
+    cstart.download(hstart.data(), nc + 1, s);
+    TBEM_HIP(hipStreamSynchronize(s));
     const int mt = (n + 15) / 16, kt = (n + 3) / 4;
     M.n_classes = nc;
     M.mt = mt;
@@ -424,25 +492,19 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     }
 
     // work units: classes binned to XCDs by pair count, units interleaved so that
-    // workgroup u lands on XCD u % 8 with the units of its bin
-    std::vector<std::vector<int>> pairs_of(nc);
-    for (int p = 0; p < n_pairs; p++) pairs_of[pair_cls[p]].push_back(p);
+    // workgroup u lands on XCD u % 8 with the units of its bin; a unit's pairs are
+    // colpair[off .. off + cnt) (the class's pairs, in pair order, in the sorted array)
     std::vector<int> order(nc);
     for (int c = 0; c < nc; c++) order[c] = c;
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return pairs_of[a].size() > pairs_of[b].size(); });
+    auto csize = [&](int c) { return hstart[c + 1] - hstart[c]; };
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return csize(a) > csize(b); });
     std::vector<size_t> load(kXcds, 0);
     std::vector<std::vector<std::array<int, 3>>> bins(kXcds);  // (class, off, cnt)
-    std::vector<int> colpair;
-    colpair.reserve(n_pairs);
     for (int c: order) {
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        const auto& pl = pairs_of[c];
-        for (size_t j = 0; j < pl.size(); j += kUnitCols) {
-            const int cnt = (int)std::min<size_t>(kUnitCols, pl.size() - j);
-            bins[x].push_back({c, (int)colpair.size(), cnt});
-            for (int q = 0; q < cnt; q++) colpair.push_back(pl[j + q]);
-        }
-        load[x] += (pl.size() + kUnitCols - 1) / kUnitCols;
+        const int sz = csize(c);
+        for (int j = 0; j < sz; j += kUnitCols) bins[x].push_back({c, hstart[c] + j, std::min(kUnitCols, sz - j)});
+        load[x] += (sz + kUnitCols - 1) / kUnitCols;
     }
     size_t rounds = 0;
     for (auto& b: bins) rounds = std::max(rounds, b.size());
@@ -458,31 +520,37 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.unit_class.upload(ucls, s);
     M.unit_off.upload(uoff, s);
     M.unit_cnt.upload(ucnt, s);
-    M.colpair.upload(colpair, s);
+    M.colpair = std::move(colpair);
 
-    // transpose support: target of every pair; per source cell, its pairs (a
-    // counting sort by source keeps each source's pairs contiguous, in cell order)
-    std::vector<int> pair_tgt(n_pairs), first(st.n_cells + 1, 0);
-    for (size_t k = 0; k + 1 < m2l.ptr.size(); k++)
-        for (int p = m2l.ptr[k]; p < m2l.ptr[k + 1]; p++) {
-            pair_tgt[p] = m2l.items[k];
-            first[m2l.cell[p] + 1]++;
-        }
-    for (int c = 0; c < st.n_cells; c++) first[c + 1] += first[c];
-    std::vector<int> adj_pairs(n_pairs), next(first.begin(), first.end() - 1);
-    for (int p = 0; p < n_pairs; p++) adj_pairs[next[m2l.cell[p]]++] = p;
-    std::vector<int> adj_items, adj_ptr(1, 0);
-    for (int c = 0; c < st.n_cells; c++)
-        if (first[c + 1] > first[c]) {
-            adj_items.push_back(c);
-            adj_ptr.push_back(first[c + 1]);
-        }
-    M.n_adj = (int)adj_items.size();
-    M.pair_tgt.upload(pair_tgt, s);
-    M.adj_items.upload(adj_items, s);
-    M.adj_ptr.upload(adj_ptr, s);
-    M.adj_tiles = make_cell_tiles(adj_items, st, s);
-    M.adj_pairs.upload(adj_pairs, s);
+    // transpose support: target of every pair; per source cell, its pairs (a stable
+    // sort of the pairs by source cell keeps each source's pairs contiguous, in pair
+    // order, the sources ascending)
+    M.pair_tgt = std::move(tgt);
+    {
+        DevBuf<unsigned int> skey(n_pairs), sktmp(n_pairs);
+        DevBuf<int> spairs(n_pairs);
+        hipLaunchKernelGGL(k_source_keys, dim3(gp), dim3(256), 0, s, n_pairs, dm2l.cell.p, skey.p, spairs.p);
+        TBEM_LAUNCH_CHECK();
+        int bits = 1;
+        while (bits < 31 && (1ll << bits) < (long long)st.n_cells) bits++;
+        radix_sort_pairs_u32(skey.p, spairs.p, sktmp.p, itmp.p, n_pairs, bits, s);
+        hipLaunchKernelGGL(k_run_starts<unsigned int>, dim3(gp), dim3(256), 0, s, n_pairs, skey.p, flag.p);
+        TBEM_LAUNCH_CHECK();
+        const int na = device_exclusive_scan(flag.p, n_pairs, pos.p, s);
+        M.adj_ptr.alloc(na + 1);
+        M.adj_items.alloc(na);
+        hipLaunchKernelGGL(k_run_compact, dim3(gp), dim3(256), 0, s, n_pairs, flag.p, pos.p, na, M.adj_ptr.p);
+        TBEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gather_int, dim3(div_up(std::max(na, 1), 256)), dim3(256), 0, s, na, M.adj_ptr.p,
+                           skey.p, M.adj_items.p);
+        TBEM_LAUNCH_CHECK();
+        std::vector<int> adj_items(na);
+        M.adj_items.download(adj_items.data(), na, s);
+        TBEM_HIP(hipStreamSynchronize(s));
+        M.n_adj = na;
+        M.adj_tiles = make_cell_tiles(adj_items, st, s);
+        M.adj_pairs = std::move(spairs);
+    }
     TBEM_HIP(hipStreamSynchronize(s));
     M.enabled = true;
     M.plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

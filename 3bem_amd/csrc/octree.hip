@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int* __restric
 }
 
 // out = exclusive scan of a (n ints); returns the total
-static int device_exclusive_scan(const int* a, int n, int* out, hipStream_t s) {
+int device_exclusive_scan(const int* a, int n, int* out, hipStream_t s) {
     if (n == 0) return 0;
     const int tiles = div_up(n, kScanTile);
     DevBuf<int> sums(tiles + 1);
@@ -251,6 +251,14 @@ static void radix_sort_pairs(K* k, int* v, K* kt, int* vt, int n, int bits, hipS
         TBEM_HIP(hipMemcpyAsync(kt, k, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, s));
         TBEM_HIP(hipMemcpyAsync(vt, v, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, s));
     }
+}
+
+void radix_sort_pairs_u64(unsigned long long* k, int* v, unsigned long long* kt, int* vt, int n, int bits,
+                          hipStream_t s) {
+    radix_sort_pairs<unsigned long long>(k, v, kt, vt, n, bits, s);
+}
+void radix_sort_pairs_u32(unsigned int* k, int* v, unsigned int* kt, int* vt, int n, int bits, hipStream_t s) {
+    radix_sort_pairs<unsigned int>(k, v, kt, vt, n, bits, s);
 }
 
 // -------------------------------------------------------------- cells, level by level

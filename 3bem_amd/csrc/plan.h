@@ -1,8 +1,14 @@
 // Plan data structures: octrees, surfaces, C2E operators and task lists.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <atomic>
+#include <cstdlib>
+#include <exception>
 #include <memory>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -11,6 +17,42 @@
 namespace tbem {
 
 constexpr int kMaxKeyLevels = 21;  // 3 bits per level in a 63-bit Morton key
+
+// Host threads for plan construction: OMP_NUM_THREADS when set, else the hardware's,
+// at most 32.
+inline int plan_threads() {
+    int t = (int)std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) t = std::atoi(e);
+    return std::max(1, std::min(32, t));
+}
+
+// fn(i) for i in [0, n) on plan_threads() host threads, items taken in order from a
+// shared counter; fn must only write data of its own i.
+template <class F>
+void parallel_for(size_t n, F&& fn) {
+    const int nt = (int)std::min<size_t>(plan_threads(), n);
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next(0);
+    std::exception_ptr err;
+    std::mutex err_mu;
+    auto work = [&] {
+        try {
+            for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(err_mu);
+            if (!err) err = std::current_exception();
+            next = n;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < nt; k++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
 
 // Octree in breadth-first numbering: level L's cells are [level_begin[L], level_begin[L+1]),
 // Morton (= depth-first) order inside a level, so the children of a cell are contiguous.
@@ -40,6 +82,15 @@ HostTree build_octree(const double* pts_host, int n, int min_pts_per_cell, hipSt
                       const double* box_lo_hi = nullptr);
 // Tight bounding box of n AoS points: lo xyz, hi xyz.
 void point_bounds(const double* pts_host, int n, double* lo_hi);
+
+// Device primitives of the plan build (octree.hip). Stable LSD radix sorts of
+// (key, value) pairs by the low `bits` bits of the keys, 8 bits per pass; the result
+// ends in k, v (kt, vt: n-element scratch).
+void radix_sort_pairs_u64(unsigned long long* k, int* v, unsigned long long* kt, int* vt, int n, int bits,
+                          hipStream_t s);
+void radix_sort_pairs_u32(unsigned int* k, int* v, unsigned int* kt, int* vt, int n, int bits, hipStream_t s);
+// out = exclusive prefix sum of the n ints of a (device); returns the total.
+int device_exclusive_scan(const int* a, int n, int* out, hipStream_t s);
 
 // Translation surfaces (cpp/fmm.h:42-66, cpp/fmm.cpp:22-51).
 struct Surfaces {

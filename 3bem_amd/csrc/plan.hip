@@ -92,38 +92,42 @@ void DevList::upload(const HostList& l, hipStream_t s) {
     rpre.upload(l.rpre, s);
 }
 
+// Items are independent: host threads merge each item's ranges, then the lists are
+// concatenated in item order.
 void cells_to_ranges(HostList& l, const HostTree& t) {
-    std::vector<int> ptr(1, 0);
-    std::vector<std::pair<int, int>> seg;
-    l.rbeg.clear();
-    l.rcnt.clear();
-    for (size_t k = 0; k + 1 < l.ptr.size(); k++) {
-        seg.clear();
+    const size_t ni = l.ptr.size() - 1;
+    std::vector<std::vector<std::pair<int, int>>> per(ni);
+    parallel_for(ni, [&](size_t k) {
+        std::vector<std::pair<int, int>>& seg = per[k];
         for (int e = l.ptr[k]; e < l.ptr[k + 1]; e++) {
             const int c = l.cell[e];
             if (t.count[c] > 0) seg.push_back({t.begin[c], t.count[c]});
         }
         std::sort(seg.begin(), seg.end());
-        for (auto& r: seg) {
-            if (!l.rbeg.empty() && (int)l.rbeg.size() > ptr.back() && l.rbeg.back() + l.rcnt.back() == r.first) {
-                l.rcnt.back() += r.second;
-            } else {
-                l.rbeg.push_back(r.first);
-                l.rcnt.push_back(r.second);
-            }
+        size_t w = 0;  // merge touching ranges in place
+        for (size_t r = 0; r < seg.size(); r++) {
+            if (w > 0 && seg[w - 1].first + seg[w - 1].second == seg[r].first) seg[w - 1].second += seg[r].second;
+            else seg[w++] = seg[r];
         }
-        ptr.push_back((int)l.rbeg.size());
-    }
+        seg.resize(w);
+    });
+    std::vector<int> ptr(ni + 1, 0);
+    for (size_t k = 0; k < ni; k++) ptr[k + 1] = ptr[k] + (int)per[k].size();
+    l.rbeg.assign(ptr[ni], 0);
+    l.rcnt.assign(ptr[ni], 0);
+    l.rpre.assign(ptr[ni], 0);
+    parallel_for(ni, [&](size_t k) {
+        int acc = 0;
+        for (size_t r = 0; r < per[k].size(); r++) {
+            const int e = ptr[k] + (int)r;
+            l.rbeg[e] = per[k][r].first;
+            l.rcnt[e] = per[k][r].second;
+            l.rpre[e] = acc;
+            acc += per[k][r].second;
+        }
+    });
     l.ptr = std::move(ptr);
     l.cell.clear();
-    l.rpre.assign(l.rbeg.size(), 0);
-    for (size_t k = 0; k + 1 < l.ptr.size(); k++) {
-        int acc = 0;
-        for (int e = l.ptr[k]; e < l.ptr[k + 1]; e++) {
-            l.rpre[e] = acc;
-            acc += l.rcnt[e];
-        }
-    }
 }
 
 // ------------------------------------------------------------------ dual traversal
@@ -135,9 +139,17 @@ struct Traversal {
     int n_e;
     bool small_cells;
     TaskLists& tl;
+    // pairs reached after `stop_depth` splits are not descended here but recorded as
+    // slots, with the list sizes at that point (stop_depth < 0: descend everything)
+    int stop_depth = -1;
+    struct Slot {
+        int o, s;
+        size_t mark[4];  // p2p, m2l, m2p, p2l sizes when the slot was reached
+    };
+    std::vector<Slot>* slots = nullptr;
 
     // cpp/fmm.cpp:577-630
-    void dual(int o, int s) {
+    void dual(int o, int s, int depth = 0) {
 #pragma clang fp contract(off)
         const double rs = st.level_r[st.level[s]], ro = ot.level_r[ot.level[o]];
         const double rmax = std::max(rs, ro), rmin = std::min(rs, ro);
@@ -156,11 +168,15 @@ struct Traversal {
             tl.p2p.push_back({o, s});
             return;
         }
+        if (depth == stop_depth) {
+            slots->push_back({o, s, {tl.p2p.size(), tl.m2l.size(), tl.m2p.size(), tl.p2l.size()}});
+            return;
+        }
         const bool split_src = (ot.level[o] > st.level[s] && !sl) || ol;
         if (split_src) {
-            for (int k = 0; k < st.n_child[s]; k++) dual(o, st.first_child[s] + k);
+            for (int k = 0; k < st.n_child[s]; k++) dual(o, st.first_child[s] + k, depth + 1);
         } else {
-            for (int k = 0; k < ot.n_child[o]; k++) dual(ot.first_child[o] + k, s);
+            for (int k = 0; k < ot.n_child[o]; k++) dual(ot.first_child[o] + k, s, depth + 1);
         }
     }
 };
@@ -171,8 +187,38 @@ void dual_tree_traversal(const HostTree& ot, const HostTree& st, double mac, int
     // upward (cpp/fmm.cpp:560-575) and downward (cpp/fmm.cpp:632-648) lists, as sets
     for (int c = 0; c < st.n_cells; c++) (st.leaf[c] ? tl.p2m : tl.m2m).push_back(c);
     for (int c = 0; c < ot.n_cells; c++) (ot.leaf[c] ? tl.l2p : tl.l2l).push_back(c);
-    Traversal tr{ot, st, mac, n_e, small_cells, tl};
-    tr.dual(0, 0);
+    // The recursion's first splits run here, the subtrees below them on host threads;
+    // each list is then spliced back in the order one recursion would produce it.
+    std::vector<Traversal::Slot> slots;
+    Traversal top{ot, st, mac, n_e, small_cells, tl};
+    top.stop_depth = plan_threads() > 1 ? 4 : -1;  // one thread: the plain recursion
+    top.slots = &slots;
+    top.dual(0, 0);
+    if (slots.empty()) return;
+    std::vector<TaskLists> part(slots.size());
+    parallel_for(slots.size(), [&](size_t k) {
+        Traversal tr{ot, st, mac, n_e, small_cells, part[k]};
+        tr.dual(slots[k].o, slots[k].s);
+    });
+    auto splice = [&](int which, std::vector<std::array<int, 2>> TaskLists::*member) {
+        std::vector<std::array<int, 2>>& direct = tl.*member;
+        size_t total = direct.size();
+        for (const auto& pt : part) total += (pt.*member).size();
+        std::vector<std::array<int, 2>> out;
+        out.reserve(total);
+        size_t from = 0;
+        for (size_t k = 0; k < slots.size(); k++) {
+            out.insert(out.end(), direct.begin() + from, direct.begin() + slots[k].mark[which]);
+            from = slots[k].mark[which];
+            out.insert(out.end(), (part[k].*member).begin(), (part[k].*member).end());
+        }
+        out.insert(out.end(), direct.begin() + from, direct.end());
+        direct = std::move(out);
+    };
+    splice(0, &TaskLists::p2p);
+    splice(1, &TaskLists::m2l);
+    splice(2, &TaskLists::m2p);
+    splice(3, &TaskLists::p2l);
 }
 
 }  // namespace tbem

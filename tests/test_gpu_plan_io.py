@@ -149,3 +149,28 @@ def test_saved_plan_with_bad_parent_link_is_refused(tbem):
         sealed = blob[:16] + plan_hash(bytes(p)).to_bytes(8, "little") + bytes(p)
         with pytest.raises(tbem.TbemError, match="parent and child links"):
             tbem.FMMOperator(K, data, cfg, plan=sealed)
+
+
+def test_threaded_plan_build_equals_one_thread(tbem, wl, monkeypatch):
+    """The plan's host work (the dual-tree traversal, the task lists and their merged
+    point ranges) runs on host threads. With one thread it is the plain recursion and
+    plain loops; the threaded build must produce the same task lists in the same
+    order: byte-identical saved plans and bit-identical outputs, with the MFMA M2L and
+    the adjoint lists (built on first use) in play."""
+    torch = pytest.importorskip("torch")
+    g = wl.cube_and_clusters(300_000)
+    data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
+    cfg = tbem.FMMConfig(0.3, 50, 64, 0.05, False)
+    x = torch.tensor(g["x"], device="cuda")
+    print("CASE threaded plan build", flush=True)
+    out = {}
+    for threads in ("1", "16"):
+        monkeypatch.setenv("OMP_NUM_THREADS", threads)
+        op = tbem.FMMOperator(tbem.LaplaceSingle(), data, cfg, precision="mixed")
+        info = op.info()
+        assert info["m2l"] > (1 << 20) and info["m2l_classes"] > 0  # long lists; the MFMA M2L runs
+        out[threads] = (op.save_plan(), op.apply(x).cpu().numpy(), op.apply_adjoint(x).cpu().numpy())
+        del op
+    assert out["1"][0] == out["16"][0]
+    for k in (1, 2):
+        assert np.array_equal(out["1"][k].view(np.uint64), out["16"][k].view(np.uint64))
