@@ -416,6 +416,17 @@ class DirectOperator:
         _lib.check(self._lib.tbem_nbody_apply(self._h, _dp(x), _dp(y)))
         return y
 
+    def apply_noalloc(self, x, out):
+        """out = A x into a caller-owned float64 CUDA tensor, allocating nothing
+        (tbem_nbody_apply_noalloc: R * n_src <= 4,194,304; calls that overlap on one
+        GPU race on its static scratch)."""
+        import torch
+        assert x.is_cuda and x.dtype == torch.float64 and x.is_contiguous() and x.numel() == self.n_cols()
+        assert out.is_cuda and out.dtype == torch.float64 and out.is_contiguous() and out.numel() == self.n_rows()
+        _lib.check(self._lib.tbem_nbody_apply_noalloc(self._h, ctypes.c_void_p(x.data_ptr()),
+                                                      ctypes.c_void_p(out.data_ptr()), _torch_stream()))
+        return out
+
     def apply_adjoint(self, g):
         g = np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(-1))
         y = np.zeros(self.n_cols())

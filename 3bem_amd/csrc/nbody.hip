@@ -8,12 +8,39 @@
 // leaf-stage kernel the FMM's P2P uses (LDS tiles, slice-split lanes).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "stages.h"
 
 namespace tbem {
 
 static constexpr int kTargetChunk = 128;
+
+// Scratch of tbem_nbody_apply_noalloc: the weighted source densities in sorted
+// order, R * n_src doubles, at most kNoallocDoubles (32 MiB). Every GPU the library
+// runs on holds one copy until the process ends. In host code `scratch` names a
+// host-side stand-in, not device memory: kernels receive the device address that
+// hipGetSymbolAddress gives for the current device (noalloc_scratch()).
+constexpr size_t kNoallocDoubles = size_t(4) << 20;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the first
+// call per device (with that device current) and kept.
+static double* noalloc_scratch(int dev) {
+    TBEM_REQUIRE(dev >= 0 && dev < kMaxDevices, TBEM_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        TBEM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        TBEM_REQUIRE(ptr != nullptr, TBEM_E_HIP, "no device address for the no-allocation scratch");
+        g_scratch[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch[dev];
+}
 
 // a flat tree of fixed-size chunks over n points in input order
 static HostTree flat_tree(const double* locs, int n, int chunk) {
@@ -78,6 +105,7 @@ struct NbodyPlan {
     DevList fwd, adj;
     DevBuf<int> obs_leaves, src_leaves;
     DevBuf<double> dens, gsort, xb, yb, outs;
+    int device = 0;  // the device the handle's buffers live on
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;  // end of the last call's work (buffers are per handle)
     ~NbodyPlan() {
@@ -109,12 +137,12 @@ static int nb_guard(F&& f) {
     }
 }
 
-static void nb_forward_impl(NbodyPlan& P, const double* x, double* y, hipStream_t s);
+static void nb_forward_impl(const NbodyPlan& P, const double* x, double* y, double* dens, hipStream_t s);
 static void nb_adjoint_impl(NbodyPlan& P, const double* g, double* y, hipStream_t s);
 
 static void nb_forward(NbodyPlan& P, const double* x, double* y, hipStream_t s) {
     TBEM_HIP(hipStreamWaitEvent(s, P.done, 0));
-    nb_forward_impl(P, x, y, s);
+    nb_forward_impl(P, x, y, P.dens.p, s);
     TBEM_HIP(hipEventRecord(P.done, s));
 }
 
@@ -124,19 +152,20 @@ static void nb_adjoint(NbodyPlan& P, const double* g, double* y, hipStream_t s) 
     TBEM_HIP(hipEventRecord(P.done, s));
 }
 
-static void nb_forward_impl(NbodyPlan& P, const double* x, double* y, hipStream_t s) {
+// dens: R * n_src doubles of scratch for the weighted densities
+static void nb_forward_impl(const NbodyPlan& P, const double* x, double* y, double* dens, hipStream_t s) {
     if (P.n_obs == 0) return;
     if (P.n_src == 0) {
         launch_fill(y, (size_t)P.R * P.n_obs, 0.0, s);
         return;
     }
-    launch_gather_dens(x, P.sp.perm.p, P.sp.w.p, P.n_src, P.R, P.dens.p, s);
+    launch_gather_dens(x, P.sp.perm.p, P.sp.w.p, P.n_src, P.R, dens, s);
     LeafStage f{};
     f.n_leaves = (int)P.obs_t.leaves.size();
     f.leaves = P.obs_leaves.p;
     f.tt = view(P.d_obs_t); f.tp = view(P.op); f.n_tgt = P.n_obs;
     f.rp_ptr = P.fwd.ptr.p; f.rp_beg = P.fwd.rbeg.p; f.rp_cnt = P.fwd.rcnt.p; f.rp_pre = P.fwd.rpre.p;
-    f.ps = view(P.sp); f.pdens = P.dens.p; f.n_ps = P.n_src;
+    f.ps = view(P.sp); f.pdens = dens; f.n_ps = P.n_src;
     f.out_sorted = y;  // identity permutation: write straight into y
     if (P.precision == TBEM_MIXED) launch_leaf_stage<float>(P.kid, false, f, P.kc, s, false);
     else launch_leaf_stage<double>(P.kid, false, f, P.kc, s, false);
@@ -178,6 +207,7 @@ int tbem_nbody_create(const tbem_nbody_desc* d, tbem_nbody_t* out) {
         TBEM_REQUIRE(d->n_obs < (1u << 30) && d->n_src < (1u << 30), TBEM_E_UNSUPPORTED, "too many points");
         auto h = std::unique_ptr<tbem_nbody_s>(new tbem_nbody_s);
         NbodyPlan& P = h->p;
+        TBEM_HIP(hipGetDevice(&P.device));
         TBEM_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
         TBEM_HIP(hipEventCreateWithFlags(&P.done, hipEventDisableTiming));
         hipStream_t s = P.stream;
@@ -208,6 +238,26 @@ int tbem_nbody_apply_dev(tbem_nbody_t h, const double* d_x, double* d_y, void* s
     return nb_guard([&] {
         TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
         nb_forward(h->p, d_x, d_y, (hipStream_t)stream);
+    });
+}
+
+// tbem_nbody_apply_dev without allocating: the weighted densities go to the
+// device's static scratch instead of the handle's buffer. Calls that overlap on
+// one device share that scratch and race.
+int tbem_nbody_apply_noalloc(tbem_nbody_t h, const double* d_x, double* d_y, void* stream) {
+    return nb_guard([&] {
+        TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
+        const NbodyPlan& P = h->p;
+        const size_t need = (size_t)P.R * P.n_src;
+        TBEM_REQUIRE(need <= kNoallocDoubles, TBEM_E_UNSUPPORTED,
+                     "tbem_nbody_apply_noalloc: " + std::to_string(need) +
+                         " source density values exceed its scratch of " + std::to_string(kNoallocDoubles));
+        TBEM_REQUIRE((d_x || need == 0) && (d_y || P.n_obs == 0), TBEM_E_INVALID, "null argument");
+        int dev = -1;
+        TBEM_HIP(hipGetDevice(&dev));
+        TBEM_REQUIRE(dev == P.device, TBEM_E_INVALID,
+                     "tbem_nbody_apply_noalloc: the current device is not the handle's");
+        nb_forward_impl(P, d_x, d_y, noalloc_scratch(dev), (hipStream_t)stream);
     });
 }
 

@@ -265,6 +265,17 @@ typedef struct {
 int tbem_nbody_create(const tbem_nbody_desc* desc, tbem_nbody_t* out);
 int tbem_nbody_apply(tbem_nbody_t h, const double* x, double* y);
 int tbem_nbody_apply_dev(tbem_nbody_t h, const double* d_x, double* d_y, void* stream);
+/* tbem_nbody_apply_dev that allocates nothing (no hipMalloc, no hipFree, no
+ * allocating library call). No counterpart in the reference: it is for callers that
+ * cannot allocate device memory, such as code under HIP graph capture, tight
+ * serving loops and harnesses that forbid allocation. The weighted source
+ * densities are staged in a static device array of 4,194,304 doubles (32 MiB) that
+ * each GPU the library runs on holds until the process ends: R * n_src above that
+ * returns TBEM_E_UNSUPPORTED. The current device must be the handle's. Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (on different
+ * streams, or through different handles) race; calls ordered on one stream do
+ * not. Same outputs as tbem_nbody_apply_dev, bit for bit. */
+int tbem_nbody_apply_noalloc(tbem_nbody_t h, const double* d_x, double* d_y, void* stream);
 /* y_j = w_j sum_i K(x_i, y_j)^T g_i — transpose of the direct operator. */
 int tbem_nbody_apply_adjoint(tbem_nbody_t h, const double* g, double* y);
 size_t tbem_nbody_n_rows(tbem_nbody_t h);
