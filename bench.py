@@ -1,23 +1,24 @@
 """Benchmark: FMM matvec throughput on one MI355X.
 
-Default workload: plain FP64 elastic-displacement (Kelvin, mu=1, nu=0.25) FMM
-matvec on 1M points (500k uniform on the free surface [-1/2,1/2]^2 x {0}, 500k
-on a 0.5 x 0.5 fault dipping 45 degrees centred at depth 0.3), 3-component N(0,1)
-densities, leaf 64, order 6, MAC 0.3, d 0.05.
+Headline workload (the metric's configuration): plain FP64 elastic-displacement
+(Kelvin, mu=1, nu=0.25) FMM matvec on 2M points (1M uniform on the free surface
+[-1/2,1/2]^2 x {0}, 1M on a 0.5 x 0.5 fault dipping 45 degrees centred at depth
+0.3), 3-component N(0,1) densities, order 100, leaf 256, MAC 0.3, d 0.05.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config fault_surface|icosphere|cube_clusters|rough_fault] [--n N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--n N]
 
 One JSON line on stdout. `value` is FP64 Mpts/s = N_obs / apply time with inputs
-resident in HBM (device pointers). Beside it: the mixed FP32 path, relative L2
-error of both against an FP64 direct sum on a 10k-target subsample (GPU FP64
-direct operator, itself checked against the CPU oracle in tests/), the
-backward/forward time ratio (adjoint matvec + observation-coordinate gradient vs
-forward), the roofline of the dominant kernel (the fused P2P+M2P+L2P leaf stage)
-timed with HIP events on its stream, and the compiled reference CPU FMM timed on
-this host on a bounded sample. With the default workload, `target_fp32_mfma_vs_fp64`
-adds the precision comparison at the accuracy target: 2M points, order 100 (FP64
-FMM ~4e-7 from the direct sum), FP64 against mixed precision (FP32 MFMA M2L, FP32
-point-to-point), with the roofline of the MFMA M2L kernel.
+resident in HBM (device pointers). Beside it, on the same workload: the mixed path
+(FP32 MFMA M2L, FP32 point-to-point) with its speed-up, the relative L2 error of
+both against an FP64 direct sum on a 10k-target subsample (GPU FP64 direct
+operator, itself checked against the CPU oracle in tests/), the backward/forward
+time ratio (adjoint matvec + observation-coordinate gradient vs forward), and the
+rooflines of the leaf stage and the M2L kernel of each precision, timed with HIP
+events on the stream they run on; `roofline` is the FP64 step's dominant kernel.
+Further sections: the 1M-point order-6 version of the workload, the 4M-point
+cube-and-clusters Laplace workload, the differentiable forward model (fault to
+stations), the rough-fault BEM solve and the near-field CSR / integral operator.
+`cpu_baseline` times the compiled reference CPU FMM on this host on a bounded sample.
 """
 import argparse
 import importlib
@@ -38,11 +39,19 @@ FLOPS_PER_PAIR = {0: 11, 1: 19, 2: 0, 3: 28, 4: 60, 5: 60, 6: 120, 7: 2, 8: 6}
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
-# HBM bytes per launch of the FP64 leaf stage on the default workload: rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE of the same command, scaled by what the counters
-# read for a plain device copy (profiles/k_leaf_stage_elastic_u_1M_order6.md).
-# Other workloads have no measured figure and report null.
-LEAF_TRAFFIC_BYTES = {("fault_surface", 1_000_000): 644_165_984}
+# HBM bytes per launch of a workload's leaf and M2L kernels: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE of the same command (mean over its dispatches), scaled
+# by what the counters read for a plain device copy (factors 2.0 and 1.0; the
+# profiles under profiles/ give the runs). Keyed by (workload, points, precision,
+# kernel); other workloads report null.
+TRAFFIC_BYTES = {
+    ("fault_surface", 1_000_000, "fp64", "leaf"): 644_165_984,
+    ("fault_surface_o100", 2_000_000, "fp64", "m2l"): 2_170_873_359,
+    ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 2_370_368_469,
+    ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 2_283_267_211,
+    ("fault_surface_o100", 2_000_000, "mixed", "m2l"): 7_941_620_238,
+}
+TRAFFIC_UNIT = "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)"
 
 
 def log(*a):
@@ -50,7 +59,19 @@ def log(*a):
 
 
 def workload(name, n):
+    w = _workload(name, n)
+    w["name"] = name
+    return w
+
+
+def _workload(name, n):
     wl = importlib.import_module("3bem_amd.workloads")
+    if name == "fault_surface_o100":
+        g = wl.fault_and_free_surface(n or 2_000_000)
+        return dict(kernel=3, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
+                    cfg=(0.3, 100, 256, 0.05, False),
+                    desc=f"elastic displacement (Kelvin) FMM matvec, {len(g['locs'])} points (half free "
+                         "surface, half 45-degree fault), mu=1 nu=0.25, order 100, leaf 256, MAC 0.3")
     if name == "icosphere":
         g = wl.icosphere_surface_15k()
         return dict(kernel=0, locs=g["locs"], normals=g["normals"], weights=g["weights"], x=g["x"],
@@ -119,106 +140,52 @@ def backward_times(op, x, g, steps, torch):
     return tuple(statistics.median(v[1:]) for v in (adj, grad, recomp))
 
 
-def cpu_baseline(kernel, cfg, budget_n):
-    """Compiled reference FMM (oracle/_ref) on a bounded sample of the same geometry;
-    falls back to the oracle restatement when the reference build is absent."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(kernel, cfg, n_sample, runs):
+    """The compiled reference FMM (oracle/_ref; the FP64 restatement when the
+    reference build is absent) on an n_sample-point sample of the headline's geometry
+    family with the same kernel and FMMConfig: median of `runs` warm applies at the
+    OpenMP thread count of this host's environment and again at one thread, plan
+    build excluded and reported."""
+    import ctypes
     from oracle.oracle import Oracle, available
     wl = importlib.import_module("3bem_amd.workloads")
     kind = "reference" if available("ref") else "port"
     o = Oracle("ref" if kind == "reference" else "orc")
-    g = wl.fault_and_free_surface(budget_n) if kernel == 3 else wl.uniform_cube(budget_n, dim=3 if kernel >= 3 else 1)
+    g = wl.fault_and_free_surface(n_sample) if kernel >= 3 else wl.uniform_cube(n_sample, dim=1)
+    R = 3 if kernel >= 3 else 1
+    x = g["x"][: R * n_sample]
     t0 = time.perf_counter()
     op = o.fmm(kernel, g["locs"], g["normals"], g["locs"], g["normals"], g["weights"], *cfg)
     t_plan = time.perf_counter() - t0
-    times = []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        op.apply(g["x"][: (3 if kernel >= 3 else 1) * budget_n])
-        times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
-    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    return {"value": round(budget_n / t / 1e6, 5), "unit": "Mpts/s", "cores": cores, "kind": kind,
-            "sample": f"same geometry family at {budget_n} points, FMMOperator::apply median of 2 runs "
-                      f"({t:.2f} s each; plan build {t_plan:.2f} s excluded), OMP threads={cores}"}
-
-
-def compare_run(tbem, torch, steps, g, kid, cfg_t, n_sub, desc):
-    """FP32/MFMA against FP64 on one workload, both precisions on the same plan
-    parameters: forward time, backward (adjoint + observation gradient) time, and
-    relative L2 errors against an FP64 direct sum on an n_sub-target subsample."""
-    n = len(g["locs"])
-    R = 1 if kid in (0, 1, 2, 7) else 3
-    K = tbem.make_kernel(kid, 1.0, 0.25)
-    data = tbem.NBodyData(g["locs"], g["normals"], g["locs"], g["normals"], g["weights"])
-    cfg = tbem.FMMConfig(*cfg_t)
-    x = torch.tensor(g["x"], device="cuda")
-    rng = np.random.default_rng(321)
-    sub = np.sort(rng.choice(n, n_sub, replace=False))
-    dsub = tbem.NBodyData(g["locs"][sub], g["normals"][sub], g["locs"], g["normals"], g["weights"])
-    yd = tbem.make_direct_nbody_operator(dsub, K).apply(g["x"])
-    idx = np.concatenate([q * n + sub for q in range(R)])
-    out = {"workload": desc}
-    ys = {}
-    for prec in ("fp64", "mixed"):
-        t0 = time.perf_counter()
-        op = tbem.FMMOperator(K, data, cfg, precision=prec)
-        plan = time.perf_counter() - t0
-        dt, m2l = time_steps(lambda: op.apply(x), steps, 1, torch, after=lambda: op.m2l_kernel_ms())
-        ys[prec] = op.apply(x).cpu().numpy()
-        info = op.info()
-        stage_ms = {k: round(v, 3) for k, v in op.stage_times().items()}
-        gv = torch.randn(R * n, dtype=torch.float64, device="cuda")
-        dt_adj, dt_grad, dt_grad_re = backward_times(op, x, gv, 2, torch)
-        out[prec] = {"ms_per_step": round(dt * 1e3, 3), "Mpts_per_s": round(n / dt / 1e6, 3),
-                     "bwd_fwd_ratio": round((dt_adj + dt_grad) / dt, 3),
-                     "bwd_ms": {"adjoint": round(dt_adj * 1e3, 2), "obs_grad": round(dt_grad * 1e3, 2),
-                                "obs_grad_recomputing": round(dt_grad_re * 1e3, 2)},
-                     "plan_build_s": round(plan, 2), "m2l_kernel_ms": round(float(np.mean(m2l)), 3),
-                     "stage_ms": stage_ms,
-                     "rel_err_vs_direct": float(np.linalg.norm(ys[prec][idx] - yd) / np.linalg.norm(yd))}
-        # the same operator rebuilt from its saved plan (octrees, task lists, C2E)
-        blob = op.save_plan()
-        t0 = time.perf_counter()
-        op_l = tbem.FMMOperator(K, data, cfg, precision=prec, plan=blob)
-        out[prec]["plan_from_saved_s"] = round(time.perf_counter() - t0, 3)
-        out[prec]["saved_plan_MB"] = round(len(blob) / 2 ** 20, 1)
-        out[prec]["saved_plan_same_output"] = bool(np.array_equal(op_l.apply(x).cpu().numpy(), ys[prec]))
-        del op_l, blob
-        if prec == "mixed" and info["m2l_classes"] > 0:
-            blk = R * info["n_e"]
-            flop = 2.0 * info["m2l"] * blk * blk
-            ach = flop / (float(np.mean(m2l)) * 1e-3) / 1e12
-            out["m2l_mfma_roofline"] = {"bound": "mfma", "kernel": "k_m2l_mfma (FP32 v_mfma_f32_16x16x4f32)",
-                                        "achieved": round(ach, 2), "peak": 157.3, "unit": "TFLOP/s",
-                                        "frac": round(ach / 157.3, 4), "algorithmic_flop": flop,
-                                        "classes": info["m2l_classes"], "pairs": info["m2l"]}
-        log(f"[bench] {desc[:40]} {prec}: {dt * 1e3:.2f} ms, rel err {out[prec]['rel_err_vs_direct']:.2e}, "
-            f"m2l kernel {out[prec]['m2l_kernel_ms']:.2f} ms")
-        del op
-        torch.cuda.empty_cache()
-    out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(ys["mixed"] - ys["fp64"]) / np.linalg.norm(ys["fp64"]))
-    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
-    return out
-
-
-def target_run(tbem, torch, steps):
-    """The FP32/MFMA-vs-FP64 comparison at the accuracy target: elastic displacement
-    FMM matvec on 2M points of the fault + free-surface geometry, order 100, leaf 256,
-    MAC 0.3 (FP64 FMM ~4e-7 from the direct sum), errors on a 5k-target subsample."""
-    wl = importlib.import_module("3bem_amd.workloads")
-    return compare_run(tbem, torch, steps, wl.fault_and_free_surface(2_000_000), 3, (0.3, 100, 256, 0.05, False),
-                       5000, "elastic displacement FMM matvec, 2,000,000 points (fault + free surface), order 100, "
-                             "leaf 256, MAC 0.3")
-
-
-def laplace_run(tbem, torch, steps):
-    """The mixed-precision Laplace workload: single-layer FMM matvec on 4M points
-    (half uniform in the unit cube, half in 8 Gaussian clusters, sigma 0.02), order
-    75, leaf 100, MAC 0.3, errors on a 10k-target subsample."""
-    wl = importlib.import_module("3bem_amd.workloads")
-    return compare_run(tbem, torch, steps, wl.cube_and_clusters(4_000_000), 0, (0.3, 75, 100, 0.05, False),
-                       10_000, "Laplace single-layer FMM matvec, 4,000,000 points (cube + 8 clusters), order 75, "
-                               "leaf 100, MAC 0.3")
+    gomp = ctypes.CDLL("libgomp.so.1")
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    med = {}
+    for nt in (threads, 1):
+        gomp.omp_set_num_threads(nt)
+        op.apply(x)  # warm
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            op.apply(x)
+            times.append(time.perf_counter() - t0)
+        med[nt] = statistics.median(times)
+    gomp.omp_set_num_threads(threads)
+    return {"value": round(n_sample / med[threads] / 1e6, 5), "unit": "Mpts/s", "cores": threads, "kind": kind,
+            "value_1_thread": round(n_sample / med[1] / 1e6, 5), "cpu_model": cpu_model(),
+            "sample": f"{n_sample} points of the same geometry (fault + free surface), same kernel and FMMConfig; "
+                      f"FMMOperator::apply median of {runs} warm runs at {threads} OpenMP threads "
+                      f"({med[threads]:.3f} s) and at 1 thread ({med[1]:.3f} s); plan build {t_plan:.2f} s excluded"}
 
 
 def forward_model_run(tbem, torch, steps):
@@ -428,18 +395,127 @@ def near_field_run(tbem, torch, steps, warmup, skip_cpu):
     return out
 
 
+def fmm_rooflines(info, n, kid, prec, leaf_ms, m2l_ms, wname=None):
+    """Rooflines of the two kernels that dominate a matvec: the fused P2P+M2P+L2P
+    leaf stage (algorithmic flops F_K per point pair, SURVEY's count) and the M2L
+    translation kernel (FP64: F_K per surface-point pair, n_e^2 per cell pair, kernel
+    evaluated on the fly; mixed: 2 blk^2 flops per cell pair on FP32 MFMA)."""
+    n_e, R = info["n_e"], (1 if kid in (0, 1, 2, 7) else 3)
+    out = {}
+    flop = FLOPS_PER_PAIR[kid] * (info["p2p_pairs"] + n * n_e)
+    ach = flop / (leaf_ms * 1e-3) / 1e12
+    if prec == "fp64":
+        out["leaf"] = {"bound": "mfma", "pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
+                       "kernel": "k_leaf_stage<double> (P2P+M2P+L2P)", "peak": PEAK_FP64_TFLOPS}
+    else:
+        out["leaf"] = {"bound": "mfma", "pipe": "FP32 VALU; the published 157.3 TFLOP/s counts packed FP32, which "
+                                                "this build leaves off, so 78.6 TFLOP/s is its issue ceiling",
+                       "kernel": "k_leaf_stage<float> (P2P in FP32, M2P+L2P FP64)", "peak": PEAK_FP32_TFLOPS}
+    out["leaf"].update({"achieved": round(ach, 3), "unit": "TFLOP/s", "frac": round(ach / out["leaf"]["peak"], 4),
+                        "kernel_avg_ms": round(leaf_ms, 4), "algorithmic_flop": flop,
+                        "flop_rule": f"{FLOPS_PER_PAIR[kid]} flop per (target, source) pair: P2P pairs + L2P pairs"})
+    if m2l_ms > 0:
+        if prec == "fp64" or info["m2l_classes"] == 0:
+            flop = FLOPS_PER_PAIR[kid] * info["m2l"] * n_e * n_e
+            ach = flop / (m2l_ms * 1e-3) / 1e12
+            out["m2l"] = {"bound": "mfma", "pipe": "FP64 VALU", "kernel": "k_surf_stage (FP64 M2L, kernel on the fly)",
+                          "peak": PEAK_FP64_TFLOPS, "flop_rule": f"{FLOPS_PER_PAIR[kid]} flop per surface-point pair, "
+                                                                 "n_e^2 pairs per M2L cell pair"}
+        else:
+            blk = R * n_e
+            flop = 2.0 * info["m2l"] * blk * blk
+            ach = flop / (m2l_ms * 1e-3) / 1e12
+            out["m2l"] = {"bound": "mfma", "pipe": "FP32 MFMA (v_mfma_f32_16x16x4f32)",
+                          "kernel": "k_m2l_mfma (FP32 MFMA M2L, translation classes)", "peak": PEAK_FP32_TFLOPS,
+                          "flop_rule": "2 blk^2 flop per M2L cell pair", "classes": info["m2l_classes"]}
+        out["m2l"].update({"achieved": round(ach, 3), "unit": "TFLOP/s", "frac": round(ach / out["m2l"]["peak"], 4),
+                           "kernel_avg_ms": round(m2l_ms, 4), "algorithmic_flop": flop, "pairs": info["m2l"]})
+    for k, r in out.items():
+        r["traffic"] = TRAFFIC_BYTES.get((wname, n, prec, k))
+        r["traffic_unit"] = TRAFFIC_UNIT
+    return out
+
+
+def run_precision(tbem, torch, K, data, cfg, x, prec, steps, warmup, bwd_steps, wname=None):
+    """Builds the operator, times `steps` matvecs after `warmup` (HIP events on the
+    handle's stream give each launch's leaf-stage and M2L-kernel time), then the
+    backward as autograd runs it. Returns (result dict, operator, output)."""
+    t0 = time.perf_counter()
+    op = tbem.FMMOperator(K, data, cfg, precision=prec)
+    plan = time.perf_counter() - t0
+    per = []
+    dt, per = time_steps(lambda: op.apply(x), steps, warmup, torch,
+                         after=lambda: (op.stage_times()["leaf"], op.m2l_kernel_ms()))
+    stages = op.stage_times()
+    y = op.apply(x)
+    info = op.info()
+    R = K.dim
+    n_obs = data.n_obs
+    gv = torch.randn(R * n_obs, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(7))
+    dt_adj, dt_grad, dt_grad_re = backward_times(op, x, gv, bwd_steps, torch)
+    leaf_ms = float(np.mean([p[0] for p in per]))
+    m2l_ms = float(np.mean([p[1] for p in per]))
+    res = {"ms_per_step": round(dt * 1e3, 4), "Mpts_per_s": round(n_obs / dt / 1e6, 3),
+           "bwd_fwd_ratio": round((dt_adj + dt_grad) / dt, 3),
+           "bwd_ms": {"adjoint": round(dt_adj * 1e3, 3), "obs_grad": round(dt_grad * 1e3, 3),
+                      "obs_grad_recomputing": round(dt_grad_re * 1e3, 3)},
+           "plan_build_s": round(plan, 3), "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+           "rooflines": fmm_rooflines(info, n_obs, K.kernel_id, prec, leaf_ms, m2l_ms, wname),
+           "m2l_output_GB": round(info["m2l_output_bytes"] / 1e9, 3), "work_GB": round(info["work_bytes"] / 1e9, 3)}
+    return res, op, y, info
+
+
+def workload_pair(tbem, torch, w, steps, warmup, n_sub, seed=123):
+    """FP64 and mixed precision on one workload: both timed, both against the FP64
+    GPU direct sum on an n_sub-target subsample and against each other."""
+    kid = w["kernel"]
+    n = len(w["locs"])
+    K = tbem.make_kernel(kid, 1.0, 0.25)
+    R = K.dim
+    data = tbem.NBodyData(w["locs"], w["normals"], w["locs"], w["normals"], w["weights"])
+    cfg = tbem.FMMConfig(*w["cfg"])
+    x = torch.tensor(w["x"], device="cuda")
+    sub = np.sort(np.random.default_rng(seed).choice(n, min(n, n_sub), replace=False))
+    dsub = tbem.NBodyData(w["locs"][sub], w["normals"][sub], w["locs"], w["normals"], w["weights"])
+    t0 = time.perf_counter()
+    yd = tbem.make_direct_nbody_operator(dsub, K).apply(w["x"])
+    t_direct = time.perf_counter() - t0
+    idx = np.concatenate([q * n + sub for q in range(R)])
+    out, ys, infos = {}, {}, {}
+    for prec in ("fp64", "mixed"):
+        res, op, y, infos[prec] = run_precision(tbem, torch, K, data, cfg, x, prec, steps, warmup,
+                                                max(2, steps // 4), w.get("name"))
+        ys[prec] = y.cpu().numpy()
+        res["rel_err_vs_direct"] = float(np.linalg.norm(ys[prec][idx] - yd) / np.linalg.norm(yd))
+        out[prec] = res
+        log(f"[bench] {w['name']}, {n} points, {prec}: {res['ms_per_step']:.2f} ms ({res['Mpts_per_s']:.1f} Mpts/s), "
+            f"rel err {res['rel_err_vs_direct']:.2e}, bwd/fwd {res['bwd_fwd_ratio']}, "
+            f"leaf {res['rooflines']['leaf']['kernel_avg_ms']:.2f} ms, stages {res['stage_ms']}")
+        del op
+        torch.cuda.empty_cache()
+    out["rel_err_mixed_vs_fp64"] = float(np.linalg.norm(ys["mixed"] - ys["fp64"]) / np.linalg.norm(ys["fp64"]))
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    out["direct_subsample"] = {"targets": int(len(sub)), "s": round(t_direct, 3)}
+    out["info"] = {k: infos["fp64"][k] for k in ("n_e", "n_src_cells", "n_up_levels", "p2p", "m2l", "p2p_pairs")}
+    out["info"]["m2l_classes_mixed"] = infos["mixed"]["m2l_classes"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="fault_surface")
+    ap.add_argument("--config", default="fault_surface_o100",
+                    help="headline workload: fault_surface_o100 (default), fault_surface, icosphere, "
+                         "cube_clusters, rough_fault")
     ap.add_argument("--n", type=int, default=0)
-    ap.add_argument("--cpu-n", type=int, default=100_000)
+    ap.add_argument("--cpu-n", type=int, default=8_000)
+    ap.add_argument("--cpu-runs", type=int, default=10)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-extra", action="store_true", help="FP64 forward only")
-    ap.add_argument("--skip-target", action="store_true",
-                    help="skip the 2M-point order-100 FP32/MFMA vs FP64 comparison")
+    ap.add_argument("--skip-extra", action="store_true", help="the headline workload only")
+    ap.add_argument("--skip-cfg2", action="store_true",
+                    help="skip the 1M-point order-6 fault + free-surface workload")
     ap.add_argument("--skip-laplace", action="store_true",
                     help="skip the 4M-point Laplace FP32/MFMA-vs-FP64 comparison")
     ap.add_argument("--skip-forward-model", action="store_true",
@@ -448,13 +524,9 @@ def main():
                     help="skip the 2M-point hypersingular GMRES solve")
     ap.add_argument("--skip-near", action="store_true",
                     help="skip the CSR near field / integral operator section")
-    ap.add_argument("--only-near", action="store_true",
-                    help="run only the headline FP64 step and the near field / integral operator section")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
-    if args.only_near:
-        args.skip_target = args.skip_laplace = args.skip_forward_model = args.skip_bem_solve = True
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); the FMM "
               "sharding over several GPUs is not implemented", file=sys.stderr)
@@ -474,98 +546,83 @@ def main():
         return
     w = workload(args.config, args.n)
     kid = w["kernel"]
-    R = 1 if kid in (0, 1, 2, 7) else 3
     n = len(w["locs"])
     K = tbem.make_kernel(kid, 1.0, 0.25)
     data = tbem.NBodyData(w["locs"], w["normals"], w["locs"], w["normals"], w["weights"])
     cfg = tbem.FMMConfig(*w["cfg"])
     log(f"[bench] {w['desc']}")
-
-    t0 = time.perf_counter()
-    op64 = tbem.FMMOperator(K, data, cfg, precision="fp64")
-    plan64 = time.perf_counter() - t0
-    info = op64.info()
-    log(f"[bench] fp64 plan {plan64:.2f}s: " + ", ".join(f"{k}={info[k]}" for k in
-        ("n_e", "n_src_cells", "n_up_levels", "p2p", "m2l", "p2p_pairs")))
     x = torch.tensor(w["x"], device="cuda")
 
-    leaf_ms = []
-    dt64, leaf_ms = time_steps(lambda: op64.apply(x), args.steps, args.warmup, torch,
-                               after=lambda: op64.stage_times()["leaf"])
-    stages = op64.stage_times()
+    # the headline: plain FP64 matvec, inputs resident in HBM
+    res64, op64, y64, info = run_precision(tbem, torch, K, data, cfg, x, "fp64", args.steps, args.warmup,
+                                           max(2, args.steps // 4), args.config)
+    dt64 = res64["ms_per_step"] * 1e-3
     value = n / dt64 / 1e6
-    log(f"[bench] fp64 apply {dt64 * 1e3:.2f} ms -> {value:.2f} Mpts/s; stages {stages}")
-
-    leaf_avg_ms = float(np.mean(leaf_ms))
-    alg_flops = FLOPS_PER_PAIR[kid] * (info["p2p_pairs"] + n * info["n_e"])
-    achieved = alg_flops / (leaf_avg_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "pipe": "fp64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
-                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                "traffic": LEAF_TRAFFIC_BYTES.get((args.config, n)), "traffic_unit": "bytes per launch",
-                "kernel": "k_leaf_stage (P2P+M2P+L2P)", "kernel_avg_ms": round(leaf_avg_ms, 4),
-                "algorithmic_flop": alg_flops}
-
+    log(f"[bench] fp64 {res64['ms_per_step']:.2f} ms -> {value:.2f} Mpts/s; stages {res64['stage_ms']}")
+    rl = res64["rooflines"]
+    dom = "m2l" if "m2l" in rl and rl["m2l"]["kernel_avg_ms"] > rl["leaf"]["kernel_avg_ms"] else "leaf"
+    roofline = dict(rl[dom])
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(dt64 * 1e3, 4), "higher_is_better": True,
+           "warmup": args.warmup, "ms_per_step": res64["ms_per_step"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": w["desc"], "n_points": n, "kernel": type(K).__name__,
-                      "mac": cfg.mac, "order": cfg.order, "n_e": info["n_e"], "min_pts_per_cell": cfg.min_pts_per_cell,
-                      "plan_build_s": round(plan64, 3), "stage_ms": {k: round(v, 4) for k, v in stages.items()}},
+           "config": {"workload": w["desc"], "n_points": n, "kernel": type(K).__name__, "mac": cfg.mac,
+                      "order": cfg.order, "n_e": info["n_e"], "min_pts_per_cell": cfg.min_pts_per_cell,
+                      "plan_build_s": res64["plan_build_s"], "stage_ms": res64["stage_ms"]},
            "roofline": roofline}
+    del op64
+    torch.cuda.empty_cache()
 
-    if not args.skip_extra and not args.only_near:
-        # accuracy vs FP64 direct sum on a 10k-target subsample
-        rng = np.random.default_rng(123)
-        sub = np.sort(rng.choice(n, min(n, 10_000), replace=False))
+    if not args.skip_extra:
+        # the same workload in mixed precision, errors of both against the direct sum
+        R = K.dim
+        sub = np.sort(np.random.default_rng(123).choice(n, min(n, 10_000), replace=False))
         dsub = tbem.NBodyData(w["locs"][sub], w["normals"][sub], w["locs"], w["normals"], w["weights"])
-        direct = tbem.make_direct_nbody_operator(dsub, K)
         t0 = time.perf_counter()
-        yd = direct.apply(w["x"])
+        yd = tbem.make_direct_nbody_operator(dsub, K).apply(w["x"])
         t_direct = time.perf_counter() - t0
         idx = np.concatenate([q * n + sub for q in range(R)])
-        y64 = op64.apply(x).cpu().numpy()
-        e64 = float(np.linalg.norm(y64[idx] - yd) / np.linalg.norm(yd))
+        y64h = y64.cpu().numpy()
+        e64 = float(np.linalg.norm(y64h[idx] - yd) / np.linalg.norm(yd))
+        res32, op32, y32, info32 = run_precision(tbem, torch, K, data, cfg, x, "mixed", args.steps, args.warmup,
+                                                 max(2, args.steps // 4), args.config)
+        y32h = y32.cpu().numpy()
+        res32["rel_err_vs_direct"] = float(np.linalg.norm(y32h[idx] - yd) / np.linalg.norm(yd))
+        res32["rel_err_vs_fp64_fmm"] = float(np.linalg.norm(y32h - y64h) / np.linalg.norm(y64h))
+        res32["speedup_vs_fp64"] = round(res64["ms_per_step"] / res32["ms_per_step"], 3)
+        res32["m2l_classes"] = info32["m2l_classes"]
+        del op32
+        torch.cuda.empty_cache()
+        res64["rel_err_vs_direct"] = e64
+        out["fp64"] = res64
+        out["mixed"] = res32
+        out["rel_err_fp64_vs_direct"] = e64
+        out["bwd_fwd_ratio"] = res64["bwd_fwd_ratio"]
+        out["speedup_mixed_vs_fp64"] = res32["speedup_vs_fp64"]
+        out["direct_subsample"] = {"targets": int(len(sub)), "s": round(t_direct, 3)}
+        log(f"[bench] mixed {res32['ms_per_step']:.2f} ms ({res32['speedup_vs_fp64']}x), rel err fp64 {e64:.2e}, "
+            f"mixed {res32['rel_err_vs_direct']:.2e}; bwd/fwd fp64 {res64['bwd_fwd_ratio']}, mixed "
+            f"{res32['bwd_fwd_ratio']}")
+    del y64
+    torch.cuda.empty_cache()
 
-        t0 = time.perf_counter()
-        op32 = tbem.FMMOperator(K, data, cfg, precision="mixed")
-        plan32 = time.perf_counter() - t0
-        dt32, _ = time_steps(lambda: op32.apply(x), args.steps, args.warmup, torch)
-        y32 = op32.apply(x).cpu().numpy()
-        e32 = float(np.linalg.norm(y32[idx] - yd) / np.linalg.norm(yd))
-        e32_64 = float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
-
-        g = torch.randn(R * n, dtype=torch.float64, device="cuda")
-        dt_adj, dt_grad, dt_grad_re = backward_times(op64, x, g, max(2, args.steps // 2), torch)
-        out.update({
-            "rel_err_fp64_vs_direct": e64,
-            "mixed": {"value": round(n / dt32 / 1e6, 3), "ms_per_step": round(dt32 * 1e3, 4), "rel_err_vs_direct": e32,
-                      "rel_err_vs_fp64_fmm": e32_64, "speedup_vs_fp64": round(dt64 / dt32, 3),
-                      "plan_build_s": round(plan32, 3)},
-            "bwd_fwd_ratio": round((dt_adj + dt_grad) / dt64, 3),
-            "bwd_ms": {"adjoint": round(dt_adj * 1e3, 3), "obs_grad": round(dt_grad * 1e3, 3),
-                       "obs_grad_recomputing": round(dt_grad_re * 1e3, 3)},
-            "direct_subsample_s": round(t_direct, 3),
-        })
-        log(f"[bench] rel err fp64 {e64:.3e}, mixed {e32:.3e}; mixed {dt32 * 1e3:.2f} ms; "
-            f"adjoint {dt_adj * 1e3:.2f} ms, obs_grad {dt_grad * 1e3:.2f} ms")
-
-    if not args.skip_extra and not args.skip_target and args.config == "fault_surface":
-        out["target_fp32_mfma_vs_fp64"] = target_run(tbem, torch, max(3, args.steps // 2))
-    if not args.skip_extra and not args.skip_laplace and args.config == "fault_surface":
-        out["laplace_4M_fp32_mfma_vs_fp64"] = laplace_run(tbem, torch, 3)
-    if not args.skip_extra and not args.skip_forward_model and args.config == "fault_surface":
+    extra = not args.skip_extra and args.config == "fault_surface_o100"
+    if extra and not args.skip_cfg2:
+        out["fault_surface_1M_order6"] = workload_pair(tbem, torch, workload("fault_surface", 1_000_000),
+                                                       args.steps, args.warmup, 10_000)
+    if extra and not args.skip_laplace:
+        out["laplace_4M_fp32_mfma_vs_fp64"] = workload_pair(tbem, torch, workload("cube_clusters", 4_000_000), 3, 1,
+                                                            10_000)
+    if extra and not args.skip_forward_model:
         out["forward_model_fault_stations"] = forward_model_run(tbem, torch, 5)
-    if not args.skip_extra and not args.skip_bem_solve and args.config == "fault_surface":
+    if extra and not args.skip_bem_solve:
         out["bem_solve_rough_fault"] = bem_solve_run(tbem, torch)
-    if not args.skip_extra and not args.skip_near and args.config == "fault_surface":
+    if extra and not args.skip_near:
         out["integral_operator_near_field"] = near_field_run(tbem, torch, args.steps, args.warmup, args.skip_cpu)
 
     if not args.skip_cpu:
         cfg_t = (cfg.mac, cfg.order, cfg.min_pts_per_cell, cfg.d, cfg.account_for_small_cells)
-        cpu_n = min(args.cpu_n, n)
-        log(f"[bench] CPU baseline on {cpu_n} points ...")
-        out["cpu_baseline"] = cpu_baseline(kid, cfg_t, cpu_n)
+        log(f"[bench] CPU baseline on {args.cpu_n} points ...")
+        out["cpu_baseline"] = cpu_baseline(kid, cfg_t, min(args.cpu_n, n), args.cpu_runs)
         log(f"[bench] CPU baseline {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
