@@ -10,6 +10,8 @@
 // result is deterministic. The reference order (P2P before M2L etc.,
 // cpp/fmm.cpp:459-545) only changes how the same terms are summed.
 #include <algorithm>
+#include <atomic>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -47,6 +49,9 @@ struct FmmPlan {
     TaskLists tl;
     // forward lists
     DevList p2m, m2l, p2l, leaf_p2p, leaf_m2p;
+    // every M2L pair's surfaces lie farther apart than the kernels' r2 < 1e-12
+    // cutoff (m2l_clear_of_cutoff), so the FP64 M2L runs without the test
+    bool m2l_no_cutoff = false;
     std::vector<DevList> m2m;    // by source level: parents at L, entries = children
     std::vector<DevList> l2l;    // by observation level L >= 1: cells at L, entry = parent
     DevBuf<int> obs_leaves, src_leaves;
@@ -64,7 +69,7 @@ struct FmmPlan {
     mutable std::mutex adj_mu;
     mutable std::unique_ptr<AdjLists> adj;
     const AdjLists& adjoint_lists(hipStream_t s) const;
-    M2lMfma m2l_mfma;            // mixed precision: M2L as grouped FP32 MFMA GEMMs
+    M2lMfma m2l_mfma;            // M2L as grouped MFMA GEMMs: FP32 modal (mixed) or FP64 (fp64)
     // GEMM tiles of the C2E products: the cells each stage writes
     CellTiles t_p2m, t_m2l, t_p2l;
     std::vector<CellTiles> t_m2m;          // by source level: the M2M parents
@@ -94,6 +99,7 @@ struct FmmPlan {
     // and enqueued one at a time under y_mu.
     mutable std::mutex y_mu;
     DevBuf<float> m2l_y;
+    DevBuf<double> m2l_y64;         // the FP64 path's class M2L output
     hipEvent_t y_done = nullptr;
     FmmPlan() = default;
     FmmPlan(const FmmPlan&) = delete;
@@ -146,7 +152,7 @@ struct Work {
         locals.alloc((size_t)P.dot.n_cells * P.blk);
         out_sorted.alloc(P.R * P.n_obs);
         fld.alloc((size_t)std::max(P.dst.n_cells, P.dot.n_cells) * P.blk);
-        if (P.m2l_mfma.enabled) {
+        if (P.m2l_mfma.enabled && !P.m2l_mfma.fp64) {
             chk_up.alloc((size_t)P.dst.n_cells * P.blk);
             chk_up.zero(stream);  // cells without a P2M/M2M stage keep a zero potential
             TBEM_HIP(hipStreamSynchronize(stream));
@@ -263,6 +269,39 @@ static HostList self_list(const HostTree& t, const std::vector<int>& cells) {
     return l;
 }
 
+// Whether every M2L pair (t, s) keeps its surfaces (the inner ones: target down-check
+// and source up-equivalent, radius rin * level_r about each center) farther apart
+// than the kernels' 1e-6 cutoff radius with a wide margin:
+//   |c_t - c_s| - rin (r_t + r_s) > max(1e-5, 1e-9 * |c|max).
+// The margin covers the rounding of the FP64 displacements (a few ulp of the
+// coordinates), so that no computed r2 can fall below 1e-12 and the test in the
+// kernels never fires: leaving it out changes no result.
+static bool m2l_clear_of_cutoff(const HostList& l, const HostTree& ot, const HostTree& st, const Surfaces& surf) {
+    double rin = 0;
+    for (int i = 0; i < surf.n_e; i++) {
+        const double* u = &surf.inner[3 * i];
+        rin = std::max(rin, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    }
+    double cmax = 0;
+    for (const HostTree* t : {&ot, &st})
+        for (int c = 0; c < t->n_cells; c++)
+            cmax = std::max({cmax, std::fabs(t->cx[c]), std::fabs(t->cy[c]), std::fabs(t->cz[c])});
+    const double need = std::max(1e-5, 1e-9 * cmax);
+    const int n = (int)l.items.size();
+    std::atomic<int> bad{0};
+    parallel_for((size_t)n, [&](size_t k) {
+        const int t = l.items[k];
+        const double rt = rin * ot.level_r[ot.level[t]];
+        for (int p = l.ptr[k]; p < l.ptr[k + 1]; p++) {
+            const int c = l.cell[p];
+            const double dx = ot.cx[t] - st.cx[c], dy = ot.cy[t] - st.cy[c], dz = ot.cz[t] - st.cz[c];
+            const double gap = std::sqrt(dx * dx + dy * dy + dz * dz) - rt - rin * st.level_r[st.level[c]];
+            if (!(gap > need)) bad.store(1, std::memory_order_relaxed);
+        }
+    });
+    return bad.load() == 0;
+}
+
 static void validate(const tbem_fmm_desc* d) {
     TBEM_REQUIRE(d != nullptr, TBEM_E_INVALID, "null descriptor");
     TBEM_REQUIRE(kernel_valid(d->kernel), TBEM_E_INVALID, "unknown kernel id");
@@ -272,6 +311,8 @@ static void validate(const tbem_fmm_desc* d) {
                  "precision must be TBEM_FP64 or TBEM_MIXED");
     TBEM_REQUIRE(d->tree_boxes >= TBEM_TREES_AUTO && d->tree_boxes <= TBEM_TREES_COMMON_BOX, TBEM_E_INVALID,
                  "tree_boxes must be TBEM_TREES_AUTO, _OWN_BOXES or _COMMON_BOX");
+    TBEM_REQUIRE(d->m2l_operators == TBEM_M2L_AUTO || d->m2l_operators == TBEM_M2L_SURFACES, TBEM_E_INVALID,
+                 "m2l_operators must be TBEM_M2L_AUTO or TBEM_M2L_SURFACES");
     TBEM_REQUIRE(d->n_obs < (1u << 30) && d->n_src < (1u << 30), TBEM_E_UNSUPPORTED,
                  "more than 2^30 points per side");
     TBEM_REQUIRE(d->n_obs == 0 || (d->obs_locs && d->obs_normals), TBEM_E_INVALID, "null observation arrays");
@@ -699,14 +740,27 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     const HostList m2l_h = list_from_pairs(cells_with(tl.m2l, 0, OT.n_cells), OT.n_cells, tl.m2l, 0, nullptr);
     P->m2l.upload(m2l_h, s);
     P->t_m2l = make_cell_tiles(m2l_h.items, OT, s);
+    P->m2l_no_cutoff = m2l_clear_of_cutoff(m2l_h, OT, ST, P->surf);
     clk.mark("lists");
-    if (P->precision == TBEM_MIXED)
-        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, P->dot, P->dst, P->tree_layout != kTreesOwnBoxes, m2l_h,
-                       P->m2l, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac, down_fac);
+    // FP64: class operators where the classes exist and their per-pair output fits an
+    // eighth of the device memory
+    bool fp64_classes = false;
+    if (P->precision == TBEM_FP64 && d->m2l_operators == TBEM_M2L_AUTO && P->tree_layout != kTreesOwnBoxes &&
+        m2l_fp64_size_ok(P->blk)) {
+        size_t free_b = 0, total_b = 0;
+        TBEM_HIP(hipMemGetInfo(&free_b, &total_b));
+        fp64_classes = m2l_output_bytes((int)m2l_h.cell.size(), P->blk, true) <= total_b / 8;
+    }
+    if (P->precision == TBEM_MIXED || fp64_classes)
+        build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, P->dot, P->dst, P->tree_layout != kTreesOwnBoxes,
+                       fp64_classes, m2l_h, P->m2l, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac, down_fac);
     clk.mark("m2l_mfma");
     TBEM_HIP(hipEventCreateWithFlags(&P->y_done, hipEventDisableTiming));
     TBEM_HIP(hipEventRecord(P->y_done, s));
-    if (P->m2l_mfma.enabled) P->m2l_y.alloc((size_t)P->m2l_mfma.n_pairs * P->m2l_mfma.ystride);
+    if (P->m2l_mfma.enabled) {
+        if (P->m2l_mfma.fp64) P->m2l_y64.alloc((size_t)P->m2l_mfma.n_pairs * P->m2l_mfma.ystride);
+        else P->m2l_y.alloc((size_t)P->m2l_mfma.n_pairs * P->m2l_mfma.ystride);
+    }
     P->d_flag.alloc(1);
     const std::vector<int> p2l_items = cells_with(tl.p2l, 0, OT.n_cells);
     P->p2l.upload(list_from_pairs(p2l_items, OT.n_cells, tl.p2l, 0, &ST), s);
@@ -822,12 +876,20 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
         TBEM_HIP(hipEventRecord(W.ev[9], s));
         launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.t_m2l, P.m2l_y.p, W.fld.p,
                           W.locals.p, s);
+    } else if (P.m2l_mfma.enabled && P.m2l_mfma.fp64) {
+        // FP64 class operators: check potential = sum over pairs of K_c mult, then DC2E
+        YLease y(P, s);
+        launch_m2l_mfma64(P.m2l_mfma, P.m2l.cell.p, W.mult.p, P.m2l_y64.p, s);
+        TBEM_HIP(hipEventRecord(W.ev[9], s));
+        launch_m2l_sum64(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.m2l_y64.p, W.fld.p, s);
+        down_c2e(P.t_m2l);
     } else {
         SurfStage m = surf_base(P);
         set_list(m, P.m2l);
         m.tt = view(P.dot); m.t_surf = P.d_inner.p;
         m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
         m.out = W.fld.p;
+        m.no_cutoff = P.m2l_no_cutoff;
         launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
         TBEM_HIP(hipEventRecord(W.ev[9], s));
         down_c2e(P.t_m2l);
@@ -983,12 +1045,18 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         YLease y(P, s);
         launch_m2l_mfma(P.m2l_mfma, nullptr, W.m2l_w.p, P.m2l_y.p, s, true);
         launch_m2l_reduce_adj(P.m2l_mfma, P.m2l_y.p, W.fld.p, W.e2.p, s);
+    } else if (P.m2l_mfma.enabled && P.m2l_mfma.fp64) {
+        // FP64 class operators: mbar[s] += sum over the pairs of s of K_c^T cbar[t]
+        YLease y(P, s);
+        launch_m2l_mfma64(P.m2l_mfma, nullptr, W.cbar.p, P.m2l_y64.p, s, true);
+        launch_m2l_sum64_adj(P.m2l_mfma, P.m2l_y64.p, W.mbar.p, s);
     } else {
         SurfStage m = surf_base(P);
         set_list(m, A.m2l);
         m.tt = view(P.dst); m.t_surf = P.d_inner.p;
         m.st = view(P.dot); m.s_surf = P.d_inner.p; m.s_coeff = W.cbar.p;
         m.out = W.mbar.p;
+        m.no_cutoff = P.m2l_no_cutoff;
         launch_surf_stage(P.kid, true, true, true, m, P.kc, s);
     }
     // M2P^T into mbar
@@ -1370,7 +1438,7 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
         o->p2p_pairs = P.p2p_pairs;
         o->m2l_classes = P.m2l_mfma.enabled ? P.m2l_mfma.n_classes : 0;
         o->plan_seconds = P.plan_seconds;
-        o->m2l_output_bytes = (double)P.m2l_y.n * sizeof(float);
+        o->m2l_output_bytes = (double)P.m2l_y.n * sizeof(float) + (double)P.m2l_y64.n * sizeof(double);
         const Work& W = *h->work;
         double wb = 0;
         for (const DevBuf<double>* b : {&W.dens, &W.mult, &W.locals, &W.out_sorted, &W.xbuf, &W.ybuf, &W.gsort,

@@ -246,12 +246,14 @@ inline KC<D3> make_kcd(const KConst& c) {
 // acc[R] += K(obs <- src) v   (ADJ = false)
 // acc[R] += K(obs <- src)^T v (ADJ = true)
 // d = src - obs, r2 = |d|^2; no / ns are the obs and src normals.
-template <int KID, bool ADJ, class T, bool FAST = false>
+// CUT = false leaves out the r2 < 1e-12 test, for callers that have shown that no
+// pair they pass is that close (the result is then the same bit for bit).
+template <int KID, bool ADJ, class T, bool FAST = false, bool CUT = true>
 __device__ __forceinline__ void kop(T dx, T dy, T dz, T r2, const T* no, const T* ns, const T* v,
                                     T* acc, const KC<T>& k) {
     if (KID == K_LH) return;
     if (KID == K_ID1 || KID == K_ID3) {
-        const T m = (r2 < cutoff_r2<T>()) ? T(0) : T(1);
+        const T m = (CUT && r2 < cutoff_r2<T>()) ? T(0) : T(1);
         acc[0] += m * v[0];
         if (KID == K_ID3) {
             acc[1] += m * v[1];
@@ -262,7 +264,7 @@ __device__ __forceinline__ void kop(T dx, T dy, T dz, T r2, const T* no, const T
     T ri;
     if constexpr (FAST) ri = rsqrt_fast(r2);
     else ri = rsqrt_t(r2);
-    ri = (r2 < cutoff_r2<T>()) ? T(0) : ri;
+    if constexpr (CUT) ri = (r2 < cutoff_r2<T>()) ? T(0) : ri;
     if (KID == K_LS) {
         acc[0] += ri * v[0];             // times 1/(4 pi) at the end
         return;

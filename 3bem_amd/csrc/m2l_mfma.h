@@ -17,11 +17,18 @@ namespace tbem {
 // input cannot survive, and in the modal basis they carry no cancellation. M_c is
 // stored pre-swizzled into v_mfma_f32_16x16x4f32 A-operand order:
 //   S[((c*mt + rt)*kt + ks)*64 + lane] = M_c[rt*16 + lane%16][ks*4 + lane/16].
+//
+// FP64 path (fp64): the same classes with K_c itself in FP64, S64[...] = K_c[...] in
+// the same order (v_mfma_f64_16x16x4f64 shares the A-operand layout), no modal
+// factors: Y[p] = K_c * mult[src(p)] in FP64, the target's check potential is the sum
+// of its pairs' Y, and the FP64 DC2E follows as on the surface-evaluation path.
 struct M2lMfma {
     bool enabled = false;
+    bool fp64 = false;
     int n_classes = 0, blk = 0, mt = 0, kt = 0, ystride = 0;
     int n_units = 0, n_pairs = 0;
     DevBuf<float> S, ST;    // M_c and M_c^T, A-operand order
+    DevBuf<double> S64, ST64;  // FP64 path: K_c and K_c^T, A-operand order
     DevBuf<double> Bu, Vd;  // per source / observation level, blk x blk row-major
     DevBuf<int> unit_class, unit_off, unit_cnt, colpair;
     DevBuf<int> pair_tgt;   // target cell of every pair
@@ -34,6 +41,10 @@ struct M2lMfma {
 // Whether the MFMA M2L serves operators of this size on the current device: blk
 // from 48 (below, the FP64 stage is faster) up to what one workgroup stages in LDS.
 bool m2l_mfma_size_ok(int blk);
+// The same for the FP64 operators (32 staged FP64 columns per unit).
+bool m2l_fp64_size_ok(int blk);
+// Bytes of the per-pair output Y for n_pairs pairs (16-row padded blk per pair).
+size_t m2l_output_bytes(int n_pairs, int blk, bool fp64);
 
 // m2l: the forward M2L list (items = target cells, cell[p] = source of pair p).
 // Leaves M.enabled false (the caller keeps the FP64 M2L) when the trees' cells are
@@ -43,8 +54,9 @@ bool m2l_mfma_size_ok(int blk);
 // (from the pseudo-inverse's own SVD, gpu_pinv_batched); without them the C2E
 // matrices are factored again.
 // dot, dst: the trees on the device; dm2l: the device copy of m2l.
+// fp64: build the FP64 path's operators (see above) instead of the mixed path's.
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    const DevTree& dot, const DevTree& dst, bool same_grid, const HostList& m2l,
+                    const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, const HostList& m2l,
                     const DevList& dm2l, const Surfaces& surf, const double* d_uc2e, const double* d_dc2e,
                     hipStream_t s, const DevBuf<double>* up_fac = nullptr, const DevBuf<double>* down_fac = nullptr);
 
@@ -53,6 +65,17 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
 // transpose: Y[p] = M_{class(p)}^T * chk_up[tgt(p)] (pair_src is then unused).
 void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s,
                      bool transpose = false);
+
+// FP64 path: Y[p] (ystride doubles per pair) = K_{class(p)} * x[src(p)] (x: blk
+// doubles per source cell, the up-equivalent densities); transpose:
+// Y[p] = K_{class(p)}^T * x[tgt(p)] (x per target cell).
+void launch_m2l_mfma64(const M2lMfma& M, const int* pair_src, const double* x, double* Y, hipStream_t s,
+                       bool transpose = false);
+// FP64 path: out[items[k]] = sum of Y over the pairs ptr[k] .. ptr[k+1] (pair order).
+void launch_m2l_sum64(const M2lMfma& M, int n_items, const int* items, const int* ptr, const double* Y, double* out,
+                      hipStream_t s);
+// FP64 path, transpose: out[s] += sum of Y over the pairs of source cell s.
+void launch_m2l_sum64_adj(const M2lMfma& M, const double* Y, double* out, hipStream_t s);
 
 // Transpose, input: w[t] = float(Vd[level t]^T lbar[t]) for every M2L target t
 // (targets: the M2L targets as GEMM tiles).

@@ -38,6 +38,8 @@ namespace tbem {
 namespace {
 
 constexpr int kUnitCols = 64;   // pairs per work unit (= LDS row length of the staged columns)
+constexpr int kUnitCols64 = 32; // the same for the FP64 operators
+constexpr int kPre64 = 8;       // k-steps of A loaded ahead by the FP64 kernel
 constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 KiB of LDS
 // Below this operator size the FP64 M2L stage is the cheaper one: measured on the
 // 1M-point elastic workload at order 6 (blk 18), the MFMA path with its per-pair
@@ -115,17 +117,19 @@ __global__ __launch_bounds__(256) void k_gemm64(int n, const double* __restrict_
         }
 }
 
-// FP64 row-major S_c (or its transpose) -> FP32 A-operand order (zero padded to
-// mt*16 x kt*4).
+// FP64 row-major S_c (or its transpose) -> A-operand order of v_mfma_f32_16x16x4f32
+// (T = float) or v_mfma_f64_16x16x4f64 (T = double), which share it; zero padded to
+// mt*16 x kt*4.
+template <class T>
 __global__ void k_m2l_swizzle(int n, int mt, int kt, int trans, const double* __restrict__ S64,
-                              float* __restrict__ S32) {
+                              T* __restrict__ out) {
     const int c = blockIdx.y;
     const size_t per = (size_t)mt * kt * 64;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x) {
         const int rt = (int)(e / ((size_t)kt * 64)), ks = (int)((e / 64) % kt), l = (int)(e % 64);
         const int row = rt * 16 + l % 16, col = ks * 4 + l / 16;
         const size_t src = trans ? (size_t)col * n + row : (size_t)row * n + col;
-        S32[c * per + e] = (row < n && col < n) ? (float)S64[(size_t)c * n * n + src] : 0.f;
+        out[c * per + e] = (row < n && col < n) ? (T)S64[(size_t)c * n * n + src] : T(0);
     }
 }
 
@@ -246,6 +250,102 @@ __global__ __launch_bounds__(256) void k_m2l_pair_sum(const int* __restrict__ it
     }
 }
 
+// ------------------------------------------------------------- FP64 operators
+// The FP64 path's M2L on the same classes: Y[p] = K_c * mult[src(p)] with K_c the
+// kernel between the target's down-check and the source's up-equivalent surface,
+// in FP64 (v_mfma_f64_16x16x4f64, FP64 accumulation), then the target's check
+// potential is the sum of its pairs' Y in pair order and the FP64 DC2E follows as in
+// the reference. Units are (class, up to 32 pairs): the staged columns (k x 32
+// doubles, 77 KB at blk 297) leave room for two workgroups per CU.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// One row tile (16 rows of K_c) times NCT (<= 2) column tiles; A from global memory
+// (L2-resident) loaded kPre k-steps ahead, continuing into the wave's next row tile
+// (An) as m2l_row_tile does.
+template <int NCT, int kPre>
+__device__ __forceinline__ void m2l_row_tile64(const double* __restrict__ A, const double* __restrict__ An,
+                                               const double* __restrict__ b, int kt, f64x4* acc, double* cur) {
+    double nxt[kPre];
+    for (int k0 = 0; k0 < kt; k0 += kPre) {
+        if (k0 + kPre >= kt) {
+            for (int i = 0; i < kPre; i++) nxt[i] = (An && i < kt) ? An[(size_t)i * 64] : 0.0;
+        } else {
+            for (int i = 0; i < kPre; i++) nxt[i] = k0 + kPre + i < kt ? A[(size_t)(k0 + kPre + i) * 64] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kPre; i++) {
+            if (k0 + i >= kt) break;
+            const double* bk = b + (k0 + i) * 4 * kUnitCols64;
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[i], bk[0], acc[0], 0, 0, 0);
+            if (NCT > 1) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[i], bk[16], acc[1], 0, 0, 0);
+        }
+        for (int i = 0; i < kPre; i++) cur[i] = nxt[i];
+    }
+}
+
+// One workgroup of NW waves per unit: the unit's source coefficients staged in LDS
+// (row k, column = pair, kpad x 32 doubles), then wave w takes row tiles w, w + NW, ...
+// The FP64 MFMA leaves D[4*r + lq][lr] in register r of lane (lq, lr) (not the f32
+// 16x16x4 order), so Y[pair][rt*16 + 4*r + lq] = D[4*r + lq][lr] of column tile ct
+// (pair ct*16 + lr): the four lanes lq of a pair write 32 contiguous bytes per r.
+template <int NW, int kPre>
+__global__ __launch_bounds__(NW * 64) void k_m2l_mfma64(int n, int mt, int kt, int ystride,
+                                                     const double* __restrict__ S, const int* __restrict__ ucls,
+                                                     const int* __restrict__ uoff, const int* __restrict__ ucnt,
+                                                     const int* __restrict__ colpair, const int* __restrict__ pair_src,
+                                                     const double* __restrict__ x, double* __restrict__ Y) {
+    extern __shared__ double Bd[];  // [kt*4][kUnitCols64]
+    const int u = blockIdx.x;
+    const int cls = uni(ucls[u]), off = uni(uoff[u]), cnt = uni(ucnt[u]);
+    if (cnt == 0) return;  // padding unit: uniform exit before any barrier
+    const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
+    const int kpad = kt * 4;
+    // stage: lane % 32 = column (pair), lane / 32 and the waves stride over k
+    const int c = lane & (kUnitCols64 - 1), h = lane >> 5;
+    const double* col = c < cnt ? x + (size_t)pair_src[colpair[off + c]] * n : nullptr;
+    for (int kk = 2 * wave + h; kk < kpad; kk += 2 * NW) Bd[kk * kUnitCols64 + c] = (col && kk < n) ? col[kk] : 0.0;
+    __syncthreads();
+    // B operand of column tile ct at k-step ks: row ks*4 + lq, column ct*16 + lr
+    const int lr = lane % 16, lq = lane / 16;
+    const double* b = Bd + lq * kUnitCols64 + lr;
+    const int nct = (cnt + 15) / 16;  // 1 or 2 (uniform)
+    const double* Acls = S + (size_t)cls * mt * kt * 64 + lane;
+    double cur[kPre];
+    for (int i = 0; i < kPre; i++) cur[i] = (wave < mt && i < kt) ? Acls[((size_t)wave * kt + i) * 64] : 0.0;
+    for (int rt = wave; rt < mt; rt += NW) {
+        f64x4 acc[2] = {};
+        const double* A = Acls + (size_t)rt * kt * 64;
+        const double* An = rt + NW < mt ? A + (size_t)NW * kt * 64 : nullptr;
+        if (nct == 1) m2l_row_tile64<1, kPre>(A, An, b, kt, acc, cur);
+        else m2l_row_tile64<2, kPre>(A, An, b, kt, acc, cur);
+        for (int ct = 0; ct < nct; ct++) {
+            const int nn = ct * 16 + lr;
+            if (nn < cnt) {
+                double* y = Y + (size_t)colpair[off + nn] * ystride + rt * 16 + lq;
+                for (int r = 0; r < 4; r++) y[4 * r] = acc[ct][r];
+            }
+        }
+    }
+}
+
+// out[t] (= or +=) sum over the pairs of item k of Y[p][0 .. blk), in pair order, FP64.
+// The pairs of item k are ptr[k] .. ptr[k+1], or pairs[ptr[k] .. ptr[k+1]] when given.
+__global__ __launch_bounds__(256) void k_m2l_pair_sum64(const int* __restrict__ items, const int* __restrict__ ptr,
+                                                        const int* __restrict__ pairs, const double* __restrict__ Y,
+                                                        int ystride, int blk, int add, double* __restrict__ out) {
+    const int k = blockIdx.x;
+    const int t = uni(items[k]), p0 = uni(ptr[k]), p1 = uni(ptr[k + 1]);
+    for (int row = threadIdx.x; row < blk; row += blockDim.x) {
+        double s = 0;
+        for (int j = p0; j < p1; j++) {
+            const size_t p = pairs ? (size_t)pairs[j] : (size_t)j;
+            s += Y[p * ystride + row];
+        }
+        double* o = out + (size_t)t * blk + row;
+        *o = add ? *o + s : s;
+    }
+}
+
 // Per-level modal factors of C2E = sum_i W_i V_i^T (rows of W and V from the
 // Jacobi SVD). Modes with |W_i| <= 1e-13 max_j |W_j| (the ones the pseudo-inverse
 // dropped) are zeroed. up: Bu rows V_i, Ru columns W_i; down: Ld rows s_i V_i,
@@ -289,10 +389,10 @@ __global__ void k_pair_targets(const int* __restrict__ items, const int* __restr
 
 // Translation class of pair p: (level t, level s, integer offset of the centres in
 // units of the finer half width per axis, each biased by 2^17 into 18 bits). bad is
-// set when an offset is not within 1e-6 of an integer (cells not on one grid) or
+// set when an offset is not within tol of an integer (cells not on one grid) or
 // out of range. idx[p] = p (the values the sort carries).
 __global__ void k_class_keys(int n, const int* __restrict__ tgt, const int* __restrict__ src, TreeView ot,
-                             const double* __restrict__ ohw, TreeView st, const double* __restrict__ shw,
+                             const double* __restrict__ ohw, TreeView st, const double* __restrict__ shw, double tol,
                              unsigned long long* __restrict__ key, int* __restrict__ idx, int* __restrict__ bad) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,7 +406,7 @@ __global__ void k_class_keys(int n, const int* __restrict__ tgt, const int* __re
         const double hf = fmin(ohw[3 * Lt + a], shw[3 * Ls + a]);
         const double q = (cs[a] - ct[a]) / hf;
         const double qi = rint(q);
-        ok = ok && fabs(q - qi) < 1e-6 && fabs(qi) < (double)(1 << 17);
+        ok = ok && fabs(q - qi) < tol && fabs(qi) < (double)(1 << 17);
         k |= (unsigned long long)((long long)qi + (1 << 17)) << (36 - 18 * a);
     }
     key[p] = k;
@@ -357,31 +457,47 @@ __global__ void k_gather_int(int n, const int* __restrict__ idx, const unsigned 
 
 }  // namespace
 
-bool m2l_mfma_size_ok(int n) {
+static size_t staged_bytes(int n, bool fp64) {
+    return (size_t)((n + 3) / 4) * 4 * (fp64 ? kUnitCols64 * sizeof(double) : kUnitCols * sizeof(float));
+}
+
+static bool size_ok(int n, bool fp64) {
     if (n > kMaxBlk || n < kMinBlk) return false;
     // the staged columns must fit one workgroup's LDS on this device
-    const size_t shm = (size_t)((n + 3) / 4) * 4 * kUnitCols * sizeof(float);
     int dev = 0, max_shm = 0;
     TBEM_HIP(hipGetDevice(&dev));
     TBEM_HIP(hipDeviceGetAttribute(&max_shm, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    return shm <= (size_t)max_shm;
+    return staged_bytes(n, fp64) <= (size_t)max_shm;
+}
+
+bool m2l_mfma_size_ok(int n) { return size_ok(n, false); }
+bool m2l_fp64_size_ok(int n) { return size_ok(n, true); }
+
+size_t m2l_output_bytes(int n_pairs, int blk, bool fp64) {
+    return (size_t)n_pairs * (size_t)((blk + 15) / 16 * 16) * (fp64 ? sizeof(double) : sizeof(float));
 }
 
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    const DevTree& dot, const DevTree& dst, bool same_grid, const HostList& m2l,
+                    const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, const HostList& m2l,
                     const DevList& dm2l, const Surfaces& surf, const double* d_uc2e, const double* d_dc2e,
                     hipStream_t s, const DevBuf<double>* up_fac, const DevBuf<double>* down_fac) {
     auto t0 = std::chrono::steady_clock::now();
     M = M2lMfma{};
     const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
     M.blk = n;
-    if (!same_grid || !m2l_mfma_size_ok(n) || m2l.cell.empty()) return;
-    const size_t shm = (size_t)((n + 3) / 4) * 4 * kUnitCols * sizeof(float);
+    M.fp64 = fp64;
+    if (!same_grid || !size_ok(n, fp64) || m2l.cell.empty()) return;
+    const size_t shm = staged_bytes(n, fp64);
     if (shm > 65536) {
-        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4, kPreNarrow>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)shm));
-        TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<8, kPreWide>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)shm));
+        if (fp64) {
+            TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma64<8, kPre64>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        } else {
+            TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4, kPreNarrow>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<8, kPreWide>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        }
     }
     const int n_pairs = (int)m2l.cell.size();
 
@@ -397,8 +513,12 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     TBEM_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_pair_targets, dim3(dm2l.n_items), dim3(64), 0, s, dm2l.items.p, dm2l.ptr.p, tgt.p);
     TBEM_LAUNCH_CHECK();
+    // FP64 operators serve every pair of a class from its first pair's geometry, so
+    // their classes must be exact to rounding: offsets within 1e-10 of the integers
+    // (a relative change of the kernel below 1e-10 even for touching cells; in
+    // practice the offsets are exact to ~1e-13). The FP32 operators allow 1e-6.
     hipLaunchKernelGGL(k_class_keys, dim3(gp), dim3(256), 0, s, n_pairs, tgt.p, dm2l.cell.p, view(dot), ohw.p,
-                       view(dst), shw.p, key.p, colpair.p, bad.p);
+                       view(dst), shw.p, fp64 ? 1e-10 : 1e-6, key.p, colpair.p, bad.p);
     TBEM_LAUNCH_CHECK();
     int h_bad = 0;
     TBEM_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -429,7 +549,7 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     // modal factors of UC2E (source tree levels) and DC2E (observation tree levels)
     const size_t nn = (size_t)n * n, per = (size_t)mt * kt * 64;
     DevBuf<double> Ru, Ld;
-    {
+    if (!fp64) {
         DevBuf<double> W, V;
         const bool have_up = up_fac && up_fac[0].n && up_fac[1].n, have_down = down_fac && down_fac[0].n && down_fac[1].n;
         if (!have_up) gpu_svd_batched(d_uc2e, st.n_levels, n, W, V, s);
@@ -447,11 +567,17 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
         TBEM_HIP(hipStreamSynchronize(s));
     }
 
-    // operators M_c = Ld * K_c * Ru, in chunks of classes (FP64 temporaries <= ~0.5 GB each)
-    M.S.alloc((size_t)nc * per);
-    M.ST.alloc((size_t)nc * per);
+    // operators M_c = Ld * K_c * Ru (FP64 path: K_c itself), in chunks of classes
+    // (FP64 temporaries <= ~0.5 GB each)
+    if (fp64) {
+        M.S64.alloc((size_t)nc * per);
+        M.ST64.alloc((size_t)nc * per);
+    } else {
+        M.S.alloc((size_t)nc * per);
+        M.ST.alloc((size_t)nc * per);
+    }
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nc, (size_t)kMaxChunk, (size_t)(5e8 / (nn * 8))}));
-    DevBuf<double> K((size_t)chunk * nn), T1((size_t)chunk * nn), S64((size_t)chunk * nn);
+    DevBuf<double> K((size_t)chunk * nn), T1(fp64 ? 0 : (size_t)chunk * nn), S64(fp64 ? 0 : (size_t)chunk * nn);
     DevBuf<double> d_tc, d_sc, d_tr, d_sr, d_inner, d_unit;
     DevBuf<int> d_iota, d_ls, d_lt;
     d_inner.upload(surf.inner, s);
@@ -478,15 +604,25 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
                            dim3(256), 0, s, kid, kc, n_e, R, d_tc.p, d_sc.p, d_tr.p, d_sr.p, d_inner.p, d_unit.p,
                            K.p);
         TBEM_LAUNCH_CHECK();
+        const dim3 gs(std::max(1, std::min(256, div_up(per, 256))), m);
+        if (fp64) {
+            hipLaunchKernelGGL(k_m2l_swizzle<double>, gs, dim3(256), 0, s, n, mt, kt, 0, K.p,
+                               M.S64.p + (size_t)c0 * per);
+            TBEM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_m2l_swizzle<double>, gs, dim3(256), 0, s, n, mt, kt, 1, K.p,
+                               M.ST64.p + (size_t)c0 * per);
+            TBEM_LAUNCH_CHECK();
+            TBEM_HIP(hipStreamSynchronize(s));  // the host vectors above are reused next chunk
+            continue;
+        }
         const dim3 gg(div_up(n, 64), div_up(n, 64), m);
         hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, K.p, d_iota.p, Ru.p, d_ls.p, T1.p);
         TBEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_gemm64, gg, dim3(256), 0, s, n, Ld.p, d_lt.p, T1.p, d_iota.p, S64.p);
         TBEM_LAUNCH_CHECK();
-        const dim3 gs(std::max(1, std::min(256, div_up(per, 256))), m);
-        hipLaunchKernelGGL(k_m2l_swizzle, gs, dim3(256), 0, s, n, mt, kt, 0, S64.p, M.S.p + (size_t)c0 * per);
+        hipLaunchKernelGGL(k_m2l_swizzle<float>, gs, dim3(256), 0, s, n, mt, kt, 0, S64.p, M.S.p + (size_t)c0 * per);
         TBEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_m2l_swizzle, gs, dim3(256), 0, s, n, mt, kt, 1, S64.p, M.ST.p + (size_t)c0 * per);
+        hipLaunchKernelGGL(k_m2l_swizzle<float>, gs, dim3(256), 0, s, n, mt, kt, 1, S64.p, M.ST.p + (size_t)c0 * per);
         TBEM_LAUNCH_CHECK();
         TBEM_HIP(hipStreamSynchronize(s));  // the host vectors above are reused next chunk
     }
@@ -500,11 +636,12 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     std::sort(order.begin(), order.end(), [&](int a, int b) { return csize(a) > csize(b); });
     std::vector<size_t> load(kXcds, 0);
     std::vector<std::vector<std::array<int, 3>>> bins(kXcds);  // (class, off, cnt)
+    const int ucols = fp64 ? kUnitCols64 : kUnitCols;
     for (int c: order) {
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         const int sz = csize(c);
-        for (int j = 0; j < sz; j += kUnitCols) bins[x].push_back({c, hstart[c] + j, std::min(kUnitCols, sz - j)});
-        load[x] += (sz + kUnitCols - 1) / kUnitCols;
+        for (int j = 0; j < sz; j += ucols) bins[x].push_back({c, hstart[c] + j, std::min(ucols, sz - j)});
+        load[x] += (sz + ucols - 1) / ucols;
     }
     size_t rounds = 0;
     for (auto& b: bins) rounds = std::max(rounds, b.size());
@@ -558,7 +695,7 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
 
 void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s,
                      bool transpose) {
-    if (!M.enabled || M.n_units == 0) return;
+    if (!M.enabled || M.fp64 || M.n_units == 0) return;
     const size_t shm = (size_t)M.kt * 4 * kUnitCols * sizeof(float);
     // eight waves when the staged columns allow at most two workgroups per CU (160 KiB of LDS)
     const bool wide = 3 * shm > kLdsPerCu;
@@ -573,13 +710,38 @@ void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up,
     TBEM_LAUNCH_CHECK();
 }
 
+void launch_m2l_mfma64(const M2lMfma& M, const int* pair_src, const double* x, double* Y, hipStream_t s,
+                       bool transpose) {
+    if (!M.enabled || !M.fp64 || M.n_units == 0) return;
+    const size_t shm = staged_bytes(M.blk, true);
+    hipLaunchKernelGGL((k_m2l_mfma64<8, kPre64>), dim3(M.n_units), dim3(512), shm, s, M.blk, M.mt, M.kt, M.ystride,
+                       transpose ? M.ST64.p : M.S64.p, M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p,
+                       transpose ? M.pair_tgt.p : pair_src, x, Y);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_m2l_sum64(const M2lMfma& M, int n_items, const int* items, const int* ptr, const double* Y, double* out,
+                      hipStream_t s) {
+    if (!M.enabled || !M.fp64 || n_items == 0) return;
+    hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(n_items), dim3(256), 0, s, items, ptr, nullptr, Y, M.ystride, M.blk, 0,
+                       out);
+    TBEM_LAUNCH_CHECK();
+}
+
+void launch_m2l_sum64_adj(const M2lMfma& M, const double* Y, double* out, hipStream_t s) {
+    if (!M.enabled || !M.fp64 || M.n_adj == 0) return;
+    hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(M.n_adj), dim3(256), 0, s, M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p,
+                       Y, M.ystride, M.blk, 1, out);
+    TBEM_LAUNCH_CHECK();
+}
+
 void launch_m2l_modal_in(const M2lMfma& M, const CellTiles& targets, const double* lbar, float* w, hipStream_t s) {
-    if (!M.enabled) return;
+    if (!M.enabled || M.fp64) return;
     launch_level_gemm(targets, M.blk, M.Vd.p, true, lbar, nullptr, nullptr, w, false, s);
 }
 
 void launch_m2l_reduce_adj(const M2lMfma& M, const float* Y, double* z, double* e2, hipStream_t s) {
-    if (!M.enabled || M.n_adj == 0) return;
+    if (!M.enabled || M.fp64 || M.n_adj == 0) return;
     const int npar = 256 / (M.ystride / 4);
     hipLaunchKernelGGL(k_m2l_pair_sum, dim3(M.n_adj), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
                        M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, Y, M.ystride, M.blk, z);
@@ -589,7 +751,7 @@ void launch_m2l_reduce_adj(const M2lMfma& M, const float* Y, double* z, double* 
 
 void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const CellTiles& targets,
                        const float* Y, double* z, double* locals, hipStream_t s) {
-    if (!M.enabled || n_items == 0) return;
+    if (!M.enabled || M.fp64 || n_items == 0) return;
     const int npar = 256 / (M.ystride / 4);
     hipLaunchKernelGGL(k_m2l_pair_sum, dim3(n_items), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
                        items, ptr, nullptr, Y, M.ystride, M.blk, z);

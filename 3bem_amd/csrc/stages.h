@@ -73,6 +73,8 @@ struct SurfStage {
     const double* unit;     // unit sphere = surface normals
     int n_e, blk;
     double* out;            // blk per cell of tt: the raw check potentials (= or += with add)
+    int no_cutoff;          // surface sources: every pair is known to lie beyond the r2 < 1e-12
+                            // cutoff, so the kernels leave the test out (same result)
 };
 
 // Tiles of at most 64 cells of one level for the batched per-level products (the

@@ -323,7 +323,7 @@ __device__ inline void stream_surfaces(Tile<double, R, NRM, NB>& tl, int e0, int
 // ---------------------------------------------------------------------- pair loop
 // One source j of the tile on one target (FP64 coordinates tx, ty, tz): the
 // displacement is formed in FP64, then the kernel runs in T.
-template <int KID, bool ADJ, bool FAST = false, class T, class VT, int R, bool NRM, int NB>
+template <int KID, bool ADJ, bool FAST = false, bool CUT = true, class T, class VT, int R, bool NRM, int NB>
 __device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, double tx, double ty, double tz,
                                      const T* tn, T* acc, const KC<T>& kc) {
     const T sn[3] = {NRM ? T(tl.nx[j]) : T(0), NRM ? T(tl.ny[j]) : T(0), NRM ? T(tl.nz[j]) : T(0)};
@@ -332,16 +332,18 @@ __device__ __forceinline__ void pair(const Tile<VT, R, NRM, NB>& tl, int j, doub
     if (!ADJ) {
         const T dx = T(tl.x[j] - tx), dy = T(tl.y[j] - ty), dz = T(tl.z[j] - tz);
         const T r2 = dx * dx + dy * dy + dz * dz;
-        kop<KID, false, T, FAST>(dx, dy, dz, r2, tn, sn, v, acc, kc);
+        kop<KID, false, T, FAST, CUT>(dx, dy, dz, r2, tn, sn, v, acc, kc);
     } else {
         const T dx = T(tx - tl.x[j]), dy = T(ty - tl.y[j]), dz = T(tz - tl.z[j]);
         const T r2 = dx * dx + dy * dy + dz * dz;
-        kop<KID, true, T, FAST>(dx, dy, dz, r2, sn, tn, v, acc, kc);
+        kop<KID, true, T, FAST, CUT>(dx, dy, dz, r2, sn, tn, v, acc, kc);
     }
 }
 
 // ------------------------------------------------------------------ surface targets
-template <int KID, bool SRC_SURF, bool ADJ, bool DUAL>
+// CUT = false (surface sources only): the r2 < 1e-12 cutoff test is left out, when
+// the plan has shown every pair of the stage to be farther apart (SurfStage.no_cutoff).
+template <int KID, bool SRC_SURF, bool ADJ, bool DUAL, bool CUT = true>
 __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<double> kc, double scale, int add) {
     constexpr int NB = kSurfBlock;
     constexpr int R = kernel_dim(KID);
@@ -382,11 +384,12 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
             if (!active) return;
             if constexpr (dual) {
                 for (int j = slice; j < cnt; j += S) {
-                    pair<KID, ADJ>(sm.st, j, p0[0], p0[1], p0[2], tn0, acc0, kc);
-                    pair<KID, ADJ>(sm.st, j, p1[0], p1[1], p1[2], tn1, acc1, kc);
+                    pair<KID, ADJ, false, CUT>(sm.st, j, p0[0], p0[1], p0[2], tn0, acc0, kc);
+                    pair<KID, ADJ, false, CUT>(sm.st, j, p1[0], p1[1], p1[2], tn1, acc1, kc);
                 }
             } else {
-                for (int j = slice; j < cnt; j += S) pair<KID, ADJ>(sm.st, j, p0[0], p0[1], p0[2], tn0, acc0, kc);
+                for (int j = slice; j < cnt; j += S)
+                    pair<KID, ADJ, false, CUT>(sm.st, j, p0[0], p0[1], p0[2], tn0, acc0, kc);
             }
         };
         if (SRC_SURF)
@@ -844,9 +847,13 @@ static void surf_launch(bool src_surf, bool add, bool adj, const SurfStage& a, c
     const double scale = (double)kop_scale<KID, double>(kc);
     const size_t shm = (size_t)kernel_dim(KID) * a.n_e * sizeof(double);
     dim3 g(a.n_items), b(kSurfBlock);
-    const int v = (src_surf ? 4 : 0) + (adj ? 2 : 0) + (a.n_e > kSurfBlock / 2 ? 1 : 0);
+    const int v = (src_surf && a.no_cutoff ? 8 : src_surf ? 4 : 0) + (adj ? 2 : 0) + (a.n_e > kSurfBlock / 2 ? 1 : 0);
     const int ad = add ? 1 : 0;
     switch (v) {
+    case 8: hipLaunchKernelGGL((k_surf_stage<KID, true, false, false, false>), g, b, shm, s, a, kc, scale, ad); break;
+    case 9: hipLaunchKernelGGL((k_surf_stage<KID, true, false, true, false>), g, b, shm, s, a, kc, scale, ad); break;
+    case 10: hipLaunchKernelGGL((k_surf_stage<KID, true, true, false, false>), g, b, shm, s, a, kc, scale, ad); break;
+    case 11: hipLaunchKernelGGL((k_surf_stage<KID, true, true, true, false>), g, b, shm, s, a, kc, scale, ad); break;
     case 0: hipLaunchKernelGGL((k_surf_stage<KID, false, false, false>), g, b, shm, s, a, kc, scale, ad); break;
     case 1: hipLaunchKernelGGL((k_surf_stage<KID, false, false, true>), g, b, shm, s, a, kc, scale, ad); break;
     case 2: hipLaunchKernelGGL((k_surf_stage<KID, false, true, false>), g, b, shm, s, a, kc, scale, ad); break;

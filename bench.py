@@ -46,7 +46,6 @@ PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 # kernel); other workloads report null.
 TRAFFIC_BYTES = {
     ("fault_surface", 1_000_000, "fp64", "leaf"): 644_165_984,
-    ("fault_surface_o100", 2_000_000, "fp64", "m2l"): 2_170_873_359,
     ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 2_370_368_469,
     ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 2_283_267_211,
     ("fault_surface_o100", 2_000_000, "mixed", "m2l"): 7_941_620_238,
@@ -397,9 +396,9 @@ def near_field_run(tbem, torch, steps, warmup, skip_cpu):
 
 def fmm_rooflines(info, n, kid, prec, leaf_ms, m2l_ms, wname=None):
     """Rooflines of the two kernels that dominate a matvec: the fused P2P+M2P+L2P
-    leaf stage (algorithmic flops F_K per point pair, SURVEY's count) and the M2L
-    translation kernel (FP64: F_K per surface-point pair, n_e^2 per cell pair, kernel
-    evaluated on the fly; mixed: 2 blk^2 flops per cell pair on FP32 MFMA)."""
+    leaf stage (algorithmic flops F_K per point pair) and the M2L translation kernel:
+    with translation classes 2 blk^2 flops per cell pair (FP64 or FP32 MFMA), without
+    them F_K per surface-point pair, n_e^2 per cell pair (kernel on the fly)."""
     n_e, R = info["n_e"], (1 if kid in (0, 1, 2, 7) else 3)
     out = {}
     flop = FLOPS_PER_PAIR[kid] * (info["p2p_pairs"] + n * n_e)
@@ -415,12 +414,19 @@ def fmm_rooflines(info, n, kid, prec, leaf_ms, m2l_ms, wname=None):
                         "kernel_avg_ms": round(leaf_ms, 4), "algorithmic_flop": flop,
                         "flop_rule": f"{FLOPS_PER_PAIR[kid]} flop per (target, source) pair: P2P pairs + L2P pairs"})
     if m2l_ms > 0:
-        if prec == "fp64" or info["m2l_classes"] == 0:
+        if info["m2l_classes"] == 0:
             flop = FLOPS_PER_PAIR[kid] * info["m2l"] * n_e * n_e
             ach = flop / (m2l_ms * 1e-3) / 1e12
             out["m2l"] = {"bound": "mfma", "pipe": "FP64 VALU", "kernel": "k_surf_stage (FP64 M2L, kernel on the fly)",
                           "peak": PEAK_FP64_TFLOPS, "flop_rule": f"{FLOPS_PER_PAIR[kid]} flop per surface-point pair, "
                                                                  "n_e^2 pairs per M2L cell pair"}
+        elif prec == "fp64":
+            blk = R * n_e
+            flop = 2.0 * info["m2l"] * blk * blk
+            ach = flop / (m2l_ms * 1e-3) / 1e12
+            out["m2l"] = {"bound": "mfma", "pipe": "FP64 MFMA (v_mfma_f64_16x16x4f64)",
+                          "kernel": "k_m2l_mfma64 (FP64 MFMA M2L, translation classes)", "peak": PEAK_FP64_TFLOPS,
+                          "flop_rule": "2 blk^2 flop per M2L cell pair", "classes": info["m2l_classes"]}
         else:
             blk = R * n_e
             flop = 2.0 * info["m2l"] * blk * blk

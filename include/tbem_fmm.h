@@ -113,9 +113,21 @@ typedef struct {
      * their cells lie on one grid and the M2L pairs fall into translation classes;
      * TBEM_TREES_OWN_BOXES and TBEM_TREES_COMMON_BOX force either layout. */
     int tree_boxes;
+
+    /* FP64 M2L (no reference counterpart; TBEM_MIXED ignores it):
+     * TBEM_M2L_AUTO (0, the default) translates with one FP64 operator per
+     * translation class (level pair and integer centre offset) on the FP64 MFMA units
+     * where the classes exist (the cells of both sides on one grid: one shared tree,
+     * or tree_boxes TBEM_TREES_COMMON_BOX) and their per-pair output fits an eighth
+     * of the device memory, and falls back to TBEM_M2L_SURFACES elsewhere;
+     * TBEM_M2L_SURFACES evaluates the kernel between the two surfaces of every M2L
+     * pair, as cpp/fmm.cpp:362-382 does. Both are FP64 throughout; they differ in
+     * rounding only. */
+    int m2l_operators;
 } tbem_fmm_desc;
 
 enum { TBEM_TREES_AUTO = 0, TBEM_TREES_OWN_BOXES = 1, TBEM_TREES_COMMON_BOX = 2 };
+enum { TBEM_M2L_AUTO = 0, TBEM_M2L_SURFACES = 1 };
 
 /* Plan statistics (tbem_fmm_info). */
 typedef struct {
@@ -124,10 +136,11 @@ typedef struct {
     size_t n_up_levels, n_down_levels;
     size_t p2m, m2m, p2p, m2p, p2l, m2l, l2l, l2p;   /* task counts as FMMTasks (cpp/fmm.h:98-120) */
     double p2p_pairs;        /* sum over P2P tasks of n_obs*n_src */
-    double m2l_classes;      /* distinct translation classes (MIXED) */
+    double m2l_classes;      /* distinct translation classes (MIXED, or FP64 with class operators) */
     double plan_seconds;     /* wall time of tbem_fmm_create */
-    double m2l_output_bytes; /* the MFMA M2L's per-pair output (MIXED): one buffer per plan,
-                                shared by its clones, whose M2L sections take turns on it */
+    double m2l_output_bytes; /* the class M2L's per-pair output (FP32 for MIXED, FP64 for FP64):
+                                one buffer per plan, shared by its clones, whose M2L sections
+                                take turns on it */
     double work_bytes;       /* device buffers of this handle's own (each clone has its own;
                                 the adjoint and gradient add theirs on first use) */
 } tbem_fmm_info_t;

@@ -31,12 +31,12 @@ def case(*a):
     print("CASE", *a, flush=True)
 
 
-def fixture_op(tbem, g, precision="fp64", inject=False):
+def fixture_op(tbem, g, precision="fp64", inject=False, m2l="auto"):
     K = tbem.make_kernel(int(g["kernel"]), 1.0, 0.25)
     data = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
     cfg = tbem.FMMConfig(float(g["mac"]), int(g["order"]), int(g["leaf"]), float(g["d"]), bool(g["small"]))
     kw = dict(up_c2e=g["uc2e"], down_c2e=g["dc2e"]) if inject else {}
-    return tbem.FMMOperator(K, data, cfg, precision, **kw)
+    return tbem.FMMOperator(K, data, cfg, precision, m2l=m2l, **kw)
 
 
 @pytest.mark.parametrize("name", ["cube_clusters_laplace_o75", "fault_surface_elastic_u_o50",
@@ -46,14 +46,16 @@ def test_high_order_fixture(tbem, name):
     y_ref, direct = g["y"], g["direct"]
     e_ref = rel_err(y_ref, direct)
     case("config fixture", name, f"reference vs direct {e_ref:.2e}")
-    y_inj = fixture_op(tbem, g, inject=True).apply(g["x"])
-    e_inj = rel_err(y_inj, y_ref)
+    op_inj = fixture_op(tbem, g, inject=True)
+    assert op_inj.info()["m2l_classes"] > 0  # FP64 M2L on class operators
+    e_inj = rel_err(op_inj.apply(g["x"]), y_ref)
+    e_inj_surf = rel_err(fixture_op(tbem, g, inject=True, m2l="surfaces").apply(g["x"]), y_ref)
     y_own = fixture_op(tbem, g).apply(g["x"])
     y_mix = fixture_op(tbem, g, "mixed").apply(g["x"])
-    print(f"  fp64 (reference C2E) vs reference {e_inj:.2e}; fp64 (own C2E) vs direct "
-          f"{rel_err(y_own, direct):.2e}; mixed vs direct {rel_err(y_mix, direct):.2e}, vs fp64 "
-          f"{rel_err(y_mix, y_own):.2e}")
-    assert e_inj < 1e-8
+    print(f"  fp64 (reference C2E) vs reference: class M2L {e_inj:.2e}, surface M2L {e_inj_surf:.2e}; "
+          f"fp64 (own C2E) vs direct {rel_err(y_own, direct):.2e}; mixed vs direct {rel_err(y_mix, direct):.2e}, "
+          f"vs fp64 {rel_err(y_mix, y_own):.2e}")
+    assert e_inj < 1e-8 and e_inj_surf < 1e-8
     assert rel_err(y_own, direct) < 1.2 * e_ref + 1e-9
     assert rel_err(y_mix, direct) < 1.2 * e_ref + 1e-6
     assert rel_err(y_mix, y_own) < 1e-6
@@ -85,13 +87,15 @@ def test_order_100_against_live_reference(tbem, wl, ref, kid):
     data = tbem.NBodyData(loc, nrm, loc, nrm, w)
     cfg = tbem.FMMConfig(0.3, 100, 64, 0.05, False)
     y_inj = tbem.FMMOperator(K, data, cfg, up_c2e=up, down_c2e=down).apply(g["x"])
+    y_inj_surf = tbem.FMMOperator(K, data, cfg, up_c2e=up, down_c2e=down, m2l="surfaces").apply(g["x"])
     op_mix = tbem.FMMOperator(K, data, cfg, precision="mixed")
     assert op_mix.info()["m2l_classes"] > 0  # the MFMA M2L runs
     y_mix = op_mix.apply(g["x"])
     e_ref = rel_err(y_ref, direct)
-    print(f"  reference vs direct {e_ref:.2e}; fp64 (reference C2E) vs reference {rel_err(y_inj, y_ref):.2e}; "
+    print(f"  reference vs direct {e_ref:.2e}; fp64 (reference C2E) vs reference: class M2L "
+          f"{rel_err(y_inj, y_ref):.2e}, surface M2L {rel_err(y_inj_surf, y_ref):.2e}; "
           f"mixed vs direct {rel_err(y_mix, direct):.2e}")
-    assert rel_err(y_inj, y_ref) < 1e-8
+    assert rel_err(y_inj, y_ref) < 1e-8 and rel_err(y_inj_surf, y_ref) < 1e-8
     assert rel_err(y_mix, direct) < 1.2 * e_ref + 1e-6
 
 
