@@ -178,13 +178,15 @@ class FMMOperator:
     m2l: the FP64 M2L, "auto" (one FP64 operator per translation class on the FP64
     MFMA units where the classes exist, else as "surfaces") or "surfaces" (the kernel
     between the two surfaces of every pair, as the reference); mixed ignores it.
+    m2l_output_budget: bytes the class M2L's per-pair output may take (None: 6 GiB);
+    beyond it the pairs run in slices.
     """
 
     TREE_BOXES = {"auto": 0, "own": 1, "common": 2}
     M2L_OPERATORS = {"auto": 0, "surfaces": 1}
 
     def __init__(self, K, data, config, precision="fp64", up_c2e=None, down_c2e=None, plan=None,
-                 tree_boxes="auto", m2l="auto"):
+                 tree_boxes="auto", m2l="auto", m2l_output_budget=None):
         lib = _lib.require_device()
         self.K = K
         self.data = data
@@ -207,6 +209,7 @@ class FMMOperator:
         desc.precision = PRECISIONS[precision]
         desc.tree_boxes = self.TREE_BOXES[tree_boxes]
         desc.m2l_operators = self.M2L_OPERATORS[m2l]
+        desc.m2l_output_budget = int(m2l_output_budget or 0)
         if self._up is not None:
             desc.up_check_to_equiv, desc.n_up_levels = _dp(self._up), self._up.shape[0]
         if self._down is not None:

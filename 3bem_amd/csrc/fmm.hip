@@ -31,6 +31,10 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
 
 // ----------------------------------------------------------------------- the plan
+// Default bytes the class M2L's per-pair output may take (one buffer per plan); more
+// pairs are processed in slices that fit.
+constexpr size_t kM2lOutputBudget = size_t(6) << 30;
+
 struct FmmPlan {
     int kid = 0, R = 1, precision = TBEM_FP64;
     KConst kc{};
@@ -742,24 +746,21 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->t_m2l = make_cell_tiles(m2l_h.items, OT, s);
     P->m2l_no_cutoff = m2l_clear_of_cutoff(m2l_h, OT, ST, P->surf);
     clk.mark("lists");
-    // FP64: class operators where the classes exist and their per-pair output fits an
-    // eighth of the device memory
-    bool fp64_classes = false;
-    if (P->precision == TBEM_FP64 && d->m2l_operators == TBEM_M2L_AUTO && P->tree_layout != kTreesOwnBoxes &&
-        m2l_fp64_size_ok(P->blk)) {
-        size_t free_b = 0, total_b = 0;
-        TBEM_HIP(hipMemGetInfo(&free_b, &total_b));
-        fp64_classes = m2l_output_bytes((int)m2l_h.cell.size(), P->blk, true) <= total_b / 8;
-    }
+    // FP64: class operators where the classes exist (the builder also requires the
+    // operators to fit an eighth of the device memory)
+    const bool fp64_classes = P->precision == TBEM_FP64 && d->m2l_operators == TBEM_M2L_AUTO &&
+                              P->tree_layout != kTreesOwnBoxes && m2l_fp64_size_ok(P->blk);
     if (P->precision == TBEM_MIXED || fp64_classes)
         build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, P->dot, P->dst, P->tree_layout != kTreesOwnBoxes,
-                       fp64_classes, m2l_h, P->m2l, P->surf, P->uc2e.p, P->dc2e.p, s, up_fac, down_fac);
+                       fp64_classes, d->m2l_output_budget ? d->m2l_output_budget : kM2lOutputBudget, m2l_h, P->m2l,
+                       P->surf, P->uc2e.p, P->dc2e.p, s, up_fac, down_fac);
     clk.mark("m2l_mfma");
     TBEM_HIP(hipEventCreateWithFlags(&P->y_done, hipEventDisableTiming));
     TBEM_HIP(hipEventRecord(P->y_done, s));
     if (P->m2l_mfma.enabled) {
-        if (P->m2l_mfma.fp64) P->m2l_y64.alloc((size_t)P->m2l_mfma.n_pairs * P->m2l_mfma.ystride);
-        else P->m2l_y.alloc((size_t)P->m2l_mfma.n_pairs * P->m2l_mfma.ystride);
+        const size_t ny = (size_t)P->m2l_mfma.y_pairs * P->m2l_mfma.ystride;
+        if (P->m2l_mfma.fp64) P->m2l_y64.alloc(ny);
+        else P->m2l_y.alloc(ny);
     }
     P->d_flag.alloc(1);
     const std::vector<int> p2l_items = cells_with(tl.p2l, 0, OT.n_cells);
@@ -872,16 +873,13 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
     if (mfma) {
         // Vd M_c Bu check = DC2E K_c UC2E check = the FP64 stage's DC2E K_c mult
         YLease y(P, s);
-        launch_m2l_mfma(P.m2l_mfma, P.m2l.cell.p, W.chk_up.p, P.m2l_y.p, s);
-        TBEM_HIP(hipEventRecord(W.ev[9], s));
-        launch_m2l_reduce(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.t_m2l, P.m2l_y.p, W.fld.p,
-                          W.locals.p, s);
+        m2l_forward32(P.m2l_mfma, P.m2l.items.p, P.m2l.ptr.p, P.m2l.cell.p, P.t_m2l, W.chk_up.p, P.m2l_y.p, W.fld.p,
+                      W.locals.p, s, W.ev[9]);
     } else if (P.m2l_mfma.enabled && P.m2l_mfma.fp64) {
         // FP64 class operators: check potential = sum over pairs of K_c mult, then DC2E
         YLease y(P, s);
-        launch_m2l_mfma64(P.m2l_mfma, P.m2l.cell.p, W.mult.p, P.m2l_y64.p, s);
-        TBEM_HIP(hipEventRecord(W.ev[9], s));
-        launch_m2l_sum64(P.m2l_mfma, P.m2l.n_items, P.m2l.items.p, P.m2l.ptr.p, P.m2l_y64.p, W.fld.p, s);
+        m2l_forward64(P.m2l_mfma, P.m2l.items.p, P.m2l.ptr.p, P.m2l.cell.p, W.mult.p, P.m2l_y64.p, W.fld.p, s,
+                      W.ev[9]);
         down_c2e(P.t_m2l);
     } else {
         SurfStage m = surf_base(P);
@@ -1040,16 +1038,13 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
 
     // M2L^T into mbar (FP64), or, on MFMA, e2 = Bu^T M_c^T Vd^T lbar, added to ebar below
     if (mfma) {
-        launch_m2l_modal_in(P.m2l_mfma, P.t_m2l, W.lbar.p, W.m2l_w.p, s);
         launch_fill(W.e2.p, ncs, 0.0, s);
         YLease y(P, s);
-        launch_m2l_mfma(P.m2l_mfma, nullptr, W.m2l_w.p, P.m2l_y.p, s, true);
-        launch_m2l_reduce_adj(P.m2l_mfma, P.m2l_y.p, W.fld.p, W.e2.p, s);
+        m2l_adjoint32(P.m2l_mfma, P.t_m2l, W.lbar.p, W.m2l_w.p, P.m2l_y.p, W.fld.p, W.e2.p, s);
     } else if (P.m2l_mfma.enabled && P.m2l_mfma.fp64) {
         // FP64 class operators: mbar[s] += sum over the pairs of s of K_c^T cbar[t]
         YLease y(P, s);
-        launch_m2l_mfma64(P.m2l_mfma, nullptr, W.cbar.p, P.m2l_y64.p, s, true);
-        launch_m2l_sum64_adj(P.m2l_mfma, P.m2l_y64.p, W.mbar.p, s);
+        m2l_adjoint64(P.m2l_mfma, W.cbar.p, P.m2l_y64.p, W.mbar.p, s);
     } else {
         SurfStage m = surf_base(P);
         set_list(m, A.m2l);

@@ -31,6 +31,12 @@ struct M2lMfma {
     DevBuf<double> S64, ST64;  // FP64 path: K_c and K_c^T, A-operand order
     DevBuf<double> Bu, Vd;  // per source / observation level, blk x blk row-major
     DevBuf<int> unit_class, unit_off, unit_cnt, colpair;
+    // The per-pair output Y holds one slice of the pairs at a time: slice q is the
+    // M2L items slice_items[q] .. slice_items[q+1] (targets, whole), i.e. the pairs
+    // slice_pairs[q] .. slice_pairs[q+1], computed by units slice_units[q] ..
+    // slice_units[q+1]; Y holds y_pairs (the largest slice) * ystride values.
+    int n_slices = 1, y_pairs = 0;
+    std::vector<int> slice_items, slice_pairs, slice_units;
     DevBuf<int> pair_tgt;   // target cell of every pair
     int n_adj = 0;          // transpose: source cells with pairs, and their pairs (CSR)
     DevBuf<int> adj_items, adj_ptr, adj_pairs;
@@ -43,51 +49,31 @@ struct M2lMfma {
 bool m2l_mfma_size_ok(int blk);
 // The same for the FP64 operators (32 staged FP64 columns per unit).
 bool m2l_fp64_size_ok(int blk);
-// Bytes of the per-pair output Y for n_pairs pairs (16-row padded blk per pair).
-size_t m2l_output_bytes(int n_pairs, int blk, bool fp64);
-
-// m2l: the forward M2L list (items = target cells, cell[p] = source of pair p).
-// Leaves M.enabled false (the caller keeps the FP64 M2L) when the trees' cells are
-// not on one grid (same_grid false: distinct trees on their own root boxes), an
-// offset is not integral, or m2l_mfma_size_ok(blk) is false.
-// up_fac, down_fac: optionally {W, V}, the SVD factors of UC2E and DC2E per level
-// (from the pseudo-inverse's own SVD, gpu_pinv_batched); without them the C2E
-// matrices are factored again.
-// dot, dst: the trees on the device; dm2l: the device copy of m2l.
 // fp64: build the FP64 path's operators (see above) instead of the mixed path's.
+// y_budget: bytes the per-pair output may take; the pairs are sliced to fit.
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, const HostList& m2l,
+                    const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, size_t y_budget,
+                    const HostList& m2l,
                     const DevList& dm2l, const Surfaces& surf, const double* d_uc2e, const double* d_dc2e,
                     hipStream_t s, const DevBuf<double>* up_fac = nullptr, const DevBuf<double>* down_fac = nullptr);
 
-// Y[p] (ystride floats per pair) = M_{class(p)} * chk_up[src(p)] for every pair,
-// chk_up = Bu * up-check potential per source cell, rounded to FP32.
-// transpose: Y[p] = M_{class(p)}^T * chk_up[tgt(p)] (pair_src is then unused).
-void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s,
-                     bool transpose = false);
-
-// FP64 path: Y[p] (ystride doubles per pair) = K_{class(p)} * x[src(p)] (x: blk
-// doubles per source cell, the up-equivalent densities); transpose:
-// Y[p] = K_{class(p)}^T * x[tgt(p)] (x per target cell).
-void launch_m2l_mfma64(const M2lMfma& M, const int* pair_src, const double* x, double* Y, hipStream_t s,
-                       bool transpose = false);
-// FP64 path: out[items[k]] = sum of Y over the pairs ptr[k] .. ptr[k+1] (pair order).
-void launch_m2l_sum64(const M2lMfma& M, int n_items, const int* items, const int* ptr, const double* Y, double* out,
-                      hipStream_t s);
-// FP64 path, transpose: out[s] += sum of Y over the pairs of source cell s.
-void launch_m2l_sum64_adj(const M2lMfma& M, const double* Y, double* out, hipStream_t s);
-
-// Transpose, input: w[t] = float(Vd[level t]^T lbar[t]) for every M2L target t
-// (targets: the M2L targets as GEMM tiles).
-void launch_m2l_modal_in(const M2lMfma& M, const CellTiles& targets, const double* lbar, float* w, hipStream_t s);
-
-// Transpose, output: e2[s] = Bu[level s]^T * (sum over the pairs p of source s of
-// Y[p]), FP64, for the sources with pairs; z (blk doubles per source cell) is scratch.
-void launch_m2l_reduce_adj(const M2lMfma& M, const float* Y, double* z, double* e2, hipStream_t s);
-
-// locals[t] += Vd[level t] * (sum over the pairs p of target t of Y[p]), in FP64
-// (targets: items as GEMM tiles; z, blk doubles per target cell, is scratch).
-void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const CellTiles& targets,
-                       const float* Y, double* z, double* locals, hipStream_t s);
+// Mixed path, forward: slice by slice, Y[p] = M_{class(p)} * chk_up[src(p)] (FP32
+// MFMA; chk_up = Bu * up-check potential per source cell, FP32) and z[t] = sum of the
+// target's Y in pair order (FP64); then locals[t] += Vd[level t] z[t] for the targets
+// (items as GEMM tiles). kernel_end, if given, is recorded after the last slice's
+// MFMA kernel. items/ptr/pair_src: the forward M2L list on the device.
+void m2l_forward32(const M2lMfma& M, const int* items, const int* ptr, const int* pair_src, const CellTiles& targets,
+                   const float* chk_up, float* Y, double* z, double* locals, hipStream_t s, hipEvent_t kernel_end);
+// FP64 path, forward: slice by slice, Y[p] = K_{class(p)} * mult[src(p)] (FP64 MFMA)
+// and fld[t] = sum of the target's Y in pair order.
+void m2l_forward64(const M2lMfma& M, const int* items, const int* ptr, const int* pair_src, const double* mult,
+                   double* Y, double* fld, hipStream_t s, hipEvent_t kernel_end);
+// Mixed path, transpose: w[t] = float(Vd[level t]^T lbar[t]) for the M2L targets,
+// then slice by slice Y[p] = M_c^T w[tgt(p)] and z[s] (+)= the source's Y in the
+// slice, and e2[s] = Bu[level s]^T z[s] for the sources with pairs (z: scratch).
+void m2l_adjoint32(const M2lMfma& M, const CellTiles& targets, const double* lbar, float* w, float* Y, double* z,
+                   double* e2, hipStream_t s);
+// FP64 path, transpose: mbar[s] += sum over the pairs of source s of K_c^T cbar[tgt(p)].
+void m2l_adjoint64(const M2lMfma& M, const double* cbar, double* Y, double* mbar, hipStream_t s);
 
 }  // namespace tbem

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 
 #include "c2e.h"
 #include "m2l_mfma.h"
@@ -40,6 +41,7 @@ namespace {
 constexpr int kUnitCols = 64;   // pairs per work unit (= LDS row length of the staged columns)
 constexpr int kUnitCols64 = 32; // the same for the FP64 operators
 constexpr int kPre64 = 8;       // k-steps of A loaded ahead by the FP64 kernel
+constexpr int kWaves64 = 8;     // waves per workgroup of the FP64 kernel
 constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 KiB of LDS
 // Below this operator size the FP64 M2L stage is the cheaper one: measured on the
 // 1M-point elastic workload at order 6 (blk 18), the MFMA path with its per-pair
@@ -173,7 +175,7 @@ template <int NW, int kPre>
 __global__ __launch_bounds__(NW * 64) void k_m2l_mfma(int n, int mt, int kt, int ystride, const float* __restrict__ S,
                                                   const int* __restrict__ ucls, const int* __restrict__ uoff,
                                                   const int* __restrict__ ucnt, const int* __restrict__ colpair,
-                                                  const int* __restrict__ pair_src,
+                                                  const int* __restrict__ pair_src, int ybase,
                                                   const float* __restrict__ chk, float* __restrict__ Y) {
     extern __shared__ float Bs[];  // [kt*4][64], swizzled
     const int u = blockIdx.x;
@@ -208,31 +210,55 @@ __global__ __launch_bounds__(NW * 64) void k_m2l_mfma(int n, int mt, int kt, int
         // D[4*lq + r][lr] of column tile ct -> Y[pair][rt*16 + 4*lq + r]
         for (int ct = 0; ct < nct; ct++) {
             const int nn = ct * 16 + lr;
-            if (nn < cnt) *(f32x4*)(Y + (size_t)colpair[off + nn] * ystride + rt * 16 + 4 * lq) = acc[ct];
+            if (nn < cnt) *(f32x4*)(Y + (size_t)(colpair[off + nn] - ybase) * ystride + rt * 16 + 4 * lq) = acc[ct];
         }
     }
 }
 
-// z[t] = sum_p Y[p] over the pairs p of target t, in FP64 (blk entries per cell);
-// the Vd product follows as a batched per-level GEMM. One 256-thread workgroup per
-// target. The target's pairs are contiguous in Y, so the block is read as
-// consecutive float4s: thread i takes float4 column i % q (q = ystride / 4) of
-// every npar-th pair starting at i / q, and the npar partial sums are combined
-// through LDS. With pairs given, the pairs of item k are pairs[ptr[k] .. ptr[k+1]]
-// (the transpose's source-side lists) instead of the range ptr[k] .. ptr[k+1].
+// The pairs of item k whose outputs are in Y (those of one slice of the pairs, Y[0]
+// holding pair ybase): the range ptr[k] .. ptr[k+1] (forward: the slice's items), or,
+// with pairs given (the transpose's source-side lists, each ascending), the entries
+// of pairs[ptr[k] .. ptr[k+1]] within [pb, pe).
+__device__ __forceinline__ void slice_pairs(const int* __restrict__ ptr, const int* __restrict__ pairs, int k, int pb,
+                                            int pe, int& j0, int& j1) {
+    j0 = uni(ptr[k]);
+    j1 = uni(ptr[k + 1]);
+    if (!pairs) return;
+    auto lower = [&](int lo, int hi, int v) {
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (pairs[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const int a = lower(j0, j1, pb);
+    j1 = uni(lower(a, j1, pe));
+    j0 = uni(a);
+}
+
+// z[t] (= or += with add) sum_p Y[p] over the pairs p of target t in this slice, in
+// FP64 (blk entries per cell); the Vd product follows as a batched per-level GEMM.
+// One 256-thread workgroup per target. The target's pairs are contiguous in Y, so the
+// block is read as consecutive float4s: thread i takes float4 column i % q (q =
+// ystride / 4) of every npar-th pair starting at i / q, and the npar partial sums are
+// combined through LDS.
 __global__ __launch_bounds__(256) void k_m2l_pair_sum(const int* __restrict__ items, const int* __restrict__ ptr,
                                                       const int* __restrict__ pairs, const float* __restrict__ Y,
-                                                      int ystride, int blk, double* __restrict__ z) {
+                                                      int ystride, int blk, int ybase, int pb, int pe, int add,
+                                                      double* __restrict__ z) {
     extern __shared__ double part[];  // npar * ystride partial sums
     const int k = blockIdx.x;
-    const int t = uni(items[k]), p0 = uni(ptr[k]), p1 = uni(ptr[k + 1]);
+    const int t = uni(items[k]);
+    int p0, p1;
+    slice_pairs(ptr, pairs, k, pb, pe, p0, p1);
     const int q = ystride / 4, npar = 256 / q;
     const int c = threadIdx.x % q, ph = threadIdx.x / q;
     if (ph < npar) {
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         const f32x4* y4 = (const f32x4*)Y;
         for (int j = p0 + ph; j < p1; j += npar) {
-            const size_t p = pairs ? (size_t)pairs[j] : (size_t)j;
+            const size_t p = (size_t)((pairs ? pairs[j] : j) - ybase);
             const f32x4 v = y4[p * q + c];
             s0 += (double)v[0];
             s1 += (double)v[1];
@@ -246,7 +272,8 @@ __global__ __launch_bounds__(256) void k_m2l_pair_sum(const int* __restrict__ it
     for (int row = threadIdx.x; row < blk; row += blockDim.x) {
         double s = 0;
         for (int h = 0; h < npar; h++) s += part[(size_t)h * ystride + row];
-        z[(size_t)t * blk + row] = s;
+        double* o = z + (size_t)t * blk + row;
+        *o = add ? *o + s : s;
     }
 }
 
@@ -259,7 +286,7 @@ __global__ __launch_bounds__(256) void k_m2l_pair_sum(const int* __restrict__ it
 // doubles, 77 KB at blk 297) leave room for two workgroups per CU.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// One row tile (16 rows of K_c) times NCT (<= 2) column tiles; A from global memory
+// One row tile (16 rows of K_c) times NCT column tiles; A from global memory
 // (L2-resident) loaded kPre k-steps ahead, continuing into the wave's next row tile
 // (An) as m2l_row_tile does.
 template <int NCT, int kPre>
@@ -276,8 +303,9 @@ __device__ __forceinline__ void m2l_row_tile64(const double* __restrict__ A, con
         for (int i = 0; i < kPre; i++) {
             if (k0 + i >= kt) break;
             const double* bk = b + (k0 + i) * 4 * kUnitCols64;
-            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[i], bk[0], acc[0], 0, 0, 0);
-            if (NCT > 1) acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[i], bk[16], acc[1], 0, 0, 0);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++)
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur[i], bk[16 * ct], acc[ct], 0, 0, 0);
         }
         for (int i = 0; i < kPre; i++) cur[i] = nxt[i];
     }
@@ -293,52 +321,60 @@ __global__ __launch_bounds__(NW * 64) void k_m2l_mfma64(int n, int mt, int kt, i
                                                      const double* __restrict__ S, const int* __restrict__ ucls,
                                                      const int* __restrict__ uoff, const int* __restrict__ ucnt,
                                                      const int* __restrict__ colpair, const int* __restrict__ pair_src,
-                                                     const double* __restrict__ x, double* __restrict__ Y) {
+                                                     int ybase, const double* __restrict__ x, double* __restrict__ Y) {
     extern __shared__ double Bd[];  // [kt*4][kUnitCols64]
     const int u = blockIdx.x;
     const int cls = uni(ucls[u]), off = uni(uoff[u]), cnt = uni(ucnt[u]);
     if (cnt == 0) return;  // padding unit: uniform exit before any barrier
     const int lane = threadIdx.x & 63, wave = uni(threadIdx.x >> 6);
     const int kpad = kt * 4;
-    // stage: lane % 32 = column (pair), lane / 32 and the waves stride over k
-    const int c = lane & (kUnitCols64 - 1), h = lane >> 5;
+    // stage: lane % kUnitCols64 = column (pair); the rest of the lane index and the
+    // waves stride over k
+    constexpr int RPW = 64 / kUnitCols64;  // k rows per wave and pass
+    const int c = lane % kUnitCols64, h = lane / kUnitCols64;
     const double* col = c < cnt ? x + (size_t)pair_src[colpair[off + c]] * n : nullptr;
-    for (int kk = 2 * wave + h; kk < kpad; kk += 2 * NW) Bd[kk * kUnitCols64 + c] = (col && kk < n) ? col[kk] : 0.0;
+    for (int kk = RPW * wave + h; kk < kpad; kk += RPW * NW) Bd[kk * kUnitCols64 + c] = (col && kk < n) ? col[kk] : 0.0;
     __syncthreads();
     // B operand of column tile ct at k-step ks: row ks*4 + lq, column ct*16 + lr
     const int lr = lane % 16, lq = lane / 16;
     const double* b = Bd + lq * kUnitCols64 + lr;
-    const int nct = (cnt + 15) / 16;  // 1 or 2 (uniform)
+    constexpr int MAXCT = kUnitCols64 / 16;
+    const int nct = (cnt + 15) / 16;  // 1 .. MAXCT (uniform)
     const double* Acls = S + (size_t)cls * mt * kt * 64 + lane;
     double cur[kPre];
     for (int i = 0; i < kPre; i++) cur[i] = (wave < mt && i < kt) ? Acls[((size_t)wave * kt + i) * 64] : 0.0;
     for (int rt = wave; rt < mt; rt += NW) {
-        f64x4 acc[2] = {};
+        f64x4 acc[MAXCT] = {};
         const double* A = Acls + (size_t)rt * kt * 64;
         const double* An = rt + NW < mt ? A + (size_t)NW * kt * 64 : nullptr;
         if (nct == 1) m2l_row_tile64<1, kPre>(A, An, b, kt, acc, cur);
-        else m2l_row_tile64<2, kPre>(A, An, b, kt, acc, cur);
+        else if (MAXCT == 2 || nct == 2) m2l_row_tile64<(MAXCT < 2 ? MAXCT : 2), kPre>(A, An, b, kt, acc, cur);
+        else if (nct == 3) m2l_row_tile64<(MAXCT < 3 ? MAXCT : 3), kPre>(A, An, b, kt, acc, cur);
+        else m2l_row_tile64<MAXCT, kPre>(A, An, b, kt, acc, cur);
         for (int ct = 0; ct < nct; ct++) {
             const int nn = ct * 16 + lr;
             if (nn < cnt) {
-                double* y = Y + (size_t)colpair[off + nn] * ystride + rt * 16 + lq;
+                double* y = Y + (size_t)(colpair[off + nn] - ybase) * ystride + rt * 16 + lq;
                 for (int r = 0; r < 4; r++) y[4 * r] = acc[ct][r];
             }
         }
     }
 }
 
-// out[t] (= or +=) sum over the pairs of item k of Y[p][0 .. blk), in pair order, FP64.
-// The pairs of item k are ptr[k] .. ptr[k+1], or pairs[ptr[k] .. ptr[k+1]] when given.
+// out[t] (= or += with add) sum over the pairs of item k in this slice (slice_pairs)
+// of Y[p][0 .. blk), in pair order, FP64.
 __global__ __launch_bounds__(256) void k_m2l_pair_sum64(const int* __restrict__ items, const int* __restrict__ ptr,
                                                         const int* __restrict__ pairs, const double* __restrict__ Y,
-                                                        int ystride, int blk, int add, double* __restrict__ out) {
+                                                        int ystride, int blk, int ybase, int pb, int pe, int add,
+                                                        double* __restrict__ out) {
     const int k = blockIdx.x;
-    const int t = uni(items[k]), p0 = uni(ptr[k]), p1 = uni(ptr[k + 1]);
+    const int t = uni(items[k]);
+    int p0, p1;
+    slice_pairs(ptr, pairs, k, pb, pe, p0, p1);
     for (int row = threadIdx.x; row < blk; row += blockDim.x) {
         double s = 0;
         for (int j = p0; j < p1; j++) {
-            const size_t p = pairs ? (size_t)pairs[j] : (size_t)j;
+            const size_t p = (size_t)((pairs ? pairs[j] : j) - ybase);
             s += Y[p * ystride + row];
         }
         double* o = out + (size_t)t * blk + row;
@@ -440,6 +476,25 @@ __global__ void k_class_reps(int nc, const int* __restrict__ cstart, const int* 
     rep_s[c] = src[p];
 }
 
+// split[c * (S+1) + q] = first position in class c's run of sorted pairs
+// (colpair[cstart[c] .. cstart[c+1]), ascending) holding a pair >= slice_pairs[q]
+__global__ void k_class_splits(int nc, int S, const int* __restrict__ cstart, const int* __restrict__ colpair,
+                               const int* __restrict__ slice_pairs, int* __restrict__ split) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = cstart[c], b = cstart[c + 1];
+    for (int q = 0; q <= S; q++) {
+        const int v = slice_pairs[q];
+        int lo = a, hi = b;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (colpair[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        split[(size_t)c * (S + 1) + q] = lo;
+    }
+}
+
 // key of a pair for the transpose's grouping: its source cell
 __global__ void k_source_keys(int n, const int* __restrict__ src, unsigned int* __restrict__ key,
                               int* __restrict__ idx) {
@@ -473,14 +528,10 @@ static bool size_ok(int n, bool fp64) {
 bool m2l_mfma_size_ok(int n) { return size_ok(n, false); }
 bool m2l_fp64_size_ok(int n) { return size_ok(n, true); }
 
-size_t m2l_output_bytes(int n_pairs, int blk, bool fp64) {
-    return (size_t)n_pairs * (size_t)((blk + 15) / 16 * 16) * (fp64 ? sizeof(double) : sizeof(float));
-}
-
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
-                    const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, const HostList& m2l,
-                    const DevList& dm2l, const Surfaces& surf, const double* d_uc2e, const double* d_dc2e,
-                    hipStream_t s, const DevBuf<double>* up_fac, const DevBuf<double>* down_fac) {
+                    const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, size_t y_budget,
+                    const HostList& m2l, const DevList& dm2l, const Surfaces& surf, const double* d_uc2e,
+                    const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac, const DevBuf<double>* down_fac) {
     auto t0 = std::chrono::steady_clock::now();
     M = M2lMfma{};
     const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
@@ -490,7 +541,7 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     const size_t shm = staged_bytes(n, fp64);
     if (shm > 65536) {
         if (fp64) {
-            TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma64<8, kPre64>,
+            TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma64<kWaves64, kPre64>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         } else {
             TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4, kPreNarrow>,
@@ -545,6 +596,12 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.kt = kt;
     M.ystride = mt * 16;
     M.n_pairs = n_pairs;
+    if (fp64) {
+        // the FP64 operators and their transposes must fit an eighth of the device
+        size_t free_b = 0, total_b = 0;
+        TBEM_HIP(hipMemGetInfo(&free_b, &total_b));
+        if ((size_t)nc * mt * kt * 64 * 2 * sizeof(double) > total_b / 8) return;
+    }
 
     // modal factors of UC2E (source tree levels) and DC2E (observation tree levels)
     const size_t nn = (size_t)n * n, per = (size_t)mt * kt * 64;
@@ -627,33 +684,89 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
         TBEM_HIP(hipStreamSynchronize(s));  // the host vectors above are reused next chunk
     }
 
-    // work units: classes binned to XCDs by pair count, units interleaved so that
-    // workgroup u lands on XCD u % 8 with the units of its bin; a unit's pairs are
-    // colpair[off .. off + cnt) (the class's pairs, in pair order, in the sorted array)
-    std::vector<int> order(nc);
-    for (int c = 0; c < nc; c++) order[c] = c;
-    auto csize = [&](int c) { return hstart[c + 1] - hstart[c]; };
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return csize(a) > csize(b); });
-    std::vector<size_t> load(kXcds, 0);
-    std::vector<std::vector<std::array<int, 3>>> bins(kXcds);  // (class, off, cnt)
-    const int ucols = fp64 ? kUnitCols64 : kUnitCols;
-    for (int c: order) {
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        const int sz = csize(c);
-        for (int j = 0; j < sz; j += ucols) bins[x].push_back({c, hstart[c] + j, std::min(ucols, sz - j)});
-        load[x] += (sz + ucols - 1) / ucols;
-    }
-    size_t rounds = 0;
-    for (auto& b: bins) rounds = std::max(rounds, b.size());
-    std::vector<int> ucls(rounds * kXcds, 0), uoff(rounds * kXcds, 0), ucnt(rounds * kXcds, 0);
-    for (size_t r = 0; r < rounds; r++)
-        for (int x = 0; x < kXcds; x++)
-            if (r < bins[x].size()) {
-                ucls[r * kXcds + x] = bins[x][r][0];
-                uoff[r * kXcds + x] = bins[x][r][1];
-                ucnt[r * kXcds + x] = bins[x][r][2];
+    // slices of the pairs (runs of whole target items) whose per-pair outputs fit
+    // y_budget bytes; one slice when all fit
+    {
+        const size_t ybytes = (size_t)M.ystride * (fp64 ? sizeof(double) : sizeof(float));
+        const int n_items = (int)m2l.items.size();
+        size_t max_item = 0;
+        for (int k = 0; k < n_items; k++) max_item = std::max<size_t>(max_item, m2l.ptr[k + 1] - m2l.ptr[k]);
+        const size_t cap = std::max<size_t>(max_item, y_budget / ybytes);
+        M.slice_items.assign(1, 0);
+        M.slice_pairs.assign(1, 0);
+        size_t acc = 0;
+        for (int k = 0; k < n_items; k++) {
+            const size_t c = m2l.ptr[k + 1] - m2l.ptr[k];
+            if (acc + c > cap) {
+                M.slice_items.push_back(k);
+                M.slice_pairs.push_back(m2l.ptr[k]);
+                acc = 0;
             }
-    M.n_units = (int)(rounds * kXcds);
+            acc += c;
+        }
+        M.slice_items.push_back(n_items);
+        M.slice_pairs.push_back(n_pairs);
+        M.n_slices = (int)M.slice_items.size() - 1;
+        M.y_pairs = 0;
+        for (int q = 0; q < M.n_slices; q++) M.y_pairs = std::max(M.y_pairs, M.slice_pairs[q + 1] - M.slice_pairs[q]);
+    }
+    // split[c * (S+1) + q]: where slice q's pairs start in class c's run of the sorted
+    // pairs (each run ascends in pair order)
+    const int S = M.n_slices;
+    std::vector<int> split((size_t)nc * (S + 1));
+    if (S == 1) {
+        for (int c = 0; c < nc; c++) {
+            split[2 * c] = hstart[c];
+            split[2 * c + 1] = hstart[c + 1];
+        }
+    } else {
+        DevBuf<int> d_sp, d_split((size_t)nc * (S + 1));
+        d_sp.upload(M.slice_pairs, s);
+        hipLaunchKernelGGL(k_class_splits, dim3(div_up(nc, 256)), dim3(256), 0, s, nc, S, cstart.p, colpair.p, d_sp.p,
+                           d_split.p);
+        TBEM_LAUNCH_CHECK();
+        d_split.download(split.data(), split.size(), s);
+        TBEM_HIP(hipStreamSynchronize(s));
+    }
+
+    // work units, slice by slice: classes binned to XCDs by pair count, units
+    // interleaved so that workgroup u lands on XCD u % 8 with the units of its bin; a
+    // unit's pairs are colpair[off .. off + cnt) (pairs of one class and one slice, in
+    // pair order, in the sorted array)
+    const int ucols = fp64 ? kUnitCols64 : kUnitCols;
+    std::vector<int> ucls, uoff, ucnt;
+    M.slice_units.assign(1, 0);
+    std::vector<int> order(nc);
+    for (int q = 0; q < S; q++) {
+        auto lo = [&](int c) { return split[(size_t)c * (S + 1) + q]; };
+        auto csize = [&](int c) { return split[(size_t)c * (S + 1) + q + 1] - lo(c); };
+        for (int c = 0; c < nc; c++) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return csize(a) > csize(b); });
+        std::vector<size_t> load(kXcds, 0);
+        std::vector<std::vector<std::array<int, 3>>> bins(kXcds);  // (class, off, cnt)
+        for (int c: order) {
+            const int sz = csize(c);
+            if (sz == 0) break;  // sorted by size: the rest are empty too
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            for (int j = 0; j < sz; j += ucols) bins[x].push_back({c, lo(c) + j, std::min(ucols, sz - j)});
+            load[x] += (sz + ucols - 1) / ucols;
+        }
+        size_t rounds = 0;
+        for (auto& b: bins) rounds = std::max(rounds, b.size());
+        const size_t u0 = ucls.size();
+        ucls.resize(u0 + rounds * kXcds, 0);
+        uoff.resize(u0 + rounds * kXcds, 0);
+        ucnt.resize(u0 + rounds * kXcds, 0);
+        for (size_t r = 0; r < rounds; r++)
+            for (int x = 0; x < kXcds; x++)
+                if (r < bins[x].size()) {
+                    ucls[u0 + r * kXcds + x] = bins[x][r][0];
+                    uoff[u0 + r * kXcds + x] = bins[x][r][1];
+                    ucnt[u0 + r * kXcds + x] = bins[x][r][2];
+                }
+        M.slice_units.push_back((int)ucls.size());
+    }
+    M.n_units = (int)ucls.size();
     M.unit_class.upload(ucls, s);
     M.unit_off.upload(uoff, s);
     M.unit_cnt.upload(ucnt, s);
@@ -693,70 +806,121 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-void launch_m2l_mfma(const M2lMfma& M, const int* pair_src, const float* chk_up, float* Y, hipStream_t s,
-                     bool transpose) {
-    if (!M.enabled || M.fp64 || M.n_units == 0) return;
-    const size_t shm = (size_t)M.kt * 4 * kUnitCols * sizeof(float);
+// The slice's units: grid and offset unit arrays
+struct SliceUnits {
+    int n;
+    const int *ucls, *uoff, *ucnt;
+};
+static SliceUnits slice_units(const M2lMfma& M, int q) {
+    const int u0 = M.slice_units[q];
+    return {M.slice_units[q + 1] - u0, M.unit_class.p + u0, M.unit_off.p + u0, M.unit_cnt.p + u0};
+}
+
+// Y (slice q) = M_c * chk_up[src(p)] (transpose: M_c^T * chk_up[tgt(p)]), FP32 MFMA
+static void mfma32_slice(const M2lMfma& M, int q, const int* pair_src, const float* chk_up, float* Y,
+                         bool transpose, hipStream_t s) {
+    const SliceUnits U = slice_units(M, q);
+    if (U.n == 0) return;
+    const size_t shm = staged_bytes(M.blk, false);
     // eight waves when the staged columns allow at most two workgroups per CU (160 KiB of LDS)
     const bool wide = 3 * shm > kLdsPerCu;
     const float* S = transpose ? M.ST.p : M.S.p;
     const int* src = transpose ? M.pair_tgt.p : pair_src;
+    const int yb = M.slice_pairs[q];
     if (wide)
-        hipLaunchKernelGGL((k_m2l_mfma<8, kPreWide>), dim3(M.n_units), dim3(512), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
-                           M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, src, chk_up, Y);
+        hipLaunchKernelGGL((k_m2l_mfma<8, kPreWide>), dim3(U.n), dim3(512), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
+                           U.ucls, U.uoff, U.ucnt, M.colpair.p, src, yb, chk_up, Y);
     else
-        hipLaunchKernelGGL((k_m2l_mfma<4, kPreNarrow>), dim3(M.n_units), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride, S,
-                           M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p, src, chk_up, Y);
+        hipLaunchKernelGGL((k_m2l_mfma<4, kPreNarrow>), dim3(U.n), dim3(256), shm, s, M.blk, M.mt, M.kt, M.ystride,
+                           S, U.ucls, U.uoff, U.ucnt, M.colpair.p, src, yb, chk_up, Y);
     TBEM_LAUNCH_CHECK();
 }
 
-void launch_m2l_mfma64(const M2lMfma& M, const int* pair_src, const double* x, double* Y, hipStream_t s,
-                       bool transpose) {
-    if (!M.enabled || !M.fp64 || M.n_units == 0) return;
+// Y (slice q) = K_c * x[src(p)] (transpose: K_c^T * x[tgt(p)]), FP64 MFMA
+static void mfma64_slice(const M2lMfma& M, int q, const int* pair_src, const double* x, double* Y, bool transpose,
+                         hipStream_t s) {
+    const SliceUnits U = slice_units(M, q);
+    if (U.n == 0) return;
     const size_t shm = staged_bytes(M.blk, true);
-    hipLaunchKernelGGL((k_m2l_mfma64<8, kPre64>), dim3(M.n_units), dim3(512), shm, s, M.blk, M.mt, M.kt, M.ystride,
-                       transpose ? M.ST64.p : M.S64.p, M.unit_class.p, M.unit_off.p, M.unit_cnt.p, M.colpair.p,
-                       transpose ? M.pair_tgt.p : pair_src, x, Y);
+    hipLaunchKernelGGL((k_m2l_mfma64<kWaves64, kPre64>), dim3(U.n), dim3(kWaves64 * 64), shm, s, M.blk, M.mt, M.kt,
+                       M.ystride, transpose ? M.ST64.p : M.S64.p, U.ucls, U.uoff, U.ucnt, M.colpair.p,
+                       transpose ? M.pair_tgt.p : pair_src, M.slice_pairs[q], x, Y);
     TBEM_LAUNCH_CHECK();
 }
 
-void launch_m2l_sum64(const M2lMfma& M, int n_items, const int* items, const int* ptr, const double* Y, double* out,
-                      hipStream_t s) {
-    if (!M.enabled || !M.fp64 || n_items == 0) return;
-    hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(n_items), dim3(256), 0, s, items, ptr, nullptr, Y, M.ystride, M.blk, 0,
-                       out);
+// the forward sums of slice q: its items' pairs, out[t] = sum (FP32 Y into FP64 z, or FP64 Y)
+template <class T>
+static void forward_sum_slice(const M2lMfma& M, int q, const int* items, const int* ptr, const T* Y, double* out,
+                              hipStream_t s) {
+    const int k0 = M.slice_items[q], nk = M.slice_items[q + 1] - k0;
+    if (nk == 0) return;
+    if constexpr (std::is_same<T, float>::value) {
+        const int npar = 256 / (M.ystride / 4);
+        hipLaunchKernelGGL(k_m2l_pair_sum, dim3(nk), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
+                           items + k0, ptr + k0, nullptr, Y, M.ystride, M.blk, M.slice_pairs[q], 0, 0, 0, out);
+    } else {
+        hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(nk), dim3(256), 0, s, items + k0, ptr + k0, nullptr, Y, M.ystride,
+                           M.blk, M.slice_pairs[q], 0, 0, 0, out);
+    }
     TBEM_LAUNCH_CHECK();
 }
 
-void launch_m2l_sum64_adj(const M2lMfma& M, const double* Y, double* out, hipStream_t s) {
-    if (!M.enabled || !M.fp64 || M.n_adj == 0) return;
-    hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(M.n_adj), dim3(256), 0, s, M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p,
-                       Y, M.ystride, M.blk, 1, out);
+// the transpose's sums of slice q: per source cell, its pairs within the slice;
+// out[s] = (q == 0 and !add_first ? 0 : out[s]) + sum
+template <class T>
+static void adjoint_sum_slice(const M2lMfma& M, int q, const T* Y, double* out, bool add_first, hipStream_t s) {
+    if (M.n_adj == 0) return;
+    const int add = (q > 0 || add_first) ? 1 : 0;
+    const int pb = M.slice_pairs[q], pe = M.slice_pairs[q + 1];
+    if constexpr (std::is_same<T, float>::value) {
+        const int npar = 256 / (M.ystride / 4);
+        hipLaunchKernelGGL(k_m2l_pair_sum, dim3(M.n_adj), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
+                           M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, Y, M.ystride, M.blk, pb, pb, pe, add, out);
+    } else {
+        hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(M.n_adj), dim3(256), 0, s, M.adj_items.p, M.adj_ptr.p,
+                           M.adj_pairs.p, Y, M.ystride, M.blk, pb, pb, pe, add, out);
+    }
     TBEM_LAUNCH_CHECK();
 }
 
-void launch_m2l_modal_in(const M2lMfma& M, const CellTiles& targets, const double* lbar, float* w, hipStream_t s) {
+void m2l_forward32(const M2lMfma& M, const int* items, const int* ptr, const int* pair_src, const CellTiles& targets,
+                   const float* chk_up, float* Y, double* z, double* locals, hipStream_t s, hipEvent_t kernel_end) {
+    if (!M.enabled || M.fp64) return;
+    for (int q = 0; q < M.n_slices; q++) {
+        mfma32_slice(M, q, pair_src, chk_up, Y, false, s);
+        if (kernel_end && q + 1 == M.n_slices) TBEM_HIP(hipEventRecord(kernel_end, s));
+        forward_sum_slice<float>(M, q, items, ptr, Y, z, s);
+    }
+    launch_level_gemm(targets, M.blk, M.Vd.p, false, z, nullptr, locals, nullptr, true, s);
+}
+
+void m2l_forward64(const M2lMfma& M, const int* items, const int* ptr, const int* pair_src, const double* mult,
+                   double* Y, double* fld, hipStream_t s, hipEvent_t kernel_end) {
+    if (!M.enabled || !M.fp64) return;
+    for (int q = 0; q < M.n_slices; q++) {
+        mfma64_slice(M, q, pair_src, mult, Y, false, s);
+        if (kernel_end && q + 1 == M.n_slices) TBEM_HIP(hipEventRecord(kernel_end, s));
+        forward_sum_slice<double>(M, q, items, ptr, Y, fld, s);
+    }
+}
+
+void m2l_adjoint32(const M2lMfma& M, const CellTiles& targets, const double* lbar, float* w, float* Y, double* z,
+                   double* e2, hipStream_t s) {
     if (!M.enabled || M.fp64) return;
     launch_level_gemm(targets, M.blk, M.Vd.p, true, lbar, nullptr, nullptr, w, false, s);
+    for (int q = 0; q < M.n_slices; q++) {
+        mfma32_slice(M, q, nullptr, w, Y, true, s);
+        adjoint_sum_slice<float>(M, q, Y, z, false, s);
+    }
+    if (M.n_adj > 0) launch_level_gemm(M.adj_tiles, M.blk, M.Bu.p, true, z, nullptr, e2, nullptr, false, s);
 }
 
-void launch_m2l_reduce_adj(const M2lMfma& M, const float* Y, double* z, double* e2, hipStream_t s) {
-    if (!M.enabled || M.fp64 || M.n_adj == 0) return;
-    const int npar = 256 / (M.ystride / 4);
-    hipLaunchKernelGGL(k_m2l_pair_sum, dim3(M.n_adj), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
-                       M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, Y, M.ystride, M.blk, z);
-    TBEM_LAUNCH_CHECK();
-    launch_level_gemm(M.adj_tiles, M.blk, M.Bu.p, true, z, nullptr, e2, nullptr, false, s);
-}
-
-void launch_m2l_reduce(const M2lMfma& M, int n_items, const int* items, const int* ptr, const CellTiles& targets,
-                       const float* Y, double* z, double* locals, hipStream_t s) {
-    if (!M.enabled || M.fp64 || n_items == 0) return;
-    const int npar = 256 / (M.ystride / 4);
-    hipLaunchKernelGGL(k_m2l_pair_sum, dim3(n_items), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
-                       items, ptr, nullptr, Y, M.ystride, M.blk, z);
-    TBEM_LAUNCH_CHECK();
-    launch_level_gemm(targets, M.blk, M.Vd.p, false, z, nullptr, locals, nullptr, true, s);
+void m2l_adjoint64(const M2lMfma& M, const double* cbar, double* Y, double* mbar, hipStream_t s) {
+    if (!M.enabled || !M.fp64) return;
+    for (int q = 0; q < M.n_slices; q++) {
+        mfma64_slice(M, q, nullptr, cbar, Y, true, s);
+        adjoint_sum_slice<double>(M, q, Y, mbar, true, s);
+    }
 }
 
 }  // namespace tbem

@@ -46,6 +46,7 @@ PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 # kernel); other workloads report null.
 TRAFFIC_BYTES = {
     ("fault_surface", 1_000_000, "fp64", "leaf"): 644_165_984,
+    ("fault_surface_o100", 2_000_000, "fp64", "m2l"): 15_485_747_200,
     ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 2_370_368_469,
     ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 2_283_267_211,
     ("fault_surface_o100", 2_000_000, "mixed", "m2l"): 7_941_620_238,

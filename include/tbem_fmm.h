@@ -124,6 +124,12 @@ typedef struct {
      * pair, as cpp/fmm.cpp:362-382 does. Both are FP64 throughout; they differ in
      * rounding only. */
     int m2l_operators;
+
+    /* Bytes the class M2L's per-pair output may take (no reference counterpart):
+     * 0 = 6 GiB. One buffer per plan, shared by its clones; when the pairs need more,
+     * they are processed in slices of whole target cells that fit, with the same
+     * forward result bit for bit. */
+    size_t m2l_output_budget;
 } tbem_fmm_desc;
 
 enum { TBEM_TREES_AUTO = 0, TBEM_TREES_OWN_BOXES = 1, TBEM_TREES_COMMON_BOX = 2 };
