@@ -622,6 +622,21 @@ struct PhaseClock {
     }
 };
 
+// The point-target stage leaves the r2 < 1e-12 cutoff test out of source tiles whose
+// ranges lie farther than this from the leaf's points (same result; see
+// launch_range_close).
+constexpr double kCloseMargin = 1e-5;
+
+// l: a point-range list indexed by the leaves `leaves` of tree tt (points tp), over
+// the source points sp: (re)computes its per-range close flags.
+static void close_flags(DevList& l, const int* leaves, const DevTree& tt, const DevPoints& tp, const DevPoints& sp,
+                        hipStream_t s) {
+    const size_t n = l.rbeg.n;
+    if (l.rclose.n != n) l.rclose.alloc(n);
+    if (n) launch_range_close(l.n_items, leaves, view(tt), view(tp), l.ptr.p, l.rbeg.p, l.rcnt.p, view(sp),
+                              kCloseMargin, l.rclose.p, s);
+}
+
 static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s, const SavedPlan* sv = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     PhaseClock clk(s);
@@ -773,6 +788,7 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->obs_leaves.upload(OT.leaves, s);
     P->src_leaves.upload(ST.leaves, s);
     P->leaf_p2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.p2p, 0, &ST), s);
+    close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, s);
     P->leaf_m2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.m2p, 0, nullptr), s);
 
     clk.mark("rest");
@@ -798,6 +814,7 @@ const FmmPlan::AdjLists& FmmPlan::adjoint_lists(hipStream_t s) const {
     A->m2m.resize(ST.n_levels);
     for (int L = 1; L < ST.n_levels; L++) A->m2m[L].upload(parent_list(ST, L), s);
     A->leaf_p2p.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, &OT), s);
+    close_flags(A->leaf_p2p, src_leaves.p, dst, sp, op, s);
     A->leaf_p2l.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr), s);
     A->t_slev.resize(ST.n_levels);
     for (int L = 0; L < ST.n_levels; L++) A->t_slev[L] = make_level_tiles(ST, L, s);
@@ -922,7 +939,7 @@ static LeafStage leaf_stage_args(const FmmPlan& P, Work& W) {
     f.leaves = P.obs_leaves.p;
     f.tt = view(P.dot); f.tp = view(P.op); f.t_weight = nullptr; f.n_tgt = No;
     f.rp_ptr = P.leaf_p2p.ptr.p; f.rp_beg = P.leaf_p2p.rbeg.p; f.rp_cnt = P.leaf_p2p.rcnt.p;
-    f.rp_pre = P.leaf_p2p.rpre.p;
+    f.rp_pre = P.leaf_p2p.rpre.p; f.rp_close = P.leaf_p2p.rclose.p;
     f.ps = view(P.sp); f.pdens = W.dens.p; f.n_ps = Ns;
     f.sl_ptr = P.leaf_m2p.ptr.p; f.sl_cell = P.leaf_m2p.cell.p;
     f.slt = view(P.dst); f.sl_surf = P.d_inner.p; f.sl_coeff = W.mult.p;
@@ -1081,7 +1098,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     f.leaves = P.src_leaves.p;
     f.tt = view(P.dst); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = Ns;
     f.rp_ptr = A.leaf_p2p.ptr.p; f.rp_beg = A.leaf_p2p.rbeg.p; f.rp_cnt = A.leaf_p2p.rcnt.p;
-    f.rp_pre = A.leaf_p2p.rpre.p;
+    f.rp_pre = A.leaf_p2p.rpre.p; f.rp_close = A.leaf_p2p.rclose.p;
     f.ps = view(P.op); f.pdens = W.gsort.p; f.n_ps = No;
     f.sl_ptr = A.leaf_p2l.ptr.p; f.sl_cell = A.leaf_p2l.cell.p;
     f.slt = view(P.dot); f.sl_surf = P.d_inner.p; f.sl_coeff = W.cbar.p;
@@ -1146,6 +1163,12 @@ static void move_obs(FmmPlan& P, const double* d_locs, hipStream_t s) {
     hipLaunchKernelGGL(k_obs_move, dim3(div_up(No, 256)), dim3(256), 0, s, d_locs, P.op.perm.p, No, P.op.x.p,
                        P.op.y.p, P.op.z.p);
     TBEM_LAUNCH_CHECK();
+    // the point-target stages' close flags follow the moved points
+    close_flags(P.leaf_p2p, P.obs_leaves.p, P.dot, P.op, P.sp, s);
+    {
+        std::lock_guard<std::mutex> lk(P.adj_mu);
+        if (P.adj) close_flags(P.adj->leaf_p2p, P.src_leaves.p, P.dst, P.sp, P.op, s);
+    }
     TBEM_HIP(hipStreamSynchronize(s));
     P.obs_moved = true;
     P.obs_version++;

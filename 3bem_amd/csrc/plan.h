@@ -129,6 +129,7 @@ struct HostList {
 struct DevList {
     int n_items = 0;
     DevBuf<int> items, ptr, cell, rbeg, rcnt, rpre;
+    DevBuf<int> rclose;  // point-range lists of the point-target stage: launch_range_close
     void upload(const HostList& l, hipStream_t s);
 };
 

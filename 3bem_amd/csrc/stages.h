@@ -105,6 +105,8 @@ struct LeafStage {
     const double* t_weight;
     int n_tgt;
     const int *rp_ptr, *rp_beg, *rp_cnt, *rp_pre;   // point ranges, indexed by leaf position
+    const int* rp_close;    // per range: 1 when its points may come within the cutoff radius of
+                            // the leaf's (launch_range_close); null: every range
     PtsView ps;
     const double* pdens;
     int n_ps;
@@ -134,6 +136,15 @@ void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, 
 // point sources in FP32 (mixed precision; kernels with closed-form gradients)
 void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s, bool fp32_points);
 
+
+// For the point-range list l of the targets' leaves (item k = leaf leaves[k] of tree
+// tt, points tp; ranges of source points sp): close[e] = 1 when a point of range e
+// lies within `margin` of the tight box of its item's target points, else 0. A range
+// with close 0 holds no pair within that distance, so the point-target stage may
+// leave the r2 < 1e-12 test out for it (margin >= 1e-5 keeps that exact).
+void launch_range_close(int n_items, const int* leaves, const TreeView& tt, const PtsView& tp, const int* ptr,
+                        const int* rbeg, const int* rcnt, const PtsView& sp, double margin, int* close,
+                        hipStream_t s);
 
 void launch_gather_dens(const double* x, const int* perm, const double* w, int n, int R,
                         double* dens, hipStream_t s);

@@ -230,15 +230,26 @@ __device__ inline void stream_ranges(Tile<VT, R, NRM, NB>& tl, int e0, int e1,
 // from the range holding the tile's first point. Ranges must be nonempty: then a
 // cache that starts at the range holding point `base` covers at least
 // [base, base + NB), which is what one tile needs.
+//
+// rclose (optional): per range, 1 when some of its points may lie within the
+// kernels' cutoff radius of the targets. With tile_close given, *tile_close is set
+// before each compute() to whether any point of the tile comes from such a range
+// (1 for every tile when rclose is null), so that compute() can leave the cutoff
+// test out of the other tiles. The flag travels in bit 30 of the cached range
+// starts (point indices are below 2^30).
+constexpr int kCloseBit = 1 << 30;
+
 template <class VT, int R, bool NRM, int NB, class F>
 __device__ inline void stream_ranges_packed(Tile<VT, R, NRM, NB>& tl, int* s_pre, int* s_beg, int e0, int e1,
                                             const int* __restrict__ rbeg, const int* __restrict__ rcnt,
                                             const int* __restrict__ rpre, const PtsView& sp,
-                                            const double* __restrict__ dens, int n_src, F&& compute) {
+                                            const double* __restrict__ dens, int n_src, F&& compute,
+                                            const int* __restrict__ rclose = nullptr, int* tile_close = nullptr) {
     const int tid = threadIdx.x;
     if (e0 >= e1) return;
     const int total = uni(rpre[e1 - 1] + rcnt[e1 - 1]);
     SrcReg<R, NRM> r;
+    int mclose = 0;  // the lane's next point comes from a close range
     int lbase = e0;  // list index of cache entry 0
     auto cache = [&](int from) {
         __syncthreads();
@@ -246,7 +257,7 @@ __device__ inline void stream_ranges_packed(Tile<VT, R, NRM, NB>& tl, int* s_pre
         const int e = from + tid;
         if (e < e1) {
             s_pre[tid] = rpre[e];
-            s_beg[tid] = rbeg[e];
+            s_beg[tid] = rbeg[e] | ((!rclose || rclose[e]) ? kCloseBit : 0);
         }
         if (tid == 0) s_pre[NB] = (from + NB < e1) ? rpre[from + NB] : total;
         __syncthreads();
@@ -255,11 +266,14 @@ __device__ inline void stream_ranges_packed(Tile<VT, R, NRM, NB>& tl, int* s_pre
     int ecur = 0;  // cache entry of the range holding the current tile's first point (uniform)
     auto fetch = [&](int base) {
         const int v = base + tid;
+        mclose = 0;
         if (v < total) {
             const int nl = min(NB, e1 - lbase);
             int f = ecur;
             while (f + 1 < nl && s_pre[f + 1] <= v) f++;
-            load_point(r, s_beg[f] + (v - s_pre[f]), sp, dens, n_src);
+            const int bf = s_beg[f];
+            mclose = bf & kCloseBit;
+            load_point(r, (bf & ~kCloseBit) + (v - s_pre[f]), sp, dens, n_src);
         }
     };
     int base = 0;
@@ -268,7 +282,8 @@ __device__ inline void stream_ranges_packed(Tile<VT, R, NRM, NB>& tl, int* s_pre
         const int cnt = min(NB, total - base);
         __syncthreads();
         if (tid < cnt) store_tile(tl, tid, r);
-        __syncthreads();
+        if (tile_close) *tile_close = __syncthreads_or(tid < cnt && mclose);
+        else __syncthreads();
         base += NB;
         const bool more = base < total;
         if (more) {
@@ -455,11 +470,19 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
         acB.init();
         double sfA[SPLIT ? R : 1], sfB[SPLIT ? R : 1];
         for (int q = 0; q < (SPLIT ? R : 1); q++) sfA[q] = sfB[q] = 0.0;
+        int tclose = 1;  // set per tile by stream_ranges_packed
         auto points = [&](int cnt) {
             if (va) {
-                for (int j = slice; j < cnt; j += S) {
-                    pair<KID, ADJ, FAST>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
-                    pair<KID, ADJ, FAST>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                if (uni(tclose)) {
+                    for (int j = slice; j < cnt; j += S) {
+                        pair<KID, ADJ, FAST, true>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                        pair<KID, ADJ, FAST, true>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                    }
+                } else {  // no point of the tile within the cutoff of the leaf: same result without the test
+                    for (int j = slice; j < cnt; j += S) {
+                        pair<KID, ADJ, FAST, false>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                        pair<KID, ADJ, FAST, false>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                    }
                 }
             }
             acA.flush();
@@ -481,8 +504,10 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
                 }
             }
         };
+        // the cutoff-free tiles pay in FP64 (leaf 19.7 -> 19.2 ms at 2M points, order 100);
+        // the FP32 instance ran 7% slower with the second loop, so it keeps the one
         stream_ranges_packed(sm.pt, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
-                             a.n_ps, points);                                                     // (a)
+                             a.n_ps, points, SPLIT ? nullptr : a.rp_close, SPLIT ? nullptr : &tclose);  // (a)
         stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk,
                         surfaces);                                                                // (b)
         if (a.self_coeff)
@@ -809,6 +834,68 @@ void launch_level_gemm(const CellTiles& T, int blk, const double* M, bool trans,
     if (T.n == 0) return;
     hipLaunchKernelGGL(k_level_gemm, dim3(div_up(blk, 64), T.n), dim3(256), 0, s, blk, T.cells.p, T.tiles.p, M,
                        trans ? 1 : 0, in, addin, out, out32, add ? 1 : 0);
+    TBEM_LAUNCH_CHECK();
+}
+
+// One workgroup per item: the tight box of the item's target points (block min/max),
+// then per range whether any of its points lies within margin of that box.
+__global__ __launch_bounds__(256) void k_range_close(const int* __restrict__ leaves, TreeView tt, PtsView tp,
+                                                     const int* __restrict__ ptr, const int* __restrict__ rbeg,
+                                                     const int* __restrict__ rcnt, PtsView sp, double margin,
+                                                     int* __restrict__ close) {
+    __shared__ double red[6][256];
+    __shared__ double box[6];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    const int leaf = uni(leaves[k]);
+    const int b = uni(tt.begin[leaf]), m = uni(tt.count[leaf]);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = tid; i < m; i += 256) {
+        const double p[3] = {tp.x[b + i], tp.y[b + i], tp.z[b + i]};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = fmin(lo[a], p[a]);
+            hi[a] = fmax(hi[a], p[a]);
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        red[a][tid] = lo[a];
+        red[3 + a][tid] = hi[a];
+    }
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int a = 0; a < 3; a++) {
+                red[a][tid] = fmin(red[a][tid], red[a][tid + h]);
+                red[3 + a][tid] = fmax(red[3 + a][tid], red[3 + a][tid + h]);
+            }
+        __syncthreads();
+    }
+    if (tid < 6) box[tid] = red[tid][0];
+    __syncthreads();
+    const double m2 = margin * margin;
+    const int e0 = uni(ptr[k]), e1 = uni(ptr[k + 1]);
+    for (int e = e0; e < e1; e++) {
+        const int rb = uni(rbeg[e]), rc = uni(rcnt[e]);
+        int near = 0;
+        for (int j = tid; j < rc; j += 256) {
+            const double p[3] = {sp.x[rb + j], sp.y[rb + j], sp.z[rb + j]};
+            double d2 = 0;
+            for (int a = 0; a < 3; a++) {
+                const double g = fmax(fmax(box[a] - p[a], p[a] - box[3 + a]), 0.0);
+                d2 += g * g;
+            }
+            near |= (d2 <= m2) ? 1 : 0;
+        }
+        near = __syncthreads_or(near);
+        if (tid == 0) close[e] = near ? 1 : 0;
+    }
+}
+
+void launch_range_close(int n_items, const int* leaves, const TreeView& tt, const PtsView& tp, const int* ptr,
+                        const int* rbeg, const int* rcnt, const PtsView& sp, double margin, int* close,
+                        hipStream_t s) {
+    if (n_items == 0) return;
+    hipLaunchKernelGGL(k_range_close, dim3(n_items), dim3(256), 0, s, leaves, tt, tp, ptr, rbeg, rcnt, sp, margin,
+                       close);
     TBEM_LAUNCH_CHECK();
 }
 
