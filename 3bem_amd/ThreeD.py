@@ -175,18 +175,20 @@ class FMMOperator:
     tree_boxes: for distinct observation and source points, "auto" (each tree on its
     own points' box as the reference, except mixed precision at orders the MFMA M2L
     serves, where both share the union box), "own" or "common" to force either.
-    m2l: the FP64 M2L, "auto" (one FP64 operator per translation class on the FP64
-    MFMA units where the classes exist, else as "surfaces") or "surfaces" (the kernel
-    between the two surfaces of every pair, as the reference); mixed ignores it.
+    translations: the translations between surfaces (M2M, L2L, and the FP64 path's
+    M2L), "auto" (one FP64 operator per translation class on the FP64 MFMA units
+    where the classes exist, else as "surfaces") or "surfaces" (the kernel between
+    the two surfaces of every pair, as the reference).
     m2l_output_budget: bytes the class M2L's per-pair output may take (None: 6 GiB);
     beyond it the pairs run in slices.
     """
 
     TREE_BOXES = {"auto": 0, "own": 1, "common": 2}
-    M2L_OPERATORS = {"auto": 0, "surfaces": 1}
+    TRANSLATIONS = {"auto": 0, "surfaces": 1}
 
     def __init__(self, K, data, config, precision="fp64", up_c2e=None, down_c2e=None, plan=None,
-                 tree_boxes="auto", m2l="auto", m2l_output_budget=None):
+                 tree_boxes="auto", translations="auto",
+                 m2l_output_budget=None):
         lib = _lib.require_device()
         self.K = K
         self.data = data
@@ -208,7 +210,7 @@ class FMMOperator:
         desc.account_for_small_cells = 1 if config.account_for_small_cells else 0
         desc.precision = PRECISIONS[precision]
         desc.tree_boxes = self.TREE_BOXES[tree_boxes]
-        desc.m2l_operators = self.M2L_OPERATORS[m2l]
+        desc.translations = self.TRANSLATIONS[translations]
         desc.m2l_output_budget = int(m2l_output_budget or 0)
         if self._up is not None:
             desc.up_check_to_equiv, desc.n_up_levels = _dp(self._up), self._up.shape[0]

@@ -47,7 +47,7 @@ class FmmDesc(ctypes.Structure):
         ("account_for_small_cells", ctypes.c_int), ("precision", ctypes.c_int),
         ("up_check_to_equiv", _D), ("n_up_levels", _SZ),
         ("down_check_to_equiv", _D), ("n_down_levels", _SZ), ("tree_boxes", ctypes.c_int),
-        ("m2l_operators", ctypes.c_int), ("m2l_output_budget", _SZ),
+        ("translations", ctypes.c_int), ("m2l_output_budget", _SZ),
     ]
 
 

@@ -74,6 +74,10 @@ struct FmmPlan {
     mutable std::unique_ptr<AdjLists> adj;
     const AdjLists& adjoint_lists(hipStream_t s) const;
     M2lMfma m2l_mfma;            // M2L as grouped MFMA GEMMs: FP32 modal (mixed) or FP64 (fp64)
+    // M2M (by source level: parents at L) and L2L (by observation level: children at L)
+    // on per-class FP64 operators (8 octant classes per level), where enabled
+    std::vector<M2lMfma> m2m_ops, l2l_ops;
+    size_t tr_y = 0;             // their largest per-pair output, in doubles (a per-handle buffer)
     // GEMM tiles of the C2E products: the cells each stage writes
     CellTiles t_p2m, t_m2l, t_p2l;
     std::vector<CellTiles> t_m2m;          // by source level: the M2M parents
@@ -139,6 +143,7 @@ struct Work {
     DevBuf<float> m2l_w;    // transpose: modal input per observation cell
     DevBuf<double> e2;      // transpose: modal M2L^T contribution to ebar per source cell
     DevBuf<double> fld;     // raw check potentials of one stage, blk per cell (input of the C2E GEMMs)
+    DevBuf<double> ty;      // per-pair output of the M2M / L2L class operators
     hipEvent_t ev[10];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
@@ -156,6 +161,7 @@ struct Work {
         locals.alloc((size_t)P.dot.n_cells * P.blk);
         out_sorted.alloc(P.R * P.n_obs);
         fld.alloc((size_t)std::max(P.dst.n_cells, P.dot.n_cells) * P.blk);
+        ty.alloc(P.tr_y);
         if (P.m2l_mfma.enabled && !P.m2l_mfma.fp64) {
             chk_up.alloc((size_t)P.dst.n_cells * P.blk);
             chk_up.zero(stream);  // cells without a P2M/M2M stage keep a zero potential
@@ -315,8 +321,8 @@ static void validate(const tbem_fmm_desc* d) {
                  "precision must be TBEM_FP64 or TBEM_MIXED");
     TBEM_REQUIRE(d->tree_boxes >= TBEM_TREES_AUTO && d->tree_boxes <= TBEM_TREES_COMMON_BOX, TBEM_E_INVALID,
                  "tree_boxes must be TBEM_TREES_AUTO, _OWN_BOXES or _COMMON_BOX");
-    TBEM_REQUIRE(d->m2l_operators == TBEM_M2L_AUTO || d->m2l_operators == TBEM_M2L_SURFACES, TBEM_E_INVALID,
-                 "m2l_operators must be TBEM_M2L_AUTO or TBEM_M2L_SURFACES");
+    TBEM_REQUIRE(d->translations == TBEM_TRANSLATIONS_AUTO || d->translations == TBEM_TRANSLATIONS_SURFACES,
+                 TBEM_E_INVALID, "translations must be TBEM_TRANSLATIONS_AUTO or TBEM_TRANSLATIONS_SURFACES");
     TBEM_REQUIRE(d->n_obs < (1u << 30) && d->n_src < (1u << 30), TBEM_E_UNSUPPORTED,
                  "more than 2^30 points per side");
     TBEM_REQUIRE(d->n_obs == 0 || (d->obs_locs && d->obs_normals), TBEM_E_INVALID, "null observation arrays");
@@ -749,12 +755,23 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
 
     P->p2m.upload(self_list(ST, ST.leaves), s);
     P->t_p2m = make_cell_tiles(ST.leaves, ST, s);
+    // M2M and L2L on class operators: the cells of one tree always lie on one grid
+    const bool tr_classes = d->translations == TBEM_TRANSLATIONS_AUTO && m2l_fp64_size_ok(P->blk);
+    auto note_y = [&](const M2lMfma& M) {
+        if (M.enabled) P->tr_y = std::max(P->tr_y, (size_t)M.y_pairs * M.ystride);
+    };
     P->m2m.resize(ST.n_levels);
     P->t_m2m.resize(ST.n_levels);
+    P->m2m_ops.resize(ST.n_levels);
     for (int L = 0; L + 1 < ST.n_levels; L++) {
         const HostList l = children_list(ST, L);
         P->m2m[L].upload(l, s);
         P->t_m2m[L] = make_cell_tiles(l.items, ST, s);
+        if (tr_classes) {
+            build_m2l_mfma(P->m2m_ops[L], P->kid, P->kc, ST, ST, P->dst, P->dst, true, true, kM2lOutputBudget, l,
+                           P->m2m[L], P->surf, nullptr, nullptr, s, nullptr, nullptr, &P->surf.outer, &P->surf.inner);
+            note_y(P->m2m_ops[L]);
+        }
     }
     const HostList m2l_h = list_from_pairs(cells_with(tl.m2l, 0, OT.n_cells), OT.n_cells, tl.m2l, 0, nullptr);
     P->m2l.upload(m2l_h, s);
@@ -763,7 +780,7 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     clk.mark("lists");
     // FP64: class operators where the classes exist (the builder also requires the
     // operators to fit an eighth of the device memory)
-    const bool fp64_classes = P->precision == TBEM_FP64 && d->m2l_operators == TBEM_M2L_AUTO &&
+    const bool fp64_classes = P->precision == TBEM_FP64 && d->translations == TBEM_TRANSLATIONS_AUTO &&
                               P->tree_layout != kTreesOwnBoxes && m2l_fp64_size_ok(P->blk);
     if (P->precision == TBEM_MIXED || fp64_classes)
         build_m2l_mfma(P->m2l_mfma, P->kid, P->kc, OT, ST, P->dot, P->dst, P->tree_layout != kTreesOwnBoxes,
@@ -784,7 +801,16 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     P->t_olev.resize(OT.n_levels);
     for (int L = 0; L < OT.n_levels; L++) P->t_olev[L] = make_level_tiles(OT, L, s);
     P->l2l.resize(OT.n_levels);
-    for (int L = 1; L < OT.n_levels; L++) P->l2l[L].upload(parent_list(OT, L), s);
+    P->l2l_ops.resize(OT.n_levels);
+    for (int L = 1; L < OT.n_levels; L++) {
+        const HostList l = parent_list(OT, L);
+        P->l2l[L].upload(l, s);
+        if (tr_classes) {
+            build_m2l_mfma(P->l2l_ops[L], P->kid, P->kc, OT, OT, P->dot, P->dot, true, true, kM2lOutputBudget, l,
+                           P->l2l[L], P->surf, nullptr, nullptr, s, nullptr, nullptr, &P->surf.inner, &P->surf.outer);
+            note_y(P->l2l_ops[L]);
+        }
+    }
     P->obs_leaves.upload(OT.leaves, s);
     P->src_leaves.upload(ST.leaves, s);
     P->leaf_p2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.p2p, 0, &ST), s);
@@ -874,12 +900,17 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
 
     // M2M (cpp/fmm.cpp:204-251), deepest parents first
     for (int L = P.st.n_levels - 2; L >= 0; L--) {
-        SurfStage m = surf_base(P);
-        set_list(m, P.m2m[L]);
-        m.tt = view(P.dst); m.t_surf = P.d_outer.p;
-        m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
-        m.out = W.fld.p;
-        launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
+        if (P.m2m_ops[L].enabled) {
+            m2l_forward64(P.m2m_ops[L], P.m2m[L].items.p, P.m2m[L].ptr.p, P.m2m[L].cell.p, W.mult.p, W.ty.p, W.fld.p,
+                          s, nullptr);
+        } else {
+            SurfStage m = surf_base(P);
+            set_list(m, P.m2m[L]);
+            m.tt = view(P.dst); m.t_surf = P.d_outer.p;
+            m.st = view(P.dst); m.s_surf = P.d_inner.p; m.s_coeff = W.mult.p;
+            m.out = W.fld.p;
+            launch_surf_stage(P.kid, true, false, false, m, P.kc, s);
+        }
         up_c2e(P.t_m2m[L]);
     }
     TBEM_HIP(hipEventRecord(W.ev[3], s));
@@ -920,12 +951,17 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
 
     // L2L (cpp/fmm.cpp:384-413), root first
     for (int L = 1; L < P.ot.n_levels; L++) {
-        SurfStage l = surf_base(P);
-        set_list(l, P.l2l[L]);
-        l.tt = view(P.dot); l.t_surf = P.d_inner.p;
-        l.st = view(P.dot); l.s_surf = P.d_outer.p; l.s_coeff = W.locals.p;
-        l.out = W.fld.p;
-        launch_surf_stage(P.kid, true, false, false, l, P.kc, s);
+        if (P.l2l_ops[L].enabled) {
+            m2l_forward64(P.l2l_ops[L], P.l2l[L].items.p, P.l2l[L].ptr.p, P.l2l[L].cell.p, W.locals.p, W.ty.p,
+                          W.fld.p, s, nullptr);
+        } else {
+            SurfStage l = surf_base(P);
+            set_list(l, P.l2l[L]);
+            l.tt = view(P.dot); l.t_surf = P.d_inner.p;
+            l.st = view(P.dot); l.s_surf = P.d_outer.p; l.s_coeff = W.locals.p;
+            l.out = W.fld.p;
+            launch_surf_stage(P.kid, true, false, false, l, P.kc, s);
+        }
         down_c2e(P.t_olev[L]);
     }
     TBEM_HIP(hipEventRecord(W.ev[5], s));
@@ -1045,6 +1081,10 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     for (int L = P.ot.n_levels - 1; L >= 0; L--) {
         launch_level_gemm(P.t_olev[L], P.blk, P.dc2e.p, true, W.lbar.p, nullptr, W.cbar.p, nullptr, false, s);
         if (L == 0) break;
+        if (P.l2l_ops[L].enabled) {  // lbar[parent] += sum over its children of K^T cbar[child]
+            m2l_adjoint64(P.l2l_ops[L], W.cbar.p, W.ty.p, W.lbar.p, s);
+            continue;
+        }
         SurfStage l = surf_base(P);
         set_list(l, A.l2l[L - 1]);
         l.tt = view(P.dot); l.t_surf = P.d_outer.p;
@@ -1084,6 +1124,10 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         launch_level_gemm(A.t_slev[L], P.blk, P.uc2e.p, true, W.mbar.p, mfma ? W.e2.p : nullptr, W.ebar.p, nullptr,
                           false, s);
         if (L + 1 >= P.st.n_levels) break;
+        if (P.m2m_ops[L].enabled) {  // mbar[child] += K^T ebar[parent]
+            m2l_adjoint64(P.m2m_ops[L], W.ebar.p, W.ty.p, W.mbar.p, s);
+            continue;
+        }
         SurfStage c = surf_base(P);
         set_list(c, A.m2m[L + 1]);
         c.tt = view(P.dst); c.t_surf = P.d_inner.p;
@@ -1460,7 +1504,7 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
         const Work& W = *h->work;
         double wb = 0;
         for (const DevBuf<double>* b : {&W.dens, &W.mult, &W.locals, &W.out_sorted, &W.xbuf, &W.ybuf, &W.gsort,
-                                        &W.lbar, &W.cbar, &W.mbar, &W.ebar, &W.grad_sorted, &W.e2, &W.fld})
+                                        &W.lbar, &W.cbar, &W.mbar, &W.ebar, &W.grad_sorted, &W.e2, &W.fld, &W.ty})
             wb += (double)b->n * sizeof(double);
         for (const DevBuf<float>* b : {&W.chk_up, &W.m2l_w}) wb += (double)b->n * sizeof(float);
         o->work_bytes = wb;

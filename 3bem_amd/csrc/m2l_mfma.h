@@ -51,11 +51,15 @@ bool m2l_mfma_size_ok(int blk);
 bool m2l_fp64_size_ok(int blk);
 // fp64: build the FP64 path's operators (see above) instead of the mixed path's.
 // y_budget: bytes the per-pair output may take; the pairs are sliced to fit.
+// t_surf, s_surf: the target's check and the source's equivalent surface (unit
+// radius; null: the inner one, as for the M2L). With the outer / inner pair the
+// FP64 operators serve the M2M (target: the parent), with inner / outer the L2L.
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
                     const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, size_t y_budget,
                     const HostList& m2l,
                     const DevList& dm2l, const Surfaces& surf, const double* d_uc2e, const double* d_dc2e,
-                    hipStream_t s, const DevBuf<double>* up_fac = nullptr, const DevBuf<double>* down_fac = nullptr);
+                    hipStream_t s, const DevBuf<double>* up_fac = nullptr, const DevBuf<double>* down_fac = nullptr,
+                    const std::vector<double>* t_surf = nullptr, const std::vector<double>* s_surf = nullptr);
 
 // Mixed path, forward: slice by slice, Y[p] = M_{class(p)} * chk_up[src(p)] (FP32
 // MFMA; chk_up = Bu * up-check potential per source cell, FP32) and z[t] = sum of the

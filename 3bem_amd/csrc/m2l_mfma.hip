@@ -59,12 +59,14 @@ constexpr int kMaxChunk = 16384;  // classes per plan-time batch (grid y/z limit
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// K_c for a chunk of classes: rows (d1, i) = component d1 at target check point i,
-// columns (d2, j) = component d2 of source equivalent point j.
+// K_c for a chunk of classes: rows (d1, i) = component d1 at target check point i
+// (tsurf, scaled by the target cell's radius), columns (d2, j) = component d2 of
+// source equivalent point j (ssurf, scaled by the source cell's radius).
 __global__ void k_m2l_assemble(int kid, KConst kc, int n_e, int R, const double* __restrict__ tc,
                                const double* __restrict__ sc, const double* __restrict__ tr,
-                               const double* __restrict__ sr, const double* __restrict__ inner,
-                               const double* __restrict__ unit, double* __restrict__ mats) {
+                               const double* __restrict__ sr, const double* __restrict__ tsurf,
+                               const double* __restrict__ ssurf, const double* __restrict__ unit,
+                               double* __restrict__ mats) {
 #pragma clang fp contract(off)
     const int c = blockIdx.y;
     const int n = R * n_e;
@@ -72,8 +74,8 @@ __global__ void k_m2l_assemble(int kid, KConst kc, int n_e, int R, const double*
         const int i = p / n_e, j = p % n_e;
         double ck[3], eq[3];
         for (int d = 0; d < 3; d++) {
-            ck[d] = tr[c] * inner[3 * i + d] + tc[3 * c + d];
-            eq[d] = sr[c] * inner[3 * j + d] + sc[3 * c + d];
+            ck[d] = tr[c] * tsurf[3 * i + d] + tc[3 * c + d];
+            eq[d] = sr[c] * ssurf[3 * j + d] + sc[3 * c + d];
         }
         double kv[9];
         tensor_ref_dispatch(kid, ck, eq, unit + 3 * i, unit + 3 * j, kc, kv);
@@ -531,7 +533,8 @@ bool m2l_fp64_size_ok(int n) { return size_ok(n, true); }
 void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, const HostTree& st,
                     const DevTree& dot, const DevTree& dst, bool same_grid, bool fp64, size_t y_budget,
                     const HostList& m2l, const DevList& dm2l, const Surfaces& surf, const double* d_uc2e,
-                    const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac, const DevBuf<double>* down_fac) {
+                    const double* d_dc2e, hipStream_t s, const DevBuf<double>* up_fac, const DevBuf<double>* down_fac,
+                    const std::vector<double>* t_surf, const std::vector<double>* s_surf) {
     auto t0 = std::chrono::steady_clock::now();
     M = M2lMfma{};
     const int R = kernel_dim(kid), n_e = surf.n_e, n = R * n_e;
@@ -635,9 +638,10 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     }
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nc, (size_t)kMaxChunk, (size_t)(5e8 / (nn * 8))}));
     DevBuf<double> K((size_t)chunk * nn), T1(fp64 ? 0 : (size_t)chunk * nn), S64(fp64 ? 0 : (size_t)chunk * nn);
-    DevBuf<double> d_tc, d_sc, d_tr, d_sr, d_inner, d_unit;
+    DevBuf<double> d_tc, d_sc, d_tr, d_sr, d_tsurf, d_ssurf, d_unit;
     DevBuf<int> d_iota, d_ls, d_lt;
-    d_inner.upload(surf.inner, s);
+    d_tsurf.upload(t_surf ? *t_surf : surf.inner, s);
+    d_ssurf.upload(s_surf ? *s_surf : surf.inner, s);
     d_unit.upload(surf.unit, s);
     std::vector<int> iota(chunk);
     for (int i = 0; i < chunk; i++) iota[i] = i;
@@ -658,7 +662,8 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
         d_tc.upload(tc, s); d_sc.upload(scn, s); d_tr.upload(tr, s); d_sr.upload(sr, s);
         d_lt.upload(lt, s); d_ls.upload(ls, s);
         hipLaunchKernelGGL(k_m2l_assemble, dim3(std::max(1, std::min(64, div_up((size_t)n_e * n_e, 256))), m),
-                           dim3(256), 0, s, kid, kc, n_e, R, d_tc.p, d_sc.p, d_tr.p, d_sr.p, d_inner.p, d_unit.p,
+                           dim3(256), 0, s, kid, kc, n_e, R, d_tc.p, d_sc.p, d_tr.p, d_sr.p, d_tsurf.p, d_ssurf.p,
+                           d_unit.p,
                            K.p);
         TBEM_LAUNCH_CHECK();
         const dim3 gs(std::max(1, std::min(256, div_up(per, 256))), m);

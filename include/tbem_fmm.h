@@ -114,16 +114,17 @@ typedef struct {
      * TBEM_TREES_OWN_BOXES and TBEM_TREES_COMMON_BOX force either layout. */
     int tree_boxes;
 
-    /* FP64 M2L (no reference counterpart; TBEM_MIXED ignores it):
-     * TBEM_M2L_AUTO (0, the default) translates with one FP64 operator per
+    /* The translations between surfaces (no reference counterpart):
+     * TBEM_TRANSLATIONS_AUTO (0, the default) applies one FP64 operator per
      * translation class (level pair and integer centre offset) on the FP64 MFMA units
-     * where the classes exist (the cells of both sides on one grid: one shared tree,
-     * or tree_boxes TBEM_TREES_COMMON_BOX) and their per-pair output fits an eighth
-     * of the device memory, and falls back to TBEM_M2L_SURFACES elsewhere;
-     * TBEM_M2L_SURFACES evaluates the kernel between the two surfaces of every M2L
-     * pair, as cpp/fmm.cpp:362-382 does. Both are FP64 throughout; they differ in
-     * rounding only. */
-    int m2l_operators;
+     * wherever the classes exist: M2M and L2L always (the cells of one tree lie on one
+     * grid), the FP64 path's M2L when the cells of both sides do (one shared tree, or
+     * tree_boxes TBEM_TREES_COMMON_BOX). Elsewhere, and everywhere with
+     * TBEM_TRANSLATIONS_SURFACES, the kernel is evaluated between the two surfaces of
+     * every pair, as cpp/fmm.cpp:204-251, 362-413 do. Both are FP64 throughout and
+     * differ in rounding only. TBEM_MIXED's M2L always runs on its FP32 modal
+     * operators. */
+    int translations;
 
     /* Bytes the class M2L's per-pair output may take (no reference counterpart):
      * 0 = 6 GiB. One buffer per plan, shared by its clones; when the pairs need more,
@@ -133,7 +134,7 @@ typedef struct {
 } tbem_fmm_desc;
 
 enum { TBEM_TREES_AUTO = 0, TBEM_TREES_OWN_BOXES = 1, TBEM_TREES_COMMON_BOX = 2 };
-enum { TBEM_M2L_AUTO = 0, TBEM_M2L_SURFACES = 1 };
+enum { TBEM_TRANSLATIONS_AUTO = 0, TBEM_TRANSLATIONS_SURFACES = 1 };
 
 /* Plan statistics (tbem_fmm_info). */
 typedef struct {

@@ -31,12 +31,12 @@ def case(*a):
     print("CASE", *a, flush=True)
 
 
-def fixture_op(tbem, g, precision="fp64", inject=False, m2l="auto"):
+def fixture_op(tbem, g, precision="fp64", inject=False, translations="auto"):
     K = tbem.make_kernel(int(g["kernel"]), 1.0, 0.25)
     data = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
     cfg = tbem.FMMConfig(float(g["mac"]), int(g["order"]), int(g["leaf"]), float(g["d"]), bool(g["small"]))
     kw = dict(up_c2e=g["uc2e"], down_c2e=g["dc2e"]) if inject else {}
-    return tbem.FMMOperator(K, data, cfg, precision, m2l=m2l, **kw)
+    return tbem.FMMOperator(K, data, cfg, precision, translations=translations, **kw)
 
 
 @pytest.mark.parametrize("name", ["cube_clusters_laplace_o75", "fault_surface_elastic_u_o50",
@@ -49,7 +49,7 @@ def test_high_order_fixture(tbem, name):
     op_inj = fixture_op(tbem, g, inject=True)
     assert op_inj.info()["m2l_classes"] > 0  # FP64 M2L on class operators
     e_inj = rel_err(op_inj.apply(g["x"]), y_ref)
-    e_inj_surf = rel_err(fixture_op(tbem, g, inject=True, m2l="surfaces").apply(g["x"]), y_ref)
+    e_inj_surf = rel_err(fixture_op(tbem, g, inject=True, translations="surfaces").apply(g["x"]), y_ref)
     y_own = fixture_op(tbem, g).apply(g["x"])
     y_mix = fixture_op(tbem, g, "mixed").apply(g["x"])
     print(f"  fp64 (reference C2E) vs reference: class M2L {e_inj:.2e}, surface M2L {e_inj_surf:.2e}; "
@@ -87,7 +87,7 @@ def test_order_100_against_live_reference(tbem, wl, ref, kid):
     data = tbem.NBodyData(loc, nrm, loc, nrm, w)
     cfg = tbem.FMMConfig(0.3, 100, 64, 0.05, False)
     y_inj = tbem.FMMOperator(K, data, cfg, up_c2e=up, down_c2e=down).apply(g["x"])
-    y_inj_surf = tbem.FMMOperator(K, data, cfg, up_c2e=up, down_c2e=down, m2l="surfaces").apply(g["x"])
+    y_inj_surf = tbem.FMMOperator(K, data, cfg, up_c2e=up, down_c2e=down, translations="surfaces").apply(g["x"])
     op_mix = tbem.FMMOperator(K, data, cfg, precision="mixed")
     assert op_mix.info()["m2l_classes"] > 0  # the MFMA M2L runs
     y_mix = op_mix.apply(g["x"])

@@ -2,7 +2,7 @@
 
 By default the FP64 FMM translates each M2L pair with the FP64 operator of its
 translation class (level pair, integer centre offset) instead of evaluating the
-kernel between the two surfaces (m2l="surfaces", the reference's way). Both are FP64
+kernel between the two surfaces (translations="surfaces", the reference's way). Both are FP64
 throughout and differ in rounding only, so they must agree far below the FMM's own
 error, for every kernel, both directions (apply, apply_adjoint) and the observation
 gradient. The reference fixtures of test_gpu_configs.py pin the class path against
@@ -38,8 +38,8 @@ def pair(tbem, kid, n, order, leaf, seed=0, obs=None):
         data = tbem.NBodyData(obs[0], obs[1], locs, nrm, w)
         kw = dict(tree_boxes="common")
     cfg = tbem.FMMConfig(0.3, order, leaf, 0.05, False)
-    a = tbem.FMMOperator(K, data, cfg, m2l="auto", **kw)
-    b = tbem.FMMOperator(K, data, cfg, m2l="surfaces", **kw)
+    a = tbem.FMMOperator(K, data, cfg, translations="auto", **kw)
+    b = tbem.FMMOperator(K, data, cfg, translations="surfaces", **kw)
     return K, data, a, b
 
 
