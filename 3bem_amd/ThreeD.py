@@ -370,6 +370,12 @@ class FMMOperator:
         _lib.check(self._lib.tbem_fmm_stage_times(self._h, _dp(out)))
         return dict(zip(("gather", "p2m", "m2m", "m2l", "l2l", "leaf", "scatter"), out.tolist()))
 
+    def set_overlap(self, on=True):
+        """Run the forward's point-to-point pairs on a second stream beside the far
+        field (the default), or as one fused leaf launch after it (each kernel then
+        has the device to itself, for timing). Same results to rounding."""
+        _lib.check(self._lib.tbem_fmm_set_overlap(self._h, 1 if on else 0))
+
     def m2l_kernel_ms(self):
         """Device milliseconds of the M2L translation kernel alone in the last apply."""
         out = np.zeros(1)

@@ -22,7 +22,7 @@ EXPORTED = (
     "tbem_fmm_forward_token", "tbem_fmm_obs_grad_reuse_dev",
     "tbem_fmm_set_obs_locs", "tbem_fmm_set_obs_locs_dev", "tbem_fmm_obs_version", "tbem_fmm_n_rows", "tbem_fmm_n_cols", "tbem_fmm_clone",
     "tbem_fmm_destroy", "tbem_fmm_info", "tbem_fmm_c2e", "tbem_fmm_pairs", "tbem_fmm_octree",
-    "tbem_fmm_stage_times", "tbem_fmm_m2l_ms", "tbem_nbody_create", "tbem_nbody_apply", "tbem_nbody_apply_dev", "tbem_nbody_apply_noalloc",
+    "tbem_fmm_stage_times", "tbem_fmm_m2l_ms", "tbem_fmm_set_overlap", "tbem_nbody_create", "tbem_nbody_apply", "tbem_nbody_apply_dev", "tbem_nbody_apply_noalloc",
     "tbem_nbody_apply_adjoint", "tbem_nbody_n_rows", "tbem_nbody_n_cols", "tbem_nbody_destroy",
     "tbem_sparse_create", "tbem_sparse_from_coo", "tbem_sparse_apply", "tbem_sparse_apply_dev",
     "tbem_sparse_csr", "tbem_sparse_clone", "tbem_sparse_right_multiply", "tbem_sparse_n_rows", "tbem_sparse_n_cols", "tbem_sparse_nnz",
@@ -117,6 +117,7 @@ def load(path=LIB_PATH):
         "tbem_fmm_octree": (ctypes.c_int, [_VP, ctypes.c_int, _D, _L, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
         "tbem_fmm_stage_times": (ctypes.c_int, [_VP, _D]),
         "tbem_fmm_m2l_ms": (ctypes.c_int, [_VP, _D]),
+        "tbem_fmm_set_overlap": (ctypes.c_int, [_VP, ctypes.c_int]),
         "tbem_nbody_create": (ctypes.c_int, [ctypes.POINTER(NbodyDesc), ctypes.POINTER(_VP)]),
         "tbem_nbody_apply": (ctypes.c_int, [_VP, _D, _D]),
         "tbem_nbody_apply_dev": (ctypes.c_int, [_VP, _VP, _VP, _VP]),

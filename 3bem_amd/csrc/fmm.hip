@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstddef>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -144,9 +145,17 @@ struct Work {
     DevBuf<double> e2;      // transpose: modal M2L^T contribution to ebar per source cell
     DevBuf<double> fld;     // raw check potentials of one stage, blk per cell (input of the C2E GEMMs)
     DevBuf<double> ty;      // per-pair output of the M2M / L2L class operators
-    hipEvent_t ev[10];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel
+    hipEvent_t ev[12];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel; 10, 11
+                        // around the point-pair (P2P) launch on the side stream
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
+    // The forward's P2P needs only the gathered densities, so by default it runs on a
+    // side stream beside the far field (P2M ... L2L: MFMA and small launches beside
+    // the P2P's VALU work), joined before the surface sources (M2P, L2P) are added.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool overlap = true;
+    bool split_last = false;  // the last forward ran split (stage times add both leaf launches)
     // far-field state: dens (of x), mult and locals hold the far field of the call
     // that set far_token (a forward or a recomputing gradient; 0: none). The adjoint
     // leaves mult and locals alone but overwrites dens.
@@ -156,6 +165,9 @@ struct Work {
         for (auto& e: ev) TBEM_HIP(hipEventCreate(&e));
         TBEM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         TBEM_HIP(hipEventRecord(done, stream));
+        TBEM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        TBEM_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        TBEM_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
         dens.alloc(P.R * P.n_src);
         mult.alloc((size_t)P.dst.n_cells * P.blk);
         locals.alloc((size_t)P.dot.n_cells * P.blk);
@@ -174,7 +186,10 @@ struct Work {
     void end(hipStream_t s) { TBEM_HIP(hipEventRecord(done, s)); }
     ~Work() {
         if (done) (void)hipEventDestroy(done);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
         for (auto& e: ev) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -870,12 +885,16 @@ static void set_list(SurfStage& a, const DevList& l) {
 
 // gather, upward pass, M2L/P2L and L2L: everything the leaf stage consumes. FP64,
 // except the M2L when mfma is set (mixed precision, MFMA operators built).
-static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, bool mfma) {
+// after_gather (optional) runs once the densities are gathered (on s), before the
+// upward pass is enqueued; without it (F = nullptr_t) nothing is called.
+template <class F = std::nullptr_t>
+static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, bool mfma, F&& after_gather = nullptr) {
     const int R = P.R;
     const int Ns = (int)P.n_src;
     TBEM_HIP(hipEventRecord(W.ev[0], s));
     launch_gather_dens(x, P.sp.perm.p, P.sp.w.p, Ns, R, W.dens.p, s);
     TBEM_HIP(hipEventRecord(W.ev[1], s));
+    if constexpr (!std::is_same<typename std::decay<F>::type, std::nullptr_t>::value) after_gather();
 
     // the up-check potentials of a stage's cells -> equivalent densities, and, for
     // the MFMA M2L, the modal FP32 input Bu * check
@@ -994,9 +1013,34 @@ static void forward_T(const FmmPlan& P, Work& W, const double* x, double* y, hip
         launch_fill(y, (size_t)R * No, 0.0, s);
         return;
     }
-    farfield(P, W, x, s, std::is_same<T, float>::value && P.m2l_mfma.enabled);
-    W.far_token = ++W.next_token;
-    launch_leaf_stage<T>(P.kid, false, leaf_stage_args(P, W), P.kc, s, true);
+    const bool mfma = std::is_same<T, float>::value && P.m2l_mfma.enabled;
+    W.split_last = W.overlap;
+    if (W.overlap) {
+        // P2P on the side stream once the densities are gathered, beside the far field
+        farfield(P, W, x, s, mfma, [&] {
+            TBEM_HIP(hipEventRecord(W.fork, s));
+            TBEM_HIP(hipStreamWaitEvent(W.side, W.fork, 0));
+            LeafStage f = leaf_stage_args(P, W);
+            f.sl_ptr = nullptr;
+            f.sl_cell = nullptr;
+            f.self_coeff = nullptr;
+            TBEM_HIP(hipEventRecord(W.ev[10], W.side));
+            launch_leaf_stage<T>(P.kid, false, f, P.kc, W.side, true);
+            TBEM_HIP(hipEventRecord(W.ev[11], W.side));
+            TBEM_HIP(hipEventRecord(W.join, W.side));
+        });
+        W.far_token = ++W.next_token;
+        // then M2P + L2P (FP64) added to the P2P output
+        TBEM_HIP(hipStreamWaitEvent(s, W.join, 0));
+        LeafStage f = leaf_stage_args(P, W);
+        f.rp_ptr = nullptr;
+        f.accumulate = 1;
+        launch_leaf_stage<double>(P.kid, false, f, P.kc, s, true);
+    } else {
+        farfield(P, W, x, s, mfma);
+        W.far_token = ++W.next_token;
+        launch_leaf_stage<T>(P.kid, false, leaf_stage_args(P, W), P.kc, s, true);
+    }
     TBEM_HIP(hipEventRecord(W.ev[6], s));
     launch_scatter_out(W.out_sorted.p, P.op.perm.p, No, R, y, s);
     TBEM_HIP(hipEventRecord(W.ev[7], s));
@@ -1064,6 +1108,33 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
     const FmmPlan::AdjLists& A = P.adjoint_lists(s);
     if (mfma && W.e2.n < ncs) { W.e2.alloc(ncs); W.m2l_w.alloc(nco); }
     launch_gather_dens(g, P.op.perm.p, nullptr, No, R, W.gsort.p, s);
+    // P2P^T + P2L^T + P2M^T on every source leaf, times the source weights
+    auto leaf_args = [&] {
+        LeafStage f{};
+        f.n_leaves = (int)P.st.leaves.size();
+        f.leaves = P.src_leaves.p;
+        f.tt = view(P.dst); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = Ns;
+        f.rp_ptr = A.leaf_p2p.ptr.p; f.rp_beg = A.leaf_p2p.rbeg.p; f.rp_cnt = A.leaf_p2p.rcnt.p;
+        f.rp_pre = A.leaf_p2p.rpre.p; f.rp_close = A.leaf_p2p.rclose.p;
+        f.ps = view(P.op); f.pdens = W.gsort.p; f.n_ps = No;
+        f.sl_ptr = A.leaf_p2l.ptr.p; f.sl_cell = A.leaf_p2l.cell.p;
+        f.slt = view(P.dot); f.sl_surf = P.d_inner.p; f.sl_coeff = W.cbar.p;
+        f.self_surf = P.d_outer.p; f.self_coeff = W.ebar.p;
+        f.unit = P.d_unit.p; f.n_e = P.n_e; f.blk = P.blk;
+        f.out_sorted = W.dens.p;
+        return f;
+    };
+    const bool split = W.overlap;
+    if (split) {  // P2P^T needs only the gathered g: on the side stream, beside the rest
+        TBEM_HIP(hipEventRecord(W.fork, s));
+        TBEM_HIP(hipStreamWaitEvent(W.side, W.fork, 0));
+        LeafStage f = leaf_args();
+        f.sl_ptr = nullptr;
+        f.sl_cell = nullptr;
+        f.self_coeff = nullptr;
+        launch_leaf_stage<T>(P.kid, true, f, P.kc, W.side, true);
+        TBEM_HIP(hipEventRecord(W.join, W.side));
+    }
     launch_fill(W.lbar.p, nco, 0.0, s);
     launch_fill(W.cbar.p, nco, 0.0, s);
     launch_fill(W.mbar.p, ncs, 0.0, s);
@@ -1136,20 +1207,15 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         launch_surf_stage(P.kid, true, true, true, c, P.kc, s);
     }
 
-    // P2P^T + P2L^T + P2M^T on every source leaf, times the source weights
-    LeafStage f{};
-    f.n_leaves = (int)P.st.leaves.size();
-    f.leaves = P.src_leaves.p;
-    f.tt = view(P.dst); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = Ns;
-    f.rp_ptr = A.leaf_p2p.ptr.p; f.rp_beg = A.leaf_p2p.rbeg.p; f.rp_cnt = A.leaf_p2p.rcnt.p;
-    f.rp_pre = A.leaf_p2p.rpre.p; f.rp_close = A.leaf_p2p.rclose.p;
-    f.ps = view(P.op); f.pdens = W.gsort.p; f.n_ps = No;
-    f.sl_ptr = A.leaf_p2l.ptr.p; f.sl_cell = A.leaf_p2l.cell.p;
-    f.slt = view(P.dot); f.sl_surf = P.d_inner.p; f.sl_coeff = W.cbar.p;
-    f.self_surf = P.d_outer.p; f.self_coeff = W.ebar.p;
-    f.unit = P.d_unit.p; f.n_e = P.n_e; f.blk = P.blk;
-    f.out_sorted = W.dens.p;
-    launch_leaf_stage<T>(P.kid, true, f, P.kc, s, true);
+    if (split) {  // P2L^T + P2M^T (FP64) added to the side stream's P2P^T
+        TBEM_HIP(hipStreamWaitEvent(s, W.join, 0));
+        LeafStage f = leaf_args();
+        f.rp_ptr = nullptr;
+        f.accumulate = 1;
+        launch_leaf_stage<double>(P.kid, true, f, P.kc, s, true);
+    } else {
+        launch_leaf_stage<T>(P.kid, true, leaf_args(), P.kc, s, true);
+    }
     launch_scatter_out(W.dens.p, P.sp.perm.p, Ns, R, xbar, s);
 }
 
@@ -1595,6 +1661,19 @@ int tbem_fmm_stage_times(tbem_fmm_t h, double* ms) {
             TBEM_HIP(hipEventElapsedTime(&t, W.ev[k], W.ev[k + 1]));
             ms[k] = t;
         }
+        if (W.split_last) {  // the leaf stage ran as P2P (side stream) + M2P/L2P
+            float t = 0;
+            TBEM_HIP(hipEventElapsedTime(&t, W.ev[10], W.ev[11]));
+            ms[5] += t;
+        }
+    });
+}
+
+int tbem_fmm_set_overlap(tbem_fmm_t h, int on) {
+    return guard([&] {
+        TBEM_REQUIRE(h, TBEM_E_INVALID, "null handle");
+        std::lock_guard<std::mutex> lk(h->mu);
+        h->work->overlap = on != 0;
     });
 }
 

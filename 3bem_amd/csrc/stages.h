@@ -96,7 +96,8 @@ void launch_level_gemm(const CellTiles& T, int blk, const double* M, bool trans,
 //     (a) the sorted source points in its ranges                (P2P,   or P2P^T)
 //     (b) the surfaces (sl_surf) of the cells in its surface list (M2P,   or P2L^T)
 //     (c) its own surface self_surf with self_coeff             (L2P,   or P2M^T)
-// wscale_i = t_weight[i] when t_weight != nullptr, else 1.
+// wscale_i = t_weight[i] when t_weight != nullptr, else 1; with accumulate the sums are
+// added to out_sorted. A null rp_ptr leaves out (a), null sl_ptr (b), null self_coeff (c).
 struct LeafStage {
     int n_leaves;
     const int* leaves;
@@ -119,6 +120,7 @@ struct LeafStage {
     const double* unit;
     int n_e, blk;
     double* out_sorted;
+    int accumulate;         // out_sorted += (instead of =) the stage's sums
     const double* grad_g;   // obs-gradient mode: upstream g (sorted, component-major)
     double* grad_out;       // obs-gradient mode: 3 per target (sorted, AoS)
 };

@@ -525,7 +525,8 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
             if (h == 1 && nslot + g >= nt) return;
             const int i = b + t0 + h * nslot + g;
             const double w = a.t_weight ? a.t_weight[i] : 1.0;
-            a.out_sorted[(size_t)q * a.n_tgt + i] = w * (s * scale);
+            double* o = a.out_sorted + (size_t)q * a.n_tgt + i;
+            *o = a.accumulate ? *o + w * (s * scale) : w * (s * scale);
         });
         __syncthreads();
     }

@@ -451,9 +451,15 @@ def run_precision(tbem, torch, K, data, cfg, x, prec, steps, warmup, bwd_steps, 
     op = tbem.FMMOperator(K, data, cfg, precision=prec)
     plan = time.perf_counter() - t0
     per = []
-    dt, per = time_steps(lambda: op.apply(x), steps, warmup, torch,
-                         after=lambda: (op.stage_times()["leaf"], op.m2l_kernel_ms()))
-    stages = op.stage_times()
+    # the timed steps (the P2P beside the far field, as the operator runs by default);
+    # then the kernels' own times for the rooflines from fused applies, where each
+    # kernel has the device to itself
+    dt, _ = time_steps(lambda: op.apply(x), steps, warmup, torch)
+    op.set_overlap(False)
+    _, per = time_steps(lambda: op.apply(x), max(3, steps // 2), 1, torch,
+                        after=lambda: (op.stage_times()["leaf"], op.m2l_kernel_ms()))
+    stages = op.stage_times()  # of a fused apply: each stage alone on the device
+    op.set_overlap(True)
     y = op.apply(x)
     info = op.info()
     R = K.dim

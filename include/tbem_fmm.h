@@ -263,6 +263,13 @@ int tbem_fmm_stage_times(tbem_fmm_t h, double* ms7);
  * reference counterpart: a profiling aid for roofline reports. */
 int tbem_fmm_m2l_ms(tbem_fmm_t h, double* ms);
 
+/* on != 0 (the default): the forward's point-to-point pairs (P2P) run on a second
+ * stream of the handle beside the far field, and the surface sources (M2P, L2P) are
+ * added after both; on == 0: one fused leaf launch after the far field, as the
+ * reference orders the stages (cpp/fmm.cpp:550-558). Same results to rounding. No
+ * reference counterpart: it lets a profiler time each kernel on an idle device. */
+int tbem_fmm_set_overlap(tbem_fmm_t h, int on);
+
 /* ---------------------------------------------------------- direct N-body operator */
 
 /* Matrix-free replacement for make_direct_nbody_operator(data, K)
