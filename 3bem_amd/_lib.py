@@ -58,6 +58,7 @@ class FmmInfo(ctypes.Structure):
         ("m2l", _SZ), ("l2l", _SZ), ("l2p", _SZ), ("p2p_pairs", ctypes.c_double),
         ("m2l_classes", ctypes.c_double), ("plan_seconds", ctypes.c_double),
         ("m2l_output_bytes", ctypes.c_double), ("work_bytes", ctypes.c_double),
+        ("leaf_wave_items", ctypes.c_int), ("leaf_chunked", ctypes.c_int), ("leaf_block", ctypes.c_int),
     ]
 
 

@@ -36,6 +36,17 @@ void set_last_error(const std::string& m) { g_last_error = m; }
 // pairs are processed in slices that fit.
 constexpr size_t kM2lOutputBudget = size_t(6) << 30;
 
+// The point-target stage's leaf positions split between the one-wave kernel (leaves of
+// at most kWaveLeafMax points: one work item each, or chunks of a large one) and the
+// one-workgroup kernel (the rest), both ordered by decreasing work (split_leaves).
+struct LeafSplit {
+    DevBuf<WaveItem> items;
+    DevBuf<WaveLeafParts> parts;
+    DevBuf<int> block;
+    int n_items = 0, n_parts = 0, n_block = 0;
+    size_t part_size = 0;  // doubles of chunk sums between the kernels (Work::wpart)
+};
+
 struct FmmPlan {
     int kid = 0, R = 1, precision = TBEM_FP64;
     KConst kc{};
@@ -43,6 +54,7 @@ struct FmmPlan {
     double mac = 0, d = 0;
     size_t order = 0, leaf = 0;
     bool small_cells = false;
+    LeafSplit leaf_split;  // of the observation leaves (forward)
     Surfaces surf;
     int n_e = 0, blk = 0;
     HostTree st, ot;
@@ -69,6 +81,7 @@ struct FmmPlan {
         DevList m2l, m2p;            // items = source cells
         std::vector<DevList> m2m;    // by src level L >= 1: cells at L, entry = parent
         DevList leaf_p2p, leaf_p2l;  // indexed by source leaf position
+        LeafSplit leaf_split;        // of the source leaves
         std::vector<CellTiles> t_slev;  // every cell of each source level
     };
     mutable std::mutex adj_mu;
@@ -145,13 +158,17 @@ struct Work {
     DevBuf<double> e2;      // transpose: modal M2L^T contribution to ebar per source cell
     DevBuf<double> fld;     // raw check potentials of one stage, blk per cell (input of the C2E GEMMs)
     DevBuf<double> ty;      // per-pair output of the M2M / L2L class operators
+    DevBuf<double> wpart;   // chunk sums of the one-wave leaf kernel (LeafSplit::part_size)
     hipEvent_t ev[12];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel; 10, 11
                         // around the point-pair (P2P) launch on the side stream
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
     bool timed = false;
-    // The forward's P2P needs only the gathered densities, so by default it runs on a
-    // side stream beside the far field (P2M ... L2L: MFMA and small launches beside
-    // the P2P's VALU work), joined before the surface sources (M2P, L2P) are added.
+    // The forward's P2P needs only the gathered densities, so it can run on a side
+    // stream beside the far field (P2M ... L2L: MFMA and small launches beside the
+    // P2P's VALU work), joined before the surface sources (M2P, L2P) are added. On by
+    // default where the far field is large (R * n_e >= 48: 2M points, order 100, FP64
+    // 30.0 -> 27.9 ms); at low order the extra surface launch costs more than the
+    // overlap gains (1M points, order 6: 4.8 -> 5.1 ms).
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     bool overlap = true;
@@ -166,6 +183,7 @@ struct Work {
         TBEM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
         TBEM_HIP(hipEventRecord(done, stream));
         TBEM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        overlap = P.blk >= 48;
         TBEM_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
         TBEM_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
         dens.alloc(P.R * P.n_src);
@@ -174,6 +192,7 @@ struct Work {
         out_sorted.alloc(P.R * P.n_obs);
         fld.alloc((size_t)std::max(P.dst.n_cells, P.dot.n_cells) * P.blk);
         ty.alloc(P.tr_y);
+        wpart.alloc(P.leaf_split.part_size);
         if (P.m2l_mfma.enabled && !P.m2l_mfma.fp64) {
             chk_up.alloc((size_t)P.dst.n_cells * P.blk);
             chk_up.zero(stream);  // cells without a P2M/M2M stage keep a zero potential
@@ -645,17 +664,97 @@ struct PhaseClock {
 
 // The point-target stage leaves the r2 < 1e-12 cutoff test out of source tiles whose
 // ranges lie farther than this from the leaf's points (same result; see
-// launch_range_close).
+// launch_range_close). Only for leaves of at least kCloseLeaf points: with small
+// leaves the per-leaf work dominates and the one-loop kernel is faster (1M points,
+// leaf 64: 4.05 against 4.54 ms; 2M points, leaf 256: 19.2 against 19.7 ms).
 constexpr double kCloseMargin = 1e-5;
+constexpr size_t kCloseLeaf = 128;
 
 // l: a point-range list indexed by the leaves `leaves` of tree tt (points tp), over
 // the source points sp: (re)computes its per-range close flags.
 static void close_flags(DevList& l, const int* leaves, const DevTree& tt, const DevPoints& tp, const DevPoints& sp,
-                        hipStream_t s) {
+                        size_t leaf, hipStream_t s) {
+    if (leaf < kCloseLeaf) return;  // no flags: the one-loop kernel
     const size_t n = l.rbeg.n;
     if (l.rclose.n != n) l.rclose.alloc(n);
     if (n) launch_range_close(l.n_items, leaves, view(tt), view(tp), l.ptr.p, l.rbeg.p, l.rcnt.p, view(sp),
                               kCloseMargin, l.rclose.p, s);
+}
+
+// Splits the positions of a point-target list (items = leaves of tree tt; point
+// ranges pts, surface cells surf, and each leaf's own surface): leaves of at most
+// kWaveLeafMax points go to the one-wave kernel, one item each when they have at most
+// kWavePairsMax pairs (points x sources); larger ones of at most kWaveChunkLeaf points
+// are cut into chunks of about equal sources (points cut anywhere, surface cells
+// whole, the own surface with the last chunk), whose sums k_leaf_parts adds in chunk
+// order; the rest go to the one-workgroup kernel. Both lists run the most work first,
+// so that the longest items start first and the short ones fill in behind them.
+// (kWaveChunkLeaf: a wave holds ceil(m/2) targets per source slice, so above 64
+// points one slice is left and down to half of its lanes idle, where the 512-thread
+// kernel keeps 96% busy.)
+constexpr int kWaveChunkLeaf = 64;
+
+static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, const HostList& pts,
+                         const HostList& surf, int n_e, int R, LeafSplit& out, hipStream_t s) {
+    std::vector<std::pair<long, WaveItem>> w;
+    std::vector<std::pair<long, int>> b;
+    std::vector<WaveLeafParts> parts;
+    long part = 0;
+    for (size_t k = 0; k < leaves.size(); k++) {
+        const int pos = (int)k;
+        const long m = tt.count[leaves[k]];
+        if (m == 0) continue;  // no targets, nothing to write
+        const int p0 = pts.ptr.empty() ? 0 : pts.ptr[k], p1 = pts.ptr.empty() ? 0 : pts.ptr[k + 1];
+        const long np = p0 < p1 ? (long)pts.rpre[p1 - 1] + pts.rcnt[p1 - 1] : 0;
+        const int c0 = surf.ptr.empty() ? 0 : surf.ptr[k], c1 = surf.ptr.empty() ? 0 : surf.ptr[k + 1];
+        const long src = np + (long)n_e * (c1 - c0 + 1);
+        const long nc = (m * src + kWavePairsMax - 1) / kWavePairsMax;
+        if (m <= kWaveLeafMax && nc <= 1) {
+            w.push_back({m * src, WaveItem{pos, p0, p1, 0, (int)np, c0, c1, 1, -1}});
+            continue;
+        }
+        if (m > kWaveChunkLeaf || part + (nc + 1) * R * m > INT32_MAX) {
+            b.push_back({m * src, pos});
+            continue;
+        }
+        const long per = (src + nc - 1) / nc;
+        std::vector<std::pair<long, WaveItem>> ch;
+        const auto pre = pts.rpre.begin();
+        for (long v = 0; v < np; v += per) {
+            const int v0 = (int)v, v1 = (int)std::min(np, v + per);
+            const int f0 = (int)(std::upper_bound(pre + p0, pre + p1, v0) - pre) - 1;
+            const int f1 = (int)(std::lower_bound(pre + p0, pre + p1, v1) - pre);
+            ch.push_back({m * (v1 - v0), WaveItem{pos, f0, f1, v0, v1, 0, 0, 0, 0}});
+        }
+        const int cpc = (int)std::max(1L, per / std::max(1, n_e));
+        for (int c = c0; c < c1; c += cpc) {
+            const int ce = std::min(c1, c + cpc);
+            ch.push_back({m * n_e * (ce - c), WaveItem{pos, 0, 0, 0, 0, c, ce, 0, 0}});
+        }
+        if (ch.empty()) ch.push_back({0, WaveItem{pos, 0, 0, 0, 0, 0, 0, 0, 0}});
+        ch.back().second.self = 1;
+        ch.back().first += m * n_e;
+        for (size_t j = 0; j < ch.size(); j++) {
+            ch[j].second.part = (int)(part + (long)j * R * m);
+            w.push_back(ch[j]);
+        }
+        parts.push_back(WaveLeafParts{pos, (int)part, (int)ch.size()});
+        part += (long)ch.size() * R * m;
+    }
+    auto most_first = [](const auto& x, const auto& y) { return x.first > y.first; };
+    std::stable_sort(w.begin(), w.end(), most_first);
+    std::stable_sort(b.begin(), b.end(), most_first);
+    std::vector<WaveItem> wi(w.size());
+    std::vector<int> bl(b.size());
+    for (size_t i = 0; i < w.size(); i++) wi[i] = w[i].second;
+    for (size_t i = 0; i < b.size(); i++) bl[i] = b[i].second;
+    out.n_items = (int)wi.size();
+    out.n_parts = (int)parts.size();
+    out.n_block = (int)bl.size();
+    out.part_size = (size_t)part;
+    out.items.upload(wi, s);
+    out.parts.upload(parts, s);
+    out.block.upload(bl, s);
 }
 
 static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s, const SavedPlan* sv = nullptr) {
@@ -828,9 +927,14 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
     }
     P->obs_leaves.upload(OT.leaves, s);
     P->src_leaves.upload(ST.leaves, s);
-    P->leaf_p2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.p2p, 0, &ST), s);
-    close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, s);
-    P->leaf_m2p.upload(list_from_pairs(OT.leaves, OT.n_cells, tl.m2p, 0, nullptr), s);
+    {
+        const HostList lp = list_from_pairs(OT.leaves, OT.n_cells, tl.p2p, 0, &ST);
+        const HostList lm = list_from_pairs(OT.leaves, OT.n_cells, tl.m2p, 0, nullptr);
+        P->leaf_p2p.upload(lp, s);
+        P->leaf_m2p.upload(lm, s);
+        split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->R, P->leaf_split, s);
+    }
+    close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, P->leaf, s);
 
     clk.mark("rest");
     TBEM_HIP(hipStreamSynchronize(s));
@@ -854,9 +958,14 @@ const FmmPlan::AdjLists& FmmPlan::adjoint_lists(hipStream_t s) const {
     A->m2p.upload(list_from_pairs(cells_with(tl.m2p, 1, ST.n_cells), ST.n_cells, tl.m2p, 1, &OT), s);
     A->m2m.resize(ST.n_levels);
     for (int L = 1; L < ST.n_levels; L++) A->m2m[L].upload(parent_list(ST, L), s);
-    A->leaf_p2p.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, &OT), s);
-    close_flags(A->leaf_p2p, src_leaves.p, dst, sp, op, s);
-    A->leaf_p2l.upload(list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr), s);
+    {
+        const HostList lp = list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, &OT);
+        const HostList ll = list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr);
+        A->leaf_p2p.upload(lp, s);
+        A->leaf_p2l.upload(ll, s);
+        split_leaves(ST, ST.leaves, lp, ll, n_e, R, A->leaf_split, s);
+    }
+    close_flags(A->leaf_p2p, src_leaves.p, dst, sp, op, leaf, s);
     A->t_slev.resize(ST.n_levels);
     for (int L = 0; L < ST.n_levels; L++) A->t_slev[L] = make_level_tiles(ST, L, s);
     TBEM_HIP(hipStreamSynchronize(s));
@@ -986,6 +1095,13 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
     TBEM_HIP(hipEventRecord(W.ev[5], s));
 }
 
+static void set_split(LeafStage& f, const LeafSplit& sp, Work& W) {
+    if (W.wpart.n < sp.part_size) W.wpart.alloc(sp.part_size);
+    f.wave_items = sp.items.p; f.n_wave = sp.n_items;
+    f.wave_parts = sp.parts.p; f.n_parts = sp.n_parts; f.part_buf = W.wpart.p;
+    f.block_pos = sp.block.p; f.n_block = sp.n_block;
+}
+
 // P2P + M2P + L2P per observation leaf (cpp/fmm.cpp:254-294, 332-360, 415-441)
 static LeafStage leaf_stage_args(const FmmPlan& P, Work& W) {
     const int Ns = (int)P.n_src, No = (int)P.n_obs;
@@ -995,6 +1111,7 @@ static LeafStage leaf_stage_args(const FmmPlan& P, Work& W) {
     f.tt = view(P.dot); f.tp = view(P.op); f.t_weight = nullptr; f.n_tgt = No;
     f.rp_ptr = P.leaf_p2p.ptr.p; f.rp_beg = P.leaf_p2p.rbeg.p; f.rp_cnt = P.leaf_p2p.rcnt.p;
     f.rp_pre = P.leaf_p2p.rpre.p; f.rp_close = P.leaf_p2p.rclose.p;
+    set_split(f, P.leaf_split, W);
     f.ps = view(P.sp); f.pdens = W.dens.p; f.n_ps = Ns;
     f.sl_ptr = P.leaf_m2p.ptr.p; f.sl_cell = P.leaf_m2p.cell.p;
     f.slt = view(P.dst); f.sl_surf = P.d_inner.p; f.sl_coeff = W.mult.p;
@@ -1116,6 +1233,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         f.tt = view(P.dst); f.tp = view(P.sp); f.t_weight = P.sp.w.p; f.n_tgt = Ns;
         f.rp_ptr = A.leaf_p2p.ptr.p; f.rp_beg = A.leaf_p2p.rbeg.p; f.rp_cnt = A.leaf_p2p.rcnt.p;
         f.rp_pre = A.leaf_p2p.rpre.p; f.rp_close = A.leaf_p2p.rclose.p;
+        set_split(f, A.leaf_split, W);
         f.ps = view(P.op); f.pdens = W.gsort.p; f.n_ps = No;
         f.sl_ptr = A.leaf_p2l.ptr.p; f.sl_cell = A.leaf_p2l.cell.p;
         f.slt = view(P.dot); f.sl_surf = P.d_inner.p; f.sl_coeff = W.cbar.p;
@@ -1274,10 +1392,10 @@ static void move_obs(FmmPlan& P, const double* d_locs, hipStream_t s) {
                        P.op.y.p, P.op.z.p);
     TBEM_LAUNCH_CHECK();
     // the point-target stages' close flags follow the moved points
-    close_flags(P.leaf_p2p, P.obs_leaves.p, P.dot, P.op, P.sp, s);
+    close_flags(P.leaf_p2p, P.obs_leaves.p, P.dot, P.op, P.sp, P.leaf, s);
     {
         std::lock_guard<std::mutex> lk(P.adj_mu);
-        if (P.adj) close_flags(P.adj->leaf_p2p, P.src_leaves.p, P.dst, P.sp, P.op, s);
+        if (P.adj) close_flags(P.adj->leaf_p2p, P.src_leaves.p, P.dst, P.sp, P.op, P.leaf, s);
     }
     TBEM_HIP(hipStreamSynchronize(s));
     P.obs_moved = true;
@@ -1570,10 +1688,14 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
         const Work& W = *h->work;
         double wb = 0;
         for (const DevBuf<double>* b : {&W.dens, &W.mult, &W.locals, &W.out_sorted, &W.xbuf, &W.ybuf, &W.gsort,
-                                        &W.lbar, &W.cbar, &W.mbar, &W.ebar, &W.grad_sorted, &W.e2, &W.fld, &W.ty})
+                                        &W.lbar, &W.cbar, &W.mbar, &W.ebar, &W.grad_sorted, &W.e2, &W.fld, &W.ty,
+                                        &W.wpart})
             wb += (double)b->n * sizeof(double);
         for (const DevBuf<float>* b : {&W.chk_up, &W.m2l_w}) wb += (double)b->n * sizeof(float);
         o->work_bytes = wb;
+        o->leaf_wave_items = P.leaf_split.n_items;
+        o->leaf_chunked = P.leaf_split.n_parts;
+        o->leaf_block = P.leaf_split.n_block;
     });
 }
 

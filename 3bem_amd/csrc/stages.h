@@ -35,6 +35,23 @@ constexpr int leaf_waves() {
     return leaf_narrow<KID, T>() ? 3 : ((KID == 4 || KID == 5) ? 4 : 1);
 }
 constexpr int kGradBlock = 256;  // observation-coordinate gradient of the point-target stage
+constexpr int kWaveLeafMax = 128;      // leaves of up to this many points go to k_leaf_wave ...
+constexpr long kWavePairsMax = 65536;  // ... one wave per leaf, or per chunk of at most this
+                                       // many pairs (points x sources) of a larger one
+
+// One work item of k_leaf_wave: the leaf at list position pos, whole or one chunk of
+// its sources. Point sources [v0, v1) of the leaf's packed ranges (offsets into their
+// concatenation; ranges f0 .. f1-1 hold them), surface cells [c0, c1) of its surface
+// list, and its own surface when self != 0. part < 0: the item is the whole leaf and
+// writes the output; else it writes its unscaled sums, R x m values at
+// part_buf[part + q*m + t], and k_leaf_parts adds the chunks of the leaf in order.
+struct WaveItem {
+    int pos, f0, f1, v0, v1, c0, c1, self, part;
+};
+// A leaf cut into n chunks: chunk c wrote part_buf[part + c*R*m ...].
+struct WaveLeafParts {
+    int pos, part, n;
+};
 
 struct TreeView {
     const double *cx, *cy, *cz, *level_r;
@@ -121,6 +138,19 @@ struct LeafStage {
     int n_e, blk;
     double* out_sorted;
     int accumulate;         // out_sorted += (instead of =) the stage's sums
+    // The leaf positions a launch serves (null: 0 .. n_leaves-1, one workgroup each).
+    const int* order;
+    int n_order;
+    // Optional split of the positions (launch_leaf_stage): wave_items, leaves of at most
+    // kWaveLeafMax points, whole or in chunks, one wave each, the most work first;
+    // block_pos, the rest, one workgroup each. wave_parts: the leaves cut into chunks,
+    // whose partial sums part_buf holds between the two kernels.
+    const WaveItem* wave_items;
+    const int* block_pos;
+    int n_wave, n_block;
+    const WaveLeafParts* wave_parts;
+    int n_parts;
+    double* part_buf;
     const double* grad_g;   // obs-gradient mode: upstream g (sorted, component-major)
     double* grad_out;       // obs-gradient mode: 3 per target (sorted, AoS)
 };

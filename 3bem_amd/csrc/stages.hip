@@ -62,6 +62,7 @@ struct StageSmem {
         double red[RED * NB];
     };
     int rpre[NB + 1], rbeg[NB];
+    int wflag[NB / 64];  // stream_ranges_packed: per wave, a close point in the tile
     double part[NB];  // group sums of slice_sum
 };
 
@@ -235,8 +236,9 @@ __device__ inline void stream_ranges(Tile<VT, R, NRM, NB>& tl, int e0, int e1,
 // kernels' cutoff radius of the targets. With tile_close given, *tile_close is set
 // before each compute() to whether any point of the tile comes from such a range
 // (1 for every tile when rclose is null), so that compute() can leave the cutoff
-// test out of the other tiles. The flag travels in bit 30 of the cached range
-// starts (point indices are below 2^30).
+// test out of the other tiles: a ballot per wave into s_wflag (NB/64 ints of LDS)
+// before the tile's second barrier, read back after it. The flag travels in bit 30
+// of the cached range starts (point indices are below 2^30).
 constexpr int kCloseBit = 1 << 30;
 
 template <class VT, int R, bool NRM, int NB, class F>
@@ -244,7 +246,8 @@ __device__ inline void stream_ranges_packed(Tile<VT, R, NRM, NB>& tl, int* s_pre
                                             const int* __restrict__ rbeg, const int* __restrict__ rcnt,
                                             const int* __restrict__ rpre, const PtsView& sp,
                                             const double* __restrict__ dens, int n_src, F&& compute,
-                                            const int* __restrict__ rclose = nullptr, int* tile_close = nullptr) {
+                                            const int* __restrict__ rclose = nullptr, int* tile_close = nullptr,
+                                            int* s_wflag = nullptr) {
     const int tid = threadIdx.x;
     if (e0 >= e1) return;
     const int total = uni(rpre[e1 - 1] + rcnt[e1 - 1]);
@@ -282,8 +285,16 @@ __device__ inline void stream_ranges_packed(Tile<VT, R, NRM, NB>& tl, int* s_pre
         const int cnt = min(NB, total - base);
         __syncthreads();
         if (tid < cnt) store_tile(tl, tid, r);
-        if (tile_close) *tile_close = __syncthreads_or(tid < cnt && mclose);
-        else __syncthreads();
+        if (tile_close) {
+            const unsigned long long bal = __ballot(tid < cnt && mclose);
+            if ((tid & 63) == 0) s_wflag[tid >> 6] = bal != 0ull;
+        }
+        __syncthreads();
+        if (tile_close) {
+            int any = 0;
+            for (int w = 0; w < NB / 64; w++) any |= s_wflag[w];
+            *tile_close = any;
+        }
         base += NB;
         const bool more = base < total;
         if (more) {
@@ -435,15 +446,19 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
 // workgroup, <= 128 VGPRs keeps two workgroups per CU; one more register halves
 // the resident waves (the elastic-U FP32 instance at 139 VGPRs ran 1.5x slower),
 // so leaf_waves() makes the compiler hold that budget (leaf_block() for the rest).
-template <int KID, class T, bool ADJ, bool FAST>
+// CLOSE (FP64 only): tiles without close points (LeafStage.rp_close) skip the cutoff
+// test; the instances without it have one pair loop and fewer registers, which pays
+// where leaves are small and per-leaf work dominates.
+template <int KID, class T, bool ADJ, bool FAST, bool CLOSE = false>
 __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) void k_leaf_stage(LeafStage a, KC<T> kc, KC<double> kc64,
                                                                   double scale) {
+    static_assert(!CLOSE || std::is_same<T, double>::value, "cutoff-free tiles: FP64 only");
     constexpr int NB = leaf_block<KID, T>();
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
     constexpr bool SPLIT = !std::is_same<T, double>::value;
     __shared__ StageSmem<T, R, NRM, NB, 2 * R> sm;
-    const int pos = blockIdx.x;
+    const int pos = a.order ? uni(a.order[blockIdx.x]) : (int)blockIdx.x;
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
     const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
@@ -504,10 +519,9 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
                 }
             }
         };
-        // the cutoff-free tiles pay in FP64 (leaf 19.7 -> 19.2 ms at 2M points, order 100);
-        // the FP32 instance ran 7% slower with the second loop, so it keeps the one
         stream_ranges_packed(sm.pt, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
-                             a.n_ps, points, SPLIT ? nullptr : a.rp_close, SPLIT ? nullptr : &tclose);  // (a)
+                             a.n_ps, points, CLOSE ? a.rp_close : nullptr, CLOSE ? &tclose : nullptr,
+                             sm.wflag);                                                              // (a)
         stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk,
                         surfaces);                                                                // (b)
         if (a.self_coeff)
@@ -529,6 +543,194 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
             *o = a.accumulate ? *o + w * (s * scale) : w * (s * scale);
         });
         __syncthreads();
+    }
+}
+
+// ----------------------------------------------------- point targets, small leaves
+// Leaves of at most kWaveLeafMax points (every leaf at the reference's leaf sizes;
+// the plan sends larger ones to k_leaf_stage): one wave64 per work item, a whole
+// leaf or, for a leaf of more than kWavePairsMax pairs, one chunk of its sources
+// (WaveItem), the items ordered by decreasing work so that the longest waves start
+// first; kWaveBlock / 64 items per workgroup, each wave streaming its own sources
+// through its own 64-entry LDS tiles. There is no workgroup barrier and no
+// workgroup-wide slice sum, and a CU holds 16 leaves in flight instead of the 512-
+// thread kernel's 2, which is what hides a small leaf's chain of dependent loads
+// (targets, lists, tiles, surfaces, output): on the 1M-point leaf-64 workload the
+// 512-thread kernel spends more than half its time outside the pair loop. The
+// arithmetic is the 512-thread kernel's: two targets per lane, S = 64 / ceil(m/2)
+// source slices, pair() on FP64 coordinates, FP32 tiles folded into compensated
+// sums per tile, FP64 surfaces; slices are summed in order at the end. Cutting the
+// largest leaves into chunks (k_leaf_parts adds them, in chunk order) keeps one leaf
+// from setting the launch's length: on the 1M-point leaf-64 workload a 3-point leaf
+// has 674k point sources, and 20 leaves have 1-2.4M pairs each.
+constexpr int kWaveBlock = 256;
+
+template <class PT, int R, bool NRM>
+struct WaveSmem {
+    union {
+        Tile<PT, R, NRM, 64> pt;
+        Tile<double, R, NRM, 64> st;
+        double red[2 * R * 64];
+    };
+};
+
+// Orders one wave's LDS writes before its later reads (and reads before later
+// writes): the lanes of a wave run in lockstep, so only the compiler's ordering is
+// at stake.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int KID, class T, bool ADJ, bool FAST>
+__global__ __launch_bounds__(kWaveBlock) void k_leaf_wave(LeafStage a, KC<T> kc, KC<double> kc64, double scale) {
+    constexpr int R = kernel_dim(KID);
+    constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
+    __shared__ WaveSmem<T, R, NRM> smw[kWaveBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int idx = blockIdx.x * (kWaveBlock / 64) + w;
+    if (idx >= a.n_wave) return;  // the whole wave: nothing below synchronizes the block
+    const WaveItem* it = a.wave_items + idx;
+    const int pos = uni(it->pos);
+    WaveSmem<T, R, NRM>& sm = smw[w];
+    const int leaf = uni(a.leaves[pos]);
+    const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);  // m <= kWaveLeafMax
+    const int nslot = (m + 1) >> 1, S = 64 / nslot;
+    const int tg = lane % nslot, slice = lane / nslot;
+    const bool va = slice < S, vb = va && nslot + tg < m;
+    const int ia = b + tg, ib = b + (vb ? nslot + tg : tg);
+    const double ax = a.tp.x[ia], ay = a.tp.y[ia], az = a.tp.z[ia];
+    const double bx = a.tp.x[ib], by = a.tp.y[ib], bz = a.tp.z[ib];
+    const double an64[3] = {NRM ? a.tp.nx[ia] : 0.0, NRM ? a.tp.ny[ia] : 0.0, NRM ? a.tp.nz[ia] : 0.0};
+    const double bn64[3] = {NRM ? a.tp.nx[ib] : 0.0, NRM ? a.tp.ny[ib] : 0.0, NRM ? a.tp.nz[ib] : 0.0};
+    const T an[3] = {T(an64[0]), T(an64[1]), T(an64[2])};
+    const T bn[3] = {T(bn64[0]), T(bn64[1]), T(bn64[2])};
+    const int f0 = uni(it->f0), f1 = uni(it->f1), v0 = uni(it->v0), v1 = uni(it->v1);
+    const int se0 = uni(it->c0), se1 = uni(it->c1), self = uni(it->self), part = uni(it->part);
+    drain_loads();
+    Acc<T, R> acA, acB;
+    acA.init();
+    acB.init();
+    double sfA[SPLIT ? R : 1], sfB[SPLIT ? R : 1];
+    for (int q = 0; q < (SPLIT ? R : 1); q++) sfA[q] = sfB[q] = 0.0;
+
+    // (a) point sources: [v0, v1) of the leaf's packed ranges, 64 per tile, the next
+    // tile in registers while the current one is computed; each lane walks the
+    // ranges f0 .. f1-1 forward (a null rp_ptr, sl_ptr or self_coeff leaves out (a),
+    // (b) or (c) as in k_leaf_stage; a chunk then sums what is left of it)
+    if (a.rp_ptr && v0 < v1) {
+        const int total = v1 - v0;
+        SrcReg<R, NRM> r;
+        int f = f0;
+        auto fetch = [&](int base) {
+            const int v = v0 + base + lane;
+            if (v < v1) {
+                while (f + 1 < f1 && a.rp_pre[f + 1] <= v) f++;
+                load_point(r, a.rp_beg[f] + (v - a.rp_pre[f]), a.ps, a.pdens, a.n_ps);
+            }
+        };
+        fetch(0);
+        for (int base = 0; base < total; base += 64) {
+            const int cnt = min(64, total - base);
+            wave_sync();
+            if (lane < cnt) store_tile(sm.pt, lane, r);
+            wave_sync();
+            if (base + 64 < total) fetch(base + 64);
+            if (va) {
+                for (int j = slice; j < cnt; j += S) {
+                    pair<KID, ADJ, FAST>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                    pair<KID, ADJ, FAST>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                }
+            }
+            acA.flush();
+            acB.flush();
+        }
+    }
+    // (b), (c) surface sources, FP64: n_e points per cell of the list, then the own surface
+    auto surfaces = [&](const int* cells, int c0, int c1, const TreeView& st, const double* surf,
+                        const double* coeff) {
+        const int total = (c1 - c0) * a.n_e;
+        if (total <= 0) return;
+        SrcReg<R, NRM> r;
+        auto fetch = [&](int base) {
+            const int v = base + lane;
+            if (v < total) {
+                const int k = v / a.n_e;
+                load_surf(r, cells[c0 + k], v - k * a.n_e, st, surf, a.unit, coeff, a.n_e, a.blk);
+            }
+        };
+        double *sa, *sb;
+        if constexpr (SPLIT) {
+            sa = sfA;
+            sb = sfB;
+        } else {
+            sa = acA.tile();
+            sb = acB.tile();
+        }
+        fetch(0);
+        for (int base = 0; base < total; base += 64) {
+            const int cnt = min(64, total - base);
+            wave_sync();
+            if (lane < cnt) store_tile(sm.st, lane, r);
+            wave_sync();
+            if (base + 64 < total) fetch(base + 64);
+            if (va) {
+                for (int j = slice; j < cnt; j += S) {
+                    pair<KID, ADJ>(sm.st, j, ax, ay, az, an64, sa, kc64);
+                    pair<KID, ADJ>(sm.st, j, bx, by, bz, bn64, sb, kc64);
+                }
+            }
+        }
+    };
+    if (a.sl_ptr && se0 < se1) surfaces(a.sl_cell, se0, se1, a.slt, a.sl_surf, a.sl_coeff);
+    if (self && a.self_coeff) surfaces(&leaf, 0, 1, a.tt, a.self_surf, a.self_coeff);
+
+    // the S slices of each target, summed in slice order: the output, or the chunk's
+    // partial sums
+    wave_sync();
+    for (int q = 0; q < R; q++) {
+        const double ea = SPLIT ? sfA[q] : 0.0, eb = SPLIT ? sfB[q] : 0.0;
+        sm.red[q * 64 + lane] = va ? acA.value(q) + ea : 0.0;
+        sm.red[(R + q) * 64 + lane] = vb ? acB.value(q) + eb : 0.0;
+    }
+    wave_sync();
+    if (lane < nslot) {
+        for (int h = 0; h < 2; h++) {
+            if (h == 1 && nslot + lane >= m) break;
+            const int t = h * nslot + lane, i = b + t;
+            const double wgt = (part < 0 && a.t_weight) ? a.t_weight[i] : 1.0;
+            for (int q = 0; q < R; q++) {
+                double s = 0;
+                for (int k = 0; k < S; k++) s += sm.red[(h * R + q) * 64 + k * nslot + lane];
+                if (part >= 0) {
+                    a.part_buf[(size_t)part + q * m + t] = s;
+                    continue;
+                }
+                double* o = a.out_sorted + (size_t)q * a.n_tgt + i;
+                *o = a.accumulate ? *o + wgt * (s * scale) : wgt * (s * scale);
+            }
+        }
+    }
+}
+
+// The leaves k_leaf_wave ran in chunks: one workgroup per leaf, one target per thread
+// (m <= kWaveLeafMax), the chunks' partial sums added in chunk order.
+__global__ __launch_bounds__(kWaveLeafMax) void k_leaf_parts(LeafStage a, int R, double scale) {
+    const WaveLeafParts lp = a.wave_parts[blockIdx.x];
+    const int leaf = a.leaves[lp.pos];
+    const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
+    const int t = threadIdx.x;
+    if (t >= m) return;
+    const int i = b + t;
+    const double wgt = a.t_weight ? a.t_weight[i] : 1.0;
+    for (int q = 0; q < R; q++) {
+        const double* p = a.part_buf + (size_t)lp.part + q * m + t;
+        double s = 0;
+        for (int c = 0; c < lp.n; c++) s += p[(size_t)c * R * m];
+        double* o = a.out_sorted + (size_t)q * a.n_tgt + i;
+        *o = a.accumulate ? *o + wgt * (s * scale) : wgt * (s * scale);
     }
 }
 
@@ -960,9 +1162,41 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
     const KC<T> kc = kc_of<T>(c);
     const KC<double> kc64 = make_kc64(c);
     const double scale = (double)kop_scale<KID, double>(kc64);
-    dim3 g(a.n_leaves), b(leaf_block<KID, T>());
+    if (a.wave_items || a.block_pos) {  // small leaves one wave per item, the rest one workgroup each
+        if (a.n_wave > 0) {
+            const dim3 g((a.n_wave + kWaveBlock / 64 - 1) / (kWaveBlock / 64)), b(kWaveBlock);
+            const bool f64 = fast && std::is_same<T, double>::value;
+            if (adj) {
+                if (f64) hipLaunchKernelGGL((k_leaf_wave<KID, T, true, true>), g, b, 0, s, a, kc, kc64, scale);
+                else hipLaunchKernelGGL((k_leaf_wave<KID, T, true, false>), g, b, 0, s, a, kc, kc64, scale);
+            } else {
+                if (f64) hipLaunchKernelGGL((k_leaf_wave<KID, T, false, true>), g, b, 0, s, a, kc, kc64, scale);
+                else hipLaunchKernelGGL((k_leaf_wave<KID, T, false, false>), g, b, 0, s, a, kc, kc64, scale);
+            }
+            TBEM_LAUNCH_CHECK();
+            if (a.n_parts > 0) {
+                hipLaunchKernelGGL(k_leaf_parts, dim3(a.n_parts), dim3(kWaveLeafMax), 0, s, a,
+                                   kernel_dim(KID), scale);
+                TBEM_LAUNCH_CHECK();
+            }
+        }
+        if (a.n_block == 0) return;
+        LeafStage bl = a;
+        bl.order = a.block_pos;
+        bl.n_order = a.n_block;
+        bl.wave_items = nullptr;
+        bl.block_pos = nullptr;
+        bl.n_wave = bl.n_parts = 0;
+        leaf_launch<KID, T>(adj, fast, bl, c, s);
+        return;
+    }
+    dim3 g(a.order ? a.n_order : a.n_leaves), b(leaf_block<KID, T>());
     // the one-step rsqrt only differs from the full one in FP64
-    if (fast && std::is_same<T, double>::value) {
+    if (fast && std::is_same<T, double>::value && a.rp_close && a.rp_ptr) {
+        constexpr bool C = std::is_same<T, double>::value;  // (always true here)
+        if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, true, C>), g, b, 0, s, a, kc, kc64, scale);
+        else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, true, C>), g, b, 0, s, a, kc, kc64, scale);
+    } else if (fast && std::is_same<T, double>::value) {
         if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, true>), g, b, 0, s, a, kc, kc64, scale);
         else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, true>), g, b, 0, s, a, kc, kc64, scale);
     } else {

@@ -150,6 +150,11 @@ typedef struct {
                                 take turns on it */
     double work_bytes;       /* device buffers of this handle's own (each clone has its own;
                                 the adjoint and gradient add theirs on first use) */
+    /* How the forward point-target stage (P2P + M2P + L2P) runs its observation leaves:
+       leaf_wave_items one-wave work items (whole small leaves and chunks), of which the
+       chunks of leaf_chunked leaves too large for one wave; leaf_block leaves on the
+       one-workgroup kernel (more than 64 points with more pairs than one wave takes). */
+    int leaf_wave_items, leaf_chunked, leaf_block;
 } tbem_fmm_info_t;
 
 /* ---------------------------------------------------------------- FMM operator */
@@ -263,11 +268,13 @@ int tbem_fmm_stage_times(tbem_fmm_t h, double* ms7);
  * reference counterpart: a profiling aid for roofline reports. */
 int tbem_fmm_m2l_ms(tbem_fmm_t h, double* ms);
 
-/* on != 0 (the default): the forward's point-to-point pairs (P2P) run on a second
+/* on != 0: the point-to-point pairs (P2P, and the transpose's P2P^T) run on a second
  * stream of the handle beside the far field, and the surface sources (M2P, L2P) are
  * added after both; on == 0: one fused leaf launch after the far field, as the
- * reference orders the stages (cpp/fmm.cpp:550-558). Same results to rounding. No
- * reference counterpart: it lets a profiler time each kernel on an idle device. */
+ * reference orders the stages (cpp/fmm.cpp:550-558). Same results to rounding. The
+ * default is on where the operator size R * n_e is at least 48 (a large far field)
+ * and off below. No reference counterpart: it also lets a profiler time each kernel
+ * on an idle device. */
 int tbem_fmm_set_overlap(tbem_fmm_t h, int on);
 
 /* ---------------------------------------------------------- direct N-body operator */

@@ -494,14 +494,15 @@ def straddling_pairs(n, m, seed):
 @pytest.mark.parametrize("precision,kid", [("fp64", 3), ("fp64", 0), ("mixed", 3), ("fp64", 6)])
 def test_cutoff_holds_across_leaf_boundaries(tbem, precision, kid):
     """Source tiles whose ranges lie farther than 1e-5 from a leaf's points skip the
-    r2 < 1e-12 test (same result). Pairs 2e-7 apart across a leaf boundary must still
-    get it: the FMM equals the direct operator (which tests every pair) on points
-    that hold 200 such pairs, forward and transpose."""
+    r2 < 1e-12 test (same result; FP64, leaves of 128 points and more). Pairs 2e-7
+    apart across a leaf boundary must still get it: the FMM equals the direct operator
+    (which tests every pair) on points that hold 200 such pairs, forward and
+    transpose."""
     case("cutoff across leaves", precision, kid)
     locs, nrm, w = straddling_pairs(6000, 200, 41)
     K = tbem.make_kernel(kid, 1.0, 0.25)
     data = tbem.NBodyData(locs, nrm, locs, nrm, w)
-    op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, 6, 32, 0.05, False), precision)
+    op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, 6, 128, 0.05, False), precision)
     direct = tbem.make_direct_nbody_operator(data, K)
     rng = np.random.default_rng(3)
     x = rng.standard_normal(K.dim * len(locs))
@@ -515,7 +516,7 @@ def test_cutoff_holds_across_leaf_boundaries(tbem, precision, kid):
     far = locs.copy()
     far[-200:, 0] += 1e-3  # the partners leave the cutoff radius: huge but finite pairs appear
     data2 = tbem.NBodyData(far, nrm, far, nrm, w)
-    y2 = tbem.FMMOperator(K, data2, tbem.FMMConfig(0.3, 6, 32, 0.05, False), precision).apply(x)
+    y2 = tbem.FMMOperator(K, data2, tbem.FMMConfig(0.3, 6, 128, 0.05, False), precision).apply(x)
     assert rel_err(y2, tbem.make_direct_nbody_operator(data2, K).apply(x)) < tol
 
 
@@ -528,7 +529,7 @@ def test_cutoff_flags_follow_moved_observation_points(tbem):
     obs = locs.copy()
     obs[-100:, 0] = 0.5 + 3e-6  # the upper partners start 3e-6 from the boundary
     K = tbem.make_kernel(3, 1.0, 0.25)
-    cfg = tbem.FMMConfig(0.3, 6, 32, 0.05, False)
+    cfg = tbem.FMMConfig(0.3, 6, 128, 0.05, False)
     op = tbem.FMMOperator(K, tbem.NBodyData(obs, nrm, locs, nrm, w), cfg)
     x = rng.standard_normal(3 * len(locs))
     moved = obs.copy()
@@ -556,6 +557,7 @@ def test_overlapped_p2p_matches_fused_leaf(tbem, precision, kid):
     op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, 20, 64, 0.05, False), precision)
     x = np.random.default_rng(2).standard_normal(K.dim * data.n_src)
     g_ = np.random.default_rng(3).standard_normal(K.dim * data.n_obs)
+    op.set_overlap(True)  # the default only from R * n_e >= 48
     y1, t1 = op.apply(x), op.apply_adjoint(g_)
     assert np.array_equal(op.apply(x), y1) and np.array_equal(op.apply_adjoint(g_), t1)
     op.set_overlap(False)
@@ -568,3 +570,49 @@ def test_overlapped_p2p_matches_fused_leaf(tbem, precision, kid):
     print(f"  overlapped vs fused: forward {e:.2e}, transpose {ea:.2e}, gradient {eg:.2e}")
     tol = 1e-13 if precision == "fp64" else 1e-6
     assert e < tol and ea < tol and eg < tol
+
+
+def cluster_and_sparse(seed, n_dense=16_000, n_sparse=400):
+    """A dense cube of points inside a sparse cloud: leaves of the cube see ~12k point
+    sources, and sparse leaves beside it few targets and many sources."""
+    rng = np.random.default_rng(seed)
+    locs = np.r_[rng.uniform(0.45, 0.55, (n_dense, 3)), rng.uniform(0, 1, (n_sparse, 3))]
+    nrm = rng.standard_normal(locs.shape)
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    return dict(obs=locs, obs_normals=nrm, src=locs, src_normals=nrm, weights=rng.uniform(0.5, 1.5, len(locs)))
+
+
+@pytest.mark.parametrize("small", [False, True])
+@pytest.mark.parametrize("kid", [0, 3, 6])
+def test_gpu_chunked_leaves_match_oracle(tbem, orc, kid, small):
+    """Leaves of at most 64 points with more than 65,536 point pairs run on the
+    one-wave kernel in chunks of their sources (up to 12 here, with the M2P surfaces
+    in chunks of their own when small_cells is set), whose sums a second kernel adds
+    in chunk order. FP64 against the oracle at the parity bar, the transpose by the
+    adjoint identity, each run deterministic, and the mixed path near FP64."""
+    case("chunked leaves", kid, small)
+    g = cluster_and_sparse(11 + kid)
+    R = components(kid)
+    n = len(g["obs"])
+    rng = np.random.default_rng(kid)
+    x, gg = rng.standard_normal(R * n), rng.standard_normal(R * n)
+    d = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
+    cfg = tbem.FMMConfig(0.3, 6, 64, 0.05, small)
+    op = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, cfg)
+    inf = op.info()
+    print(f"  wave items {inf['leaf_wave_items']}, chunked leaves {inf['leaf_chunked']}, "
+          f"block leaves {inf['leaf_block']}")
+    assert inf["leaf_chunked"] > 100 and inf["leaf_block"] == 0
+    y = op.apply(x)
+    assert np.array_equal(op.apply(x), y)
+    ref = orc.fmm(kid, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.3, 6, 64, 0.05,
+                  small).apply(x)
+    e = rel_err(y, ref)
+    xt = op.apply_adjoint(gg)
+    assert np.array_equal(op.apply_adjoint(gg), xt)
+    scale = np.abs(y).max() * np.abs(gg).sum() + np.abs(x).max() * np.abs(xt).sum()
+    ea = abs(y @ gg - x @ xt) / scale
+    mixed = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, cfg, "mixed")
+    em = rel_err(mixed.apply(x), y)
+    print(f"  vs oracle {e:.2e}, adjoint identity {ea:.2e}, mixed vs fp64 {em:.2e}")
+    assert e < FP64_TOL and ea < 1e-13 and em < 1e-5
