@@ -43,7 +43,8 @@ struct LeafSplit {
     DevBuf<WaveItem> items;
     DevBuf<WaveLeafParts> parts;
     DevBuf<int> block;
-    int n_items = 0, n_parts = 0, n_block = 0;
+    int n_items = 0, n_parts = 0, n_block = 0;  // entries of items, parts, block (with padding)
+    int n_work = 0, n_block_leaves = 0;         // work items and workgroup leaves without it
     size_t part_size = 0;  // doubles of chunk sums between the kernels (Work::wpart)
 };
 
@@ -687,12 +688,44 @@ static void close_flags(DevList& l, const int* leaves, const DevTree& tt, const 
 // kWavePairsMax pairs (points x sources); larger ones of at most kWaveChunkLeaf points
 // are cut into chunks of about equal sources (points cut anywhere, surface cells
 // whole, the own surface with the last chunk), whose sums k_leaf_parts adds in chunk
-// order; the rest go to the one-workgroup kernel. Both lists run the most work first,
-// so that the longest items start first and the short ones fill in behind them.
+// order; the rest go to the one-workgroup kernel. Both lists are laid out by
+// xcd_layout: per XCD a run of neighbouring leaves, the most work first.
 // (kWaveChunkLeaf: a wave holds ceil(m/2) targets per source slice, so above 64
 // points one slice is left and down to half of its lanes idle, where the 512-thread
 // kernel keeps 96% busy.)
 constexpr int kWaveChunkLeaf = 64;
+
+// Lays out work items (work, item), given in tree order, for a launch whose workgroup
+// b runs on XCD b % 8 (the hardware deals workgroups to the XCDs in turn), g items per
+// workgroup. The items are cut into 8 runs of about equal work, one per XCD, so that
+// neighbouring leaves, which share most of their sources, meet in one XCD's L2. Within
+// a run the items go by work in power-of-two classes, largest first (so that the
+// longest start first), tree order within a class; workgroup b takes group b / 8 of
+// run b % 8, and the shorter runs are padded with `empty`.
+template <class I>
+static std::vector<I> xcd_layout(const std::vector<std::pair<long, I>>& items, int g, const I& empty) {
+    constexpr int kXcds = 8;
+    long total = 0;
+    for (const auto& it: items) total += it.first;
+    std::vector<std::vector<std::pair<int, I>>> run(kXcds);
+    long acc = 0;
+    for (const auto& it: items) {
+        const int x = total > 0 ? (int)std::min<long>(kXcds - 1, (acc * kXcds) / total) : 0;
+        int cls = 0;
+        for (long w = it.first; w > 1; w >>= 1) cls++;
+        run[x].push_back({cls, it.second});
+        acc += it.first;
+    }
+    size_t groups = 0;
+    for (auto& r: run) {
+        std::stable_sort(r.begin(), r.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        groups = std::max(groups, (r.size() + g - 1) / g);
+    }
+    std::vector<I> out(groups * kXcds * g, empty);
+    for (int x = 0; x < kXcds; x++)
+        for (size_t i = 0; i < run[x].size(); i++) out[((i / g) * kXcds + x) * g + i % g] = run[x][i].second;
+    return out;
+}
 
 static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, const HostList& pts,
                          const HostList& surf, int n_e, int R, LeafSplit& out, hipStream_t s) {
@@ -741,13 +774,11 @@ static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, con
         parts.push_back(WaveLeafParts{pos, (int)part, (int)ch.size()});
         part += (long)ch.size() * R * m;
     }
-    auto most_first = [](const auto& x, const auto& y) { return x.first > y.first; };
-    std::stable_sort(w.begin(), w.end(), most_first);
-    std::stable_sort(b.begin(), b.end(), most_first);
-    std::vector<WaveItem> wi(w.size());
-    std::vector<int> bl(b.size());
-    for (size_t i = 0; i < w.size(); i++) wi[i] = w[i].second;
-    for (size_t i = 0; i < b.size(); i++) bl[i] = b[i].second;
+    const std::vector<WaveItem> wi =
+        xcd_layout(w, kWaveBlock / 64, WaveItem{-1, 0, 0, 0, 0, 0, 0, 0, -1});
+    const std::vector<int> bl = xcd_layout(b, 1, -1);
+    out.n_work = (int)w.size();
+    out.n_block_leaves = (int)b.size();
     out.n_items = (int)wi.size();
     out.n_parts = (int)parts.size();
     out.n_block = (int)bl.size();
@@ -1693,9 +1724,9 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
             wb += (double)b->n * sizeof(double);
         for (const DevBuf<float>* b : {&W.chk_up, &W.m2l_w}) wb += (double)b->n * sizeof(float);
         o->work_bytes = wb;
-        o->leaf_wave_items = P.leaf_split.n_items;
+        o->leaf_wave_items = P.leaf_split.n_work;
         o->leaf_chunked = P.leaf_split.n_parts;
-        o->leaf_block = P.leaf_split.n_block;
+        o->leaf_block = P.leaf_split.n_block_leaves;
     });
 }
 

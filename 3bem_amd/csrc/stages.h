@@ -35,6 +35,7 @@ constexpr int leaf_waves() {
     return leaf_narrow<KID, T>() ? 3 : ((KID == 4 || KID == 5) ? 4 : 1);
 }
 constexpr int kGradBlock = 256;  // observation-coordinate gradient of the point-target stage
+constexpr int kWaveBlock = 256;        // k_leaf_wave: four work items per workgroup
 constexpr int kWaveLeafMax = 128;      // leaves of up to this many points go to k_leaf_wave ...
 constexpr long kWavePairsMax = 65536;  // ... one wave per leaf, or per chunk of at most this
                                        // many pairs (points x sources) of a larger one

@@ -459,6 +459,7 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
     constexpr bool SPLIT = !std::is_same<T, double>::value;
     __shared__ StageSmem<T, R, NRM, NB, 2 * R> sm;
     const int pos = a.order ? uni(a.order[blockIdx.x]) : (int)blockIdx.x;
+    if (pos < 0) return;  // padding of the layout (the whole workgroup)
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
     const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
@@ -550,8 +551,8 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
 // Leaves of at most kWaveLeafMax points (every leaf at the reference's leaf sizes;
 // the plan sends larger ones to k_leaf_stage): one wave64 per work item, a whole
 // leaf or, for a leaf of more than kWavePairsMax pairs, one chunk of its sources
-// (WaveItem), the items ordered by decreasing work so that the longest waves start
-// first; kWaveBlock / 64 items per workgroup, each wave streaming its own sources
+// (WaveItem), laid out per XCD in runs of neighbouring leaves, the most work first
+// (padding items have pos < 0); kWaveBlock / 64 items per workgroup, each wave streaming its own sources
 // through its own 64-entry LDS tiles. There is no workgroup barrier and no
 // workgroup-wide slice sum, and a CU holds 16 leaves in flight instead of the 512-
 // thread kernel's 2, which is what hides a small leaf's chain of dependent loads
@@ -563,7 +564,6 @@ __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) voi
 // largest leaves into chunks (k_leaf_parts adds them, in chunk order) keeps one leaf
 // from setting the launch's length: on the 1M-point leaf-64 workload a 3-point leaf
 // has 674k point sources, and 20 leaves have 1-2.4M pairs each.
-constexpr int kWaveBlock = 256;
 
 template <class PT, int R, bool NRM>
 struct WaveSmem {
@@ -594,6 +594,7 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_wave(LeafStage a, KC<T> kc,
     if (idx >= a.n_wave) return;  // the whole wave: nothing below synchronizes the block
     const WaveItem* it = a.wave_items + idx;
     const int pos = uni(it->pos);
+    if (pos < 0) return;  // padding of the layout
     WaveSmem<T, R, NRM>& sm = smw[w];
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);  // m <= kWaveLeafMax

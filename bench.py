@@ -404,13 +404,21 @@ def fmm_rooflines(info, n, kid, prec, leaf_ms, m2l_ms, wname=None):
     out = {}
     flop = FLOPS_PER_PAIR[kid] * (info["p2p_pairs"] + n * n_e)
     ach = flop / (leaf_ms * 1e-3) / 1e12
+    # the leaf stage's kernels: one wave per small leaf or chunk of a large one (k_leaf_wave,
+    # k_leaf_parts adding the chunks), one workgroup per remaining leaf (k_leaf_stage)
+    t = "double" if prec == "fp64" else "float"
+    parts = ([f"k_leaf_wave<{t}>"] if info["leaf_wave_items"] else []) + \
+        (["k_leaf_parts"] if info["leaf_chunked"] else []) + ([f"k_leaf_stage<{t}>"] if info["leaf_block"] else [])
+    split = {"wave_items": info["leaf_wave_items"], "chunked_leaves": info["leaf_chunked"],
+             "workgroup_leaves": info["leaf_block"]}
     if prec == "fp64":
         out["leaf"] = {"bound": "mfma", "pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
-                       "kernel": "k_leaf_stage<double> (P2P+M2P+L2P)", "peak": PEAK_FP64_TFLOPS}
+                       "kernel": " + ".join(parts) + " (P2P+M2P+L2P)", "peak": PEAK_FP64_TFLOPS}
     else:
         out["leaf"] = {"bound": "mfma", "pipe": "FP32 VALU; the published 157.3 TFLOP/s counts packed FP32, which "
                                                 "this build leaves off, so 78.6 TFLOP/s is its issue ceiling",
-                       "kernel": "k_leaf_stage<float> (P2P in FP32, M2P+L2P FP64)", "peak": PEAK_FP32_TFLOPS}
+                       "kernel": " + ".join(parts) + " (P2P in FP32, M2P+L2P FP64)", "peak": PEAK_FP32_TFLOPS}
+    out["leaf"]["split"] = split
     out["leaf"].update({"achieved": round(ach, 3), "unit": "TFLOP/s", "frac": round(ach / out["leaf"]["peak"], 4),
                         "kernel_avg_ms": round(leaf_ms, 4), "algorithmic_flop": flop,
                         "flop_rule": f"{FLOPS_PER_PAIR[kid]} flop per (target, source) pair: P2P pairs + L2P pairs"})
