@@ -728,7 +728,7 @@ static std::vector<I> xcd_layout(const std::vector<std::pair<long, I>>& items, i
 }
 
 static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, const HostList& pts,
-                         const HostList& surf, int n_e, int R, LeafSplit& out, hipStream_t s) {
+                         const HostList& surf, int n_e, LeafSplit& out, hipStream_t s) {
     std::vector<std::pair<long, WaveItem>> w;
     std::vector<std::pair<long, int>> b;
     std::vector<WaveLeafParts> parts;
@@ -746,7 +746,7 @@ static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, con
             w.push_back({m * src, WaveItem{pos, p0, p1, 0, (int)np, c0, c1, 1, -1}});
             continue;
         }
-        if (m > kWaveChunkLeaf || part + (nc + 1) * R * m > INT32_MAX) {
+        if (m > kWaveChunkLeaf || part + (nc + 1) * kPartRows * m > INT32_MAX) {
             b.push_back({m * src, pos});
             continue;
         }
@@ -768,11 +768,11 @@ static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, con
         ch.back().second.self = 1;
         ch.back().first += m * n_e;
         for (size_t j = 0; j < ch.size(); j++) {
-            ch[j].second.part = (int)(part + (long)j * R * m);
+            ch[j].second.part = (int)(part + (long)j * kPartRows * m);
             w.push_back(ch[j]);
         }
         parts.push_back(WaveLeafParts{pos, (int)part, (int)ch.size()});
-        part += (long)ch.size() * R * m;
+        part += (long)ch.size() * kPartRows * m;
     }
     const std::vector<WaveItem> wi =
         xcd_layout(w, kWaveBlock / 64, WaveItem{-1, 0, 0, 0, 0, 0, 0, 0, -1});
@@ -963,7 +963,7 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         const HostList lm = list_from_pairs(OT.leaves, OT.n_cells, tl.m2p, 0, nullptr);
         P->leaf_p2p.upload(lp, s);
         P->leaf_m2p.upload(lm, s);
-        split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->R, P->leaf_split, s);
+        split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->leaf_split, s);
     }
     close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, P->leaf, s);
 
@@ -994,7 +994,7 @@ const FmmPlan::AdjLists& FmmPlan::adjoint_lists(hipStream_t s) const {
         const HostList ll = list_from_pairs(ST.leaves, ST.n_cells, tl.p2l, 1, nullptr);
         A->leaf_p2p.upload(lp, s);
         A->leaf_p2l.upload(ll, s);
-        split_leaves(ST, ST.leaves, lp, ll, n_e, R, A->leaf_split, s);
+        split_leaves(ST, ST.leaves, lp, ll, n_e, A->leaf_split, s);
     }
     close_flags(A->leaf_p2p, src_leaves.p, dst, sp, op, leaf, s);
     A->t_slev.resize(ST.n_levels);

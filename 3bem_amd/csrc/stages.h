@@ -44,12 +44,15 @@ constexpr long kWavePairsMax = 65536;  // ... one wave per leaf, or per chunk of
 // its sources. Point sources [v0, v1) of the leaf's packed ranges (offsets into their
 // concatenation; ranges f0 .. f1-1 hold them), surface cells [c0, c1) of its surface
 // list, and its own surface when self != 0. part < 0: the item is the whole leaf and
-// writes the output; else it writes its unscaled sums, R x m values at
-// part_buf[part + q*m + t], and k_leaf_parts adds the chunks of the leaf in order.
+// writes the output; else it writes its unscaled sums, R x m values (the gradient's
+// 3 x m) at part_buf[part + q*m + t], and k_leaf_parts (k_leaf_grad_parts) adds the
+// chunks of the leaf in order.
 struct WaveItem {
     int pos, f0, f1, v0, v1, c0, c1, self, part;
 };
-// A leaf cut into n chunks: chunk c wrote part_buf[part + c*R*m ...].
+// A leaf cut into n chunks: chunk c wrote part_buf[part + c*kPartRows*m ...] (rows:
+// the R output components, or the gradient's 3).
+constexpr int kPartRows = 3;
 struct WaveLeafParts {
     int pos, part, n;
 };

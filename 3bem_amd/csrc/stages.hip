@@ -729,7 +729,7 @@ __global__ __launch_bounds__(kWaveLeafMax) void k_leaf_parts(LeafStage a, int R,
     for (int q = 0; q < R; q++) {
         const double* p = a.part_buf + (size_t)lp.part + q * m + t;
         double s = 0;
-        for (int c = 0; c < lp.n; c++) s += p[(size_t)c * R * m];
+        for (int c = 0; c < lp.n; c++) s += p[(size_t)c * kPartRows * m];
         double* o = a.out_sorted + (size_t)q * a.n_tgt + i;
         *o = a.accumulate ? *o + wgt * (s * scale) : wgt * (s * scale);
     }
@@ -828,7 +828,8 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
     constexpr bool SPLIT = !std::is_same<T, double>::value;
     __shared__ StageSmem<T, R, NRM, NB, 6> sm;
-    const int pos = blockIdx.x;
+    const int pos = a.order ? uni(a.order[blockIdx.x]) : (int)blockIdx.x;
+    if (pos < 0) return;  // padding of the layout (the whole workgroup)
     const int leaf = uni(a.leaves[pos]);
     const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
     const int pe0 = a.rp_ptr ? uni(a.rp_ptr[pos]) : 0, pe1 = a.rp_ptr ? uni(a.rp_ptr[pos + 1]) : 0;
@@ -909,6 +910,173 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
             a.grad_out[3 * (size_t)(b + t0 + h * nslot + g) + mm] = s * scale;
         });
         __syncthreads();
+    }
+}
+
+// k_leaf_grad2 one wave per work item (a small leaf or a chunk of a large one), as
+// k_leaf_wave is to k_leaf_stage: the same items and layout, two targets per lane,
+// S = 64 / ceil(m/2) source slices, each wave on its own 64-entry LDS tiles with no
+// workgroup barrier. A chunk writes its 3 x m sums to part_buf and k_leaf_grad_parts
+// adds a leaf's chunks in chunk order.
+template <class PT, int R, bool NRM>
+struct GradWaveSmem {
+    union {
+        Tile<PT, R, NRM, 64> pt;
+        Tile<double, R, NRM, 64> st;
+        double red[6 * 64];
+    };
+};
+
+template <int KID, class T>
+__global__ __launch_bounds__(kWaveBlock) void k_leaf_grad2_wave(LeafStage a, KC<double> kc64, KC<T> kct,
+                                                                double scale) {
+    static_assert(has_analytic_grad(KID), "closed-form gradient kernels only");
+    constexpr int R = kernel_dim(KID);
+    constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
+    constexpr bool SPLIT = !std::is_same<T, double>::value;
+    __shared__ GradWaveSmem<T, R, NRM> smw[kWaveBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int idx = blockIdx.x * (kWaveBlock / 64) + w;
+    if (idx >= a.n_wave) return;  // the whole wave: nothing below synchronizes the block
+    const WaveItem* it = a.wave_items + idx;
+    const int pos = uni(it->pos);
+    if (pos < 0) return;  // padding of the layout
+    GradWaveSmem<T, R, NRM>& sm = smw[w];
+    const int leaf = uni(a.leaves[pos]);
+    const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);  // m <= kWaveLeafMax
+    const int nslot = (m + 1) >> 1, S = 64 / nslot;
+    const int tg = lane % nslot, slice = lane / nslot;
+    const bool va = slice < S, vb = va && nslot + tg < m;
+    const int ia = b + tg, ib = b + (vb ? nslot + tg : tg);
+    const double ax = a.tp.x[ia], ay = a.tp.y[ia], az = a.tp.z[ia];
+    const double bx = a.tp.x[ib], by = a.tp.y[ib], bz = a.tp.z[ib];
+    double gA[R], gB[R], GA[3] = {0, 0, 0}, GB[3] = {0, 0, 0};
+    T gAT[R], gBT[R];
+    for (int q = 0; q < R; q++) {
+        gA[q] = a.grad_g[(size_t)q * a.n_tgt + ia];
+        gB[q] = a.grad_g[(size_t)q * a.n_tgt + ib];
+        gAT[q] = T(gA[q]);
+        gBT[q] = T(gB[q]);
+    }
+    const int f0 = uni(it->f0), f1 = uni(it->f1), v0 = uni(it->v0), v1 = uni(it->v1);
+    const int se0 = uni(it->c0), se1 = uni(it->c1), self = uni(it->self), part = uni(it->part);
+    drain_loads();
+    Acc<T, 3> PA, PB;
+    PA.init();
+    PB.init();
+
+    // (a) point sources [v0, v1) of the leaf's packed ranges, as in k_leaf_wave
+    if (a.rp_ptr && v0 < v1) {
+        const int total = v1 - v0;
+        SrcReg<R, NRM> r;
+        int f = f0;
+        auto fetch = [&](int base) {
+            const int v = v0 + base + lane;
+            if (v < v1) {
+                while (f + 1 < f1 && a.rp_pre[f + 1] <= v) f++;
+                load_point(r, a.rp_beg[f] + (v - a.rp_pre[f]), a.ps, a.pdens, a.n_ps);
+            }
+        };
+        fetch(0);
+        for (int base = 0; base < total; base += 64) {
+            const int cnt = min(64, total - base);
+            wave_sync();
+            if (lane < cnt) store_tile(sm.pt, lane, r);
+            wave_sync();
+            if (base + 64 < total) fetch(base + 64);
+            if (va) {
+                const Tile<T, R, NRM, 64>& tl = sm.pt;
+                for (int j = slice; j < cnt; j += S) {
+                    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
+                    T v[R];
+                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                    const double sx = tl.x[j], sy = tl.y[j], sz = tl.z[j];
+                    const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
+                    const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
+                    if constexpr (SPLIT) {
+                        grad_pair<KID, T>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
+                        grad_pair<KID, T>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
+                    } else {
+                        grad_pair<KID, double>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                        grad_pair<KID, double>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+                    }
+                }
+            }
+            PA.flush();
+            PB.flush();
+        }
+    }
+    // (b), (c) surface sources, FP64
+    auto surfaces = [&](const int* cells, int c0, int c1, const TreeView& st, const double* surf,
+                        const double* coeff) {
+        const int total = (c1 - c0) * a.n_e;
+        if (total <= 0) return;
+        SrcReg<R, NRM> r;
+        auto fetch = [&](int base) {
+            const int v = base + lane;
+            if (v < total) {
+                const int k = v / a.n_e;
+                load_surf(r, cells[c0 + k], v - k * a.n_e, st, surf, a.unit, coeff, a.n_e, a.blk);
+            }
+        };
+        fetch(0);
+        for (int base = 0; base < total; base += 64) {
+            const int cnt = min(64, total - base);
+            wave_sync();
+            if (lane < cnt) store_tile(sm.st, lane, r);
+            wave_sync();
+            if (base + 64 < total) fetch(base + 64);
+            if (va) {
+                const Tile<double, R, NRM, 64>& tl = sm.st;
+                for (int j = slice; j < cnt; j += S) {
+                    const double sn[3] = {NRM ? tl.nx[j] : 0.0, NRM ? tl.ny[j] : 0.0, NRM ? tl.nz[j] : 0.0};
+                    double v[R];
+                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                    const double dxa = tl.x[j] - ax, dya = tl.y[j] - ay, dza = tl.z[j] - az;
+                    const double dxb = tl.x[j] - bx, dyb = tl.y[j] - by, dzb = tl.z[j] - bz;
+                    grad_pair<KID, double>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                    grad_pair<KID, double>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+                }
+            }
+        }
+    };
+    if (a.sl_ptr && se0 < se1) surfaces(a.sl_cell, se0, se1, a.slt, a.sl_surf, a.sl_coeff);
+    if (self && a.self_coeff) surfaces(&leaf, 0, 1, a.tt, a.self_surf, a.self_coeff);
+
+    // the S slices of each target, summed in slice order
+    wave_sync();
+    for (int mm = 0; mm < 3; mm++) {
+        const double pa = SPLIT ? PA.value(mm) : 0.0, pb = SPLIT ? PB.value(mm) : 0.0;
+        sm.red[mm * 64 + lane] = va ? GA[mm] + pa : 0.0;
+        sm.red[(3 + mm) * 64 + lane] = vb ? GB[mm] + pb : 0.0;
+    }
+    wave_sync();
+    if (lane < nslot) {
+        for (int h = 0; h < 2; h++) {
+            if (h == 1 && nslot + lane >= m) break;
+            const int t = h * nslot + lane;
+            for (int mm = 0; mm < 3; mm++) {
+                double s = 0;
+                for (int k = 0; k < S; k++) s += sm.red[(h * 3 + mm) * 64 + k * nslot + lane];
+                if (part >= 0) a.part_buf[(size_t)part + mm * m + t] = s;
+                else a.grad_out[3 * (size_t)(b + t) + mm] = s * scale;
+            }
+        }
+    }
+}
+
+// The gradient's chunked leaves: grad_out[3 i + mm] = the chunks' sums, in chunk order.
+__global__ __launch_bounds__(kWaveLeafMax) void k_leaf_grad_parts(LeafStage a, double scale) {
+    const WaveLeafParts lp = a.wave_parts[blockIdx.x];
+    const int leaf = a.leaves[lp.pos];
+    const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
+    const int t = threadIdx.x;
+    if (t >= m) return;
+    for (int mm = 0; mm < 3; mm++) {
+        const double* p = a.part_buf + (size_t)lp.part + mm * m + t;
+        double s = 0;
+        for (int c = 0; c < lp.n; c++) s += p[(size_t)c * kPartRows * m];
+        a.grad_out[3 * (size_t)(b + t) + mm] = s * scale;
     }
 }
 
@@ -1212,12 +1380,38 @@ static void grad_launch(bool fp32_points, const LeafStage& a, const KConst& c, h
     if (a.n_leaves == 0) return;
     const double scale = (double)kop_scale<KID, double>(make_kc64(c));
     if constexpr (has_analytic_grad(KID)) {
+        if (a.wave_items || a.block_pos) {  // the forward's split: small leaves and chunks on waves
+            if (a.n_wave > 0) {
+                const dim3 g((a.n_wave + kWaveBlock / 64 - 1) / (kWaveBlock / 64)), b(kWaveBlock);
+                if (fp32_points)
+                    hipLaunchKernelGGL((k_leaf_grad2_wave<KID, float>), g, b, 0, s, a, make_kc64(c), make_kc32(c),
+                                       scale);
+                else
+                    hipLaunchKernelGGL((k_leaf_grad2_wave<KID, double>), g, b, 0, s, a, make_kc64(c), make_kc64(c),
+                                       scale);
+                TBEM_LAUNCH_CHECK();
+                if (a.n_parts > 0) {
+                    hipLaunchKernelGGL(k_leaf_grad_parts, dim3(a.n_parts), dim3(kWaveLeafMax), 0, s, a, scale);
+                    TBEM_LAUNCH_CHECK();
+                }
+            }
+            if (a.n_block == 0) return;
+            LeafStage bl = a;
+            bl.order = a.block_pos;
+            bl.n_order = a.n_block;
+            bl.wave_items = nullptr;
+            bl.block_pos = nullptr;
+            bl.n_wave = bl.n_parts = 0;
+            grad_launch<KID, T>(fp32_points, bl, c, s);
+            return;
+        }
+        const dim3 g(a.order ? a.n_order : a.n_leaves);
         if (fp32_points)
-            hipLaunchKernelGGL((k_leaf_grad2<KID, float>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kc64(c),
+            hipLaunchKernelGGL((k_leaf_grad2<KID, float>), g, dim3(kGradBlock), 0, s, a, make_kc64(c),
                                make_kc32(c), scale);
         else
-            hipLaunchKernelGGL((k_leaf_grad2<KID, double>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a,
-                               make_kc64(c), make_kc64(c), scale);
+            hipLaunchKernelGGL((k_leaf_grad2<KID, double>), g, dim3(kGradBlock), 0, s, a, make_kc64(c),
+                               make_kc64(c), scale);
     } else {
         hipLaunchKernelGGL((k_leaf_grad<KID>), dim3(a.n_leaves), dim3(kGradBlock), 0, s, a, make_kcd(c), scale);
     }

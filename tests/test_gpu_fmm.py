@@ -616,3 +616,50 @@ def test_gpu_chunked_leaves_match_oracle(tbem, orc, kid, small):
     em = rel_err(mixed.apply(x), y)
     print(f"  vs oracle {e:.2e}, adjoint identity {ea:.2e}, mixed vs fp64 {em:.2e}")
     assert e < FP64_TOL and ea < 1e-13 and em < 1e-5
+
+
+@pytest.mark.parametrize("kid", [0, 3, 4, 6])
+def test_gpu_chunked_leaves_obs_grad(tbem, kid):
+    """The observation gradient over the same work items (its own one-wave kernel and
+    chunk sums for the closed-form kernels 0, 3, 4; one workgroup per leaf for 6):
+    frozen-plan central differences of <g, A(obs) x> (h = 1e-6 and 5e-7, extrapolated)
+    at points of chunked cube leaves and of sparse leaves, deterministic, and the mixed
+    gradient near FP64. Sources and targets are distinct draws of the same geometry (a target moved off a
+    coincident source would meet the kernels' r^2 < 1e-12 cutoff at distance h)."""
+    case("chunked leaves obs grad", kid)
+    g = cluster_and_sparse(31 + kid)
+    s = cluster_and_sparse(131 + kid)
+    R = components(kid)
+    n = len(g["obs"])
+    rng = np.random.default_rng(kid)
+    x, gg = rng.standard_normal(R * n), rng.standard_normal(R * n)
+    d = tbem.NBodyData(g["obs"], g["obs_normals"], s["src"], s["src_normals"], s["weights"])
+    cfg = tbem.FMMConfig(0.3, 6, 64, 0.05, False)
+    op = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, cfg)
+    assert op.info()["leaf_chunked"] > 100
+    grad = op.obs_grad(x, gg)
+    assert np.array_equal(op.obs_grad(x, gg), grad)
+    idx = np.r_[rng.choice(16_000, 6, replace=False), 16_000 + rng.choice(400, 3, replace=False)]
+
+    def central(i, m, h):
+        p = g["obs"].copy()
+        p[i, m] += h
+        op.set_obs_locs(p)
+        fp = op.apply(x) @ gg
+        p[i, m] -= 2 * h
+        op.set_obs_locs(p)
+        return (fp - op.apply(x) @ gg) / (2 * h)
+
+    worst = 0.0
+    for i in idx:
+        for m in range(3):
+            # Richardson: sources 1e-3 away leave O(h^2 / r^2) = 1e-6 in one difference
+            fd = (4 * central(i, m, 5e-7) - central(i, m, 1e-6)) / 3
+            err = abs(fd - grad[i, m]) / (abs(fd) + 1e-3 * np.abs(grad).max())
+            worst = max(worst, err)
+            assert err <= 1e-6, (i, m, fd, grad[i, m])
+    op.set_obs_locs(g["obs"])
+    mixed = tbem.FMMOperator(tbem.make_kernel(kid, 1.0, 0.25), d, cfg, "mixed")
+    em = rel_err(mixed.obs_grad(x, gg), grad)
+    print(f"  worst central-difference error {worst:.2e}, mixed vs fp64 {em:.2e}")
+    assert em < 1e-5
