@@ -40,15 +40,16 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per launch of a workload's leaf and M2L kernels: rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE of the same command (mean over its dispatches), scaled
-# by what the counters read for a plain device copy (factors 2.0 and 1.0; the
-# profiles under profiles/ give the runs). Keyed by (workload, points, precision,
-# kernel); other workloads report null.
+# FETCH_SIZE and WRITE_SIZE of the same command (mean over its dispatches; the leaf
+# stage's are summed over its kernels, k_leaf_wave + k_leaf_parts + k_leaf_stage,
+# for the fused launches the rooflines time), scaled by what the counters read for
+# a plain device copy (factors 2.0 and 1.0; the profiles under profiles/ give the
+# runs). Keyed by (workload, points, precision, kernel); other workloads report null.
 TRAFFIC_BYTES = {
-    ("fault_surface", 1_000_000, "fp64", "leaf"): 644_165_984,
+    ("fault_surface", 1_000_000, "fp64", "leaf"): 334_100_000,
     ("fault_surface_o100", 2_000_000, "fp64", "m2l"): 15_485_747_200,
-    ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 2_370_368_469,
-    ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 2_283_267_211,
+    ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 1_261_500_000,
+    ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 1_270_400_000,
     ("fault_surface_o100", 2_000_000, "mixed", "m2l"): 7_941_620_238,
 }
 TRAFFIC_UNIT = "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)"
