@@ -15,6 +15,7 @@
 //    The r^2 < 1e-12 cutoff is applied by zeroing 1/r, which every term of
 //    these kernels carries as a positive power.
 #pragma once
+#include <type_traits>
 
 #include <hip/hip_runtime.h>
 #include <cmath>
@@ -345,15 +346,25 @@ __device__ __forceinline__ void kop(T dx, T dy, T dz, T r2, const T* no, const T
 // ------------------------------------------------- analytic observation gradients
 // G[m] += sum_k g_k d(K v)_k / d obs_m for kernels with a closed form
 // (d = src - obs, so d d / d obs = -I and d (1/r) / d obs_m = d_m / r^3).
-// The constant factor kop_scale<KID>() is applied by the caller.
+// The constant factor kop_scale<KID>() is applied by the caller. G has
+// grad_slots<KID, T>() entries: for K_EU in FP64 the term -g_m (d.v) / r^3 is summed
+// as the scalar G[3] and applied once per target by the caller (G[m] -= g_m G[3]),
+// which saves two FMAs per pair (in FP32 the extra compensated sum costs more than
+// that). FAST: the one-Newton-step FP64 rsqrt (as the forward point pairs of the FMM).
 __host__ __device__ constexpr bool has_analytic_grad(int kid) {
     return kid == K_LS || kid == K_LD || kid == K_EU || kid == K_ET;
 }
-
 template <int KID, class T>
+__host__ __device__ constexpr int grad_slots() {
+    return (KID == K_EU && std::is_same<T, double>::value) ? 4 : 3;
+}
+
+template <int KID, class T, bool FAST = false>
 __device__ __forceinline__ void grad_pair(T dx, T dy, T dz, T r2, const T* ns, const T* v, const T* g, T* G,
                                           const KC<T>& k) {
-    T ri = rsqrt_t(r2);
+    T ri;
+    if constexpr (FAST) ri = rsqrt_fast(r2);
+    else ri = rsqrt_t(r2);
     ri = (r2 < cutoff_r2<T>()) ? T(0) : ri;
     const T ri2 = ri * ri, ri3 = ri2 * ri;
     const T d[3] = {dx, dy, dz};
@@ -372,7 +383,13 @@ __device__ __forceinline__ void grad_pair(T dx, T dy, T dz, T r2, const T* ns, c
         const T gd = g[0] * dx + g[1] * dy + g[2] * dz;
         const T gv = g[0] * v[0] + g[1] * v[1] + g[2] * v[2];
         const T c1 = k.u2 * gv + T(3) * gd * a * ri2;
-        for (int m = 0; m < 3; m++) G[m] += ri3 * (c1 * d[m] - g[m] * a - gd * v[m]);
+        if constexpr (grad_slots<KID, T>() == 4) {
+            const T p = ri3 * c1, w = ri3 * gd;
+            for (int m = 0; m < 3; m++) G[m] += p * d[m] - w * v[m];
+            G[3] += ri3 * a;  // times -g_m, per target
+        } else {
+            for (int m = 0; m < 3; m++) G[m] += ri3 * (c1 * d[m] - g[m] * a - gd * v[m]);
+        }
     } else if (KID == K_ET) {
         // u_k = -[C2 v_k b / r^3 + 3 d_k a b / r^5 - C2 d_k c / r^3 + C2 n_k a / r^3]
         const T a = dx * v[0] + dy * v[1] + dz * v[2];

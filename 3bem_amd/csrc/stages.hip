@@ -814,6 +814,16 @@ __global__ __launch_bounds__(kGradBlock) void k_leaf_grad(LeafStage a, KC<D3> kc
 // Four waves per SIMD (<= 128 VGPRs) except the FP32 elastic-U instance, which
 // spills under that cap (measured 2M points, order 100: 30.1 ms uncapped against
 // 33.5 capped; the FP64 and traction instances are 1-5% faster capped).
+// A lane's gradient for one target: the FP64 sums G (and, when SPLIT, the FP32 tile
+// sums P) of grad_pair, with the FP64 scalar slot of K_EU applied (G[m] - g_m G[3]).
+template <int KID, bool SPLIT, class P>
+__device__ __forceinline__ void grad_total(const double* G, const P& p, const double* g, double* out) {
+    for (int m = 0; m < 3; m++) {
+        out[m] = G[m] + (SPLIT ? p.value(m) : 0.0);
+        if constexpr (grad_slots<KID, double>() == 4) out[m] -= g[m] * G[3];
+    }
+}
+
 template <int KID, class T>
 constexpr int grad_waves() {
     return (KID == K_EU && std::is_same<T, float>::value) ? 1 : 4;
@@ -845,7 +855,8 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
         const int ib = b + t0 + (vb ? nslot + tg : tg);
         const double ax = a.tp.x[ia], ay = a.tp.y[ia], az = a.tp.z[ia];
         const double bx = a.tp.x[ib], by = a.tp.y[ib], bz = a.tp.z[ib];
-        double gA[R], gB[R], GA[3] = {0, 0, 0}, GB[3] = {0, 0, 0};
+        constexpr int NG = grad_slots<KID, double>();
+    double gA[R], gB[R], GA[NG] = {}, GB[NG] = {};
         T gAT[R], gBT[R];
         for (int q = 0; q < R; q++) {
             gA[q] = a.grad_g[(size_t)q * a.n_tgt + ia];
@@ -854,7 +865,7 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
             gBT[q] = T(gB[q]);
         }
         drain_loads();
-        Acc<T, 3> PA, PB;
+        Acc<T, grad_slots<KID, T>()> PA, PB;
         PA.init();
         PB.init();
         auto points = [&](int cnt) {
@@ -868,11 +879,11 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
                     const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
                     const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
                     if constexpr (SPLIT) {
-                        grad_pair<KID, T>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
-                        grad_pair<KID, T>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
+                        grad_pair<KID, T, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
+                        grad_pair<KID, T, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
                     } else {
-                        grad_pair<KID, double>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
-                        grad_pair<KID, double>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+                        grad_pair<KID, double, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                        grad_pair<KID, double, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
                     }
                 }
             }
@@ -898,10 +909,12 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
         if (a.self_coeff)
             stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, surfaces);
         __syncthreads();
+        double fa[3], fb[3];
+        grad_total<KID, SPLIT>(GA, PA, gA, fa);
+        grad_total<KID, SPLIT>(GB, PB, gB, fb);
         for (int mm = 0; mm < 3; mm++) {
-            const double pa = SPLIT ? PA.value(mm) : 0.0, pb = SPLIT ? PB.value(mm) : 0.0;
-            sm.red[mm * NB + tid] = va ? GA[mm] + pa : 0.0;
-            sm.red[(3 + mm) * NB + tid] = vb ? GB[mm] + pb : 0.0;
+            sm.red[mm * NB + tid] = va ? fa[mm] : 0.0;
+            sm.red[(3 + mm) * NB + tid] = vb ? fb[mm] : 0.0;
         }
         __syncthreads();
         slice_sum<NB>(sm.red, sm.part, 6, ntp, S, [&](int v, int g, double s) {
@@ -950,7 +963,8 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_grad2_wave(LeafStage a, KC<
     const int ia = b + tg, ib = b + (vb ? nslot + tg : tg);
     const double ax = a.tp.x[ia], ay = a.tp.y[ia], az = a.tp.z[ia];
     const double bx = a.tp.x[ib], by = a.tp.y[ib], bz = a.tp.z[ib];
-    double gA[R], gB[R], GA[3] = {0, 0, 0}, GB[3] = {0, 0, 0};
+    constexpr int NG = grad_slots<KID, double>();
+    double gA[R], gB[R], GA[NG] = {}, GB[NG] = {};
     T gAT[R], gBT[R];
     for (int q = 0; q < R; q++) {
         gA[q] = a.grad_g[(size_t)q * a.n_tgt + ia];
@@ -961,7 +975,7 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_grad2_wave(LeafStage a, KC<
     const int f0 = uni(it->f0), f1 = uni(it->f1), v0 = uni(it->v0), v1 = uni(it->v1);
     const int se0 = uni(it->c0), se1 = uni(it->c1), self = uni(it->self), part = uni(it->part);
     drain_loads();
-    Acc<T, 3> PA, PB;
+    Acc<T, grad_slots<KID, T>()> PA, PB;
     PA.init();
     PB.init();
 
@@ -994,11 +1008,11 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_grad2_wave(LeafStage a, KC<
                     const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
                     const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
                     if constexpr (SPLIT) {
-                        grad_pair<KID, T>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
-                        grad_pair<KID, T>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
+                        grad_pair<KID, T, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
+                        grad_pair<KID, T, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
                     } else {
-                        grad_pair<KID, double>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
-                        grad_pair<KID, double>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+                        grad_pair<KID, double, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                        grad_pair<KID, double, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
                     }
                 }
             }
@@ -1045,10 +1059,12 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_grad2_wave(LeafStage a, KC<
 
     // the S slices of each target, summed in slice order
     wave_sync();
+    double fa[3], fb[3];
+    grad_total<KID, SPLIT>(GA, PA, gA, fa);
+    grad_total<KID, SPLIT>(GB, PB, gB, fb);
     for (int mm = 0; mm < 3; mm++) {
-        const double pa = SPLIT ? PA.value(mm) : 0.0, pb = SPLIT ? PB.value(mm) : 0.0;
-        sm.red[mm * 64 + lane] = va ? GA[mm] + pa : 0.0;
-        sm.red[(3 + mm) * 64 + lane] = vb ? GB[mm] + pb : 0.0;
+        sm.red[mm * 64 + lane] = va ? fa[mm] : 0.0;
+        sm.red[(3 + mm) * 64 + lane] = vb ? fb[mm] : 0.0;
     }
     wave_sync();
     if (lane < nslot) {
