@@ -7,8 +7,12 @@ entry point.
     x, info = gmres(apply, b, tol=1e-8, restart=50, maxiter=1000)
     x, info = refine(apply_fp64, apply_inner, b, tol=1e-8, inner_tol=1e-4)
 
-gmres: GMRES(m) (Saad & Schultz 1986) with modified Gram-Schmidt Arnoldi and
-Givens rotations on the small Hessenberg matrix (host, FP64).
+gmres: GMRES(m) (Saad & Schultz 1986). The Arnoldi step orthogonalizes on the device
+by classical Gram-Schmidt with one reorthogonalization (CGS2, Giraud et al. 2005: as
+stable as modified Gram-Schmidt), two matrix-vector products against the basis held
+as the rows of one tensor, so each iteration makes one device-to-host transfer (the
+new Hessenberg column and norm) instead of one per basis vector. Givens rotations on
+the small Hessenberg matrix run on the host in FP64.
 refine: classic mixed-precision iterative refinement: r = b - A64 x with the
 FP64 operator, correction from GMRES on the cheaper inner operator (e.g. the
 FP32 FMM), x += e, until ||r|| / ||b|| <= tol.
@@ -32,9 +36,10 @@ def gmres(apply, b, x0=None, tol=1e-8, restart=50, maxiter=1000, callback=None):
     r = b - apply(x) if x0 is not None else b.clone()
     beta = float(torch.linalg.norm(r))
     hist.append(beta / bnorm)
+    m = restart
+    V = torch.empty((m + 1, b.numel()), dtype=b.dtype, device=b.device)  # basis, one row per vector
     while it < maxiter and beta / bnorm > tol:
-        m = restart
-        V = [r / beta]
+        V[0] = r / beta
         H = np.zeros((m + 1, m))
         cs, sn = np.zeros(m), np.zeros(m)
         g = np.zeros(m + 1)
@@ -42,11 +47,14 @@ def gmres(apply, b, x0=None, tol=1e-8, restart=50, maxiter=1000, callback=None):
         k_done = 0
         for k in range(m):
             w = apply(V[k]).contiguous()
-            for j in range(k + 1):
-                h = float(torch.dot(w, V[j]))
-                H[j, k] = h
-                w.sub_(V[j], alpha=h)
-            hn = float(torch.linalg.norm(w))
+            Vk = V[:k + 1]
+            h = Vk @ w
+            w = w - h @ Vk
+            h2 = Vk @ w  # reorthogonalization pass
+            w = w - h2 @ Vk
+            col = torch.cat([h + h2, torch.linalg.norm(w).reshape(1)]).cpu().numpy()
+            H[:k + 1, k] = col[:k + 1]
+            hn = float(col[k + 1])
             H[k + 1, k] = hn
             for j in range(k):  # apply earlier rotations
                 t = cs[j] * H[j, k] + sn[j] * H[j + 1, k]
@@ -66,10 +74,9 @@ def gmres(apply, b, x0=None, tol=1e-8, restart=50, maxiter=1000, callback=None):
                 callback(it, res)
             if res <= tol or it >= maxiter or hn == 0.0:
                 break
-            V.append(w / hn)
+            V[k + 1] = w / hn
         y = np.linalg.solve(np.triu(H[:k_done, :k_done]), g[:k_done])
-        for j in range(k_done):
-            x.add_(V[j], alpha=float(y[j]))
+        x.add_(torch.from_numpy(y).to(device=b.device, dtype=b.dtype) @ V[:k_done])
         r = b - apply(x)
         beta = float(torch.linalg.norm(r))
         hist[-1] = beta / bnorm  # true residual at the restart
