@@ -446,13 +446,12 @@ __global__ __launch_bounds__(kSurfBlock) void k_surf_stage(SurfStage a, KC<doubl
 // workgroup, <= 128 VGPRs keeps two workgroups per CU; one more register halves
 // the resident waves (the elastic-U FP32 instance at 139 VGPRs ran 1.5x slower),
 // so leaf_waves() makes the compiler hold that budget (leaf_block() for the rest).
-// CLOSE (FP64 only): tiles without close points (LeafStage.rp_close) skip the cutoff
-// test; the instances without it have one pair loop and fewer registers, which pays
-// where leaves are small and per-leaf work dominates.
+// CLOSE: tiles without close points (LeafStage.rp_close) skip the cutoff test; the
+// instances without it have one pair loop and fewer registers, which pays where
+// leaves are small and per-leaf work dominates.
 template <int KID, class T, bool ADJ, bool FAST, bool CLOSE = false>
 __global__ __launch_bounds__((leaf_block<KID, T>()), (leaf_waves<KID, T>())) void k_leaf_stage(LeafStage a, KC<T> kc, KC<double> kc64,
                                                                   double scale) {
-    static_assert(!CLOSE || std::is_same<T, double>::value, "cutoff-free tiles: FP64 only");
     constexpr int NB = leaf_block<KID, T>();
     constexpr int R = kernel_dim(KID);
     constexpr bool NRM = kernel_uses_nsrc(KID) || kernel_uses_nobs(KID);
@@ -1378,9 +1377,11 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
     dim3 g(a.order ? a.n_order : a.n_leaves), b(leaf_block<KID, T>());
     // the one-step rsqrt only differs from the full one in FP64
     if (fast && std::is_same<T, double>::value && a.rp_close && a.rp_ptr) {
-        constexpr bool C = std::is_same<T, double>::value;  // (always true here)
-        if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, true, C>), g, b, 0, s, a, kc, kc64, scale);
-        else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, true, C>), g, b, 0, s, a, kc, kc64, scale);
+        if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, true, true>), g, b, 0, s, a, kc, kc64, scale);
+        else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, true, true>), g, b, 0, s, a, kc, kc64, scale);
+    } else if (std::is_same<T, float>::value && a.rp_close && a.rp_ptr) {
+        if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, false, true>), g, b, 0, s, a, kc, kc64, scale);
+        else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, false, true>), g, b, 0, s, a, kc, kc64, scale);
     } else if (fast && std::is_same<T, double>::value) {
         if (adj) hipLaunchKernelGGL((k_leaf_stage<KID, T, true, true>), g, b, 0, s, a, kc, kc64, scale);
         else hipLaunchKernelGGL((k_leaf_stage<KID, T, false, true>), g, b, 0, s, a, kc, kc64, scale);
