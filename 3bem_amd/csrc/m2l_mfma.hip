@@ -40,8 +40,14 @@ namespace {
 
 constexpr int kUnitCols = 64;   // pairs per work unit (= LDS row length of the staged columns)
 constexpr int kUnitCols64 = 32; // the same for the FP64 operators
-constexpr int kPre64 = 8;       // k-steps of A loaded ahead by the FP64 kernel
-constexpr int kWaves64 = 8;     // waves per workgroup of the FP64 kernel
+// The FP64 kernel's shape, as for the FP32 one by how many workgroups the staged
+// columns leave per CU: eight waves with A 8 k-steps ahead where at most two fit
+// (order 100 elastic, blk 297), four waves 4 k-steps ahead otherwise (order 75
+// Laplace, blk 74: M2L stage 52.5 -> 42.3 ms; 4 waves with 8 ahead 46.5, 2 waves 43.4).
+constexpr int kPre64 = 8;
+constexpr int kWaves64 = 8;
+constexpr int kPre64Narrow = 4;
+constexpr int kWaves64Narrow = 4;
 constexpr int kMaxBlk = 512;    // staged columns: up to 512 x 64 x 4 B = 128 KiB of LDS
 // Below this operator size the FP64 M2L stage is the cheaper one: measured on the
 // 1M-point elastic workload at order 6 (blk 18), the MFMA path with its per-pair
@@ -364,21 +370,59 @@ __global__ __launch_bounds__(NW * 64) void k_m2l_mfma64(int n, int mt, int kt, i
 }
 
 // out[t] (= or += with add) sum over the pairs of item k in this slice (slice_pairs)
-// of Y[p][0 .. blk), in pair order, FP64.
+// of Y[p][0 .. blk), FP64. As k_m2l_pair_sum: thread i takes double2 column i % q
+// (q = ystride / 2) of every npar-th pair from i / q on, and the npar partial sums
+// (each in pair order) are added in order through LDS. With small operators most of
+// the workgroup would otherwise idle while blk threads each walk all the target's
+// pairs (4M-point Laplace, order 75: blk 74 of 256 threads, 2.1 ms per 6 GB slice).
+// Above 256 rows (q > 128) npar is 1 and each thread takes two columns in turn.
 __global__ __launch_bounds__(256) void k_m2l_pair_sum64(const int* __restrict__ items, const int* __restrict__ ptr,
                                                         const int* __restrict__ pairs, const double* __restrict__ Y,
                                                         int ystride, int blk, int ybase, int pb, int pe, int add,
                                                         double* __restrict__ out) {
+    extern __shared__ double part64[];  // npar * ystride partial sums (npar > 1 only)
     const int k = blockIdx.x;
     const int t = uni(items[k]);
     int p0, p1;
     slice_pairs(ptr, pairs, k, pb, pe, p0, p1);
+    const int q = ystride / 2, npar = max(1, 256 / q);
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const f64x2* y2 = (const f64x2*)Y;
+    if (npar == 1) {
+        for (int c = threadIdx.x; c < q; c += blockDim.x) {
+            double s0 = 0, s1 = 0;
+            for (int j = p0; j < p1; j++) {
+                const size_t p = (size_t)((pairs ? pairs[j] : j) - ybase);
+                const f64x2 v = y2[p * q + c];
+                s0 += v[0];
+                s1 += v[1];
+            }
+            for (int h = 0; h < 2; h++) {
+                const int row = 2 * c + h;
+                if (row >= blk) break;
+                double* o = out + (size_t)t * blk + row;
+                const double s = h ? s1 : s0;
+                *o = add ? *o + s : s;
+            }
+        }
+        return;
+    }
+    const int c = threadIdx.x % q, ph = threadIdx.x / q;
+    if (ph < npar) {
+        double s0 = 0, s1 = 0;
+        for (int j = p0 + ph; j < p1; j += npar) {
+            const size_t p = (size_t)((pairs ? pairs[j] : j) - ybase);
+            const f64x2 v = y2[p * q + c];
+            s0 += v[0];
+            s1 += v[1];
+        }
+        part64[(size_t)ph * ystride + 2 * c] = s0;
+        part64[(size_t)ph * ystride + 2 * c + 1] = s1;
+    }
+    __syncthreads();
     for (int row = threadIdx.x; row < blk; row += blockDim.x) {
         double s = 0;
-        for (int j = p0; j < p1; j++) {
-            const size_t p = (size_t)((pairs ? pairs[j] : j) - ybase);
-            s += Y[p * ystride + row];
-        }
+        for (int h = 0; h < npar; h++) s += part64[(size_t)h * ystride + row];
         double* o = out + (size_t)t * blk + row;
         *o = add ? *o + s : s;
     }
@@ -545,6 +589,8 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     if (shm > 65536) {
         if (fp64) {
             TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma64<kWaves64, kPre64>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma64<kWaves64Narrow, kPre64Narrow>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         } else {
             TBEM_HIP(hipFuncSetAttribute((const void*)k_m2l_mfma<4, kPreNarrow>,
@@ -811,6 +857,12 @@ void build_m2l_mfma(M2lMfma& M, int kid, const KConst& kc, const HostTree& ot, c
     M.plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// LDS of k_m2l_pair_sum64: npar partial rows of ystride doubles when npar > 1
+static size_t pair_sum64_shm(int ystride) {
+    const int q = ystride / 2, npar = std::max(1, 256 / q);
+    return npar > 1 ? (size_t)npar * ystride * sizeof(double) : 0;
+}
+
 // The slice's units: grid and offset unit arrays
 struct SliceUnits {
     int n;
@@ -847,9 +899,15 @@ static void mfma64_slice(const M2lMfma& M, int q, const int* pair_src, const dou
     const SliceUnits U = slice_units(M, q);
     if (U.n == 0) return;
     const size_t shm = staged_bytes(M.blk, true);
-    hipLaunchKernelGGL((k_m2l_mfma64<kWaves64, kPre64>), dim3(U.n), dim3(kWaves64 * 64), shm, s, M.blk, M.mt, M.kt,
-                       M.ystride, transpose ? M.ST64.p : M.S64.p, U.ucls, U.uoff, U.ucnt, M.colpair.p,
-                       transpose ? M.pair_tgt.p : pair_src, M.slice_pairs[q], x, Y);
+    const double* S = transpose ? M.ST64.p : M.S64.p;
+    const int* src = transpose ? M.pair_tgt.p : pair_src;
+    if (3 * shm > kLdsPerCu)
+        hipLaunchKernelGGL((k_m2l_mfma64<kWaves64, kPre64>), dim3(U.n), dim3(kWaves64 * 64), shm, s, M.blk, M.mt,
+                           M.kt, M.ystride, S, U.ucls, U.uoff, U.ucnt, M.colpair.p, src, M.slice_pairs[q], x, Y);
+    else
+        hipLaunchKernelGGL((k_m2l_mfma64<kWaves64Narrow, kPre64Narrow>), dim3(U.n), dim3(kWaves64Narrow * 64), shm,
+                           s, M.blk, M.mt, M.kt, M.ystride, S, U.ucls, U.uoff, U.ucnt, M.colpair.p, src,
+                           M.slice_pairs[q], x, Y);
     TBEM_LAUNCH_CHECK();
 }
 
@@ -864,8 +922,8 @@ static void forward_sum_slice(const M2lMfma& M, int q, const int* items, const i
         hipLaunchKernelGGL(k_m2l_pair_sum, dim3(nk), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
                            items + k0, ptr + k0, nullptr, Y, M.ystride, M.blk, M.slice_pairs[q], 0, 0, 0, out);
     } else {
-        hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(nk), dim3(256), 0, s, items + k0, ptr + k0, nullptr, Y, M.ystride,
-                           M.blk, M.slice_pairs[q], 0, 0, 0, out);
+        hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(nk), dim3(256), pair_sum64_shm(M.ystride), s, items + k0,
+                           ptr + k0, nullptr, Y, M.ystride, M.blk, M.slice_pairs[q], 0, 0, 0, out);
     }
     TBEM_LAUNCH_CHECK();
 }
@@ -882,8 +940,8 @@ static void adjoint_sum_slice(const M2lMfma& M, int q, const T* Y, double* out, 
         hipLaunchKernelGGL(k_m2l_pair_sum, dim3(M.n_adj), dim3(256), (size_t)npar * M.ystride * sizeof(double), s,
                            M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, Y, M.ystride, M.blk, pb, pb, pe, add, out);
     } else {
-        hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(M.n_adj), dim3(256), 0, s, M.adj_items.p, M.adj_ptr.p,
-                           M.adj_pairs.p, Y, M.ystride, M.blk, pb, pb, pe, add, out);
+        hipLaunchKernelGGL(k_m2l_pair_sum64, dim3(M.n_adj), dim3(256), pair_sum64_shm(M.ystride), s,
+                           M.adj_items.p, M.adj_ptr.p, M.adj_pairs.p, Y, M.ystride, M.blk, pb, pb, pe, add, out);
     }
     TBEM_LAUNCH_CHECK();
 }
