@@ -665,17 +665,16 @@ struct PhaseClock {
 
 // The point-target stage leaves the r2 < 1e-12 cutoff test out of source tiles whose
 // ranges lie farther than this from the leaf's points (same result; see
-// launch_range_close). Only for leaves of at least kCloseLeaf points: with small
-// leaves the per-leaf work dominates and the one-loop kernel is faster (1M points,
-// leaf 64: 4.05 against 4.54 ms; 2M points, leaf 256: 19.2 against 19.7 ms).
+// launch_range_close). The one-wave kernel checks a tile with one ballot; the
+// 512-thread one with a ballot per wave into LDS (measured with flags on every leaf
+// against none: 4M-point Laplace, leaf 100, FP64 leaf stage 40.8 -> 38.1 ms; 1M-point
+// elastic, leaf 64, and 2M points, leaf 256: within 1%).
 constexpr double kCloseMargin = 1e-5;
-constexpr size_t kCloseLeaf = 128;
 
 // l: a point-range list indexed by the leaves `leaves` of tree tt (points tp), over
 // the source points sp: (re)computes its per-range close flags.
 static void close_flags(DevList& l, const int* leaves, const DevTree& tt, const DevPoints& tp, const DevPoints& sp,
-                        size_t leaf, hipStream_t s) {
-    if (leaf < kCloseLeaf) return;  // no flags: the one-loop kernel
+                        hipStream_t s) {
     const size_t n = l.rbeg.n;
     if (l.rclose.n != n) l.rclose.alloc(n);
     if (n) launch_range_close(l.n_items, leaves, view(tt), view(tp), l.ptr.p, l.rbeg.p, l.rcnt.p, view(sp),
@@ -965,7 +964,7 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         P->leaf_m2p.upload(lm, s);
         split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->leaf_split, s);
     }
-    close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, P->leaf, s);
+    close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, s);
 
     clk.mark("rest");
     TBEM_HIP(hipStreamSynchronize(s));
@@ -996,7 +995,7 @@ const FmmPlan::AdjLists& FmmPlan::adjoint_lists(hipStream_t s) const {
         A->leaf_p2l.upload(ll, s);
         split_leaves(ST, ST.leaves, lp, ll, n_e, A->leaf_split, s);
     }
-    close_flags(A->leaf_p2p, src_leaves.p, dst, sp, op, leaf, s);
+    close_flags(A->leaf_p2p, src_leaves.p, dst, sp, op, s);
     A->t_slev.resize(ST.n_levels);
     for (int L = 0; L < ST.n_levels; L++) A->t_slev[L] = make_level_tiles(ST, L, s);
     TBEM_HIP(hipStreamSynchronize(s));
@@ -1423,10 +1422,10 @@ static void move_obs(FmmPlan& P, const double* d_locs, hipStream_t s) {
                        P.op.y.p, P.op.z.p);
     TBEM_LAUNCH_CHECK();
     // the point-target stages' close flags follow the moved points
-    close_flags(P.leaf_p2p, P.obs_leaves.p, P.dot, P.op, P.sp, P.leaf, s);
+    close_flags(P.leaf_p2p, P.obs_leaves.p, P.dot, P.op, P.sp, s);
     {
         std::lock_guard<std::mutex> lk(P.adj_mu);
-        if (P.adj) close_flags(P.adj->leaf_p2p, P.src_leaves.p, P.dst, P.sp, P.op, P.leaf, s);
+        if (P.adj) close_flags(P.adj->leaf_p2p, P.src_leaves.p, P.dst, P.sp, P.op, s);
     }
     TBEM_HIP(hipStreamSynchronize(s));
     P.obs_moved = true;

@@ -624,11 +624,14 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_wave(LeafStage a, KC<T> kc,
         const int total = v1 - v0;
         SrcReg<R, NRM> r;
         int f = f0;
+        int mclose = 0;  // the lane's fetched point comes from a range flagged close (rp_close)
         auto fetch = [&](int base) {
             const int v = v0 + base + lane;
+            mclose = 0;
             if (v < v1) {
                 while (f + 1 < f1 && a.rp_pre[f + 1] <= v) f++;
                 load_point(r, a.rp_beg[f] + (v - a.rp_pre[f]), a.ps, a.pdens, a.n_ps);
+                mclose = a.rp_close ? a.rp_close[f] : 1;
             }
         };
         fetch(0);
@@ -636,12 +639,22 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_wave(LeafStage a, KC<T> kc,
             const int cnt = min(64, total - base);
             wave_sync();
             if (lane < cnt) store_tile(sm.pt, lane, r);
+            // a tile none of whose points is flagged close runs without the cutoff test
+            // (the same result: every pair is farther apart than the cutoff radius)
+            const bool tclose = __ballot(mclose) != 0ull;
             wave_sync();
             if (base + 64 < total) fetch(base + 64);
             if (va) {
-                for (int j = slice; j < cnt; j += S) {
-                    pair<KID, ADJ, FAST>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
-                    pair<KID, ADJ, FAST>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                if (tclose) {
+                    for (int j = slice; j < cnt; j += S) {
+                        pair<KID, ADJ, FAST, true>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                        pair<KID, ADJ, FAST, true>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                    }
+                } else {
+                    for (int j = slice; j < cnt; j += S) {
+                        pair<KID, ADJ, FAST, false>(sm.pt, j, ax, ay, az, an, acA.tile(), kc);
+                        pair<KID, ADJ, FAST, false>(sm.pt, j, bx, by, bz, bn, acB.tile(), kc);
+                    }
                 }
             }
             acA.flush();

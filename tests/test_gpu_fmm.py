@@ -491,18 +491,19 @@ def straddling_pairs(n, m, seed):
     return locs, nrm, rng.uniform(0.5, 1.5, len(locs))
 
 
+@pytest.mark.parametrize("leaf", [128, 32])
 @pytest.mark.parametrize("precision,kid", [("fp64", 3), ("fp64", 0), ("mixed", 3), ("fp64", 6)])
-def test_cutoff_holds_across_leaf_boundaries(tbem, precision, kid):
+def test_cutoff_holds_across_leaf_boundaries(tbem, precision, kid, leaf):
     """Source tiles whose ranges lie farther than 1e-5 from a leaf's points skip the
-    r2 < 1e-12 test (same result; FP64, leaves of 128 points and more). Pairs 2e-7
-    apart across a leaf boundary must still get it: the FMM equals the direct operator
-    (which tests every pair) on points that hold 200 such pairs, forward and
-    transpose."""
-    case("cutoff across leaves", precision, kid)
+    r2 < 1e-12 test (same result; both precisions, the 512-thread kernel at leaf 128
+    and the one-wave kernel at leaf 32). Pairs 2e-7 apart across a leaf boundary must
+    still get it: the FMM equals the direct operator (which tests every pair) on
+    points that hold 200 such pairs, forward and transpose."""
+    case("cutoff across leaves", precision, kid, leaf)
     locs, nrm, w = straddling_pairs(6000, 200, 41)
     K = tbem.make_kernel(kid, 1.0, 0.25)
     data = tbem.NBodyData(locs, nrm, locs, nrm, w)
-    op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, 6, 128, 0.05, False), precision)
+    op = tbem.FMMOperator(K, data, tbem.FMMConfig(0.3, 6, leaf, 0.05, False), precision)
     direct = tbem.make_direct_nbody_operator(data, K)
     rng = np.random.default_rng(3)
     x = rng.standard_normal(K.dim * len(locs))
@@ -516,20 +517,22 @@ def test_cutoff_holds_across_leaf_boundaries(tbem, precision, kid):
     far = locs.copy()
     far[-200:, 0] += 1e-3  # the partners leave the cutoff radius: huge but finite pairs appear
     data2 = tbem.NBodyData(far, nrm, far, nrm, w)
-    y2 = tbem.FMMOperator(K, data2, tbem.FMMConfig(0.3, 6, 128, 0.05, False), precision).apply(x)
+    y2 = tbem.FMMOperator(K, data2, tbem.FMMConfig(0.3, 6, leaf, 0.05, False), precision).apply(x)
     assert rel_err(y2, tbem.make_direct_nbody_operator(data2, K).apply(x)) < tol
 
 
-def test_cutoff_flags_follow_moved_observation_points(tbem):
+@pytest.mark.parametrize("leaf", [128, 32])
+def test_cutoff_flags_follow_moved_observation_points(tbem, leaf):
     """set_obs_locs recomputes the close flags: an observation point moved (inside its
-    leaf ball) to 2e-7 from a source across a leaf boundary keeps the cutoff."""
-    case("cutoff after set_obs_locs")
+    leaf ball) to 2e-7 from a source across a leaf boundary keeps the cutoff (the
+    512-thread kernel at leaf 128, the one-wave kernel at leaf 32)."""
+    case("cutoff after set_obs_locs", leaf)
     rng = np.random.default_rng(8)
     locs, nrm, w = straddling_pairs(5000, 100, 12)
     obs = locs.copy()
     obs[-100:, 0] = 0.5 + 3e-6  # the upper partners start 3e-6 from the boundary
     K = tbem.make_kernel(3, 1.0, 0.25)
-    cfg = tbem.FMMConfig(0.3, 6, 128, 0.05, False)
+    cfg = tbem.FMMConfig(0.3, 6, leaf, 0.05, False)
     op = tbem.FMMOperator(K, tbem.NBodyData(obs, nrm, locs, nrm, w), cfg)
     x = rng.standard_normal(3 * len(locs))
     moved = obs.copy()
