@@ -359,13 +359,13 @@ __host__ __device__ constexpr int grad_slots() {
     return (KID == K_EU && std::is_same<T, double>::value) ? 4 : 3;
 }
 
-template <int KID, class T, bool FAST = false>
+template <int KID, class T, bool FAST = false, bool CUT = true>
 __device__ __forceinline__ void grad_pair(T dx, T dy, T dz, T r2, const T* ns, const T* v, const T* g, T* G,
                                           const KC<T>& k) {
     T ri;
     if constexpr (FAST) ri = rsqrt_fast(r2);
     else ri = rsqrt_t(r2);
-    ri = (r2 < cutoff_r2<T>()) ? T(0) : ri;
+    if constexpr (CUT) ri = (r2 < cutoff_r2<T>()) ? T(0) : ri;
     const T ri2 = ri * ri, ri3 = ri2 * ri;
     const T d[3] = {dx, dy, dz};
     if (KID == K_LS) {

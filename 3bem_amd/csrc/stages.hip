@@ -880,24 +880,30 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
         Acc<T, grad_slots<KID, T>()> PA, PB;
         PA.init();
         PB.init();
+        int tclose = 1;  // set per tile by stream_ranges_packed when the plan has close flags
+        auto point_loop = [&](int cnt, auto cut) {
+            constexpr bool C = decltype(cut)::value;
+            const Tile<T, R, NRM, NB>& tl = sm.pt;
+            for (int j = slice; j < cnt; j += S) {
+                const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
+                T v[R];
+                for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                const double sx = tl.x[j], sy = tl.y[j], sz = tl.z[j];
+                const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
+                const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
+                if constexpr (SPLIT) {
+                    grad_pair<KID, T, true, C>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
+                    grad_pair<KID, T, true, C>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
+                } else {
+                    grad_pair<KID, double, true, C>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                    grad_pair<KID, double, true, C>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+                }
+            }
+        };
         auto points = [&](int cnt) {
             if (va) {
-                const Tile<T, R, NRM, NB>& tl = sm.pt;
-                for (int j = slice; j < cnt; j += S) {
-                    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
-                    T v[R];
-                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
-                    const double sx = tl.x[j], sy = tl.y[j], sz = tl.z[j];
-                    const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
-                    const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
-                    if constexpr (SPLIT) {
-                        grad_pair<KID, T, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
-                        grad_pair<KID, T, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
-                    } else {
-                        grad_pair<KID, double, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
-                        grad_pair<KID, double, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
-                    }
-                }
+                if (uni(tclose)) point_loop(cnt, std::true_type{});
+                else point_loop(cnt, std::false_type{});  // no point of the tile near the leaf
             }
             PA.flush();
             PB.flush();
@@ -916,7 +922,7 @@ __global__ __launch_bounds__(kGradBlock, (grad_waves<KID, T>())) void k_leaf_gra
             }
         };
         stream_ranges_packed(sm.pt, sm.rpre, sm.rbeg, pe0, pe1, a.rp_beg, a.rp_cnt, a.rp_pre, a.ps, a.pdens,
-                             a.n_ps, points);
+                             a.n_ps, points, a.rp_close, a.rp_close ? &tclose : nullptr, sm.wflag);
         stream_surfaces(sm.st, se0, se1, a.sl_cell, a.slt, a.sl_surf, a.unit, a.sl_coeff, a.n_e, a.blk, surfaces);
         if (a.self_coeff)
             stream_surfaces(sm.st, 0, 1, &leaf, a.tt, a.self_surf, a.unit, a.self_coeff, a.n_e, a.blk, surfaces);
@@ -996,11 +1002,33 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_grad2_wave(LeafStage a, KC<
         const int total = v1 - v0;
         SrcReg<R, NRM> r;
         int f = f0;
+        int mclose = 0;  // as in k_leaf_wave
         auto fetch = [&](int base) {
             const int v = v0 + base + lane;
+            mclose = 0;
             if (v < v1) {
                 while (f + 1 < f1 && a.rp_pre[f + 1] <= v) f++;
                 load_point(r, a.rp_beg[f] + (v - a.rp_pre[f]), a.ps, a.pdens, a.n_ps);
+                mclose = a.rp_close ? a.rp_close[f] : 1;
+            }
+        };
+        auto point_loop = [&](int cnt, auto cut) {
+            constexpr bool C = decltype(cut)::value;
+            const Tile<T, R, NRM, 64>& tl = sm.pt;
+            for (int j = slice; j < cnt; j += S) {
+                const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
+                T v[R];
+                for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
+                const double sx = tl.x[j], sy = tl.y[j], sz = tl.z[j];
+                const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
+                const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
+                if constexpr (SPLIT) {
+                    grad_pair<KID, T, true, C>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
+                    grad_pair<KID, T, true, C>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
+                } else {
+                    grad_pair<KID, double, true, C>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
+                    grad_pair<KID, double, true, C>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
+                }
             }
         };
         fetch(0);
@@ -1008,25 +1036,12 @@ __global__ __launch_bounds__(kWaveBlock) void k_leaf_grad2_wave(LeafStage a, KC<
             const int cnt = min(64, total - base);
             wave_sync();
             if (lane < cnt) store_tile(sm.pt, lane, r);
+            const bool tclose = __ballot(mclose) != 0ull;
             wave_sync();
             if (base + 64 < total) fetch(base + 64);
             if (va) {
-                const Tile<T, R, NRM, 64>& tl = sm.pt;
-                for (int j = slice; j < cnt; j += S) {
-                    const T sn[3] = {NRM ? tl.nx[j] : T(0), NRM ? tl.ny[j] : T(0), NRM ? tl.nz[j] : T(0)};
-                    T v[R];
-                    for (int q = 0; q < R; q++) v[q] = tl.d[q][j];
-                    const double sx = tl.x[j], sy = tl.y[j], sz = tl.z[j];
-                    const T dxa = T(sx - ax), dya = T(sy - ay), dza = T(sz - az);
-                    const T dxb = T(sx - bx), dyb = T(sy - by), dzb = T(sz - bz);
-                    if constexpr (SPLIT) {
-                        grad_pair<KID, T, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gAT, PA.tile(), kct);
-                        grad_pair<KID, T, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gBT, PB.tile(), kct);
-                    } else {
-                        grad_pair<KID, double, true>(dxa, dya, dza, dxa * dxa + dya * dya + dza * dza, sn, v, gA, GA, kc64);
-                        grad_pair<KID, double, true>(dxb, dyb, dzb, dxb * dxb + dyb * dyb + dzb * dzb, sn, v, gB, GB, kc64);
-                    }
-                }
+                if (tclose) point_loop(cnt, std::true_type{});
+                else point_loop(cnt, std::false_type{});
             }
             PA.flush();
             PB.flush();
