@@ -1126,7 +1126,12 @@ static void farfield(const FmmPlan& P, Work& W, const double* x, hipStream_t s, 
 }
 
 static void set_split(LeafStage& f, const LeafSplit& sp, Work& W) {
-    if (W.wpart.n < sp.part_size) W.wpart.alloc(sp.part_size);
+    if (W.wpart.n < sp.part_size) {
+        // (the transpose's split on its first call) the previous call's leaf kernels may
+        // still read the old buffer: wait for them before it is freed
+        TBEM_HIP(hipEventSynchronize(W.done));
+        W.wpart.alloc(sp.part_size);
+    }
     f.wave_items = sp.items.p; f.n_wave = sp.n_items;
     f.wave_parts = sp.parts.p; f.n_parts = sp.n_parts; f.part_buf = W.wpart.p;
     f.block_pos = sp.block.p; f.n_block = sp.n_block;

@@ -114,3 +114,37 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120, cwd=ROOT)
     assert p.returncode == 2
     assert "one GPU only" in p.stderr and p.stdout == ""
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """The Python binding's descriptor and info structs have the C header's size and
+    field offsets (compiled with the host C compiler; a field added on one side only
+    would shift every field after it)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    binding = importlib.import_module("3bem_amd._lib")
+    structs = {"tbem_fmm_desc": binding.FmmDesc, "tbem_fmm_info_t": binding.FmmInfo,
+               "tbem_nbody_desc": binding.NbodyDesc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tbem_fmm.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    seen = {}
+    for line in out:
+        if line.strip():
+            cname, field, value = line.split()
+            seen[(cname, field)] = int(value)
+    for cname, py in structs.items():
+        assert seen[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert seen[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
