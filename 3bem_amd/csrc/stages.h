@@ -21,10 +21,13 @@ constexpr int kLeafBlock = 512;  // point-target stage (P2P + M2P + L2P)
 //    workgroups per CU): adjoint-traction forward 75 -> 65 ms, traction adjoint 84
 //    -> 74 ms;
 //  * the rest (<= 128 VGPRs already): 512 threads, no cap; a cap at four waves made
-//    the FP64 elastic-U leaf 16% slower (101 VGPRs, less latency hiding).
+//    the FP64 elastic-U leaf 16% slower (101 VGPRs, less latency hiding);
+//  * Laplace single layer, whose pair is cheap, at 256 threads (4M points, leaf 100,
+//    the larger leaves on this kernel: FP64 leaf stage 38.2 -> 36.9 ms, mixed 27.9
+//    -> 26.5 ms).
 template <int KID, class T>
 constexpr bool leaf_narrow() {
-    return KID == 6 || ((KID == 4 || KID == 5) && std::is_same<T, float>::value);
+    return KID == 0 || KID == 6 || ((KID == 4 || KID == 5) && std::is_same<T, float>::value);
 }
 template <int KID, class T>
 constexpr int leaf_block() {
