@@ -134,8 +134,9 @@ __global__ __launch_bounds__(1024) void k_scan_single(int* __restrict__ a, int m
 }
 
 // out = exclusive scan of a, given the exclusive scan of the tile sums
-__global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int* __restrict__ a, int n,
-                                                            const int* __restrict__ tile_off, int* __restrict__ out) {
+// (a and out may be the same array: each thread reads its elements before it writes them)
+__global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int* a, int n, const int* __restrict__ tile_off,
+                                                            int* out) {
     __shared__ int sh[kScanThreads];
     const size_t b0 = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     int v[kScanItems], s = 0;
@@ -148,6 +149,21 @@ __global__ __launch_bounds__(kScanThreads) void k_tile_scan(const int* __restric
         if (b0 + k < (size_t)n) out[b0 + k] = run;
         run += v[k];
     }
+}
+
+// a = exclusive scan of a (n ints), in place and without a host synchronization:
+// tile sums, one workgroup over the tile sums, then each tile (whose threads read
+// their elements before writing them). sums: scratch of at least div_up(n,
+// kScanTile) + 1 ints.
+static void device_exclusive_scan_inplace(int* a, int n, int* sums, hipStream_t s) {
+    if (n == 0) return;
+    const int tiles = div_up(n, kScanTile);
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, a, n, sums);
+    TBEM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, sums, tiles + 1);
+    TBEM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tile_scan, dim3(tiles), dim3(kScanThreads), 0, s, a, n, sums, a);
+    TBEM_LAUNCH_CHECK();
 }
 
 // out = exclusive scan of a (n ints); returns the total
@@ -236,11 +252,11 @@ static void radix_sort_pairs(K* k, int* v, K* kt, int* vt, int n, int bits, hipS
     if (n <= 1) return;
     const int tiles = div_up(n, kRsTile);
     DevBuf<int> counts((size_t)256 * tiles);
+    DevBuf<int> sums(div_up(256 * tiles, kScanTile) + 1);
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist<K>, dim3(tiles), dim3(kRsThreads), 0, s, k, n, shift, counts.p, tiles);
         TBEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, counts.p, 256 * tiles);
-        TBEM_LAUNCH_CHECK();
+        device_exclusive_scan_inplace(counts.p, 256 * tiles, sums.p, s);
         hipLaunchKernelGGL(k_rs_scatter<K>, dim3(tiles), dim3(kRsThreads), 0, s, k, v, n, shift, counts.p, tiles,
                            kt, vt);
         TBEM_LAUNCH_CHECK();
