@@ -181,14 +181,19 @@ class FMMOperator:
     the two surfaces of every pair, as the reference).
     m2l_output_budget: bytes the class M2L's per-pair output may take (None: 6 GiB);
     beyond it the pairs run in slices.
+    point_pairs: "auto" (the forward evaluates each pair of points in neighbouring
+    leaves once for both directions where the kernel is symmetric: elastic
+    displacement, one point set on both sides, fp64, leaves of 64 to 256 points) or
+    "ordered" (every ordered pair, as the reference).
     """
 
     TREE_BOXES = {"auto": 0, "own": 1, "common": 2}
     TRANSLATIONS = {"auto": 0, "surfaces": 1}
+    POINT_PAIRS = {"auto": 0, "ordered": 1}
 
     def __init__(self, K, data, config, precision="fp64", up_c2e=None, down_c2e=None, plan=None,
                  tree_boxes="auto", translations="auto",
-                 m2l_output_budget=None):
+                 m2l_output_budget=None, point_pairs="auto"):
         lib = _lib.require_device()
         self.K = K
         self.data = data
@@ -212,6 +217,7 @@ class FMMOperator:
         desc.tree_boxes = self.TREE_BOXES[tree_boxes]
         desc.translations = self.TRANSLATIONS[translations]
         desc.m2l_output_budget = int(m2l_output_budget or 0)
+        desc.point_pairs = self.POINT_PAIRS[point_pairs]
         if self._up is not None:
             desc.up_check_to_equiv, desc.n_up_levels = _dp(self._up), self._up.shape[0]
         if self._down is not None:

@@ -47,7 +47,7 @@ class FmmDesc(ctypes.Structure):
         ("account_for_small_cells", ctypes.c_int), ("precision", ctypes.c_int),
         ("up_check_to_equiv", _D), ("n_up_levels", _SZ),
         ("down_check_to_equiv", _D), ("n_down_levels", _SZ), ("tree_boxes", ctypes.c_int),
-        ("translations", ctypes.c_int), ("m2l_output_budget", _SZ),
+        ("translations", ctypes.c_int), ("m2l_output_budget", _SZ), ("point_pairs", ctypes.c_int),
     ]
 
 
@@ -59,6 +59,7 @@ class FmmInfo(ctypes.Structure):
         ("m2l_classes", ctypes.c_double), ("plan_seconds", ctypes.c_double),
         ("m2l_output_bytes", ctypes.c_double), ("work_bytes", ctypes.c_double),
         ("leaf_wave_items", ctypes.c_int), ("leaf_chunked", ctypes.c_int), ("leaf_block", ctypes.c_int),
+        ("leaf_symmetric", ctypes.c_int), ("symmetric_pairs", ctypes.c_double),
     ]
 
 

@@ -56,6 +56,21 @@ struct FmmPlan {
     size_t order = 0, leaf = 0;
     bool small_cells = false;
     LeafSplit leaf_split;  // of the observation leaves (forward)
+    // Symmetric point pairs (Kelvin U, one shared tree, FP64 forward): the pairs of
+    // neighbouring leaves of kSymLeafMin .. kSymLeafMax points, listed both ways, are
+    // evaluated once for both directions (k_leaf_sym), by the leaf that comes first in
+    // point order. ord: leaf_p2p without them, with its own split; own: per leaf
+    // position, the ranges of the leaves it evaluates for both; base: its first slot
+    // of the reverse sums; g_*: per receiving position, the slots addressed to it.
+    struct SymP2P {
+        bool on = false;
+        DevList ord, own;
+        LeafSplit split;
+        DevBuf<int> base, order, g_pos, g_ptr, g_off;
+        int n_order = 0, n_g = 0, n_leaves = 0;
+        size_t n_rev = 0;
+        double pairs = 0;  // unordered point pairs
+    } sym;
     Surfaces surf;
     int n_e = 0, blk = 0;
     HostTree st, ot;
@@ -160,6 +175,7 @@ struct Work {
     DevBuf<double> fld;     // raw check potentials of one stage, blk per cell (input of the C2E GEMMs)
     DevBuf<double> ty;      // per-pair output of the M2M / L2L class operators
     DevBuf<double> wpart;   // chunk sums of the one-wave leaf kernel (LeafSplit::part_size)
+    DevBuf<double> wrev;    // reverse sums of the symmetric pairs (R rows of FmmPlan::sym.n_rev)
     hipEvent_t ev[12];  // 0..7 stage boundaries; 8, 9 around the M2L translation kernel; 10, 11
                         // around the point-pair (P2P) launch on the side stream
     hipEvent_t done = nullptr;  // end of the last call's work, on whichever stream it ran
@@ -193,7 +209,8 @@ struct Work {
         out_sorted.alloc(P.R * P.n_obs);
         fld.alloc((size_t)std::max(P.dst.n_cells, P.dot.n_cells) * P.blk);
         ty.alloc(P.tr_y);
-        wpart.alloc(P.leaf_split.part_size);
+        wpart.alloc(std::max(P.leaf_split.part_size, P.sym.split.part_size));
+        if (P.sym.on) wrev.alloc((size_t)P.R * P.sym.n_rev);
         if (P.m2l_mfma.enabled && !P.m2l_mfma.fp64) {
             chk_up.alloc((size_t)P.dst.n_cells * P.blk);
             chk_up.zero(stream);  // cells without a P2M/M2M stage keep a zero potential
@@ -787,6 +804,98 @@ static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, con
     out.block.upload(bl, s);
 }
 
+// FmmPlan::sym from the P2P pairs of the shared tree T (forward lists by leaf
+// position; lm: the surface lists the split also weighs). A pair (A, B) of leaves
+// A != B goes to the symmetric evaluation when both hold kSymLeafMin .. kSymLeafMax
+// points and the pairs list it both ways; the leaf with the smaller first point owns
+// it, and B's points take slots base[A] + (their offset in A's owned stream).
+static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::array<int, 2>>& p2p,
+                      const HostList& lm, hipStream_t s) {
+    FmmPlan::SymP2P& Y = P.sym;
+    const std::vector<int>& leaves = T.leaves;
+    const int nl = (int)leaves.size();
+    std::vector<int> pos_of(T.n_cells, -1);
+    for (int k = 0; k < nl; k++) pos_of[leaves[k]] = k;
+    auto fits = [&](int k) {
+        const int m = T.count[leaves[k]];
+        return m >= kSymLeafMin && m <= kSymLeafMax;
+    };
+    const HostList lc = list_from_pairs(leaves, T.n_cells, p2p, 0, nullptr);
+    std::vector<int> srt = lc.cell;  // each position's cells, sorted, for the mirror test
+    parallel_for((size_t)nl, [&](size_t k) { std::sort(srt.begin() + lc.ptr[k], srt.begin() + lc.ptr[k + 1]); });
+    auto listed = [&](int k, int c) {
+        return std::binary_search(srt.begin() + lc.ptr[k], srt.begin() + lc.ptr[k + 1], c);
+    };
+    HostList ord, own;
+    ord.items = own.items = leaves;
+    ord.ptr.assign(1, 0);
+    own.ptr.assign(1, 0);
+    for (int k = 0; k < nl; k++) {
+        const int ca = leaves[k];
+        for (int e = lc.ptr[k]; e < lc.ptr[k + 1]; e++) {
+            const int c = lc.cell[e], j = pos_of[c];
+            const bool both = c != ca && j >= 0 && fits(k) && fits(j) && listed(j, ca);
+            if (!both) ord.cell.push_back(c);
+            else if (T.begin[ca] < T.begin[c]) own.cell.push_back(c);
+        }
+        ord.ptr.push_back((int)ord.cell.size());
+        own.ptr.push_back((int)own.cell.size());
+    }
+    if (own.cell.empty()) return;
+    const std::vector<int> own_cells = own.cell, own_ptr = own.ptr;
+    cells_to_ranges(ord, T);
+    cells_to_ranges(own, T);
+    std::vector<int> base(nl, 0);
+    std::vector<std::pair<long, int>> work;
+    size_t n_rev = 0;
+    for (int k = 0; k < nl; k++) {
+        base[k] = (int)n_rev;
+        const int e0 = own.ptr[k], e1 = own.ptr[k + 1];
+        if (e0 == e1) continue;
+        const long n = (long)own.rpre[e1 - 1] + own.rcnt[e1 - 1];
+        work.push_back({n * T.count[leaves[k]], k});
+        Y.pairs += (double)n * T.count[leaves[k]];
+        n_rev += (size_t)n;
+        TBEM_REQUIRE(n_rev <= (size_t)INT32_MAX, TBEM_E_UNSUPPORTED, "symmetric pairs: too many slots");
+    }
+    // slots addressed to each position, by owner in point order
+    std::vector<std::vector<std::pair<int, int>>> recv(nl);
+    for (int k = 0; k < nl; k++) {
+        const int e0 = own.ptr[k], e1 = own.ptr[k + 1];
+        for (int q = own_ptr[k]; q < own_ptr[k + 1]; q++) {
+            const int c = own_cells[q];
+            if (T.count[c] == 0) continue;
+            const int e = (int)(std::upper_bound(own.rbeg.begin() + e0, own.rbeg.begin() + e1, T.begin[c]) -
+                                own.rbeg.begin()) - 1;
+            recv[pos_of[c]].push_back({T.begin[leaves[k]], base[k] + own.rpre[e] + (T.begin[c] - own.rbeg[e])});
+        }
+    }
+    std::vector<int> g_pos, g_ptr(1, 0), g_off;
+    for (int k = 0; k < nl; k++) {
+        if (recv[k].empty()) continue;
+        std::sort(recv[k].begin(), recv[k].end());
+        g_pos.push_back(k);
+        for (auto& r: recv[k]) g_off.push_back(r.second);
+        g_ptr.push_back((int)g_off.size());
+    }
+    const std::vector<int> order = xcd_layout(work, 1, -1);
+    Y.ord.upload(ord, s);
+    Y.own.upload(own, s);
+    Y.base.upload(base, s);
+    Y.order.upload(order, s);
+    Y.g_pos.upload(g_pos, s);
+    Y.g_ptr.upload(g_ptr, s);
+    Y.g_off.upload(g_off, s);
+    Y.n_order = (int)order.size();
+    Y.n_leaves = (int)work.size();
+    Y.n_g = (int)g_pos.size();
+    Y.n_rev = n_rev;
+    split_leaves(T, leaves, ord, lm, P.n_e, Y.split, s);
+    close_flags(Y.ord, P.obs_leaves.p, P.dot, P.op, P.sp, s);
+    close_flags(Y.own, P.obs_leaves.p, P.dot, P.op, P.sp, s);
+    Y.on = true;
+}
+
 static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s, const SavedPlan* sv = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     PhaseClock clk(s);
@@ -963,6 +1072,9 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         P->leaf_p2p.upload(lp, s);
         P->leaf_m2p.upload(lm, s);
         split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->leaf_split, s);
+        if (P->same_tree && P->kid == TBEM_K_ELASTIC_DISPLACEMENT && P->precision == TBEM_FP64 &&
+            d->point_pairs == TBEM_POINT_PAIRS_AUTO)
+            build_sym(*P, OT, tl.p2p, lm, s);
     }
     close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, s);
 
@@ -1137,6 +1249,20 @@ static void set_split(LeafStage& f, const LeafSplit& sp, Work& W) {
     f.block_pos = sp.block.p; f.n_block = sp.n_block;
 }
 
+// The forward's point pairs through the symmetric evaluation, where the plan has it
+// and the observation points have not moved off the sources.
+static void use_sym(LeafStage& f, const FmmPlan& P, Work& W) {
+    const FmmPlan::SymP2P& Y = P.sym;
+    if (!Y.on || P.obs_moved) return;
+    f.rp_ptr = Y.ord.ptr.p; f.rp_beg = Y.ord.rbeg.p; f.rp_cnt = Y.ord.rcnt.p;
+    f.rp_pre = Y.ord.rpre.p; f.rp_close = Y.ord.rclose.p;
+    set_split(f, Y.split, W);
+    f.sy_ptr = Y.own.ptr.p; f.sy_beg = Y.own.rbeg.p; f.sy_cnt = Y.own.rcnt.p; f.sy_pre = Y.own.rpre.p;
+    f.sy_close = Y.own.rclose.p; f.sy_base = Y.base.p; f.sy_order = Y.order.p; f.n_sy_order = Y.n_order;
+    f.sg_pos = Y.g_pos.p; f.sg_ptr = Y.g_ptr.p; f.sg_off = Y.g_off.p; f.n_sg = Y.n_g;
+    f.rev = W.wrev.p; f.n_rev = Y.n_rev;
+}
+
 // P2P + M2P + L2P per observation leaf (cpp/fmm.cpp:254-294, 332-360, 415-441)
 static LeafStage leaf_stage_args(const FmmPlan& P, Work& W) {
     const int Ns = (int)P.n_src, No = (int)P.n_obs;
@@ -1176,6 +1302,7 @@ static void forward_T(const FmmPlan& P, Work& W, const double* x, double* y, hip
             f.sl_ptr = nullptr;
             f.sl_cell = nullptr;
             f.self_coeff = nullptr;
+            use_sym(f, P, W);
             TBEM_HIP(hipEventRecord(W.ev[10], W.side));
             launch_leaf_stage<T>(P.kid, false, f, P.kc, W.side, true);
             TBEM_HIP(hipEventRecord(W.ev[11], W.side));
@@ -1191,7 +1318,9 @@ static void forward_T(const FmmPlan& P, Work& W, const double* x, double* y, hip
     } else {
         farfield(P, W, x, s, mfma);
         W.far_token = ++W.next_token;
-        launch_leaf_stage<T>(P.kid, false, leaf_stage_args(P, W), P.kc, s, true);
+        LeafStage f = leaf_stage_args(P, W);
+        use_sym(f, P, W);
+        launch_leaf_stage<T>(P.kid, false, f, P.kc, s, true);
     }
     TBEM_HIP(hipEventRecord(W.ev[6], s));
     launch_scatter_out(W.out_sorted.p, P.op.perm.p, No, R, y, s);
@@ -1731,6 +1860,8 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
         o->leaf_wave_items = P.leaf_split.n_work;
         o->leaf_chunked = P.leaf_split.n_parts;
         o->leaf_block = P.leaf_split.n_block_leaves;
+        o->leaf_symmetric = P.sym.n_leaves;
+        o->symmetric_pairs = P.sym.pairs;
     });
 }
 

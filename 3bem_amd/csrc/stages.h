@@ -42,6 +42,10 @@ constexpr int kWaveBlock = 256;        // k_leaf_wave: four work items per workg
 constexpr int kWaveLeafMax = 128;      // leaves of up to this many points go to k_leaf_wave ...
 constexpr long kWavePairsMax = 65536;  // ... one wave per leaf, or per chunk of at most this
                                        // many pairs (points x sources) of a larger one
+constexpr int kSymBlock = 256;         // k_leaf_sym: four waves per leaf
+constexpr int kSymLeafMin = 64;        // leaves of kSymLeafMin .. kSymLeafMax points take part in
+constexpr int kSymLeafMax = 256;       // the symmetric pair evaluation
+constexpr int kSymSlots = kSymLeafMax + 64;  // k_leaf_sym's target slots (the first 64 twice)
 
 // One work item of k_leaf_wave: the leaf at list position pos, whole or one chunk of
 // its sources. Point sources [v0, v1) of the leaf's packed ranges (offsets into their
@@ -160,6 +164,19 @@ struct LeafStage {
     double* part_buf;
     const double* grad_g;   // obs-gradient mode: upstream g (sorted, component-major)
     double* grad_out;       // obs-gradient mode: 3 per target (sorted, AoS)
+    // Symmetric pairs (Kelvin U with one point set on both sides; null: none). Per leaf
+    // position, the point ranges of the leaves whose pairs with it k_leaf_sym
+    // evaluates once for both directions (the rp_* lists then leave those pairs
+    // out): sy_base[pos] is the first slot of the leaf's reverse sums in rev (R
+    // component rows of n_rev); sy_order: the positions that own pairs, one
+    // workgroup each; sg_pos / sg_ptr / sg_off: per receiving position, the slot
+    // offsets (in rev) of the sums addressed to its first point, added in list order.
+    const int *sy_ptr, *sy_beg, *sy_cnt, *sy_pre, *sy_close, *sy_base, *sy_order;
+    int n_sy_order;
+    const int *sg_pos, *sg_ptr, *sg_off;
+    int n_sg;
+    double* rev;
+    size_t n_rev;
 };
 
 // FP64 only: every check potential here feeds a check-to-equivalent pseudo-inverse

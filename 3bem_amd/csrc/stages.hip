@@ -747,6 +747,165 @@ __global__ __launch_bounds__(kWaveLeafMax) void k_leaf_parts(LeafStage a, int R,
     }
 }
 
+// ------------------------------------------------------------ symmetric point pairs
+// Kelvin U is symmetric, U(x <- y) = U(y <- x) (a symmetric 3x3 tensor, even in
+// y - x), so with one point set on both sides a pair (i, j) of neighbouring leaves
+// serves both u_i += U v_j and u_j += U v_i. The workgroup of a leaf (its m points,
+// kSymLeafMin <= m <= kSymLeafMax, the targets) evaluates each pair with the points of
+// the leaves it owns (the sources; every unordered leaf pair has one owner) once. A
+// wave takes 128 sources at a time, two per lane, and visits the m targets in the
+// rotated order t = (lane + k) mod m, so that the 64 lanes of a step update 64
+// distinct targets (m >= 64): the target sums gather per wave in LDS, one LDS add per
+// component and step (a target's adds arrive in instruction order, the same on every
+// run), the source sums in registers over all m targets, which then go to the
+// reverse buffer at the source's slot; k_sym_gather adds those slots in list order.
+struct alignas(16) SymTarget {
+    double x, y, z, v0, v1, v2;
+};
+struct SymSource {
+    double x, y, z, v0, v1, v2, r0, r1, r2;
+};
+
+// U v = C1 / r (C2 v + d (d.v) / r^2), C1 applied by the caller: f (the target's
+// sum, set when SET) gets the source's density, s.r the target's
+template <bool CUT, bool SET>
+__device__ __forceinline__ void sym_pair(SymSource& s, double tx, double ty, double tz, double tv0, double tv1,
+                                         double tv2, double u2, double* f) {
+    const double dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double ri = rsqrt_fast(r2);
+    if (CUT) ri = r2 < cutoff_r2<double>() ? 0.0 : ri;
+    const double ri3 = ri * ri * ri, au = u2 * ri;
+    const double bs = ri3 * (dx * s.v0 + dy * s.v1 + dz * s.v2);
+    const double bt = ri3 * (dx * tv0 + dy * tv1 + dz * tv2);
+    if (SET) {
+        f[0] = fma(au, s.v0, bs * dx);
+        f[1] = fma(au, s.v1, bs * dy);
+        f[2] = fma(au, s.v2, bs * dz);
+    } else {
+        f[0] = fma(bs, dx, fma(au, s.v0, f[0]));
+        f[1] = fma(bs, dy, fma(au, s.v1, f[1]));
+        f[2] = fma(bs, dz, fma(au, s.v2, f[2]));
+    }
+    s.r0 = fma(bt, dx, fma(au, tv0, s.r0));
+    s.r1 = fma(bt, dy, fma(au, tv1, s.r1));
+    s.r2 = fma(bt, dz, fma(au, tv2, s.r2));
+}
+
+// Stream position p of the leaf's owned ranges [e0, e1) into s (beyond n: a point far
+// away with zero density, whose pairs add exact zeros); close |= the range's flag.
+__device__ inline void sym_load(SymSource& s, int p, int n, int e0, int e1, const LeafStage& a, int& close) {
+    s.r0 = s.r1 = s.r2 = 0.0;
+    if (p >= n) {
+        s.x = s.y = s.z = 1e30;
+        s.v0 = s.v1 = s.v2 = 0.0;
+        return;
+    }
+    int lo = e0, hi = e1 - 1;  // the last range starting at or before p
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.sy_pre[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    const int k = a.sy_beg[lo] + (p - a.sy_pre[lo]);
+    close |= a.sy_close ? a.sy_close[lo] : 1;
+    const size_t ns = (size_t)a.n_ps;
+    s.x = a.ps.x[k]; s.y = a.ps.y[k]; s.z = a.ps.z[k];
+    s.v0 = a.pdens[k]; s.v1 = a.pdens[ns + k]; s.v2 = a.pdens[2 * ns + k];
+}
+
+// Step k takes target slot lane + k: the records of targets 0..63 repeat at slots
+// m..m+63, and the sums of slots m.. are folded onto their targets at the end.
+template <bool CUT>
+__device__ __forceinline__ void sym_block(SymSource& s0, SymSource& s1, const SymTarget* tg, double* fa, int m,
+                                          int lane, double u2) {
+    const SymTarget* rec = tg + lane;
+    double* acc = fa + lane;
+    for (int k = 0; k < m; k++) {
+        const double2* p = reinterpret_cast<const double2*>(rec);
+        const double2 A = p[0], B = p[1], C = p[2];
+        double f[3];
+        sym_pair<CUT, true>(s0, A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
+        sym_pair<CUT, false>(s1, A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
+        for (int q = 0; q < 3; q++)
+            __hip_atomic_fetch_add(acc + q * kSymSlots, f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        rec++;
+        acc++;
+    }
+}
+
+__global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<double> kc, double scale) {
+    constexpr int NW = kSymBlock / 64;
+    __shared__ SymTarget tg[kSymSlots];
+    __shared__ double facc[NW][3 * kSymSlots];
+    const int pos = uni(a.sy_order[blockIdx.x]);
+    if (pos < 0) return;  // padding of the layout (the whole workgroup)
+    const int leaf = uni(a.leaves[pos]);
+    const int b = uni(a.tt.begin[leaf]), m = uni(a.tt.count[leaf]);
+    const int e0 = uni(a.sy_ptr[pos]), e1 = uni(a.sy_ptr[pos + 1]);
+    const int n = uni(a.sy_pre[e1 - 1] + a.sy_cnt[e1 - 1]);  // listed positions own at least one range
+    const size_t base = (size_t)uni(a.sy_base[pos]);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const size_t ns = (size_t)a.n_ps;
+    for (int i = tid; i < m + 64; i += kSymBlock) {
+        const int j = b + (i < m ? i : i - m);
+        SymTarget t;
+        t.x = a.tp.x[j]; t.y = a.tp.y[j]; t.z = a.tp.z[j];
+        t.v0 = a.pdens[j]; t.v1 = a.pdens[ns + j]; t.v2 = a.pdens[2 * ns + j];
+        tg[i] = t;
+    }
+    for (int i = tid; i < NW * 3 * kSymSlots; i += kSymBlock) (&facc[0][0])[i] = 0.0;
+    __syncthreads();
+    double* fa = facc[w];
+    for (int p0 = w * 128; p0 < n; p0 += NW * 128) {
+        SymSource s0, s1;
+        int close = 0;
+        sym_load(s0, p0 + lane, n, e0, e1, a, close);
+        sym_load(s1, p0 + 64 + lane, n, e0, e1, a, close);
+        if (__ballot(close) != 0ull) sym_block<true>(s0, s1, tg, fa, m, lane, kc.u2);
+        else sym_block<false>(s0, s1, tg, fa, m, lane, kc.u2);
+        double* r = a.rev + base + p0 + lane;
+        if (p0 + lane < n) {
+            r[0] = s0.r0;
+            r[a.n_rev] = s0.r1;
+            r[2 * a.n_rev] = s0.r2;
+        }
+        if (p0 + 64 + lane < n) {
+            r[64] = s1.r0;
+            r[a.n_rev + 64] = s1.r1;
+            r[2 * a.n_rev + 64] = s1.r2;
+        }
+    }
+    __syncthreads();
+    for (int o = tid; o < 3 * m; o += kSymBlock) {
+        const int q = o / m, i = o - q * m;
+        double s = 0;
+        for (int v = 0; v < NW; v++) {
+            s += facc[v][q * kSymSlots + i];
+            if (i < 64) s += facc[v][q * kSymSlots + m + i];
+        }
+        double* out = a.out_sorted + (size_t)q * a.n_tgt + b + i;
+        *out += s * scale;
+    }
+}
+
+// u of the leaves at sg_pos += the reverse sums addressed to them, slot by slot in
+// list order
+__global__ __launch_bounds__(256) void k_sym_gather(LeafStage a, double scale) {
+    const int pos = a.sg_pos[blockIdx.x];
+    const int leaf = a.leaves[pos];
+    const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
+    const int e0 = a.sg_ptr[blockIdx.x], e1 = a.sg_ptr[blockIdx.x + 1];
+    for (int o = threadIdx.x; o < 3 * m; o += 256) {
+        const int q = o / m, i = o - q * m;
+        const double* r = a.rev + (size_t)q * a.n_rev + i;
+        double s = 0;
+        for (int e = e0; e < e1; e++) s += r[a.sg_off[e]];
+        double* out = a.out_sorted + (size_t)q * a.n_tgt + b + i;
+        *out += s * scale;
+    }
+}
+
 // ------------------------------------------------- observation-coordinate gradient
 // Forward leaf stage (P2P + M2P + L2P) differentiated with respect to the target
 // coordinates; writes grad[3*i + m] = sum_q g_q(i) d u_q(i) / d x_i[m]. This
@@ -1392,14 +1551,28 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
                 TBEM_LAUNCH_CHECK();
             }
         }
-        if (a.n_block == 0) return;
-        LeafStage bl = a;
-        bl.order = a.block_pos;
-        bl.n_order = a.n_block;
-        bl.wave_items = nullptr;
-        bl.block_pos = nullptr;
-        bl.n_wave = bl.n_parts = 0;
-        leaf_launch<KID, T>(adj, fast, bl, c, s);
+        if (a.n_block > 0) {
+            LeafStage bl = a;
+            bl.order = a.block_pos;
+            bl.n_order = a.n_block;
+            bl.wave_items = nullptr;
+            bl.block_pos = nullptr;
+            bl.n_wave = bl.n_parts = 0;
+            bl.sy_order = nullptr;
+            leaf_launch<KID, T>(adj, fast, bl, c, s);
+        }
+        if (a.sy_order && a.rp_ptr) {  // the pairs left out of the rp_* lists, then their reverse sums
+            TBEM_REQUIRE((KID == K_EU && std::is_same<T, double>::value && fast && !adj && !a.t_weight),
+                         TBEM_E_INVALID, "symmetric pairs: Kelvin U forward in FP64 only");
+            if (a.n_sy_order > 0) {
+                hipLaunchKernelGGL(k_leaf_sym, dim3(a.n_sy_order), dim3(kSymBlock), 0, s, a, kc64, scale);
+                TBEM_LAUNCH_CHECK();
+            }
+            if (a.n_sg > 0) {
+                hipLaunchKernelGGL(k_sym_gather, dim3(a.n_sg), dim3(256), 0, s, a, scale);
+                TBEM_LAUNCH_CHECK();
+            }
+        }
         return;
     }
     dim3 g(a.order ? a.n_order : a.n_leaves), b(leaf_block<KID, T>());

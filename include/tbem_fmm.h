@@ -131,10 +131,19 @@ typedef struct {
      * they are processed in slices of whole target cells that fit, with the same
      * forward result bit for bit. */
     size_t m2l_output_budget;
+
+    /* The point pairs of the forward near field (no reference counterpart):
+     * TBEM_POINT_PAIRS_AUTO (0, the default) evaluates each pair of points in
+     * neighbouring leaves once for both directions where the kernel is symmetric
+     * (TBEM_K_ELASTIC_DISPLACEMENT, one point set on both sides, TBEM_FP64, leaves of
+     * 64 to 256 points), as cpp/fmm.cpp:254-294 would twice; TBEM_POINT_PAIRS_ORDERED
+     * evaluates every ordered pair, as the reference does. They differ in rounding only. */
+    int point_pairs;
 } tbem_fmm_desc;
 
 enum { TBEM_TREES_AUTO = 0, TBEM_TREES_OWN_BOXES = 1, TBEM_TREES_COMMON_BOX = 2 };
 enum { TBEM_TRANSLATIONS_AUTO = 0, TBEM_TRANSLATIONS_SURFACES = 1 };
+enum { TBEM_POINT_PAIRS_AUTO = 0, TBEM_POINT_PAIRS_ORDERED = 1 };
 
 /* Plan statistics (tbem_fmm_info). */
 typedef struct {
@@ -153,8 +162,12 @@ typedef struct {
     /* How the forward point-target stage (P2P + M2P + L2P) runs its observation leaves:
        leaf_wave_items one-wave work items (whole small leaves and chunks), of which the
        chunks of leaf_chunked leaves too large for one wave; leaf_block leaves on the
-       one-workgroup kernel (more than 64 points with more pairs than one wave takes). */
-    int leaf_wave_items, leaf_chunked, leaf_block;
+       one-workgroup kernel (more than 64 points with more pairs than one wave takes).
+       leaf_symmetric leaves evaluate symmetric_pairs point pairs of their neighbours'
+       once for both directions (TBEM_POINT_PAIRS_AUTO); the counts above then describe
+       the rest of the pairs. */
+    int leaf_wave_items, leaf_chunked, leaf_block, leaf_symmetric;
+    double symmetric_pairs;
 } tbem_fmm_info_t;
 
 /* ---------------------------------------------------------------- FMM operator */

@@ -666,3 +666,38 @@ def test_gpu_chunked_leaves_obs_grad(tbem, kid):
     em = rel_err(mixed.obs_grad(x, gg), grad)
     print(f"  worst central-difference error {worst:.2e}, mixed vs fp64 {em:.2e}")
     assert em < 1e-5
+
+
+# ------------------------------------------------------------- symmetric point pairs
+@pytest.mark.parametrize("leaf", [100, 256])
+def test_gpu_symmetric_pairs_match_oracle(tbem, orc, leaf):
+    """Kelvin U on one point set: the pairs of neighbouring leaves of 64 to 256 points
+    are evaluated once for both directions (k_leaf_sym) and the reverse sums gathered
+    per leaf in a fixed order. FP64 against the oracle at the parity bar, within 1e-14
+    of point_pairs="ordered" (every ordered pair, as the reference), deterministic; the
+    adjoint and the observation gradient, which keep the ordered pairs, unchanged."""
+    case("symmetric pairs", leaf)
+    g = cluster_and_sparse(5, n_dense=24_000, n_sparse=3000)
+    n = len(g["obs"])
+    K = tbem.make_kernel(3, 1.0, 0.25)
+    d = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
+    cfg = tbem.FMMConfig(0.3, 6, leaf, 0.05, False)
+    op = tbem.FMMOperator(K, d, cfg)
+    ordered = tbem.FMMOperator(K, d, cfg, point_pairs="ordered")
+    inf, inf0 = op.info(), ordered.info()
+    share = 2 * inf["symmetric_pairs"] / inf["p2p_pairs"]
+    print(f"  symmetric leaves {inf['leaf_symmetric']}, {share:.1%} of the ordered point pairs")
+    assert inf["leaf_symmetric"] > 50 and share > 0.3
+    assert inf0["leaf_symmetric"] == 0 and inf0["symmetric_pairs"] == 0
+    rng = np.random.default_rng(leaf)
+    x, gg = rng.standard_normal(3 * n), rng.standard_normal(3 * n)
+    y = op.apply(x)
+    assert np.array_equal(op.apply(x), y)
+    y0 = ordered.apply(x)
+    ref = orc.fmm(3, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.3, 6, leaf, 0.05,
+                  False).apply(x)
+    e, eo = rel_err(y, ref), rel_err(y, y0)
+    assert np.array_equal(op.apply_adjoint(gg), ordered.apply_adjoint(gg))
+    assert np.array_equal(op.obs_grad(x, gg), ordered.obs_grad(x, gg))
+    print(f"  vs oracle {e:.2e}, vs ordered pairs {eo:.2e}")
+    assert e < FP64_TOL and eo < 1e-14
