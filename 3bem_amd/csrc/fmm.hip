@@ -56,10 +56,11 @@ struct FmmPlan {
     size_t order = 0, leaf = 0;
     bool small_cells = false;
     LeafSplit leaf_split;  // of the observation leaves (forward)
-    // Symmetric point pairs (Kelvin U, one shared tree, FP64 forward): the pairs of
-    // neighbouring leaves of kSymLeafMin .. kSymLeafMax points, listed both ways, are
-    // evaluated once for both directions (k_leaf_sym), by the leaf that comes first in
-    // point order. ord: leaf_p2p without them, with its own split; own: per leaf
+    // Symmetric point pairs (Kelvin U, one shared tree, FP64; forward and transpose):
+    // the pairs of neighbouring leaves of kSymLeafMin .. kSymLeafMax points, listed both
+    // ways, are evaluated once for both directions (k_leaf_sym), by the leaf that comes
+    // first in point order. (The mixed path keeps its FP32 ordered pairs, which run
+    // faster per pair than this kernel's FP64 ones.) ord: leaf_p2p without them, with its own split; own: per leaf
     // position, the ranges of the leaves it evaluates for both; base: its first slot
     // of the reverse sums; g_*: per receiving position, the slots addressed to it.
     struct SymP2P {
@@ -70,6 +71,8 @@ struct FmmPlan {
         int n_order = 0, n_g = 0, n_leaves = 0;
         size_t n_rev = 0;
         double pairs = 0;  // unordered point pairs
+        std::vector<int> part_ptr, part;  // per position, its partner leaves (sorted): the
+                                          // adjoint's lists leave them out too
     } sym;
     Surfaces surf;
     int n_e = 0, blk = 0;
@@ -98,6 +101,8 @@ struct FmmPlan {
         std::vector<DevList> m2m;    // by src level L >= 1: cells at L, entry = parent
         DevList leaf_p2p, leaf_p2l;  // indexed by source leaf position
         LeafSplit leaf_split;        // of the source leaves
+        DevList sym_ord;             // leaf_p2p without the symmetric pairs (FmmPlan::sym)
+        LeafSplit sym_split;
         std::vector<CellTiles> t_slev;  // every cell of each source level
     };
     mutable std::mutex adj_mu;
@@ -685,8 +690,7 @@ struct PhaseClock {
 // launch_range_close). The one-wave kernel checks a tile with one ballot; the
 // 512-thread one with a ballot per wave into LDS (measured with flags on every leaf
 // against none: 4M-point Laplace, leaf 100, FP64 leaf stage 40.8 -> 38.1 ms; 1M-point
-// elastic, leaf 64, and 2M points, leaf 256: within 1%).
-constexpr double kCloseMargin = 1e-5;
+// elastic, leaf 64, and 2M points, leaf 256: within 1%). kCloseMargin (stages.h).
 
 // l: a point-range list indexed by the leaves `leaves` of tree tt (points tp), over
 // the source points sp: (re)computes its per-range close flags.
@@ -830,16 +834,25 @@ static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::arra
     ord.items = own.items = leaves;
     ord.ptr.assign(1, 0);
     own.ptr.assign(1, 0);
+    Y.part_ptr.assign(1, 0);
+    Y.part.clear();
     for (int k = 0; k < nl; k++) {
         const int ca = leaves[k];
+        const size_t p0 = Y.part.size();
         for (int e = lc.ptr[k]; e < lc.ptr[k + 1]; e++) {
             const int c = lc.cell[e], j = pos_of[c];
             const bool both = c != ca && j >= 0 && fits(k) && fits(j) && listed(j, ca);
-            if (!both) ord.cell.push_back(c);
-            else if (T.begin[ca] < T.begin[c]) own.cell.push_back(c);
+            if (!both) {
+                ord.cell.push_back(c);
+                continue;
+            }
+            Y.part.push_back(c);
+            if (T.begin[ca] < T.begin[c]) own.cell.push_back(c);
         }
+        std::sort(Y.part.begin() + p0, Y.part.end());
         ord.ptr.push_back((int)ord.cell.size());
         own.ptr.push_back((int)own.cell.size());
+        Y.part_ptr.push_back((int)Y.part.size());
     }
     if (own.cell.empty()) return;
     const std::vector<int> own_cells = own.cell, own_ptr = own.ptr;
@@ -892,7 +905,6 @@ static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::arra
     Y.n_rev = n_rev;
     split_leaves(T, leaves, ord, lm, P.n_e, Y.split, s);
     close_flags(Y.ord, P.obs_leaves.p, P.dot, P.op, P.sp, s);
-    close_flags(Y.own, P.obs_leaves.p, P.dot, P.op, P.sp, s);
     Y.on = true;
 }
 
@@ -1106,6 +1118,22 @@ const FmmPlan::AdjLists& FmmPlan::adjoint_lists(hipStream_t s) const {
         A->leaf_p2p.upload(lp, s);
         A->leaf_p2l.upload(ll, s);
         split_leaves(ST, ST.leaves, lp, ll, n_e, A->leaf_split, s);
+        if (sym.on) {  // one shared tree: source leaf positions are the forward's
+            HostList ord = list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, nullptr);
+            std::vector<int> ptr(1, 0), cell;
+            for (size_t k = 0; k + 1 < ord.ptr.size(); k++) {
+                const auto b = sym.part.begin() + sym.part_ptr[k], e = sym.part.begin() + sym.part_ptr[k + 1];
+                for (int q = ord.ptr[k]; q < ord.ptr[k + 1]; q++)
+                    if (!std::binary_search(b, e, ord.cell[q])) cell.push_back(ord.cell[q]);
+                ptr.push_back((int)cell.size());
+            }
+            ord.ptr = std::move(ptr);
+            ord.cell = std::move(cell);
+            cells_to_ranges(ord, ST);
+            A->sym_ord.upload(ord, s);
+            split_leaves(ST, ST.leaves, ord, ll, n_e, A->sym_split, s);
+            close_flags(A->sym_ord, src_leaves.p, dst, sp, op, s);
+        }
     }
     close_flags(A->leaf_p2p, src_leaves.p, dst, sp, op, s);
     A->t_slev.resize(ST.n_levels);
@@ -1251,14 +1279,17 @@ static void set_split(LeafStage& f, const LeafSplit& sp, Work& W) {
 
 // The forward's point pairs through the symmetric evaluation, where the plan has it
 // and the observation points have not moved off the sources.
-static void use_sym(LeafStage& f, const FmmPlan& P, Work& W) {
+// In the transpose (A non-null) U^T = U: the same pairs, each sum times the source
+// weight of the point it goes to (t_weight).
+static void use_sym(LeafStage& f, const FmmPlan& P, Work& W, const FmmPlan::AdjLists* A = nullptr) {
     const FmmPlan::SymP2P& Y = P.sym;
     if (!Y.on || P.obs_moved) return;
-    f.rp_ptr = Y.ord.ptr.p; f.rp_beg = Y.ord.rbeg.p; f.rp_cnt = Y.ord.rcnt.p;
-    f.rp_pre = Y.ord.rpre.p; f.rp_close = Y.ord.rclose.p;
-    set_split(f, Y.split, W);
+    const DevList& o = A ? A->sym_ord : Y.ord;
+    f.rp_ptr = o.ptr.p; f.rp_beg = o.rbeg.p; f.rp_cnt = o.rcnt.p;
+    f.rp_pre = o.rpre.p; f.rp_close = o.rclose.p;
+    set_split(f, A ? A->sym_split : Y.split, W);
     f.sy_ptr = Y.own.ptr.p; f.sy_beg = Y.own.rbeg.p; f.sy_cnt = Y.own.rcnt.p; f.sy_pre = Y.own.rpre.p;
-    f.sy_close = Y.own.rclose.p; f.sy_base = Y.base.p; f.sy_order = Y.order.p; f.n_sy_order = Y.n_order;
+    f.sy_base = Y.base.p; f.sy_order = Y.order.p; f.n_sy_order = Y.n_order;
     f.sg_pos = Y.g_pos.p; f.sg_ptr = Y.g_ptr.p; f.sg_off = Y.g_off.p; f.n_sg = Y.n_g;
     f.rev = W.wrev.p; f.n_rev = Y.n_rev;
 }
@@ -1414,6 +1445,7 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         f.sl_ptr = nullptr;
         f.sl_cell = nullptr;
         f.self_coeff = nullptr;
+        use_sym(f, P, W, &A);
         launch_leaf_stage<T>(P.kid, true, f, P.kc, W.side, true);
         TBEM_HIP(hipEventRecord(W.join, W.side));
     }
@@ -1496,7 +1528,9 @@ static void adjoint_T(const FmmPlan& P, Work& W, const double* g, double* xbar, 
         f.accumulate = 1;
         launch_leaf_stage<double>(P.kid, true, f, P.kc, s, true);
     } else {
-        launch_leaf_stage<T>(P.kid, true, leaf_args(), P.kc, s, true);
+        LeafStage f = leaf_args();
+        use_sym(f, P, W, &A);
+        launch_leaf_stage<T>(P.kid, true, f, P.kc, s, true);
     }
     launch_scatter_out(W.dens.p, P.sp.perm.p, Ns, R, xbar, s);
 }

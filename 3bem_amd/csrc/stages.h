@@ -46,6 +46,7 @@ constexpr int kSymBlock = 256;         // k_leaf_sym: four waves per leaf
 constexpr int kSymLeafMin = 64;        // leaves of kSymLeafMin .. kSymLeafMax points take part in
 constexpr int kSymLeafMax = 256;       // the symmetric pair evaluation
 constexpr int kSymSlots = kSymLeafMax + 64;  // k_leaf_sym's target slots (the first 64 twice)
+constexpr int kSymSrc = 3;             // k_leaf_sym: sources per lane
 
 // One work item of k_leaf_wave: the leaf at list position pos, whole or one chunk of
 // its sources. Point sources [v0, v1) of the leaf's packed ranges (offsets into their
@@ -171,7 +172,7 @@ struct LeafStage {
     // component rows of n_rev); sy_order: the positions that own pairs, one
     // workgroup each; sg_pos / sg_ptr / sg_off: per receiving position, the slot
     // offsets (in rev) of the sums addressed to its first point, added in list order.
-    const int *sy_ptr, *sy_beg, *sy_cnt, *sy_pre, *sy_close, *sy_base, *sy_order;
+    const int *sy_ptr, *sy_beg, *sy_cnt, *sy_pre, *sy_base, *sy_order;
     int n_sy_order;
     const int *sg_pos, *sg_ptr, *sg_off;
     int n_sg;
@@ -192,6 +193,10 @@ void launch_leaf_stage(int kid, bool adj, const LeafStage& a, const KConst& kc, 
 // point sources in FP32 (mixed precision; kernels with closed-form gradients)
 void launch_leaf_grad(int kid, const LeafStage& a, const KConst& kc, hipStream_t s, bool fp32_points);
 
+
+// A source farther than this from the box of a leaf's points pairs with none of them
+// inside the r2 < 1e-12 cutoff, so the test may be left out (same result).
+constexpr double kCloseMargin = 1e-5;
 
 // For the point-range list l of the targets' leaves (item k = leaf leaves[k] of tree
 // tt, points tp; ranges of source points sp): close[e] = 1 when a point of range e

@@ -753,12 +753,14 @@ __global__ __launch_bounds__(kWaveLeafMax) void k_leaf_parts(LeafStage a, int R,
 // serves both u_i += U v_j and u_j += U v_i. The workgroup of a leaf (its m points,
 // kSymLeafMin <= m <= kSymLeafMax, the targets) evaluates each pair with the points of
 // the leaves it owns (the sources; every unordered leaf pair has one owner) once. A
-// wave takes 128 sources at a time, two per lane, and visits the m targets in the
+// wave takes 64 x kSymSrc sources at a time, kSymSrc per lane, and visits the m targets in the
 // rotated order t = (lane + k) mod m, so that the 64 lanes of a step update 64
 // distinct targets (m >= 64): the target sums gather per wave in LDS, one LDS add per
 // component and step (a target's adds arrive in instruction order, the same on every
 // run), the source sums in registers over all m targets, which then go to the
 // reverse buffer at the source's slot; k_sym_gather adds those slots in list order.
+// The transpose runs the same pairs (U^T = U) with each sum times t_weight, the
+// source weight of the point it goes to.
 struct alignas(16) SymTarget {
     double x, y, z, v0, v1, v2;
 };
@@ -793,43 +795,58 @@ __device__ __forceinline__ void sym_pair(SymSource& s, double tx, double ty, dou
 }
 
 // Stream position p of the leaf's owned ranges [e0, e1) into s (beyond n: a point far
-// away with zero density, whose pairs add exact zeros); close |= the range's flag.
-__device__ inline void sym_load(SymSource& s, int p, int n, int e0, int e1, const LeafStage& a, int& close) {
-    s.r0 = s.r1 = s.r2 = 0.0;
-    if (p >= n) {
-        s.x = s.y = s.z = 1e30;
-        s.v0 = s.v1 = s.v2 = 0.0;
-        return;
-    }
+// away with zero density, whose pairs add exact zeros); close |= the point lies within
+// kCloseMargin of the box of the leaf's points (box: lo x, y, z, hi x, y, z), where a
+// pair might fall inside the r2 < 1e-12 cutoff. (Straight-line: a position past n
+// reads point 0 and then takes the far point.)
+__device__ inline void sym_load(SymSource& s, int p, int n, int e0, int e1, const LeafStage& a,
+                                const double* box, int& close) {
+    const bool ok = p < n;
     int lo = e0, hi = e1 - 1;  // the last range starting at or before p
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (a.sy_pre[mid] <= p) lo = mid;
         else hi = mid - 1;
     }
-    const int k = a.sy_beg[lo] + (p - a.sy_pre[lo]);
-    close |= a.sy_close ? a.sy_close[lo] : 1;
+    const int k = ok ? a.sy_beg[lo] + (p - a.sy_pre[lo]) : 0;
     const size_t ns = (size_t)a.n_ps;
-    s.x = a.ps.x[k]; s.y = a.ps.y[k]; s.z = a.ps.z[k];
-    s.v0 = a.pdens[k]; s.v1 = a.pdens[ns + k]; s.v2 = a.pdens[2 * ns + k];
+    const double x = a.ps.x[k], y = a.ps.y[k], z = a.ps.z[k];
+    const double gx = fmax(fmax(box[0] - x, x - box[3]), 0.0), gy = fmax(fmax(box[1] - y, y - box[4]), 0.0),
+                 gz = fmax(fmax(box[2] - z, z - box[5]), 0.0);
+    if (ok && gx * gx + gy * gy + gz * gz <= kCloseMargin * kCloseMargin) close = 1;
+    const double v0 = a.pdens[k], v1 = a.pdens[ns + k], v2 = a.pdens[2 * ns + k];
+    s.x = ok ? x : 1e30;
+    s.y = ok ? y : 1e30;
+    s.z = ok ? z : 1e30;
+    s.v0 = ok ? v0 : 0.0;
+    s.v1 = ok ? v1 : 0.0;
+    s.v2 = ok ? v2 : 0.0;
+    s.r0 = 0.0;
+    s.r1 = 0.0;
+    s.r2 = 0.0;
 }
 
 // Step k takes target slot lane + k: the records of targets 0..63 repeat at slots
 // m..m+63, and the sums of slots m.. are folded onto their targets at the end.
 template <bool CUT>
-__device__ __forceinline__ void sym_block(SymSource& s0, SymSource& s1, const SymTarget* tg, double* fa, int m,
-                                          int lane, double u2) {
-    const SymTarget* rec = tg + lane;
+__device__ __forceinline__ void sym_block(SymSource* sv, const SymTarget* tg, double* fa, int m, int lane,
+                                          double u2) {
+    const double2* rec = reinterpret_cast<const double2*>(tg + lane);
     double* acc = fa + lane;
+    double2 nA = rec[0], nB = rec[1], nC = rec[2];  // the next step's record, read a step ahead
     for (int k = 0; k < m; k++) {
-        const double2* p = reinterpret_cast<const double2*>(rec);
-        const double2 A = p[0], B = p[1], C = p[2];
+        const double2 A = nA, B = nB, C = nC;
+        rec += 3;  // read while k + 1 < m: slot lane + k + 1 <= m + 62
+        if (k + 1 < m) {
+            nA = rec[0];
+            nB = rec[1];
+            nC = rec[2];
+        }
         double f[3];
-        sym_pair<CUT, true>(s0, A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
-        sym_pair<CUT, false>(s1, A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
+        sym_pair<CUT, true>(sv[0], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
+        for (int u = 1; u < kSymSrc; u++) sym_pair<CUT, false>(sv[u], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
         for (int q = 0; q < 3; q++)
             __hip_atomic_fetch_add(acc + q * kSymSlots, f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        rec++;
         acc++;
     }
 }
@@ -855,25 +872,47 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<double> 
         tg[i] = t;
     }
     for (int i = tid; i < NW * 3 * kSymSlots; i += kSymBlock) (&facc[0][0])[i] = 0.0;
+    // the box of the leaf's points (one per thread: m <= kSymBlock)
+    static_assert(kSymLeafMax <= kSymBlock, "one target per thread");
+    __shared__ double wbox[NW][6];
+    double bx[6];
+    {
+        const int i = tid < m ? b + tid : b;
+        bx[0] = bx[3] = a.tp.x[i];
+        bx[1] = bx[4] = a.tp.y[i];
+        bx[2] = bx[5] = a.tp.z[i];
+        for (int off = 32; off > 0; off >>= 1)
+            for (int c = 0; c < 3; c++) {
+                bx[c] = fmin(bx[c], __shfl_xor(bx[c], off));
+                bx[3 + c] = fmax(bx[3 + c], __shfl_xor(bx[3 + c], off));
+            }
+        if (lane == 0)
+            for (int c = 0; c < 6; c++) wbox[w][c] = bx[c];
+    }
     __syncthreads();
-    double* fa = facc[w];
-    for (int p0 = w * 128; p0 < n; p0 += NW * 128) {
-        SymSource s0, s1;
-        int close = 0;
-        sym_load(s0, p0 + lane, n, e0, e1, a, close);
-        sym_load(s1, p0 + 64 + lane, n, e0, e1, a, close);
-        if (__ballot(close) != 0ull) sym_block<true>(s0, s1, tg, fa, m, lane, kc.u2);
-        else sym_block<false>(s0, s1, tg, fa, m, lane, kc.u2);
-        double* r = a.rev + base + p0 + lane;
-        if (p0 + lane < n) {
-            r[0] = s0.r0;
-            r[a.n_rev] = s0.r1;
-            r[2 * a.n_rev] = s0.r2;
+    for (int c = 0; c < 3; c++) {
+        bx[c] = wbox[0][c];
+        bx[3 + c] = wbox[0][3 + c];
+        for (int v = 1; v < NW; v++) {
+            bx[c] = fmin(bx[c], wbox[v][c]);
+            bx[3 + c] = fmax(bx[3 + c], wbox[v][3 + c]);
         }
-        if (p0 + 64 + lane < n) {
-            r[64] = s1.r0;
-            r[a.n_rev + 64] = s1.r1;
-            r[2 * a.n_rev + 64] = s1.r2;
+    }
+    double* fa = facc[w];
+    constexpr int PB = 64 * kSymSrc;  // stream positions per wave block
+    for (int p0 = w * PB; p0 < n; p0 += NW * PB) {
+        SymSource sv[kSymSrc];
+        int close = 0;
+        for (int u = 0; u < kSymSrc; u++) sym_load(sv[u], p0 + 64 * u + lane, n, e0, e1, a, bx, close);
+        if (__ballot(close) != 0ull) sym_block<true>(sv, tg, fa, m, lane, kc.u2);
+        else sym_block<false>(sv, tg, fa, m, lane, kc.u2);
+        for (int u = 0; u < kSymSrc; u++) {
+            const int p = p0 + 64 * u + lane;
+            if (p >= n) continue;
+            double* r = a.rev + base + p;
+            r[0] = sv[u].r0;
+            r[a.n_rev] = sv[u].r1;
+            r[2 * a.n_rev] = sv[u].r2;
         }
     }
     __syncthreads();
@@ -884,8 +923,9 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<double> 
             s += facc[v][q * kSymSlots + i];
             if (i < 64) s += facc[v][q * kSymSlots + m + i];
         }
+        const double wgt = a.t_weight ? a.t_weight[b + i] : 1.0;
         double* out = a.out_sorted + (size_t)q * a.n_tgt + b + i;
-        *out += s * scale;
+        *out += wgt * (s * scale);
     }
 }
 
@@ -900,9 +940,16 @@ __global__ __launch_bounds__(256) void k_sym_gather(LeafStage a, double scale) {
         const int q = o / m, i = o - q * m;
         const double* r = a.rev + (size_t)q * a.n_rev + i;
         double s = 0;
-        for (int e = e0; e < e1; e++) s += r[a.sg_off[e]];
+        int e = e0;
+        for (; e + 8 <= e1; e += 8) {  // eight loads in flight, added in list order
+            double v[8];
+            for (int u = 0; u < 8; u++) v[u] = r[a.sg_off[e + u]];
+            for (int u = 0; u < 8; u++) s += v[u];
+        }
+        for (; e < e1; e++) s += r[a.sg_off[e]];
+        const double wgt = a.t_weight ? a.t_weight[b + i] : 1.0;
         double* out = a.out_sorted + (size_t)q * a.n_tgt + b + i;
-        *out += s * scale;
+        *out += wgt * (s * scale);
     }
 }
 
@@ -1562,8 +1609,7 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
             leaf_launch<KID, T>(adj, fast, bl, c, s);
         }
         if (a.sy_order && a.rp_ptr) {  // the pairs left out of the rp_* lists, then their reverse sums
-            TBEM_REQUIRE((KID == K_EU && std::is_same<T, double>::value && fast && !adj && !a.t_weight),
-                         TBEM_E_INVALID, "symmetric pairs: Kelvin U forward in FP64 only");
+            TBEM_REQUIRE(KID == K_EU && fast, TBEM_E_INVALID, "symmetric pairs: Kelvin U only");
             if (a.n_sy_order > 0) {
                 hipLaunchKernelGGL(k_leaf_sym, dim3(a.n_sy_order), dim3(kSymBlock), 0, s, a, kc64, scale);
                 TBEM_LAUNCH_CHECK();

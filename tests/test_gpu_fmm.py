@@ -675,7 +675,9 @@ def test_gpu_symmetric_pairs_match_oracle(tbem, orc, leaf):
     are evaluated once for both directions (k_leaf_sym) and the reverse sums gathered
     per leaf in a fixed order. FP64 against the oracle at the parity bar, within 1e-14
     of point_pairs="ordered" (every ordered pair, as the reference), deterministic; the
-    adjoint and the observation gradient, which keep the ordered pairs, unchanged."""
+    transpose (the same pairs, weighted on the receiving side) within 1e-14 of the
+    ordered one and by the adjoint identity; the observation gradient, which keeps the
+    ordered pairs, unchanged."""
     case("symmetric pairs", leaf)
     g = cluster_and_sparse(5, n_dense=24_000, n_sparse=3000)
     n = len(g["obs"])
@@ -697,7 +699,16 @@ def test_gpu_symmetric_pairs_match_oracle(tbem, orc, leaf):
     ref = orc.fmm(3, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.3, 6, leaf, 0.05,
                   False).apply(x)
     e, eo = rel_err(y, ref), rel_err(y, y0)
-    assert np.array_equal(op.apply_adjoint(gg), ordered.apply_adjoint(gg))
+    xt = op.apply_adjoint(gg)
+    assert np.array_equal(op.apply_adjoint(gg), xt)
+    et = rel_err(xt, ordered.apply_adjoint(gg))
+    ea = abs(y @ gg - x @ xt) / (np.abs(y).max() * np.abs(gg).sum() + np.abs(x).max() * np.abs(xt).sum())
     assert np.array_equal(op.obs_grad(x, gg), ordered.obs_grad(x, gg))
-    print(f"  vs oracle {e:.2e}, vs ordered pairs {eo:.2e}")
-    assert e < FP64_TOL and eo < 1e-14
+    print(f"  vs oracle {e:.2e}, vs ordered pairs {eo:.2e}, transpose vs ordered {et:.2e}, adjoint identity {ea:.2e}")
+    assert e < FP64_TOL and eo < 1e-14 and et < 1e-14 and ea < 1e-13
+    # mixed precision keeps its FP32 ordered pairs
+    mixed = tbem.FMMOperator(K, d, cfg, "mixed")
+    assert mixed.info()["leaf_symmetric"] == 0
+    em, emt = rel_err(mixed.apply(x), y), rel_err(mixed.apply_adjoint(gg), xt)
+    print(f"  mixed vs fp64 {em:.2e}, transpose {emt:.2e}")
+    assert em < 1e-5 and emt < 1e-5
