@@ -829,26 +829,30 @@ __device__ inline void sym_load(SymSource& s, int p, int n, int e0, int e1, cons
 // Step k takes target slot lane + k: the records of targets 0..63 repeat at slots
 // m..m+63, and the sums of slots m.. are folded onto their targets at the end.
 template <bool CUT>
+__device__ __forceinline__ void sym_step(SymSource* sv, double2 A, double2 B, double2 C, double* acc, double u2) {
+    double f[3];
+    sym_pair<CUT, true>(sv[0], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
+    for (int u = 1; u < kSymSrc; u++) sym_pair<CUT, false>(sv[u], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
+    for (int q = 0; q < 3; q++)
+        __hip_atomic_fetch_add(acc + q * kSymSlots, f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Two steps per iteration, both records read first: LDS operations complete in
+// order, so a read issued after a step's adds waits for them; this waits once per two.
+template <bool CUT>
 __device__ __forceinline__ void sym_block(SymSource* sv, const SymTarget* tg, double* fa, int m, int lane,
                                           double u2) {
     const double2* rec = reinterpret_cast<const double2*>(tg + lane);
     double* acc = fa + lane;
-    double2 nA = rec[0], nB = rec[1], nC = rec[2];  // the next step's record, read a step ahead
-    for (int k = 0; k < m; k++) {
-        const double2 A = nA, B = nB, C = nC;
-        rec += 3;  // read while k + 1 < m: slot lane + k + 1 <= m + 62
-        if (k + 1 < m) {
-            nA = rec[0];
-            nB = rec[1];
-            nC = rec[2];
-        }
-        double f[3];
-        sym_pair<CUT, true>(sv[0], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
-        for (int u = 1; u < kSymSrc; u++) sym_pair<CUT, false>(sv[u], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
-        for (int q = 0; q < 3; q++)
-            __hip_atomic_fetch_add(acc + q * kSymSlots, f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        acc++;
+    int k = 0;
+    for (; k + 2 <= m; k += 2) {
+        const double2 A0 = rec[0], B0 = rec[1], C0 = rec[2], A1 = rec[3], B1 = rec[4], C1 = rec[5];
+        sym_step<CUT>(sv, A0, B0, C0, acc, u2);
+        sym_step<CUT>(sv, A1, B1, C1, acc + 1, u2);
+        rec += 6;
+        acc += 2;
     }
+    if (k < m) sym_step<CUT>(sv, rec[0], rec[1], rec[2], acc, u2);
 }
 
 __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<double> kc, double scale) {

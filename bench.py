@@ -36,12 +36,16 @@ sys.path.insert(0, ROOT)
 METRIC = "FMM matvec Mpts/s at stated rel err, FP32-MFMA vs FP64; bwd/fwd time ratio"
 # algorithmic flops per source-target pair (FMA = 2, rsqrt = 1), per kernel id
 FLOPS_PER_PAIR = {0: 11, 1: 19, 2: 0, 3: 28, 4: 60, 5: 60, 6: 120, 7: 2, 8: 6}
+# Kelvin U evaluated once for both directions (k_leaf_sym): 12 shared (displacement,
+# r^2, rsqrt, r^-3, C2/r) + 18 per direction (d.v, its factor, 3 x 2 FMA)
+FLOPS_PER_SYM_PAIR = 48
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per launch of a workload's leaf and M2L kernels: rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE of the same command (mean over its dispatches; the leaf
-# stage's are summed over its kernels, k_leaf_wave + k_leaf_parts + k_leaf_stage,
+# stage's are summed over its kernels, k_leaf_wave + k_leaf_parts + k_leaf_stage
+# (+ k_leaf_sym + k_sym_gather where pairs are evaluated once for both directions),
 # for the fused launches the rooflines time), scaled by what the counters read for
 # a plain device copy (factors 2.0 and 1.0; the profiles under profiles/ give the
 # runs). Keyed by (workload, points, precision, kernel); other workloads report null.
@@ -406,12 +410,17 @@ def fmm_rooflines(info, n, kid, prec, leaf_ms, m2l_ms, wname=None):
     flop = FLOPS_PER_PAIR[kid] * (info["p2p_pairs"] + n * n_e)
     ach = flop / (leaf_ms * 1e-3) / 1e12
     # the leaf stage's kernels: one wave per small leaf or chunk of a large one (k_leaf_wave,
-    # k_leaf_parts adding the chunks), one workgroup per remaining leaf (k_leaf_stage)
+    # k_leaf_parts adding the chunks), one workgroup per remaining leaf (k_leaf_stage), and
+    # where Kelvin U's pairs of neighbouring leaves run once for both directions,
+    # k_leaf_sym and k_sym_gather (the reverse sums)
     t = "double" if prec == "fp64" else "float"
+    sym = info.get("symmetric_pairs", 0.0)
     parts = ([f"k_leaf_wave<{t}>"] if info["leaf_wave_items"] else []) + \
-        (["k_leaf_parts"] if info["leaf_chunked"] else []) + ([f"k_leaf_stage<{t}>"] if info["leaf_block"] else [])
+        (["k_leaf_parts"] if info["leaf_chunked"] else []) + ([f"k_leaf_stage<{t}>"] if info["leaf_block"] else []) + \
+        (["k_leaf_sym", "k_sym_gather"] if sym else [])
     split = {"wave_items": info["leaf_wave_items"], "chunked_leaves": info["leaf_chunked"],
-             "workgroup_leaves": info["leaf_block"]}
+             "workgroup_leaves": info["leaf_block"], "symmetric_leaves": info.get("leaf_symmetric", 0),
+             "symmetric_pairs": sym}
     if prec == "fp64":
         out["leaf"] = {"bound": "mfma", "pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak)",
                        "kernel": " + ".join(parts) + " (P2P+M2P+L2P)", "peak": PEAK_FP64_TFLOPS}
@@ -423,6 +432,13 @@ def fmm_rooflines(info, n, kid, prec, leaf_ms, m2l_ms, wname=None):
     out["leaf"].update({"achieved": round(ach, 3), "unit": "TFLOP/s", "frac": round(ach / out["leaf"]["peak"], 4),
                         "kernel_avg_ms": round(leaf_ms, 4), "algorithmic_flop": flop,
                         "flop_rule": f"{FLOPS_PER_PAIR[kid]} flop per (target, source) pair: P2P pairs + L2P pairs"})
+    if sym:
+        # the arithmetic the kernels execute: each symmetric pair once for both directions
+        ex = flop - FLOPS_PER_PAIR[kid] * 2 * sym + FLOPS_PER_SYM_PAIR * sym
+        out["leaf"].update({"executed_flop": ex, "executed_frac": round(ex / (leaf_ms * 1e-3) / 1e12 /
+                                                                          out["leaf"]["peak"], 4),
+                            "executed_rule": f"{FLOPS_PER_SYM_PAIR} flop per symmetric pair (both directions), "
+                                             f"{FLOPS_PER_PAIR[kid]} per other pair"})
     if m2l_ms > 0:
         if info["m2l_classes"] == 0:
             flop = FLOPS_PER_PAIR[kid] * info["m2l"] * n_e * n_e
