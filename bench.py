@@ -52,7 +52,7 @@ PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 TRAFFIC_BYTES = {
     ("fault_surface", 1_000_000, "fp64", "leaf"): 334_100_000,
     ("fault_surface_o100", 2_000_000, "fp64", "m2l"): 15_485_747_200,
-    ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 1_261_500_000,
+    ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 4_213_500_000,
     ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 1_270_400_000,
     ("fault_surface_o100", 2_000_000, "mixed", "m2l"): 7_941_620_238,
 }
