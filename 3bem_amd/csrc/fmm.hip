@@ -1887,7 +1887,7 @@ int tbem_fmm_info(tbem_fmm_t h, tbem_fmm_info_t* o) {
         double wb = 0;
         for (const DevBuf<double>* b : {&W.dens, &W.mult, &W.locals, &W.out_sorted, &W.xbuf, &W.ybuf, &W.gsort,
                                         &W.lbar, &W.cbar, &W.mbar, &W.ebar, &W.grad_sorted, &W.e2, &W.fld, &W.ty,
-                                        &W.wpart})
+                                        &W.wpart, &W.wrev})
             wb += (double)b->n * sizeof(double);
         for (const DevBuf<float>* b : {&W.chk_up, &W.m2l_w}) wb += (double)b->n * sizeof(float);
         o->work_bytes = wb;
