@@ -1084,8 +1084,7 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         P->leaf_p2p.upload(lp, s);
         P->leaf_m2p.upload(lm, s);
         split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->leaf_split, s);
-        if (P->same_tree && P->kid == TBEM_K_ELASTIC_DISPLACEMENT && P->precision == TBEM_FP64 &&
-            d->point_pairs == TBEM_POINT_PAIRS_AUTO)
+        if (P->same_tree && P->kid == TBEM_K_ELASTIC_DISPLACEMENT && d->point_pairs == TBEM_POINT_PAIRS_AUTO)
             build_sym(*P, OT, tl.p2p, lm, s);
     }
     close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, s);

@@ -764,22 +764,26 @@ __global__ __launch_bounds__(kWaveLeafMax) void k_leaf_parts(LeafStage a, int R,
 struct alignas(16) SymTarget {
     double x, y, z, v0, v1, v2;
 };
+// (T: the pair arithmetic; positions stay FP64, so the displacement is exact before
+// it is rounded to T, as in the other point-pair kernels)
+template <class T>
 struct SymSource {
-    double x, y, z, v0, v1, v2, r0, r1, r2;
+    double x, y, z;
+    T v0, v1, v2, r0, r1, r2;
 };
 
 // U v = C1 / r (C2 v + d (d.v) / r^2), C1 applied by the caller: f (the target's
 // sum, set when SET) gets the source's density, s.r the target's
-template <bool CUT, bool SET>
-__device__ __forceinline__ void sym_pair(SymSource& s, double tx, double ty, double tz, double tv0, double tv1,
-                                         double tv2, double u2, double* f) {
-    const double dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    double ri = rsqrt_fast(r2);
-    if (CUT) ri = r2 < cutoff_r2<double>() ? 0.0 : ri;
-    const double ri3 = ri * ri * ri, au = u2 * ri;
-    const double bs = ri3 * (dx * s.v0 + dy * s.v1 + dz * s.v2);
-    const double bt = ri3 * (dx * tv0 + dy * tv1 + dz * tv2);
+template <class T, bool CUT, bool SET>
+__device__ __forceinline__ void sym_pair(SymSource<T>& s, double tx, double ty, double tz, T tv0, T tv1, T tv2,
+                                         T u2, T* f) {
+    const T dx = T(s.x - tx), dy = T(s.y - ty), dz = T(s.z - tz);
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    T ri = rsqrt_fast(r2);
+    if (CUT) ri = r2 < cutoff_r2<T>() ? T(0) : ri;
+    const T ri3 = ri * ri * ri, au = u2 * ri;
+    const T bs = ri3 * (dx * s.v0 + dy * s.v1 + dz * s.v2);
+    const T bt = ri3 * (dx * tv0 + dy * tv1 + dz * tv2);
     if (SET) {
         f[0] = fma(au, s.v0, bs * dx);
         f[1] = fma(au, s.v1, bs * dy);
@@ -799,7 +803,8 @@ __device__ __forceinline__ void sym_pair(SymSource& s, double tx, double ty, dou
 // kCloseMargin of the box of the leaf's points (box: lo x, y, z, hi x, y, z), where a
 // pair might fall inside the r2 < 1e-12 cutoff. (Straight-line: a position past n
 // reads point 0 and then takes the far point.)
-__device__ inline void sym_load(SymSource& s, int p, int n, int e0, int e1, const LeafStage& a,
+template <class T>
+__device__ inline void sym_load(SymSource<T>& s, int p, int n, int e0, int e1, const LeafStage& a,
                                 const double* box, int& close) {
     const bool ok = p < n;
     int lo = e0, hi = e1 - 1;  // the last range starting at or before p
@@ -818,44 +823,46 @@ __device__ inline void sym_load(SymSource& s, int p, int n, int e0, int e1, cons
     s.x = ok ? x : 1e30;
     s.y = ok ? y : 1e30;
     s.z = ok ? z : 1e30;
-    s.v0 = ok ? v0 : 0.0;
-    s.v1 = ok ? v1 : 0.0;
-    s.v2 = ok ? v2 : 0.0;
-    s.r0 = 0.0;
-    s.r1 = 0.0;
-    s.r2 = 0.0;
+    s.v0 = ok ? T(v0) : T(0);
+    s.v1 = ok ? T(v1) : T(0);
+    s.v2 = ok ? T(v2) : T(0);
+    s.r0 = T(0);
+    s.r1 = T(0);
+    s.r2 = T(0);
 }
 
 // Step k takes target slot lane + k: the records of targets 0..63 repeat at slots
-// m..m+63, and the sums of slots m.. are folded onto their targets at the end.
-template <bool CUT>
-__device__ __forceinline__ void sym_step(SymSource* sv, double2 A, double2 B, double2 C, double* acc, double u2) {
-    double f[3];
-    sym_pair<CUT, true>(sv[0], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
-    for (int u = 1; u < kSymSrc; u++) sym_pair<CUT, false>(sv[u], A.x, A.y, B.x, B.y, C.x, C.y, u2, f);
+// m..m+63, and the sums of slots m.. are folded onto their targets at the end. The
+// target sums are FP64 in LDS whatever T is.
+template <class T, bool CUT>
+__device__ __forceinline__ void sym_step(SymSource<T>* sv, double2 A, double2 B, double2 C, double* acc, T u2) {
+    const T tv0 = T(B.y), tv1 = T(C.x), tv2 = T(C.y);
+    T f[3];
+    sym_pair<T, CUT, true>(sv[0], A.x, A.y, B.x, tv0, tv1, tv2, u2, f);
+    for (int u = 1; u < kSymSrc; u++) sym_pair<T, CUT, false>(sv[u], A.x, A.y, B.x, tv0, tv1, tv2, u2, f);
     for (int q = 0; q < 3; q++)
-        __hip_atomic_fetch_add(acc + q * kSymSlots, f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(acc + q * kSymSlots, (double)f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Two steps per iteration, both records read first: LDS operations complete in
 // order, so a read issued after a step's adds waits for them; this waits once per two.
-template <bool CUT>
-__device__ __forceinline__ void sym_block(SymSource* sv, const SymTarget* tg, double* fa, int m, int lane,
-                                          double u2) {
+template <class T, bool CUT>
+__device__ __forceinline__ void sym_block(SymSource<T>* sv, const SymTarget* tg, double* fa, int m, int lane, T u2) {
     const double2* rec = reinterpret_cast<const double2*>(tg + lane);
     double* acc = fa + lane;
     int k = 0;
     for (; k + 2 <= m; k += 2) {
         const double2 A0 = rec[0], B0 = rec[1], C0 = rec[2], A1 = rec[3], B1 = rec[4], C1 = rec[5];
-        sym_step<CUT>(sv, A0, B0, C0, acc, u2);
-        sym_step<CUT>(sv, A1, B1, C1, acc + 1, u2);
+        sym_step<T, CUT>(sv, A0, B0, C0, acc, u2);
+        sym_step<T, CUT>(sv, A1, B1, C1, acc + 1, u2);
         rec += 6;
         acc += 2;
     }
-    if (k < m) sym_step<CUT>(sv, rec[0], rec[1], rec[2], acc, u2);
+    if (k < m) sym_step<T, CUT>(sv, rec[0], rec[1], rec[2], acc, u2);
 }
 
-__global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<double> kc, double scale) {
+template <class T>
+__global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, double scale) {
     constexpr int NW = kSymBlock / 64;
     __shared__ SymTarget tg[kSymSlots];
     __shared__ double facc[NW][3 * kSymSlots];
@@ -905,18 +912,18 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<double> 
     double* fa = facc[w];
     constexpr int PB = 64 * kSymSrc;  // stream positions per wave block
     for (int p0 = w * PB; p0 < n; p0 += NW * PB) {
-        SymSource sv[kSymSrc];
+        SymSource<T> sv[kSymSrc];
         int close = 0;
         for (int u = 0; u < kSymSrc; u++) sym_load(sv[u], p0 + 64 * u + lane, n, e0, e1, a, bx, close);
-        if (__ballot(close) != 0ull) sym_block<true>(sv, tg, fa, m, lane, kc.u2);
-        else sym_block<false>(sv, tg, fa, m, lane, kc.u2);
+        if (__ballot(close) != 0ull) sym_block<T, true>(sv, tg, fa, m, lane, kc.u2);
+        else sym_block<T, false>(sv, tg, fa, m, lane, kc.u2);
         for (int u = 0; u < kSymSrc; u++) {
             const int p = p0 + 64 * u + lane;
             if (p >= n) continue;
             double* r = a.rev + base + p;
-            r[0] = sv[u].r0;
-            r[a.n_rev] = sv[u].r1;
-            r[2 * a.n_rev] = sv[u].r2;
+            r[0] = (double)sv[u].r0;
+            r[a.n_rev] = (double)sv[u].r1;
+            r[2 * a.n_rev] = (double)sv[u].r2;
         }
     }
     __syncthreads();
@@ -1615,7 +1622,7 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
         if (a.sy_order && a.rp_ptr) {  // the pairs left out of the rp_* lists, then their reverse sums
             TBEM_REQUIRE(KID == K_EU && fast, TBEM_E_INVALID, "symmetric pairs: Kelvin U only");
             if (a.n_sy_order > 0) {
-                hipLaunchKernelGGL(k_leaf_sym, dim3(a.n_sy_order), dim3(kSymBlock), 0, s, a, kc64, scale);
+                hipLaunchKernelGGL(k_leaf_sym<T>, dim3(a.n_sy_order), dim3(kSymBlock), 0, s, a, kc, scale);
                 TBEM_LAUNCH_CHECK();
             }
             if (a.n_sg > 0) {
