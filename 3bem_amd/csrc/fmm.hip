@@ -56,11 +56,10 @@ struct FmmPlan {
     size_t order = 0, leaf = 0;
     bool small_cells = false;
     LeafSplit leaf_split;  // of the observation leaves (forward)
-    // Symmetric point pairs (Kelvin U, one shared tree, FP64; forward and transpose):
-    // the pairs of neighbouring leaves of kSymLeafMin .. kSymLeafMax points, listed both
-    // ways, are evaluated once for both directions (k_leaf_sym), by the leaf that comes
-    // first in point order. (The mixed path keeps its FP32 ordered pairs, which run
-    // faster per pair than this kernel's FP64 ones.) ord: leaf_p2p without them, with its own split; own: per leaf
+    // Symmetric point pairs (Kelvin U, one shared tree; forward and transpose): the
+    // pairs of neighbouring leaves of kSymLeafMin .. kSymLeafMax points, listed both
+    // ways, are evaluated once for both directions (k_leaf_sym, in the precision's pair
+    // arithmetic), by the leaf that comes first in point order. ord: leaf_p2p without them, with its own split; own: per leaf
     // position, the ranges of the leaves it evaluates for both; base: its first slot
     // of the reverse sums; g_*: per receiving position, the slots addressed to it.
     struct SymP2P {

@@ -53,7 +53,7 @@ TRAFFIC_BYTES = {
     ("fault_surface", 1_000_000, "fp64", "leaf"): 334_100_000,
     ("fault_surface_o100", 2_000_000, "fp64", "m2l"): 15_485_747_200,
     ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 4_213_500_000,
-    ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 1_270_400_000,
+    ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 4_216_000_000,
     ("fault_surface_o100", 2_000_000, "mixed", "m2l"): 7_941_620_238,
 }
 TRAFFIC_UNIT = "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)"
