@@ -56,10 +56,11 @@ struct FmmPlan {
     size_t order = 0, leaf = 0;
     bool small_cells = false;
     LeafSplit leaf_split;  // of the observation leaves (forward)
-    // Symmetric point pairs (Kelvin U, one shared tree; forward and transpose): the
-    // pairs of neighbouring leaves of kSymLeafMin .. kSymLeafMax points, listed both
-    // ways, are evaluated once for both directions (k_leaf_sym, in the precision's pair
-    // arithmetic), by the leaf that comes first in point order. ord: leaf_p2p without them, with its own split; own: per leaf
+    // Symmetric point pairs (Kelvin U or the Laplace single layer, one shared tree;
+    // forward and transpose): the pairs of neighbouring leaves of kSymLeafMin ..
+    // kSymLeafMax points, listed both ways, are evaluated once for both directions
+    // (k_leaf_sym, in the precision's pair arithmetic), by the leaf that comes first in
+    // point order. ord: leaf_p2p without them, with its own split; own: per leaf
     // position, the ranges of the leaves it evaluates for both; base: its first slot
     // of the reverse sums; g_*: per receiving position, the slots addressed to it.
     struct SymP2P {
@@ -1083,7 +1084,8 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         P->leaf_p2p.upload(lp, s);
         P->leaf_m2p.upload(lm, s);
         split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->leaf_split, s);
-        if (P->same_tree && P->kid == TBEM_K_ELASTIC_DISPLACEMENT && d->point_pairs == TBEM_POINT_PAIRS_AUTO)
+        if (P->same_tree && (P->kid == TBEM_K_ELASTIC_DISPLACEMENT || P->kid == TBEM_K_LAPLACE_SINGLE) &&
+            d->point_pairs == TBEM_POINT_PAIRS_AUTO)
             build_sym(*P, OT, tl.p2p, lm, s);
     }
     close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, s);

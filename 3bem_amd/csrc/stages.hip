@@ -749,8 +749,8 @@ __global__ __launch_bounds__(kWaveLeafMax) void k_leaf_parts(LeafStage a, int R,
 
 // ------------------------------------------------------------ symmetric point pairs
 // Kelvin U is symmetric, U(x <- y) = U(y <- x) (a symmetric 3x3 tensor, even in
-// y - x), so with one point set on both sides a pair (i, j) of neighbouring leaves
-// serves both u_i += U v_j and u_j += U v_i. The workgroup of a leaf (its m points,
+// y - x), and so is the Laplace single layer 1/r, so with one point set on both sides
+// a pair (i, j) of neighbouring leaves serves both u_i += U v_j and u_j += U v_i. The workgroup of a leaf (its m points,
 // kSymLeafMin <= m <= kSymLeafMax, the targets) evaluates each pair with the points of
 // the leaves it owns (the sources; every unordered leaf pair has one owner) once. A
 // wave takes 64 x kSymSrc sources at a time, kSymSrc per lane, and visits the m targets in the
@@ -774,13 +774,20 @@ struct SymSource {
 
 // U v = C1 / r (C2 v + d (d.v) / r^2), C1 applied by the caller: f (the target's
 // sum, set when SET) gets the source's density, s.r the target's
-template <class T, bool CUT, bool SET>
+// (K_LS: u = v / r, 1/(4 pi) applied by the caller)
+template <int KID, class T, bool CUT, bool SET>
 __device__ __forceinline__ void sym_pair(SymSource<T>& s, double tx, double ty, double tz, T tv0, T tv1, T tv2,
                                          T u2, T* f) {
     const T dx = T(s.x - tx), dy = T(s.y - ty), dz = T(s.z - tz);
     const T r2 = dx * dx + dy * dy + dz * dz;
     T ri = rsqrt_fast(r2);
     if (CUT) ri = r2 < cutoff_r2<T>() ? T(0) : ri;
+    if constexpr (KID == K_LS) {
+        if (SET) f[0] = ri * s.v0;
+        else f[0] = fma(ri, s.v0, f[0]);
+        s.r0 = fma(ri, tv0, s.r0);
+        return;
+    }
     const T ri3 = ri * ri * ri, au = u2 * ri;
     const T bs = ri3 * (dx * s.v0 + dy * s.v1 + dz * s.v2);
     const T bt = ri3 * (dx * tv0 + dy * tv1 + dz * tv2);
@@ -803,7 +810,7 @@ __device__ __forceinline__ void sym_pair(SymSource<T>& s, double tx, double ty, 
 // kCloseMargin of the box of the leaf's points (box: lo x, y, z, hi x, y, z), where a
 // pair might fall inside the r2 < 1e-12 cutoff. (Straight-line: a position past n
 // reads point 0 and then takes the far point.)
-template <class T>
+template <int R, class T>
 __device__ inline void sym_load(SymSource<T>& s, int p, int n, int e0, int e1, const LeafStage& a,
                                 const double* box, int& close) {
     const bool ok = p < n;
@@ -819,7 +826,7 @@ __device__ inline void sym_load(SymSource<T>& s, int p, int n, int e0, int e1, c
     const double gx = fmax(fmax(box[0] - x, x - box[3]), 0.0), gy = fmax(fmax(box[1] - y, y - box[4]), 0.0),
                  gz = fmax(fmax(box[2] - z, z - box[5]), 0.0);
     if (ok && gx * gx + gy * gy + gz * gz <= kCloseMargin * kCloseMargin) close = 1;
-    const double v0 = a.pdens[k], v1 = a.pdens[ns + k], v2 = a.pdens[2 * ns + k];
+    const double v0 = a.pdens[k], v1 = R > 1 ? a.pdens[ns + k] : 0.0, v2 = R > 1 ? a.pdens[2 * ns + k] : 0.0;
     s.x = ok ? x : 1e30;
     s.y = ok ? y : 1e30;
     s.z = ok ? z : 1e30;
@@ -834,36 +841,38 @@ __device__ inline void sym_load(SymSource<T>& s, int p, int n, int e0, int e1, c
 // Step k takes target slot lane + k: the records of targets 0..63 repeat at slots
 // m..m+63, and the sums of slots m.. are folded onto their targets at the end. The
 // target sums are FP64 in LDS whatever T is.
-template <class T, bool CUT>
+template <int KID, class T, bool CUT>
 __device__ __forceinline__ void sym_step(SymSource<T>* sv, double2 A, double2 B, double2 C, double* acc, T u2) {
-    const T tv0 = T(B.y), tv1 = T(C.x), tv2 = T(C.y);
+    constexpr int R = kernel_dim(KID);
+    const T tv0 = T(B.y), tv1 = R > 1 ? T(C.x) : T(0), tv2 = R > 1 ? T(C.y) : T(0);
     T f[3];
-    sym_pair<T, CUT, true>(sv[0], A.x, A.y, B.x, tv0, tv1, tv2, u2, f);
-    for (int u = 1; u < kSymSrc; u++) sym_pair<T, CUT, false>(sv[u], A.x, A.y, B.x, tv0, tv1, tv2, u2, f);
-    for (int q = 0; q < 3; q++)
+    sym_pair<KID, T, CUT, true>(sv[0], A.x, A.y, B.x, tv0, tv1, tv2, u2, f);
+    for (int u = 1; u < kSymSrc; u++) sym_pair<KID, T, CUT, false>(sv[u], A.x, A.y, B.x, tv0, tv1, tv2, u2, f);
+    for (int q = 0; q < R; q++)
         __hip_atomic_fetch_add(acc + q * kSymSlots, (double)f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Two steps per iteration, both records read first: LDS operations complete in
 // order, so a read issued after a step's adds waits for them; this waits once per two.
-template <class T, bool CUT>
+template <int KID, class T, bool CUT>
 __device__ __forceinline__ void sym_block(SymSource<T>* sv, const SymTarget* tg, double* fa, int m, int lane, T u2) {
     const double2* rec = reinterpret_cast<const double2*>(tg + lane);
     double* acc = fa + lane;
     int k = 0;
     for (; k + 2 <= m; k += 2) {
         const double2 A0 = rec[0], B0 = rec[1], C0 = rec[2], A1 = rec[3], B1 = rec[4], C1 = rec[5];
-        sym_step<T, CUT>(sv, A0, B0, C0, acc, u2);
-        sym_step<T, CUT>(sv, A1, B1, C1, acc + 1, u2);
+        sym_step<KID, T, CUT>(sv, A0, B0, C0, acc, u2);
+        sym_step<KID, T, CUT>(sv, A1, B1, C1, acc + 1, u2);
         rec += 6;
         acc += 2;
     }
-    if (k < m) sym_step<T, CUT>(sv, rec[0], rec[1], rec[2], acc, u2);
+    if (k < m) sym_step<KID, T, CUT>(sv, rec[0], rec[1], rec[2], acc, u2);
 }
 
-template <class T>
+template <int KID, class T>
 __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, double scale) {
     constexpr int NW = kSymBlock / 64;
+    constexpr int R = kernel_dim(KID);
     __shared__ SymTarget tg[kSymSlots];
     __shared__ double facc[NW][3 * kSymSlots];
     const int pos = uni(a.sy_order[blockIdx.x]);
@@ -879,7 +888,9 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, d
         const int j = b + (i < m ? i : i - m);
         SymTarget t;
         t.x = a.tp.x[j]; t.y = a.tp.y[j]; t.z = a.tp.z[j];
-        t.v0 = a.pdens[j]; t.v1 = a.pdens[ns + j]; t.v2 = a.pdens[2 * ns + j];
+        t.v0 = a.pdens[j];
+        t.v1 = R > 1 ? a.pdens[ns + j] : 0.0;
+        t.v2 = R > 1 ? a.pdens[2 * ns + j] : 0.0;
         tg[i] = t;
     }
     for (int i = tid; i < NW * 3 * kSymSlots; i += kSymBlock) (&facc[0][0])[i] = 0.0;
@@ -914,20 +925,22 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, d
     for (int p0 = w * PB; p0 < n; p0 += NW * PB) {
         SymSource<T> sv[kSymSrc];
         int close = 0;
-        for (int u = 0; u < kSymSrc; u++) sym_load(sv[u], p0 + 64 * u + lane, n, e0, e1, a, bx, close);
-        if (__ballot(close) != 0ull) sym_block<T, true>(sv, tg, fa, m, lane, kc.u2);
-        else sym_block<T, false>(sv, tg, fa, m, lane, kc.u2);
+        for (int u = 0; u < kSymSrc; u++) sym_load<R>(sv[u], p0 + 64 * u + lane, n, e0, e1, a, bx, close);
+        if (__ballot(close) != 0ull) sym_block<KID, T, true>(sv, tg, fa, m, lane, kc.u2);
+        else sym_block<KID, T, false>(sv, tg, fa, m, lane, kc.u2);
         for (int u = 0; u < kSymSrc; u++) {
             const int p = p0 + 64 * u + lane;
             if (p >= n) continue;
             double* r = a.rev + base + p;
             r[0] = (double)sv[u].r0;
-            r[a.n_rev] = (double)sv[u].r1;
-            r[2 * a.n_rev] = (double)sv[u].r2;
+            if (R > 1) {
+                r[a.n_rev] = (double)sv[u].r1;
+                r[2 * a.n_rev] = (double)sv[u].r2;
+            }
         }
     }
     __syncthreads();
-    for (int o = tid; o < 3 * m; o += kSymBlock) {
+    for (int o = tid; o < R * m; o += kSymBlock) {
         const int q = o / m, i = o - q * m;
         double s = 0;
         for (int v = 0; v < NW; v++) {
@@ -942,12 +955,12 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, d
 
 // u of the leaves at sg_pos += the reverse sums addressed to them, slot by slot in
 // list order
-__global__ __launch_bounds__(256) void k_sym_gather(LeafStage a, double scale) {
+__global__ __launch_bounds__(256) void k_sym_gather(LeafStage a, int R, double scale) {
     const int pos = a.sg_pos[blockIdx.x];
     const int leaf = a.leaves[pos];
     const int b = a.tt.begin[leaf], m = a.tt.count[leaf];
     const int e0 = a.sg_ptr[blockIdx.x], e1 = a.sg_ptr[blockIdx.x + 1];
-    for (int o = threadIdx.x; o < 3 * m; o += 256) {
+    for (int o = threadIdx.x; o < R * m; o += 256) {
         const int q = o / m, i = o - q * m;
         const double* r = a.rev + (size_t)q * a.n_rev + i;
         double s = 0;
@@ -1620,13 +1633,16 @@ static void leaf_launch(bool adj, bool fast, const LeafStage& a, const KConst& c
             leaf_launch<KID, T>(adj, fast, bl, c, s);
         }
         if (a.sy_order && a.rp_ptr) {  // the pairs left out of the rp_* lists, then their reverse sums
-            TBEM_REQUIRE(KID == K_EU && fast, TBEM_E_INVALID, "symmetric pairs: Kelvin U only");
+            TBEM_REQUIRE((KID == K_EU || KID == K_LS) && fast, TBEM_E_INVALID,
+                         "symmetric pairs: Kelvin U and the Laplace single layer only");
             if (a.n_sy_order > 0) {
-                hipLaunchKernelGGL(k_leaf_sym<T>, dim3(a.n_sy_order), dim3(kSymBlock), 0, s, a, kc, scale);
+                if constexpr (KID == K_EU || KID == K_LS)
+                    hipLaunchKernelGGL((k_leaf_sym<KID, T>), dim3(a.n_sy_order), dim3(kSymBlock), 0, s, a, kc,
+                                       scale);
                 TBEM_LAUNCH_CHECK();
             }
             if (a.n_sg > 0) {
-                hipLaunchKernelGGL(k_sym_gather, dim3(a.n_sg), dim3(256), 0, s, a, scale);
+                hipLaunchKernelGGL(k_sym_gather, dim3(a.n_sg), dim3(256), 0, s, a, kernel_dim(KID), scale);
                 TBEM_LAUNCH_CHECK();
             }
         }

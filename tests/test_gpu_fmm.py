@@ -669,19 +669,21 @@ def test_gpu_chunked_leaves_obs_grad(tbem, kid):
 
 
 # ------------------------------------------------------------- symmetric point pairs
-@pytest.mark.parametrize("leaf", [100, 256])
-def test_gpu_symmetric_pairs_match_oracle(tbem, orc, leaf):
-    """Kelvin U on one point set: the pairs of neighbouring leaves of 64 to 256 points
+@pytest.mark.parametrize("kid,leaf", [(3, 100), (3, 256), (0, 100), (0, 256)])
+def test_gpu_symmetric_pairs_match_oracle(tbem, orc, kid, leaf):
+    """Kelvin U (kid 3) and the Laplace single layer (kid 0) on one point set: the
+    pairs of neighbouring leaves of 64 to 256 points
     are evaluated once for both directions (k_leaf_sym) and the reverse sums gathered
     per leaf in a fixed order. FP64 against the oracle at the parity bar, within 1e-14
     of point_pairs="ordered" (every ordered pair, as the reference), deterministic; the
     transpose (the same pairs, weighted on the receiving side) within 1e-14 of the
     ordered one and by the adjoint identity; the observation gradient, which keeps the
     ordered pairs, unchanged."""
-    case("symmetric pairs", leaf)
+    case("symmetric pairs", kid, leaf)
     g = cluster_and_sparse(5, n_dense=24_000, n_sparse=3000)
     n = len(g["obs"])
-    K = tbem.make_kernel(3, 1.0, 0.25)
+    R = components(kid)
+    K = tbem.make_kernel(kid, 1.0, 0.25)
     d = tbem.NBodyData(g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"])
     cfg = tbem.FMMConfig(0.3, 6, leaf, 0.05, False)
     op = tbem.FMMOperator(K, d, cfg)
@@ -692,11 +694,11 @@ def test_gpu_symmetric_pairs_match_oracle(tbem, orc, leaf):
     assert inf["leaf_symmetric"] > 50 and share > 0.3
     assert inf0["leaf_symmetric"] == 0 and inf0["symmetric_pairs"] == 0
     rng = np.random.default_rng(leaf)
-    x, gg = rng.standard_normal(3 * n), rng.standard_normal(3 * n)
+    x, gg = rng.standard_normal(R * n), rng.standard_normal(R * n)
     y = op.apply(x)
     assert np.array_equal(op.apply(x), y)
     y0 = ordered.apply(x)
-    ref = orc.fmm(3, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.3, 6, leaf, 0.05,
+    ref = orc.fmm(kid, g["obs"], g["obs_normals"], g["src"], g["src_normals"], g["weights"], 0.3, 6, leaf, 0.05,
                   False).apply(x)
     e, eo = rel_err(y, ref), rel_err(y, y0)
     xt = op.apply_adjoint(gg)
