@@ -36,9 +36,10 @@ sys.path.insert(0, ROOT)
 METRIC = "FMM matvec Mpts/s at stated rel err, FP32-MFMA vs FP64; bwd/fwd time ratio"
 # algorithmic flops per source-target pair (FMA = 2, rsqrt = 1), per kernel id
 FLOPS_PER_PAIR = {0: 11, 1: 19, 2: 0, 3: 28, 4: 60, 5: 60, 6: 120, 7: 2, 8: 6}
-# Kelvin U evaluated once for both directions (k_leaf_sym): 12 shared (displacement,
-# r^2, rsqrt, r^-3, C2/r) + 18 per direction (d.v, its factor, 3 x 2 FMA)
-FLOPS_PER_SYM_PAIR = 48
+# a pair evaluated once for both directions (k_leaf_sym), per kernel id: Kelvin U 12
+# shared (displacement, r^2, rsqrt, r^-3, C2/r) + 18 per direction (d.v, its factor,
+# 3 x 2 FMA); Laplace single layer 9 shared (displacement, r^2, rsqrt) + 2 per direction
+FLOPS_PER_SYM_PAIR = {0: 13, 3: 48}
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
@@ -434,10 +435,10 @@ def fmm_rooflines(info, n, kid, prec, leaf_ms, m2l_ms, wname=None):
                         "flop_rule": f"{FLOPS_PER_PAIR[kid]} flop per (target, source) pair: P2P pairs + L2P pairs"})
     if sym:
         # the arithmetic the kernels execute: each symmetric pair once for both directions
-        ex = flop - FLOPS_PER_PAIR[kid] * 2 * sym + FLOPS_PER_SYM_PAIR * sym
+        ex = flop - FLOPS_PER_PAIR[kid] * 2 * sym + FLOPS_PER_SYM_PAIR[kid] * sym
         out["leaf"].update({"executed_flop": ex, "executed_frac": round(ex / (leaf_ms * 1e-3) / 1e12 /
                                                                           out["leaf"]["peak"], 4),
-                            "executed_rule": f"{FLOPS_PER_SYM_PAIR} flop per symmetric pair (both directions), "
+                            "executed_rule": f"{FLOPS_PER_SYM_PAIR[kid]} flop per symmetric pair (both directions), "
                                              f"{FLOPS_PER_PAIR[kid]} per other pair"})
     if m2l_ms > 0:
         if info["m2l_classes"] == 0:
