@@ -183,7 +183,8 @@ class FMMOperator:
     beyond it the pairs run in slices.
     point_pairs: "auto" (the forward evaluates each pair of points in neighbouring
     leaves once for both directions where the kernel is symmetric: elastic
-    displacement or Laplace single layer, one point set on both sides, leaves of 64 to 256 points; forward and
+    displacement or Laplace single layer, one point set on both sides, leaves of
+    64 (Laplace: 16) to 256 points; forward and
     transpose) or "ordered" (every ordered pair, as the reference).
     """
 

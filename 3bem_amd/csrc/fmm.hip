@@ -810,8 +810,8 @@ static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, con
 
 // FmmPlan::sym from the P2P pairs of the shared tree T (forward lists by leaf
 // position; lm: the surface lists the split also weighs). A pair (A, B) of leaves
-// A != B goes to the symmetric evaluation when both hold kSymLeafMin .. kSymLeafMax
-// points and the pairs list it both ways; the leaf with the smaller first point owns
+// A != B goes to the symmetric evaluation when both hold lo .. kSymLeafMax points
+// (lo below) and the pairs list it both ways; the leaf with the smaller first point owns
 // it, and B's points take slots base[A] + (their offset in A's owned stream).
 static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::array<int, 2>>& p2p,
                       const HostList& lm, hipStream_t s) {
@@ -820,9 +820,15 @@ static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::arra
     const int nl = (int)leaves.size();
     std::vector<int> pos_of(T.n_cells, -1);
     for (int k = 0; k < nl; k++) pos_of[leaves[k]] = k;
+    // Below 64 points a leaf's pairs pay for the Laplace single layer (4M points, cube +
+    // clusters, leaf 100: FP64 matvec 71.1 -> 66.7 ms with leaves from 16 points, mixed
+    // 47.4 -> 45.6) but not for Kelvin U (1M points, fault + surface, leaf 64: FP64
+    // 2.90 -> 3.15 ms, mixed 2.30 -> 2.75; from 32 points 3.09 and 2.51), whose one-wave
+    // ordered kernel serves small leaves better.
+    const int lo = P.kid == TBEM_K_LAPLACE_SINGLE ? kSymLeafMin : 64;
     auto fits = [&](int k) {
         const int m = T.count[leaves[k]];
-        return m >= kSymLeafMin && m <= kSymLeafMax;
+        return m >= lo && m <= kSymLeafMax;
     };
     const HostList lc = list_from_pairs(leaves, T.n_cells, p2p, 0, nullptr);
     std::vector<int> srt = lc.cell;  // each position's cells, sorted, for the mirror test

@@ -43,9 +43,9 @@ constexpr int kWaveLeafMax = 128;      // leaves of up to this many points go to
 constexpr long kWavePairsMax = 65536;  // ... one wave per leaf, or per chunk of at most this
                                        // many pairs (points x sources) of a larger one
 constexpr int kSymBlock = 256;         // k_leaf_sym: four waves per leaf
-constexpr int kSymLeafMin = 64;        // leaves of kSymLeafMin .. kSymLeafMax points take part in
-constexpr int kSymLeafMax = 256;       // the symmetric pair evaluation
-constexpr int kSymSlots = kSymLeafMax + 64;  // k_leaf_sym's target slots (the first 64 twice)
+constexpr int kSymLeafMin = 16;        // k_leaf_sym takes leaves of kSymLeafMin .. kSymLeafMax
+constexpr int kSymLeafMax = 256;       // points (the plan chooses which, per kernel)
+constexpr int kSymSlots = kSymLeafMax + 64;  // k_leaf_sym's target slots per wave and component
 constexpr int kSymSrc = 3;             // k_leaf_sym: sources per lane
 
 // One work item of k_leaf_wave: the leaf at list position pos, whole or one chunk of

@@ -750,15 +750,17 @@ __global__ __launch_bounds__(kWaveLeafMax) void k_leaf_parts(LeafStage a, int R,
 // ------------------------------------------------------------ symmetric point pairs
 // Kelvin U is symmetric, U(x <- y) = U(y <- x) (a symmetric 3x3 tensor, even in
 // y - x), and so is the Laplace single layer 1/r, so with one point set on both sides
-// a pair (i, j) of neighbouring leaves serves both u_i += U v_j and u_j += U v_i. The workgroup of a leaf (its m points,
-// kSymLeafMin <= m <= kSymLeafMax, the targets) evaluates each pair with the points of
-// the leaves it owns (the sources; every unordered leaf pair has one owner) once. A
-// wave takes 64 x kSymSrc sources at a time, kSymSrc per lane, and visits the m targets in the
-// rotated order t = (lane + k) mod m, so that the 64 lanes of a step update 64
-// distinct targets (m >= 64): the target sums gather per wave in LDS, one LDS add per
-// component and step (a target's adds arrive in instruction order, the same on every
-// run), the source sums in registers over all m targets, which then go to the
-// reverse buffer at the source's slot; k_sym_gather adds those slots in list order.
+// a pair (i, j) of neighbouring leaves serves both u_i += U v_j and u_j += U v_i. The
+// workgroup of a leaf (its m points, kSymLeafMin <= m <= kSymLeafMax, the targets)
+// evaluates each pair with the points of the leaves it owns (the sources; every
+// unordered leaf pair has one owner) once. A wave takes 64 x kSymSrc sources at a
+// time, kSymSrc per lane, and visits the m targets in the rotated order
+// t = (r + k) mod m, r = lane mod m: the lanes of one group (lane / m; one group when
+// m >= 64) update distinct targets at every step, and each group has its own copy of
+// the wave's target sums in LDS (one LDS add per component and step; a slot's adds
+// arrive in instruction order, the same on every run); the source sums stay in
+// registers over all m targets, then go to the reverse buffer at the source's slot;
+// k_sym_gather adds those slots in list order.
 // The transpose runs the same pairs (U^T = U) with each sum times t_weight, the
 // source weight of the point it goes to.
 struct alignas(16) SymTarget {
@@ -838,9 +840,9 @@ __device__ inline void sym_load(SymSource<T>& s, int p, int n, int e0, int e1, c
     s.r2 = T(0);
 }
 
-// Step k takes target slot lane + k: the records of targets 0..63 repeat at slots
-// m..m+63, and the sums of slots m.. are folded onto their targets at the end. The
-// target sums are FP64 in LDS whatever T is.
+// Step k takes slot r + k of the records and of the group's sums: records repeat the
+// targets past slot m (slot s holds target s mod m), and the sums of slots m.. are
+// folded onto their targets at the end. The target sums are FP64 in LDS whatever T is.
 template <int KID, class T, bool CUT>
 __device__ __forceinline__ void sym_step(SymSource<T>* sv, double2 A, double2 B, double2 C, double* acc, T u2) {
     constexpr int R = kernel_dim(KID);
@@ -855,9 +857,9 @@ __device__ __forceinline__ void sym_step(SymSource<T>* sv, double2 A, double2 B,
 // Two steps per iteration, both records read first: LDS operations complete in
 // order, so a read issued after a step's adds waits for them; this waits once per two.
 template <int KID, class T, bool CUT>
-__device__ __forceinline__ void sym_block(SymSource<T>* sv, const SymTarget* tg, double* fa, int m, int lane, T u2) {
-    const double2* rec = reinterpret_cast<const double2*>(tg + lane);
-    double* acc = fa + lane;
+__device__ __forceinline__ void sym_block(SymSource<T>* sv, const SymTarget* tg, double* fa, int m, int r, T u2) {
+    const double2* rec = reinterpret_cast<const double2*>(tg + r);
+    double* acc = fa + r;
     int k = 0;
     for (; k + 2 <= m; k += 2) {
         const double2 A0 = rec[0], B0 = rec[1], C0 = rec[2], A1 = rec[3], B1 = rec[4], C1 = rec[5];
@@ -883,8 +885,12 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, d
     const int n = uni(a.sy_pre[e1 - 1] + a.sy_cnt[e1 - 1]);  // listed positions own at least one range
     const size_t base = (size_t)uni(a.sy_base[pos]);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // slots per group copy (r + k < S), the copies, and this lane's group and offset:
+    // C * S <= 128 + 2m <= kSymSlots below 64 targets, S = m + 64 <= kSymSlots above
+    const int S = m < 64 ? 2 * m : m + 64, C = m < 64 ? (64 + m - 1) / m : 1;
+    const int gc = lane / m, gr = lane - gc * m;
     const size_t ns = (size_t)a.n_ps;
-    for (int i = tid; i < m + 64; i += kSymBlock) {
+    for (int i = tid; i < S; i += kSymBlock) {
         const int j = b + (i < m ? i : i - m);
         SymTarget t;
         t.x = a.tp.x[j]; t.y = a.tp.y[j]; t.z = a.tp.z[j];
@@ -920,14 +926,14 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, d
             bx[3 + c] = fmax(bx[3 + c], wbox[v][3 + c]);
         }
     }
-    double* fa = facc[w];
+    double* fa = facc[w] + gc * S;
     constexpr int PB = 64 * kSymSrc;  // stream positions per wave block
     for (int p0 = w * PB; p0 < n; p0 += NW * PB) {
         SymSource<T> sv[kSymSrc];
         int close = 0;
         for (int u = 0; u < kSymSrc; u++) sym_load<R>(sv[u], p0 + 64 * u + lane, n, e0, e1, a, bx, close);
-        if (__ballot(close) != 0ull) sym_block<KID, T, true>(sv, tg, fa, m, lane, kc.u2);
-        else sym_block<KID, T, false>(sv, tg, fa, m, lane, kc.u2);
+        if (__ballot(close) != 0ull) sym_block<KID, T, true>(sv, tg, fa, m, gr, kc.u2);
+        else sym_block<KID, T, false>(sv, tg, fa, m, gr, kc.u2);
         for (int u = 0; u < kSymSrc; u++) {
             const int p = p0 + 64 * u + lane;
             if (p >= n) continue;
@@ -943,10 +949,12 @@ __global__ __launch_bounds__(kSymBlock) void k_leaf_sym(LeafStage a, KC<T> kc, d
     for (int o = tid; o < R * m; o += kSymBlock) {
         const int q = o / m, i = o - q * m;
         double s = 0;
-        for (int v = 0; v < NW; v++) {
-            s += facc[v][q * kSymSlots + i];
-            if (i < 64) s += facc[v][q * kSymSlots + m + i];
-        }
+        for (int v = 0; v < NW; v++)
+            for (int c = 0; c < C; c++) {
+                const double* f = &facc[v][q * kSymSlots + c * S];
+                s += f[i];
+                if (i + m < S) s += f[i + m];
+            }
         const double wgt = a.t_weight ? a.t_weight[b + i] : 1.0;
         double* out = a.out_sorted + (size_t)q * a.n_tgt + b + i;
         *out += wgt * (s * scale);

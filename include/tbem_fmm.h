@@ -136,8 +136,8 @@ typedef struct {
      * TBEM_POINT_PAIRS_AUTO (0, the default) evaluates each pair of points in
      * neighbouring leaves once for both directions where the kernel is symmetric
      * (TBEM_K_ELASTIC_DISPLACEMENT and TBEM_K_LAPLACE_SINGLE, one point set on both
-     * sides, leaves of 64 to 256 points; forward and transpose, not the observation gradient; FP32 pair arithmetic
-     * in TBEM_MIXED), where
+     * sides, leaves of 64 (Laplace: 16) to 256 points; forward and transpose, not the
+     * observation gradient; FP32 pair arithmetic in TBEM_MIXED), where
      * cpp/fmm.cpp:254-294 evaluates it twice; TBEM_POINT_PAIRS_ORDERED evaluates every
      * ordered pair, as the reference does. They differ in rounding only. */
     int point_pairs;

@@ -669,10 +669,11 @@ def test_gpu_chunked_leaves_obs_grad(tbem, kid):
 
 
 # ------------------------------------------------------------- symmetric point pairs
-@pytest.mark.parametrize("kid,leaf", [(3, 100), (3, 256), (0, 100), (0, 256)])
+@pytest.mark.parametrize("kid,leaf", [(3, 32), (3, 100), (3, 256), (0, 32), (0, 100), (0, 256)])
 def test_gpu_symmetric_pairs_match_oracle(tbem, orc, kid, leaf):
     """Kelvin U (kid 3) and the Laplace single layer (kid 0) on one point set: the
-    pairs of neighbouring leaves of 64 to 256 points
+    pairs of neighbouring leaves of 16 to 256 points (below 64, several lane groups
+    per wave with their own target sums)
     are evaluated once for both directions (k_leaf_sym) and the reverse sums gathered
     per leaf in a fixed order. FP64 against the oracle at the parity bar, within 1e-14
     of point_pairs="ordered" (every ordered pair, as the reference), deterministic; the
@@ -691,7 +692,10 @@ def test_gpu_symmetric_pairs_match_oracle(tbem, orc, kid, leaf):
     inf, inf0 = op.info(), ordered.info()
     share = 2 * inf["symmetric_pairs"] / inf["p2p_pairs"]
     print(f"  symmetric leaves {inf['leaf_symmetric']}, {share:.1%} of the ordered point pairs")
-    assert inf["leaf_symmetric"] > 50 and share > 0.3
+    if kid == 3 and leaf < 64:  # Kelvin U's leaves below 64 points keep their ordered pairs
+        assert inf["leaf_symmetric"] == 0
+    else:
+        assert inf["leaf_symmetric"] > 50 and share > (0.03 if leaf < 64 else 0.3)
     assert inf0["leaf_symmetric"] == 0 and inf0["symmetric_pairs"] == 0
     rng = np.random.default_rng(leaf)
     x, gg = rng.standard_normal(R * n), rng.standard_normal(R * n)
