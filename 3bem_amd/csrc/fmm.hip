@@ -812,9 +812,10 @@ static void split_leaves(const HostTree& tt, const std::vector<int>& leaves, con
 // position; lm: the surface lists the split also weighs). A pair (A, B) of leaves
 // A != B goes to the symmetric evaluation when both hold lo .. kSymLeafMax points
 // (lo below) and the pairs list it both ways; the leaf with the smaller first point owns
-// it, and B's points take slots base[A] + (their offset in A's owned stream).
+// it, and B's points take slots base[A] + (their offset in A's owned stream); B's
+// slots are listed by owner position.
 static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::array<int, 2>>& p2p,
-                      const HostList& lm, hipStream_t s) {
+                      const HostList& lm, hipStream_t s, PhaseClock& clk) {
     FmmPlan::SymP2P& Y = P.sym;
     const std::vector<int>& leaves = T.leaves;
     const int nl = (int)leaves.size();
@@ -836,34 +837,65 @@ static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::arra
     auto listed = [&](int k, int c) {
         return std::binary_search(srt.begin() + lc.ptr[k], srt.begin() + lc.ptr[k + 1], c);
     };
+    // two passes over chunks of positions: mark and count, then fill at the prefix offsets
+    constexpr int kChunk = 1024;
+    const size_t nch = ((size_t)nl + kChunk - 1) / kChunk;
+    std::vector<char> both(lc.cell.size());
+    std::vector<int> n_ord(nl), n_own(nl), n_part(nl);
+    parallel_for(nch, [&](size_t ch) {
+        for (int k = (int)ch * kChunk; k < std::min(nl, (int)(ch + 1) * kChunk); k++) {
+            const int ca = leaves[k];
+            int o = 0, w = 0, q = 0;
+            for (int e = lc.ptr[k]; e < lc.ptr[k + 1]; e++) {
+                const int c = lc.cell[e], j = pos_of[c];
+                const bool b = c != ca && j >= 0 && fits(k) && fits(j) && listed(j, ca);
+                both[e] = b;
+                if (!b) o++;
+                else {
+                    q++;
+                    if (T.begin[ca] < T.begin[c]) w++;
+                }
+            }
+            n_ord[k] = o;
+            n_own[k] = w;
+            n_part[k] = q;
+        }
+    });
     HostList ord, own;
     ord.items = own.items = leaves;
-    ord.ptr.assign(1, 0);
-    own.ptr.assign(1, 0);
-    Y.part_ptr.assign(1, 0);
-    Y.part.clear();
+    ord.ptr.assign(nl + 1, 0);
+    own.ptr.assign(nl + 1, 0);
+    Y.part_ptr.assign(nl + 1, 0);
     for (int k = 0; k < nl; k++) {
-        const int ca = leaves[k];
-        const size_t p0 = Y.part.size();
-        for (int e = lc.ptr[k]; e < lc.ptr[k + 1]; e++) {
-            const int c = lc.cell[e], j = pos_of[c];
-            const bool both = c != ca && j >= 0 && fits(k) && fits(j) && listed(j, ca);
-            if (!both) {
-                ord.cell.push_back(c);
-                continue;
-            }
-            Y.part.push_back(c);
-            if (T.begin[ca] < T.begin[c]) own.cell.push_back(c);
-        }
-        std::sort(Y.part.begin() + p0, Y.part.end());
-        ord.ptr.push_back((int)ord.cell.size());
-        own.ptr.push_back((int)own.cell.size());
-        Y.part_ptr.push_back((int)Y.part.size());
+        ord.ptr[k + 1] = ord.ptr[k] + n_ord[k];
+        own.ptr[k + 1] = own.ptr[k] + n_own[k];
+        Y.part_ptr[k + 1] = Y.part_ptr[k] + n_part[k];
     }
-    if (own.cell.empty()) return;
+    if (own.ptr[nl] == 0) return;
+    ord.cell.resize(ord.ptr[nl]);
+    own.cell.resize(own.ptr[nl]);
+    Y.part.resize(Y.part_ptr[nl]);
+    parallel_for(nch, [&](size_t ch) {
+        for (int k = (int)ch * kChunk; k < std::min(nl, (int)(ch + 1) * kChunk); k++) {
+            const int ca = leaves[k];
+            int o = ord.ptr[k], w = own.ptr[k], q = Y.part_ptr[k];
+            for (int e = lc.ptr[k]; e < lc.ptr[k + 1]; e++) {
+                const int c = lc.cell[e];
+                if (!both[e]) {
+                    ord.cell[o++] = c;
+                    continue;
+                }
+                Y.part[q++] = c;
+                if (T.begin[ca] < T.begin[c]) own.cell[w++] = c;
+            }
+            std::sort(Y.part.begin() + Y.part_ptr[k], Y.part.begin() + Y.part_ptr[k + 1]);
+        }
+    });
+    clk.mark("sym: pairs");
     const std::vector<int> own_cells = own.cell, own_ptr = own.ptr;
     cells_to_ranges(ord, T);
     cells_to_ranges(own, T);
+    clk.mark("sym: ranges");
     std::vector<int> base(nl, 0);
     std::vector<std::pair<long, int>> work;
     size_t n_rev = 0;
@@ -877,27 +909,34 @@ static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::arra
         n_rev += (size_t)n;
         TBEM_REQUIRE(n_rev <= (size_t)INT32_MAX, TBEM_E_UNSUPPORTED, "symmetric pairs: too many slots");
     }
-    // slots addressed to each position, by owner in point order
-    std::vector<std::vector<std::pair<int, int>>> recv(nl);
-    for (int k = 0; k < nl; k++) {
-        const int e0 = own.ptr[k], e1 = own.ptr[k + 1];
-        for (int q = own_ptr[k]; q < own_ptr[k + 1]; q++) {
-            const int c = own_cells[q];
-            if (T.count[c] == 0) continue;
-            const int e = (int)(std::upper_bound(own.rbeg.begin() + e0, own.rbeg.begin() + e1, T.begin[c]) -
-                                own.rbeg.begin()) - 1;
-            recv[pos_of[c]].push_back({T.begin[leaves[k]], base[k] + own.rpre[e] + (T.begin[c] - own.rbeg[e])});
+    // the slot of each owned leaf's first point (in parallel), then the slots addressed
+    // to each position, by owner position (counted, then filled in owner order)
+    std::vector<int> slot(own_cells.size());
+    parallel_for(nch, [&](size_t ch) {
+        for (int k = (int)ch * kChunk; k < std::min(nl, (int)(ch + 1) * kChunk); k++) {
+            const int e0 = own.ptr[k], e1 = own.ptr[k + 1];
+            for (int q = own_ptr[k]; q < own_ptr[k + 1]; q++) {
+                const int c = own_cells[q];
+                const int e = (int)(std::upper_bound(own.rbeg.begin() + e0, own.rbeg.begin() + e1, T.begin[c]) -
+                                    own.rbeg.begin()) - 1;
+                slot[q] = base[k] + own.rpre[e] + (T.begin[c] - own.rbeg[e]);
+            }
         }
-    }
-    std::vector<int> g_pos, g_ptr(1, 0), g_off;
+    });
+    std::vector<int> g_cnt(nl, 0);
+    for (size_t q = 0; q < own_cells.size(); q++) g_cnt[pos_of[own_cells[q]]]++;
+    std::vector<int> g_pos, g_ptr(1, 0), g_at(nl, -1);
     for (int k = 0; k < nl; k++) {
-        if (recv[k].empty()) continue;
-        std::sort(recv[k].begin(), recv[k].end());
+        if (g_cnt[k] == 0) continue;
+        g_at[k] = g_ptr.back();
         g_pos.push_back(k);
-        for (auto& r: recv[k]) g_off.push_back(r.second);
-        g_ptr.push_back((int)g_off.size());
+        g_ptr.push_back(g_ptr.back() + g_cnt[k]);
     }
+    std::vector<int> g_off(g_ptr.back());
+    for (int k = 0; k < nl; k++)
+        for (int q = own_ptr[k]; q < own_ptr[k + 1]; q++) g_off[g_at[pos_of[own_cells[q]]]++] = slot[q];
     const std::vector<int> order = xcd_layout(work, 1, -1);
+    clk.mark("sym: slots");
     Y.ord.upload(ord, s);
     Y.own.upload(own, s);
     Y.base.upload(base, s);
@@ -909,7 +948,9 @@ static void build_sym(FmmPlan& P, const HostTree& T, const std::vector<std::arra
     Y.n_leaves = (int)work.size();
     Y.n_g = (int)g_pos.size();
     Y.n_rev = n_rev;
+    clk.mark("sym: upload");
     split_leaves(T, leaves, ord, lm, P.n_e, Y.split, s);
+    clk.mark("sym: split");
     close_flags(Y.ord, P.obs_leaves.p, P.dot, P.op, P.sp, s);
     Y.on = true;
 }
@@ -1090,9 +1131,11 @@ static std::shared_ptr<FmmPlan> build_plan(const tbem_fmm_desc* d, hipStream_t s
         P->leaf_p2p.upload(lp, s);
         P->leaf_m2p.upload(lm, s);
         split_leaves(OT, OT.leaves, lp, lm, P->n_e, P->leaf_split, s);
+        clk.mark("leaf lists");
         if (P->same_tree && (P->kid == TBEM_K_ELASTIC_DISPLACEMENT || P->kid == TBEM_K_LAPLACE_SINGLE) &&
             d->point_pairs == TBEM_POINT_PAIRS_AUTO)
-            build_sym(*P, OT, tl.p2p, lm, s);
+            build_sym(*P, OT, tl.p2p, lm, s, clk);
+        clk.mark("symmetric pairs");
     }
     close_flags(P->leaf_p2p, P->obs_leaves.p, P->dot, P->op, P->sp, s);
 
