@@ -1169,13 +1169,29 @@ const FmmPlan::AdjLists& FmmPlan::adjoint_lists(hipStream_t s) const {
         split_leaves(ST, ST.leaves, lp, ll, n_e, A->leaf_split, s);
         if (sym.on) {  // one shared tree: source leaf positions are the forward's
             HostList ord = list_from_pairs(ST.leaves, ST.n_cells, tl.p2p, 1, nullptr);
-            std::vector<int> ptr(1, 0), cell;
-            for (size_t k = 0; k + 1 < ord.ptr.size(); k++) {
-                const auto b = sym.part.begin() + sym.part_ptr[k], e = sym.part.begin() + sym.part_ptr[k + 1];
-                for (int q = ord.ptr[k]; q < ord.ptr[k + 1]; q++)
-                    if (!std::binary_search(b, e, ord.cell[q])) cell.push_back(ord.cell[q]);
-                ptr.push_back((int)cell.size());
-            }
+            const size_t nl = ord.ptr.size() - 1;
+            std::vector<char> keep(ord.cell.size());
+            std::vector<int> ptr(nl + 1, 0), cell;
+            parallel_for((nl + 1023) / 1024, [&](size_t ch) {  // mark, then fill at the prefix offsets
+                for (size_t k = ch * 1024; k < std::min(nl, (ch + 1) * 1024); k++) {
+                    const auto b = sym.part.begin() + sym.part_ptr[k], e = sym.part.begin() + sym.part_ptr[k + 1];
+                    int n = 0;
+                    for (int q = ord.ptr[k]; q < ord.ptr[k + 1]; q++) {
+                        keep[q] = !std::binary_search(b, e, ord.cell[q]);
+                        n += keep[q];
+                    }
+                    ptr[k + 1] = n;
+                }
+            });
+            for (size_t k = 0; k < nl; k++) ptr[k + 1] += ptr[k];
+            cell.resize(ptr[nl]);
+            parallel_for((nl + 1023) / 1024, [&](size_t ch) {
+                for (size_t k = ch * 1024; k < std::min(nl, (ch + 1) * 1024); k++) {
+                    int o = ptr[k];
+                    for (int q = ord.ptr[k]; q < ord.ptr[k + 1]; q++)
+                        if (keep[q]) cell[o++] = ord.cell[q];
+                }
+            });
             ord.ptr = std::move(ptr);
             ord.cell = std::move(cell);
             cells_to_ranges(ord, ST);
