@@ -56,6 +56,7 @@ TRAFFIC_BYTES = {
     ("fault_surface_o100", 2_000_000, "fp64", "leaf"): 4_213_500_000,
     ("fault_surface_o100", 2_000_000, "mixed", "leaf"): 4_216_000_000,
     ("fault_surface_o100", 2_000_000, "mixed", "m2l"): 7_941_620_238,
+    ("cube_clusters", 4_000_000, "fp64", "leaf"): 14_155_000_000,
 }
 TRAFFIC_UNIT = "HBM bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)"
 
